@@ -1,0 +1,175 @@
+/* markovkit_amd — MI355X (gfx950) RDME resolvent solve behind a C ABI.
+ *
+ * The reference boundary is C++ (templates and concepts, no C ABI). Each entry point below
+ * replaces one reference construct; the citation (paths relative to the markovkit repository
+ * root) names it. Header-only C++ wrappers with the reference's names sit on top of these
+ * entry points in include/markovkit_amd/markovkit.hpp.
+ *
+ * Conventions
+ *  - Every function returns an mk_status; MK_OK is 0. The message of the last failure on
+ *    the calling thread is mk_last_error(). Status values map to the reference's exception
+ *    types: MK_ERR_INVALID_ARGUMENT -> std::invalid_argument, MK_ERR_OVERFLOW ->
+ *    std::overflow_error, MK_ERR_RUNTIME -> std::runtime_error; MK_ERR_DEVICE is a HIP failure.
+ *  - Handles are library-owned (create/destroy). Vectors are caller-owned: the plain entry
+ *    points take host buffers, the *_device entry points take device (HBM) pointers.
+ *  - Non-convergence is not an error: mk_solver_result.converged == 0.
+ *  - Calls on one handle are serialised on the library's HIP stream; a handle is not
+ *    reentrant (like the reference's mutable multigrid scratch, multigrid.hpp:359-360).
+ */
+#ifndef MARKOVKIT_AMD_H
+#define MARKOVKIT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t mk_status;
+enum {
+    MK_OK = 0,
+    MK_ERR_INVALID_ARGUMENT = 1,
+    MK_ERR_OVERFLOW = 2,
+    MK_ERR_RUNTIME = 3,
+    MK_ERR_DEVICE = 4
+};
+
+typedef struct mk_box_s *mk_box;
+typedef struct mk_domain_s *mk_domain;
+typedef struct mk_generator_s *mk_generator;
+typedef struct mk_sector_mg_s *mk_sector_mg;
+
+/* boundary (generator.hpp:11-13) */
+enum { MK_REFLECTING = 0, MK_ABSORBING = 1 };
+/* relaxation (multigrid.hpp:120-123) */
+enum { MK_JACOBI = 0, MK_AGGREGATE_BLOCK = 1 };
+/* working precision of the multigrid preconditioner (no reference counterpart) */
+enum { MK_FP64 = 0, MK_FP32 = 1 };
+
+/* multigrid_options (multigrid.hpp:125-134), same fields and defaults */
+typedef struct mk_multigrid_options {
+    int32_t pre_sweeps;     /* 2 */
+    int32_t post_sweeps;    /* 2 */
+    double jacobi_weight;   /* 2/3 */
+    int64_t coarsest_size;  /* 4 */
+    int32_t max_levels;     /* 32 */
+    int32_t coarse_cycles;  /* 1 = V-cycle, 2 = W-cycle */
+    int32_t sweep_growth;   /* 0 */
+    int32_t smoother;       /* MK_JACOBI */
+    int32_t precision;      /* MK_FP64 or MK_FP32 (device-only extension) */
+} mk_multigrid_options;
+
+/* num::gmres_options (numerics, call site tests/test_rdme.cpp:238-240); tolerance is an
+ * absolute Euclidean residual norm, x on input is the initial guess */
+typedef struct mk_gmres_options {
+    double tolerance;
+    int64_t max_iterations;
+    int64_t restart;
+} mk_gmres_options;
+
+/* num::solver_result (examples/rdme_multigrid_benchmark.cpp:55); residual is the final true
+ * residual ||b - A x||_2 recomputed in FP64 */
+typedef struct mk_solver_result {
+    int64_t iterations;
+    double residual;
+    int32_t converged;
+} mk_solver_result;
+
+const char *mk_last_error(void);
+void mk_default_multigrid_options(mk_multigrid_options *out);
+
+/* ---- population_box (rdme/population_space.hpp:131-196) ---------------------------- */
+/* population_box(n_voxels, max_totals): population_space.hpp:185-190 */
+mk_status mk_box_create(int64_t n_voxels, int32_t n_species, const int32_t *max_totals,
+                        mk_box *out);
+mk_status mk_box_destroy(mk_box box);
+/* size(), n_sectors(): population_space.hpp:192, :206 */
+mk_status mk_box_size(mk_box box, int64_t *size);
+mk_status mk_box_n_sectors(mk_box box, int64_t *n_sectors);
+/* sector_begin(k) / sector_totals(k): population_space.hpp:209-211 */
+mk_status mk_box_sector(mk_box box, int64_t k, int64_t *begin, int64_t *end,
+                        int32_t *totals /* n_species */);
+/* operator[](i): population_space.hpp:193; state is voxel-major, n_voxels*n_species ints */
+mk_status mk_box_state(mk_box box, int64_t i, int32_t *state);
+/* find(x): population_space.hpp:194-202; *index = -1 when x is outside the box */
+mk_status mk_box_find(mk_box box, const int32_t *state, int64_t *index);
+
+/* ---- voxel_domain (rdme/reaction_diffusion.hpp:23-56) over a grid<D> ---------------- */
+/* voxel_domain(grid<D>(dims, spacing), [volumes,] diffusivities): reaction_diffusion.hpp:29-47.
+ * volumes may be NULL (uniform cell volume). */
+mk_status mk_domain_create_grid(int32_t dim, const int64_t *dims, const double *spacing,
+                                const double *volumes, int32_t n_species,
+                                const double *diffusivities, mk_domain *out);
+/* voxel_domain over an explicit graph (num::graph; tests/test_rdme.cpp:35-43):
+ * CSR adjacency offsets[n_voxels+1], to[], weight[]; the multigrid needs a grid domain */
+mk_status mk_domain_create_graph(int64_t n_voxels, const int64_t *offsets, const int64_t *to,
+                                 const double *weight, const double *volumes, int32_t n_species,
+                                 const double *diffusivities, mk_domain *out);
+mk_status mk_domain_destroy(mk_domain domain);
+
+/* ---- generator (generator.hpp:19-112) ----------------------------------------------- */
+/* generator(box, reaction_diffusion{domain, reactions}, boundary): generator.hpp:27-44 with
+ * the model of reaction_diffusion.hpp:82-118. changes[r*n_species + s] is reaction r's change;
+ * rates[(r*n_voxels + v)*T + t] is its rate in voxel v at the count tuple t (mixed radix
+ * max_totals[s]+1, species 0 most significant), T = prod(max_totals[s]+1). */
+mk_status mk_generator_create_rdme(mk_box box, mk_domain domain, int32_t n_reactions,
+                                   const int32_t *changes, const double *rates,
+                                   int32_t boundary, mk_generator *out);
+/* generator from source-major jump lists (the layout of generator.hpp:109-112) for models the
+ * RDME builder does not cover: first[n+1], to[first[n]], rate[first[n]], exit[n], leak[n] */
+mk_status mk_generator_create_jumps(int64_t n, const int64_t *first, const int64_t *to,
+                                    const double *rate, const double *exit, const double *leak,
+                                    mk_generator *out);
+mk_status mk_generator_destroy(mk_generator gen);
+/* rows(), n_jumps(): generator.hpp:46-48 */
+mk_status mk_generator_size(mk_generator gen, int64_t *rows, int64_t *n_jumps);
+/* exit_rates(), leak_rates(): generator.hpp:55-59 */
+mk_status mk_generator_rates(mk_generator gen, double *exit, double *leak);
+/* apply(x, y): y = Q x, generator.hpp:62-70 */
+mk_status mk_generator_apply(mk_generator gen, const double *x, double *y);
+/* shifted_generator::apply -> apply_shifted(shift, x, y): y = shift x - Q x,
+ * generator.hpp:73-81 and :127 */
+mk_status mk_shifted_apply(mk_generator gen, double shift, const double *x, double *y);
+mk_status mk_shifted_apply_device(mk_generator gen, double shift, const double *d_x,
+                                  double *d_y);
+
+/* ---- sector_multigrid (rdme/multigrid.hpp:371-408) ---------------------------------- */
+/* sector_multigrid(domain, box, shift, options): multigrid.hpp:377-384; options may be NULL */
+mk_status mk_sector_mg_create(mk_domain domain, mk_box box, double shift,
+                              const mk_multigrid_options *options, mk_sector_mg *out);
+mk_status mk_sector_mg_destroy(mk_sector_mg mg);
+/* max_levels(), n_sectors(): multigrid.hpp:388-395 */
+mk_status mk_sector_mg_info(mk_sector_mg mg, int64_t *rows, int64_t *n_sectors,
+                            int64_t *max_levels);
+/* apply(b, x): one cycle per sector from a zero guess, multigrid.hpp:397-403 */
+mk_status mk_sector_mg_apply(mk_sector_mg mg, const double *b, double *x);
+mk_status mk_sector_mg_apply_device(mk_sector_mg mg, const double *d_b, double *d_x);
+
+/* ---- num::gmres (numerics; call sites tests/test_rdme.cpp:242-243) ------------------ */
+/* gmres(shifted_generator(gen, shift), [mg,] b, x, options): flexible GMRES(restart) with
+ * CGS2 Arnoldi and Givens rotations in FP64; mg may be NULL (plain GMRES). */
+mk_status mk_gmres(mk_generator gen, double shift, mk_sector_mg mg, const double *b, double *x,
+                   const mk_gmres_options *options, mk_solver_result *result);
+mk_status mk_gmres_device(mk_generator gen, double shift, mk_sector_mg mg, const double *d_b,
+                          double *d_x, const mk_gmres_options *options,
+                          mk_solver_result *result);
+
+/* ---- device memory and timing (no reference counterpart: HBM-resident callers) ------ */
+mk_status mk_device_alloc(int64_t bytes, void **d_ptr);
+mk_status mk_device_free(void *d_ptr);
+mk_status mk_copy_to_device(void *d_dst, const void *src, int64_t bytes);
+mk_status mk_copy_to_host(void *dst, const void *d_src, int64_t bytes);
+mk_status mk_synchronize(void);
+/* per-kernel-class timing with HIP events on the library stream; names are
+ * "shifted_spmv", "jacobi_sweep", "residual", "restrict", "prolong", "terminal",
+ * "gmres_dot", "gmres_update", "gmres_other" */
+mk_status mk_profile_enable(int32_t on);
+mk_status mk_profile_reset(void);
+mk_status mk_profile_read(const char *kernel, int64_t *launches, double *milliseconds,
+                          double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MARKOVKIT_AMD_H */

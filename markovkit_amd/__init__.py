@@ -1,0 +1,19 @@
+"""markovkit_amd: MarkovKit's RDME resolvent solve (GMRES on shift I - Q over a
+population_box, preconditioned by sector_multigrid) as hand-written HIP kernels for MI355X.
+
+The model description (``model``) is plain host logic; the solver API (``api``) calls the
+in-tree HIP library through its C ABI and has no CPU fallback.
+"""
+from .model import (Graph, Grid, LocalReaction, ReactionDiffusion, VoxelDomain, birth_death,
+                    box_size, compositions)
+from .api import (ABSORBING, AGGREGATE_BLOCK, FP32, FP64, JACOBI, REFLECTING, DeviceVector,
+                  Generator, MultigridOptions, PopulationBox, SectorMultigrid, ShiftedGenerator,
+                  SolverResult, gmres, profile_enable, profile_read, profile_reset, synchronize)
+
+__all__ = [
+    "Graph", "Grid", "LocalReaction", "ReactionDiffusion", "VoxelDomain", "birth_death",
+    "box_size", "compositions", "ABSORBING", "AGGREGATE_BLOCK", "FP32", "FP64", "JACOBI",
+    "REFLECTING", "DeviceVector", "Generator", "MultigridOptions", "PopulationBox",
+    "SectorMultigrid", "ShiftedGenerator", "SolverResult", "gmres", "profile_enable",
+    "profile_read", "profile_reset", "synchronize",
+]

@@ -1,0 +1,397 @@
+"""Python mirror of MarkovKit's resolvent-solve interface over the HIP library's C ABI.
+
+Names, argument meaning and error behaviour follow the reference (paths relative to the
+markovkit repository root):
+
+* ``PopulationBox``      population_box        include/markovkit/rdme/population_space.hpp:131-196
+* ``Generator``          generator             include/markovkit/generator.hpp:19-112
+* ``ShiftedGenerator``   shifted_generator     include/markovkit/generator.hpp:116-133
+* ``SectorMultigrid``    sector_multigrid<D>   include/markovkit/rdme/multigrid.hpp:371-408
+* ``gmres``              num::gmres            (numerics; call sites tests/test_rdme.cpp:242-243)
+
+Vectors are numpy arrays on the host (copied to and from HBM per call) or ``DeviceVector``
+objects that stay resident in HBM.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+from ._lib import GmresOptions as _GmresOptions
+from ._lib import MultigridOptions as _MultigridOptions
+from ._lib import SolverResult as _SolverResult
+from ._lib import check, lib
+from .model import Grid, ReactionDiffusion, VoxelDomain
+
+REFLECTING = 0
+ABSORBING = 1
+JACOBI = 0
+AGGREGATE_BLOCK = 1
+FP64 = 0
+FP32 = 1
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class PopulationBox:
+    """population_box(n_voxels, max_totals)."""
+
+    def __init__(self, n_voxels: int, max_totals: Sequence[int]):
+        self.max_totals = np.ascontiguousarray(max_totals, np.int32)
+        self._h = C.c_void_p()
+        check(lib().mk_box_create(int(n_voxels), len(self.max_totals), _p(self.max_totals),
+                                  C.byref(self._h)))
+        self.voxels = int(n_voxels)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mk_box_destroy(self._h)
+            self._h = None
+
+    def size(self) -> int:
+        v = C.c_int64()
+        check(lib().mk_box_size(self._h, C.byref(v)))
+        return v.value
+
+    def n_voxels(self) -> int:
+        return self.voxels
+
+    def n_species(self) -> int:
+        return len(self.max_totals)
+
+    def n_sectors(self) -> int:
+        v = C.c_int64()
+        check(lib().mk_box_n_sectors(self._h, C.byref(v)))
+        return v.value
+
+    def _sector(self, k):
+        b, e = C.c_int64(), C.c_int64()
+        t = np.zeros(self.n_species(), np.int32)
+        check(lib().mk_box_sector(self._h, int(k), C.byref(b), C.byref(e), _p(t)))
+        return b.value, e.value, t
+
+    def sector_begin(self, k: int) -> int:
+        return self._sector(k)[0]
+
+    def sector_end(self, k: int) -> int:
+        return self._sector(k)[1]
+
+    def sector_totals(self, k: int):
+        return list(self._sector(k)[2])
+
+    def sector_of(self, i: int) -> int:
+        lo, hi = 0, self.n_sectors() - 1
+        while lo < hi:
+            mid = (lo + hi + 1) // 2
+            if self.sector_begin(mid) <= i:
+                lo = mid
+            else:
+                hi = mid - 1
+        return lo
+
+    def __len__(self):
+        return self.size()
+
+    def __getitem__(self, i: int):
+        x = np.zeros(self.voxels * self.n_species(), np.int32)
+        check(lib().mk_box_state(self._h, int(i), _p(x)))
+        return list(x)
+
+    def find(self, x) -> Optional[int]:
+        arr = np.ascontiguousarray(x, np.int32)
+        if arr.size != self.voxels * self.n_species():
+            return None
+        v = C.c_int64()
+        check(lib().mk_box_find(self._h, _p(arr), C.byref(v)))
+        return None if v.value < 0 else v.value
+
+    def find_sector(self, totals) -> Optional[int]:
+        k = 0
+        for s, t in enumerate(totals):
+            if t < 0 or t > self.max_totals[s]:
+                return None
+            k = k * (int(self.max_totals[s]) + 1) + int(t)
+        return k
+
+
+class _Domain:
+    """voxel_domain handle on the library side."""
+
+    def __init__(self, domain: VoxelDomain):
+        self.domain = domain
+        self._h = C.c_void_p()
+        diff = np.ascontiguousarray(domain.diffusivities, np.float64)
+        vols = np.ascontiguousarray(domain.volumes, np.float64)
+        g = domain.graph
+        if isinstance(g, Grid):
+            dims = np.asarray(g.dims, np.int64)
+            spacing = np.asarray(g.spacing, np.float64)
+            check(lib().mk_domain_create_grid(g.dim, _p(dims), _p(spacing), _p(vols), len(diff),
+                                              _p(diff), C.byref(self._h)))
+        else:
+            off, to, w = g.adjacency()
+            check(lib().mk_domain_create_graph(g.n_vertices(), _p(off), _p(to), _p(w), _p(vols),
+                                               len(diff), _p(diff), C.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mk_domain_destroy(self._h)
+            self._h = None
+
+
+class DeviceVector:
+    """A float64 vector resident in HBM (allocated by the library)."""
+
+    def __init__(self, n: int, host: Optional[np.ndarray] = None):
+        self.n = int(n)
+        self._p = C.c_void_p()
+        check(lib().mk_device_alloc(self.n * 8, C.byref(self._p)))
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host: np.ndarray):
+        a = np.ascontiguousarray(host, np.float64)
+        assert a.size == self.n
+        check(lib().mk_copy_to_device(self._p, _p(a), self.n * 8))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, np.float64)
+        check(lib().mk_copy_to_host(_p(out), self._p, self.n * 8))
+        return out
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            lib().mk_device_free(self._p)
+            self._p = None
+
+
+class Generator:
+    """generator(space, model, boundary): Q assembled by destination and resident in HBM."""
+
+    def __init__(self, box: PopulationBox, model: ReactionDiffusion, boundary: int = ABSORBING):
+        self.box = box
+        self.model = model
+        self._domain = _Domain(model.domain)
+        changes = np.ascontiguousarray(model.changes(), np.int32)
+        rates = np.ascontiguousarray(model.rate_tables(box.max_totals), np.float64)
+        self._h = C.c_void_p()
+        check(lib().mk_generator_create_rdme(box._h, self._domain._h, len(model.reactions),
+                                             _p(changes), _p(rates), int(boundary),
+                                             C.byref(self._h)))
+        r, j = C.c_int64(), C.c_int64()
+        check(lib().mk_generator_size(self._h, C.byref(r), C.byref(j)))
+        self._rows, self._jumps = r.value, j.value
+
+    @classmethod
+    def from_jumps(cls, first, to, rate, exit_rates, leak_rates):
+        """A generator from source-major jump lists (generator.hpp:109-112 layout)."""
+        self = cls.__new__(cls)
+        self.box = None
+        self.model = None
+        first = np.ascontiguousarray(first, np.int64)
+        to = np.ascontiguousarray(to, np.int64)
+        rate = np.ascontiguousarray(rate, np.float64)
+        e = np.ascontiguousarray(exit_rates, np.float64)
+        l = np.ascontiguousarray(leak_rates, np.float64)
+        self._h = C.c_void_p()
+        check(lib().mk_generator_create_jumps(len(e), _p(first), _p(to), _p(rate), _p(e), _p(l),
+                                              C.byref(self._h)))
+        r, j = C.c_int64(), C.c_int64()
+        check(lib().mk_generator_size(self._h, C.byref(r), C.byref(j)))
+        self._rows, self._jumps = r.value, j.value
+        return self
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mk_generator_destroy(self._h)
+            self._h = None
+
+    def rows(self) -> int:
+        return self._rows
+
+    def cols(self) -> int:
+        return self._rows
+
+    def n_jumps(self) -> int:
+        return self._jumps
+
+    def _rates(self):
+        e = np.zeros(self._rows)
+        l = np.zeros(self._rows)
+        check(lib().mk_generator_rates(self._h, _p(e), _p(l)))
+        return e, l
+
+    def exit_rates(self) -> np.ndarray:
+        return self._rates()[0]
+
+    def leak_rates(self) -> np.ndarray:
+        return self._rates()[1]
+
+    def apply(self, x) -> np.ndarray:
+        xa = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self._rows)
+        check(lib().mk_generator_apply(self._h, _p(xa), _p(y)))
+        return y
+
+    def apply_shifted(self, shift: float, x) -> np.ndarray:
+        xa = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self._rows)
+        check(lib().mk_shifted_apply(self._h, float(shift), _p(xa), _p(y)))
+        return y
+
+    def apply_shifted_device(self, shift: float, x: DeviceVector, y: DeviceVector):
+        check(lib().mk_shifted_apply_device(self._h, float(shift), x.ptr, y.ptr))
+
+
+class ShiftedGenerator:
+    """shifted_generator(Q, shift): the operator shift I - Q; it reads Q by reference."""
+
+    def __init__(self, Q: Generator, shift: float):
+        self.Q = Q
+        self._shift = float(shift)
+
+    def rows(self) -> int:
+        return self.Q.rows()
+
+    def cols(self) -> int:
+        return self.Q.rows()
+
+    def shift(self) -> float:
+        return self._shift
+
+    def apply(self, x) -> np.ndarray:
+        return self.Q.apply_shifted(self._shift, x)
+
+
+@dataclass
+class MultigridOptions:
+    """multigrid_options (multigrid.hpp:125-134) plus the device working precision."""
+    pre_sweeps: int = 2
+    post_sweeps: int = 2
+    jacobi_weight: float = 2.0 / 3.0
+    coarsest_size: int = 4
+    max_levels: int = 32
+    coarse_cycles: int = 1
+    sweep_growth: int = 0
+    smoother: int = JACOBI
+    precision: int = FP64
+
+    def to_c(self) -> _MultigridOptions:
+        return _MultigridOptions(self.pre_sweeps, self.post_sweeps, self.jacobi_weight,
+                                 self.coarsest_size, self.max_levels, self.coarse_cycles,
+                                 self.sweep_growth, self.smoother, self.precision)
+
+
+class SectorMultigrid:
+    """sector_multigrid(domain, box, shift, options): one population multigrid cycle per
+    sector, all sectors of a level in one launch."""
+
+    def __init__(self, domain: VoxelDomain, box: PopulationBox, shift: float,
+                 options: Optional[MultigridOptions] = None):
+        self.options = options or MultigridOptions()
+        self._domain = _Domain(domain)
+        self.box = box
+        self._shift = float(shift)
+        self._h = C.c_void_p()
+        opts = self.options.to_c()
+        check(lib().mk_sector_mg_create(self._domain._h, box._h, float(shift), C.byref(opts),
+                                        C.byref(self._h)))
+        r, k, L = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().mk_sector_mg_info(self._h, C.byref(r), C.byref(k), C.byref(L)))
+        self._rows, self._sectors, self._levels = r.value, k.value, L.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mk_sector_mg_destroy(self._h)
+            self._h = None
+
+    def rows(self) -> int:
+        return self._rows
+
+    def cols(self) -> int:
+        return self._rows
+
+    def n_sectors(self) -> int:
+        return self._sectors
+
+    def max_levels(self) -> int:
+        return self._levels
+
+    def shift(self) -> float:
+        return self._shift
+
+    def apply(self, b) -> np.ndarray:
+        ba = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self._rows)
+        check(lib().mk_sector_mg_apply(self._h, _p(ba), _p(x)))
+        return x
+
+    def apply_device(self, b: DeviceVector, x: DeviceVector):
+        check(lib().mk_sector_mg_apply_device(self._h, b.ptr, x.ptr))
+
+
+@dataclass
+class SolverResult:
+    iterations: int
+    residual: float
+    converged: bool
+
+
+def gmres(A: ShiftedGenerator, *args, tolerance: float = 1e-10, max_iterations: int = 1000,
+          restart: int = 30) -> SolverResult:
+    """num::gmres(A, [M,] b, x, options): x (numpy array or DeviceVector) is the initial
+    guess on input and the solution on output; the tolerance is an absolute Euclidean
+    residual norm; non-convergence is reported, not raised."""
+    if len(args) == 3:
+        M, b, x = args
+    elif len(args) == 2:
+        M = None
+        b, x = args
+    else:
+        raise TypeError("gmres(A, [M,] b, x, ...)")
+    opts = _GmresOptions(float(tolerance), int(max_iterations), int(restart))
+    res = _SolverResult()
+    mh = M._h if M is not None else None
+    if isinstance(b, DeviceVector):
+        check(lib().mk_gmres_device(A.Q._h, A.shift(), mh, b.ptr, x.ptr, C.byref(opts),
+                                    C.byref(res)))
+    else:
+        ba = np.ascontiguousarray(b, np.float64)
+        if x.dtype != np.float64 or not x.flags.c_contiguous:
+            raise TypeError("x must be a contiguous float64 array")
+        check(lib().mk_gmres(A.Q._h, A.shift(), mh, _p(ba), _p(x), C.byref(opts), C.byref(res)))
+    return SolverResult(res.iterations, res.residual, bool(res.converged))
+
+
+def profile_enable(on: bool = True):
+    check(lib().mk_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    check(lib().mk_profile_reset())
+
+
+KERNEL_CLASSES = ("shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict",
+                  "prolong", "terminal", "gmres_dot", "gmres_update", "gmres_other")
+
+
+def profile_read():
+    out = {}
+    for k in KERNEL_CLASSES:
+        n, ms, by = C.c_int64(), C.c_double(), C.c_double()
+        check(lib().mk_profile_read(k.encode(), C.byref(n), C.byref(ms), C.byref(by)))
+        out[k] = dict(launches=n.value, ms=ms.value, bytes=by.value)
+    return out
+
+
+def synchronize():
+    check(lib().mk_synchronize())

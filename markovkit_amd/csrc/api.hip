@@ -1,0 +1,1082 @@
+// C-ABI of markovkit_amd: handles, device residency, the multigrid V-cycle and flexible GMRES
+// orchestrated on one HIP stream. Reference constructs replaced by each entry point are cited
+// in include/markovkit_amd.h.
+#include "../../include/markovkit_amd.h"
+#include "kernels.hpp"
+#include "setup.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+using mk::i64;
+using mk::u32;
+using mk::u64;
+namespace gpu = mk::gpu;
+
+struct device_error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+thread_local std::string g_error;
+
+#define HIPCHK(call)                                                                              \
+    do {                                                                                          \
+        const hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                                     \
+            throw device_error(std::string(#call) + ": " + hipGetErrorString(e_));               \
+    } while (0)
+
+void check_launch(const char *what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw device_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+mk_status guarded(F &&f) {
+    try {
+        f();
+        return MK_OK;
+    } catch (const std::invalid_argument &e) {
+        g_error = e.what();
+        return MK_ERR_INVALID_ARGUMENT;
+    } catch (const std::overflow_error &e) {
+        g_error = e.what();
+        return MK_ERR_OVERFLOW;
+    } catch (const device_error &e) {
+        g_error = e.what();
+        return MK_ERR_DEVICE;
+    } catch (const std::bad_alloc &) {
+        g_error = "out of host memory";
+        return MK_ERR_RUNTIME;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return MK_ERR_RUNTIME;
+    }
+}
+
+void require(bool ok, const char *what) {
+    if (!ok) throw std::invalid_argument(what);
+}
+
+// ------------------------------------------------------------------ device memory
+struct dbuf {
+    void *p = nullptr;
+    std::size_t bytes = 0;
+    dbuf() = default;
+    explicit dbuf(std::size_t n) : bytes(n) {
+        if (n) HIPCHK(hipMalloc(&p, n));
+    }
+    dbuf(const dbuf &) = delete;
+    dbuf &operator=(const dbuf &) = delete;
+    dbuf(dbuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    dbuf &operator=(dbuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~dbuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+template <typename T>
+T *dp(const dbuf &b) {
+    return static_cast<T *>(b.p);
+}
+
+template <typename T>
+dbuf upload(const std::vector<T> &v) {
+    dbuf b(v.size() * sizeof(T));
+    if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return b;
+}
+
+// ------------------------------------------------------------------ library state
+hipStream_t stream() {
+    static hipStream_t s = [] {
+        hipStream_t t = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+        return t;
+    }();
+    return s;
+}
+
+enum kclass {
+    K_SPMV,
+    K_FIRST,
+    K_SWEEP,
+    K_RESIDUAL,
+    K_RESTRICT,
+    K_PROLONG,
+    K_TERMINAL,
+    K_DOT,
+    K_UPDATE,
+    K_OTHER,
+    K_COUNT
+};
+const char *const kclass_names[K_COUNT] = {"shifted_spmv", "jacobi_first", "jacobi_sweep",
+                                           "residual",     "restrict",     "prolong",
+                                           "terminal",     "gmres_dot",    "gmres_update",
+                                           "gmres_other"};
+
+struct profiler {
+    bool on = false;
+    struct rec {
+        int cls;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<rec> pending;
+    std::vector<hipEvent_t> pool;
+    i64 launches[K_COUNT] = {};
+    double ms[K_COUNT] = {};
+    double bytes[K_COUNT] = {};
+
+    hipEvent_t event() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        return e;
+    }
+    template <typename F>
+    void run(int cls, double nbytes, F &&launch) {
+        if (!on) {
+            launch();
+            check_launch(kclass_names[cls]);
+            return;
+        }
+        rec r{cls, event(), event(), nbytes};
+        HIPCHK(hipEventRecord(r.a, stream()));
+        launch();
+        check_launch(kclass_names[cls]);
+        HIPCHK(hipEventRecord(r.b, stream()));
+        pending.push_back(r);
+        if (pending.size() > 4096) resolve();
+    }
+    void resolve() {
+        for (auto &r : pending) {
+            HIPCHK(hipEventSynchronize(r.b));
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+            launches[r.cls] += 1;
+            ms[r.cls] += t;
+            bytes[r.cls] += r.bytes;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+    void reset() {
+        resolve();
+        std::fill(std::begin(launches), std::end(launches), 0);
+        std::fill(std::begin(ms), std::end(ms), 0.0);
+        std::fill(std::begin(bytes), std::end(bytes), 0.0);
+    }
+};
+
+profiler &prof() {
+    static profiler p;
+    return p;
+}
+
+void drain() {
+    HIPCHK(hipStreamSynchronize(stream()));
+    prof().resolve();
+}
+
+// ------------------------------------------------------------------ gather CSR on device
+template <typename VT>
+struct dev_csr {
+    i64 n = 0;
+    u64 nnz = 0;
+    dbuf ptr, col, val, blocks;
+    gpu::csr_view<VT> view;
+};
+
+template <typename VT>
+dev_csr<VT> upload_csr(const mk::generator_csr &g) {
+    dev_csr<VT> d;
+    d.n = g.n;
+    d.nnz = g.ptr.empty() ? 0 : g.ptr.back();
+    if (d.nnz >= (u64{1} << 32)) throw std::overflow_error("operator has more than 2^32 entries");
+    std::vector<u32> ptr(g.n + 1);
+    for (i64 i = 0; i <= g.n; ++i) ptr[i] = static_cast<u32>(g.ptr[i]);
+    // row blocks: up to kRowsPerBlock rows whose entries fit the LDS tile; a row longer than
+    // the tile gets a block of its own
+    std::vector<u32> blocks{0};
+    i64 r = 0;
+    while (r < g.n) {
+        const i64 r0 = r;
+        const u64 p0 = g.ptr[r0];
+        ++r;
+        while (r < g.n && r - r0 < gpu::kRowsPerBlock && g.ptr[r + 1] - p0 <= static_cast<u64>(gpu::kTile)) ++r;
+        blocks.push_back(static_cast<u32>(r));
+    }
+    d.ptr = upload(ptr);
+    d.col = upload(g.col);
+    if constexpr (std::is_same_v<VT, double>) {
+        d.val = upload(g.val);
+    } else {
+        std::vector<VT> v(g.val.begin(), g.val.end());
+        d.val = upload(v);
+    }
+    d.blocks = upload(blocks);
+    d.view.n = g.n;
+    d.view.ptr = dp<u32>(d.ptr);
+    d.view.col = dp<u32>(d.col);
+    d.view.val = dp<VT>(d.val);
+    d.view.blocks = dp<u32>(d.blocks);
+    d.view.n_blocks = static_cast<i64>(blocks.size()) - 1;
+    return d;
+}
+
+double csr_bytes(i64 n, u64 nnz, std::size_t vs, std::size_t xs, std::size_t ts, int vectors) {
+    // entries (value + column) + row pointers + per-row vectors
+    return double(nnz) * double(vs + 4) + double(n + 1) * 4.0 + double(n) * double(xs) +
+           double(n) * double(ts) * vectors;
+}
+
+} // namespace
+
+// ------------------------------------------------------------------ handle types
+struct mk_box_s {
+    mk::population_space space;
+};
+
+struct mk_domain_s {
+    mk::voxel_domain dom;
+};
+
+struct mk_generator_s {
+    i64 n = 0;
+    i64 n_jumps = 0;
+    std::vector<double> exit, leak;
+    dev_csr<double> A;
+    dbuf d_exit;
+    dbuf d_rpart;  // one partial residual norm per row block
+    dbuf d_x, d_y; // staging for host-buffer calls
+};
+
+namespace {
+
+template <typename T>
+struct level_dev {
+    i64 n = 0;
+    dev_csr<T> A;
+    dbuf exit;
+    // transfer to the next level
+    dbuf coarse_of, weight;
+    i64 n_fine_linked = 0; // rows that have a coarse parent
+    dbuf cptr, cidx;       // children of the next level's rows
+    i64 n_next = 0;
+    // terminal blocks on this level
+    i64 n_term = 0, term_max = 0;
+    dbuf tstart, tsize, toff, tinv;
+    // work vectors
+    dbuf rhs, x, xt, res, crhs, cx;
+    int pre = 2, post = 2;
+};
+
+} // namespace
+
+struct mk_sector_mg_s {
+    mk::multigrid_options opt;
+    double shift = 0.0;
+    int precision = MK_FP64;
+    i64 rows = 0, n_sectors = 0, max_levels = 0;
+    std::vector<level_dev<double>> L64;
+    std::vector<level_dev<float>> L32;
+    dbuf d_b, d_x64, d_out; // staging for host-buffer calls
+};
+
+namespace {
+
+template <typename T>
+void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H,
+                  std::vector<level_dev<T>> &L) {
+    const i64 K = static_cast<i64>(H.size());
+    i64 depth = 0;
+    for (const auto &h : H) depth = std::max<i64>(depth, static_cast<i64>(h.levels.size()));
+    mg.max_levels = depth;
+    // row offset of every active sector on every level
+    std::vector<std::vector<i64>> offset(depth, std::vector<i64>(K, -1));
+    std::vector<i64> level_rows(depth, 0);
+    for (i64 l = 0; l < depth; ++l)
+        for (i64 k = 0; k < K; ++k)
+            if (static_cast<i64>(H[k].levels.size()) > l) {
+                offset[l][k] = level_rows[l];
+                level_rows[l] += H[k].levels[l].rows;
+            }
+    L.resize(depth);
+    for (i64 l = 0; l < depth; ++l) {
+        auto &lv = L[l];
+        lv.n = level_rows[l];
+        if (lv.n >= (i64{1} << 32) - 1) throw std::overflow_error("multigrid level has too many states");
+        // concatenated diffusion operator of the active sectors
+        mk::generator_csr cat;
+        cat.n = lv.n;
+        cat.ptr.assign(1, 0);
+        std::vector<u32> coarse_of;
+        std::vector<double> weight;
+        std::vector<u32> tstart, tsize;
+        std::vector<u64> toff;
+        std::vector<double> tinv;
+        bool has_next = false;
+        for (i64 k = 0; k < K; ++k) {
+            if (offset[l][k] < 0) continue;
+            const auto &s = H[k].levels[l];
+            const u32 base = static_cast<u32>(offset[l][k]);
+            lv.pre = s.pre_sweeps;
+            lv.post = s.post_sweeps;
+            for (i64 i = 0; i < s.rows; ++i) {
+                for (u64 p = s.A.ptr[i]; p < s.A.ptr[i + 1]; ++p) {
+                    cat.col.push_back(base + s.A.col[p]);
+                    cat.val.push_back(s.A.val[p]);
+                }
+                cat.ptr.push_back(cat.col.size());
+            }
+            cat.exit.insert(cat.exit.end(), s.A.exit.begin(), s.A.exit.end());
+            if (s.terminal) {
+                tstart.push_back(base);
+                tsize.push_back(static_cast<u32>(s.rows));
+                toff.push_back(tinv.size());
+                tinv.insert(tinv.end(), s.inverse.begin(), s.inverse.end());
+                lv.term_max = std::max<i64>(lv.term_max, s.rows);
+                coarse_of.insert(coarse_of.end(), s.rows, 0xFFFFFFFFu);
+                weight.insert(weight.end(), s.rows, 0.0);
+            } else {
+                has_next = true;
+                const i64 cb = offset[l + 1][k];
+                for (i64 i = 0; i < s.rows; ++i) {
+                    coarse_of.push_back(static_cast<u32>(cb + s.coarse_of[i]));
+                    weight.push_back(s.weight[i]);
+                }
+                lv.n_fine_linked += s.rows;
+            }
+        }
+        lv.A = upload_csr<T>(cat);
+        std::vector<T> exit_t(cat.exit.begin(), cat.exit.end());
+        lv.exit = upload(exit_t);
+        lv.n_term = static_cast<i64>(tstart.size());
+        if (lv.n_term > 0) {
+            lv.tstart = upload(tstart);
+            lv.tsize = upload(tsize);
+            lv.toff = upload(toff);
+            std::vector<T> inv_t(tinv.begin(), tinv.end());
+            lv.tinv = upload(inv_t);
+        }
+        if (has_next) {
+            const i64 nc = level_rows[l + 1];
+            lv.n_next = nc;
+            // children of every coarse row, fine rows ascending (the reference's scatter order)
+            std::vector<u32> cptr(nc + 1, 0), cidx(lv.n_fine_linked);
+            for (u32 c : coarse_of)
+                if (c != 0xFFFFFFFFu) ++cptr[c + 1];
+            for (i64 c = 0; c < nc; ++c) cptr[c + 1] += cptr[c];
+            std::vector<u32> fillp(cptr.begin(), cptr.end() - 1);
+            for (i64 i = 0; i < lv.n; ++i)
+                if (coarse_of[i] != 0xFFFFFFFFu) cidx[fillp[coarse_of[i]]++] = static_cast<u32>(i);
+            lv.coarse_of = upload(coarse_of);
+            std::vector<T> w_t(weight.begin(), weight.end());
+            lv.weight = upload(w_t);
+            lv.cptr = upload(cptr);
+            lv.cidx = upload(cidx);
+        }
+        const std::size_t vb = static_cast<std::size_t>(std::max<i64>(lv.n, 1)) * sizeof(T);
+        lv.rhs = dbuf(vb);
+        lv.x = dbuf(vb);
+        lv.xt = dbuf(vb);
+        lv.res = dbuf(vb);
+        if (mg.opt.coarse_cycles > 1) {
+            lv.crhs = dbuf(vb);
+            lv.cx = dbuf(vb);
+        }
+    }
+}
+
+// one cycle of level l from a zero guess (population_multigrid::cycle, multigrid.hpp:201-228)
+// on all active sectors at once. rhs is this level's right-hand side; rhs64 (level 0 of the
+// FP32 hierarchy only) is the FP64 input that the first sweep converts into lv.rhs. The result
+// is written to out when given, else left in a level buffer whose pointer is returned.
+template <typename T>
+T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t l, const T *rhs,
+             const double *rhs64, T *out) {
+    auto &lv = L[l];
+    auto &P = prof();
+    const hipStream_t s = stream();
+    const double shift = mg.shift, omega = mg.opt.jacobi_weight;
+    const i64 n = lv.n;
+    const std::size_t ts = sizeof(T);
+    const double sweep_bytes = csr_bytes(n, lv.A.nnz, ts, ts, ts, 3);
+    T *cur = dp<T>(lv.x);
+    T *other = dp<T>(lv.xt);
+    if (rhs64) {
+        T *rb = dp<T>(lv.rhs);
+        if (lv.pre > 0) {
+            P.run(K_FIRST, double(n) * (8 + 3 * ts), [&] {
+                gpu::jacobi_first<T, double>(n, shift, omega, dp<T>(lv.exit), rhs64, rb, cur, s);
+            });
+        } else {
+            P.run(K_OTHER, double(n) * (8 + ts), [&] { gpu::convert<double, T>(n, rhs64, rb, s); });
+        }
+        rhs = rb;
+    } else if (lv.pre > 0) {
+        P.run(K_FIRST, double(n) * 3 * ts, [&] {
+            gpu::jacobi_first<T, T>(n, shift, omega, dp<T>(lv.exit), rhs, nullptr, cur, s);
+        });
+    }
+    if (lv.pre == 0) P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
+    for (int k = 1; k < lv.pre; ++k) {
+        P.run(K_SWEEP, sweep_bytes, [&] {
+            gpu::jacobi_sweep<T>(lv.A.view, shift, omega, dp<T>(lv.exit), rhs, cur, other, s);
+        });
+        std::swap(cur, other);
+    }
+    T *res = dp<T>(lv.res);
+    P.run(K_RESIDUAL, sweep_bytes, [&] {
+        gpu::level_residual<T>(lv.A.view, shift, dp<T>(lv.exit), rhs, cur, res, s);
+    });
+    if (lv.n_term > 0) {
+        P.run(K_TERMINAL, 0.0, [&] {
+            gpu::terminal_solve<T>(lv.n_term, dp<u32>(lv.tstart), dp<u32>(lv.tsize),
+                                   dp<u64>(lv.toff), dp<T>(lv.tinv), lv.term_max, res, cur, s);
+        });
+    }
+    if (lv.n_next > 0) {
+        auto &c = L[l + 1];
+        const double rbytes = double(lv.n_fine_linked) * (ts + 4) + double(c.n + 1) * 4 + double(c.n) * ts;
+        const double pbytes = double(lv.n) * (4 + 3 * ts) + double(c.n) * ts;
+        if (mg.opt.coarse_cycles <= 1) {
+            P.run(K_RESTRICT, rbytes, [&] {
+                gpu::restrict_children<T>(c.n, dp<u32>(lv.cptr), dp<u32>(lv.cidx), res,
+                                          dp<T>(c.rhs), s);
+            });
+            T *cx = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr);
+            P.run(K_PROLONG, pbytes, [&] {
+                gpu::prolong_add<T>(n, dp<u32>(lv.coarse_of), dp<T>(lv.weight), cx, cur, s);
+            });
+        } else {
+            P.run(K_RESTRICT, rbytes, [&] {
+                gpu::restrict_children<T>(c.n, dp<u32>(lv.cptr), dp<u32>(lv.cidx), res,
+                                          dp<T>(c.crhs), s);
+            });
+            P.run(K_OTHER, double(c.n) * ts, [&] { gpu::fill<T>(c.n, T(0), dp<T>(c.cx), s); });
+            const double cbytes = csr_bytes(c.n, c.A.nnz, ts, ts, ts, 3);
+            for (int cyc = 0; cyc < mg.opt.coarse_cycles; ++cyc) {
+                P.run(K_RESIDUAL, cbytes, [&] {
+                    gpu::level_residual<T>(c.A.view, shift, dp<T>(c.exit), dp<T>(c.crhs),
+                                           dp<T>(c.cx), dp<T>(c.rhs), s);
+                });
+                T *ce = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr);
+                P.run(K_OTHER, double(c.n) * 3 * ts, [&] { gpu::axpy<T>(c.n, T(1), ce, dp<T>(c.cx), s); });
+            }
+            P.run(K_PROLONG, pbytes, [&] {
+                gpu::prolong_add<T>(n, dp<u32>(lv.coarse_of), dp<T>(lv.weight), dp<T>(c.cx),
+                                    cur, s);
+            });
+        }
+    }
+    for (int k = 0; k < lv.post; ++k) {
+        T *dst = (k == lv.post - 1 && out) ? out : other;
+        P.run(K_SWEEP, sweep_bytes, [&] {
+            gpu::jacobi_sweep<T>(lv.A.view, shift, omega, dp<T>(lv.exit), rhs, cur, dst, s);
+        });
+        if (dst == other) {
+            std::swap(cur, other);
+        } else {
+            cur = dst;
+        }
+    }
+    if (out && cur != out) {
+        P.run(K_OTHER, double(n) * 2 * ts,
+              [&] { HIPCHK(hipMemcpyAsync(out, cur, n * ts, hipMemcpyDeviceToDevice, s)); });
+        cur = out;
+    }
+    return cur;
+}
+
+// z = M b on the device: b FP64, z in the hierarchy's precision
+void mg_apply_device(mk_sector_mg_s &mg, const double *b, void *z) {
+    if (mg.precision == MK_FP32) {
+        run_level<float>(mg, mg.L32, 0, nullptr, b, static_cast<float *>(z));
+    } else {
+        run_level<double>(mg, mg.L64, 0, b, nullptr, static_cast<double *>(z));
+    }
+}
+
+// ------------------------------------------------------------------ flexible GMRES
+struct gmres_workspace {
+    i64 n = 0;
+    i64 m = 0;
+    std::size_t zs = 0;
+    dbuf V, Z, w, r, partial, hd, yd;
+    double *host = nullptr; // pinned: h1, h2 (m+1 each), norm^2, then y (m+1)
+    ~gmres_workspace() {
+        if (host) (void)hipHostFree(host);
+    }
+    void ensure(i64 n_, i64 m_, std::size_t zs_) {
+        if (n_ <= n && m_ <= m && (zs_ == 0 || zs_ <= zs) && V.p) return;
+        n = std::max(n, n_);
+        m = std::max(m, m_);
+        zs = std::max(zs, zs_);
+        V = dbuf();
+        Z = dbuf();
+        V = dbuf(static_cast<std::size_t>(n) * (m + 1) * sizeof(double));
+        if (zs) Z = dbuf(static_cast<std::size_t>(n) * m * zs);
+        w = dbuf(n * sizeof(double));
+        r = dbuf(n * sizeof(double));
+        const int nb = gpu::reduction_blocks(n);
+        partial = dbuf(static_cast<std::size_t>(nb) * (m + 2) * sizeof(double));
+        hd = dbuf((2 * (m + 1) + 2) * sizeof(double));
+        yd = dbuf((m + 1) * sizeof(double));
+        if (host) (void)hipHostFree(host);
+        host = nullptr;
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&host), (3 * (m + 1) + 2) * sizeof(double)));
+    }
+};
+
+gmres_workspace &workspace() {
+    static gmres_workspace w;
+    return w;
+}
+
+double device_norm2(const double *partial, int nb, double *hd) {
+    auto &P = prof();
+    P.run(K_DOT, nb * 8.0, [&] { gpu::reduce_partials(partial, nb, 1, hd, stream()); });
+    double v = 0.0;
+    HIPCHK(hipMemcpyAsync(&v, hd, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    drain();
+    return v;
+}
+
+template <typename ZT>
+mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, const double *b,
+                        double *x, const mk_gmres_options &o) {
+    const i64 n = A.n;
+    const i64 m = std::max<i64>(1, std::min<i64>(o.restart, n));
+    require(m + 1 <= 512, "gmres: restart above 511 is not supported on the device");
+    auto &ws = workspace();
+    ws.ensure(n, m, M ? sizeof(ZT) : 0);
+    auto &P = prof();
+    const hipStream_t s = stream();
+    const int nb = gpu::reduction_blocks(n);
+    double *V = dp<double>(ws.V), *w = dp<double>(ws.w), *r = dp<double>(ws.r);
+    double *partial = dp<double>(ws.partial), *hd = dp<double>(ws.hd);
+    ZT *Z = M ? dp<ZT>(ws.Z) : nullptr;
+    const double spmv_bytes = csr_bytes(n, A.A.nnz, 8, 8, 8, 2);
+
+    mk_solver_result res{0, 0.0, 0};
+    // r = b - A x and ||r||^2 (left in hd[0] for the first basis vector)
+    const int rb = static_cast<int>(std::max<i64>(A.A.view.n_blocks, 1));
+    auto true_residual = [&]() {
+        P.run(K_SPMV, spmv_bytes + n * 8.0, [&] {
+            gpu::shifted_apply<double>(A.A.view, shift, dp<double>(A.d_exit), x, b, r,
+                                       dp<double>(A.d_rpart), s);
+        });
+        return std::sqrt(device_norm2(dp<double>(A.d_rpart), rb, hd));
+    };
+    double beta = true_residual();
+    res.residual = beta;
+    if (!(beta > o.tolerance)) {
+        res.converged = std::isfinite(beta) ? 1 : 0;
+        return res;
+    }
+    std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m);
+    while (res.iterations < o.max_iterations) {
+        // v_0 = r / ||r||, with ||r||^2 still in hd[0] from the true residual
+        P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, r, hd, V, s); });
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        i64 j = 0;
+        for (; j < m && res.iterations < o.max_iterations; ++j) {
+            ++res.iterations;
+            double *Vj = V + j * n;
+            // z_j = M v_j ; w = A z_j
+            if (M) {
+                ZT *Zj = Z + j * n;
+                mg_apply_device(*M, Vj, Zj);
+                P.run(K_SPMV, csr_bytes(n, A.A.nnz, 8, sizeof(ZT), 8, 1) + n * 8.0, [&] {
+                    gpu::shifted_apply<ZT>(A.A.view, shift, dp<double>(A.d_exit), Zj, nullptr, w,
+                                           nullptr, s);
+                });
+            } else {
+                P.run(K_SPMV, spmv_bytes, [&] {
+                    gpu::shifted_apply<double>(A.A.view, shift, dp<double>(A.d_exit), Vj,
+                                               nullptr, w, nullptr, s);
+                });
+            }
+            // classical Gram-Schmidt, twice (CGS2)
+            const int cnt = static_cast<int>(j + 1);
+            for (int pass = 0; pass < 2; ++pass) {
+                double *h = hd + pass * (m + 1);
+                P.run(K_DOT, double(cnt + 1) * n * 8.0, [&] {
+                    gpu::multi_dot(V, n, cnt, w, n, partial, nb, s);
+                    gpu::reduce_partials(partial, nb, cnt, h, s);
+                });
+                double *np = pass == 1 ? partial + static_cast<i64>(nb) * (m + 1) : nullptr;
+                P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {
+                    gpu::basis_update(w, V, n, cnt, h, n, np, nb, s);
+                });
+            }
+            double *norm2 = hd + 2 * (m + 1);
+            P.run(K_DOT, nb * 8.0, [&] {
+                gpu::reduce_partials(partial + static_cast<i64>(nb) * (m + 1), nb, 1, norm2, s);
+            });
+            P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, w, norm2, V + (j + 1) * n, s); });
+            HIPCHK(hipMemcpyAsync(ws.host, hd, (2 * (m + 1) + 1) * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+            drain();
+            const double *h1 = ws.host, *h2 = ws.host + (m + 1);
+            const double hn = std::sqrt(ws.host[2 * (m + 1)]);
+            for (i64 k = 0; k <= j; ++k) H[k * m + j] = h1[k] + h2[k];
+            H[(j + 1) * m + j] = hn;
+            for (i64 k = 0; k < j; ++k) {
+                const double a = H[k * m + j], c = H[(k + 1) * m + j];
+                H[k * m + j] = cs[k] * a + sn[k] * c;
+                H[(k + 1) * m + j] = -sn[k] * a + cs[k] * c;
+            }
+            const double a = H[j * m + j], c = H[(j + 1) * m + j];
+            const double d = std::hypot(a, c);
+            cs[j] = d > 0.0 ? a / d : 1.0;
+            sn[j] = d > 0.0 ? c / d : 0.0;
+            H[j * m + j] = d;
+            H[(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            if (!(std::abs(g[j + 1]) > o.tolerance) || !(hn > 0.0) || !std::isfinite(hn)) {
+                ++j;
+                break;
+            }
+        }
+        for (i64 i = j; i-- > 0;) {
+            double sum = g[i];
+            for (i64 k = i + 1; k < j; ++k) sum -= H[i * m + k] * y[k];
+            y[i] = sum / H[i * m + i];
+        }
+        double *ypin = ws.host + 2 * (m + 1) + 2;
+        std::copy(y.begin(), y.begin() + j, ypin);
+        HIPCHK(hipMemcpyAsync(ws.yd.p, ypin, j * sizeof(double), hipMemcpyHostToDevice, s));
+        P.run(K_OTHER, double(j) * n * (M ? sizeof(ZT) : 8) + n * 16.0, [&] {
+            if (M) {
+                gpu::combine<ZT>(x, Z, n, static_cast<int>(j), dp<double>(ws.yd), n, s);
+            } else {
+                gpu::combine<double>(x, V, n, static_cast<int>(j), dp<double>(ws.yd), n, s);
+            }
+        });
+        beta = true_residual();
+        res.residual = beta;
+        if (!std::isfinite(beta)) break;
+        if (!(beta > o.tolerance)) {
+            res.converged = 1;
+            break;
+        }
+    }
+    return res;
+}
+
+mk_solver_result gmres_device(mk_generator_s &A, double shift, mk_sector_mg_s *M, const double *b,
+                              double *x, const mk_gmres_options &o) {
+    if (M) {
+        require(M->rows == A.n, "gmres: the preconditioner does not match the operator");
+        require(M->shift == shift, "gmres: the preconditioner was built for another shift");
+        if (M->precision == MK_FP32) return fgmres<float>(A, shift, M, b, x, o);
+        return fgmres<double>(A, shift, M, b, x, o);
+    }
+    return fgmres<double>(A, shift, nullptr, b, x, o);
+}
+
+} // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char *mk_last_error(void) { return g_error.c_str(); }
+
+void mk_default_multigrid_options(mk_multigrid_options *o) {
+    o->pre_sweeps = 2;
+    o->post_sweeps = 2;
+    o->jacobi_weight = 2.0 / 3.0;
+    o->coarsest_size = 4;
+    o->max_levels = 32;
+    o->coarse_cycles = 1;
+    o->sweep_growth = 0;
+    o->smoother = MK_JACOBI;
+    o->precision = MK_FP64;
+}
+
+mk_status mk_box_create(int64_t n_voxels, int32_t n_species, const int32_t *max_totals,
+                        mk_box *out) {
+    return guarded([&] {
+        require(out && max_totals && n_species > 0, "population_box: bad arguments");
+        auto b = std::make_unique<mk_box_s>();
+        b->space = mk::population_space::box(n_voxels, std::vector<int>(max_totals, max_totals + n_species));
+        *out = b.release();
+    });
+}
+
+mk_status mk_box_destroy(mk_box box) {
+    delete box;
+    return MK_OK;
+}
+
+mk_status mk_box_size(mk_box box, int64_t *size) {
+    return guarded([&] { *size = box->space.size(); });
+}
+
+mk_status mk_box_n_sectors(mk_box box, int64_t *n) {
+    return guarded([&] { *n = box->space.n_sectors(); });
+}
+
+mk_status mk_box_sector(mk_box box, int64_t k, int64_t *begin, int64_t *end, int32_t *totals) {
+    return guarded([&] {
+        require(k >= 0 && k < box->space.n_sectors(), "population_box: sector out of range");
+        if (begin) *begin = box->space.begin(k);
+        if (end) *end = box->space.end(k);
+        if (totals)
+            for (int s = 0; s < box->space.species(); ++s) totals[s] = box->space.totals(k)[s];
+    });
+}
+
+mk_status mk_box_state(mk_box box, int64_t i, int32_t *state) {
+    return guarded([&] {
+        require(i >= 0 && i < box->space.size(), "population_box: index out of range");
+        box->space.state(i, state);
+    });
+}
+
+mk_status mk_box_find(mk_box box, const int32_t *state, int64_t *index) {
+    return guarded([&] { *index = box->space.find(state); });
+}
+
+mk_status mk_domain_create_grid(int32_t dim, const int64_t *dims, const double *spacing,
+                                const double *volumes, int32_t n_species,
+                                const double *diffusivities, mk_domain *out) {
+    return guarded([&] {
+        require(dim >= 1 && dim <= 3 && dims && spacing && n_species > 0 && diffusivities,
+                "voxel_domain: bad grid arguments");
+        auto d = std::make_unique<mk_domain_s>();
+        d->dom = mk::voxel_domain::grid(std::vector<i64>(dims, dims + dim),
+                                        std::vector<double>(spacing, spacing + dim), volumes,
+                                        std::vector<double>(diffusivities, diffusivities + n_species));
+        *out = d.release();
+    });
+}
+
+mk_status mk_domain_create_graph(int64_t n_voxels, const int64_t *offsets, const int64_t *to,
+                                 const double *weight, const double *volumes, int32_t n_species,
+                                 const double *diffusivities, mk_domain *out) {
+    return guarded([&] {
+        require(n_species > 0 && diffusivities, "voxel_domain: bad graph arguments");
+        auto d = std::make_unique<mk_domain_s>();
+        d->dom = mk::voxel_domain::graph(n_voxels, offsets, to, weight, volumes,
+                                         std::vector<double>(diffusivities, diffusivities + n_species));
+        *out = d.release();
+    });
+}
+
+mk_status mk_domain_destroy(mk_domain d) {
+    delete d;
+    return MK_OK;
+}
+
+namespace {
+void finish_generator(mk_generator_s &G, mk::generator_csr &&g) {
+    G.n = g.n;
+    G.n_jumps = g.n_jumps;
+    G.A = upload_csr<double>(g);
+    G.d_exit = upload(g.exit);
+    G.d_rpart = dbuf(std::max<i64>(G.A.view.n_blocks, 1) * sizeof(double));
+    G.exit = std::move(g.exit);
+    G.leak = std::move(g.leak);
+}
+} // namespace
+
+mk_status mk_generator_create_rdme(mk_box box, mk_domain domain, int32_t n_reactions,
+                                   const int32_t *changes, const double *rates, int32_t boundary,
+                                   mk_generator *out) {
+    return guarded([&] {
+        require(box && domain && out, "generator: bad arguments");
+        require(n_reactions >= 0 && (n_reactions == 0 || (changes && rates)),
+                "generator: reactions need changes and rate tables");
+        const auto &space = box->space;
+        const int S = space.species();
+        require(domain->dom.species() == S && domain->dom.voxels == space.voxels(),
+                "generator: the domain does not match the box");
+        mk::reaction_tables rt;
+        rt.n = n_reactions;
+        rt.stride.assign(S, 1);
+        for (int s = S; s-- > 0;) {
+            rt.stride[s] = rt.T;
+            rt.T *= space.caps()[s] + 1;
+        }
+        if (n_reactions > 0) {
+            rt.changes.assign(changes, changes + static_cast<i64>(n_reactions) * S);
+            rt.rates.assign(rates, rates + static_cast<i64>(n_reactions) * space.voxels() * rt.T);
+        }
+        auto G = std::make_unique<mk_generator_s>();
+        finish_generator(*G, mk::assemble_generator(space, domain->dom, rt, boundary == MK_ABSORBING));
+        *out = G.release();
+    });
+}
+
+mk_status mk_generator_create_jumps(int64_t n, const int64_t *first, const int64_t *to,
+                                    const double *rate, const double *exit, const double *leak,
+                                    mk_generator *out) {
+    return guarded([&] {
+        require(first && exit && leak && out, "generator: bad arguments");
+        auto G = std::make_unique<mk_generator_s>();
+        finish_generator(*G, mk::generator_from_jumps(n, first, to, rate, exit, leak));
+        *out = G.release();
+    });
+}
+
+mk_status mk_generator_destroy(mk_generator g) {
+    delete g;
+    return MK_OK;
+}
+
+mk_status mk_generator_size(mk_generator g, int64_t *rows, int64_t *n_jumps) {
+    return guarded([&] {
+        if (rows) *rows = g->n;
+        if (n_jumps) *n_jumps = g->n_jumps;
+    });
+}
+
+mk_status mk_generator_rates(mk_generator g, double *exit, double *leak) {
+    return guarded([&] {
+        if (exit) std::copy(g->exit.begin(), g->exit.end(), exit);
+        if (leak) std::copy(g->leak.begin(), g->leak.end(), leak);
+    });
+}
+
+namespace {
+void stage(mk_generator_s &g) {
+    if (!g.d_x.p) g.d_x = dbuf(std::max<i64>(g.n, 1) * sizeof(double));
+    if (!g.d_y.p) g.d_y = dbuf(std::max<i64>(g.n, 1) * sizeof(double));
+}
+} // namespace
+
+mk_status mk_shifted_apply_device(mk_generator g, double shift, const double *d_x, double *d_y) {
+    return guarded([&] {
+        prof().run(K_SPMV, csr_bytes(g->n, g->A.nnz, 8, 8, 8, 2), [&] {
+            gpu::shifted_apply<double>(g->A.view, shift, dp<double>(g->d_exit), d_x, nullptr,
+                                       d_y, nullptr, stream());
+        });
+    });
+}
+
+mk_status mk_shifted_apply(mk_generator g, double shift, const double *x, double *y) {
+    return guarded([&] {
+        stage(*g);
+        HIPCHK(hipMemcpy(g->d_x.p, x, g->n * sizeof(double), hipMemcpyHostToDevice));
+        prof().run(K_SPMV, csr_bytes(g->n, g->A.nnz, 8, 8, 8, 2), [&] {
+            gpu::shifted_apply<double>(g->A.view, shift, dp<double>(g->d_exit),
+                                       dp<double>(g->d_x), nullptr, dp<double>(g->d_y), nullptr,
+                                       stream());
+        });
+        drain();
+        HIPCHK(hipMemcpy(y, g->d_y.p, g->n * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+mk_status mk_generator_apply(mk_generator g, const double *x, double *y) {
+    // Q x = -(0 I - Q) x
+    const mk_status st = mk_shifted_apply(g, 0.0, x, y);
+    if (st == MK_OK)
+        for (i64 i = 0; i < g->n; ++i) y[i] = -y[i];
+    return st;
+}
+
+mk_status mk_sector_mg_create(mk_domain domain, mk_box box, double shift,
+                              const mk_multigrid_options *options, mk_sector_mg *out) {
+    return guarded([&] {
+        require(domain && box && out, "sector_multigrid: bad arguments");
+        mk_multigrid_options o;
+        mk_default_multigrid_options(&o);
+        if (options) o = *options;
+        require(o.smoother == MK_JACOBI, "sector_multigrid: only the Jacobi smoother runs on the device");
+        require(o.precision == MK_FP64 || o.precision == MK_FP32, "sector_multigrid: unknown precision");
+        require(o.pre_sweeps >= 0 && o.post_sweeps >= 0 && o.coarse_cycles >= 1,
+                "sector_multigrid: bad sweep or cycle counts");
+        if (!(shift > 0.0)) throw std::invalid_argument("population_multigrid: shift must be positive");
+        const auto &space = box->space;
+        require(domain->dom.species() == space.species() && domain->dom.voxels == space.voxels(),
+                "sector_multigrid: the domain does not match the box");
+        auto mg = std::make_unique<mk_sector_mg_s>();
+        mg->opt.pre_sweeps = o.pre_sweeps;
+        mg->opt.post_sweeps = o.post_sweeps;
+        mg->opt.jacobi_weight = o.jacobi_weight;
+        mg->opt.coarsest_size = o.coarsest_size;
+        mg->opt.max_levels = o.max_levels;
+        mg->opt.coarse_cycles = o.coarse_cycles;
+        mg->opt.sweep_growth = o.sweep_growth;
+        mg->shift = shift;
+        mg->precision = o.precision;
+        mg->rows = space.size();
+        mg->n_sectors = space.n_sectors();
+        std::vector<mk::sector_hierarchy> H;
+        for (i64 k = 0; k < space.n_sectors(); ++k)
+            H.push_back(mk::build_sector_hierarchy(domain->dom, space.totals(k), shift, mg->opt));
+        if (o.precision == MK_FP32) {
+            build_levels<float>(*mg, H, mg->L32);
+        } else {
+            build_levels<double>(*mg, H, mg->L64);
+        }
+        *out = mg.release();
+    });
+}
+
+mk_status mk_sector_mg_destroy(mk_sector_mg mg) {
+    delete mg;
+    return MK_OK;
+}
+
+mk_status mk_sector_mg_info(mk_sector_mg mg, int64_t *rows, int64_t *n_sectors,
+                            int64_t *max_levels) {
+    return guarded([&] {
+        if (rows) *rows = mg->rows;
+        if (n_sectors) *n_sectors = mg->n_sectors;
+        if (max_levels) *max_levels = mg->max_levels;
+    });
+}
+
+mk_status mk_sector_mg_apply_device(mk_sector_mg mg, const double *d_b, double *d_x) {
+    return guarded([&] {
+        if (mg->precision == MK_FP32) {
+            if (!mg->d_out.p) mg->d_out = dbuf(std::max<i64>(mg->rows, 1) * sizeof(float));
+            mg_apply_device(*mg, d_b, mg->d_out.p);
+            prof().run(K_OTHER, mg->rows * 12.0, [&] {
+                gpu::convert<float, double>(mg->rows, dp<float>(mg->d_out), d_x, stream());
+            });
+        } else {
+            mg_apply_device(*mg, d_b, d_x);
+        }
+    });
+}
+
+mk_status mk_sector_mg_apply(mk_sector_mg mg, const double *b, double *x) {
+    return guarded([&] {
+        const std::size_t bytes = std::max<i64>(mg->rows, 1) * sizeof(double);
+        if (!mg->d_b.p) mg->d_b = dbuf(bytes);
+        if (!mg->d_x64.p) mg->d_x64 = dbuf(bytes);
+        HIPCHK(hipMemcpy(mg->d_b.p, b, mg->rows * sizeof(double), hipMemcpyHostToDevice));
+        if (mg->precision == MK_FP32) {
+            if (!mg->d_out.p) mg->d_out = dbuf(std::max<i64>(mg->rows, 1) * sizeof(float));
+            mg_apply_device(*mg, dp<double>(mg->d_b), mg->d_out.p);
+            prof().run(K_OTHER, mg->rows * 12.0, [&] {
+                gpu::convert<float, double>(mg->rows, dp<float>(mg->d_out), dp<double>(mg->d_x64), stream());
+            });
+        } else {
+            mg_apply_device(*mg, dp<double>(mg->d_b), dp<double>(mg->d_x64));
+        }
+        drain();
+        HIPCHK(hipMemcpy(x, mg->d_x64.p, mg->rows * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+mk_status mk_gmres_device(mk_generator gen, double shift, mk_sector_mg mg, const double *d_b,
+                          double *d_x, const mk_gmres_options *options, mk_solver_result *result) {
+    return guarded([&] {
+        require(gen && options && result, "gmres: bad arguments");
+        require(options->max_iterations >= 0 && options->restart >= 1, "gmres: bad options");
+        *result = gmres_device(*gen, shift, mg, d_b, d_x, *options);
+        drain();
+    });
+}
+
+mk_status mk_gmres(mk_generator gen, double shift, mk_sector_mg mg, const double *b, double *x,
+                   const mk_gmres_options *options, mk_solver_result *result) {
+    return guarded([&] {
+        require(gen && options && result, "gmres: bad arguments");
+        require(options->max_iterations >= 0 && options->restart >= 1, "gmres: bad options");
+        stage(*gen);
+        HIPCHK(hipMemcpy(gen->d_x.p, x, gen->n * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(gen->d_y.p, b, gen->n * sizeof(double), hipMemcpyHostToDevice));
+        *result = gmres_device(*gen, shift, mg, dp<double>(gen->d_y), dp<double>(gen->d_x), *options);
+        drain();
+        HIPCHK(hipMemcpy(x, gen->d_x.p, gen->n * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+mk_status mk_device_alloc(int64_t bytes, void **d_ptr) {
+    return guarded([&] {
+        require(bytes >= 0 && d_ptr, "device_alloc: bad arguments");
+        *d_ptr = nullptr;
+        if (bytes > 0) HIPCHK(hipMalloc(d_ptr, bytes));
+    });
+}
+
+mk_status mk_device_free(void *d_ptr) {
+    return guarded([&] {
+        if (d_ptr) HIPCHK(hipFree(d_ptr));
+    });
+}
+
+mk_status mk_copy_to_device(void *d_dst, const void *src, int64_t bytes) {
+    return guarded([&] { HIPCHK(hipMemcpy(d_dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+
+mk_status mk_copy_to_host(void *dst, const void *d_src, int64_t bytes) {
+    return guarded([&] {
+        drain();
+        HIPCHK(hipMemcpy(dst, d_src, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+mk_status mk_synchronize(void) {
+    return guarded([&] { drain(); });
+}
+
+mk_status mk_profile_enable(int32_t on) {
+    return guarded([&] {
+        drain();
+        prof().on = on != 0;
+    });
+}
+
+mk_status mk_profile_reset(void) {
+    return guarded([&] {
+        drain();
+        prof().reset();
+    });
+}
+
+mk_status mk_profile_read(const char *kernel, int64_t *launches, double *milliseconds,
+                          double *bytes) {
+    return guarded([&] {
+        drain();
+        for (int c = 0; c < K_COUNT; ++c) {
+            if (std::strcmp(kernel, kclass_names[c]) == 0) {
+                if (launches) *launches = prof().launches[c];
+                if (milliseconds) *milliseconds = prof().ms[c];
+                if (bytes) *bytes = prof().bytes[c];
+                return;
+            }
+        }
+        throw std::invalid_argument("profile: unknown kernel class");
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,437 @@
+// Device kernels of the resolvent solve for gfx950 (CDNA4, wave64). See kernels.hpp for the
+// data layout. All reductions run in a fixed order (per-wave shuffles, then waves in index
+// order, then workgroup partials in index order), so results do not depend on timing.
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace mk::gpu {
+
+namespace {
+
+constexpr int kWaves = kThreads / 64;
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// sum over the workgroup; the result is valid in thread 0
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) r += lds[w];
+    }
+    __syncthreads();
+    return r;
+}
+
+enum class csr_mode { shifted, shifted_residual, sweep, residual };
+
+template <typename VT, typename XT, typename T>
+struct csr_args {
+    csr_view<VT> A;
+    double shift;
+    T omega;
+    const T *exit;   // per row, in the output's precision
+    const XT *x;
+    const T *rhs;    // b (shifted_residual) or rhs (sweep, residual)
+    T *out;
+    double *partial; // shifted_residual: sum of out^2 per workgroup
+};
+
+// one workgroup per row block: stream columns/values, stage products in LDS, sum rows
+template <csr_mode M, typename VT, typename XT, typename T>
+__global__ void __launch_bounds__(kThreads) k_csr(csr_args<VT, XT, T> a) {
+    using AT = T; // accumulate in the output's precision
+    __shared__ AT prod[kTile];
+    __shared__ AT red[kWaves];
+    __shared__ double red64[kWaves];
+    const u32 r0 = a.A.blocks[blockIdx.x], r1 = a.A.blocks[blockIdx.x + 1];
+    const u32 p0 = a.A.ptr[r0], p1 = a.A.ptr[r1];
+    const int tid = threadIdx.x;
+    const AT shift = static_cast<AT>(a.shift);
+
+    auto finish = [&](u32 r, AT sum) -> double {
+        const AT xr = static_cast<AT>(a.x[r]);
+        const AT d = shift + a.exit[r];
+        const AT ax = d * xr - sum;
+        if constexpr (M == csr_mode::shifted) {
+            a.out[r] = ax;
+            return 0.0;
+        } else if constexpr (M == csr_mode::shifted_residual) {
+            const AT res = a.rhs[r] - ax;
+            a.out[r] = res;
+            return static_cast<double>(res) * static_cast<double>(res);
+        } else if constexpr (M == csr_mode::sweep) {
+            const AT res = a.rhs[r] - ax;
+            a.out[r] = xr + a.omega * res / d;
+            return 0.0;
+        } else {
+            a.out[r] = a.rhs[r] - ax;
+            return 0.0;
+        }
+    };
+
+    double sq = 0.0;
+    if (p1 - p0 <= static_cast<u32>(kTile)) {
+        for (u32 p = p0 + tid; p < p1; p += kThreads)
+            prod[p - p0] = static_cast<AT>(a.A.val[p]) * static_cast<AT>(a.x[a.A.col[p]]);
+        __syncthreads();
+        const u32 r = r0 + tid;
+        if (r < r1) {
+            AT sum = 0;
+            const u32 q1 = a.A.ptr[r + 1];
+            for (u32 q = a.A.ptr[r]; q < q1; ++q) sum += prod[q - p0];
+            sq = finish(r, sum);
+        }
+    } else {
+        // a block holding one row longer than the tile
+        AT sum = 0;
+        for (u32 p = p0 + tid; p < p1; p += kThreads)
+            sum += static_cast<AT>(a.A.val[p]) * static_cast<AT>(a.x[a.A.col[p]]);
+        sum = block_sum(sum, red);
+        if (tid == 0) sq = finish(r0, sum);
+    }
+    if constexpr (M == csr_mode::shifted_residual) {
+        const double t = block_sum(sq, red64);
+        if (tid == 0 && a.partial) a.partial[blockIdx.x] = t;
+    }
+}
+
+template <typename T, typename IN>
+__global__ void __launch_bounds__(kThreads)
+    k_jacobi_first(i64 n, T shift, T omega, const T *exit, const IN *rhs_in, T *rhs_out, T *x) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreads) {
+        const T b = static_cast<T>(rhs_in[i]);
+        if (rhs_out) rhs_out[i] = b;
+        x[i] = omega * b / (shift + exit[i]);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_restrict(i64 nc, const u32 *cptr, const u32 *cidx, const T *fine, T *coarse) {
+    for (i64 c = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; c < nc;
+         c += static_cast<i64>(gridDim.x) * kThreads) {
+        T s = 0;
+        const u32 q1 = cptr[c + 1];
+        for (u32 q = cptr[c]; q < q1; ++q) s += fine[cidx[q]];
+        coarse[c] = s;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreads) {
+        const u32 c = coarse_of[i];
+        if (c != 0xFFFFFFFFu) x[i] += weight[i] * coarse[c];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_terminal(i64 nb, const u32 *start, const u32 *size, const u64 *offset, const T *inv,
+               i64 max_size, const T *r, T *x) {
+    const i64 g = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x;
+    if (g >= nb * max_size) return;
+    const i64 b = g / max_size, i = g % max_size;
+    const u32 m = size[b];
+    if (i >= m) return;
+    const T *row = inv + offset[b] + static_cast<u64>(i) * m;
+    const T *rr = r + start[b];
+    T s = 0;
+    for (u32 j = 0; j < m; ++j) s += row[j] * rr[j];
+    x[start[b] + i] += s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_fill(i64 n, T v, T *x) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreads)
+        x[i] = v;
+}
+
+template <typename A, typename B>
+__global__ void __launch_bounds__(kThreads) k_convert(i64 n, const A *in, B *out) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreads)
+        out[i] = static_cast<B>(in[i]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_axpy(i64 n, T alpha, const T *x, T *y) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreads)
+        y[i] += alpha * x[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_residual_update(i64 n, const T *rhs, const T *ax, T *r) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreads)
+        r[i] = rhs[i] - ax[i];
+}
+
+// ---- Krylov kernels ----
+__global__ void __launch_bounds__(kThreads)
+    k_multi_dot(const double *V, i64 ld, int count, const double *w, i64 n, double *partial,
+                int nb) {
+    __shared__ double red[kWaves];
+    const int k0 = blockIdx.y * kDotChunk;
+    const int kc = min(kDotChunk, count - k0);
+    double acc[kDotChunk];
+#pragma unroll
+    for (int k = 0; k < kDotChunk; ++k) acc[k] = 0.0;
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(nb) * kThreads) {
+        const double wi = w[i];
+#pragma unroll
+        for (int k = 0; k < kDotChunk; ++k)
+            if (k < kc) acc[k] += V[(k0 + k) * ld + i] * wi;
+    }
+#pragma unroll
+    for (int k = 0; k < kDotChunk; ++k) {
+        if (k < kc) {
+            const double s = block_sum(acc[k], red);
+            if (threadIdx.x == 0) partial[static_cast<i64>(k0 + k) * nb + blockIdx.x] = s;
+        }
+    }
+}
+
+constexpr int kMaxBasis = 512;
+
+__global__ void __launch_bounds__(kThreads)
+    k_basis_update(double *w, const double *V, i64 ld, int count, const double *h, i64 n,
+                   double *partial, int nb) {
+    __shared__ double hs[kMaxBasis];
+    __shared__ double red[kWaves];
+    for (int k = threadIdx.x; k < count; k += kThreads) hs[k] = h[k];
+    __syncthreads();
+    double sq = 0.0;
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(nb) * kThreads) {
+        double wi = w[i];
+        for (int k = 0; k < count; ++k) wi -= hs[k] * V[k * ld + i];
+        w[i] = wi;
+        sq += wi * wi;
+    }
+    if (partial) {
+        const double s = block_sum(sq, red);
+        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_reduce(const double *partial, int nb, double *out) {
+    __shared__ double red[kWaves];
+    const double *p = partial + static_cast<i64>(blockIdx.x) * nb;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kThreads) s += p[b];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_scale(i64 n, const double *w, const double *norm2, double *v) {
+    const double nrm = sqrt(norm2[0]);
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreads)
+        v[i] = w[i] / nrm;
+}
+
+template <typename ZT>
+__global__ void __launch_bounds__(kThreads)
+    k_combine(double *x, const ZT *Z, i64 ld, int count, const double *y, i64 n) {
+    __shared__ double ys[kMaxBasis];
+    for (int k = threadIdx.x; k < count; k += kThreads) ys[k] = y[k];
+    __syncthreads();
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreads) {
+        double s = x[i];
+        for (int k = 0; k < count; ++k) s += ys[k] * static_cast<double>(Z[k * ld + i]);
+        x[i] = s;
+    }
+}
+
+int grid_for(i64 n) {
+    const i64 g = (n + kThreads - 1) / kThreads;
+    return static_cast<int>(std::max<i64>(1, std::min<i64>(g, 8192)));
+}
+
+} // namespace
+
+int reduction_blocks(i64 n) {
+    const i64 g = (n + 4 * kThreads - 1) / (4 * kThreads);
+    return static_cast<int>(std::max<i64>(1, std::min<i64>(g, 2048)));
+}
+
+template <typename XT>
+void shifted_apply(const csr_view<double> &A, double shift, const double *exit, const XT *x,
+                   const double *b, double *y, double *partial, hipStream_t s) {
+    if (A.n_blocks == 0) return;
+    csr_args<double, XT, double> a{A, shift, 0.0, exit, x, b, y, partial};
+    if (b) {
+        hipLaunchKernelGGL((k_csr<csr_mode::shifted_residual, double, XT, double>),
+                           dim3(A.n_blocks), dim3(kThreads), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_csr<csr_mode::shifted, double, XT, double>), dim3(A.n_blocks),
+                           dim3(kThreads), 0, s, a);
+    }
+}
+
+template <typename T, typename IN>
+void jacobi_first(i64 n, double shift, double omega, const T *exit, const IN *rhs_in, T *rhs_out,
+                  T *x, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_jacobi_first<T, IN>), dim3(grid_for(n)), dim3(kThreads), 0, s, n,
+                       static_cast<T>(shift), static_cast<T>(omega), exit, rhs_in, rhs_out, x);
+}
+
+template <typename T>
+void jacobi_sweep(const csr_view<T> &A, double shift, double omega, const T *exit, const T *rhs,
+                  const T *x, T *x_out, hipStream_t s) {
+    if (A.n_blocks == 0) return;
+    csr_args<T, T, T> a{A, shift, static_cast<T>(omega), exit, x, rhs, x_out, nullptr};
+    hipLaunchKernelGGL((k_csr<csr_mode::sweep, T, T, T>), dim3(A.n_blocks), dim3(kThreads), 0, s,
+                       a);
+}
+
+template <typename T>
+void level_residual(const csr_view<T> &A, double shift, const T *exit, const T *rhs, const T *x,
+                    T *r, hipStream_t s) {
+    if (A.n_blocks == 0) return;
+    csr_args<T, T, T> a{A, shift, T(0), exit, x, rhs, r, nullptr};
+    hipLaunchKernelGGL((k_csr<csr_mode::residual, T, T, T>), dim3(A.n_blocks), dim3(kThreads), 0,
+                       s, a);
+}
+
+template <typename T>
+void restrict_children(i64 n_coarse, const u32 *cptr, const u32 *cidx, const T *fine, T *coarse,
+                       hipStream_t s) {
+    if (n_coarse == 0) return;
+    hipLaunchKernelGGL((k_restrict<T>), dim3(grid_for(n_coarse)), dim3(kThreads), 0, s, n_coarse,
+                       cptr, cidx, fine, coarse);
+}
+
+template <typename T>
+void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x,
+                 hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_prolong_add<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, coarse_of,
+                       weight, coarse, x);
+}
+
+template <typename T>
+void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
+                    const T *inverse, i64 max_size, const T *r, T *x, hipStream_t s) {
+    if (n_blocks == 0) return;
+    const i64 threads = n_blocks * max_size;
+    const int grid = static_cast<int>((threads + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL((k_terminal<T>), dim3(grid), dim3(kThreads), 0, s, n_blocks, start, size,
+                       offset, inverse, max_size, r, x);
+}
+
+template <typename T>
+void fill(i64 n, T value, T *x, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, value, x);
+}
+
+template <typename A, typename B>
+void convert(i64 n, const A *in, B *out, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_convert<A, B>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, in, out);
+}
+
+template <typename T>
+void axpy(i64 n, T alpha, const T *x, T *y, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_axpy<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, alpha, x, y);
+}
+
+template <typename T>
+void residual_update(i64 n, const T *rhs, const T *ax, T *r, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_residual_update<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, rhs,
+                       ax, r);
+}
+
+void multi_dot(const double *V, i64 ld, int count, const double *w, i64 n, double *partial,
+               int nb, hipStream_t s) {
+    if (count <= 0) return;
+    const int chunks = (count + kDotChunk - 1) / kDotChunk;
+    hipLaunchKernelGGL(k_multi_dot, dim3(nb, chunks), dim3(kThreads), 0, s, V, ld, count, w, n,
+                       partial, nb);
+}
+
+void basis_update(double *w, const double *V, i64 ld, int count, const double *h, i64 n,
+                  double *partial, int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_basis_update, dim3(nb), dim3(kThreads), 0, s, w, V, ld, count, h, n,
+                       partial, nb);
+}
+
+void reduce_partials(const double *partial, int nb, int count, double *out, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_reduce, dim3(count), dim3(kThreads), 0, s, partial, nb, out);
+}
+
+void scale_by_norm(i64 n, const double *w, const double *norm2, double *v, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(kThreads), 0, s, n, w, norm2, v);
+}
+
+template <typename ZT>
+void combine(double *x, const ZT *Z, i64 ld, int count, const double *y, i64 n, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL((k_combine<ZT>), dim3(grid_for(n)), dim3(kThreads), 0, s, x, Z, ld, count,
+                       y, n);
+}
+
+// explicit instantiations
+template void shifted_apply<double>(const csr_view<double> &, double, const double *,
+                                    const double *, const double *, double *, double *,
+                                    hipStream_t);
+template void shifted_apply<float>(const csr_view<double> &, double, const double *,
+                                   const float *, const double *, double *, double *,
+                                   hipStream_t);
+#define MK_LEVEL(T)                                                                             \
+    template void jacobi_sweep<T>(const csr_view<T> &, double, double, const T *, const T *,    \
+                                  const T *, T *, hipStream_t);                                 \
+    template void level_residual<T>(const csr_view<T> &, double, const T *, const T *,          \
+                                    const T *, T *, hipStream_t);                               \
+    template void restrict_children<T>(i64, const u32 *, const u32 *, const T *, T *,           \
+                                       hipStream_t);                                            \
+    template void prolong_add<T>(i64, const u32 *, const T *, const T *, T *, hipStream_t);     \
+    template void terminal_solve<T>(i64, const u32 *, const u32 *, const u64 *, const T *, i64, \
+                                    const T *, T *, hipStream_t);                               \
+    template void fill<T>(i64, T, T *, hipStream_t);                                            \
+    template void axpy<T>(i64, T, const T *, T *, hipStream_t);                                 \
+    template void residual_update<T>(i64, const T *, const T *, T *, hipStream_t);              \
+    template void combine<T>(double *, const T *, i64, int, const double *, i64, hipStream_t);
+MK_LEVEL(float)
+MK_LEVEL(double)
+#undef MK_LEVEL
+template void jacobi_first<double, double>(i64, double, double, const double *, const double *,
+                                           double *, double *, hipStream_t);
+template void jacobi_first<float, double>(i64, double, double, const float *, const double *,
+                                          float *, float *, hipStream_t);
+template void jacobi_first<float, float>(i64, double, double, const float *, const float *,
+                                         float *, float *, hipStream_t);
+template void convert<double, float>(i64, const double *, float *, hipStream_t);
+template void convert<float, double>(i64, const float *, double *, hipStream_t);
+template void convert<double, double>(i64, const double *, double *, hipStream_t);
+template void convert<float, float>(i64, const float *, float *, hipStream_t);
+
+} // namespace mk::gpu

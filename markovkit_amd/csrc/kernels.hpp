@@ -1,0 +1,95 @@
+// Device kernels of the resolvent solve (gfx950, wave64) and their launchers.
+//
+// Sparse layout: every operator is stored by destination ("gather" CSR): row i lists the
+// jumps j -> i as (column j, positive rate), so y_i = (shift + exit_i) x_i - sum rate x_j.
+// Rows are grouped into row blocks whose entries fit one workgroup's LDS tile; a workgroup
+// streams its block's columns and values with coalesced loads, stages the products in LDS,
+// and each thread then sums one row (CSR-stream). Blocks holding one row longer than the tile
+// fall back to a workgroup-wide reduction of that row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mk::gpu {
+
+using i64 = std::int64_t;
+using u64 = std::uint64_t;
+using u32 = std::uint32_t;
+
+constexpr int kThreads = 256;     // workgroup size (4 wave64)
+constexpr int kRowsPerBlock = 256; // rows per CSR row block (one per thread)
+constexpr int kTile = 4096;       // LDS product tile (entries) per row block
+constexpr int kDotChunk = 8;      // basis vectors per multi-dot pass
+
+// device view of a gather CSR operator
+template <typename VT>
+struct csr_view {
+    i64 n = 0;
+    const u32 *ptr = nullptr;    // n + 1
+    const u32 *col = nullptr;
+    const VT *val = nullptr;
+    const u32 *blocks = nullptr; // n_blocks + 1 row boundaries
+    i64 n_blocks = 0;
+};
+
+// y = (shift + exit) o x - C x            (shifted generator, FP64 accumulate)
+// with b: y = b - ((shift + exit) o x - C x) and, if partial != null, per-workgroup sums of
+// y^2 in partial[blockIdx]
+template <typename XT>
+void shifted_apply(const csr_view<double> &A, double shift, const double *exit, const XT *x,
+                   const double *b, double *y, double *partial, hipStream_t s);
+
+// Jacobi smoothing on one multigrid level, working precision T:
+//   first sweep from x = 0:  rhs_out = (T) rhs_in;  x = w * rhs / (shift + exit)
+template <typename T, typename IN>
+void jacobi_first(i64 n, double shift, double omega, const T *exit, const IN *rhs_in, T *rhs_out,
+                  T *x, hipStream_t s);
+//   sweep: x_out = x + w * (rhs - A x) / (shift + exit)
+template <typename T>
+void jacobi_sweep(const csr_view<T> &A, double shift, double omega, const T *exit, const T *rhs,
+                  const T *x, T *x_out, hipStream_t s);
+//   residual: r = rhs - A x
+template <typename T>
+void level_residual(const csr_view<T> &A, double shift, const T *exit, const T *rhs, const T *x,
+                    T *r, hipStream_t s);
+// restriction by coarse rows: coarse[c] = sum of fine[children of c] in ascending order
+template <typename T>
+void restrict_children(i64 n_coarse, const u32 *cptr, const u32 *cidx, const T *fine, T *coarse,
+                       hipStream_t s);
+// prolongation fused with the correction: x[i] += weight[i] * coarse[coarse_of[i]]
+template <typename T>
+void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x,
+                 hipStream_t s);
+// dense terminal solves: x[start_b + i] += sum_j inv_b[i][j] * r[start_b + j]
+template <typename T>
+void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
+                    const T *inverse, i64 max_size, const T *r, T *x, hipStream_t s);
+
+template <typename T>
+void fill(i64 n, T value, T *x, hipStream_t s);
+template <typename A, typename B>
+void convert(i64 n, const A *in, B *out, hipStream_t s);
+template <typename T>
+void axpy(i64 n, T alpha, const T *x, T *y, hipStream_t s);   // y += alpha x
+template <typename T>
+void residual_update(i64 n, const T *rhs, const T *ax, T *r, hipStream_t s); // r = rhs - ax
+
+// ---- Krylov vector kernels (FP64) ----
+int reduction_blocks(i64 n);
+// partial[k * nb + block] = sum over the block's rows of V_k . w, k in [0, count)
+void multi_dot(const double *V, i64 ld, int count, const double *w, i64 n, double *partial,
+               int nb, hipStream_t s);
+// w -= sum_k h[k] V_k; if partial != null, partial[block] = sum of w^2 over the block's rows
+void basis_update(double *w, const double *V, i64 ld, int count, const double *h, i64 n,
+                  double *partial, int nb, hipStream_t s);
+// out[k] = sum_b partial[k * nb + b] (fixed order), k in [0, count)
+void reduce_partials(const double *partial, int nb, int count, double *out, hipStream_t s);
+// v = w / sqrt(norm2[0])
+void scale_by_norm(i64 n, const double *w, const double *norm2, double *v, hipStream_t s);
+// x += sum_k y[k] Z_k (Z stored in ZT)
+template <typename ZT>
+void combine(double *x, const ZT *Z, i64 ld, int count, const double *y, i64 n, hipStream_t s);
+
+} // namespace mk::gpu

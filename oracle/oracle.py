@@ -1,0 +1,225 @@
+"""ORACLE binding — test infrastructure only.
+
+ctypes access to ``oracle/liboracle.so``, the FP64 CPU restatement of MarkovKit's RDME
+resolvent path (``oracle/mk_oracle.hpp``; reference citations there). Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import this module, and only
+as the checker or the timed CPU baseline; the product (``markovkit_amd``) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        i64p = np.ctypeslib.ndpointer(np.int64, flags="C")
+        f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C")
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_create.argtypes = [C.c_int, P, P, C.c_int64, P, P, P, P, C.c_int, P, P, C.c_int,
+                                 P, P, C.c_int, C.POINTER(C.c_void_p)]
+        L.orc_destroy.argtypes = [P]
+        for name in ("orc_size", "orc_n_sectors", "orc_n_jumps"):
+            getattr(L, name).argtypes = [P]
+            getattr(L, name).restype = C.c_int64
+        L.orc_sector_begins.argtypes = [P, i64p]
+        L.orc_states.argtypes = [P, i32p]
+        L.orc_find.argtypes = [P, i32p]
+        L.orc_find.restype = C.c_int64
+        L.orc_rates.argtypes = [P, f64p, f64p]
+        L.orc_jumps.argtypes = [P, i64p, i64p, f64p]
+        L.orc_apply.argtypes = [P, f64p, f64p]
+        L.orc_apply_shifted.argtypes = [P, C.c_double, f64p, f64p]
+        L.orc_mg_create.argtypes = [P, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int64,
+                                    C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.orc_mg_destroy.argtypes = [P]
+        L.orc_mg_max_levels.argtypes = [P]
+        L.orc_mg_max_levels.restype = C.c_int64
+        L.orc_mg_apply.argtypes = [P, f64p, f64p]
+        L.orc_mg_level_rows.argtypes = [P, i64p]
+        L.orc_mg_transfer.argtypes = [P, C.c_int64, C.c_int64, i64p, f64p]
+        L.orc_mg_level_apply.argtypes = [P, C.c_int64, C.c_int64, f64p, f64p]
+        L.orc_gmres.argtypes = [P, P, C.c_double, f64p, f64p, C.c_double, C.c_int64, C.c_int64,
+                                C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.orc_gmres.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        msg = lib().orc_last_error().decode()
+        if status == 1:
+            raise ValueError(msg)
+        if status == 2:
+            raise OverflowError(msg)
+        raise RuntimeError(msg)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Problem:
+    """population_box + reaction_diffusion + generator on the CPU (FP64)."""
+
+    def __init__(self, model, max_totals, absorbing=True):
+        from markovkit_amd.model import Grid
+        L = lib()
+        self.model = model
+        self.max_totals = np.asarray(max_totals, np.int32)
+        dom = model.domain
+        S = dom.n_species()
+        V = dom.n_voxels()
+        self._keep = []
+        if isinstance(dom.graph, Grid):
+            g = dom.graph
+            dims = np.asarray(g.dims, np.int64)
+            spacing = np.asarray(g.spacing, np.float64)
+            vols = np.ascontiguousarray(dom.volumes, np.float64)
+            args = [g.dim, _ptr(dims), _ptr(spacing), V, None, None, None, _ptr(vols)]
+            self._keep += [dims, spacing, vols]
+        else:
+            off, to, w = dom.graph.adjacency()
+            vols = np.ascontiguousarray(dom.volumes, np.float64)
+            args = [0, None, None, V, _ptr(off), _ptr(to), _ptr(w), _ptr(vols)]
+            self._keep += [off, to, w, vols]
+        diff = np.ascontiguousarray(dom.diffusivities, np.float64)
+        changes = np.ascontiguousarray(model.changes(), np.int32)
+        rates = np.ascontiguousarray(model.rate_tables(self.max_totals), np.float64)
+        self._keep += [diff, changes, rates]
+        h = C.c_void_p()
+        _check(L.orc_create(*args, S, _ptr(diff), _ptr(self.max_totals), len(model.reactions),
+                            _ptr(changes), _ptr(rates), 1 if absorbing else 0, C.byref(h)))
+        self.h = h
+        self.n = L.orc_size(h)
+        self.S = S
+        self.V = V
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def sector_begins(self):
+        k = lib().orc_n_sectors(self.h)
+        out = np.zeros(k + 1, np.int64)
+        lib().orc_sector_begins(self.h, out)
+        return out
+
+    def states(self):
+        out = np.zeros(self.n * self.V * self.S, np.int32)
+        lib().orc_states(self.h, out)
+        return out.reshape(self.n, self.V * self.S)
+
+    def find(self, x):
+        return lib().orc_find(self.h, np.ascontiguousarray(x, np.int32))
+
+    def rates(self):
+        e = np.zeros(self.n)
+        l = np.zeros(self.n)
+        lib().orc_rates(self.h, e, l)
+        return e, l
+
+    def jumps(self):
+        m = lib().orc_n_jumps(self.h)
+        first = np.zeros(self.n + 1, np.int64)
+        to = np.zeros(m, np.int64)
+        rate = np.zeros(m)
+        lib().orc_jumps(self.h, first, to, rate)
+        return first, to, rate
+
+    def apply(self, x):
+        y = np.zeros(self.n)
+        lib().orc_apply(self.h, np.ascontiguousarray(x, np.float64), y)
+        return y
+
+    def apply_shifted(self, shift, x):
+        y = np.zeros(self.n)
+        lib().orc_apply_shifted(self.h, float(shift), np.ascontiguousarray(x, np.float64), y)
+        return y
+
+    def matrix(self, shift=0.0, sign=1.0):
+        """sign * (Q - shift I) as a scipy CSR matrix (generator.hpp:93-107)."""
+        import scipy.sparse as sp
+        first, to, rate = self.jumps()
+        exit_, _ = self.rates()
+        cols = np.repeat(np.arange(self.n), np.diff(first))
+        rows = np.concatenate([np.arange(self.n), to])
+        cc = np.concatenate([np.arange(self.n), cols])
+        vals = np.concatenate([-sign * (exit_ + shift), sign * rate])
+        return sp.csr_matrix((vals, (rows, cc)), shape=(self.n, self.n))
+
+    def gmres(self, shift, b, x0=None, mg=None, tol=1e-10, max_iterations=1000, restart=30):
+        x = np.zeros(self.n) if x0 is None else np.array(x0, np.float64)
+        it = C.c_int64()
+        res = C.c_double()
+        conv = lib().orc_gmres(self.h, mg.h if mg is not None else None, float(shift),
+                               np.ascontiguousarray(b, np.float64), x, float(tol),
+                               int(max_iterations), int(restart), C.byref(it), C.byref(res))
+        return x, dict(iterations=it.value, residual=res.value, converged=bool(conv))
+
+
+class SectorMultigrid:
+    """sector_multigrid on the CPU (Jacobi smoother)."""
+
+    def __init__(self, problem, shift, pre_sweeps=2, post_sweeps=2, jacobi_weight=2.0 / 3.0,
+                 coarsest_size=4, max_levels=32, coarse_cycles=1, sweep_growth=0):
+        h = C.c_void_p()
+        _check(lib().orc_mg_create(problem.h, float(shift), pre_sweeps, post_sweeps,
+                                   jacobi_weight, coarsest_size, max_levels, coarse_cycles,
+                                   sweep_growth, C.byref(h)))
+        self.h = h
+        self.problem = problem
+        self.n = problem.n
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mg_destroy(self.h)
+            self.h = None
+
+    def max_levels(self):
+        return lib().orc_mg_max_levels(self.h)
+
+    def level_rows(self):
+        k = len(self.problem.sector_begins()) - 1
+        L = self.max_levels()
+        out = np.zeros(k * L, np.int64)
+        lib().orc_mg_level_rows(self.h, out)
+        return out.reshape(k, L)
+
+    def apply(self, b):
+        x = np.zeros(self.n)
+        lib().orc_mg_apply(self.h, np.ascontiguousarray(b, np.float64), x)
+        return x
+
+    def transfer(self, sector, level):
+        rows = self.level_rows()[sector, level]
+        co = np.zeros(rows, np.int64)
+        w = np.zeros(rows)
+        _check(lib().orc_mg_transfer(self.h, sector, level, co, w))
+        return co, w
+
+    def level_apply(self, sector, level, x):
+        y = np.zeros(len(x))
+        _check(lib().orc_mg_level_apply(self.h, sector, level,
+                                        np.ascontiguousarray(x, np.float64), y))
+        return y

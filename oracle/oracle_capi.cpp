@@ -1,0 +1,260 @@
+// ORACLE C API — test infrastructure only (tests/, __graft_entry__.smoke(), bench.py's
+// cpu_baseline leg). Plain-pointer entry points over mk_oracle.hpp so Python can drive the
+// FP64 CPU restatement of the reference path on the same inputs as the HIP library.
+//
+// Local reaction rates are passed as tables: rates[(r * V + v) * T + t] is reaction r's rate
+// in voxel v when that voxel holds the count tuple t = sum_s n_s * stride_s (species 0 most
+// significant, radix max_totals[s] + 1). Every state of a population_box holds at most
+// max_totals[s] molecules of species s in a voxel, so the table covers every evaluation the
+// reference's std::function rates receive (reaction_diffusion.hpp:99-116). A reference rate
+// sees only (counts, volume), so the oracle's closures look the table up by volume: the
+// first voxel with that volume supplies the row.
+#include "mk_oracle.hpp"
+
+#include <cstring>
+#include <exception>
+#include <string>
+
+using namespace mk_oracle;
+
+namespace {
+
+thread_local std::string last_error;
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::invalid_argument &e) {
+        last_error = e.what();
+        return 1;
+    } catch (const std::overflow_error &e) {
+        last_error = e.what();
+        return 2;
+    } catch (const std::exception &e) {
+        last_error = e.what();
+        return 3;
+    }
+}
+
+struct problem {
+    std::unique_ptr<voxel_domain> domain;
+    std::unique_ptr<population_box> box;
+    std::unique_ptr<reaction_diffusion> model;
+    std::unique_ptr<generator> Q;
+};
+
+voxel_domain make_domain(int dim, const int64_t *dims, const double *spacing, int64_t n_voxels,
+                         const int64_t *adj_offsets, const int64_t *adj_to,
+                         const double *adj_weight, const double *volumes, int n_species,
+                         const double *diffusivities) {
+    std::vector<real> diff(diffusivities, diffusivities + n_species);
+    if (dim > 0) {
+        std::vector<idx> d(dims, dims + dim);
+        std::vector<real> h(spacing, spacing + dim);
+        grid g(d, h);
+        if (volumes) return voxel_domain(g, std::vector<real>(volumes, volumes + g.n_vertices()), diff);
+        return voxel_domain(g, diff);
+    }
+    std::vector<std::vector<edge>> adj(n_voxels);
+    for (int64_t v = 0; v < n_voxels; ++v)
+        for (int64_t e = adj_offsets[v]; e < adj_offsets[v + 1]; ++e)
+            adj[v].push_back({static_cast<idx>(adj_to[e]), adj_weight[e]});
+    return voxel_domain(std::move(adj), std::vector<real>(volumes, volumes + n_voxels), diff);
+}
+
+struct rate_table {
+    std::vector<real> values; // [voxel][tuple]
+    std::vector<real> volumes;
+    std::vector<idx> stride;
+    idx T = 1;
+    real operator()(const int *counts, real volume) const {
+        idx v = 0;
+        while (v < volumes.size() && volumes[v] != volume) ++v;
+        if (v == volumes.size()) throw std::invalid_argument("oracle: rate asked at an unknown volume");
+        idx t = 0;
+        for (idx s = 0; s < stride.size(); ++s) t += static_cast<idx>(counts[s]) * stride[s];
+        return values[v * T + t];
+    }
+};
+
+} // namespace
+
+extern "C" {
+
+const char *orc_last_error() { return last_error.c_str(); }
+
+// dim > 0: structured grid (dims, spacing, optional volumes); dim == 0: explicit voxel graph
+// (CSR adjacency, volumes). Builds the box, the model and its generator.
+int orc_create(int dim, const int64_t *dims, const double *spacing, int64_t n_voxels,
+               const int64_t *adj_offsets, const int64_t *adj_to, const double *adj_weight,
+               const double *volumes, int n_species, const double *diffusivities,
+               const int *max_totals, int n_reactions, const int *changes, const double *rates,
+               int absorbing, void **out) {
+    return guarded([&] {
+        auto p = std::make_unique<problem>();
+        p->domain = std::make_unique<voxel_domain>(make_domain(dim, dims, spacing, n_voxels,
+                                                               adj_offsets, adj_to, adj_weight,
+                                                               volumes, n_species, diffusivities));
+        const idx V = p->domain->n_voxels();
+        std::vector<int> caps(max_totals, max_totals + n_species);
+        p->box = std::make_unique<population_box>(V, caps);
+        idx T = 1;
+        std::vector<idx> stride(n_species, 1);
+        for (int s = n_species; s-- > 0;) {
+            stride[s] = T;
+            T *= static_cast<idx>(caps[s] + 1);
+        }
+        p->model = std::make_unique<reaction_diffusion>(reaction_diffusion{*p->domain, {}});
+        for (int r = 0; r < n_reactions; ++r) {
+            rate_table table;
+            table.values.assign(rates + static_cast<idx>(r) * V * T,
+                                rates + static_cast<idx>(r + 1) * V * T);
+            table.volumes = p->domain->volumes;
+            table.stride = stride;
+            table.T = T;
+            std::vector<int> change(changes + r * n_species, changes + (r + 1) * n_species);
+            p->model->reactions.push_back({change, table});
+        }
+        p->Q = std::make_unique<generator>(*p->box, *p->model,
+                                           absorbing ? boundary::absorbing : boundary::reflecting);
+        *out = p.release();
+    });
+}
+
+void orc_destroy(void *h) { delete static_cast<problem *>(h); }
+
+int64_t orc_size(void *h) { return static_cast<int64_t>(static_cast<problem *>(h)->box->size()); }
+int64_t orc_n_sectors(void *h) {
+    return static_cast<int64_t>(static_cast<problem *>(h)->box->n_sectors());
+}
+int64_t orc_n_jumps(void *h) { return static_cast<int64_t>(static_cast<problem *>(h)->Q->n_jumps()); }
+
+void orc_sector_begins(void *h, int64_t *out) {
+    const auto &box = *static_cast<problem *>(h)->box;
+    for (idx k = 0; k <= box.n_sectors(); ++k)
+        out[k] = static_cast<int64_t>(k < box.n_sectors() ? box.sector_begin(k) : box.size());
+}
+
+void orc_states(void *h, int *out) {
+    const auto &box = *static_cast<problem *>(h)->box;
+    for (idx i = 0; i < box.size(); ++i) std::memcpy(out + i * box.width(), box[i], box.width() * sizeof(int));
+}
+
+int64_t orc_find(void *h, const int *x) {
+    const auto &box = *static_cast<problem *>(h)->box;
+    auto i = box.find(x, box.width());
+    return i ? static_cast<int64_t>(*i) : -1;
+}
+
+void orc_rates(void *h, double *exit, double *leak) {
+    const auto &Q = *static_cast<problem *>(h)->Q;
+    std::copy(Q.exit_rates().begin(), Q.exit_rates().end(), exit);
+    std::copy(Q.leak_rates().begin(), Q.leak_rates().end(), leak);
+}
+
+// source-major jump lists: first[n+1], to[n_jumps], rate[n_jumps]
+void orc_jumps(void *h, int64_t *first, int64_t *to, double *rate) {
+    const auto &Q = *static_cast<problem *>(h)->Q;
+    for (idx j = 0; j <= Q.rows(); ++j) first[j] = static_cast<int64_t>(Q.first()[j]);
+    const auto &J = Q.all_jumps();
+    for (idx k = 0; k < J.size(); ++k) {
+        to[k] = static_cast<int64_t>(J[k].to);
+        rate[k] = J[k].rate;
+    }
+}
+
+void orc_apply(void *h, const double *x, double *y) { static_cast<problem *>(h)->Q->apply(x, y); }
+
+void orc_apply_shifted(void *h, double shift, const double *x, double *y) {
+    static_cast<problem *>(h)->Q->apply_shifted(shift, x, y);
+}
+
+int orc_mg_create(void *h, double shift, int pre, int post, double omega, int64_t coarsest,
+                  int max_levels, int coarse_cycles, int sweep_growth, void **out) {
+    return guarded([&] {
+        auto *p = static_cast<problem *>(h);
+        multigrid_options o;
+        o.pre_sweeps = pre;
+        o.post_sweeps = post;
+        o.jacobi_weight = omega;
+        o.coarsest_size = static_cast<idx>(coarsest);
+        o.max_levels = max_levels;
+        o.coarse_cycles = coarse_cycles;
+        o.sweep_growth = sweep_growth;
+        *out = new sector_multigrid(*p->domain, *p->box, shift, o);
+    });
+}
+
+void orc_mg_destroy(void *mg) { delete static_cast<sector_multigrid *>(mg); }
+
+int64_t orc_mg_max_levels(void *mg) {
+    return static_cast<int64_t>(static_cast<sector_multigrid *>(mg)->max_levels());
+}
+
+void orc_mg_apply(void *mg, const double *b, double *x) {
+    static_cast<sector_multigrid *>(mg)->apply(b, x);
+}
+
+// per-sector level structure: out[k * max_levels + l] = rows of sector k on level l (0 past
+// its last level)
+void orc_mg_level_rows(void *mg, int64_t *out) {
+    const auto &M = *static_cast<sector_multigrid *>(mg);
+    const idx L = M.max_levels();
+    for (idx k = 0; k < M.n_sectors(); ++k) {
+        const population_multigrid *lvl = &M.sector(k);
+        for (idx l = 0; l < L; ++l) {
+            out[k * L + l] = lvl ? static_cast<int64_t>(lvl->rows()) : 0;
+            lvl = lvl ? lvl->coarse_level() : nullptr;
+        }
+    }
+}
+
+// the fine-to-coarse transfer of sector k on level l: coarse_of[fine], weight[fine]
+int orc_mg_transfer(void *mg, int64_t k, int64_t level, int64_t *coarse_of, double *weight) {
+    return guarded([&] {
+        const population_multigrid *lvl = &static_cast<sector_multigrid *>(mg)->sector(k);
+        for (int64_t l = 0; l < level && lvl; ++l) lvl = lvl->coarse_level();
+        if (!lvl || !lvl->coarse_level()) throw std::invalid_argument("no transfer on that level");
+        const auto &t = lvl->transfer();
+        for (idx i = 0; i < t.fine_size(); ++i) {
+            coarse_of[i] = static_cast<int64_t>(t.coarse_of()[i]);
+            weight[i] = t.weight()[i];
+        }
+    });
+}
+
+// y = A_l x for sector k on level l (A_l = shift I - Q_diffusion on that level)
+int orc_mg_level_apply(void *mg, int64_t k, int64_t level, const double *x, double *y) {
+    return guarded([&] {
+        const population_multigrid *lvl = &static_cast<sector_multigrid *>(mg)->sector(k);
+        for (int64_t l = 0; l < level && lvl; ++l) lvl = lvl->coarse_level();
+        if (!lvl) throw std::invalid_argument("no such level");
+        lvl->apply_system(x, y);
+    });
+}
+
+// restarted GMRES on (shift I - Q) x = b, preconditioned by mg when not null
+int orc_gmres(void *h, void *mg, double shift, const double *b, double *x, double tol,
+              int64_t max_it, int64_t restart, int64_t *iterations, double *residual) {
+    auto *p = static_cast<problem *>(h);
+    auto *M = static_cast<sector_multigrid *>(mg);
+    gmres_options o;
+    o.tolerance = tol;
+    o.max_iterations = static_cast<idx>(max_it);
+    o.restart = static_cast<idx>(restart);
+    const idx n = p->box->size();
+    auto A = [&](const real *in, real *out) { p->Q->apply_shifted(shift, in, out); };
+    solver_result r;
+    if (M) {
+        r = gmres(n, A, [&](const real *v, real *z) { M->apply(v, z); }, b, x, o);
+    } else {
+        r = gmres(n, A, [&](const real *v, real *z) { std::copy(v, v + n, z); }, b, x, o);
+    }
+    *iterations = static_cast<int64_t>(r.iterations);
+    *residual = r.residual;
+    return r.converged ? 1 : 0;
+}
+
+} // extern "C"

@@ -1,0 +1,177 @@
+"""GPU parity: the HIP library (through its C ABI) against the FP64 CPU oracle on the same
+seeded inputs, plus size-independent properties at the full 11.2M-state size.
+
+Tolerances: the shifted-generator SpMV and the FP64 multigrid cycle differ from the oracle
+only in summation order (gather by destination vs scatter by source), so they must agree to
+64 ulps of the absolute row sums. The FP32 multigrid is compared at 2e-5 relative. Resolvent
+solutions must reach the 1e-10 relative residual and agree with the oracle's solution within
+the L1 bound ||x - x*||_1 <= ||b - A x||_1 / s (rdme/cpp/rdme_full_multigrid.cpp:884-887) of
+both solves.
+"""
+import numpy as np
+import pytest
+
+import markovkit_amd as mk
+from markovkit_amd.workloads import (annihilation_2d, birth_death_1d, config_large,
+                                     config_small, lumped_sector_sums)
+from oracle.oracle import Problem, SectorMultigrid as OracleMG
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(np.float64).eps
+
+
+def small_cases():
+    return [
+        ("1d_v16_cap3", config_small()),
+        ("1d_v8_cap2_rand", birth_death_1d(8, 2, 0.7, "random")),
+        ("2d_4x4_caps22", annihilation_2d(4, (2, 2), 1.0)),
+        ("1d_v32_cap4", birth_death_1d(32, 4, 0.5, "random")),
+        ("1d_v12_cap3_odd", birth_death_1d(12, 3, 2.0, "random")),  # 12 -> 6 -> 3 stops coarsening
+    ]
+
+
+@pytest.fixture(scope="module", params=small_cases(), ids=lambda c: c[0])
+def case(request):
+    name, w = request.param
+    print(f"[case {name}] states={w.states}", flush=True)
+    box = mk.PopulationBox(w.n_voxels, w.max_totals)
+    Q = mk.Generator(box, w.model)
+    P = Problem(w.model, w.max_totals)
+    return dict(name=name, w=w, box=box, Q=Q, P=P)
+
+
+def abs_rowsum(P, shift, x):
+    A = P.matrix(shift, -1.0)
+    A.data = np.abs(A.data)
+    return A @ np.abs(x)
+
+
+def test_generator_rates_match(case):
+    ex, lk = case["P"].rates()
+    assert case["Q"].rows() == case["P"].n
+    np.testing.assert_array_equal(case["Q"].exit_rates(), ex)
+    np.testing.assert_array_equal(case["Q"].leak_rates(), lk)
+    assert case["Q"].n_jumps() == len(case["P"].jumps()[1])
+
+
+def test_shifted_apply(case):
+    P, Q = case["P"], case["Q"]
+    rng = np.random.default_rng(7)
+    shift = case["w"].shifts[0]
+    for trial in range(3):
+        x = rng.standard_normal(P.n)
+        y = Q.apply_shifted(shift, x)
+        y_ref = P.apply_shifted(shift, x)
+        scale = abs_rowsum(P, shift, x)
+        assert np.all(np.abs(y - y_ref) <= 64 * EPS * scale + 1e-300), case["name"]
+    # generator::apply (y = Q x)
+    x = rng.random(P.n)
+    np.testing.assert_allclose(Q.apply(x), P.apply(x), rtol=0,
+                               atol=64 * EPS * abs_rowsum(P, 0.0, x).max())
+
+
+def test_mass_balance(case):
+    """1^T (sI - Q) x = sum_j (s + leak_j) x_j (rdme_full_multigrid.cpp:955-962)."""
+    P, Q = case["P"], case["Q"]
+    shift = case["w"].shifts[0]
+    x = np.random.default_rng(3).random(P.n)
+    y = Q.apply_shifted(shift, x)
+    lhs = y.sum()
+    rhs = ((shift + Q.leak_rates()) * x).sum()
+    assert abs(lhs - rhs) <= 1e-12 * abs(rhs)
+
+
+@pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
+def test_multigrid_cycle(case, precision):
+    w, P, box = case["w"], case["P"], case["box"]
+    shift = w.shifts[0]
+    M = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=precision))
+    Mo = OracleMG(P, shift)
+    assert M.max_levels() == Mo.max_levels()
+    assert M.n_sectors() == len(P.sector_begins()) - 1
+    rng = np.random.default_rng(11)
+    for trial in range(2):
+        b = rng.standard_normal(P.n)
+        z = M.apply(b)
+        z_ref = Mo.apply(b)
+        err = np.abs(z - z_ref).max() / np.abs(z_ref).max()
+        tol = 1e-12 if precision == mk.FP64 else 2e-5
+        assert err <= tol, (case["name"], precision, err)
+
+
+@pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
+def test_resolvent_solve(case, precision):
+    w, P, box, Q = case["w"], case["P"], case["box"], case["Q"]
+    shift = w.shifts[0]
+    b = w.rhs(0, P.n)
+    tol = 1e-10 * np.linalg.norm(b)
+    A = mk.ShiftedGenerator(Q, shift)
+    M = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=precision))
+    x = np.zeros(P.n)
+    res = mk.gmres(A, M, b, x, tolerance=tol, max_iterations=500, restart=50)
+    assert res.converged, res
+    r = b - P.apply_shifted(shift, x)
+    assert np.linalg.norm(r) <= 1.0001 * tol
+    assert abs(res.residual - np.linalg.norm(r)) <= 1e-3 * tol
+    Mo = OracleMG(P, shift)
+    x_ref, info = P.gmres(shift, b, mg=Mo, tol=tol, max_iterations=500, restart=50)
+    assert info["converged"]
+    r_ref = b - P.apply_shifted(shift, x_ref)
+    bound = (np.abs(r).sum() + np.abs(r_ref).sum()) / shift
+    assert np.abs(x - x_ref).sum() <= bound, (np.abs(x - x_ref).sum(), bound)
+    print(f"[{case['name']}] gpu its={res.iterations} oracle its={info['iterations']} "
+          f"l1={np.abs(x - x_ref).sum():.3e} bound={bound:.3e}")
+
+
+def test_plain_gmres(case):
+    w, P, Q = case["w"], case["P"], case["Q"]
+    if P.n > 20000:
+        pytest.skip("plain GMRES is only checked on the small boxes")
+    shift = w.shifts[0]
+    b = w.rhs(0, P.n)
+    tol = 1e-10 * np.linalg.norm(b)
+    x = np.zeros(P.n)
+    res = mk.gmres(mk.ShiftedGenerator(Q, shift), b, x, tolerance=tol, max_iterations=2000,
+                   restart=100)
+    assert res.converged
+    r = b - P.apply_shifted(shift, x)
+    assert np.linalg.norm(r) <= 1.0001 * tol
+
+
+def test_nonconvergence_is_reported(case):
+    w, Q, box = case["w"], case["Q"], case["box"]
+    shift = w.shifts[0]
+    b = w.rhs(0, Q.rows())
+    x = np.zeros(Q.rows())
+    res = mk.gmres(mk.ShiftedGenerator(Q, shift), b, x, tolerance=1e-300, max_iterations=2,
+                   restart=50)
+    assert not res.converged and res.iterations == 2
+
+
+def test_large_box_lumped_solution():
+    """11,238,513-state 1D box (V=64, cap 5), s=0.5: the FP32-multigrid FGMRES solution's
+    sector sums equal the lumped closed form, and the true FP64 residual is <= 1e-10 ||b||."""
+    w = config_large()
+    print(f"[large] building {w.states} states", flush=True)
+    box = mk.PopulationBox(w.n_voxels, w.max_totals)
+    assert box.size() == 11238513
+    Q = mk.Generator(box, w.model)
+    shift = w.shifts[0]
+    b = w.rhs(0, box.size())
+    tol = 1e-10 * np.linalg.norm(b)
+    M = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=mk.FP32))
+    x = np.zeros(box.size())
+    res = mk.gmres(mk.ShiftedGenerator(Q, shift), M, b, x, tolerance=tol, max_iterations=300,
+                   restart=50)
+    print(f"[large] its={res.iterations} residual={res.residual:.3e}", flush=True)
+    assert res.converged
+    r = b - Q.apply_shifted(shift, x)
+    assert np.linalg.norm(r) <= 1.0001 * tol
+    # mass balance on the solution: 1^T b = sum (s + leak) x
+    assert abs(b.sum() - ((shift + Q.leak_rates()) * x).sum()) <= 1e-9
+    begins = [box.sector_begin(k) for k in range(box.n_sectors())] + [box.size()]
+    Bk = np.array([b[begins[k]:begins[k + 1]].sum() for k in range(6)])
+    Xk = np.array([x[begins[k]:begins[k + 1]].sum() for k in range(6)])
+    X = lumped_sector_sums(shift, 0.05, 0.02, 5, Bk)
+    assert np.abs(Xk - X).sum() <= np.abs(r).sum() / shift + 1e-13

@@ -1,0 +1,249 @@
+#!/usr/bin/env python3
+"""RDME resolvent-solve benchmark on one MI355X.
+
+Workload (default): 1D RDME on 64 voxels (h = 1/64, D = 1, births 0.05 * volume, deaths
+0.02 * n), box cap 5 = 11,238,513 states, one shift s = 0.5, b ~ U[0,1) (numpy PCG64, seed 0)
+normalised to unit L1 norm. One step = one resolvent solve (sI - Q) x = b from x = 0 to a
+true FP64 relative residual <= 1e-10 by flexible GMRES(50) with CGS2, preconditioned by
+sector_multigrid (Jacobi V-cycle, pre = post = 2, w = 2/3). b, x, the operator and the
+hierarchy are resident in HBM before the timed region starts.
+
+value = solves/s of the mixed path (FP32 multigrid inside FP64 FGMRES); the plain FP64 path
+(FP64 multigrid) is timed beside it. The roofline of the dominant kernel class comes from HIP
+events recorded around that class's launches inside the timed mixed region; cpu_baseline is
+the FP64 CPU oracle timed on a bounded sample of the same solve on this host (1 core).
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "RDME resolvent solves/s + p99 latency at ≤1e-10 residual, mixed vs FP64"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def percentile(v, q):
+    return float(np.percentile(np.asarray(v), q))
+
+
+def run_solves(mk, A, M, b_dev, x_dev, tol, steps, restart, profile_classes=None):
+    lat = []
+    iters = []
+    if profile_classes:
+        mk.profile_reset()
+        mk.profile_enable(profile_classes)
+    mk.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t = time.perf_counter()
+        x_dev.zero()
+        res = mk.gmres(A, M, b_dev, x_dev, tolerance=tol, max_iterations=1000, restart=restart)
+        lat.append(time.perf_counter() - t)
+        iters.append(res.iterations)
+        if not res.converged:
+            raise RuntimeError(f"solve did not converge: {res}")
+    mk.synchronize()
+    total = time.perf_counter() - t0
+    prof = None
+    if profile_classes:
+        prof = mk.profile_read()
+        mk.profile_enable(False)
+    return total, lat, iters, res, prof
+
+
+def cpu_baseline(w, b, iterations, tol, restart):
+    """The FP64 oracle (1 core) on a bounded sample of the same solve: a 1-iteration and a
+    3-iteration FGMRES run give the fixed and per-iteration costs, scaled to the number of
+    iterations the GPU solve needs."""
+    from oracle.oracle import Problem, SectorMultigrid
+    t = time.perf_counter()
+    P = Problem(w.model, w.max_totals)
+    Mo = SectorMultigrid(P, w.shifts[0])
+    setup = time.perf_counter() - t
+    log(f"[cpu] oracle setup {setup:.1f}s")
+    times = {}
+    for k in (1, 3):
+        t = time.perf_counter()
+        P.gmres(w.shifts[0], b, mg=Mo, tol=tol, max_iterations=k, restart=restart)
+        times[k] = time.perf_counter() - t
+        log(f"[cpu] {k} FGMRES iteration(s): {times[k]:.2f}s")
+    per_iter = (times[3] - times[1]) / 2.0
+    fixed = max(times[1] - per_iter, 0.0)
+    solve_s = fixed + iterations * per_iter
+    return {
+        "value": 1.0 / solve_s,
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"FP64 CPU oracle (scatter apply_shifted, per-sector Jacobi V-cycle, restarted "
+                   f"GMRES) on the same {P.n}-state solve: 1- and 3-iteration runs "
+                   f"({times[1]:.2f}s, {times[3]:.2f}s) scaled to {iterations} iterations; "
+                   f"setup {setup:.1f}s excluded"),
+        "seconds_per_solve": solve_s,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="large", choices=["large", "small", "2d"])
+    ap.add_argument("--restart", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (each solve "
+                          "runs on one MI355X; sharding is described in DESIGN.md)"}))
+        sys.exit(2)
+
+    import markovkit_amd as mk
+    from markovkit_amd import workloads
+
+    w = {"large": workloads.config_large, "small": workloads.config_small,
+         "2d": workloads.config_2d}[args.workload]()
+    shift = w.shifts[0]
+    log(f"[setup] {w.name}: {w.states} states")
+    t = time.perf_counter()
+    box = mk.PopulationBox(w.n_voxels, w.max_totals)
+    Q = mk.Generator(box, w.model)
+    t_gen = time.perf_counter() - t
+    log(f"[setup] generator {t_gen:.1f}s, {Q.n_jumps()} jumps")
+    t = time.perf_counter()
+    M32 = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=mk.FP32))
+    t_mg = time.perf_counter() - t
+    log(f"[setup] FP32 multigrid {t_mg:.1f}s, {M32.max_levels()} levels")
+    A = mk.ShiftedGenerator(Q, shift)
+    n = box.size()
+    b = w.rhs(0, n)
+    tol = 1e-10 * float(np.linalg.norm(b))
+    b_dev = mk.DeviceVector(n, b)
+    x_dev = mk.DeviceVector(n)
+
+    # warm-up, and a fully profiled solve to find the dominant kernel class
+    run_solves(mk, A, M32, b_dev, x_dev, tol, max(args.warmup, 1), args.restart)
+    _, _, _, _, prof_all = run_solves(mk, A, M32, b_dev, x_dev, tol, 1, args.restart,
+                                      profile_classes=True)
+    dominant = max(prof_all, key=lambda k: prof_all[k]["ms"])
+    log("[profile] " + ", ".join(f"{k}={v['ms']:.2f}ms/{v['launches']}" for k, v in
+                                 prof_all.items() if v["launches"]))
+
+    # timed mixed region, events only around the dominant class
+    total, lat, iters, res, prof = run_solves(mk, A, M32, b_dev, x_dev, tol, args.steps,
+                                              args.restart, profile_classes=[dominant])
+    x_mixed = x_dev.download()
+    log(f"[mixed] {args.steps} solves in {total:.3f}s, {iters[-1]} iterations, "
+        f"residual {res.residual:.3e}")
+    r = b - Q.apply_shifted(shift, x_mixed)
+    rel_res = float(np.linalg.norm(r) / np.linalg.norm(b))
+    l1_bound = float(np.abs(r).sum() / shift)
+
+    fp64 = None
+    if not args.no_fp64:
+        t = time.perf_counter()
+        M64 = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=mk.FP64))
+        log(f"[setup] FP64 multigrid {time.perf_counter() - t:.1f}s")
+        run_solves(mk, A, M64, b_dev, x_dev, tol, max(args.warmup, 1), args.restart)
+        total64, lat64, iters64, res64, _ = run_solves(mk, A, M64, b_dev, x_dev, tol, args.steps,
+                                                       args.restart)
+        x64 = x_dev.download()
+        log(f"[fp64] {args.steps} solves in {total64:.3f}s, {iters64[-1]} iterations")
+        r64 = b - Q.apply_shifted(shift, x64)
+        fp64 = {
+            "value": args.steps / total64,
+            "p50_ms": 1e3 * percentile(lat64, 50),
+            "p99_ms": 1e3 * percentile(lat64, 99),
+            "iterations": iters64[-1],
+            "relative_residual": float(np.linalg.norm(r64) / np.linalg.norm(b)),
+            "l1_mixed_vs_fp64": float(np.abs(x_mixed - x64).sum()),
+            "l1_bound": float((np.abs(r).sum() + np.abs(r64).sum()) / shift),
+        }
+        del M64
+
+    # sector sums against the lumped closed form (1D linear birth-death boxes)
+    lumped = None
+    if len(w.max_totals) == 1:
+        K = box.n_sectors()
+        begins = [box.sector_begin(k) for k in range(K)] + [n]
+        Bk = np.array([b[begins[k]:begins[k + 1]].sum() for k in range(K)])
+        Xk = np.array([x_mixed[begins[k]:begins[k + 1]].sum() for k in range(K)])
+        X = workloads.lumped_sector_sums(shift, 0.05, 0.02, w.max_totals[0], Bk)
+        lumped = float(np.abs(Xk - X).sum())
+
+    d = prof[dominant]
+    launches = d["launches"]
+    avg_ms = d["ms"] / launches
+    bytes_per_launch = d["bytes"] / launches
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{dominant}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(w.name, {}).get("bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": args.steps / total,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (f32 multigrid)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{w.name}: 1D RDME resolvent, {n} states, s={shift}, "
+                        f"{'b~U[0,1) seed 0, L1-normalised' if w.rhs_kind == 'random' else 'b=e_0'}, "
+                        f"FGMRES({args.restart}) + sector multigrid, tol 1e-10 relative",
+            "states": n,
+            "jumps": Q.n_jumps(),
+            "multigrid_levels": M32.max_levels(),
+        },
+        "p50_ms": 1e3 * percentile(lat, 50),
+        "p99_ms": 1e3 * percentile(lat, 99),
+        "iterations": iters[-1],
+        "relative_residual": rel_res,
+        "l1_bound": l1_bound,
+        "lumped_sector_sum_error": lumped,
+        "fp64": fp64,
+        "mixed_over_fp64": (args.steps / total) / fp64["value"] if fp64 else None,
+        "setup_s": {"generator": t_gen, "multigrid_fp32": t_mg},
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": traffic,
+            "launches": launches,
+            "avg_launch_ms": avg_ms,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "kernel_ms_per_solve": {k: v["ms"] for k, v in prof_all.items() if v["launches"]},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, b, iters[-1], tol, args.restart)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

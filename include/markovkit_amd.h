@@ -159,11 +159,13 @@ mk_status mk_device_alloc(int64_t bytes, void **d_ptr);
 mk_status mk_device_free(void *d_ptr);
 mk_status mk_copy_to_device(void *d_dst, const void *src, int64_t bytes);
 mk_status mk_copy_to_host(void *dst, const void *d_src, int64_t bytes);
+mk_status mk_device_zero(void *d_ptr, int64_t bytes);
 mk_status mk_synchronize(void);
-/* per-kernel-class timing with HIP events on the library stream; names are
- * "shifted_spmv", "jacobi_sweep", "residual", "restrict", "prolong", "terminal",
- * "gmres_dot", "gmres_update", "gmres_other" */
-mk_status mk_profile_enable(int32_t on);
+/* per-kernel-class timing with HIP events on the library stream. Classes, in bit order:
+ * "shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict", "prolong",
+ * "terminal", "gmres_dot", "gmres_update", "gmres_other". mask: 0 = off, -1 = all, else the
+ * classes whose bits are set. */
+mk_status mk_profile_enable(int32_t mask);
 mk_status mk_profile_reset(void);
 mk_status mk_profile_read(const char *kernel, int64_t *launches, double *milliseconds,
                           double *bytes);

@@ -159,6 +159,9 @@ class DeviceVector:
         assert a.size == self.n
         check(lib().mk_copy_to_device(self._p, _p(a), self.n * 8))
 
+    def zero(self):
+        check(lib().mk_device_zero(self._p, self.n * 8))
+
     def download(self) -> np.ndarray:
         out = np.empty(self.n, np.float64)
         check(lib().mk_copy_to_host(_p(out), self._p, self.n * 8))
@@ -372,16 +375,26 @@ def gmres(A: ShiftedGenerator, *args, tolerance: float = 1e-10, max_iterations: 
     return SolverResult(res.iterations, res.residual, bool(res.converged))
 
 
-def profile_enable(on: bool = True):
-    check(lib().mk_profile_enable(1 if on else 0))
+KERNEL_CLASSES = ("shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict",
+                  "prolong", "terminal", "gmres_dot", "gmres_update", "gmres_other")
+
+
+def profile_enable(on=True):
+    """Time kernel launches with HIP events: True = every class, False = off, or an iterable
+    of class names."""
+    if on is True:
+        mask = -1
+    elif not on:
+        mask = 0
+    else:
+        mask = 0
+        for name in on:
+            mask |= 1 << KERNEL_CLASSES.index(name)
+    check(lib().mk_profile_enable(mask))
 
 
 def profile_reset():
     check(lib().mk_profile_reset())
-
-
-KERNEL_CLASSES = ("shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict",
-                  "prolong", "terminal", "gmres_dot", "gmres_update", "gmres_other")
 
 
 def profile_read():

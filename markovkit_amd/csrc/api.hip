@@ -138,7 +138,7 @@ const char *const kclass_names[K_COUNT] = {"shifted_spmv", "jacobi_first", "jaco
                                            "gmres_other"};
 
 struct profiler {
-    bool on = false;
+    std::uint32_t mask = 0;
     struct rec {
         int cls;
         hipEvent_t a, b;
@@ -162,7 +162,7 @@ struct profiler {
     }
     template <typename F>
     void run(int cls, double nbytes, F &&launch) {
-        if (!on) {
+        if (!(mask & (1u << cls))) {
             launch();
             check_launch(kclass_names[cls]);
             return;
@@ -1045,14 +1045,21 @@ mk_status mk_copy_to_host(void *dst, const void *d_src, int64_t bytes) {
     });
 }
 
+mk_status mk_device_zero(void *d_ptr, int64_t bytes) {
+    return guarded([&] {
+        require(bytes >= 0, "device_zero: negative size");
+        if (bytes > 0) HIPCHK(hipMemsetAsync(d_ptr, 0, bytes, stream()));
+    });
+}
+
 mk_status mk_synchronize(void) {
     return guarded([&] { drain(); });
 }
 
-mk_status mk_profile_enable(int32_t on) {
+mk_status mk_profile_enable(int32_t mask) {
     return guarded([&] {
         drain();
-        prof().on = on != 0;
+        prof().mask = static_cast<std::uint32_t>(mask) & ((1u << K_COUNT) - 1u);
     });
 }
 

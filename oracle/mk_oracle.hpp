@@ -255,9 +255,31 @@ inline idx enumerate(idx voxels, const std::vector<int> &totals, std::vector<int
     return size;
 }
 
-// population_space.hpp:75-100
+// C(rem + rest, rest) for rest < voxels and rem <= max_total, filled with binomial() once per
+// space: the same integers the reference's rank recomputes per call
+struct binomial_table {
+    idx stride = 0;
+    std::vector<idx> values;
+    binomial_table() = default;
+    binomial_table(idx voxels, int max_total) : stride(static_cast<idx>(max_total) + 1) {
+        values.resize(voxels * stride);
+        for (idx rest = 0; rest < voxels; ++rest)
+            for (int rem = 0; rem <= max_total; ++rem) {
+                idx v;
+                try {
+                    v = binomial(static_cast<idx>(rem) + rest, rest);
+                } catch (const std::overflow_error &) {
+                    v = std::numeric_limits<idx>::max();
+                }
+                values[rest * stride + rem] = v;
+            }
+    }
+    [[nodiscard]] idx operator()(idx rest, int rem) const { return values[rest * stride + rem]; }
+};
+
+// population_space.hpp:75-100 (binomials from the table when one is given)
 inline std::optional<idx> rank(idx voxels, const std::vector<int> &totals, const int *x,
-                               idx size) {
+                               idx size, const binomial_table *table = nullptr) {
     const idx S = totals.size();
     if (size != voxels * S) return std::nullopt;
     idx result = 0;
@@ -268,11 +290,15 @@ inline std::optional<idx> rank(idx voxels, const std::vector<int> &totals, const
             const int count = x[v * S + s];
             if (count < 0 || count > remaining) return std::nullopt;
             const idx rest = voxels - v - 1;
-            local += binomial(remaining + rest, rest) - binomial(remaining - count + rest, rest);
+            if (table) {
+                local += (*table)(rest, remaining) - (*table)(rest, remaining - count);
+            } else {
+                local += binomial(remaining + rest, rest) - binomial(remaining - count + rest, rest);
+            }
             remaining -= count;
         }
         if (x[(voxels - 1) * S + s] != remaining) return std::nullopt;
-        result = result * compositions(voxels, totals[s]) + local;
+        result = result * (table ? (*table)(voxels - 1, totals[s]) : compositions(voxels, totals[s])) + local;
     }
     return result;
 }
@@ -285,12 +311,13 @@ class population_sector {
     population_sector(idx n_voxels, std::vector<int> totals)
         : voxels_(n_voxels), totals_(std::move(totals)) {
         size_ = detail::enumerate(voxels_, totals_, states_);
+        table_ = detail::binomial_table(voxels_, *std::max_element(totals_.begin(), totals_.end()));
     }
     [[nodiscard]] idx size() const { return size_; }
     [[nodiscard]] const int *operator[](idx i) const { return states_.data() + i * width(); }
     [[nodiscard]] state at(idx i) const { return state((*this)[i], (*this)[i] + width()); }
     [[nodiscard]] std::optional<idx> find(const int *x, idx n) const {
-        return detail::rank(voxels_, totals_, x, n);
+        return detail::rank(voxels_, totals_, x, n, &table_);
     }
     [[nodiscard]] std::optional<idx> find(const state &x) const { return find(x.data(), x.size()); }
     [[nodiscard]] idx n_voxels() const { return voxels_; }
@@ -303,6 +330,7 @@ class population_sector {
     std::vector<int> totals_;
     std::vector<int> states_;
     idx size_ = 0;
+    detail::binomial_table table_;
 };
 
 // population_space.hpp:131-196
@@ -313,16 +341,20 @@ class population_box {
         begin_.push_back(0);
         std::vector<int> totals(max_totals_.size(), 0);
         add_sectors(0, totals);
+        table_ = detail::binomial_table(voxels_, *std::max_element(max_totals_.begin(), max_totals_.end()));
     }
     [[nodiscard]] idx size() const { return begin_.back(); }
     [[nodiscard]] const int *operator[](idx i) const { return states_.data() + i * width(); }
     [[nodiscard]] state at(idx i) const { return state((*this)[i], (*this)[i] + width()); }
     [[nodiscard]] std::optional<idx> find(const int *x, idx n) const {
         if (n != width()) return std::nullopt;
-        std::vector<int> totals(n_species(), 0);
-        for (idx i = 0; i < n; ++i) totals[i % n_species()] += x[i];
+        const idx S = n_species();
+        thread_local std::vector<int> totals;
+        totals.assign(S, 0);
+        for (idx v = 0; v < voxels_; ++v)
+            for (idx s = 0; s < S; ++s) totals[s] += x[v * S + s];
         const auto k = find_sector(totals);
-        const auto local = k ? detail::rank(voxels_, totals, x, n) : std::nullopt;
+        const auto local = k ? detail::rank(voxels_, totals, x, n, &table_) : std::nullopt;
         return local ? std::optional<idx>(begin_[*k] + *local) : std::nullopt;
     }
     [[nodiscard]] std::optional<idx> find(const state &x) const { return find(x.data(), x.size()); }
@@ -364,6 +396,7 @@ class population_box {
     std::vector<std::vector<int>> totals_;
     std::vector<idx> begin_;
     std::vector<int> states_;
+    detail::binomial_table table_;
 };
 
 // ---------------------------------------------------------------- the RDME model

@@ -65,14 +65,23 @@ voxel_domain make_domain(int dim, const int64_t *dims, const double *spacing, in
 }
 
 struct rate_table {
-    std::vector<real> values; // [voxel][tuple]
-    std::vector<real> volumes;
+    std::vector<real> values;   // [voxel][tuple]
+    std::vector<real> distinct; // distinct volumes, in first-seen order
+    std::vector<idx> first;     // the first voxel with each distinct volume
     std::vector<idx> stride;
     idx T = 1;
+    void set_volumes(const std::vector<real> &volumes) {
+        for (idx v = 0; v < volumes.size(); ++v)
+            if (std::find(distinct.begin(), distinct.end(), volumes[v]) == distinct.end()) {
+                distinct.push_back(volumes[v]);
+                first.push_back(v);
+            }
+    }
     real operator()(const int *counts, real volume) const {
-        idx v = 0;
-        while (v < volumes.size() && volumes[v] != volume) ++v;
-        if (v == volumes.size()) throw std::invalid_argument("oracle: rate asked at an unknown volume");
+        idx d = 0;
+        while (d < distinct.size() && distinct[d] != volume) ++d;
+        if (d == distinct.size()) throw std::invalid_argument("oracle: rate asked at an unknown volume");
+        const idx v = first[d];
         idx t = 0;
         for (idx s = 0; s < stride.size(); ++s) t += static_cast<idx>(counts[s]) * stride[s];
         return values[v * T + t];
@@ -111,7 +120,7 @@ int orc_create(int dim, const int64_t *dims, const double *spacing, int64_t n_vo
             rate_table table;
             table.values.assign(rates + static_cast<idx>(r) * V * T,
                                 rates + static_cast<idx>(r + 1) * V * T);
-            table.volumes = p->domain->volumes;
+            table.set_volumes(p->domain->volumes);
             table.stride = stride;
             table.T = T;
             std::vector<int> change(changes + r * n_species, changes + (r + 1) * n_species);

@@ -211,51 +211,155 @@ template <typename VT>
 struct dev_csr {
     i64 n = 0;
     u64 nnz = 0;
-    dbuf ptr, col, val, blocks;
+    bool coded = false;
+    dbuf blocks, ptr, col, val, exit, bptr, ent, rlen, rcode, vdict, edict;
     gpu::csr_view<VT> view;
 };
 
+// the distinct values of v converted to VT, sorted; empty when there are more than kDictMax
+template <typename VT>
+std::vector<VT> dictionary(const std::vector<double> &v) {
+    const i64 n = static_cast<i64>(v.size());
+    const i64 chunk = 1 << 20;
+    const i64 tasks = (n + chunk - 1) / chunk;
+    std::vector<std::vector<VT>> parts(tasks);
+    std::vector<char> over(tasks, 0);
+    mk::parallel_tasks(tasks, [&](i64 t) {
+        auto &d = parts[t];
+        for (i64 i = t * chunk; i < std::min(n, (t + 1) * chunk); ++i) {
+            const VT x = static_cast<VT>(v[i]);
+            auto it = std::lower_bound(d.begin(), d.end(), x);
+            if (it == d.end() || *it != x) {
+                if (d.size() >= static_cast<std::size_t>(gpu::kDictMax)) {
+                    over[t] = 1;
+                    return;
+                }
+                d.insert(it, x);
+            }
+        }
+    });
+    std::vector<VT> all;
+    for (i64 t = 0; t < tasks; ++t) {
+        if (over[t]) return {};
+        for (VT x : parts[t]) {
+            auto it = std::lower_bound(all.begin(), all.end(), x);
+            if (it == all.end() || *it != x) all.insert(it, x);
+        }
+        if (all.size() > static_cast<std::size_t>(gpu::kDictMax)) return {};
+    }
+    if (all.empty()) all.push_back(VT(0));
+    return all;
+}
+
+template <typename VT>
+u32 code_of(const std::vector<VT> &dict, double v) {
+    return static_cast<u32>(std::lower_bound(dict.begin(), dict.end(), static_cast<VT>(v)) - dict.begin());
+}
+
+// every array the selected kernel reads must exist before anything is launched on it
+template <typename VT>
+void check_view(const gpu::csr_view<VT> &v, u64 nnz) {
+    bool ok = v.n == 0 || (v.blocks && v.n_blocks > 0);
+    if (v.coded) {
+        ok = ok && (v.n == 0 || (v.bptr && v.rlen && v.rcode && v.edict && v.vdict)) &&
+             (nnz == 0 || v.ent) && v.n_vdict > 0 && v.n_vdict <= gpu::kDictMax &&
+             v.n_edict > 0 && v.n_edict <= gpu::kDictMax;
+    } else {
+        ok = ok && (v.n == 0 || (v.ptr && v.exit)) && (nnz == 0 || (v.col && v.val));
+    }
+    if (!ok) throw std::logic_error("operator upload left a required device array missing");
+}
+
+// upload an operator (gather CSR + exit rates) in the coded layout when it fits, else plain
 template <typename VT>
 dev_csr<VT> upload_csr(const mk::generator_csr &g) {
     dev_csr<VT> d;
     d.n = g.n;
     d.nnz = g.ptr.empty() ? 0 : g.ptr.back();
     if (d.nnz >= (u64{1} << 32)) throw std::overflow_error("operator has more than 2^32 entries");
-    std::vector<u32> ptr(g.n + 1);
-    for (i64 i = 0; i <= g.n; ++i) ptr[i] = static_cast<u32>(g.ptr[i]);
     // row blocks: up to kRowsPerBlock rows whose entries fit the LDS tile; a row longer than
     // the tile gets a block of its own
-    std::vector<u32> blocks{0};
+    std::vector<u32> blocks{0}, bptr{0};
     i64 r = 0;
+    u64 max_len = 0;
     while (r < g.n) {
         const i64 r0 = r;
         const u64 p0 = g.ptr[r0];
+        max_len = std::max<u64>(max_len, g.ptr[r0 + 1] - p0);
         ++r;
-        while (r < g.n && r - r0 < gpu::kRowsPerBlock && g.ptr[r + 1] - p0 <= static_cast<u64>(gpu::kTile)) ++r;
+        while (r < g.n && r - r0 < gpu::kRowsPerBlock &&
+               g.ptr[r + 1] - p0 <= static_cast<u64>(gpu::kTile)) {
+            max_len = std::max<u64>(max_len, g.ptr[r + 1] - g.ptr[r]);
+            ++r;
+        }
         blocks.push_back(static_cast<u32>(r));
-    }
-    d.ptr = upload(ptr);
-    d.col = upload(g.col);
-    if constexpr (std::is_same_v<VT, double>) {
-        d.val = upload(g.val);
-    } else {
-        std::vector<VT> v(g.val.begin(), g.val.end());
-        d.val = upload(v);
+        bptr.push_back(static_cast<u32>(g.ptr[r]));
     }
     d.blocks = upload(blocks);
     d.view.n = g.n;
+    d.view.n_blocks = static_cast<i64>(blocks.size()) - 1;
+    d.view.blocks = dp<u32>(d.blocks);
+
+    std::vector<VT> vdict, edict;
+    if (g.n <= static_cast<i64>(gpu::kColMask) + 1 && max_len <= 255) {
+        vdict = dictionary<VT>(g.val);
+        if (!vdict.empty()) edict = dictionary<VT>(g.exit);
+    }
+    if (!vdict.empty() && !edict.empty()) {
+        d.coded = true;
+        std::vector<u32> ent(d.nnz);
+        std::vector<std::uint8_t> rlen(g.n), rcode(g.n);
+        const i64 chunk = 1 << 16;
+        mk::parallel_tasks((g.n + chunk - 1) / chunk, [&](i64 t) {
+            for (i64 i = t * chunk; i < std::min(g.n, (t + 1) * chunk); ++i) {
+                rlen[i] = static_cast<std::uint8_t>(g.ptr[i + 1] - g.ptr[i]);
+                rcode[i] = static_cast<std::uint8_t>(code_of(edict, g.exit[i]));
+                for (u64 p = g.ptr[i]; p < g.ptr[i + 1]; ++p)
+                    ent[p] = g.col[p] | (code_of(vdict, g.val[p]) << gpu::kColBits);
+            }
+        });
+        d.bptr = upload(bptr);
+        d.ent = upload(ent);
+        d.rlen = upload(rlen);
+        d.rcode = upload(rcode);
+        d.vdict = upload(vdict);
+        d.edict = upload(edict);
+        d.view.bptr = dp<u32>(d.bptr);
+        d.view.ent = dp<u32>(d.ent);
+        d.view.rlen = dp<std::uint8_t>(d.rlen);
+        d.view.rcode = dp<std::uint8_t>(d.rcode);
+        d.view.vdict = dp<VT>(d.vdict);
+        d.view.edict = dp<VT>(d.edict);
+        d.view.n_vdict = static_cast<int>(vdict.size());
+        d.view.n_edict = static_cast<int>(edict.size());
+        d.view.coded = 1;
+        check_view(d.view, d.nnz);
+        return d;
+    }
+    std::vector<u32> ptr(g.n + 1);
+    for (i64 i = 0; i <= g.n; ++i) ptr[i] = static_cast<u32>(g.ptr[i]);
+    d.ptr = upload(ptr);
+    d.col = upload(g.col);
+    std::vector<VT> v(g.val.begin(), g.val.end()), e(g.exit.begin(), g.exit.end());
+    d.val = upload(v);
+    d.exit = upload(e);
     d.view.ptr = dp<u32>(d.ptr);
     d.view.col = dp<u32>(d.col);
     d.view.val = dp<VT>(d.val);
-    d.view.blocks = dp<u32>(d.blocks);
-    d.view.n_blocks = static_cast<i64>(blocks.size()) - 1;
+    d.view.exit = dp<VT>(d.exit);
+    check_view(d.view, d.nnz);
     return d;
 }
 
-double csr_bytes(i64 n, u64 nnz, std::size_t vs, std::size_t xs, std::size_t ts, int vectors) {
-    // entries (value + column) + row pointers + per-row vectors
-    return double(nnz) * double(vs + 4) + double(n + 1) * 4.0 + double(n) * double(xs) +
-           double(n) * double(ts) * vectors;
+// algorithmic bytes of one pass over an operator in its implemented layout: the encoded
+// entries and per-row data once, x once (xs bytes per row), and `vectors` other per-row
+// vectors of ts bytes (rhs, output, ...)
+template <typename VT>
+double op_bytes(const dev_csr<VT> &A, std::size_t xs, std::size_t ts, int vectors) {
+    const double n = double(A.n), nnz = double(A.nnz);
+    double b = A.coded ? nnz * 4.0 + n * 2.0 + double(A.view.n_blocks + 1) * 8.0
+                       : nnz * double(sizeof(VT) + 4) + (n + 1) * 4.0 + n * sizeof(VT);
+    return b + n * double(xs) + n * double(ts) * vectors;
 }
 
 } // namespace
@@ -274,7 +378,6 @@ struct mk_generator_s {
     i64 n_jumps = 0;
     std::vector<double> exit, leak;
     dev_csr<double> A;
-    dbuf d_exit;
     dbuf d_rpart;  // one partial residual norm per row block
     dbuf d_x, d_y; // staging for host-buffer calls
 };
@@ -285,7 +388,6 @@ template <typename T>
 struct level_dev {
     i64 n = 0;
     dev_csr<T> A;
-    dbuf exit;
     // transfer to the next level
     dbuf coarse_of, weight;
     i64 n_fine_linked = 0; // rows that have a coarse parent
@@ -377,8 +479,6 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
             }
         }
         lv.A = upload_csr<T>(cat);
-        std::vector<T> exit_t(cat.exit.begin(), cat.exit.end());
-        lv.exit = upload(exit_t);
         lv.n_term = static_cast<i64>(tstart.size());
         if (lv.n_term > 0) {
             lv.tstart = upload(tstart);
@@ -429,14 +529,14 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
     const double shift = mg.shift, omega = mg.opt.jacobi_weight;
     const i64 n = lv.n;
     const std::size_t ts = sizeof(T);
-    const double sweep_bytes = csr_bytes(n, lv.A.nnz, ts, ts, ts, 3);
+    const double sweep_bytes = op_bytes(lv.A, ts, ts, 2);
     T *cur = dp<T>(lv.x);
     T *other = dp<T>(lv.xt);
     if (rhs64) {
         T *rb = dp<T>(lv.rhs);
         if (lv.pre > 0) {
             P.run(K_FIRST, double(n) * (8 + 3 * ts), [&] {
-                gpu::jacobi_first<T, double>(n, shift, omega, dp<T>(lv.exit), rhs64, rb, cur, s);
+                gpu::jacobi_first<T, double>(lv.A.view, shift, omega, rhs64, rb, cur, s);
             });
         } else {
             P.run(K_OTHER, double(n) * (8 + ts), [&] { gpu::convert<double, T>(n, rhs64, rb, s); });
@@ -444,19 +544,19 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
         rhs = rb;
     } else if (lv.pre > 0) {
         P.run(K_FIRST, double(n) * 3 * ts, [&] {
-            gpu::jacobi_first<T, T>(n, shift, omega, dp<T>(lv.exit), rhs, nullptr, cur, s);
+            gpu::jacobi_first<T, T>(lv.A.view, shift, omega, rhs, nullptr, cur, s);
         });
     }
     if (lv.pre == 0) P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
     for (int k = 1; k < lv.pre; ++k) {
         P.run(K_SWEEP, sweep_bytes, [&] {
-            gpu::jacobi_sweep<T>(lv.A.view, shift, omega, dp<T>(lv.exit), rhs, cur, other, s);
+            gpu::jacobi_sweep<T>(lv.A.view, shift, omega, rhs, cur, other, s);
         });
         std::swap(cur, other);
     }
     T *res = dp<T>(lv.res);
     P.run(K_RESIDUAL, sweep_bytes, [&] {
-        gpu::level_residual<T>(lv.A.view, shift, dp<T>(lv.exit), rhs, cur, res, s);
+        gpu::level_residual<T>(lv.A.view, shift, rhs, cur, res, s);
     });
     if (lv.n_term > 0) {
         P.run(K_TERMINAL, 0.0, [&] {
@@ -483,11 +583,11 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
                                           dp<T>(c.crhs), s);
             });
             P.run(K_OTHER, double(c.n) * ts, [&] { gpu::fill<T>(c.n, T(0), dp<T>(c.cx), s); });
-            const double cbytes = csr_bytes(c.n, c.A.nnz, ts, ts, ts, 3);
+            const double cbytes = op_bytes(c.A, ts, ts, 2);
             for (int cyc = 0; cyc < mg.opt.coarse_cycles; ++cyc) {
                 P.run(K_RESIDUAL, cbytes, [&] {
-                    gpu::level_residual<T>(c.A.view, shift, dp<T>(c.exit), dp<T>(c.crhs),
-                                           dp<T>(c.cx), dp<T>(c.rhs), s);
+                    gpu::level_residual<T>(c.A.view, shift, dp<T>(c.crhs), dp<T>(c.cx),
+                                           dp<T>(c.rhs), s);
                 });
                 T *ce = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr);
                 P.run(K_OTHER, double(c.n) * 3 * ts, [&] { gpu::axpy<T>(c.n, T(1), ce, dp<T>(c.cx), s); });
@@ -501,7 +601,7 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
     for (int k = 0; k < lv.post; ++k) {
         T *dst = (k == lv.post - 1 && out) ? out : other;
         P.run(K_SWEEP, sweep_bytes, [&] {
-            gpu::jacobi_sweep<T>(lv.A.view, shift, omega, dp<T>(lv.exit), rhs, cur, dst, s);
+            gpu::jacobi_sweep<T>(lv.A.view, shift, omega, rhs, cur, dst, s);
         });
         if (dst == other) {
             std::swap(cur, other);
@@ -585,15 +685,14 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
     double *V = dp<double>(ws.V), *w = dp<double>(ws.w), *r = dp<double>(ws.r);
     double *partial = dp<double>(ws.partial), *hd = dp<double>(ws.hd);
     ZT *Z = M ? dp<ZT>(ws.Z) : nullptr;
-    const double spmv_bytes = csr_bytes(n, A.A.nnz, 8, 8, 8, 2);
+    const double spmv_bytes = op_bytes(A.A, 8, 8, 1);
 
     mk_solver_result res{0, 0.0, 0};
     // r = b - A x and ||r||^2 (left in hd[0] for the first basis vector)
     const int rb = static_cast<int>(std::max<i64>(A.A.view.n_blocks, 1));
     auto true_residual = [&]() {
         P.run(K_SPMV, spmv_bytes + n * 8.0, [&] {
-            gpu::shifted_apply<double>(A.A.view, shift, dp<double>(A.d_exit), x, b, r,
-                                       dp<double>(A.d_rpart), s);
+            gpu::shifted_apply<double>(A.A.view, shift, x, b, r, dp<double>(A.d_rpart), s);
         });
         return std::sqrt(device_norm2(dp<double>(A.d_rpart), rb, hd));
     };
@@ -617,14 +716,12 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
             if (M) {
                 ZT *Zj = Z + j * n;
                 mg_apply_device(*M, Vj, Zj);
-                P.run(K_SPMV, csr_bytes(n, A.A.nnz, 8, sizeof(ZT), 8, 1) + n * 8.0, [&] {
-                    gpu::shifted_apply<ZT>(A.A.view, shift, dp<double>(A.d_exit), Zj, nullptr, w,
-                                           nullptr, s);
+                P.run(K_SPMV, op_bytes(A.A, sizeof(ZT), 8, 1), [&] {
+                    gpu::shifted_apply<ZT>(A.A.view, shift, Zj, nullptr, w, nullptr, s);
                 });
             } else {
                 P.run(K_SPMV, spmv_bytes, [&] {
-                    gpu::shifted_apply<double>(A.A.view, shift, dp<double>(A.d_exit), Vj,
-                                               nullptr, w, nullptr, s);
+                    gpu::shifted_apply<double>(A.A.view, shift, Vj, nullptr, w, nullptr, s);
                 });
             }
             // classical Gram-Schmidt, twice (CGS2)
@@ -806,7 +903,6 @@ void finish_generator(mk_generator_s &G, mk::generator_csr &&g) {
     G.n = g.n;
     G.n_jumps = g.n_jumps;
     G.A = upload_csr<double>(g);
-    G.d_exit = upload(g.exit);
     G.d_rpart = dbuf(std::max<i64>(G.A.view.n_blocks, 1) * sizeof(double));
     G.exit = std::move(g.exit);
     G.leak = std::move(g.leak);
@@ -880,9 +976,8 @@ void stage(mk_generator_s &g) {
 
 mk_status mk_shifted_apply_device(mk_generator g, double shift, const double *d_x, double *d_y) {
     return guarded([&] {
-        prof().run(K_SPMV, csr_bytes(g->n, g->A.nnz, 8, 8, 8, 2), [&] {
-            gpu::shifted_apply<double>(g->A.view, shift, dp<double>(g->d_exit), d_x, nullptr,
-                                       d_y, nullptr, stream());
+        prof().run(K_SPMV, op_bytes(g->A, 8, 8, 1), [&] {
+            gpu::shifted_apply<double>(g->A.view, shift, d_x, nullptr, d_y, nullptr, stream());
         });
     });
 }
@@ -891,8 +986,8 @@ mk_status mk_shifted_apply(mk_generator g, double shift, const double *x, double
     return guarded([&] {
         stage(*g);
         HIPCHK(hipMemcpy(g->d_x.p, x, g->n * sizeof(double), hipMemcpyHostToDevice));
-        prof().run(K_SPMV, csr_bytes(g->n, g->A.nnz, 8, 8, 8, 2), [&] {
-            gpu::shifted_apply<double>(g->A.view, shift, dp<double>(g->d_exit),
+        prof().run(K_SPMV, op_bytes(g->A, 8, 8, 1), [&] {
+            gpu::shifted_apply<double>(g->A.view, shift,
                                        dp<double>(g->d_x), nullptr, dp<double>(g->d_y), nullptr,
                                        stream());
         });

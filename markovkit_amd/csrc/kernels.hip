@@ -1,6 +1,6 @@
 // Device kernels of the resolvent solve for gfx950 (CDNA4, wave64). See kernels.hpp for the
-// data layout. All reductions run in a fixed order (per-wave shuffles, then waves in index
-// order, then workgroup partials in index order), so results do not depend on timing.
+// operator layouts. All reductions run in a fixed order (per-wave shuffles, then waves in
+// index order, then workgroup partials in index order), so results do not depend on timing.
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -10,6 +10,7 @@ namespace mk::gpu {
 namespace {
 
 constexpr int kWaves = kThreads / 64;
+constexpr int kUnroll = kTile / kThreads; // entries per thread per row block
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -34,6 +35,13 @@ __device__ __forceinline__ T block_sum(T v, T *lds) {
     return r;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin; give XCD j a contiguous range of
+// row blocks so that neighbouring rows share its L2.
+__device__ __forceinline__ u32 xcd_block(u32 bid, u32 nb) {
+    const u32 xcd = bid & 7u, slot = bid >> 3, q = nb >> 3, r = nb & 7u;
+    return xcd * q + min(xcd, r) + slot;
+}
+
 enum class csr_mode { shifted, shifted_residual, sweep, residual };
 
 template <typename VT, typename XT, typename T>
@@ -41,80 +49,173 @@ struct csr_args {
     csr_view<VT> A;
     double shift;
     T omega;
-    const T *exit;   // per row, in the output's precision
     const XT *x;
     const T *rhs;    // b (shifted_residual) or rhs (sweep, residual)
     T *out;
-    double *partial; // shifted_residual: sum of out^2 per workgroup
+    double *partial; // shifted_residual: sum of out^2 per row block
 };
 
-// one workgroup per row block: stream columns/values, stage products in LDS, sum rows
 template <csr_mode M, typename VT, typename XT, typename T>
-__global__ void __launch_bounds__(kThreads) k_csr(csr_args<VT, XT, T> a) {
-    using AT = T; // accumulate in the output's precision
-    __shared__ AT prod[kTile];
-    __shared__ AT red[kWaves];
-    __shared__ double red64[kWaves];
-    const u32 r0 = a.A.blocks[blockIdx.x], r1 = a.A.blocks[blockIdx.x + 1];
-    const u32 p0 = a.A.ptr[r0], p1 = a.A.ptr[r1];
-    const int tid = threadIdx.x;
-    const AT shift = static_cast<AT>(a.shift);
-
-    auto finish = [&](u32 r, AT sum) -> double {
-        const AT xr = static_cast<AT>(a.x[r]);
-        const AT d = shift + a.exit[r];
-        const AT ax = d * xr - sum;
-        if constexpr (M == csr_mode::shifted) {
-            a.out[r] = ax;
-            return 0.0;
-        } else if constexpr (M == csr_mode::shifted_residual) {
-            const AT res = a.rhs[r] - ax;
-            a.out[r] = res;
-            return static_cast<double>(res) * static_cast<double>(res);
-        } else if constexpr (M == csr_mode::sweep) {
-            const AT res = a.rhs[r] - ax;
-            a.out[r] = xr + a.omega * res / d;
-            return 0.0;
-        } else {
-            a.out[r] = a.rhs[r] - ax;
-            return 0.0;
-        }
-    };
-
-    double sq = 0.0;
-    if (p1 - p0 <= static_cast<u32>(kTile)) {
-        for (u32 p = p0 + tid; p < p1; p += kThreads)
-            prod[p - p0] = static_cast<AT>(a.A.val[p]) * static_cast<AT>(a.x[a.A.col[p]]);
-        __syncthreads();
-        const u32 r = r0 + tid;
-        if (r < r1) {
-            AT sum = 0;
-            const u32 q1 = a.A.ptr[r + 1];
-            for (u32 q = a.A.ptr[r]; q < q1; ++q) sum += prod[q - p0];
-            sq = finish(r, sum);
-        }
+__device__ __forceinline__ double finish(const csr_args<VT, XT, T> &a, u32 r, T sum, T exit_r) {
+    const T xr = static_cast<T>(a.x[r]);
+    const T d = static_cast<T>(a.shift) + exit_r;
+    const T ax = d * xr - sum;
+    if constexpr (M == csr_mode::shifted) {
+        a.out[r] = ax;
+        return 0.0;
+    } else if constexpr (M == csr_mode::shifted_residual) {
+        const T res = a.rhs[r] - ax;
+        a.out[r] = res;
+        return static_cast<double>(res) * static_cast<double>(res);
+    } else if constexpr (M == csr_mode::sweep) {
+        const T res = a.rhs[r] - ax;
+        a.out[r] = xr + a.omega * res / d;
+        return 0.0;
     } else {
-        // a block holding one row longer than the tile
-        AT sum = 0;
-        for (u32 p = p0 + tid; p < p1; p += kThreads)
-            sum += static_cast<AT>(a.A.val[p]) * static_cast<AT>(a.x[a.A.col[p]]);
-        sum = block_sum(sum, red);
-        if (tid == 0) sq = finish(r0, sum);
+        a.out[r] = a.rhs[r] - ax;
+        return 0.0;
+    }
+}
+
+// coded operator: one workgroup per row block
+template <csr_mode M, typename VT, typename XT, typename T>
+__global__ void __launch_bounds__(kThreads) k_csr_coded(csr_args<VT, XT, T> a) {
+    __shared__ T prod[kTile];
+    __shared__ T vdict[kDictMax];
+    __shared__ T edict[kDictMax];
+    __shared__ u32 wsum[kWaves];
+    __shared__ double red64[kWaves];
+    const u32 b = xcd_block(blockIdx.x, gridDim.x);
+    const u32 r0 = a.A.blocks[b], r1 = a.A.blocks[b + 1];
+    const u32 p0 = a.A.bptr[b], p1 = a.A.bptr[b + 1];
+    const int tid = threadIdx.x;
+
+    // stream the block's entries (all loads issued before any is used)
+    u32 e[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const u32 p = p0 + tid + u * kThreads;
+        e[u] = p < p1 ? __builtin_nontemporal_load(a.A.ent + p) : 0u;
+    }
+    const u32 r = r0 + tid;
+    const bool has_row = r < r1;
+    const u32 len = has_row ? a.A.rlen[r] : 0u;
+    const u32 ec = has_row ? a.A.rcode[r] : 0u;
+    for (int i = tid; i < a.A.n_vdict; i += kThreads) vdict[i] = static_cast<T>(a.A.vdict[i]);
+    for (int i = tid; i < a.A.n_edict; i += kThreads) edict[i] = static_cast<T>(a.A.edict[i]);
+    // gather x
+    XT xv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const u32 p = p0 + tid + u * kThreads;
+        xv[u] = p < p1 ? a.x[e[u] & kColMask] : XT(0);
+    }
+    // each row's first product: exclusive scan of the row lengths
+    const int lane = tid & 63, wave = tid >> 6;
+    u32 incl = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const u32 up = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += up;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    u32 start = incl - len;
+    for (int w = 0; w < wave; ++w) start += wsum[w];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const u32 q = tid + u * kThreads;
+        if (p0 + q < p1) prod[q] = vdict[e[u] >> kColBits] * static_cast<T>(xv[u]);
+    }
+    __syncthreads();
+    double sq = 0.0;
+    if (has_row) {
+        T sum = 0;
+        for (u32 q = start; q < start + len; ++q) sum += prod[q];
+        sq = finish<M>(a, r, sum, edict[ec]);
     }
     if constexpr (M == csr_mode::shifted_residual) {
         const double t = block_sum(sq, red64);
-        if (tid == 0 && a.partial) a.partial[blockIdx.x] = t;
+        if (tid == 0 && a.partial) a.partial[b] = t;
     }
+}
+
+// plain operator: one workgroup per row block; a block holding one row longer than the tile
+// reduces that row across the workgroup
+template <csr_mode M, typename VT, typename XT, typename T>
+__global__ void __launch_bounds__(kThreads) k_csr_plain(csr_args<VT, XT, T> a) {
+    __shared__ T prod[kTile];
+    __shared__ T red[kWaves];
+    __shared__ double red64[kWaves];
+    const u32 b = xcd_block(blockIdx.x, gridDim.x);
+    const u32 r0 = a.A.blocks[b], r1 = a.A.blocks[b + 1];
+    const u32 p0 = a.A.ptr[r0], p1 = a.A.ptr[r1];
+    const int tid = threadIdx.x;
+    double sq = 0.0;
+    if (p1 - p0 <= static_cast<u32>(kTile)) {
+        u32 c[kUnroll];
+        VT v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const u32 p = p0 + tid + u * kThreads;
+            c[u] = p < p1 ? __builtin_nontemporal_load(a.A.col + p) : 0u;
+            v[u] = p < p1 ? __builtin_nontemporal_load(a.A.val + p) : VT(0);
+        }
+        XT xv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const u32 p = p0 + tid + u * kThreads;
+            xv[u] = p < p1 ? a.x[c[u]] : XT(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const u32 q = tid + u * kThreads;
+            if (p0 + q < p1) prod[q] = static_cast<T>(v[u]) * static_cast<T>(xv[u]);
+        }
+        __syncthreads();
+        const u32 r = r0 + tid;
+        if (r < r1) {
+            T sum = 0;
+            const u32 q1 = a.A.ptr[r + 1];
+            for (u32 q = a.A.ptr[r]; q < q1; ++q) sum += prod[q - p0];
+            sq = finish<M>(a, r, sum, static_cast<T>(a.A.exit[r]));
+        }
+    } else {
+        T sum = 0;
+        for (u32 p = p0 + tid; p < p1; p += kThreads)
+            sum += static_cast<T>(a.A.val[p]) * static_cast<T>(a.x[a.A.col[p]]);
+        sum = block_sum(sum, red);
+        if (tid == 0) sq = finish<M>(a, r0, sum, static_cast<T>(a.A.exit[r0]));
+    }
+    if constexpr (M == csr_mode::shifted_residual) {
+        const double t = block_sum(sq, red64);
+        if (tid == 0 && a.partial) a.partial[b] = t;
+    }
+}
+
+template <csr_mode M, typename VT, typename XT, typename T>
+void launch_csr(const csr_args<VT, XT, T> &a, hipStream_t s) {
+    if (a.A.n_blocks == 0) return;
+    if (a.A.coded) {
+        hipLaunchKernelGGL((k_csr_coded<M, VT, XT, T>), dim3(a.A.n_blocks), dim3(kThreads), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_csr_plain<M, VT, XT, T>), dim3(a.A.n_blocks), dim3(kThreads), 0, s, a);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T row_exit(const csr_view<T> &A, i64 r) {
+    return A.coded ? A.edict[A.rcode[r]] : A.exit[r];
 }
 
 template <typename T, typename IN>
 __global__ void __launch_bounds__(kThreads)
-    k_jacobi_first(i64 n, T shift, T omega, const T *exit, const IN *rhs_in, T *rhs_out, T *x) {
-    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+    k_jacobi_first(csr_view<T> A, T shift, T omega, const IN *rhs_in, T *rhs_out, T *x) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < A.n;
          i += static_cast<i64>(gridDim.x) * kThreads) {
         const T b = static_cast<T>(rhs_in[i]);
         if (rhs_out) rhs_out[i] = b;
-        x[i] = omega * b / (shift + exit[i]);
+        x[i] = omega * b / (shift + row_exit(A, i));
     }
 }
 
@@ -175,14 +276,6 @@ __global__ void __launch_bounds__(kThreads) k_axpy(i64 n, T alpha, const T *x, T
     for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
          i += static_cast<i64>(gridDim.x) * kThreads)
         y[i] += alpha * x[i];
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kThreads)
-    k_residual_update(i64 n, const T *rhs, const T *ax, T *r) {
-    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
-         i += static_cast<i64>(gridDim.x) * kThreads)
-        r[i] = rhs[i] - ax[i];
 }
 
 // ---- Krylov kernels ----
@@ -279,43 +372,35 @@ int reduction_blocks(i64 n) {
 }
 
 template <typename XT>
-void shifted_apply(const csr_view<double> &A, double shift, const double *exit, const XT *x,
-                   const double *b, double *y, double *partial, hipStream_t s) {
-    if (A.n_blocks == 0) return;
-    csr_args<double, XT, double> a{A, shift, 0.0, exit, x, b, y, partial};
+void shifted_apply(const csr_view<double> &A, double shift, const XT *x, const double *b,
+                   double *y, double *partial, hipStream_t s) {
+    const csr_args<double, XT, double> a{A, shift, 0.0, x, b, y, partial};
     if (b) {
-        hipLaunchKernelGGL((k_csr<csr_mode::shifted_residual, double, XT, double>),
-                           dim3(A.n_blocks), dim3(kThreads), 0, s, a);
+        launch_csr<csr_mode::shifted_residual>(a, s);
     } else {
-        hipLaunchKernelGGL((k_csr<csr_mode::shifted, double, XT, double>), dim3(A.n_blocks),
-                           dim3(kThreads), 0, s, a);
+        launch_csr<csr_mode::shifted>(a, s);
     }
 }
 
 template <typename T, typename IN>
-void jacobi_first(i64 n, double shift, double omega, const T *exit, const IN *rhs_in, T *rhs_out,
+void jacobi_first(const csr_view<T> &A, double shift, double omega, const IN *rhs_in, T *rhs_out,
                   T *x, hipStream_t s) {
-    if (n == 0) return;
-    hipLaunchKernelGGL((k_jacobi_first<T, IN>), dim3(grid_for(n)), dim3(kThreads), 0, s, n,
-                       static_cast<T>(shift), static_cast<T>(omega), exit, rhs_in, rhs_out, x);
+    if (A.n == 0) return;
+    hipLaunchKernelGGL((k_jacobi_first<T, IN>), dim3(grid_for(A.n)), dim3(kThreads), 0, s, A,
+                       static_cast<T>(shift), static_cast<T>(omega), rhs_in, rhs_out, x);
 }
 
 template <typename T>
-void jacobi_sweep(const csr_view<T> &A, double shift, double omega, const T *exit, const T *rhs,
-                  const T *x, T *x_out, hipStream_t s) {
-    if (A.n_blocks == 0) return;
-    csr_args<T, T, T> a{A, shift, static_cast<T>(omega), exit, x, rhs, x_out, nullptr};
-    hipLaunchKernelGGL((k_csr<csr_mode::sweep, T, T, T>), dim3(A.n_blocks), dim3(kThreads), 0, s,
-                       a);
+void jacobi_sweep(const csr_view<T> &A, double shift, double omega, const T *rhs, const T *x,
+                  T *x_out, hipStream_t s) {
+    launch_csr<csr_mode::sweep>(csr_args<T, T, T>{A, shift, static_cast<T>(omega), x, rhs, x_out,
+                                                  nullptr}, s);
 }
 
 template <typename T>
-void level_residual(const csr_view<T> &A, double shift, const T *exit, const T *rhs, const T *x,
-                    T *r, hipStream_t s) {
-    if (A.n_blocks == 0) return;
-    csr_args<T, T, T> a{A, shift, T(0), exit, x, rhs, r, nullptr};
-    hipLaunchKernelGGL((k_csr<csr_mode::residual, T, T, T>), dim3(A.n_blocks), dim3(kThreads), 0,
-                       s, a);
+void level_residual(const csr_view<T> &A, double shift, const T *rhs, const T *x, T *r,
+                    hipStream_t s) {
+    launch_csr<csr_mode::residual>(csr_args<T, T, T>{A, shift, T(0), x, rhs, r, nullptr}, s);
 }
 
 template <typename T>
@@ -362,13 +447,6 @@ void axpy(i64 n, T alpha, const T *x, T *y, hipStream_t s) {
     hipLaunchKernelGGL((k_axpy<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, alpha, x, y);
 }
 
-template <typename T>
-void residual_update(i64 n, const T *rhs, const T *ax, T *r, hipStream_t s) {
-    if (n == 0) return;
-    hipLaunchKernelGGL((k_residual_update<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, rhs,
-                       ax, r);
-}
-
 void multi_dot(const double *V, i64 ld, int count, const double *w, i64 n, double *partial,
                int nb, hipStream_t s) {
     if (count <= 0) return;
@@ -401,16 +479,14 @@ void combine(double *x, const ZT *Z, i64 ld, int count, const double *y, i64 n, 
 
 // explicit instantiations
 template void shifted_apply<double>(const csr_view<double> &, double, const double *,
-                                    const double *, const double *, double *, double *,
-                                    hipStream_t);
-template void shifted_apply<float>(const csr_view<double> &, double, const double *,
-                                   const float *, const double *, double *, double *,
-                                   hipStream_t);
+                                    const double *, double *, double *, hipStream_t);
+template void shifted_apply<float>(const csr_view<double> &, double, const float *,
+                                   const double *, double *, double *, hipStream_t);
 #define MK_LEVEL(T)                                                                             \
     template void jacobi_sweep<T>(const csr_view<T> &, double, double, const T *, const T *,    \
-                                  const T *, T *, hipStream_t);                                 \
-    template void level_residual<T>(const csr_view<T> &, double, const T *, const T *,          \
-                                    const T *, T *, hipStream_t);                               \
+                                  T *, hipStream_t);                                            \
+    template void level_residual<T>(const csr_view<T> &, double, const T *, const T *, T *,     \
+                                    hipStream_t);                                               \
     template void restrict_children<T>(i64, const u32 *, const u32 *, const T *, T *,           \
                                        hipStream_t);                                            \
     template void prolong_add<T>(i64, const u32 *, const T *, const T *, T *, hipStream_t);     \
@@ -418,16 +494,15 @@ template void shifted_apply<float>(const csr_view<double> &, double, const doubl
                                     const T *, T *, hipStream_t);                               \
     template void fill<T>(i64, T, T *, hipStream_t);                                            \
     template void axpy<T>(i64, T, const T *, T *, hipStream_t);                                 \
-    template void residual_update<T>(i64, const T *, const T *, T *, hipStream_t);              \
     template void combine<T>(double *, const T *, i64, int, const double *, i64, hipStream_t);
 MK_LEVEL(float)
 MK_LEVEL(double)
 #undef MK_LEVEL
-template void jacobi_first<double, double>(i64, double, double, const double *, const double *,
-                                           double *, double *, hipStream_t);
-template void jacobi_first<float, double>(i64, double, double, const float *, const double *,
-                                          float *, float *, hipStream_t);
-template void jacobi_first<float, float>(i64, double, double, const float *, const float *,
+template void jacobi_first<double, double>(const csr_view<double> &, double, double,
+                                           const double *, double *, double *, hipStream_t);
+template void jacobi_first<float, double>(const csr_view<float> &, double, double,
+                                          const double *, float *, float *, hipStream_t);
+template void jacobi_first<float, float>(const csr_view<float> &, double, double, const float *,
                                          float *, float *, hipStream_t);
 template void convert<double, float>(i64, const double *, float *, hipStream_t);
 template void convert<float, double>(i64, const float *, double *, hipStream_t);

@@ -1,11 +1,19 @@
 // Device kernels of the resolvent solve (gfx950, wave64) and their launchers.
 //
-// Sparse layout: every operator is stored by destination ("gather" CSR): row i lists the
-// jumps j -> i as (column j, positive rate), so y_i = (shift + exit_i) x_i - sum rate x_j.
-// Rows are grouped into row blocks whose entries fit one workgroup's LDS tile; a workgroup
-// streams its block's columns and values with coalesced loads, stages the products in LDS,
-// and each thread then sums one row (CSR-stream). Blocks holding one row longer than the tile
-// fall back to a workgroup-wide reduction of that row.
+// Sparse layout: every operator A = shift I - C is stored by destination ("gather" form): row i
+// lists the jumps j -> i as (column j, positive rate), so (A x)_i = (shift + exit_i) x_i -
+// sum rate x_j. Rows are grouped into row blocks of at most kRowsPerBlock rows whose entries
+// fit one workgroup's LDS tile. A workgroup streams its block's entries with coalesced
+// non-temporal loads, gathers x, stages the products in LDS, and each thread sums one row.
+//
+// Two encodings of the same operator:
+//  * coded (used whenever it fits): the rates of a Markov generator take few distinct values,
+//    so an entry is one u32 (column in the low 24 bits, index into a rate dictionary in the
+//    high 8 bits), a row is a u8 length plus a u8 index into an exit-rate dictionary, and
+//    each row block carries its first entry. 4 bytes per entry and 2 per row.
+//  * plain: u32 row pointers and columns, rates and exit rates in VT.
+// Workgroups are mapped so that each XCD (8 per MI355X, dispatched round-robin) processes a
+// contiguous range of row blocks, which keeps neighbouring rows' gathers in one L2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,43 +25,60 @@ namespace mk::gpu {
 using i64 = std::int64_t;
 using u64 = std::uint64_t;
 using u32 = std::uint32_t;
+using u8 = std::uint8_t;
 
-constexpr int kThreads = 256;     // workgroup size (4 wave64)
-constexpr int kRowsPerBlock = 256; // rows per CSR row block (one per thread)
-constexpr int kTile = 4096;       // LDS product tile (entries) per row block
-constexpr int kDotChunk = 8;      // basis vectors per multi-dot pass
+constexpr int kThreads = 256;      // workgroup size (4 wave64)
+constexpr int kRowsPerBlock = 256; // rows per row block (one per thread)
+constexpr int kTile = 4096;        // LDS product tile (entries) per row block
+constexpr int kDotChunk = 8;       // basis vectors per multi-dot pass
+constexpr int kDictMax = 256;
+constexpr u32 kColBits = 24;
+constexpr u32 kColMask = (1u << kColBits) - 1u;
 
-// device view of a gather CSR operator
+// device view of one operator (shift I - C) without its shift
 template <typename VT>
 struct csr_view {
     i64 n = 0;
+    i64 n_blocks = 0;
+    const u32 *blocks = nullptr; // n_blocks + 1 row boundaries
+    // plain encoding
     const u32 *ptr = nullptr;    // n + 1
     const u32 *col = nullptr;
     const VT *val = nullptr;
-    const u32 *blocks = nullptr; // n_blocks + 1 row boundaries
-    i64 n_blocks = 0;
+    const VT *exit = nullptr;    // n
+    // coded encoding (active when coded != 0; ent may be null when the operator has no
+    // entries)
+    int coded = 0;
+    const u32 *bptr = nullptr;   // n_blocks + 1: first entry of each row block
+    const u32 *ent = nullptr;    // column | rate code << 24
+    const u8 *rlen = nullptr;    // row lengths
+    const u8 *rcode = nullptr;   // exit-rate codes
+    const VT *vdict = nullptr;
+    const VT *edict = nullptr;
+    int n_vdict = 0;
+    int n_edict = 0;
 };
 
-// y = (shift + exit) o x - C x            (shifted generator, FP64 accumulate)
-// with b: y = b - ((shift + exit) o x - C x) and, if partial != null, per-workgroup sums of
-// y^2 in partial[blockIdx]
+// y = (shift + exit) o x - C x             (FP64 accumulate; x in XT)
+// with b: y = b - ((shift + exit) o x - C x) and, if partial != null, one partial sum of y^2
+// per row block in partial[block]
 template <typename XT>
-void shifted_apply(const csr_view<double> &A, double shift, const double *exit, const XT *x,
-                   const double *b, double *y, double *partial, hipStream_t s);
+void shifted_apply(const csr_view<double> &A, double shift, const XT *x, const double *b,
+                   double *y, double *partial, hipStream_t s);
 
 // Jacobi smoothing on one multigrid level, working precision T:
-//   first sweep from x = 0:  rhs_out = (T) rhs_in;  x = w * rhs / (shift + exit)
+//   first sweep from x = 0:  rhs_out = (T) rhs_in (if rhs_out);  x = w * rhs / (shift + exit)
 template <typename T, typename IN>
-void jacobi_first(i64 n, double shift, double omega, const T *exit, const IN *rhs_in, T *rhs_out,
+void jacobi_first(const csr_view<T> &A, double shift, double omega, const IN *rhs_in, T *rhs_out,
                   T *x, hipStream_t s);
 //   sweep: x_out = x + w * (rhs - A x) / (shift + exit)
 template <typename T>
-void jacobi_sweep(const csr_view<T> &A, double shift, double omega, const T *exit, const T *rhs,
-                  const T *x, T *x_out, hipStream_t s);
+void jacobi_sweep(const csr_view<T> &A, double shift, double omega, const T *rhs, const T *x,
+                  T *x_out, hipStream_t s);
 //   residual: r = rhs - A x
 template <typename T>
-void level_residual(const csr_view<T> &A, double shift, const T *exit, const T *rhs, const T *x,
-                    T *r, hipStream_t s);
+void level_residual(const csr_view<T> &A, double shift, const T *rhs, const T *x, T *r,
+                    hipStream_t s);
 // restriction by coarse rows: coarse[c] = sum of fine[children of c] in ascending order
 template <typename T>
 void restrict_children(i64 n_coarse, const u32 *cptr, const u32 *cidx, const T *fine, T *coarse,
@@ -72,9 +97,7 @@ void fill(i64 n, T value, T *x, hipStream_t s);
 template <typename A, typename B>
 void convert(i64 n, const A *in, B *out, hipStream_t s);
 template <typename T>
-void axpy(i64 n, T alpha, const T *x, T *y, hipStream_t s);   // y += alpha x
-template <typename T>
-void residual_update(i64 n, const T *rhs, const T *ax, T *r, hipStream_t s); // r = rhs - ax
+void axpy(i64 n, T alpha, const T *x, T *y, hipStream_t s); // y += alpha x
 
 // ---- Krylov vector kernels (FP64) ----
 int reduction_blocks(i64 n);

@@ -53,6 +53,12 @@ void run_tasks(i64 n_tasks, F &&task) {
     if (failure) std::rethrow_exception(failure);
 }
 
+} // namespace
+
+void parallel_tasks(i64 n_tasks, const std::function<void(i64)> &task) { run_tasks(n_tasks, task); }
+
+namespace {
+
 u64 sat_add(u64 a, u64 b) { return a > std::numeric_limits<u64>::max() - b ? std::numeric_limits<u64>::max() : a + b; }
 
 } // namespace

@@ -11,6 +11,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <optional>
 #include <stdexcept>
 #include <string>
@@ -171,5 +172,8 @@ sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::v
 bool invert_dense(std::vector<double> a, i64 n, std::vector<double> &inverse);
 
 unsigned worker_threads();
+
+// task(t) for t in [0, n_tasks) on the worker pool (dynamically scheduled)
+void parallel_tasks(i64 n_tasks, const std::function<void(i64)> &task);
 
 } // namespace mk

@@ -180,3 +180,32 @@ def test_large_box_lumped_solution():
     bound = (np.abs(r).sum() + 16 * EPS * absAx) / shift
     print(f"[large] sector-sum error {np.abs(Xk - X).sum():.3e} bound {bound:.3e}")
     assert np.abs(Xk - X).sum() <= bound
+
+
+def test_plain_layout_generator():
+    """A generator whose rates take more than 256 distinct values is stored in the plain
+    layout (u32 columns, FP64 rates); its SpMV and resolvent solve must match too."""
+    w = birth_death_1d(8, 3, 0.5, "random")
+    P = Problem(w.model, w.max_totals)
+    first, to, rate = P.jumps()
+    rng = np.random.default_rng(5)
+    rate = rate * (1.0 + 0.5 * rng.random(len(rate)))  # distinct rates: plain layout
+    ex, lk = P.rates()
+    exit_ = lk + np.bincount(np.repeat(np.arange(P.n), np.diff(first)), weights=rate,
+                             minlength=P.n)
+    Q = mk.Generator.from_jumps(first, to, rate, exit_, lk)
+    import scipy.sparse as sp
+    src = np.repeat(np.arange(P.n), np.diff(first))
+    C = sp.csr_matrix((rate, (to, src)), shape=(P.n, P.n))
+    x = rng.standard_normal(P.n)
+    y = Q.apply_shifted(0.5, x)
+    y_ref = (0.5 + exit_) * x - C @ x
+    scale = (0.5 + exit_) * np.abs(x) + abs(C) @ np.abs(x)
+    assert np.all(np.abs(y - y_ref) <= 64 * EPS * scale)
+    b = w.rhs(0, P.n)
+    xs = np.zeros(P.n)
+    res = mk.gmres(mk.ShiftedGenerator(Q, 0.5), b, xs, tolerance=1e-10 * np.linalg.norm(b),
+                   max_iterations=500, restart=100)
+    assert res.converged
+    r = b - ((0.5 + exit_) * xs - C @ xs)
+    assert np.linalg.norm(r) <= 1.0001e-10 * np.linalg.norm(b)

@@ -160,6 +160,8 @@ mk_status mk_device_free(void *d_ptr);
 mk_status mk_copy_to_device(void *d_dst, const void *src, int64_t bytes);
 mk_status mk_copy_to_host(void *dst, const void *d_src, int64_t bytes);
 mk_status mk_device_zero(void *d_ptr, int64_t bytes);
+/* d_dst = d_src with the library's plain copy kernel (HBM calibration of byte counters) */
+mk_status mk_device_copy(void *d_dst, const void *d_src, int64_t bytes);
 mk_status mk_synchronize(void);
 /* per-kernel-class timing with HIP events on the library stream. Classes, in bit order:
  * "shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict", "prolong",

@@ -69,6 +69,7 @@ SIGNATURES = {
     "mk_copy_to_device": ([P, P, i64], i32),
     "mk_copy_to_host": ([P, P, i64], i32),
     "mk_device_zero": ([P, i64], i32),
+    "mk_device_copy": ([P, P, i64], i32),
     "mk_synchronize": ([], i32),
     "mk_profile_enable": ([i32], i32),
     "mk_profile_reset": ([], i32),

@@ -212,130 +212,83 @@ struct dev_csr {
     i64 n = 0;
     u64 nnz = 0;
     bool coded = false;
-    dbuf blocks, ptr, col, val, exit, bptr, ent, rlen, rcode, vdict, edict;
+    u64 padded = 0; // stored entries of the coded layout, slice padding included
+    dbuf blocks, ptr, col, val, exit, soff, ent, rlen, rcode, vdict, edict;
     gpu::csr_view<VT> view;
 };
-
-// the distinct values of v converted to VT, sorted; empty when there are more than kDictMax
-template <typename VT>
-std::vector<VT> dictionary(const std::vector<double> &v) {
-    const i64 n = static_cast<i64>(v.size());
-    const i64 chunk = 1 << 20;
-    const i64 tasks = (n + chunk - 1) / chunk;
-    std::vector<std::vector<VT>> parts(tasks);
-    std::vector<char> over(tasks, 0);
-    mk::parallel_tasks(tasks, [&](i64 t) {
-        auto &d = parts[t];
-        for (i64 i = t * chunk; i < std::min(n, (t + 1) * chunk); ++i) {
-            const VT x = static_cast<VT>(v[i]);
-            auto it = std::lower_bound(d.begin(), d.end(), x);
-            if (it == d.end() || *it != x) {
-                if (d.size() >= static_cast<std::size_t>(gpu::kDictMax)) {
-                    over[t] = 1;
-                    return;
-                }
-                d.insert(it, x);
-            }
-        }
-    });
-    std::vector<VT> all;
-    for (i64 t = 0; t < tasks; ++t) {
-        if (over[t]) return {};
-        for (VT x : parts[t]) {
-            auto it = std::lower_bound(all.begin(), all.end(), x);
-            if (it == all.end() || *it != x) all.insert(it, x);
-        }
-        if (all.size() > static_cast<std::size_t>(gpu::kDictMax)) return {};
-    }
-    if (all.empty()) all.push_back(VT(0));
-    return all;
-}
-
-template <typename VT>
-u32 code_of(const std::vector<VT> &dict, double v) {
-    return static_cast<u32>(std::lower_bound(dict.begin(), dict.end(), static_cast<VT>(v)) - dict.begin());
-}
 
 // every array the selected kernel reads must exist before anything is launched on it
 template <typename VT>
 void check_view(const gpu::csr_view<VT> &v, u64 nnz) {
-    bool ok = v.n == 0 || (v.blocks && v.n_blocks > 0);
+    bool ok = v.n == 0 || v.n_blocks > 0;
     if (v.coded) {
-        ok = ok && (v.n == 0 || (v.bptr && v.rlen && v.rcode && v.edict && v.vdict)) &&
+        ok = ok && v.n_slices == (v.n + gpu::kSlice - 1) / gpu::kSlice &&
+             v.n_blocks * (gpu::kThreads / gpu::kSlice) >= v.n_slices &&
+             (v.n == 0 || (v.soff && v.rlen && v.rcode && v.edict && v.vdict)) &&
              (nnz == 0 || v.ent) && v.n_vdict > 0 && v.n_vdict <= gpu::kDictMax &&
              v.n_edict > 0 && v.n_edict <= gpu::kDictMax;
     } else {
-        ok = ok && (v.n == 0 || (v.ptr && v.exit)) && (nnz == 0 || (v.col && v.val));
+        ok = ok && (v.n == 0 || (v.blocks && v.ptr && v.exit)) && (nnz == 0 || (v.col && v.val));
     }
     if (!ok) throw std::logic_error("operator upload left a required device array missing");
 }
 
-// upload an operator (gather CSR + exit rates) in the coded layout when it fits, else plain
+// upload an operator (gather CSR + exit rates) in the coded sliced layout when it fits, else
+// in the plain CSR-stream layout
 template <typename VT>
 dev_csr<VT> upload_csr(const mk::generator_csr &g) {
     dev_csr<VT> d;
     d.n = g.n;
     d.nnz = g.ptr.empty() ? 0 : g.ptr.back();
     if (d.nnz >= (u64{1} << 32)) throw std::overflow_error("operator has more than 2^32 entries");
-    // row blocks: up to kRowsPerBlock rows whose entries fit the LDS tile; a row longer than
-    // the tile gets a block of its own
-    std::vector<u32> blocks{0}, bptr{0};
-    i64 r = 0;
-    u64 max_len = 0;
-    while (r < g.n) {
-        const i64 r0 = r;
-        const u64 p0 = g.ptr[r0];
-        max_len = std::max<u64>(max_len, g.ptr[r0 + 1] - p0);
-        ++r;
-        while (r < g.n && r - r0 < gpu::kRowsPerBlock &&
-               g.ptr[r + 1] - p0 <= static_cast<u64>(gpu::kTile)) {
-            max_len = std::max<u64>(max_len, g.ptr[r + 1] - g.ptr[r]);
-            ++r;
-        }
-        blocks.push_back(static_cast<u32>(r));
-        bptr.push_back(static_cast<u32>(g.ptr[r]));
-    }
-    d.blocks = upload(blocks);
     d.view.n = g.n;
-    d.view.n_blocks = static_cast<i64>(blocks.size()) - 1;
-    d.view.blocks = dp<u32>(d.blocks);
+    u64 max_len = 0;
+    for (i64 i = 0; i < g.n; ++i) max_len = std::max<u64>(max_len, g.ptr[i + 1] - g.ptr[i]);
 
-    std::vector<VT> vdict, edict;
-    if (g.n <= static_cast<i64>(gpu::kColMask) + 1 && max_len <= 255) {
-        vdict = dictionary<VT>(g.val);
-        if (!vdict.empty()) edict = dictionary<VT>(g.exit);
-    }
-    if (!vdict.empty() && !edict.empty()) {
+    static_assert(mk::kSliceRows == gpu::kSlice && mk::kCodedColumnBits == static_cast<int>(gpu::kColBits),
+                  "host and device agree on the coded layout");
+    (void)max_len;
+    mk::sliced_operator<VT> so;
+    if (mk::encode_sliced<VT>(g, so)) {
+        const i64 n_slices = static_cast<i64>(so.soff.size()) - 1;
         d.coded = true;
-        std::vector<u32> ent(d.nnz);
-        std::vector<std::uint8_t> rlen(g.n), rcode(g.n);
-        const i64 chunk = 1 << 16;
-        mk::parallel_tasks((g.n + chunk - 1) / chunk, [&](i64 t) {
-            for (i64 i = t * chunk; i < std::min(g.n, (t + 1) * chunk); ++i) {
-                rlen[i] = static_cast<std::uint8_t>(g.ptr[i + 1] - g.ptr[i]);
-                rcode[i] = static_cast<std::uint8_t>(code_of(edict, g.exit[i]));
-                for (u64 p = g.ptr[i]; p < g.ptr[i + 1]; ++p)
-                    ent[p] = g.col[p] | (code_of(vdict, g.val[p]) << gpu::kColBits);
-            }
-        });
-        d.bptr = upload(bptr);
-        d.ent = upload(ent);
-        d.rlen = upload(rlen);
-        d.rcode = upload(rcode);
-        d.vdict = upload(vdict);
-        d.edict = upload(edict);
-        d.view.bptr = dp<u32>(d.bptr);
+        d.padded = so.padded;
+        d.soff = upload(so.soff);
+        d.ent = upload(so.ent);
+        d.rlen = upload(so.rlen);
+        d.rcode = upload(so.rcode);
+        d.vdict = upload(so.vdict);
+        d.edict = upload(so.edict);
+        d.view.n_slices = n_slices;
+        d.view.n_blocks = (n_slices + (gpu::kThreads / gpu::kSlice) - 1) / (gpu::kThreads / gpu::kSlice);
+        d.view.soff = dp<u32>(d.soff);
         d.view.ent = dp<u32>(d.ent);
         d.view.rlen = dp<std::uint8_t>(d.rlen);
         d.view.rcode = dp<std::uint8_t>(d.rcode);
         d.view.vdict = dp<VT>(d.vdict);
         d.view.edict = dp<VT>(d.edict);
-        d.view.n_vdict = static_cast<int>(vdict.size());
-        d.view.n_edict = static_cast<int>(edict.size());
+        d.view.n_vdict = static_cast<int>(so.vdict.size());
+        d.view.n_edict = static_cast<int>(so.edict.size());
         d.view.coded = 1;
         check_view(d.view, d.nnz);
         return d;
     }
+    // plain CSR-stream: row blocks of up to kRowsPerBlock rows whose entries fit the LDS tile;
+    // a row longer than the tile gets a block of its own
+    std::vector<u32> blocks{0};
+    i64 r = 0;
+    while (r < g.n) {
+        const i64 r0 = r;
+        const u64 p0 = g.ptr[r0];
+        ++r;
+        while (r < g.n && r - r0 < gpu::kRowsPerBlock &&
+               g.ptr[r + 1] - p0 <= static_cast<u64>(gpu::kTile))
+            ++r;
+        blocks.push_back(static_cast<u32>(r));
+    }
+    d.blocks = upload(blocks);
+    d.view.n_blocks = static_cast<i64>(blocks.size()) - 1;
+    d.view.blocks = dp<u32>(d.blocks);
     std::vector<u32> ptr(g.n + 1);
     for (i64 i = 0; i <= g.n; ++i) ptr[i] = static_cast<u32>(g.ptr[i]);
     d.ptr = upload(ptr);
@@ -357,7 +310,7 @@ dev_csr<VT> upload_csr(const mk::generator_csr &g) {
 template <typename VT>
 double op_bytes(const dev_csr<VT> &A, std::size_t xs, std::size_t ts, int vectors) {
     const double n = double(A.n), nnz = double(A.nnz);
-    double b = A.coded ? nnz * 4.0 + n * 2.0 + double(A.view.n_blocks + 1) * 8.0
+    double b = A.coded ? double(A.padded) * 4.0 + n * 2.0 + double(A.view.n_slices + 1) * 4.0
                        : nnz * double(sizeof(VT) + 4) + (n + 1) * 4.0 + n * sizeof(VT);
     return b + n * double(xs) + n * double(ts) * vectors;
 }
@@ -391,7 +344,7 @@ struct level_dev {
     // transfer to the next level
     dbuf coarse_of, weight;
     i64 n_fine_linked = 0; // rows that have a coarse parent
-    dbuf cptr, cidx;       // children of the next level's rows
+    dev_csr<T> R;          // restriction: next level's rows x this level's rows, rates 1
     i64 n_next = 0;
     // terminal blocks on this level
     i64 n_term = 0, term_max = 0;
@@ -501,8 +454,13 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
             lv.coarse_of = upload(coarse_of);
             std::vector<T> w_t(weight.begin(), weight.end());
             lv.weight = upload(w_t);
-            lv.cptr = upload(cptr);
-            lv.cidx = upload(cidx);
+            mk::generator_csr Rg;
+            Rg.n = nc;
+            Rg.ptr.assign(cptr.begin(), cptr.end());
+            Rg.col = std::move(cidx);
+            Rg.val.assign(Rg.col.size(), 1.0);
+            Rg.exit.assign(nc, 0.0);
+            lv.R = upload_csr<T>(Rg);
         }
         const std::size_t vb = static_cast<std::size_t>(std::max<i64>(lv.n, 1)) * sizeof(T);
         lv.rhs = dbuf(vb);
@@ -566,12 +524,12 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
     }
     if (lv.n_next > 0) {
         auto &c = L[l + 1];
-        const double rbytes = double(lv.n_fine_linked) * (ts + 4) + double(c.n + 1) * 4 + double(c.n) * ts;
+        // restriction reads each linked fine row once (x) plus R's encoding, writes the coarse rows
+        const double rbytes = op_bytes(lv.R, 0, ts, 1) + double(lv.n_fine_linked) * ts;
         const double pbytes = double(lv.n) * (4 + 3 * ts) + double(c.n) * ts;
         if (mg.opt.coarse_cycles <= 1) {
             P.run(K_RESTRICT, rbytes, [&] {
-                gpu::restrict_children<T>(c.n, dp<u32>(lv.cptr), dp<u32>(lv.cidx), res,
-                                          dp<T>(c.rhs), s);
+                gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.rhs), s);
             });
             T *cx = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr);
             P.run(K_PROLONG, pbytes, [&] {
@@ -579,8 +537,7 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             });
         } else {
             P.run(K_RESTRICT, rbytes, [&] {
-                gpu::restrict_children<T>(c.n, dp<u32>(lv.cptr), dp<u32>(lv.cidx), res,
-                                          dp<T>(c.crhs), s);
+                gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.crhs), s);
             });
             P.run(K_OTHER, double(c.n) * ts, [&] { gpu::fill<T>(c.n, T(0), dp<T>(c.cx), s); });
             const double cbytes = op_bytes(c.A, ts, ts, 2);
@@ -632,7 +589,9 @@ struct gmres_workspace {
     i64 m = 0;
     std::size_t zs = 0;
     dbuf V, Z, w, r, partial, hd, yd;
-    double *host = nullptr; // pinned: h1, h2 (m+1 each), norm^2, then y (m+1)
+    // device scalars hd and their pinned host copy: pass-1 projections at 0, pass-2
+    // projections and |w|^2 at m + 2, (beta, flag) at 2 (m + 2); y follows at 3 (m + 2)
+    double *host = nullptr;
     ~gmres_workspace() {
         if (host) (void)hipHostFree(host);
     }
@@ -649,11 +608,11 @@ struct gmres_workspace {
         r = dbuf(n * sizeof(double));
         const int nb = gpu::reduction_blocks(n);
         partial = dbuf(static_cast<std::size_t>(nb) * (m + 2) * sizeof(double));
-        hd = dbuf((2 * (m + 1) + 2) * sizeof(double));
+        hd = dbuf(3 * (m + 2) * sizeof(double));
         yd = dbuf((m + 1) * sizeof(double));
         if (host) (void)hipHostFree(host);
         host = nullptr;
-        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&host), (3 * (m + 1) + 2) * sizeof(double)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&host), (4 * (m + 2)) * sizeof(double)));
     }
 };
 
@@ -726,27 +685,63 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
             }
             // classical Gram-Schmidt, twice (CGS2)
             const int cnt = static_cast<int>(j + 1);
-            for (int pass = 0; pass < 2; ++pass) {
-                double *h = hd + pass * (m + 1);
+            double *h1d = hd, *cwd = hd + (m + 2), *bfd = hd + 2 * (m + 2);
+            double *vnext = V + (j + 1) * n;
+            const bool fused = cnt <= gpu::kFusedBasis;
+            if (fused) {
+                // pass 1 projections; update fused with the pass-2 projections and |w|^2;
+                // pass-2 update fused with the normalisation
                 P.run(K_DOT, double(cnt + 1) * n * 8.0, [&] {
-                    gpu::multi_dot(V, n, cnt, w, n, partial, nb, s);
-                    gpu::reduce_partials(partial, nb, cnt, h, s);
+                    gpu::dot_tiled(V, n, cnt, w, n, partial, nb, s);
+                    gpu::reduce_partials(partial, nb, cnt, h1d, s);
                 });
-                double *np = pass == 1 ? partial + static_cast<i64>(nb) * (m + 1) : nullptr;
                 P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {
-                    gpu::basis_update(w, V, n, cnt, h, n, np, nb, s);
+                    gpu::update_dot(w, V, n, cnt, h1d, n, partial, nb, s);
+                    gpu::reduce_partials(partial, nb, cnt + 1, cwd, s);
                 });
+                P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {
+                    gpu::update_scale(w, V, n, cnt, cwd, n, vnext, bfd, s);
+                });
+            } else {
+                for (int pass = 0; pass < 2; ++pass) {
+                    double *h = pass == 0 ? h1d : cwd;
+                    P.run(K_DOT, double(cnt + 1) * n * 8.0, [&] {
+                        gpu::multi_dot(V, n, cnt, w, n, partial, nb, s);
+                        gpu::reduce_partials(partial, nb, cnt, h, s);
+                    });
+                    double *np = pass == 1 ? partial + static_cast<i64>(nb) * (m + 1) : nullptr;
+                    P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {
+                        gpu::basis_update(w, V, n, cnt, h, n, np, nb, s);
+                    });
+                }
+                P.run(K_DOT, nb * 8.0, [&] {
+                    gpu::reduce_partials(partial + static_cast<i64>(nb) * (m + 1), nb, 1, bfd + 2, s);
+                });
+                P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, w, bfd + 2, vnext, s); });
             }
-            double *norm2 = hd + 2 * (m + 1);
-            P.run(K_DOT, nb * 8.0, [&] {
-                gpu::reduce_partials(partial + static_cast<i64>(nb) * (m + 1), nb, 1, norm2, s);
-            });
-            P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, w, norm2, V + (j + 1) * n, s); });
-            HIPCHK(hipMemcpyAsync(ws.host, hd, (2 * (m + 1) + 1) * sizeof(double),
+            HIPCHK(hipMemcpyAsync(ws.host, hd, 3 * (m + 2) * sizeof(double),
                                   hipMemcpyDeviceToHost, s));
             drain();
-            const double *h1 = ws.host, *h2 = ws.host + (m + 1);
-            const double hn = std::sqrt(ws.host[2 * (m + 1)]);
+            const double *h1 = ws.host, *h2 = ws.host + (m + 2);
+            double hn;
+            if (fused) {
+                hn = ws.host[2 * (m + 2)];
+                if (ws.host[2 * (m + 2) + 1] != 0.0) {
+                    // the second pass removed too much for the Pythagorean norm: v_{j+1} holds
+                    // the unnormalised vector; normalise it explicitly
+                    P.run(K_DOT, n * 8.0, [&] {
+                        gpu::multi_dot(vnext, n, 1, vnext, n, partial, nb, s);
+                        gpu::reduce_partials(partial, nb, 1, bfd + 2, s);
+                    });
+                    P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, vnext, bfd + 2, vnext, s); });
+                    HIPCHK(hipMemcpyAsync(ws.host + 2 * (m + 2) + 2, bfd + 2, sizeof(double),
+                                          hipMemcpyDeviceToHost, s));
+                    drain();
+                    hn = std::sqrt(ws.host[2 * (m + 2) + 2]);
+                }
+            } else {
+                hn = std::sqrt(ws.host[2 * (m + 2) + 2]);
+            }
             for (i64 k = 0; k <= j; ++k) H[k * m + j] = h1[k] + h2[k];
             H[(j + 1) * m + j] = hn;
             for (i64 k = 0; k < j; ++k) {
@@ -772,7 +767,7 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
             for (i64 k = i + 1; k < j; ++k) sum -= H[i * m + k] * y[k];
             y[i] = sum / H[i * m + i];
         }
-        double *ypin = ws.host + 2 * (m + 1) + 2;
+        double *ypin = ws.host + 3 * (m + 2);
         std::copy(y.begin(), y.begin() + j, ypin);
         HIPCHK(hipMemcpyAsync(ws.yd.p, ypin, j * sizeof(double), hipMemcpyHostToDevice, s));
         P.run(K_OTHER, double(j) * n * (M ? sizeof(ZT) : 8) + n * 16.0, [&] {
@@ -1144,6 +1139,15 @@ mk_status mk_device_zero(void *d_ptr, int64_t bytes) {
     return guarded([&] {
         require(bytes >= 0, "device_zero: negative size");
         if (bytes > 0) HIPCHK(hipMemsetAsync(d_ptr, 0, bytes, stream()));
+    });
+}
+
+mk_status mk_device_copy(void *d_dst, const void *d_src, int64_t bytes) {
+    return guarded([&] {
+        require(bytes >= 0 && bytes % 16 == 0, "device_copy: size must be a multiple of 16 bytes");
+        require((reinterpret_cast<std::uintptr_t>(d_dst) | reinterpret_cast<std::uintptr_t>(d_src)) % 16 == 0,
+                "device_copy: pointers must be 16-byte aligned");
+        prof().run(K_OTHER, 2.0 * bytes, [&] { gpu::copy16(bytes / 16, d_src, d_dst, stream()); });
     });
 }
 

@@ -42,7 +42,7 @@ __device__ __forceinline__ u32 xcd_block(u32 bid, u32 nb) {
     return xcd * q + min(xcd, r) + slot;
 }
 
-enum class csr_mode { shifted, shifted_residual, sweep, residual };
+enum class csr_mode { shifted, shifted_residual, sweep, residual, gather_sum };
 
 template <typename VT, typename XT, typename T>
 struct csr_args {
@@ -57,6 +57,10 @@ struct csr_args {
 
 template <csr_mode M, typename VT, typename XT, typename T>
 __device__ __forceinline__ double finish(const csr_args<VT, XT, T> &a, u32 r, T sum, T exit_r) {
+    if constexpr (M == csr_mode::gather_sum) {
+        a.out[r] = sum;
+        return 0.0;
+    }
     const T xr = static_cast<T>(a.x[r]);
     const T d = static_cast<T>(a.shift) + exit_r;
     const T ax = d * xr - sum;
@@ -71,69 +75,53 @@ __device__ __forceinline__ double finish(const csr_args<VT, XT, T> &a, u32 r, T 
         const T res = a.rhs[r] - ax;
         a.out[r] = xr + a.omega * res / d;
         return 0.0;
+    } else if constexpr (M == csr_mode::gather_sum) {
+        a.out[r] = sum;
+        return 0.0;
     } else {
         a.out[r] = a.rhs[r] - ax;
         return 0.0;
     }
 }
 
-// coded operator: one workgroup per row block
+// coded sliced operator: one wavefront per 64-row slice, four slices per workgroup
+constexpr int kSliceUnroll = 16;
+
 template <csr_mode M, typename VT, typename XT, typename T>
-__global__ void __launch_bounds__(kThreads) k_csr_coded(csr_args<VT, XT, T> a) {
-    __shared__ T prod[kTile];
+__global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) {
     __shared__ T vdict[kDictMax];
     __shared__ T edict[kDictMax];
-    __shared__ u32 wsum[kWaves];
     __shared__ double red64[kWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 b = xcd_block(blockIdx.x, gridDim.x);
-    const u32 r0 = a.A.blocks[b], r1 = a.A.blocks[b + 1];
-    const u32 p0 = a.A.bptr[b], p1 = a.A.bptr[b + 1];
-    const int tid = threadIdx.x;
-
-    // stream the block's entries (all loads issued before any is used)
-    u32 e[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-        const u32 p = p0 + tid + u * kThreads;
-        e[u] = p < p1 ? __builtin_nontemporal_load(a.A.ent + p) : 0u;
-    }
-    const u32 r = r0 + tid;
-    const bool has_row = r < r1;
-    const u32 len = has_row ? a.A.rlen[r] : 0u;
-    const u32 ec = has_row ? a.A.rcode[r] : 0u;
     for (int i = tid; i < a.A.n_vdict; i += kThreads) vdict[i] = static_cast<T>(a.A.vdict[i]);
     for (int i = tid; i < a.A.n_edict; i += kThreads) edict[i] = static_cast<T>(a.A.edict[i]);
-    // gather x
-    XT xv[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-        const u32 p = p0 + tid + u * kThreads;
-        xv[u] = p < p1 ? a.x[e[u] & kColMask] : XT(0);
-    }
-    // each row's first product: exclusive scan of the row lengths
-    const int lane = tid & 63, wave = tid >> 6;
-    u32 incl = len;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const u32 up = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += up;
-    }
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    u32 start = incl - len;
-    for (int w = 0; w < wave; ++w) start += wsum[w];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-        const u32 q = tid + u * kThreads;
-        if (p0 + q < p1) prod[q] = vdict[e[u] >> kColBits] * static_cast<T>(xv[u]);
-    }
-    __syncthreads();
-    double sq = 0.0;
+    const i64 slice = static_cast<i64>(b) * kWaves + wave;
+    const i64 r = slice * kSlice + lane;
+    const bool has_row = slice < a.A.n_slices && r < a.A.n;
+    u32 len = 0, ec = 0, base = 0;
     if (has_row) {
-        T sum = 0;
-        for (u32 q = start; q < start + len; ++q) sum += prod[q];
-        sq = finish<M>(a, r, sum, edict[ec]);
+        len = a.A.rlen[r];
+        ec = a.A.rcode[r];
+        base = a.A.soff[slice] + lane;
     }
+    __syncthreads();
+    T sum = 0;
+    for (u32 k0 = 0; k0 < len; k0 += kSliceUnroll) {
+        u32 e[kSliceUnroll];
+#pragma unroll
+        for (int u = 0; u < kSliceUnroll; ++u)
+            e[u] = k0 + u < len ? __builtin_nontemporal_load(a.A.ent + base + (k0 + u) * kSlice) : 0u;
+        XT xv[kSliceUnroll];
+#pragma unroll
+        for (int u = 0; u < kSliceUnroll; ++u)
+            xv[u] = k0 + u < len ? a.x[e[u] & kColMask] : XT(0);
+#pragma unroll
+        for (int u = 0; u < kSliceUnroll; ++u)
+            if (k0 + u < len) sum += vdict[e[u] >> kColBits] * static_cast<T>(xv[u]);
+    }
+    double sq = 0.0;
+    if (has_row) sq = finish<M>(a, static_cast<u32>(r), sum, edict[ec]);
     if constexpr (M == csr_mode::shifted_residual) {
         const double t = block_sum(sq, red64);
         if (tid == 0 && a.partial) a.partial[b] = t;
@@ -197,7 +185,7 @@ template <csr_mode M, typename VT, typename XT, typename T>
 void launch_csr(const csr_args<VT, XT, T> &a, hipStream_t s) {
     if (a.A.n_blocks == 0) return;
     if (a.A.coded) {
-        hipLaunchKernelGGL((k_csr_coded<M, VT, XT, T>), dim3(a.A.n_blocks), dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T>), dim3(a.A.n_blocks), dim3(kThreads), 0, s, a);
     } else {
         hipLaunchKernelGGL((k_csr_plain<M, VT, XT, T>), dim3(a.A.n_blocks), dim3(kThreads), 0, s, a);
     }
@@ -276,6 +264,12 @@ __global__ void __launch_bounds__(kThreads) k_axpy(i64 n, T alpha, const T *x, T
     for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
          i += static_cast<i64>(gridDim.x) * kThreads)
         y[i] += alpha * x[i];
+}
+
+__global__ void __launch_bounds__(kThreads) k_copy16(i64 n, const uint4 *src, uint4 *dst) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreads)
+        dst[i] = src[i];
 }
 
 // ---- Krylov kernels ----
@@ -359,6 +353,120 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+constexpr int kTileRows = 64;
+constexpr int kPerWave = kFusedBasis / kWaves; // basis vectors per wave
+
+__global__ void __launch_bounds__(kThreads)
+    k_dot_tiled(const double *V, i64 ld, int count, const double *w, i64 n, double *partial,
+                int nb) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double acc[kPerWave];
+#pragma unroll
+    for (int kk = 0; kk < kPerWave; ++kk) acc[kk] = 0.0;
+    for (i64 r0 = static_cast<i64>(blockIdx.x) * kTileRows; r0 < n;
+         r0 += static_cast<i64>(nb) * kTileRows) {
+        const i64 r = r0 + lane;
+        if (r < n) {
+            const double wr = w[r];
+#pragma unroll
+            for (int kk = 0; kk < kPerWave; ++kk) {
+                const int k = kk * kWaves + wave;
+                if (k < count) acc[kk] += V[k * ld + r] * wr;
+            }
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < kPerWave; ++kk) {
+        const int k = kk * kWaves + wave;
+        if (k < count) {
+            const double t = wave_sum(acc[kk]);
+            if (lane == 0) partial[static_cast<i64>(k) * nb + blockIdx.x] = t;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_update_dot(double *w, const double *V, i64 ld, int count, const double *h, i64 n,
+                 double *partial, int nb) {
+    __shared__ double part[kWaves][kTileRows];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double hk[kPerWave], acc[kPerWave];
+#pragma unroll
+    for (int kk = 0; kk < kPerWave; ++kk) {
+        const int k = kk * kWaves + wave;
+        hk[kk] = k < count ? h[k] : 0.0;
+        acc[kk] = 0.0;
+    }
+    double nrm = 0.0;
+    for (i64 r0 = static_cast<i64>(blockIdx.x) * kTileRows; r0 < n;
+         r0 += static_cast<i64>(nb) * kTileRows) {
+        const i64 r = r0 + lane;
+        const bool ok = r < n;
+        double v[kPerWave];
+        double u = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < kPerWave; ++kk) {
+            const int k = kk * kWaves + wave;
+            v[kk] = (ok && k < count) ? V[k * ld + r] : 0.0;
+            u += hk[kk] * v[kk];
+        }
+        part[wave][lane] = u;
+        __syncthreads();
+        double wr = 0.0;
+        if (ok) wr = w[r] - (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
+        __syncthreads();
+        if (ok) {
+            if (wave == 0) {
+                w[r] = wr;
+                nrm += wr * wr;
+            }
+#pragma unroll
+            for (int kk = 0; kk < kPerWave; ++kk) acc[kk] += v[kk] * wr;
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < kPerWave; ++kk) {
+        const int k = kk * kWaves + wave;
+        if (k < count) {
+            const double t = wave_sum(acc[kk]);
+            if (lane == 0) partial[static_cast<i64>(k) * nb + blockIdx.x] = t;
+        }
+    }
+    if (wave == 0) {
+        const double t = wave_sum(nrm);
+        if (lane == 0) partial[static_cast<i64>(count) * nb + blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_update_scale(const double *w, const double *V, i64 ld, int count, const double *cw, i64 n,
+                   double *v, double *beta_flag) {
+    __shared__ double cs[kFusedBasis];
+    __shared__ double beta_s;
+    for (int k = threadIdx.x; k < count; k += kThreads) cs[k] = cw[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double c2 = 0.0;
+        for (int k = 0; k < count; ++k) c2 += cs[k] * cs[k];
+        const double w2 = cw[count];
+        const bool trusted = c2 <= 0.25 * w2;
+        const double beta = trusted ? sqrt(w2 - c2) : 1.0;
+        beta_s = beta;
+        if (blockIdx.x == 0) {
+            beta_flag[0] = beta;
+            beta_flag[1] = trusted ? 0.0 : 1.0;
+        }
+    }
+    __syncthreads();
+    const double beta = beta_s;
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreads) {
+        double wi = w[i];
+        for (int k = 0; k < count; ++k) wi -= cs[k] * V[k * ld + i];
+        v[i] = wi / beta;
+    }
+}
+
 int grid_for(i64 n) {
     const i64 g = (n + kThreads - 1) / kThreads;
     return static_cast<int>(std::max<i64>(1, std::min<i64>(g, 8192)));
@@ -401,6 +509,11 @@ template <typename T>
 void level_residual(const csr_view<T> &A, double shift, const T *rhs, const T *x, T *r,
                     hipStream_t s) {
     launch_csr<csr_mode::residual>(csr_args<T, T, T>{A, shift, T(0), x, rhs, r, nullptr}, s);
+}
+
+template <typename T>
+void gather_sum(const csr_view<T> &R, const T *x, T *y, hipStream_t s) {
+    launch_csr<csr_mode::gather_sum>(csr_args<T, T, T>{R, 0.0, T(0), x, nullptr, y, nullptr}, s);
 }
 
 template <typename T>
@@ -447,6 +560,12 @@ void axpy(i64 n, T alpha, const T *x, T *y, hipStream_t s) {
     hipLaunchKernelGGL((k_axpy<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, alpha, x, y);
 }
 
+void copy16(i64 n, const void *src, void *dst, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_copy16, dim3(grid_for(n)), dim3(kThreads), 0, s, n,
+                       static_cast<const uint4 *>(src), static_cast<uint4 *>(dst));
+}
+
 void multi_dot(const double *V, i64 ld, int count, const double *w, i64 n, double *partial,
                int nb, hipStream_t s) {
     if (count <= 0) return;
@@ -470,6 +589,24 @@ void scale_by_norm(i64 n, const double *w, const double *norm2, double *v, hipSt
     hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(kThreads), 0, s, n, w, norm2, v);
 }
 
+void dot_tiled(const double *V, i64 ld, int count, const double *w, i64 n, double *partial,
+               int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_dot_tiled, dim3(nb), dim3(kThreads), 0, s, V, ld, count, w, n, partial,
+                       nb);
+}
+
+void update_dot(double *w, const double *V, i64 ld, int count, const double *h, i64 n,
+                double *partial, int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_update_dot, dim3(nb), dim3(kThreads), 0, s, w, V, ld, count, h, n,
+                       partial, nb);
+}
+
+void update_scale(const double *w, const double *V, i64 ld, int count, const double *cw, i64 n,
+                  double *v, double *beta_flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_update_scale, dim3(grid_for(n)), dim3(kThreads), 0, s, w, V, ld, count,
+                       cw, n, v, beta_flag);
+}
+
 template <typename ZT>
 void combine(double *x, const ZT *Z, i64 ld, int count, const double *y, i64 n, hipStream_t s) {
     if (count <= 0) return;
@@ -489,6 +626,7 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
                                     hipStream_t);                                               \
     template void restrict_children<T>(i64, const u32 *, const u32 *, const T *, T *,           \
                                        hipStream_t);                                            \
+    template void gather_sum<T>(const csr_view<T> &, const T *, T *, hipStream_t);              \
     template void prolong_add<T>(i64, const u32 *, const T *, const T *, T *, hipStream_t);     \
     template void terminal_solve<T>(i64, const u32 *, const u32 *, const u64 *, const T *, i64, \
                                     const T *, T *, hipStream_t);                               \

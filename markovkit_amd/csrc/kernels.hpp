@@ -2,18 +2,21 @@
 //
 // Sparse layout: every operator A = shift I - C is stored by destination ("gather" form): row i
 // lists the jumps j -> i as (column j, positive rate), so (A x)_i = (shift + exit_i) x_i -
-// sum rate x_j. Rows are grouped into row blocks of at most kRowsPerBlock rows whose entries
-// fit one workgroup's LDS tile. A workgroup streams its block's entries with coalesced
-// non-temporal loads, gathers x, stages the products in LDS, and each thread sums one row.
+// sum rate x_j, summed over the row's entries in ascending column order.
 //
 // Two encodings of the same operator:
-//  * coded (used whenever it fits): the rates of a Markov generator take few distinct values,
-//    so an entry is one u32 (column in the low 24 bits, index into a rate dictionary in the
-//    high 8 bits), a row is a u8 length plus a u8 index into an exit-rate dictionary, and
-//    each row block carries its first entry. 4 bytes per entry and 2 per row.
-//  * plain: u32 row pointers and columns, rates and exit rates in VT.
+//  * coded sliced ELL (used whenever it fits): the rates of a Markov generator take few
+//    distinct values, so an entry is one u32 (column in the low 24 bits, index into a rate
+//    dictionary in the high 8 bits) and a row is a u8 length plus a u8 index into an
+//    exit-rate dictionary. Rows are cut into slices of 64 (one wavefront); a slice stores its
+//    entries column-major, entry k of row (64 s + l) at soff[s] + 64 k + l, padded to the
+//    slice's longest row. Lane l owns one row: the entry loads are coalesced, and entry k of
+//    64 neighbouring states is usually the same kind of jump, so the x gathers of one load
+//    instruction fall into few cache lines.
+//  * plain CSR-stream: u32 row pointers and columns, rates and exit rates in VT; a workgroup
+//    streams a row block's entries, stages products in LDS, and each thread sums one row.
 // Workgroups are mapped so that each XCD (8 per MI355X, dispatched round-robin) processes a
-// contiguous range of row blocks, which keeps neighbouring rows' gathers in one L2.
+// contiguous range of rows, which keeps neighbouring rows' gathers in one L2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,8 +31,9 @@ using u32 = std::uint32_t;
 using u8 = std::uint8_t;
 
 constexpr int kThreads = 256;      // workgroup size (4 wave64)
-constexpr int kRowsPerBlock = 256; // rows per row block (one per thread)
-constexpr int kTile = 4096;        // LDS product tile (entries) per row block
+constexpr int kSlice = 64;         // rows per slice of the coded layout (one wavefront)
+constexpr int kRowsPerBlock = 256; // rows per row block of the plain layout (one per thread)
+constexpr int kTile = 4096;        // LDS product tile (entries) per plain row block
 constexpr int kDotChunk = 8;       // basis vectors per multi-dot pass
 constexpr int kDictMax = 256;
 constexpr u32 kColBits = 24;
@@ -39,18 +43,19 @@ constexpr u32 kColMask = (1u << kColBits) - 1u;
 template <typename VT>
 struct csr_view {
     i64 n = 0;
-    i64 n_blocks = 0;
-    const u32 *blocks = nullptr; // n_blocks + 1 row boundaries
+    i64 n_blocks = 0;            // workgroups per pass (row blocks or groups of 4 slices)
     // plain encoding
+    const u32 *blocks = nullptr; // n_blocks + 1 row boundaries
     const u32 *ptr = nullptr;    // n + 1
     const u32 *col = nullptr;
     const VT *val = nullptr;
     const VT *exit = nullptr;    // n
-    // coded encoding (active when coded != 0; ent may be null when the operator has no
-    // entries)
+    // coded sliced encoding (active when coded != 0; ent may be null when the operator has
+    // no entries)
     int coded = 0;
-    const u32 *bptr = nullptr;   // n_blocks + 1: first entry of each row block
-    const u32 *ent = nullptr;    // column | rate code << 24
+    i64 n_slices = 0;
+    const u32 *soff = nullptr;   // n_slices + 1: first entry of each slice
+    const u32 *ent = nullptr;    // column | rate code << 24, column-major per slice
     const u8 *rlen = nullptr;    // row lengths
     const u8 *rcode = nullptr;   // exit-rate codes
     const VT *vdict = nullptr;
@@ -61,7 +66,7 @@ struct csr_view {
 
 // y = (shift + exit) o x - C x             (FP64 accumulate; x in XT)
 // with b: y = b - ((shift + exit) o x - C x) and, if partial != null, one partial sum of y^2
-// per row block in partial[block]
+// per workgroup in partial[0 .. n_blocks)
 template <typename XT>
 void shifted_apply(const csr_view<double> &A, double shift, const XT *x, const double *b,
                    double *y, double *partial, hipStream_t s);
@@ -79,6 +84,10 @@ void jacobi_sweep(const csr_view<T> &A, double shift, double omega, const T *rhs
 template <typename T>
 void level_residual(const csr_view<T> &A, double shift, const T *rhs, const T *x, T *r,
                     hipStream_t s);
+// y[r] = sum over row r's entries of rate * x[column] (restriction: rates 1, rows = coarse
+// states, columns = their fine children in ascending order)
+template <typename T>
+void gather_sum(const csr_view<T> &R, const T *x, T *y, hipStream_t s);
 // restriction by coarse rows: coarse[c] = sum of fine[children of c] in ascending order
 template <typename T>
 void restrict_children(i64 n_coarse, const u32 *cptr, const u32 *cidx, const T *fine, T *coarse,
@@ -98,6 +107,8 @@ template <typename A, typename B>
 void convert(i64 n, const A *in, B *out, hipStream_t s);
 template <typename T>
 void axpy(i64 n, T alpha, const T *x, T *y, hipStream_t s); // y += alpha x
+// plain copy of n 16-byte words (the byte-counter calibration kernel)
+void copy16(i64 n, const void *src, void *dst, hipStream_t s);
 
 // ---- Krylov vector kernels (FP64) ----
 int reduction_blocks(i64 n);
@@ -111,6 +122,22 @@ void basis_update(double *w, const double *V, i64 ld, int count, const double *h
 void reduce_partials(const double *partial, int nb, int count, double *out, hipStream_t s);
 // v = w / sqrt(norm2[0])
 void scale_by_norm(i64 n, const double *w, const double *norm2, double *v, hipStream_t s);
+// Fused classical Gram-Schmidt, twice (CGS2), for count <= kFusedBasis basis vectors. Waves
+// split the basis (k = 4 kk + wave), lanes own the rows of 64-row tiles, so every V load is
+// one coalesced 512-byte row segment and V values are reused from registers.
+constexpr int kFusedBasis = 64;
+//   pass 1 projections: partial[k * nb + block] = sum_rows V_k w
+void dot_tiled(const double *V, i64 ld, int count, const double *w, i64 n, double *partial,
+               int nb, hipStream_t s);
+//   w <- w - V h, then the pass-2 projections of the new w and its squared norm:
+//   partial[k * nb + block] = sum_rows V_k w (k < count), partial[count * nb + block] = sum w^2
+void update_dot(double *w, const double *V, i64 ld, int count, const double *h, i64 n,
+                double *partial, int nb, hipStream_t s);
+//   v = (w - V c) / beta with beta^2 = cw[count] - sum_k c_k^2 (c = cw[0..count)). When the
+//   second pass removes more than a quarter of |w|^2 the norm is not trusted: beta = 1 and
+//   flag = 1, and the caller normalises explicitly. beta_flag[0] = beta, beta_flag[1] = flag.
+void update_scale(const double *w, const double *V, i64 ld, int count, const double *cw, i64 n,
+                  double *v, double *beta_flag, hipStream_t s);
 // x += sum_k y[k] Z_k (Z stored in ZT)
 template <typename ZT>
 void combine(double *x, const ZT *Z, i64 ld, int count, const double *y, i64 n, hipStream_t s);

@@ -668,4 +668,99 @@ sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::v
     return h;
 }
 
+// ------------------------------------------------------------------ coded sliced layout
+namespace {
+
+// the distinct values of v converted to VT, sorted; empty when there are more than 256
+template <typename VT>
+std::vector<VT> dictionary(const std::vector<double> &v) {
+    const i64 n = static_cast<i64>(v.size());
+    const i64 chunk = 1 << 20;
+    const i64 tasks = (n + chunk - 1) / chunk;
+    std::vector<std::vector<VT>> parts(tasks);
+    std::vector<char> over(tasks, 0);
+    run_tasks(tasks, [&](i64 t) {
+        auto &d = parts[t];
+        for (i64 i = t * chunk; i < std::min(n, (t + 1) * chunk); ++i) {
+            const VT x = static_cast<VT>(v[i]);
+            auto it = std::lower_bound(d.begin(), d.end(), x);
+            if (it == d.end() || *it != x) {
+                if (d.size() >= 256) {
+                    over[t] = 1;
+                    return;
+                }
+                d.insert(it, x);
+            }
+        }
+    });
+    std::vector<VT> all;
+    for (i64 t = 0; t < tasks; ++t) {
+        if (over[t]) return {};
+        for (VT x : parts[t]) {
+            auto it = std::lower_bound(all.begin(), all.end(), x);
+            if (it == all.end() || *it != x) all.insert(it, x);
+        }
+        if (all.size() > 256) return {};
+    }
+    if (all.empty()) all.push_back(VT(0));
+    return all;
+}
+
+template <typename VT>
+u32 code_of(const std::vector<VT> &dict, double v) {
+    return static_cast<u32>(std::lower_bound(dict.begin(), dict.end(), static_cast<VT>(v)) -
+                            dict.begin());
+}
+
+} // namespace
+
+template <typename VT>
+bool encode_sliced(const generator_csr &g, sliced_operator<VT> &out) {
+    const i64 n = g.n;
+    for (i64 i = 0; i < n; ++i)
+        if (g.ptr[i + 1] - g.ptr[i] > 255) return false;
+    // every column (rows of a square operator, finer-level rows of a restriction) fits 24 bits
+    u32 max_col = 0;
+    for (u32 c : g.col) max_col = std::max(max_col, c);
+    if (!g.col.empty() && max_col >= (1u << kCodedColumnBits)) return false;
+    out.vdict = dictionary<VT>(g.val);
+    if (out.vdict.empty()) return false;
+    out.edict = dictionary<VT>(g.exit);
+    if (out.edict.empty()) return false;
+    const i64 n_slices = (n + kSliceRows - 1) / kSliceRows;
+    out.soff.assign(n_slices + 1, 0);
+    u64 total = 0;
+    for (i64 sl = 0; sl < n_slices; ++sl) {
+        u64 w = 0;
+        for (i64 r = sl * kSliceRows; r < std::min(n, (sl + 1) * kSliceRows); ++r)
+            w = std::max<u64>(w, g.ptr[r + 1] - g.ptr[r]);
+        total += w * kSliceRows;
+        if (total >= (u64{1} << 32)) return false;
+        out.soff[sl + 1] = static_cast<u32>(total);
+    }
+    out.padded = total;
+    out.ent.assign(total, 0u);
+    out.rlen.assign(n, 0);
+    out.rcode.assign(n, 0);
+    const i64 chunk = 1024; // slices per task
+    run_tasks((n_slices + chunk - 1) / chunk, [&](i64 t) {
+        for (i64 sl = t * chunk; sl < std::min(n_slices, (t + 1) * chunk); ++sl) {
+            for (i64 l = 0; l < kSliceRows; ++l) {
+                const i64 i = sl * kSliceRows + l;
+                if (i >= n) break;
+                out.rlen[i] = static_cast<std::uint8_t>(g.ptr[i + 1] - g.ptr[i]);
+                out.rcode[i] = static_cast<std::uint8_t>(code_of(out.edict, g.exit[i]));
+                u64 k = 0;
+                for (u64 p = g.ptr[i]; p < g.ptr[i + 1]; ++p, ++k)
+                    out.ent[out.soff[sl] + k * kSliceRows + l] =
+                        g.col[p] | (code_of(out.vdict, g.val[p]) << kCodedColumnBits);
+            }
+        }
+    });
+    return true;
+}
+
+template bool encode_sliced<double>(const generator_csr &, sliced_operator<double> &);
+template bool encode_sliced<float>(const generator_csr &, sliced_operator<float> &);
+
 } // namespace mk

@@ -171,6 +171,25 @@ sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::v
 // dense inverse by Gauss-Jordan with partial pivoting; false when singular
 bool invert_dense(std::vector<double> a, i64 n, std::vector<double> &inverse);
 
+// The coded sliced-ELL encoding of a gather-form operator (the device's primary layout):
+// entry k of row 64 s + l is ent[soff[s] + 64 k + l] = column | rate code << 24, with the
+// slice padded to its longest row; rlen / rcode hold each row's length and exit-rate code.
+constexpr int kSliceRows = 64;
+constexpr int kCodedColumnBits = 24;
+template <typename VT>
+struct sliced_operator {
+    std::vector<u32> soff;
+    std::vector<u32> ent;
+    std::vector<std::uint8_t> rlen, rcode;
+    std::vector<VT> vdict, edict;
+    u64 padded = 0;
+};
+
+// false when the operator does not fit the coded layout (a column at or above 2^24, more than 256
+// distinct rates or exit rates, rows longer than 255 entries, or 2^32 stored entries)
+template <typename VT>
+bool encode_sliced(const generator_csr &g, sliced_operator<VT> &out);
+
 unsigned worker_threads();
 
 // task(t) for t in [0, n_tasks) on the worker pool (dynamically scheduled)

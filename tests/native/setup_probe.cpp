@@ -71,6 +71,17 @@ int probe_box_states(i64 V, int S, const int *caps, int *states, i64 *begin, i64
     });
 }
 
+int probe_generator_level(int dim, const i64 *dims, const double *spacing, const double *vols,
+                          int S, const double *diff, const int *totals, void **out) {
+    return guard([&] {
+        auto dom = make_domain(dim, dims, spacing, 0, nullptr, nullptr, nullptr, vols, S, diff);
+        auto space = population_space::sector(dom.voxels, std::vector<int>(totals, totals + S));
+        auto r = std::make_unique<gen_result>();
+        r->g = assemble_generator(space, dom, reaction_tables{}, true);
+        *out = r.release();
+    });
+}
+
 int probe_generator(int dim, const i64 *dims, const double *spacing, i64 V, const i64 *off,
                     const i64 *to, const double *w, const double *vols, int S, const double *diff,
                     const int *caps, int R, const int *changes, const double *rates, int absorbing,
@@ -114,6 +125,43 @@ void probe_generator_arrays(void *h, i64 *ptr, i64 *col, double *val, double *ex
 }
 
 void probe_generator_free(void *h) { delete static_cast<gen_result *>(h); }
+
+// wrap a CSR given by the caller (rows may differ from the column range)
+void *probe_csr(i64 n, const i64 *ptr, const i64 *col, const double *val, const double *exit) {
+    auto r = std::make_unique<gen_result>();
+    r->g.n = n;
+    r->g.ptr.assign(ptr, ptr + n + 1);
+    r->g.col.assign(col, col + ptr[n]);
+    r->g.val.assign(val, val + ptr[n]);
+    r->g.exit.assign(exit, exit + n);
+    r->g.leak.assign(n, 0.0);
+    return r.release();
+}
+
+// the coded sliced layout of the generator (FP64 dictionaries); returns 0 when it does not fit
+int probe_sliced_sizes(void *h, i64 *n_slices, i64 *padded, i64 *n_vdict, i64 *n_edict) {
+    auto &g = static_cast<gen_result *>(h)->g;
+    sliced_operator<double> so;
+    if (!encode_sliced<double>(g, so)) return 0;
+    *n_slices = static_cast<i64>(so.soff.size()) - 1;
+    *padded = static_cast<i64>(so.padded);
+    *n_vdict = static_cast<i64>(so.vdict.size());
+    *n_edict = static_cast<i64>(so.edict.size());
+    return 1;
+}
+
+void probe_sliced_arrays(void *h, u32 *soff, u32 *ent, unsigned char *rlen, unsigned char *rcode,
+                         double *vdict, double *edict) {
+    auto &g = static_cast<gen_result *>(h)->g;
+    sliced_operator<double> so;
+    encode_sliced<double>(g, so);
+    std::memcpy(soff, so.soff.data(), so.soff.size() * sizeof(u32));
+    std::memcpy(ent, so.ent.data(), so.ent.size() * sizeof(u32));
+    std::memcpy(rlen, so.rlen.data(), so.rlen.size());
+    std::memcpy(rcode, so.rcode.data(), so.rcode.size());
+    std::memcpy(vdict, so.vdict.data(), so.vdict.size() * sizeof(double));
+    std::memcpy(edict, so.edict.data(), so.edict.size() * sizeof(double));
+}
 
 int probe_hierarchy(int dim, const i64 *dims, const double *spacing, const double *vols, int S,
                     const double *diff, const int *caps, double shift, int pre, int post,

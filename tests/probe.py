@@ -77,6 +77,69 @@ def box_states(V, caps):
     return states.reshape(n.value, -1), begin, found
 
 
+def sliced(model, caps, absorbing=True):
+    """the coded sliced layout of the generator, or None when it does not fit"""
+    caps = np.asarray(caps, np.int32)
+    keep, args = _domain_args(model)
+    changes = np.ascontiguousarray(model.changes(), np.int32)
+    rates = np.ascontiguousarray(model.rate_tables(caps), np.float64)
+    h = C.c_void_p()
+    _check(lib().probe_generator(*args, _p(caps), len(model.reactions), _p(changes), _p(rates),
+                                 1 if absorbing else 0, C.byref(h)))
+    from markovkit_amd.model import box_size
+    return _sliced_from(h, box_size(model.n_voxels(), list(caps)))
+
+
+def sliced_csr(ptr, col, val, exit_):
+    """the coded sliced layout of an arbitrary CSR (rows x any column range), or None"""
+    n = len(ptr) - 1
+    ptr = np.ascontiguousarray(ptr, np.int64)
+    col = np.ascontiguousarray(col, np.int64)
+    val = np.ascontiguousarray(val, np.float64)
+    exit_ = np.ascontiguousarray(exit_, np.float64)
+    lib().probe_csr.restype = C.c_void_p
+    h = C.c_void_p(lib().probe_csr(C.c_int64(n), _p(ptr), _p(col), _p(val), _p(exit_)))
+    return _sliced_from(h, n)
+
+
+def _sliced_from(h, rows):
+    ns, padded, nv, ne = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+    ok = lib().probe_sliced_sizes(h, C.byref(ns), C.byref(padded), C.byref(nv), C.byref(ne))
+    if not ok:
+        lib().probe_generator_free(h)
+        return None
+    soff = np.zeros(ns.value + 1, np.uint32)
+    ent = np.zeros(padded.value, np.uint32)
+    rlen = np.zeros(rows, np.uint8)
+    rcode = np.zeros(rows, np.uint8)
+    vdict = np.zeros(nv.value)
+    edict = np.zeros(ne.value)
+    lib().probe_sliced_arrays(h, _p(soff), _p(ent), _p(rlen), _p(rcode), _p(vdict), _p(edict))
+    lib().probe_generator_free(h)
+    return dict(soff=soff, ent=ent, rlen=rlen, rcode=rcode, vdict=vdict, edict=edict, n=rows)
+
+
+def sliced_apply(S, shift, x, gather_sum=False):
+    """(shift I - C) x evaluated with the device kernel's index arithmetic: slice s = row // 64,
+    lane l = row % 64, entry k at soff[s] + 64 k + l, column = low 24 bits, code = high 8;
+    gather_sum returns the plain row sums of rate * x[column] instead"""
+    n = S["n"]
+    y = np.zeros(n)
+    for r in range(n):
+        s, l = divmod(r, 64)
+        base = int(S["soff"][s]) + l
+        acc = 0.0
+        for k in range(int(S["rlen"][r])):
+            idx = base + 64 * k
+            assert idx < int(S["soff"][s + 1])
+            e = int(S["ent"][idx])
+            c, code = e & 0xFFFFFF, e >> 24
+            assert c < len(x) and code < len(S["vdict"])
+            acc += S["vdict"][code] * x[c]
+        y[r] = acc if gather_sum else (shift + S["edict"][S["rcode"][r]]) * x[r] - acc
+    return y
+
+
 def generator(model, caps, absorbing=True):
     """(ptr, col, val, exit, leak, n_jumps) of the gather-form generator."""
     caps = np.asarray(caps, np.int32)

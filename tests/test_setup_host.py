@@ -97,6 +97,52 @@ def test_hierarchy_matches_oracle(name, model, caps, shift):
                 np.testing.assert_allclose(y, Mo.level_apply(k, l, x), rtol=1e-13, atol=1e-13)
 
 
+@pytest.mark.parametrize("name,model,caps", cases(), ids=[c[0] for c in cases()])
+def test_sliced_layout_indexing(name, model, caps):
+    """the device kernel's sliced-layout index arithmetic, emulated on the host, reproduces the
+    oracle's shifted generator (every entry read lies inside its slice)"""
+    S = probe.sliced(model, caps)
+    assert S is not None
+    P = Problem(model, caps)
+    x = np.random.default_rng(1).standard_normal(P.n)
+    y = probe.sliced_apply(S, 0.7, x)
+    y_ref = P.apply_shifted(0.7, x)
+    A = P.matrix(0.7, -1.0)
+    A.data = np.abs(A.data)
+    assert np.all(np.abs(y - y_ref) <= 64 * np.finfo(float).eps * (A @ np.abs(x)))
+    # padding: every slice is as wide as its longest row
+    for s in range(len(S["soff"]) - 1):
+        rows = S["rlen"][64 * s: 64 * (s + 1)]
+        assert S["soff"][s + 1] - S["soff"][s] == 64 * int(rows.max())
+
+
+@pytest.mark.parametrize("name,model,caps", cases()[:4], ids=[c[0] for c in cases()[:4]])
+def test_restriction_as_sliced_operator(name, model, caps):
+    """restriction = the children of every coarse row summed in ascending order; its sliced
+    encoding (columns index the finer level) reproduces the oracle's transfer bit for bit"""
+    Mo = SectorMultigrid(Problem(model, caps), 1.0)
+    rows = Mo.level_rows()
+    for k in range(rows.shape[0]):
+        if rows[k, 1] == 0:
+            continue
+        co, _ = Mo.transfer(k, 0)
+        nc = rows[k, 1]
+        order = np.argsort(co, kind="stable")
+        ptr = np.concatenate([[0], np.cumsum(np.bincount(co, minlength=nc))])
+        S = probe.sliced_csr(ptr, order, np.ones(len(order)), np.zeros(nc))
+        if S is None:  # a coarse row with more than 255 children keeps the plain layout
+            assert np.diff(ptr).max() > 255
+            continue
+        fine = np.random.default_rng(k).standard_normal(len(co))
+        y = probe.sliced_apply(S, 0.0, fine, gather_sum=True)
+        ref = np.zeros(nc)
+        for i in range(len(co)):
+            ref[co[i]] += fine[i]
+        np.testing.assert_array_equal(y, ref)
+    # a column range beyond 24 bits does not fit the coded layout
+    assert probe.sliced_csr(np.array([0, 1]), np.array([1 << 24]), np.ones(1), np.zeros(1)) is None
+
+
 def test_multigrid_errors():
     model = config_small().model
     with pytest.raises(ValueError):

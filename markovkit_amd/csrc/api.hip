@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -114,6 +115,8 @@ hipStream_t stream() {
     static hipStream_t s = [] {
         hipStream_t t = nullptr;
         HIPCHK(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+        // tuning override of the operator kernels' XCD interleave chunk (0 = identity mapping)
+        if (const char *env = std::getenv("MK_XCD_CHUNK")) gpu::set_xcd_chunk(std::atoi(env));
         return t;
     }();
     return s;
@@ -692,7 +695,7 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
                 // pass 1 projections; update fused with the pass-2 projections and |w|^2;
                 // pass-2 update fused with the normalisation
                 P.run(K_DOT, double(cnt + 1) * n * 8.0, [&] {
-                    gpu::dot_tiled(V, n, cnt, w, n, partial, nb, s);
+                    gpu::multi_dot(V, n, cnt, w, n, partial, nb, s);
                     gpu::reduce_partials(partial, nb, cnt, h1d, s);
                 });
                 P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {

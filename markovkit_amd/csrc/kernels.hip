@@ -35,11 +35,20 @@ __device__ __forceinline__ T block_sum(T v, T *lds) {
     return r;
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin; give XCD j a contiguous range of
-// row blocks so that neighbouring rows share its L2.
-__device__ __forceinline__ u32 xcd_block(u32 bid, u32 nb) {
-    const u32 xcd = bid & 7u, slot = bid >> 3, q = nb >> 3, r = nb & 7u;
-    return xcd * q + min(xcd, r) + slot;
+int g_xcd_chunk = 16;
+
+// logical row block of workgroup bid: chunk > 0 interleaves chunks of `chunk` blocks over the
+// XCDs (the launch grid is padded to a multiple of 8 * chunk; blocks past nb exit)
+__device__ __forceinline__ u32 xcd_block(u32 bid, u32 chunk) {
+    if (chunk == 0) return bid;
+    const u32 xcd = bid & 7u, slot = bid >> 3;
+    return ((slot / chunk) * 8u + xcd) * chunk + slot % chunk;
+}
+
+u32 padded_grid(i64 nb, u32 chunk) {
+    if (chunk == 0) return static_cast<u32>(nb);
+    const i64 unit = 8 * static_cast<i64>(chunk);
+    return static_cast<u32>((nb + unit - 1) / unit * unit);
 }
 
 enum class csr_mode { shifted, shifted_residual, sweep, residual, gather_sum };
@@ -53,6 +62,7 @@ struct csr_args {
     const T *rhs;    // b (shifted_residual) or rhs (sweep, residual)
     T *out;
     double *partial; // shifted_residual: sum of out^2 per row block
+    u32 chunk;       // XCD interleave chunk (set by the launcher)
 };
 
 template <csr_mode M, typename VT, typename XT, typename T>
@@ -93,7 +103,8 @@ __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) 
     __shared__ T edict[kDictMax];
     __shared__ double red64[kWaves];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const u32 b = xcd_block(blockIdx.x, gridDim.x);
+    const u32 b = xcd_block(blockIdx.x, a.chunk);
+    if (b >= a.A.n_blocks) return;
     for (int i = tid; i < a.A.n_vdict; i += kThreads) vdict[i] = static_cast<T>(a.A.vdict[i]);
     for (int i = tid; i < a.A.n_edict; i += kThreads) edict[i] = static_cast<T>(a.A.edict[i]);
     const i64 slice = static_cast<i64>(b) * kWaves + wave;
@@ -135,7 +146,8 @@ __global__ void __launch_bounds__(kThreads) k_csr_plain(csr_args<VT, XT, T> a) {
     __shared__ T prod[kTile];
     __shared__ T red[kWaves];
     __shared__ double red64[kWaves];
-    const u32 b = xcd_block(blockIdx.x, gridDim.x);
+    const u32 b = xcd_block(blockIdx.x, a.chunk);
+    if (b >= a.A.n_blocks) return;
     const u32 r0 = a.A.blocks[b], r1 = a.A.blocks[b + 1];
     const u32 p0 = a.A.ptr[r0], p1 = a.A.ptr[r1];
     const int tid = threadIdx.x;
@@ -182,12 +194,14 @@ __global__ void __launch_bounds__(kThreads) k_csr_plain(csr_args<VT, XT, T> a) {
 }
 
 template <csr_mode M, typename VT, typename XT, typename T>
-void launch_csr(const csr_args<VT, XT, T> &a, hipStream_t s) {
+void launch_csr(csr_args<VT, XT, T> a, hipStream_t s) {
     if (a.A.n_blocks == 0) return;
+    a.chunk = static_cast<u32>(g_xcd_chunk);
+    const dim3 grid(padded_grid(a.A.n_blocks, a.chunk));
     if (a.A.coded) {
-        hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T>), dim3(a.A.n_blocks), dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T>), grid, dim3(kThreads), 0, s, a);
     } else {
-        hipLaunchKernelGGL((k_csr_plain<M, VT, XT, T>), dim3(a.A.n_blocks), dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((k_csr_plain<M, VT, XT, T>), grid, dim3(kThreads), 0, s, a);
     }
 }
 
@@ -357,35 +371,6 @@ constexpr int kTileRows = 64;
 constexpr int kPerWave = kFusedBasis / kWaves; // basis vectors per wave
 
 __global__ void __launch_bounds__(kThreads)
-    k_dot_tiled(const double *V, i64 ld, int count, const double *w, i64 n, double *partial,
-                int nb) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double acc[kPerWave];
-#pragma unroll
-    for (int kk = 0; kk < kPerWave; ++kk) acc[kk] = 0.0;
-    for (i64 r0 = static_cast<i64>(blockIdx.x) * kTileRows; r0 < n;
-         r0 += static_cast<i64>(nb) * kTileRows) {
-        const i64 r = r0 + lane;
-        if (r < n) {
-            const double wr = w[r];
-#pragma unroll
-            for (int kk = 0; kk < kPerWave; ++kk) {
-                const int k = kk * kWaves + wave;
-                if (k < count) acc[kk] += V[k * ld + r] * wr;
-            }
-        }
-    }
-#pragma unroll
-    for (int kk = 0; kk < kPerWave; ++kk) {
-        const int k = kk * kWaves + wave;
-        if (k < count) {
-            const double t = wave_sum(acc[kk]);
-            if (lane == 0) partial[static_cast<i64>(k) * nb + blockIdx.x] = t;
-        }
-    }
-}
-
-__global__ void __launch_bounds__(kThreads)
     k_update_dot(double *w, const double *V, i64 ld, int count, const double *h, i64 n,
                  double *partial, int nb) {
     __shared__ double part[kWaves][kTileRows];
@@ -474,6 +459,9 @@ int grid_for(i64 n) {
 
 } // namespace
 
+void set_xcd_chunk(int chunk) { g_xcd_chunk = chunk < 0 ? 0 : chunk; }
+int xcd_chunk() { return g_xcd_chunk; }
+
 int reduction_blocks(i64 n) {
     const i64 g = (n + 4 * kThreads - 1) / (4 * kThreads);
     return static_cast<int>(std::max<i64>(1, std::min<i64>(g, 2048)));
@@ -482,7 +470,7 @@ int reduction_blocks(i64 n) {
 template <typename XT>
 void shifted_apply(const csr_view<double> &A, double shift, const XT *x, const double *b,
                    double *y, double *partial, hipStream_t s) {
-    const csr_args<double, XT, double> a{A, shift, 0.0, x, b, y, partial};
+    const csr_args<double, XT, double> a{A, shift, 0.0, x, b, y, partial, 0u};
     if (b) {
         launch_csr<csr_mode::shifted_residual>(a, s);
     } else {
@@ -502,18 +490,18 @@ template <typename T>
 void jacobi_sweep(const csr_view<T> &A, double shift, double omega, const T *rhs, const T *x,
                   T *x_out, hipStream_t s) {
     launch_csr<csr_mode::sweep>(csr_args<T, T, T>{A, shift, static_cast<T>(omega), x, rhs, x_out,
-                                                  nullptr}, s);
+                                                  nullptr, 0u}, s);
 }
 
 template <typename T>
 void level_residual(const csr_view<T> &A, double shift, const T *rhs, const T *x, T *r,
                     hipStream_t s) {
-    launch_csr<csr_mode::residual>(csr_args<T, T, T>{A, shift, T(0), x, rhs, r, nullptr}, s);
+    launch_csr<csr_mode::residual>(csr_args<T, T, T>{A, shift, T(0), x, rhs, r, nullptr, 0u}, s);
 }
 
 template <typename T>
 void gather_sum(const csr_view<T> &R, const T *x, T *y, hipStream_t s) {
-    launch_csr<csr_mode::gather_sum>(csr_args<T, T, T>{R, 0.0, T(0), x, nullptr, y, nullptr}, s);
+    launch_csr<csr_mode::gather_sum>(csr_args<T, T, T>{R, 0.0, T(0), x, nullptr, y, nullptr, 0u}, s);
 }
 
 template <typename T>
@@ -589,12 +577,6 @@ void scale_by_norm(i64 n, const double *w, const double *norm2, double *v, hipSt
     hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(kThreads), 0, s, n, w, norm2, v);
 }
 
-void dot_tiled(const double *V, i64 ld, int count, const double *w, i64 n, double *partial,
-               int nb, hipStream_t s) {
-    hipLaunchKernelGGL(k_dot_tiled, dim3(nb), dim3(kThreads), 0, s, V, ld, count, w, n, partial,
-                       nb);
-}
-
 void update_dot(double *w, const double *V, i64 ld, int count, const double *h, i64 n,
                 double *partial, int nb, hipStream_t s) {
     hipLaunchKernelGGL(k_update_dot, dim3(nb), dim3(kThreads), 0, s, w, V, ld, count, h, n,
@@ -603,8 +585,9 @@ void update_dot(double *w, const double *V, i64 ld, int count, const double *h, 
 
 void update_scale(const double *w, const double *V, i64 ld, int count, const double *cw, i64 n,
                   double *v, double *beta_flag, hipStream_t s) {
-    hipLaunchKernelGGL(k_update_scale, dim3(grid_for(n)), dim3(kThreads), 0, s, w, V, ld, count,
-                       cw, n, v, beta_flag);
+    const i64 g = std::max<i64>(1, (n + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_update_scale, dim3(static_cast<u32>(std::min<i64>(g, 1 << 30))), dim3(kThreads), 0, s,
+                       w, V, ld, count, cw, n, v, beta_flag);
 }
 
 template <typename ZT>

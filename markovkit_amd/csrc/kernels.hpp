@@ -39,6 +39,13 @@ constexpr int kDictMax = 256;
 constexpr u32 kColBits = 24;
 constexpr u32 kColMask = (1u << kColBits) - 1u;
 
+// Workgroup-to-XCD mapping of the operator kernels. The hardware dispatches workgroup b to
+// XCD b % 8. With chunk C > 0 the grid is padded to a multiple of 8 C and XCD j processes the
+// logical chunks j, j + 8, j + 16, ... of C consecutive row blocks (neighbouring rows share an
+// L2, while costly row ranges still spread over all XCDs); C = 0 keeps block b on row block b.
+void set_xcd_chunk(int chunk);
+int xcd_chunk();
+
 // device view of one operator (shift I - C) without its shift
 template <typename VT>
 struct csr_view {
@@ -122,13 +129,11 @@ void basis_update(double *w, const double *V, i64 ld, int count, const double *h
 void reduce_partials(const double *partial, int nb, int count, double *out, hipStream_t s);
 // v = w / sqrt(norm2[0])
 void scale_by_norm(i64 n, const double *w, const double *norm2, double *v, hipStream_t s);
-// Fused classical Gram-Schmidt, twice (CGS2), for count <= kFusedBasis basis vectors. Waves
-// split the basis (k = 4 kk + wave), lanes own the rows of 64-row tiles, so every V load is
-// one coalesced 512-byte row segment and V values are reused from registers.
+// Fused classical Gram-Schmidt, twice (CGS2), for count <= kFusedBasis basis vectors: pass 1
+// projections by multi_dot; then the update fused with the pass-2 projections (waves split the
+// basis, k = 4 kk + wave, lanes own the rows of 64-row tiles, V values reused from registers);
+// then the pass-2 update fused with the normalisation.
 constexpr int kFusedBasis = 64;
-//   pass 1 projections: partial[k * nb + block] = sum_rows V_k w
-void dot_tiled(const double *V, i64 ld, int count, const double *w, i64 n, double *partial,
-               int nb, hipStream_t s);
 //   w <- w - V h, then the pass-2 projections of the new w and its squared norm:
 //   partial[k * nb + block] = sum_rows V_k w (k < count), partial[count * nb + block] = sum w^2
 void update_dot(double *w, const double *V, i64 ld, int count, const double *h, i64 n,

@@ -237,6 +237,9 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "kernel_ms_per_solve": {k: v["ms"] for k, v in prof_all.items() if v["launches"]},
+        # achieved algorithmic GB/s of every kernel class over the fully profiled solve
+        "kernel_gbs": {k: v["bytes"] / v["ms"] / 1e6 for k, v in prof_all.items()
+                       if v["launches"] and v["bytes"] > 0},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

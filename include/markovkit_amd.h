@@ -59,12 +59,19 @@ typedef struct mk_multigrid_options {
     int32_t precision;      /* MK_FP64 or MK_FP32 (device-only extension) */
 } mk_multigrid_options;
 
+/* Arnoldi orthogonalisation of the device solver (extension; the reference's field set ends at
+ * restart). MK_DCGS2: classical Gram-Schmidt twice with the second projection delayed into the
+ * next step, one global reduction per step; used for preconditioned solves with restart <= 64,
+ * otherwise the solver falls back to MK_CGS2 (two projections per step). */
+enum { MK_DCGS2 = 0, MK_CGS2 = 1 };
+
 /* num::gmres_options (numerics, call site tests/test_rdme.cpp:238-240); tolerance is an
  * absolute Euclidean residual norm, x on input is the initial guess */
 typedef struct mk_gmres_options {
     double tolerance;
     int64_t max_iterations;
     int64_t restart;
+    int32_t orthogonalization; /* MK_DCGS2 (0, default) or MK_CGS2 */
 } mk_gmres_options;
 
 /* num::solver_result (examples/rdme_multigrid_benchmark.cpp:55); residual is the final true

@@ -134,6 +134,8 @@ struct gmres_options {
     real tolerance = 1e-10;
     idx max_iterations = 1000;
     idx restart = 30;
+    // extension: Arnoldi orthogonalisation of the device solver (MK_DCGS2 or MK_CGS2)
+    int orthogonalization = MK_DCGS2;
 };
 
 struct solver_result {
@@ -506,7 +508,7 @@ inline solver_result run_gmres(const markovkit::shifted_generator &A, mk_sector_
     if (b.size() != A.rows() || x.size() != A.rows())
         throw std::invalid_argument("gmres: vector sizes do not match the operator");
     mk_gmres_options o{options.tolerance, static_cast<int64_t>(options.max_iterations),
-                       static_cast<int64_t>(options.restart)};
+                       static_cast<int64_t>(options.restart), options.orthogonalization};
     mk_solver_result r{};
     markovkit::detail::check(
         mk_gmres(A.base().native(), A.shift(), M, b.data(), x.data(), &o, &r));

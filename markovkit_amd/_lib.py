@@ -29,7 +29,8 @@ class MultigridOptions(C.Structure):
 
 
 class GmresOptions(C.Structure):
-    _fields_ = [("tolerance", f64), ("max_iterations", i64), ("restart", i64)]
+    _fields_ = [("tolerance", f64), ("max_iterations", i64), ("restart", i64),
+                ("orthogonalization", i32)]
 
 
 class SolverResult(C.Structure):

@@ -349,11 +349,16 @@ class SolverResult:
     converged: bool
 
 
+ORTHOGONALIZATION = {"dcgs2": 0, "cgs2": 1}
+
+
 def gmres(A: ShiftedGenerator, *args, tolerance: float = 1e-10, max_iterations: int = 1000,
-          restart: int = 30) -> SolverResult:
+          restart: int = 30, orthogonalization: str = "dcgs2") -> SolverResult:
     """num::gmres(A, [M,] b, x, options): x (numpy array or DeviceVector) is the initial
     guess on input and the solution on output; the tolerance is an absolute Euclidean
-    residual norm; non-convergence is reported, not raised."""
+    residual norm; non-convergence is reported, not raised. orthogonalization "dcgs2"
+    (delayed CGS2, one reduction per step; preconditioned solves with restart <= 64) or
+    "cgs2"."""
     if len(args) == 3:
         M, b, x = args
     elif len(args) == 2:
@@ -361,7 +366,10 @@ def gmres(A: ShiftedGenerator, *args, tolerance: float = 1e-10, max_iterations: 
         b, x = args
     else:
         raise TypeError("gmres(A, [M,] b, x, ...)")
-    opts = _GmresOptions(float(tolerance), int(max_iterations), int(restart))
+    if orthogonalization not in ORTHOGONALIZATION:
+        raise ValueError(f"orthogonalization must be one of {sorted(ORTHOGONALIZATION)}")
+    opts = _GmresOptions(float(tolerance), int(max_iterations), int(restart),
+                         ORTHOGONALIZATION[orthogonalization])
     res = _SolverResult()
     mh = M._h if M is not None else None
     if isinstance(b, DeviceVector):

@@ -478,12 +478,12 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
 }
 
 // one cycle of level l from a zero guess (population_multigrid::cycle, multigrid.hpp:201-228)
-// on all active sectors at once. rhs is this level's right-hand side; rhs64 (level 0 of the
-// FP32 hierarchy only) is the FP64 input that the first sweep converts into lv.rhs. The result
-// is written to out when given, else left in a level buffer whose pointer is returned.
+// on all active sectors at once. rhs is this level's right-hand side; rhs64 (level 0 only) is an
+// FP64 input that the first sweep multiplies by scale and stores into lv.rhs in precision T. The
+// result is written to out when given, else left in a level buffer whose pointer is returned.
 template <typename T>
 T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t l, const T *rhs,
-             const double *rhs64, T *out) {
+             const double *rhs64, T *out, double scale = 1.0) {
     auto &lv = L[l];
     auto &P = prof();
     const hipStream_t s = stream();
@@ -497,10 +497,11 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
         T *rb = dp<T>(lv.rhs);
         if (lv.pre > 0) {
             P.run(K_FIRST, double(n) * (8 + 3 * ts), [&] {
-                gpu::jacobi_first<T, double>(lv.A.view, shift, omega, rhs64, rb, cur, s);
+                gpu::jacobi_first<T, double>(lv.A.view, shift, omega, rhs64, rb, cur, s, scale);
             });
         } else {
-            P.run(K_OTHER, double(n) * (8 + ts), [&] { gpu::convert<double, T>(n, rhs64, rb, s); });
+            P.run(K_OTHER, double(n) * (8 + ts),
+                  [&] { gpu::convert<double, T>(n, rhs64, rb, s, scale); });
         }
         rhs = rb;
     } else if (lv.pre > 0) {
@@ -577,12 +578,14 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
     return cur;
 }
 
-// z = M b on the device: b FP64, z in the hierarchy's precision
-void mg_apply_device(mk_sector_mg_s &mg, const double *b, void *z) {
+// z = M (scale b) on the device: b FP64, z in the hierarchy's precision
+void mg_apply_device(mk_sector_mg_s &mg, const double *b, void *z, double scale = 1.0) {
     if (mg.precision == MK_FP32) {
-        run_level<float>(mg, mg.L32, 0, nullptr, b, static_cast<float *>(z));
-    } else {
+        run_level<float>(mg, mg.L32, 0, nullptr, b, static_cast<float *>(z), scale);
+    } else if (scale == 1.0) {
         run_level<double>(mg, mg.L64, 0, b, nullptr, static_cast<double *>(z));
+    } else {
+        run_level<double>(mg, mg.L64, 0, nullptr, b, static_cast<double *>(z), scale);
     }
 }
 
@@ -591,7 +594,7 @@ struct gmres_workspace {
     i64 n = 0;
     i64 m = 0;
     std::size_t zs = 0;
-    dbuf V, Z, w, r, partial, hd, yd;
+    dbuf V, Z, w, partial, hd, yd;
     // device scalars hd and their pinned host copy: pass-1 projections at 0, pass-2
     // projections and |w|^2 at m + 2, (beta, flag) at 2 (m + 2); y follows at 3 (m + 2)
     double *host = nullptr;
@@ -608,7 +611,6 @@ struct gmres_workspace {
         V = dbuf(static_cast<std::size_t>(n) * (m + 1) * sizeof(double));
         if (zs) Z = dbuf(static_cast<std::size_t>(n) * m * zs);
         w = dbuf(n * sizeof(double));
-        r = dbuf(n * sizeof(double));
         const int nb = gpu::reduction_blocks(n);
         partial = dbuf(static_cast<std::size_t>(nb) * (m + 2) * sizeof(double));
         hd = dbuf(3 * (m + 2) * sizeof(double));
@@ -633,30 +635,99 @@ double device_norm2(const double *partial, int nb, double *hd) {
     return v;
 }
 
+// Givens-rotated least-squares problem of one restart cycle (column-major upper Hessenberg H of
+// size (m + 1) x m, stored row-major with leading dimension m)
+struct hessenberg_ls {
+    i64 m;
+    std::vector<double> H, cs, sn, g, y;
+    explicit hessenberg_ls(i64 m_) : m(m_), H((m_ + 1) * m_), cs(m_), sn(m_), g(m_ + 1), y(m_) {}
+    void start(double beta) {
+        std::fill(H.begin(), H.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+    }
+    // column j = (col[0 .. j], hn); returns |g[j + 1]|, the residual norm of the cycle so far
+    double add_column(i64 j, const double *col, double hn) {
+        for (i64 k = 0; k <= j; ++k) H[k * m + j] = col[k];
+        H[(j + 1) * m + j] = hn;
+        for (i64 k = 0; k < j; ++k) {
+            const double a = H[k * m + j], c = H[(k + 1) * m + j];
+            H[k * m + j] = cs[k] * a + sn[k] * c;
+            H[(k + 1) * m + j] = -sn[k] * a + cs[k] * c;
+        }
+        const double a = H[j * m + j], c = H[(j + 1) * m + j];
+        const double d = std::hypot(a, c);
+        cs[j] = d > 0.0 ? a / d : 1.0;
+        sn[j] = d > 0.0 ? c / d : 0.0;
+        H[j * m + j] = d;
+        H[(j + 1) * m + j] = 0.0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        return std::abs(g[j + 1]);
+    }
+    // y = R^{-1} g over the first j columns
+    void solve(i64 j) {
+        for (i64 i = j; i-- > 0;) {
+            double sum = g[i];
+            for (i64 k = i + 1; k < j; ++k) sum -= H[i * m + k] * y[k];
+            y[i] = sum / H[i * m + i];
+        }
+    }
+};
+
+// Flexible GMRES(m), right-preconditioned: x = x0 + Z y with z_j = M v_j (the numerics gmres
+// with a preconditioner, called at tests/test_rdme.cpp:242-243). The Arnoldi basis is
+// orthogonalised by CGS2 or, for preconditioned solves with m <= kFusedBasis, by delayed CGS2:
+// step j applies M and A to the candidate u_j = Q s_j + beta_j q_j (orthogonalised once, scaled
+// by 1 / beta_j), and one reduction gives Q^T w and u_j^T w; the update kernel then finalises
+// q_j and forms the next candidate with its projections. Because w_j = A z_j is expanded as
+// Q (t + s_{j+1}) + beta_{j+1} q_{j+1}, column j of H is (t + s_{j+1}, beta_{j+1}) as in CGS2.
 template <typename ZT>
 mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, const double *b,
                         double *x, const mk_gmres_options &o) {
     const i64 n = A.n;
     const i64 m = std::max<i64>(1, std::min<i64>(o.restart, n));
     require(m + 1 <= 512, "gmres: restart above 511 is not supported on the device");
+    require(o.orthogonalization == MK_DCGS2 || o.orthogonalization == MK_CGS2,
+            "gmres: unknown orthogonalization");
+    const bool delayed = M && o.orthogonalization == MK_DCGS2 && m <= gpu::kFusedBasis;
     auto &ws = workspace();
     ws.ensure(n, m, M ? sizeof(ZT) : 0);
     auto &P = prof();
     const hipStream_t s = stream();
     const int nb = gpu::reduction_blocks(n);
-    double *V = dp<double>(ws.V), *w = dp<double>(ws.w), *r = dp<double>(ws.r);
+    double *V = dp<double>(ws.V), *w = dp<double>(ws.w);
     double *partial = dp<double>(ws.partial), *hd = dp<double>(ws.hd);
     ZT *Z = M ? dp<ZT>(ws.Z) : nullptr;
     const double spmv_bytes = op_bytes(A.A, 8, 8, 1);
+    const i64 hs = m + 2; // stride of the device scalar slots in hd
 
     mk_solver_result res{0, 0.0, 0};
-    // r = b - A x and ||r||^2 (left in hd[0] for the first basis vector)
+    // V_0 = r = b - A x and ||r||^2 (left in hd[0])
     const int rb = static_cast<int>(std::max<i64>(A.A.view.n_blocks, 1));
     auto true_residual = [&]() {
         P.run(K_SPMV, spmv_bytes + n * 8.0, [&] {
-            gpu::shifted_apply<double>(A.A.view, shift, x, b, r, dp<double>(A.d_rpart), s);
+            gpu::shifted_apply<double>(A.A.view, shift, x, b, V, dp<double>(A.d_rpart), s);
         });
         return std::sqrt(device_norm2(dp<double>(A.d_rpart), rb, hd));
+    };
+    // w = A M (scale v)
+    auto operator_image = [&](i64 j, const double *v, double scale) {
+        if (M) {
+            ZT *Zj = Z + j * n;
+            mg_apply_device(*M, v, Zj, scale);
+            P.run(K_SPMV, op_bytes(A.A, sizeof(ZT), 8, 1), [&] {
+                gpu::shifted_apply<ZT>(A.A.view, shift, Zj, nullptr, w, nullptr, s);
+            });
+        } else {
+            P.run(K_SPMV, spmv_bytes, [&] {
+                gpu::shifted_apply<double>(A.A.view, shift, v, nullptr, w, nullptr, s);
+            });
+        }
+    };
+    auto fetch = [&](std::size_t count) {
+        HIPCHK(hipMemcpyAsync(ws.host, hd, count * sizeof(double), hipMemcpyDeviceToHost, s));
+        drain();
     };
     double beta = true_residual();
     res.residual = beta;
@@ -664,114 +735,141 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
         res.converged = std::isfinite(beta) ? 1 : 0;
         return res;
     }
-    std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m);
+    hessenberg_ls ls(m);
+    std::vector<double> col(m + 1);
     while (res.iterations < o.max_iterations) {
-        // v_0 = r / ||r||, with ||r||^2 still in hd[0] from the true residual
-        P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, r, hd, V, s); });
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
+        ls.start(beta);
         i64 j = 0;
-        for (; j < m && res.iterations < o.max_iterations; ++j) {
-            ++res.iterations;
-            double *Vj = V + j * n;
-            // z_j = M v_j ; w = A z_j
-            if (M) {
-                ZT *Zj = Z + j * n;
-                mg_apply_device(*M, Vj, Zj);
-                P.run(K_SPMV, op_bytes(A.A, sizeof(ZT), 8, 1), [&] {
-                    gpu::shifted_apply<ZT>(A.A.view, shift, Zj, nullptr, w, nullptr, s);
+        if (delayed) {
+            // V_0 holds the candidate u_0 = r with s_0 empty and beta_0 = |r|
+            double bj = beta;
+            double *sd = hd + hs, *sd_next = hd + 2 * hs;
+            for (; j < m && res.iterations < o.max_iterations; ++j) {
+                ++res.iterations;
+                const int cnt = static_cast<int>(j);
+                operator_image(j, V + j * n, 1.0 / bj);
+                // t = Q^T w and u_j^T w in hd[0 .. j]
+                P.run(K_DOT, double(cnt + 2) * n * 8.0, [&] {
+                    gpu::multi_dot(V, n, cnt + 1, w, n, partial, nb, s);
+                    gpu::reduce_partials(partial, nb, cnt + 1, hd, s);
                 });
-            } else {
-                P.run(K_SPMV, spmv_bytes, [&] {
-                    gpu::shifted_apply<double>(A.A.view, shift, Vj, nullptr, w, nullptr, s);
+                // q_j, eta (hd[m + 1]), u_{j+1} and its projections s_{j+1}, |u_{j+1}|^2
+                P.run(K_UPDATE, double(cnt + 4) * n * 8.0, [&] {
+                    gpu::dcgs_update(V, n, cnt, sd, hd, bj, w, n, partial, nb, hd + m + 1, s);
+                    gpu::reduce_partials(partial, nb, cnt + 2, sd_next, s);
                 });
+                fetch(3 * hs);
+                const double *t = ws.host, *sn = ws.host + (sd_next - hd);
+                const double eta = ws.host[m + 1];
+                for (i64 k = 0; k < j; ++k) col[k] = t[k] + sn[k];
+                col[j] = eta + sn[j];
+                double ss = 0.0;
+                for (i64 k = 0; k <= j; ++k) ss += sn[k] * sn[k];
+                double alpha = sn[j + 1];
+                if (!(ss <= 0.25 * alpha)) {
+                    // the candidate kept too much of span(Q) for the Pythagorean norm: project
+                    // it once more explicitly and keep the remaining coefficients for the next
+                    // step's finalisation
+                    double *u = V + (j + 1) * n;
+                    P.run(K_UPDATE, double(cnt + 3) * n * 8.0, [&] {
+                        gpu::basis_update(u, V, n, cnt + 1, sd_next, n, nullptr, nb, s);
+                    });
+                    P.run(K_DOT, double(cnt + 3) * n * 8.0, [&] {
+                        gpu::multi_dot(V, n, cnt + 2, u, n, partial, nb, s);
+                        gpu::reduce_partials(partial, nb, cnt + 2, sd_next, s);
+                    });
+                    fetch(3 * hs);
+                    ss = 0.0;
+                    for (i64 k = 0; k <= j; ++k) {
+                        col[k] += sn[k];
+                        ss += sn[k] * sn[k];
+                    }
+                    alpha = sn[j + 1];
+                }
+                const double hn = std::sqrt(std::max(alpha - ss, 0.0));
+                std::swap(sd, sd_next);
+                bj = hn;
+                const double est = ls.add_column(j, col.data(), hn);
+                if (!(est > o.tolerance) || !(hn > 0.0) || !std::isfinite(hn)) {
+                    ++j;
+                    break;
+                }
             }
-            // classical Gram-Schmidt, twice (CGS2)
-            const int cnt = static_cast<int>(j + 1);
-            double *h1d = hd, *cwd = hd + (m + 2), *bfd = hd + 2 * (m + 2);
-            double *vnext = V + (j + 1) * n;
-            const bool fused = cnt <= gpu::kFusedBasis;
-            if (fused) {
-                // pass 1 projections; update fused with the pass-2 projections and |w|^2;
-                // pass-2 update fused with the normalisation
-                P.run(K_DOT, double(cnt + 1) * n * 8.0, [&] {
-                    gpu::multi_dot(V, n, cnt, w, n, partial, nb, s);
-                    gpu::reduce_partials(partial, nb, cnt, h1d, s);
-                });
-                P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {
-                    gpu::update_dot(w, V, n, cnt, h1d, n, partial, nb, s);
-                    gpu::reduce_partials(partial, nb, cnt + 1, cwd, s);
-                });
-                P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {
-                    gpu::update_scale(w, V, n, cnt, cwd, n, vnext, bfd, s);
-                });
-            } else {
-                for (int pass = 0; pass < 2; ++pass) {
-                    double *h = pass == 0 ? h1d : cwd;
+        } else {
+            // v_0 = r / ||r||, with ||r||^2 still in hd[0] from the true residual
+            P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, V, hd, V, s); });
+            for (; j < m && res.iterations < o.max_iterations; ++j) {
+                ++res.iterations;
+                double *Vj = V + j * n;
+                operator_image(j, Vj, 1.0);
+                // classical Gram-Schmidt, twice (CGS2)
+                const int cnt = static_cast<int>(j + 1);
+                double *h1d = hd, *cwd = hd + hs, *bfd = hd + 2 * hs;
+                double *vnext = V + (j + 1) * n;
+                const bool fused = cnt <= gpu::kFusedBasis;
+                if (fused) {
+                    // pass 1 projections; update fused with the pass-2 projections and |w|^2;
+                    // pass-2 update fused with the normalisation
                     P.run(K_DOT, double(cnt + 1) * n * 8.0, [&] {
                         gpu::multi_dot(V, n, cnt, w, n, partial, nb, s);
-                        gpu::reduce_partials(partial, nb, cnt, h, s);
+                        gpu::reduce_partials(partial, nb, cnt, h1d, s);
                     });
-                    double *np = pass == 1 ? partial + static_cast<i64>(nb) * (m + 1) : nullptr;
                     P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {
-                        gpu::basis_update(w, V, n, cnt, h, n, np, nb, s);
+                        gpu::update_dot(w, V, n, cnt, h1d, n, partial, nb, s);
+                        gpu::reduce_partials(partial, nb, cnt + 1, cwd, s);
                     });
-                }
-                P.run(K_DOT, nb * 8.0, [&] {
-                    gpu::reduce_partials(partial + static_cast<i64>(nb) * (m + 1), nb, 1, bfd + 2, s);
-                });
-                P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, w, bfd + 2, vnext, s); });
-            }
-            HIPCHK(hipMemcpyAsync(ws.host, hd, 3 * (m + 2) * sizeof(double),
-                                  hipMemcpyDeviceToHost, s));
-            drain();
-            const double *h1 = ws.host, *h2 = ws.host + (m + 2);
-            double hn;
-            if (fused) {
-                hn = ws.host[2 * (m + 2)];
-                if (ws.host[2 * (m + 2) + 1] != 0.0) {
-                    // the second pass removed too much for the Pythagorean norm: v_{j+1} holds
-                    // the unnormalised vector; normalise it explicitly
-                    P.run(K_DOT, n * 8.0, [&] {
-                        gpu::multi_dot(vnext, n, 1, vnext, n, partial, nb, s);
-                        gpu::reduce_partials(partial, nb, 1, bfd + 2, s);
+                    P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {
+                        gpu::update_scale(w, V, n, cnt, cwd, n, vnext, bfd, s);
                     });
-                    P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, vnext, bfd + 2, vnext, s); });
-                    HIPCHK(hipMemcpyAsync(ws.host + 2 * (m + 2) + 2, bfd + 2, sizeof(double),
-                                          hipMemcpyDeviceToHost, s));
-                    drain();
-                    hn = std::sqrt(ws.host[2 * (m + 2) + 2]);
+                } else {
+                    for (int pass = 0; pass < 2; ++pass) {
+                        double *h = pass == 0 ? h1d : cwd;
+                        P.run(K_DOT, double(cnt + 1) * n * 8.0, [&] {
+                            gpu::multi_dot(V, n, cnt, w, n, partial, nb, s);
+                            gpu::reduce_partials(partial, nb, cnt, h, s);
+                        });
+                        double *np = pass == 1 ? partial + static_cast<i64>(nb) * (m + 1) : nullptr;
+                        P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {
+                            gpu::basis_update(w, V, n, cnt, h, n, np, nb, s);
+                        });
+                    }
+                    P.run(K_DOT, nb * 8.0, [&] {
+                        gpu::reduce_partials(partial + static_cast<i64>(nb) * (m + 1), nb, 1,
+                                             bfd + 2, s);
+                    });
+                    P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, w, bfd + 2, vnext, s); });
                 }
-            } else {
-                hn = std::sqrt(ws.host[2 * (m + 2) + 2]);
-            }
-            for (i64 k = 0; k <= j; ++k) H[k * m + j] = h1[k] + h2[k];
-            H[(j + 1) * m + j] = hn;
-            for (i64 k = 0; k < j; ++k) {
-                const double a = H[k * m + j], c = H[(k + 1) * m + j];
-                H[k * m + j] = cs[k] * a + sn[k] * c;
-                H[(k + 1) * m + j] = -sn[k] * a + cs[k] * c;
-            }
-            const double a = H[j * m + j], c = H[(j + 1) * m + j];
-            const double d = std::hypot(a, c);
-            cs[j] = d > 0.0 ? a / d : 1.0;
-            sn[j] = d > 0.0 ? c / d : 0.0;
-            H[j * m + j] = d;
-            H[(j + 1) * m + j] = 0.0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
-            if (!(std::abs(g[j + 1]) > o.tolerance) || !(hn > 0.0) || !std::isfinite(hn)) {
-                ++j;
-                break;
+                fetch(3 * hs);
+                const double *h1 = ws.host, *h2 = ws.host + hs;
+                double hn;
+                if (fused) {
+                    hn = ws.host[2 * hs];
+                    if (ws.host[2 * hs + 1] != 0.0) {
+                        // the second pass removed too much for the Pythagorean norm: v_{j+1}
+                        // holds the unnormalised vector; normalise it explicitly
+                        P.run(K_DOT, n * 8.0, [&] {
+                            gpu::multi_dot(vnext, n, 1, vnext, n, partial, nb, s);
+                            gpu::reduce_partials(partial, nb, 1, bfd + 2, s);
+                        });
+                        P.run(K_OTHER, n * 16.0,
+                              [&] { gpu::scale_by_norm(n, vnext, bfd + 2, vnext, s); });
+                        fetch(3 * hs);
+                        hn = std::sqrt(ws.host[2 * hs + 2]);
+                    }
+                } else {
+                    hn = std::sqrt(ws.host[2 * hs + 2]);
+                }
+                for (i64 k = 0; k <= j; ++k) col[k] = h1[k] + h2[k];
+                const double est = ls.add_column(j, col.data(), hn);
+                if (!(est > o.tolerance) || !(hn > 0.0) || !std::isfinite(hn)) {
+                    ++j;
+                    break;
+                }
             }
         }
-        for (i64 i = j; i-- > 0;) {
-            double sum = g[i];
-            for (i64 k = i + 1; k < j; ++k) sum -= H[i * m + k] * y[k];
-            y[i] = sum / H[i * m + i];
-        }
-        double *ypin = ws.host + 3 * (m + 2);
-        std::copy(y.begin(), y.begin() + j, ypin);
+        ls.solve(j);
+        double *ypin = ws.host + 3 * hs;
+        std::copy(ls.y.begin(), ls.y.begin() + j, ypin);
         HIPCHK(hipMemcpyAsync(ws.yd.p, ypin, j * sizeof(double), hipMemcpyHostToDevice, s));
         P.run(K_OTHER, double(j) * n * (M ? sizeof(ZT) : 8) + n * 16.0, [&] {
             if (M) {

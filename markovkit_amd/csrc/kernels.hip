@@ -212,10 +212,11 @@ __device__ __forceinline__ T row_exit(const csr_view<T> &A, i64 r) {
 
 template <typename T, typename IN>
 __global__ void __launch_bounds__(kThreads)
-    k_jacobi_first(csr_view<T> A, T shift, T omega, const IN *rhs_in, T *rhs_out, T *x) {
+    k_jacobi_first(csr_view<T> A, T shift, T omega, const IN *rhs_in, IN scale, T *rhs_out,
+                   T *x) {
     for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < A.n;
          i += static_cast<i64>(gridDim.x) * kThreads) {
-        const T b = static_cast<T>(rhs_in[i]);
+        const T b = static_cast<T>(rhs_in[i] * scale);
         if (rhs_out) rhs_out[i] = b;
         x[i] = omega * b / (shift + row_exit(A, i));
     }
@@ -267,10 +268,10 @@ __global__ void __launch_bounds__(kThreads) k_fill(i64 n, T v, T *x) {
 }
 
 template <typename A, typename B>
-__global__ void __launch_bounds__(kThreads) k_convert(i64 n, const A *in, B *out) {
+__global__ void __launch_bounds__(kThreads) k_convert(i64 n, const A *in, A scale, B *out) {
     for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
          i += static_cast<i64>(gridDim.x) * kThreads)
-        out[i] = static_cast<B>(in[i]);
+        out[i] = static_cast<B>(in[i] * scale);
 }
 
 template <typename T>
@@ -452,6 +453,88 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// One step of delayed CGS2. V[0 .. count) is the final basis Q, V[count] the candidate u
+// (orthogonalised once), w = A z. s = Q^T u, tg[0 .. count) = Q^T w, tg[count] = u^T w,
+// beta = |u - Q s|. Rows own lanes of 64-row tiles, waves split the basis (k = 4 kk + wave) and
+// keep their V values in registers for the projections of the new candidate:
+//   q   = (u - Q s) / beta                          -> V[count]
+//   eta = (u^T w - s^T Q^T w) / beta   (= q^T w)     -> eta_out (block 0)
+//   u'  = w - Q (Q^T w) - eta q                     -> V[count + 1]
+//   partial[k nb + block]: Q_k . u' (k < count), q . u' (k = count), u' . u' (k = count + 1)
+__global__ void __launch_bounds__(kThreads)
+    k_dcgs_update(double *V, i64 ld, int count, const double *sc, const double *tg, double beta,
+                  const double *w, i64 n, double *partial, int nb, double *eta_out) {
+    __shared__ double part[2][kWaves][kTileRows];
+    __shared__ double uw[2][kTileRows];
+    __shared__ double coef[2][kFusedBasis]; // s and Q^T w, zero beyond count
+    __shared__ double eta_s;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int k = threadIdx.x; k < kFusedBasis; k += kThreads) {
+        coef[0][k] = k < count ? sc[k] : 0.0;
+        coef[1][k] = k < count ? tg[k] : 0.0;
+    }
+    if (threadIdx.x == 0) {
+        double st = 0.0;
+        for (int k = 0; k < count; ++k) st += sc[k] * tg[k];
+        const double eta = (tg[count] - st) / beta;
+        eta_s = eta;
+        if (blockIdx.x == 0) eta_out[0] = eta;
+    }
+    double acc[kPerWave];
+#pragma unroll
+    for (int kk = 0; kk < kPerWave; ++kk) acc[kk] = 0.0;
+    __syncthreads();
+    const double eta = eta_s;
+    const double ib = 1.0 / beta;
+    double* const q_out = V + static_cast<i64>(count) * ld;
+    double* const u_out = V + static_cast<i64>(count + 1) * ld;
+    double own = 0.0; // wave 2: q . u', wave 3: u' . u'
+    for (i64 r0 = static_cast<i64>(blockIdx.x) * kTileRows; r0 < n;
+         r0 += static_cast<i64>(nb) * kTileRows) {
+        const i64 r = r0 + lane;
+        const bool ok = r < n;
+        double v[kPerWave];
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < kPerWave; ++kk) {
+            const int k = kk * kWaves + wave;
+            v[kk] = (ok && k < count) ? V[k * ld + r] : 0.0;
+            a += coef[0][k] * v[kk];
+            b += coef[1][k] * v[kk];
+        }
+        part[0][wave][lane] = a;
+        part[1][wave][lane] = b;
+        if (wave < 2) uw[wave][lane] = ok ? (wave == 0 ? q_out[r] : w[r]) : 0.0;
+        __syncthreads();
+        const double S = ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) + part[0][3][lane];
+        const double T = ((part[1][0][lane] + part[1][1][lane]) + part[1][2][lane]) + part[1][3][lane];
+        const double q = (uw[0][lane] - S) * ib;
+        const double up = (uw[1][lane] - T) - eta * q;
+        __syncthreads();
+        if (ok) {
+            if (wave == 0) q_out[r] = q;
+            if (wave == 1) u_out[r] = up;
+            if (wave == 2) own += q * up;
+            if (wave == 3) own += up * up;
+#pragma unroll
+            for (int kk = 0; kk < kPerWave; ++kk) acc[kk] += v[kk] * up;
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < kPerWave; ++kk) {
+        const int k = kk * kWaves + wave;
+        if (k < count) {
+            const double t = wave_sum(acc[kk]);
+            if (lane == 0) partial[static_cast<i64>(k) * nb + blockIdx.x] = t;
+        }
+    }
+    if (wave >= 2) {
+        const double t = wave_sum(own);
+        if (lane == 0) partial[static_cast<i64>(count + wave - 2) * nb + blockIdx.x] = t;
+    }
+}
+
 int grid_for(i64 n) {
     const i64 g = (n + kThreads - 1) / kThreads;
     return static_cast<int>(std::max<i64>(1, std::min<i64>(g, 8192)));
@@ -480,10 +563,11 @@ void shifted_apply(const csr_view<double> &A, double shift, const XT *x, const d
 
 template <typename T, typename IN>
 void jacobi_first(const csr_view<T> &A, double shift, double omega, const IN *rhs_in, T *rhs_out,
-                  T *x, hipStream_t s) {
+                  T *x, hipStream_t s, double scale) {
     if (A.n == 0) return;
     hipLaunchKernelGGL((k_jacobi_first<T, IN>), dim3(grid_for(A.n)), dim3(kThreads), 0, s, A,
-                       static_cast<T>(shift), static_cast<T>(omega), rhs_in, rhs_out, x);
+                       static_cast<T>(shift), static_cast<T>(omega), rhs_in,
+                       static_cast<IN>(scale), rhs_out, x);
 }
 
 template <typename T>
@@ -537,9 +621,10 @@ void fill(i64 n, T value, T *x, hipStream_t s) {
 }
 
 template <typename A, typename B>
-void convert(i64 n, const A *in, B *out, hipStream_t s) {
+void convert(i64 n, const A *in, B *out, hipStream_t s, double scale) {
     if (n == 0) return;
-    hipLaunchKernelGGL((k_convert<A, B>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, in, out);
+    hipLaunchKernelGGL((k_convert<A, B>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, in,
+                       static_cast<A>(scale), out);
 }
 
 template <typename T>
@@ -590,6 +675,13 @@ void update_scale(const double *w, const double *V, i64 ld, int count, const dou
                        w, V, ld, count, cw, n, v, beta_flag);
 }
 
+void dcgs_update(double *V, i64 ld, int count, const double *sc, const double *tg, double beta,
+                 const double *w, i64 n, double *partial, int nb, double *eta_out,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_dcgs_update, dim3(nb), dim3(kThreads), 0, s, V, ld, count, sc, tg, beta,
+                       w, n, partial, nb, eta_out);
+}
+
 template <typename ZT>
 void combine(double *x, const ZT *Z, i64 ld, int count, const double *y, i64 n, hipStream_t s) {
     if (count <= 0) return;
@@ -620,14 +712,15 @@ MK_LEVEL(float)
 MK_LEVEL(double)
 #undef MK_LEVEL
 template void jacobi_first<double, double>(const csr_view<double> &, double, double,
-                                           const double *, double *, double *, hipStream_t);
+                                           const double *, double *, double *, hipStream_t,
+                                           double);
 template void jacobi_first<float, double>(const csr_view<float> &, double, double,
-                                          const double *, float *, float *, hipStream_t);
+                                          const double *, float *, float *, hipStream_t, double);
 template void jacobi_first<float, float>(const csr_view<float> &, double, double, const float *,
-                                         float *, float *, hipStream_t);
-template void convert<double, float>(i64, const double *, float *, hipStream_t);
-template void convert<float, double>(i64, const float *, double *, hipStream_t);
-template void convert<double, double>(i64, const double *, double *, hipStream_t);
-template void convert<float, float>(i64, const float *, float *, hipStream_t);
+                                         float *, float *, hipStream_t, double);
+template void convert<double, float>(i64, const double *, float *, hipStream_t, double);
+template void convert<float, double>(i64, const float *, double *, hipStream_t, double);
+template void convert<double, double>(i64, const double *, double *, hipStream_t, double);
+template void convert<float, float>(i64, const float *, float *, hipStream_t, double);
 
 } // namespace mk::gpu

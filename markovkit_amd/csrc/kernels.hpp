@@ -79,10 +79,11 @@ void shifted_apply(const csr_view<double> &A, double shift, const XT *x, const d
                    double *y, double *partial, hipStream_t s);
 
 // Jacobi smoothing on one multigrid level, working precision T:
-//   first sweep from x = 0:  rhs_out = (T) rhs_in (if rhs_out);  x = w * rhs / (shift + exit)
+//   first sweep from x = 0:  rhs = (T)(scale * rhs_in), stored to rhs_out if given;
+//   x = w * rhs / (shift + exit)
 template <typename T, typename IN>
 void jacobi_first(const csr_view<T> &A, double shift, double omega, const IN *rhs_in, T *rhs_out,
-                  T *x, hipStream_t s);
+                  T *x, hipStream_t s, double scale = 1.0);
 //   sweep: x_out = x + w * (rhs - A x) / (shift + exit)
 template <typename T>
 void jacobi_sweep(const csr_view<T> &A, double shift, double omega, const T *rhs, const T *x,
@@ -111,7 +112,7 @@ void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *
 template <typename T>
 void fill(i64 n, T value, T *x, hipStream_t s);
 template <typename A, typename B>
-void convert(i64 n, const A *in, B *out, hipStream_t s);
+void convert(i64 n, const A *in, B *out, hipStream_t s, double scale = 1.0); // out = (B)(in * scale)
 template <typename T>
 void axpy(i64 n, T alpha, const T *x, T *y, hipStream_t s); // y += alpha x
 // plain copy of n 16-byte words (the byte-counter calibration kernel)
@@ -143,6 +144,16 @@ void update_dot(double *w, const double *V, i64 ld, int count, const double *h, 
 //   flag = 1, and the caller normalises explicitly. beta_flag[0] = beta, beta_flag[1] = flag.
 void update_scale(const double *w, const double *V, i64 ld, int count, const double *cw, i64 n,
                   double *v, double *beta_flag, hipStream_t s);
+// Delayed CGS2 (one reduction per Arnoldi step; count < kFusedBasis). V[0 .. count) is the
+// final orthonormal basis Q and V[count] the next candidate u, orthogonalised once against Q;
+// sc = Q^T u, tg[0 .. count) = Q^T w and tg[count] = u^T w for the new operator image w, and
+// beta = |u - Q sc| from the Pythagorean identity. Finalises q = (u - Q sc) / beta in V[count],
+// writes eta = q^T w to eta_out[0] and the new candidate u' = w - Q (Q^T w) - eta q to
+// V[count + 1], with the partial sums of its projections Q_k . u', q . u' and of |u'|^2 at
+// partial[k * nb + block] for k = 0 .. count + 1.
+void dcgs_update(double *V, i64 ld, int count, const double *sc, const double *tg, double beta,
+                 const double *w, i64 n, double *partial, int nb, double *eta_out,
+                 hipStream_t s);
 // x += sum_k y[k] Z_k (Z stored in ZT)
 template <typename ZT>
 void combine(double *x, const ZT *Z, i64 ld, int count, const double *y, i64 n, hipStream_t s);

@@ -101,7 +101,9 @@ def test_multigrid_cycle(case, precision):
 
 
 @pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
-def test_resolvent_solve(case, precision):
+@pytest.mark.parametrize("ortho,restart", [("dcgs2", 50), ("cgs2", 50), ("dcgs2", 6)],
+                         ids=["dcgs2", "cgs2", "dcgs2_restart6"])
+def test_resolvent_solve(case, precision, ortho, restart):
     w, P, box, Q = case["w"], case["P"], case["box"], case["Q"]
     shift = w.shifts[0]
     b = w.rhs(0, P.n)
@@ -109,7 +111,8 @@ def test_resolvent_solve(case, precision):
     A = mk.ShiftedGenerator(Q, shift)
     M = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=precision))
     x = np.zeros(P.n)
-    res = mk.gmres(A, M, b, x, tolerance=tol, max_iterations=500, restart=50)
+    res = mk.gmres(A, M, b, x, tolerance=tol, max_iterations=500, restart=restart,
+                   orthogonalization=ortho)
     assert res.converged, res
     r = b - P.apply_shifted(shift, x)
     assert np.linalg.norm(r) <= 1.0001 * tol
@@ -120,8 +123,30 @@ def test_resolvent_solve(case, precision):
     r_ref = b - P.apply_shifted(shift, x_ref)
     bound = (np.abs(r).sum() + np.abs(r_ref).sum()) / shift
     assert np.abs(x - x_ref).sum() <= bound, (np.abs(x - x_ref).sum(), bound)
-    print(f"[{case['name']}] gpu its={res.iterations} oracle its={info['iterations']} "
-          f"l1={np.abs(x - x_ref).sum():.3e} bound={bound:.3e}")
+    print(f"[{case['name']}] {ortho}(m={restart}) gpu its={res.iterations} "
+          f"oracle its={info['iterations']} l1={np.abs(x - x_ref).sum():.3e} bound={bound:.3e}")
+    if restart == 50:
+        # one FGMRES(50) cycle in exact arithmetic: the iteration counts agree up to rounding
+        assert abs(res.iterations - info["iterations"]) <= 2, (res.iterations, info["iterations"])
+
+
+def test_dcgs2_matches_cgs2_iterates(case):
+    """Delayed and plain CGS2 span the same Krylov space: after a fixed number of iterations
+    (no convergence test) the two FGMRES iterates agree to rounding."""
+    w, P, box, Q = case["w"], case["P"], case["box"], case["Q"]
+    shift = w.shifts[0]
+    b = w.rhs(0, P.n)
+    A = mk.ShiftedGenerator(Q, shift)
+    M = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=mk.FP64))
+    xs = {}
+    for ortho in ("dcgs2", "cgs2"):
+        x = np.zeros(P.n)
+        res = mk.gmres(A, M, b, x, tolerance=1e-300, max_iterations=5, restart=50,
+                       orthogonalization=ortho)
+        assert res.iterations == 5 and not res.converged
+        xs[ortho] = x
+    scale = np.abs(xs["cgs2"]).sum()
+    assert np.abs(xs["dcgs2"] - xs["cgs2"]).sum() <= 1e-9 * scale
 
 
 def test_plain_gmres(case):

@@ -455,25 +455,27 @@ __global__ void __launch_bounds__(kThreads)
 
 // One step of delayed CGS2. V[0 .. count) is the final basis Q, V[count] the candidate u
 // (orthogonalised once), w = A z. s = Q^T u, tg[0 .. count) = Q^T w, tg[count] = u^T w,
-// beta = |u - Q s|. Rows own lanes of 64-row tiles, waves split the basis (k = 4 kk + wave) and
-// keep their V values in registers for the projections of the new candidate:
+// beta = |u - Q s|. Rows own lanes of 64-row tiles; waves split the basis, wave wv holding the
+// PW vectors k = 4 kk + wv (PW = ceil(count / 4); slots past count load vector count - 1 with
+// zero coefficients, so the loads need no branch) in registers for the projections of the new
+// candidate:
 //   q   = (u - Q s) / beta                          -> V[count]
 //   eta = (u^T w - s^T Q^T w) / beta   (= q^T w)     -> eta_out (block 0)
 //   u'  = w - Q (Q^T w) - eta q                     -> V[count + 1]
 //   partial[k nb + block]: Q_k . u' (k < count), q . u' (k = count), u' . u' (k = count + 1)
+template <int PW>
 __global__ void __launch_bounds__(kThreads)
     k_dcgs_update(double *V, i64 ld, int count, const double *sc, const double *tg, double beta,
                   const double *w, i64 n, double *partial, int nb, double *eta_out) {
     __shared__ double part[2][kWaves][kTileRows];
     __shared__ double uw[2][kTileRows];
-    __shared__ double coef[2][kFusedBasis]; // s and Q^T w, zero beyond count
+    __shared__ double2 coef[kFusedBasis]; // (s_k, (Q^T w)_k), zero from count on
     __shared__ double eta_s;
+    constexpr int kSlots = PW > 0 ? PW : 1;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int k = threadIdx.x; k < kFusedBasis; k += kThreads) {
-        coef[0][k] = k < count ? sc[k] : 0.0;
-        coef[1][k] = k < count ? tg[k] : 0.0;
-    }
+    for (int k = threadIdx.x; k < kFusedBasis; k += kThreads)
+        coef[k] = k < count ? make_double2(sc[k], tg[k]) : make_double2(0.0, 0.0);
     if (threadIdx.x == 0) {
         double st = 0.0;
         for (int k = 0; k < count; ++k) st += sc[k] * tg[k];
@@ -481,48 +483,55 @@ __global__ void __launch_bounds__(kThreads)
         eta_s = eta;
         if (blockIdx.x == 0) eta_out[0] = eta;
     }
-    double acc[kPerWave];
-#pragma unroll
-    for (int kk = 0; kk < kPerWave; ++kk) acc[kk] = 0.0;
     __syncthreads();
     const double eta = eta_s;
     const double ib = 1.0 / beta;
-    double* const q_out = V + static_cast<i64>(count) * ld;
-    double* const u_out = V + static_cast<i64>(count + 1) * ld;
+    double *const q_out = V + static_cast<i64>(count) * ld;
+    double *const u_out = V + static_cast<i64>(count + 1) * ld;
+    const double *src[kSlots];
+#pragma unroll
+    for (int kk = 0; kk < PW; ++kk)
+        src[kk] = V + static_cast<i64>(min(kk * kWaves + wave, count - 1)) * ld;
+    const double *xsrc = wave == 0 ? q_out : w; // waves 0 and 1 stage u and w
+    double acc[kSlots], v[kSlots];
+#pragma unroll
+    for (int kk = 0; kk < PW; ++kk) acc[kk] = 0.0;
     double own = 0.0; // wave 2: q . u', wave 3: u' . u'
     for (i64 r0 = static_cast<i64>(blockIdx.x) * kTileRows; r0 < n;
          r0 += static_cast<i64>(nb) * kTileRows) {
         const i64 r = r0 + lane;
         const bool ok = r < n;
-        double v[kPerWave];
+        const i64 rl = ok ? r : n - 1;
+#pragma unroll
+        for (int kk = 0; kk < PW; ++kk) v[kk] = src[kk][rl];
+        const double x = wave < 2 ? xsrc[rl] : 0.0;
         double a = 0.0, b = 0.0;
 #pragma unroll
-        for (int kk = 0; kk < kPerWave; ++kk) {
-            const int k = kk * kWaves + wave;
-            v[kk] = (ok && k < count) ? V[k * ld + r] : 0.0;
-            a += coef[0][k] * v[kk];
-            b += coef[1][k] * v[kk];
+        for (int kk = 0; kk < PW; ++kk) {
+            const double2 c = coef[kk * kWaves + wave];
+            a += c.x * v[kk];
+            b += c.y * v[kk];
         }
         part[0][wave][lane] = a;
         part[1][wave][lane] = b;
-        if (wave < 2) uw[wave][lane] = ok ? (wave == 0 ? q_out[r] : w[r]) : 0.0;
+        if (wave < 2) uw[wave][lane] = x;
         __syncthreads();
         const double S = ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) + part[0][3][lane];
         const double T = ((part[1][0][lane] + part[1][1][lane]) + part[1][2][lane]) + part[1][3][lane];
         const double q = (uw[0][lane] - S) * ib;
-        const double up = (uw[1][lane] - T) - eta * q;
+        const double up = ok ? (uw[1][lane] - T) - eta * q : 0.0;
         __syncthreads();
         if (ok) {
             if (wave == 0) q_out[r] = q;
             if (wave == 1) u_out[r] = up;
-            if (wave == 2) own += q * up;
-            if (wave == 3) own += up * up;
-#pragma unroll
-            for (int kk = 0; kk < kPerWave; ++kk) acc[kk] += v[kk] * up;
         }
+        if (wave == 2) own += q * up;
+        if (wave == 3) own += up * up;
+#pragma unroll
+        for (int kk = 0; kk < PW; ++kk) acc[kk] += v[kk] * up;
     }
 #pragma unroll
-    for (int kk = 0; kk < kPerWave; ++kk) {
+    for (int kk = 0; kk < PW; ++kk) {
         const int k = kk * kWaves + wave;
         if (k < count) {
             const double t = wave_sum(acc[kk]);
@@ -678,8 +687,18 @@ void update_scale(const double *w, const double *V, i64 ld, int count, const dou
 void dcgs_update(double *V, i64 ld, int count, const double *sc, const double *tg, double beta,
                  const double *w, i64 n, double *partial, int nb, double *eta_out,
                  hipStream_t s) {
-    hipLaunchKernelGGL(k_dcgs_update, dim3(nb), dim3(kThreads), 0, s, V, ld, count, sc, tg, beta,
-                       w, n, partial, nb, eta_out);
+    if (count < 0 || count >= kFusedBasis || n <= 0) return;
+    switch ((count + kWaves - 1) / kWaves) {
+#define MK_DCGS(P)                                                                               \
+    case P:                                                                                      \
+        hipLaunchKernelGGL(k_dcgs_update<P>, dim3(nb), dim3(kThreads), 0, s, V, ld, count, sc,   \
+                           tg, beta, w, n, partial, nb, eta_out);                                \
+        break;
+        MK_DCGS(0) MK_DCGS(1) MK_DCGS(2) MK_DCGS(3) MK_DCGS(4) MK_DCGS(5) MK_DCGS(6) MK_DCGS(7)
+        MK_DCGS(8) MK_DCGS(9) MK_DCGS(10) MK_DCGS(11) MK_DCGS(12) MK_DCGS(13) MK_DCGS(14)
+        MK_DCGS(15) MK_DCGS(16)
+#undef MK_DCGS
+    }
 }
 
 template <typename ZT>

@@ -38,6 +38,7 @@ typedef struct mk_box_s *mk_box;
 typedef struct mk_domain_s *mk_domain;
 typedef struct mk_generator_s *mk_generator;
 typedef struct mk_sector_mg_s *mk_sector_mg;
+typedef struct mk_batch_s *mk_batch;
 
 /* boundary (generator.hpp:11-13) */
 enum { MK_REFLECTING = 0, MK_ABSORBING = 1 };
@@ -160,6 +161,27 @@ mk_status mk_gmres(mk_generator gen, double shift, mk_sector_mg mg, const double
 mk_status mk_gmres_device(mk_generator gen, double shift, mk_sector_mg mg, const double *d_b,
                           double *d_x, const mk_gmres_options *options,
                           mk_solver_result *result);
+
+/* ---- batched resolvent solves ----------------------------------------------------------
+ * No single-call counterpart: the reference solves each (shift, rhs) pair with its own
+ * num::gmres call and sector_multigrid (examples/rdme_multigrid_benchmark.cpp:92-121). A batch
+ * solves (s_j I - Q) x_j = b_j for j < n_systems together: gen is the shared generator, mg a
+ * sector multigrid of the same box (its level operators and transfers are shared, its own
+ * shift is not used: the terminal blocks are re-inverted for every distinct s_j, and its
+ * precision selects the FP32 or FP64 preconditioner). gen and mg must outlive the batch. Every
+ * system runs flexible GMRES(restart <= 64) with delayed CGS2; systems stop individually. */
+mk_status mk_batch_create(mk_generator gen, mk_sector_mg mg, int64_t n_systems,
+                          const double *shifts, mk_batch *out);
+mk_status mk_batch_destroy(mk_batch batch);
+/* B and X are system-major (system j's vector at j * rows), host buffers; X holds the initial
+ * guesses on input. tolerances: n_systems absolute residual norms, or NULL for
+ * options->tolerance; results: n_systems entries (orthogonalization is ignored). */
+mk_status mk_batch_solve(mk_batch batch, const double *B, double *X, const double *tolerances,
+                         const mk_gmres_options *options, mk_solver_result *results);
+/* the same with B and X in HBM */
+mk_status mk_batch_solve_device(mk_batch batch, const double *d_B, double *d_X,
+                                const double *tolerances, const mk_gmres_options *options,
+                                mk_solver_result *results);
 
 /* ---- device memory and timing (no reference counterpart: HBM-resident callers) ------ */
 mk_status mk_device_alloc(int64_t bytes, void **d_ptr);

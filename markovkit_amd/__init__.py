@@ -6,14 +6,15 @@ in-tree HIP library through its C ABI and has no CPU fallback.
 """
 from .model import (Graph, Grid, LocalReaction, ReactionDiffusion, VoxelDomain, birth_death,
                     box_size, compositions)
-from .api import (ABSORBING, AGGREGATE_BLOCK, FP32, FP64, JACOBI, REFLECTING, DeviceVector,
+from .api import (ABSORBING, AGGREGATE_BLOCK, FP32, FP64, JACOBI, REFLECTING, BatchSolver,
+                  DeviceVector,
                   Generator, MultigridOptions, PopulationBox, SectorMultigrid, ShiftedGenerator,
                   SolverResult, gmres, profile_enable, profile_read, profile_reset, synchronize)
 
 __all__ = [
     "Graph", "Grid", "LocalReaction", "ReactionDiffusion", "VoxelDomain", "birth_death",
     "box_size", "compositions", "ABSORBING", "AGGREGATE_BLOCK", "FP32", "FP64", "JACOBI",
-    "REFLECTING", "DeviceVector", "Generator", "MultigridOptions", "PopulationBox",
+    "REFLECTING", "BatchSolver", "DeviceVector", "Generator", "MultigridOptions", "PopulationBox",
     "SectorMultigrid", "ShiftedGenerator", "SolverResult", "gmres", "profile_enable",
     "profile_read", "profile_reset", "synchronize",
 ]

@@ -65,6 +65,10 @@ SIGNATURES = {
     "mk_sector_mg_apply_device": ([P, P, P], i32),
     "mk_gmres": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_gmres_device": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
+    "mk_batch_create": ([P, P, i64, P, C.POINTER(P)], i32),
+    "mk_batch_destroy": ([P], i32),
+    "mk_batch_solve": ([P, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
+    "mk_batch_solve_device": ([P, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_device_alloc": ([i64, C.POINTER(P)], i32),
     "mk_device_free": ([P], i32),
     "mk_copy_to_device": ([P, P, i64], i32),

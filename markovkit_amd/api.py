@@ -383,6 +383,54 @@ def gmres(A: ShiftedGenerator, *args, tolerance: float = 1e-10, max_iterations: 
     return SolverResult(res.iterations, res.residual, bool(res.converged))
 
 
+class BatchSolver:
+    """Batched resolvent solves (s_j I - Q) x_j = b_j, j < len(shifts), over one population box:
+    the generator and the sector multigrid hierarchy are shared, the terminal blocks are
+    re-inverted per distinct shift, and every system runs flexible GMRES with delayed CGS2,
+    stopping on its own tolerance (mk_batch_create / mk_batch_solve). The multigrid's
+    precision selects the preconditioner; its own shift is not used."""
+
+    def __init__(self, Q: Generator, M: SectorMultigrid, shifts: Sequence[float]):
+        self.shifts = np.ascontiguousarray(shifts, np.float64)
+        self._Q, self._M = Q, M  # the batch reads both handles
+        self._h = C.c_void_p()
+        check(lib().mk_batch_create(Q._h, M._h, len(self.shifts), _p(self.shifts),
+                                    C.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mk_batch_destroy(self._h)
+            self._h = None
+
+    def __len__(self) -> int:
+        return len(self.shifts)
+
+    def solve(self, B, X=None, *, tolerance: float = 1e-10, tolerances=None,
+              max_iterations: int = 1000, restart: int = 50):
+        """B: (n_systems, rows) right-hand sides; X: initial guesses of the same shape
+        (zeros if None), overwritten with the solutions. tolerance / tolerances: absolute
+        residual norm for all / each system. Returns (X, [SolverResult per system])."""
+        n = self._Q.rows()
+        k = len(self.shifts)
+        Ba = np.ascontiguousarray(B, np.float64)
+        if Ba.shape != (k, n):
+            raise ValueError(f"B must have shape ({k}, {n})")
+        if X is None:
+            X = np.zeros((k, n))
+        if X.dtype != np.float64 or not X.flags.c_contiguous or X.shape != (k, n):
+            raise TypeError("X must be a contiguous float64 array of B's shape")
+        tol = None
+        if tolerances is not None:
+            tol = np.ascontiguousarray(tolerances, np.float64)
+            if tol.shape != (k,):
+                raise ValueError("one tolerance per system")
+        opts = _GmresOptions(float(tolerance), int(max_iterations), int(restart), 0)
+        res = (_SolverResult * k)()
+        check(lib().mk_batch_solve(self._h, _p(Ba), _p(X), _p(tol) if tol is not None else None,
+                                   C.byref(opts), res))
+        return X, [SolverResult(r.iterations, r.residual, bool(r.converged)) for r in res]
+
+
 KERNEL_CLASSES = ("shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict",
                   "prolong", "terminal", "gmres_dot", "gmres_update", "gmres_other")
 

@@ -1,373 +1,11 @@
 // C-ABI of markovkit_amd: handles, device residency, the multigrid V-cycle and flexible GMRES
 // orchestrated on one HIP stream. Reference constructs replaced by each entry point are cited
 // in include/markovkit_amd.h.
-#include "../../include/markovkit_amd.h"
-#include "kernels.hpp"
-#include "setup.hpp"
+#include "runtime.hpp"
 
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstdlib>
-#include <cstring>
 #include <map>
-#include <memory>
-#include <stdexcept>
-#include <string>
-#include <vector>
 
-namespace {
-
-using mk::i64;
-using mk::u32;
-using mk::u64;
-namespace gpu = mk::gpu;
-
-struct device_error : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-thread_local std::string g_error;
-
-#define HIPCHK(call)                                                                              \
-    do {                                                                                          \
-        const hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess)                                                                     \
-            throw device_error(std::string(#call) + ": " + hipGetErrorString(e_));               \
-    } while (0)
-
-void check_launch(const char *what) {
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) throw device_error(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <typename F>
-mk_status guarded(F &&f) {
-    try {
-        f();
-        return MK_OK;
-    } catch (const std::invalid_argument &e) {
-        g_error = e.what();
-        return MK_ERR_INVALID_ARGUMENT;
-    } catch (const std::overflow_error &e) {
-        g_error = e.what();
-        return MK_ERR_OVERFLOW;
-    } catch (const device_error &e) {
-        g_error = e.what();
-        return MK_ERR_DEVICE;
-    } catch (const std::bad_alloc &) {
-        g_error = "out of host memory";
-        return MK_ERR_RUNTIME;
-    } catch (const std::exception &e) {
-        g_error = e.what();
-        return MK_ERR_RUNTIME;
-    }
-}
-
-void require(bool ok, const char *what) {
-    if (!ok) throw std::invalid_argument(what);
-}
-
-// ------------------------------------------------------------------ device memory
-struct dbuf {
-    void *p = nullptr;
-    std::size_t bytes = 0;
-    dbuf() = default;
-    explicit dbuf(std::size_t n) : bytes(n) {
-        if (n) HIPCHK(hipMalloc(&p, n));
-    }
-    dbuf(const dbuf &) = delete;
-    dbuf &operator=(const dbuf &) = delete;
-    dbuf(dbuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
-    dbuf &operator=(dbuf &&o) noexcept {
-        if (this != &o) {
-            release();
-            p = o.p;
-            bytes = o.bytes;
-            o.p = nullptr;
-            o.bytes = 0;
-        }
-        return *this;
-    }
-    ~dbuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
-
-template <typename T>
-T *dp(const dbuf &b) {
-    return static_cast<T *>(b.p);
-}
-
-template <typename T>
-dbuf upload(const std::vector<T> &v) {
-    dbuf b(v.size() * sizeof(T));
-    if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return b;
-}
-
-// ------------------------------------------------------------------ library state
-hipStream_t stream() {
-    static hipStream_t s = [] {
-        hipStream_t t = nullptr;
-        HIPCHK(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
-        // tuning override of the operator kernels' XCD interleave chunk (0 = identity mapping)
-        if (const char *env = std::getenv("MK_XCD_CHUNK")) gpu::set_xcd_chunk(std::atoi(env));
-        return t;
-    }();
-    return s;
-}
-
-enum kclass {
-    K_SPMV,
-    K_FIRST,
-    K_SWEEP,
-    K_RESIDUAL,
-    K_RESTRICT,
-    K_PROLONG,
-    K_TERMINAL,
-    K_DOT,
-    K_UPDATE,
-    K_OTHER,
-    K_COUNT
-};
-const char *const kclass_names[K_COUNT] = {"shifted_spmv", "jacobi_first", "jacobi_sweep",
-                                           "residual",     "restrict",     "prolong",
-                                           "terminal",     "gmres_dot",    "gmres_update",
-                                           "gmres_other"};
-
-struct profiler {
-    std::uint32_t mask = 0;
-    struct rec {
-        int cls;
-        hipEvent_t a, b;
-        double bytes;
-    };
-    std::vector<rec> pending;
-    std::vector<hipEvent_t> pool;
-    i64 launches[K_COUNT] = {};
-    double ms[K_COUNT] = {};
-    double bytes[K_COUNT] = {};
-
-    hipEvent_t event() {
-        if (!pool.empty()) {
-            hipEvent_t e = pool.back();
-            pool.pop_back();
-            return e;
-        }
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        return e;
-    }
-    template <typename F>
-    void run(int cls, double nbytes, F &&launch) {
-        if (!(mask & (1u << cls))) {
-            launch();
-            check_launch(kclass_names[cls]);
-            return;
-        }
-        rec r{cls, event(), event(), nbytes};
-        HIPCHK(hipEventRecord(r.a, stream()));
-        launch();
-        check_launch(kclass_names[cls]);
-        HIPCHK(hipEventRecord(r.b, stream()));
-        pending.push_back(r);
-        if (pending.size() > 4096) resolve();
-    }
-    void resolve() {
-        for (auto &r : pending) {
-            HIPCHK(hipEventSynchronize(r.b));
-            float t = 0.f;
-            HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
-            launches[r.cls] += 1;
-            ms[r.cls] += t;
-            bytes[r.cls] += r.bytes;
-            pool.push_back(r.a);
-            pool.push_back(r.b);
-        }
-        pending.clear();
-    }
-    void reset() {
-        resolve();
-        std::fill(std::begin(launches), std::end(launches), 0);
-        std::fill(std::begin(ms), std::end(ms), 0.0);
-        std::fill(std::begin(bytes), std::end(bytes), 0.0);
-    }
-};
-
-profiler &prof() {
-    static profiler p;
-    return p;
-}
-
-void drain() {
-    HIPCHK(hipStreamSynchronize(stream()));
-    prof().resolve();
-}
-
-// ------------------------------------------------------------------ gather CSR on device
-template <typename VT>
-struct dev_csr {
-    i64 n = 0;
-    u64 nnz = 0;
-    bool coded = false;
-    u64 padded = 0; // stored entries of the coded layout, slice padding included
-    dbuf blocks, ptr, col, val, exit, soff, ent, rlen, rcode, vdict, edict;
-    gpu::csr_view<VT> view;
-};
-
-// every array the selected kernel reads must exist before anything is launched on it
-template <typename VT>
-void check_view(const gpu::csr_view<VT> &v, u64 nnz) {
-    bool ok = v.n == 0 || v.n_blocks > 0;
-    if (v.coded) {
-        ok = ok && v.n_slices == (v.n + gpu::kSlice - 1) / gpu::kSlice &&
-             v.n_blocks * (gpu::kThreads / gpu::kSlice) >= v.n_slices &&
-             (v.n == 0 || (v.soff && v.rlen && v.rcode && v.edict && v.vdict)) &&
-             (nnz == 0 || v.ent) && v.n_vdict > 0 && v.n_vdict <= gpu::kDictMax &&
-             v.n_edict > 0 && v.n_edict <= gpu::kDictMax;
-    } else {
-        ok = ok && (v.n == 0 || (v.blocks && v.ptr && v.exit)) && (nnz == 0 || (v.col && v.val));
-    }
-    if (!ok) throw std::logic_error("operator upload left a required device array missing");
-}
-
-// upload an operator (gather CSR + exit rates) in the coded sliced layout when it fits, else
-// in the plain CSR-stream layout
-template <typename VT>
-dev_csr<VT> upload_csr(const mk::generator_csr &g) {
-    dev_csr<VT> d;
-    d.n = g.n;
-    d.nnz = g.ptr.empty() ? 0 : g.ptr.back();
-    if (d.nnz >= (u64{1} << 32)) throw std::overflow_error("operator has more than 2^32 entries");
-    d.view.n = g.n;
-    u64 max_len = 0;
-    for (i64 i = 0; i < g.n; ++i) max_len = std::max<u64>(max_len, g.ptr[i + 1] - g.ptr[i]);
-
-    static_assert(mk::kSliceRows == gpu::kSlice && mk::kCodedColumnBits == static_cast<int>(gpu::kColBits),
-                  "host and device agree on the coded layout");
-    (void)max_len;
-    mk::sliced_operator<VT> so;
-    if (mk::encode_sliced<VT>(g, so)) {
-        const i64 n_slices = static_cast<i64>(so.soff.size()) - 1;
-        d.coded = true;
-        d.padded = so.padded;
-        d.soff = upload(so.soff);
-        d.ent = upload(so.ent);
-        d.rlen = upload(so.rlen);
-        d.rcode = upload(so.rcode);
-        d.vdict = upload(so.vdict);
-        d.edict = upload(so.edict);
-        d.view.n_slices = n_slices;
-        d.view.n_blocks = (n_slices + (gpu::kThreads / gpu::kSlice) - 1) / (gpu::kThreads / gpu::kSlice);
-        d.view.soff = dp<u32>(d.soff);
-        d.view.ent = dp<u32>(d.ent);
-        d.view.rlen = dp<std::uint8_t>(d.rlen);
-        d.view.rcode = dp<std::uint8_t>(d.rcode);
-        d.view.vdict = dp<VT>(d.vdict);
-        d.view.edict = dp<VT>(d.edict);
-        d.view.n_vdict = static_cast<int>(so.vdict.size());
-        d.view.n_edict = static_cast<int>(so.edict.size());
-        d.view.coded = 1;
-        check_view(d.view, d.nnz);
-        return d;
-    }
-    // plain CSR-stream: row blocks of up to kRowsPerBlock rows whose entries fit the LDS tile;
-    // a row longer than the tile gets a block of its own
-    std::vector<u32> blocks{0};
-    i64 r = 0;
-    while (r < g.n) {
-        const i64 r0 = r;
-        const u64 p0 = g.ptr[r0];
-        ++r;
-        while (r < g.n && r - r0 < gpu::kRowsPerBlock &&
-               g.ptr[r + 1] - p0 <= static_cast<u64>(gpu::kTile))
-            ++r;
-        blocks.push_back(static_cast<u32>(r));
-    }
-    d.blocks = upload(blocks);
-    d.view.n_blocks = static_cast<i64>(blocks.size()) - 1;
-    d.view.blocks = dp<u32>(d.blocks);
-    std::vector<u32> ptr(g.n + 1);
-    for (i64 i = 0; i <= g.n; ++i) ptr[i] = static_cast<u32>(g.ptr[i]);
-    d.ptr = upload(ptr);
-    d.col = upload(g.col);
-    std::vector<VT> v(g.val.begin(), g.val.end()), e(g.exit.begin(), g.exit.end());
-    d.val = upload(v);
-    d.exit = upload(e);
-    d.view.ptr = dp<u32>(d.ptr);
-    d.view.col = dp<u32>(d.col);
-    d.view.val = dp<VT>(d.val);
-    d.view.exit = dp<VT>(d.exit);
-    check_view(d.view, d.nnz);
-    return d;
-}
-
-// algorithmic bytes of one pass over an operator in its implemented layout: the encoded
-// entries and per-row data once, x once (xs bytes per row), and `vectors` other per-row
-// vectors of ts bytes (rhs, output, ...)
-template <typename VT>
-double op_bytes(const dev_csr<VT> &A, std::size_t xs, std::size_t ts, int vectors) {
-    const double n = double(A.n), nnz = double(A.nnz);
-    double b = A.coded ? double(A.padded) * 4.0 + n * 2.0 + double(A.view.n_slices + 1) * 4.0
-                       : nnz * double(sizeof(VT) + 4) + (n + 1) * 4.0 + n * sizeof(VT);
-    return b + n * double(xs) + n * double(ts) * vectors;
-}
-
-} // namespace
-
-// ------------------------------------------------------------------ handle types
-struct mk_box_s {
-    mk::population_space space;
-};
-
-struct mk_domain_s {
-    mk::voxel_domain dom;
-};
-
-struct mk_generator_s {
-    i64 n = 0;
-    i64 n_jumps = 0;
-    std::vector<double> exit, leak;
-    dev_csr<double> A;
-    dbuf d_rpart;  // one partial residual norm per row block
-    dbuf d_x, d_y; // staging for host-buffer calls
-};
-
-namespace {
-
-template <typename T>
-struct level_dev {
-    i64 n = 0;
-    dev_csr<T> A;
-    // transfer to the next level
-    dbuf coarse_of, weight;
-    i64 n_fine_linked = 0; // rows that have a coarse parent
-    dev_csr<T> R;          // restriction: next level's rows x this level's rows, rates 1
-    i64 n_next = 0;
-    // terminal blocks on this level
-    i64 n_term = 0, term_max = 0;
-    dbuf tstart, tsize, toff, tinv;
-    // work vectors
-    dbuf rhs, x, xt, res, crhs, cx;
-    int pre = 2, post = 2;
-};
-
-} // namespace
-
-struct mk_sector_mg_s {
-    mk::multigrid_options opt;
-    double shift = 0.0;
-    int precision = MK_FP64;
-    i64 rows = 0, n_sectors = 0, max_levels = 0;
-    std::vector<level_dev<double>> L64;
-    std::vector<level_dev<float>> L32;
-    dbuf d_b, d_x64, d_out; // staging for host-buffer calls
-};
+using namespace mk::rt;
 
 namespace {
 
@@ -421,6 +59,13 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
                 tsize.push_back(static_cast<u32>(s.rows));
                 toff.push_back(tinv.size());
                 tinv.insert(tinv.end(), s.inverse.begin(), s.inverse.end());
+                const std::size_t q0 = lv.h_tq.size();
+                lv.h_tq.resize(q0 + static_cast<std::size_t>(s.rows * s.rows), 0.0);
+                for (i64 i = 0; i < s.rows; ++i) {
+                    double *row = lv.h_tq.data() + q0 + i * s.rows;
+                    row[i] += s.A.exit[i];
+                    for (u64 p = s.A.ptr[i]; p < s.A.ptr[i + 1]; ++p) row[s.A.col[p]] -= s.A.val[p];
+                }
                 lv.term_max = std::max<i64>(lv.term_max, s.rows);
                 coarse_of.insert(coarse_of.end(), s.rows, 0xFFFFFFFFu);
                 weight.insert(weight.end(), s.rows, 0.0);
@@ -436,6 +81,9 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
         }
         lv.A = upload_csr<T>(cat);
         lv.n_term = static_cast<i64>(tstart.size());
+        lv.h_tstart = tstart;
+        lv.h_tsize = tsize;
+        lv.h_toff = toff;
         if (lv.n_term > 0) {
             lv.tstart = upload(tstart);
             lv.tsize = upload(tsize);
@@ -634,46 +282,6 @@ double device_norm2(const double *partial, int nb, double *hd) {
     drain();
     return v;
 }
-
-// Givens-rotated least-squares problem of one restart cycle (column-major upper Hessenberg H of
-// size (m + 1) x m, stored row-major with leading dimension m)
-struct hessenberg_ls {
-    i64 m;
-    std::vector<double> H, cs, sn, g, y;
-    explicit hessenberg_ls(i64 m_) : m(m_), H((m_ + 1) * m_), cs(m_), sn(m_), g(m_ + 1), y(m_) {}
-    void start(double beta) {
-        std::fill(H.begin(), H.end(), 0.0);
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
-    }
-    // column j = (col[0 .. j], hn); returns |g[j + 1]|, the residual norm of the cycle so far
-    double add_column(i64 j, const double *col, double hn) {
-        for (i64 k = 0; k <= j; ++k) H[k * m + j] = col[k];
-        H[(j + 1) * m + j] = hn;
-        for (i64 k = 0; k < j; ++k) {
-            const double a = H[k * m + j], c = H[(k + 1) * m + j];
-            H[k * m + j] = cs[k] * a + sn[k] * c;
-            H[(k + 1) * m + j] = -sn[k] * a + cs[k] * c;
-        }
-        const double a = H[j * m + j], c = H[(j + 1) * m + j];
-        const double d = std::hypot(a, c);
-        cs[j] = d > 0.0 ? a / d : 1.0;
-        sn[j] = d > 0.0 ? c / d : 0.0;
-        H[j * m + j] = d;
-        H[(j + 1) * m + j] = 0.0;
-        g[j + 1] = -sn[j] * g[j];
-        g[j] = cs[j] * g[j];
-        return std::abs(g[j + 1]);
-    }
-    // y = R^{-1} g over the first j columns
-    void solve(i64 j) {
-        for (i64 i = j; i-- > 0;) {
-            double sum = g[i];
-            for (i64 k = i + 1; k < j; ++k) sum -= H[i * m + k] * y[k];
-            y[i] = sum / H[i * m + i];
-        }
-    }
-};
 
 // Flexible GMRES(m), right-preconditioned: x = x0 + Z y with z_j = M v_j (the numerics gmres
 // with a preconditioner, called at tests/test_rdme.cpp:242-243). The Arnoldi basis is
@@ -905,7 +513,7 @@ mk_solver_result gmres_device(mk_generator_s &A, double shift, mk_sector_mg_s *M
 // ====================================================================== C ABI
 extern "C" {
 
-const char *mk_last_error(void) { return g_error.c_str(); }
+const char *mk_last_error(void) { return last_error().c_str(); }
 
 void mk_default_multigrid_options(mk_multigrid_options *o) {
     o->pre_sweeps = 2;

@@ -1,0 +1,538 @@
+// Batched resolvent-solve kernels (gfx950, wave64): lane = system, wavefront = one state row of
+// 64 systems. See batch_kernels.hpp for the layout.
+#include "batch_kernels.hpp"
+
+#include <algorithm>
+
+namespace mk::gpu::batch {
+
+namespace {
+
+constexpr int kWaves = kThreads / kLanes;
+constexpr int kEntryUnroll = 8;
+constexpr int kRowsPerPart = 256;
+constexpr int kMaxParts = 512;
+constexpr int kSlotsMax = kFusedBasis / kWaves; // basis slots per wave in the update kernel
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+enum class bmode { shifted, shifted_residual, sweep, residual, gather_sum };
+
+template <typename VT, typename XT, typename T>
+struct bop_args {
+    csr_view<VT> A;
+    const double *shifts;
+    T omega;
+    const XT *x;
+    const T *rhs;
+    T *out;
+    double *partial;
+    i64 Bp;
+    const int *active;
+};
+
+// sum over the row's entries of rate * x[column][j], ascending entry order; row r is
+// wave-uniform, so the entry, length and dictionary loads are scalar
+template <typename VT, typename XT, typename T>
+__device__ __forceinline__ T row_gather(const csr_view<VT> &A, i64 r, const XT *x, i64 Bp, i64 j,
+                                        bool on, T &exit_r) {
+    T sum = 0;
+    if (A.coded) {
+        const i64 slice = r >> 6;
+        const u32 len = A.rlen[r];
+        const u32 base = A.soff[slice] + static_cast<u32>(r & 63);
+        exit_r = static_cast<T>(A.edict[A.rcode[r]]);
+        for (u32 k0 = 0; k0 < len; k0 += kEntryUnroll) {
+            u32 e[kEntryUnroll];
+#pragma unroll
+            for (int u = 0; u < kEntryUnroll; ++u)
+                e[u] = k0 + u < len ? A.ent[base + (k0 + u) * kSlice] : 0u;
+            XT xv[kEntryUnroll];
+#pragma unroll
+            for (int u = 0; u < kEntryUnroll; ++u)
+                xv[u] = (on && k0 + u < len) ? x[static_cast<i64>(e[u] & kColMask) * Bp + j] : XT(0);
+#pragma unroll
+            for (int u = 0; u < kEntryUnroll; ++u)
+                if (k0 + u < len) sum += static_cast<T>(A.vdict[e[u] >> kColBits]) * static_cast<T>(xv[u]);
+        }
+    } else {
+        const u32 p0 = A.ptr[r], p1 = A.ptr[r + 1];
+        exit_r = A.exit ? static_cast<T>(A.exit[r]) : T(0);
+        for (u32 p = p0; p < p1; p += kEntryUnroll) {
+            u32 c[kEntryUnroll];
+            T v[kEntryUnroll];
+#pragma unroll
+            for (int u = 0; u < kEntryUnroll; ++u) {
+                c[u] = p + u < p1 ? A.col[p + u] : 0u;
+                v[u] = p + u < p1 ? static_cast<T>(A.val[p + u]) : T(0);
+            }
+            XT xv[kEntryUnroll];
+#pragma unroll
+            for (int u = 0; u < kEntryUnroll; ++u)
+                xv[u] = (on && p + u < p1) ? x[static_cast<i64>(c[u]) * Bp + j] : XT(0);
+#pragma unroll
+            for (int u = 0; u < kEntryUnroll; ++u)
+                if (p + u < p1) sum += v[u] * static_cast<T>(xv[u]);
+        }
+    }
+    return sum;
+}
+
+// grid (row partitions, Bp / 64); wave wv of partition p takes rows p * 4 + wv + k * 4 * np
+template <bmode M, typename VT, typename XT, typename T>
+__global__ void __launch_bounds__(kThreads) k_bop(bop_args<VT, XT, T> a) {
+    __shared__ double red[kWaves][kLanes];
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const i64 j = static_cast<i64>(blockIdx.y) * kLanes + lane;
+    const bool on = !a.active || a.active[j] != 0;
+    T sh = 0;
+    if constexpr (M != bmode::gather_sum) sh = static_cast<T>(a.shifts[j]);
+    double sq = 0.0;
+    const i64 stride = static_cast<i64>(gridDim.x) * kWaves;
+    for (i64 r = static_cast<i64>(blockIdx.x) * kWaves + wave; r < a.A.n; r += stride) {
+        T ex = 0;
+        const T sum = row_gather<VT, XT, T>(a.A, r, a.x, a.Bp, j, on, ex);
+        if (!on) continue;
+        const i64 o = r * a.Bp + j;
+        if constexpr (M == bmode::gather_sum) {
+            a.out[o] = sum;
+        } else {
+            const T xr = static_cast<T>(a.x[o]);
+            const T d = sh + ex;
+            const T ax = d * xr - sum;
+            if constexpr (M == bmode::shifted) {
+                a.out[o] = ax;
+            } else if constexpr (M == bmode::shifted_residual) {
+                const T res = a.rhs[o] - ax;
+                a.out[o] = res;
+                sq += static_cast<double>(res) * static_cast<double>(res);
+            } else if constexpr (M == bmode::sweep) {
+                const T res = a.rhs[o] - ax;
+                a.out[o] = xr + a.omega * res / d;
+            } else {
+                a.out[o] = a.rhs[o] - ax;
+            }
+        }
+    }
+    if constexpr (M == bmode::shifted_residual) {
+        red[wave][lane] = sq;
+        __syncthreads();
+        if (wave == 0 && a.partial)
+            a.partial[static_cast<i64>(blockIdx.x) * a.Bp + j] =
+                ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    }
+}
+
+int op_parts(i64 n) {
+    return static_cast<int>(std::max<i64>(1, std::min<i64>((n + kRowsPerPart - 1) / kRowsPerPart, kMaxParts)));
+}
+
+template <bmode M, typename VT, typename XT, typename T>
+void launch(const bop_args<VT, XT, T> &a, hipStream_t s) {
+    if (a.A.n == 0) return;
+    hipLaunchKernelGGL((k_bop<M, VT, XT, T>), dim3(op_parts(a.A.n), a.Bp / kLanes), dim3(kThreads),
+                       0, s, a);
+}
+
+template <typename T>
+__device__ __forceinline__ T row_exit(const csr_view<T> &A, i64 r) {
+    return A.coded ? A.edict[A.rcode[r]] : A.exit[r];
+}
+
+int grid_for(i64 n) {
+    const i64 g = (n + kThreads - 1) / kThreads;
+    return static_cast<int>(std::max<i64>(1, std::min<i64>(g, 16384)));
+}
+
+template <typename T, typename IN>
+__global__ void __launch_bounds__(kThreads)
+    k_jacobi_first(csr_view<T> A, const double *shifts, T omega, const IN *in,
+                   const double *scale, T *rhs_out, T *x, i64 Bp, const int *active) {
+    const i64 total = A.n * Bp;
+    for (i64 o = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; o < total;
+         o += static_cast<i64>(gridDim.x) * kThreads) {
+        const i64 r = o / Bp, j = o - r * Bp;
+        if (active && !active[j]) continue;
+        const IN sc = scale ? static_cast<IN>(scale[j]) : IN(1);
+        const T b = static_cast<T>(in[o] * sc);
+        if (rhs_out) rhs_out[o] = b;
+        x[o] = omega * b / (static_cast<T>(shifts[j]) + row_exit(A, r));
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x, i64 Bp,
+                  const int *active) {
+    const i64 total = n * Bp;
+    for (i64 o = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; o < total;
+         o += static_cast<i64>(gridDim.x) * kThreads) {
+        const i64 i = o / Bp, j = o - i * Bp;
+        if (active && !active[j]) continue;
+        const u32 c = coarse_of[i];
+        if (c != 0xFFFFFFFFu) x[o] += weight[i] * coarse[static_cast<i64>(c) * Bp + j];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_terminal(i64 nb, const u32 *start, const u32 *size, const u64 *offset, const T *inv,
+               u64 per_shift, const int *shift_index, i64 max_size, const T *r, T *x, i64 Bp,
+               const int *active) {
+    const i64 total = nb * max_size * Bp;
+    for (i64 g = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; g < total;
+         g += static_cast<i64>(gridDim.x) * kThreads) {
+        const i64 j = g % Bp, bi = g / Bp;
+        const i64 b = bi / max_size, i = bi % max_size;
+        const u32 m = size[b];
+        if (i >= m || (active && !active[j])) continue;
+        const T *row = inv + static_cast<u64>(shift_index[j]) * per_shift + offset[b] +
+                       static_cast<u64>(i) * m;
+        const i64 s0 = start[b];
+        T acc = 0;
+        for (u32 c = 0; c < m; ++c) acc += row[c] * r[(s0 + c) * Bp + j];
+        x[(s0 + i) * Bp + j] += acc;
+    }
+}
+
+// ---- Krylov
+// grid (np, chunks of 8 basis vectors, Bp / 64)
+__global__ void __launch_bounds__(kThreads)
+    k_multi_dot(const double *V, int count, const double *w, i64 n, i64 Bp, double *partial,
+                int np, const int *active) {
+    __shared__ double red[kWaves][kDotChunk][kLanes];
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const i64 j = static_cast<i64>(blockIdx.z) * kLanes + lane;
+    const bool on = !active || active[j] != 0;
+    const int k0 = blockIdx.y * kDotChunk;
+    const int kc = min(kDotChunk, count - k0);
+    const i64 ld = n * Bp;
+    double acc[kDotChunk];
+#pragma unroll
+    for (int k = 0; k < kDotChunk; ++k) acc[k] = 0.0;
+    if (on) {
+        const i64 stride = static_cast<i64>(np) * kWaves;
+        for (i64 r = static_cast<i64>(blockIdx.x) * kWaves + wave; r < n; r += stride) {
+            const i64 o = r * Bp + j;
+            const double wr = w[o];
+#pragma unroll
+            for (int k = 0; k < kDotChunk; ++k)
+                if (k < kc) acc[k] += V[(k0 + k) * ld + o] * wr;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kDotChunk; ++k) red[wave][k][lane] = acc[k];
+    __syncthreads();
+    if (wave == 0) {
+        const i64 m = static_cast<i64>(count) * Bp;
+        for (int k = 0; k < kc; ++k)
+            partial[static_cast<i64>(blockIdx.x) * m + (k0 + k) * Bp + j] =
+                ((red[0][k][lane] + red[1][k][lane]) + red[2][k][lane]) + red[3][k][lane];
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_reduce(const double *partial, int np, i64 m, double *out) {
+    for (i64 o = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; o < m;
+         o += static_cast<i64>(gridDim.x) * kThreads) {
+        double s = 0.0;
+        for (int p = 0; p < np; ++p) s += partial[static_cast<i64>(p) * m + o];
+        out[o] = s;
+    }
+}
+
+// delayed CGS2 step, grid (np, Bp / 64): partition p takes rows p, p + np, ...; wave wv keeps
+// the basis slots k = 4 kk + wv (clamped to count - 1, zero coefficients past count)
+template <int PW>
+__global__ void __launch_bounds__(kThreads)
+    k_dcgs_update(double *V, int count, const double *sc, const double *tg, const double *beta,
+                  const double *w, i64 n, i64 Bp, double *partial, int np, double *eta_out,
+                  const int *active) {
+    __shared__ double part[2][kWaves][kLanes];
+    __shared__ double uw[2][kLanes];
+    __shared__ double red[2][kLanes];
+    constexpr int kSlots = PW > 0 ? PW : 1;
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const i64 j = static_cast<i64>(blockIdx.y) * kLanes + lane;
+    const bool on = !active || active[j] != 0;
+    const i64 ld = n * Bp;
+    double cs[kSlots], ct[kSlots], acc[kSlots];
+    const double *src[kSlots];
+#pragma unroll
+    for (int kk = 0; kk < PW; ++kk) {
+        const int k = kk * kWaves + wave;
+        cs[kk] = (on && k < count) ? sc[static_cast<i64>(k) * Bp + j] : 0.0;
+        ct[kk] = (on && k < count) ? tg[static_cast<i64>(k) * Bp + j] : 0.0;
+        src[kk] = V + static_cast<i64>(min(k, count - 1)) * ld;
+        acc[kk] = 0.0;
+    }
+    double eta = 0.0, ib = 0.0;
+    if (on) {
+        double st = 0.0;
+        for (int k = 0; k < count; ++k) st += sc[static_cast<i64>(k) * Bp + j] * tg[static_cast<i64>(k) * Bp + j];
+        const double b = beta[j];
+        eta = (tg[static_cast<i64>(count) * Bp + j] - st) / b;
+        ib = 1.0 / b;
+        if (blockIdx.x == 0 && wave == 0) eta_out[j] = eta;
+    }
+    double *const q_out = V + static_cast<i64>(count) * ld;
+    double *const u_out = V + static_cast<i64>(count + 1) * ld;
+    double own = 0.0; // wave 2: q . u', wave 3: u' . u'
+    for (i64 r = blockIdx.x; r < n; r += np) {
+        const i64 o = r * Bp + j;
+        double v[kSlots];
+#pragma unroll
+        for (int kk = 0; kk < PW; ++kk) v[kk] = on ? src[kk][o] : 0.0;
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < PW; ++kk) {
+            a += cs[kk] * v[kk];
+            b += ct[kk] * v[kk];
+        }
+        part[0][wave][lane] = a;
+        part[1][wave][lane] = b;
+        if (wave < 2) uw[wave][lane] = on ? (wave == 0 ? q_out[o] : w[o]) : 0.0;
+        __syncthreads();
+        const double S = ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) + part[0][3][lane];
+        const double T = ((part[1][0][lane] + part[1][1][lane]) + part[1][2][lane]) + part[1][3][lane];
+        const double q = on ? (uw[0][lane] - S) * ib : 0.0;
+        const double up = on ? (uw[1][lane] - T) - eta * q : 0.0;
+        __syncthreads();
+        if (on) {
+            if (wave == 0) q_out[o] = q;
+            if (wave == 1) u_out[o] = up;
+        }
+        if (wave == 2) own += q * up;
+        if (wave == 3) own += up * up;
+#pragma unroll
+        for (int kk = 0; kk < PW; ++kk) acc[kk] += v[kk] * up;
+    }
+    const i64 m = static_cast<i64>(count + 2) * Bp;
+    double *out = partial + static_cast<i64>(blockIdx.x) * m;
+#pragma unroll
+    for (int kk = 0; kk < PW; ++kk) {
+        const int k = kk * kWaves + wave;
+        if (k < count) out[static_cast<i64>(k) * Bp + j] = acc[kk];
+    }
+    if (wave >= 2) red[wave - 2][lane] = own;
+    __syncthreads();
+    if (wave == 0) {
+        out[static_cast<i64>(count) * Bp + j] = red[0][lane];
+        out[static_cast<i64>(count + 1) * Bp + j] = red[1][lane];
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_basis_update(double *w, const double *V, int count, const double *h, i64 n, i64 Bp,
+                   const int *active) {
+    const i64 total = n * Bp, ld = n * Bp;
+    for (i64 o = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; o < total;
+         o += static_cast<i64>(gridDim.x) * kThreads) {
+        const i64 j = o % Bp;
+        if (active && !active[j]) continue;
+        double wi = w[o];
+        for (int k = 0; k < count; ++k) wi -= h[static_cast<i64>(k) * Bp + j] * V[k * ld + o];
+        w[o] = wi;
+    }
+}
+
+template <typename ZT>
+__global__ void __launch_bounds__(kThreads)
+    k_combine(double *x, const ZT *Z, const int *cnt, const double *y, i64 n, i64 Bp) {
+    const i64 total = n * Bp, ld = n * Bp;
+    for (i64 o = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; o < total;
+         o += static_cast<i64>(gridDim.x) * kThreads) {
+        const i64 j = o % Bp;
+        const int c = cnt[j];
+        if (c <= 0) continue;
+        double s = x[o];
+        for (int k = 0; k < c; ++k) s += y[static_cast<i64>(k) * Bp + j] * static_cast<double>(Z[k * ld + o]);
+        x[o] = s;
+    }
+}
+
+// 64 x 64 tiles through LDS; grid (ceil(n / 64), Bp / 64)
+constexpr int kT = 64;
+
+__global__ void __launch_bounds__(kThreads)
+    k_pack(const double *src, i64 n, i64 nsys, double *dst, i64 Bp) {
+    __shared__ double t[kT][kT + 1];
+    const i64 i0 = static_cast<i64>(blockIdx.x) * kT, j0 = static_cast<i64>(blockIdx.y) * kT;
+    for (int e = threadIdx.x; e < kT * kT; e += kThreads) {
+        const int jj = e / kT, ii = e % kT; // read along states (contiguous in src)
+        const i64 i = i0 + ii, j = j0 + jj;
+        t[jj][ii] = (i < n && j < nsys) ? src[j * n + i] : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kT * kT; e += kThreads) {
+        const int ii = e / kT, jj = e % kT; // write along systems (contiguous in dst)
+        const i64 i = i0 + ii, j = j0 + jj;
+        if (i < n) dst[i * Bp + j] = t[jj][ii];
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_unpack(const double *src, i64 n, i64 nsys, double *dst, i64 Bp) {
+    __shared__ double t[kT][kT + 1];
+    const i64 i0 = static_cast<i64>(blockIdx.x) * kT, j0 = static_cast<i64>(blockIdx.y) * kT;
+    for (int e = threadIdx.x; e < kT * kT; e += kThreads) {
+        const int ii = e / kT, jj = e % kT; // read along systems
+        const i64 i = i0 + ii, j = j0 + jj;
+        t[ii][jj] = i < n ? src[i * Bp + j] : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kT * kT; e += kThreads) {
+        const int jj = e / kT, ii = e % kT; // write along states
+        const i64 i = i0 + ii, j = j0 + jj;
+        if (i < n && j < nsys) dst[j * n + i] = t[ii][jj];
+    }
+}
+
+} // namespace
+
+int parts(i64 n) { return op_parts(n); }
+
+template <typename XT>
+void shifted_apply(const csr_view<double> &A, const double *shifts, const XT *x, const double *b,
+                   double *y, double *partial, i64 Bp, const int *active, hipStream_t s) {
+    const bop_args<double, XT, double> a{A, shifts, 0.0, x, b, y, partial, Bp, active};
+    if (b) {
+        launch<bmode::shifted_residual>(a, s);
+    } else {
+        launch<bmode::shifted>(a, s);
+    }
+}
+
+template <typename T, typename IN>
+void jacobi_first(const csr_view<T> &A, const double *shifts, double omega, const IN *in,
+                  const double *scale, T *rhs_out, T *x, i64 Bp, const int *active,
+                  hipStream_t s) {
+    if (A.n == 0) return;
+    hipLaunchKernelGGL((k_jacobi_first<T, IN>), dim3(grid_for(A.n * Bp)), dim3(kThreads), 0, s, A,
+                       shifts, static_cast<T>(omega), in, scale, rhs_out, x, Bp, active);
+}
+
+template <typename T>
+void jacobi_sweep(const csr_view<T> &A, const double *shifts, double omega, const T *rhs,
+                  const T *x, T *x_out, i64 Bp, const int *active, hipStream_t s) {
+    launch<bmode::sweep>(bop_args<T, T, T>{A, shifts, static_cast<T>(omega), x, rhs, x_out, nullptr,
+                                           Bp, active}, s);
+}
+
+template <typename T>
+void level_residual(const csr_view<T> &A, const double *shifts, const T *rhs, const T *x, T *r,
+                    i64 Bp, const int *active, hipStream_t s) {
+    launch<bmode::residual>(bop_args<T, T, T>{A, shifts, T(0), x, rhs, r, nullptr, Bp, active}, s);
+}
+
+template <typename T>
+void gather_sum(const csr_view<T> &R, const T *x, T *y, i64 Bp, const int *active, hipStream_t s) {
+    launch<bmode::gather_sum>(bop_args<T, T, T>{R, nullptr, T(0), x, nullptr, y, nullptr, Bp, active}, s);
+}
+
+template <typename T>
+void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x, i64 Bp,
+                 const int *active, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_prolong_add<T>), dim3(grid_for(n * Bp)), dim3(kThreads), 0, s, n,
+                       coarse_of, weight, coarse, x, Bp, active);
+}
+
+template <typename T>
+void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
+                    const T *inverse, u64 per_shift, const int *shift_index, i64 max_size,
+                    const T *r, T *x, i64 Bp, const int *active, hipStream_t s) {
+    if (n_blocks == 0) return;
+    hipLaunchKernelGGL((k_terminal<T>), dim3(grid_for(n_blocks * max_size * Bp)), dim3(kThreads), 0,
+                       s, n_blocks, start, size, offset, inverse, per_shift, shift_index, max_size,
+                       r, x, Bp, active);
+}
+
+void multi_dot(const double *V, int count, const double *w, i64 n, i64 Bp, double *partial,
+               int np, const int *active, hipStream_t s) {
+    if (count <= 0 || n == 0) return;
+    hipLaunchKernelGGL(k_multi_dot, dim3(np, (count + kDotChunk - 1) / kDotChunk, Bp / kLanes),
+                       dim3(kThreads), 0, s, V, count, w, n, Bp, partial, np, active);
+}
+
+void reduce_partials(const double *partial, int np, i64 m, double *out, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_reduce, dim3(grid_for(m)), dim3(kThreads), 0, s, partial, np, m, out);
+}
+
+void dcgs_update(double *V, int count, const double *sc, const double *tg, const double *beta,
+                 const double *w, i64 n, i64 Bp, double *partial, int np, double *eta,
+                 const int *active, hipStream_t s) {
+    if (count < 0 || count >= kFusedBasis || n == 0) return;
+    const dim3 grid(np, Bp / kLanes);
+    switch ((count + kWaves - 1) / kWaves) {
+#define MK_BDCGS(P)                                                                              \
+    case P:                                                                                      \
+        hipLaunchKernelGGL(k_dcgs_update<P>, grid, dim3(kThreads), 0, s, V, count, sc, tg, beta, \
+                           w, n, Bp, partial, np, eta, active);                                  \
+        break;
+        MK_BDCGS(0) MK_BDCGS(1) MK_BDCGS(2) MK_BDCGS(3) MK_BDCGS(4) MK_BDCGS(5) MK_BDCGS(6)
+        MK_BDCGS(7) MK_BDCGS(8) MK_BDCGS(9) MK_BDCGS(10) MK_BDCGS(11) MK_BDCGS(12)
+        MK_BDCGS(13) MK_BDCGS(14) MK_BDCGS(15) MK_BDCGS(16)
+#undef MK_BDCGS
+    }
+}
+
+void basis_update(double *w, const double *V, int count, const double *h, i64 n, i64 Bp,
+                  const int *active, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_basis_update, dim3(grid_for(n * Bp)), dim3(kThreads), 0, s, w, V, count,
+                       h, n, Bp, active);
+}
+
+template <typename ZT>
+void combine(double *x, const ZT *Z, const int *cnt, const double *y, i64 n, i64 Bp,
+             hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_combine<ZT>), dim3(grid_for(n * Bp)), dim3(kThreads), 0, s, x, Z, cnt, y,
+                       n, Bp);
+}
+
+void pack(const double *src, i64 n, i64 nsys, double *dst, i64 Bp, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_pack, dim3((n + kT - 1) / kT, Bp / kT), dim3(kThreads), 0, s, src, n,
+                       nsys, dst, Bp);
+}
+
+void unpack(const double *src, i64 n, i64 nsys, double *dst, i64 Bp, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_unpack, dim3((n + kT - 1) / kT, Bp / kT), dim3(kThreads), 0, s, src, n,
+                       nsys, dst, Bp);
+}
+
+template void shifted_apply<double>(const csr_view<double> &, const double *, const double *,
+                                    const double *, double *, double *, i64, const int *,
+                                    hipStream_t);
+template void shifted_apply<float>(const csr_view<double> &, const double *, const float *,
+                                   const double *, double *, double *, i64, const int *,
+                                   hipStream_t);
+#define MK_BLEVEL(T)                                                                              \
+    template void jacobi_first<T, double>(const csr_view<T> &, const double *, double,            \
+                                          const double *, const double *, T *, T *, i64,         \
+                                          const int *, hipStream_t);                              \
+    template void jacobi_sweep<T>(const csr_view<T> &, const double *, double, const T *,         \
+                                  const T *, T *, i64, const int *, hipStream_t);                 \
+    template void level_residual<T>(const csr_view<T> &, const double *, const T *, const T *,    \
+                                    T *, i64, const int *, hipStream_t);                          \
+    template void gather_sum<T>(const csr_view<T> &, const T *, T *, i64, const int *,            \
+                                hipStream_t);                                                     \
+    template void prolong_add<T>(i64, const u32 *, const T *, const T *, T *, i64, const int *,   \
+                                 hipStream_t);                                                    \
+    template void terminal_solve<T>(i64, const u32 *, const u32 *, const u64 *, const T *, u64,   \
+                                    const int *, i64, const T *, T *, i64, const int *,          \
+                                    hipStream_t);                                                 \
+    template void combine<T>(double *, const T *, const int *, const double *, i64, i64,         \
+                             hipStream_t);
+MK_BLEVEL(float)
+MK_BLEVEL(double)
+#undef MK_BLEVEL
+template void jacobi_first<float, float>(const csr_view<float> &, const double *, double,
+                                         const float *, const double *, float *, float *, i64,
+                                         const int *, hipStream_t);
+
+} // namespace mk::gpu::batch

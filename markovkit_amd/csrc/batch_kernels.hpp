@@ -1,0 +1,81 @@
+// Device kernels of batched resolvent solves: many systems (shift s_j, right-hand side b_j)
+// over one state space, advanced together.
+//
+// Layout: a batched vector stores state i of system j at X[i * Bp + j], with Bp the number of
+// systems padded to a multiple of 64. A wavefront works on one state row for 64 consecutive
+// systems (lane = system): the row's operator entries are wave-uniform (scalar loads, read once
+// for all systems of the wave), and every gather x[col * Bp + j] is one contiguous 256-512 B
+// access. The operators are the single-system ones (coded sliced or plain, see kernels.hpp);
+// only the diagonal shift varies with the system.
+//
+// active (when not null) holds one int per system: lanes of inactive systems load and store
+// nothing and contribute zero partial sums. Reductions run in a fixed order.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace mk::gpu::batch {
+
+constexpr int kLanes = 64; // systems per wavefront
+
+// row partitions used by the batched reductions for an n-row operand
+int parts(i64 n);
+
+// y = (s_j + exit) o x - C x                       (mode shifted)
+// y = b - ((s_j + exit) o x - C x), partial sums of y^2 per system (mode shifted_residual):
+//   partial[p * Bp + j] for row partition p < parts(n)
+template <typename XT>
+void shifted_apply(const csr_view<double> &A, const double *shifts, const XT *x, const double *b,
+                   double *y, double *partial, i64 Bp, const int *active, hipStream_t s);
+
+// multigrid level operations in precision T (see kernels.hpp for the single-system forms)
+//   first sweep from zero: rhs = (T)(in * scale_j) (stored to rhs_out if given),
+//   x = w rhs / (s_j + exit)
+template <typename T, typename IN>
+void jacobi_first(const csr_view<T> &A, const double *shifts, double omega, const IN *in,
+                  const double *scale, T *rhs_out, T *x, i64 Bp, const int *active,
+                  hipStream_t s);
+template <typename T>
+void jacobi_sweep(const csr_view<T> &A, const double *shifts, double omega, const T *rhs,
+                  const T *x, T *x_out, i64 Bp, const int *active, hipStream_t s);
+template <typename T>
+void level_residual(const csr_view<T> &A, const double *shifts, const T *rhs, const T *x, T *r,
+                    i64 Bp, const int *active, hipStream_t s);
+template <typename T>
+void gather_sum(const csr_view<T> &R, const T *x, T *y, i64 Bp, const int *active, hipStream_t s);
+template <typename T>
+void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x, i64 Bp,
+                 const int *active, hipStream_t s);
+// dense terminal blocks with one inverse set per distinct shift: system j uses
+// inverse + shift_index[j] * per_shift
+template <typename T>
+void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
+                    const T *inverse, u64 per_shift, const int *shift_index, i64 max_size,
+                    const T *r, T *x, i64 Bp, const int *active, hipStream_t s);
+
+// ---- Krylov (FP64), count basis vectors V_k = V + k * n * Bp
+// partial[p * (count * Bp) + k * Bp + j] = sum over partition p's rows of V_k[i][j] w[i][j]
+void multi_dot(const double *V, int count, const double *w, i64 n, i64 Bp, double *partial,
+               int np, const int *active, hipStream_t s);
+// out[o] = sum_p partial[p * m + o] for o < m (fixed order)
+void reduce_partials(const double *partial, int np, i64 m, double *out, hipStream_t s);
+// delayed CGS2 step per system (count < kFusedBasis): V[count] holds u_j, w = A z_j;
+// sc[k * Bp + j] = (Q^T u)_k, tg[k * Bp + j] = (Q^T w)_k for k < count and tg[count * Bp + j] =
+// u^T w; beta[j] = |u - Q s|. Writes q to V[count], eta_j to eta[j], u' to V[count + 1], and
+// partial[p * ((count + 2) * Bp) + k * Bp + j] for k <= count + 1 as in the single-system form
+void dcgs_update(double *V, int count, const double *sc, const double *tg, const double *beta,
+                 const double *w, i64 n, i64 Bp, double *partial, int np, double *eta,
+                 const int *active, hipStream_t s);
+// w[i][j] -= sum_{k < count} h[k * Bp + j] V_k[i][j]
+void basis_update(double *w, const double *V, int count, const double *h, i64 n, i64 Bp,
+                  const int *active, hipStream_t s);
+// x[i][j] += sum_{k < cnt[j]} y[k * Bp + j] Z_k[i][j]
+template <typename ZT>
+void combine(double *x, const ZT *Z, const int *cnt, const double *y, i64 n, i64 Bp,
+             hipStream_t s);
+// system-major (src[j * n + i], j < nsys) to state-major (dst[i * Bp + j], zero for j >= nsys)
+void pack(const double *src, i64 n, i64 nsys, double *dst, i64 Bp, hipStream_t s);
+// state-major to system-major for j < nsys
+void unpack(const double *src, i64 n, i64 nsys, double *dst, i64 Bp, hipStream_t s);
+
+} // namespace mk::gpu::batch

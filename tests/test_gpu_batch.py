@@ -1,0 +1,188 @@
+"""GPU parity of batched resolvent solves (mk_batch_*): every system of a batch against the
+single-system solver and the FP64 CPU oracle on the same inputs, plus the lumped closed form of
+the sector sums for 1D linear birth-death boxes.
+
+Tolerances: every system must reach its own absolute tolerance on the true FP64 residual, and
+two solutions x, x' of the same system agree within the L1 bound
+||x - x'||_1 <= (||b - A x||_1 + ||b - A x'||_1) / s (rdme/cpp/rdme_full_multigrid.cpp:884-887).
+"""
+import numpy as np
+import pytest
+
+import markovkit_amd as mk
+from markovkit_amd.workloads import (annihilation_2d, birth_death_1d, config_multishift,
+                                     lumped_sector_sums)
+from oracle.oracle import Problem, SectorMultigrid as OracleMG
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(np.float64).eps
+
+
+def build(w):
+    box = mk.PopulationBox(w.n_voxels, w.max_totals)
+    Q = mk.Generator(box, w.model)
+    return box, Q
+
+
+def rhs_set(n, seeds):
+    out = []
+    for s in seeds:
+        b = np.random.default_rng(s).random(n)
+        out.append(b / b.sum())
+    return out
+
+
+def l1_residual(Q, s, b, x):
+    return np.abs(b - Q.apply_shifted(s, x)).sum()
+
+
+@pytest.fixture(scope="module")
+def small():
+    w = birth_death_1d(16, 3, 0.5, "random")
+    box, Q = build(w)
+    return w, box, Q
+
+
+@pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("restart", [50, 5])
+def test_batch_matches_single_solves(small, precision, restart):
+    w, box, Q = small
+    n = box.size()
+    shifts = [0.05, 0.5, 5.0, 50.0, 0.05, 0.5, 5.0, 50.0]
+    B = np.array(rhs_set(n, [0, 0, 0, 0, 1, 1, 1, 1]))
+    tol = 1e-10 * np.linalg.norm(B, axis=1)
+    M = mk.SectorMultigrid(w.model.domain, box, 0.5, mk.MultigridOptions(precision=precision))
+    batch = mk.BatchSolver(Q, M, shifts)
+    X, res = batch.solve(B, tolerances=tol, restart=restart, max_iterations=500)
+    for j, s in enumerate(shifts):
+        assert res[j].converged, (j, res[j])
+        r = B[j] - Q.apply_shifted(s, X[j])
+        assert np.linalg.norm(r) <= 1.0001 * tol[j], (j, np.linalg.norm(r), tol[j])
+        assert abs(res[j].residual - np.linalg.norm(r)) <= 1e-3 * tol[j]
+        Ms = mk.SectorMultigrid(w.model.domain, box, s, mk.MultigridOptions(precision=precision))
+        x1 = np.zeros(n)
+        r1 = mk.gmres(mk.ShiftedGenerator(Q, s), Ms, B[j], x1, tolerance=tol[j], restart=restart,
+                      max_iterations=500)
+        assert r1.converged
+        bound = (l1_residual(Q, s, B[j], X[j]) + l1_residual(Q, s, B[j], x1)) / s
+        assert np.abs(X[j] - x1).sum() <= bound, (j, np.abs(X[j] - x1).sum(), bound)
+        if precision == mk.FP64:
+            # same algorithm and preconditioner: the iteration counts agree up to rounding
+            assert abs(res[j].iterations - r1.iterations) <= 1, (j, res[j].iterations, r1.iterations)
+        print(f"[batch m={restart}] s={s} its={res[j].iterations} single its={r1.iterations}")
+
+
+def test_batch_against_oracle_and_lumped(small):
+    w, box, Q = small
+    n = box.size()
+    shifts = [0.1, 1.0, 10.0]
+    B = np.array(rhs_set(n, [3, 4, 5]))
+    tol = 1e-10 * np.linalg.norm(B, axis=1)
+    M = mk.SectorMultigrid(w.model.domain, box, 1.0, mk.MultigridOptions(precision=mk.FP32))
+    X, res = mk.BatchSolver(Q, M, shifts).solve(B, tolerances=tol)
+    P = Problem(w.model, w.max_totals)
+    begins = [box.sector_begin(k) for k in range(box.n_sectors())] + [n]
+    for j, s in enumerate(shifts):
+        assert res[j].converged
+        x_ref, info = P.gmres(s, B[j], mg=OracleMG(P, s), tol=tol[j], restart=50)
+        assert info["converged"]
+        r = B[j] - P.apply_shifted(s, X[j])
+        r_ref = B[j] - P.apply_shifted(s, x_ref)
+        bound = (np.abs(r).sum() + np.abs(r_ref).sum()) / s
+        assert np.abs(X[j] - x_ref).sum() <= bound
+        Bk = np.array([B[j][begins[k]:begins[k + 1]].sum() for k in range(4)])
+        Xk = np.array([X[j][begins[k]:begins[k + 1]].sum() for k in range(4)])
+        X_closed = lumped_sector_sums(s, 0.05, 0.02, 3, Bk)
+        assert np.abs(Xk - X_closed).sum() <= np.abs(r).sum() / s + 1e-15
+
+
+def test_batch_more_than_one_wave_of_systems(small):
+    """70 systems: two 64-lane groups, the second padded; repeated solves reuse the workspace
+    and a second call with other right-hand sides gives the same answers as a fresh batch."""
+    w, box, Q = small
+    n = box.size()
+    shifts = list(np.geomspace(0.02, 20.0, 70))
+    B = np.array(rhs_set(n, range(70)))
+    tol = 1e-10 * np.linalg.norm(B, axis=1)
+    M = mk.SectorMultigrid(w.model.domain, box, 1.0, mk.MultigridOptions(precision=mk.FP32))
+    batch = mk.BatchSolver(Q, M, shifts)
+    X1, res1 = batch.solve(B[::-1].copy(), tolerances=tol[::-1].copy())
+    X, res = batch.solve(B, tolerances=tol)
+    Xf, _ = mk.BatchSolver(Q, M, shifts).solve(B, tolerances=tol)
+    assert np.array_equal(X, Xf)
+    for j, s in enumerate(shifts):
+        assert res[j].converged and res1[j].converged
+        assert np.linalg.norm(B[j] - Q.apply_shifted(s, X[j])) <= 1.0001 * tol[j]
+        assert np.linalg.norm(B[::-1][j] - Q.apply_shifted(s, X1[j])) <= 1.0001 * tol[::-1][j]
+
+
+def test_batch_converged_guess_and_zero_iterations(small):
+    w, box, Q = small
+    n = box.size()
+    shifts = [0.5, 2.0]
+    B = np.array(rhs_set(n, [7, 8]))
+    tol = 1e-10 * np.linalg.norm(B, axis=1)
+    M = mk.SectorMultigrid(w.model.domain, box, 0.5, mk.MultigridOptions(precision=mk.FP64))
+    batch = mk.BatchSolver(Q, M, shifts)
+    X, res = batch.solve(B, tolerances=tol)
+    X2 = X.copy()
+    _, res2 = batch.solve(B, X2, tolerances=tol)
+    assert all(r.converged and r.iterations == 0 for r in res2)
+    assert np.array_equal(X, X2)
+    _, res3 = batch.solve(B, np.zeros_like(B), tolerances=tol, max_iterations=3)
+    assert all(not r.converged and r.iterations == 3 for r in res3)
+
+
+def test_batch_two_species_2d():
+    w = annihilation_2d(4, (2, 2), 1.0)
+    box, Q = build(w)
+    n = box.size()
+    shifts = [0.3, 1.0, 3.0]
+    B = np.array(rhs_set(n, [0, 1, 2]))
+    tol = 1e-10 * np.linalg.norm(B, axis=1)
+    M = mk.SectorMultigrid(w.model.domain, box, 1.0, mk.MultigridOptions(precision=mk.FP32))
+    X, res = mk.BatchSolver(Q, M, shifts).solve(B, tolerances=tol)
+    for j, s in enumerate(shifts):
+        assert res[j].converged
+        assert np.linalg.norm(B[j] - Q.apply_shifted(s, X[j])) <= 1.0001 * tol[j]
+        Ms = mk.SectorMultigrid(w.model.domain, box, s, mk.MultigridOptions(precision=mk.FP32))
+        x1 = np.zeros(n)
+        assert mk.gmres(mk.ShiftedGenerator(Q, s), Ms, B[j], x1, tolerance=tol[j]).converged
+        bound = (l1_residual(Q, s, B[j], X[j]) + l1_residual(Q, s, B[j], x1)) / s
+        assert np.abs(X[j] - x1).sum() <= bound
+
+
+def test_multishift_256_solves():
+    """1D, 32 voxels, cap 4 (58,905 states): 64 shifts 10^(-2 + 4 i / 63) x 4 random rhs, one
+    batch of 256 systems with the FP32 preconditioner; every system reaches 1e-10 relative
+    residual and its sector sums match the lumped closed form."""
+    w = config_multishift()
+    box, Q = build(w)
+    n = box.size()
+    assert n == 58905
+    shifts, B = [], []
+    rhs = rhs_set(n, w.seeds)
+    for s in w.shifts:
+        for b in rhs:
+            shifts.append(s)
+            B.append(b)
+    B = np.array(B)
+    tol = 1e-10 * np.linalg.norm(B, axis=1)
+    M = mk.SectorMultigrid(w.model.domain, box, 1.0, mk.MultigridOptions(precision=mk.FP32))
+    X, res = mk.BatchSolver(Q, M, shifts).solve(B, tolerances=tol)
+    begins = [box.sector_begin(k) for k in range(box.n_sectors())] + [n]
+    absA = 2 * Q.exit_rates() - Q.leak_rates()
+    its = [r.iterations for r in res]
+    print(f"[multishift] iterations min {min(its)} max {max(its)}")
+    for j, s in enumerate(shifts):
+        assert res[j].converged, (j, res[j])
+        r = B[j] - Q.apply_shifted(s, X[j])
+        assert np.linalg.norm(r) <= 1.0001 * tol[j]
+        Bk = np.array([B[j][begins[k]:begins[k + 1]].sum() for k in range(5)])
+        Xk = np.array([X[j][begins[k]:begins[k + 1]].sum() for k in range(5)])
+        X_closed = lumped_sector_sums(s, 0.05, 0.02, 4, Bk)
+        # ||x - x*||_1 <= ||b - A x||_1 / s, plus the FP64 evaluation error of b - A x
+        # (about eps * sum_j (s + 2 exit_j - leak_j) |x_j|)
+        bound = (np.abs(r).sum() + 16 * EPS * ((s + absA) * np.abs(X[j])).sum()) / s
+        assert np.abs(Xk - X_closed).sum() <= bound, (j, np.abs(Xk - X_closed).sum(), bound)

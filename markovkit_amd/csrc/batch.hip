@@ -39,7 +39,7 @@ struct mk_batch_s {
     // Krylov workspace for restart m
     i64 m = 0;
     int np = 1;
-    dbuf V, Z, w, partial, hd, d_b, d_x, d_io, d_scale, d_beta, d_mask, d_active, d_cnt, d_y;
+    dbuf V, Z, w, partial, rpart, rtmp, hd, d_b, d_x, d_io, d_scale, d_beta, d_mask, d_active, d_cnt, d_y;
     double *host = nullptr; // pinned: hd mirror, y, scale, beta
     int *ihost = nullptr;   // pinned: mask, active, cnt
     ~mk_batch_s() {
@@ -206,6 +206,9 @@ void ensure_workspace(mk_batch_s &B, i64 m) {
     B.w = dbuf(vec * sizeof(double));
     B.np = bk::parts(n);
     B.partial = dbuf(static_cast<std::size_t>(B.np) * (m + 2) * Bp * sizeof(double));
+    B.rpart = dbuf(static_cast<std::size_t>(bk::residual_parts(n)) * Bp * sizeof(double));
+    B.rtmp = dbuf(std::max<i64>(1, bk::reduce_tmp_size(static_cast<int>(bk::residual_parts(n)), Bp)) *
+                  sizeof(double));
     const slots sl{(m + 2) * Bp};
     B.hd = dbuf(sl.total(Bp) * sizeof(double));
     B.d_scale = dbuf(Bp * sizeof(double));
@@ -256,10 +259,13 @@ void bgmres(mk_batch_s &B, const double *b, double *x, const std::vector<double>
     std::vector<double> beta(Bp, 0.0);
     auto true_residual = [&]() {
         push_ints(active_h, d_active);
+        double *rp = dp<double>(B.rpart);
+        const int nrp = static_cast<int>(bk::residual_parts(n));
         P.run(K_SPMV, spmv_bytes + nb * 8.0, [&] {
-            bk::shifted_apply<double>(A.A.view, sh, x, b, V, partial, Bp, d_active, s);
+            bk::shifted_apply<double>(A.A.view, sh, x, b, V, rp, Bp, d_active, s);
         });
-        P.run(K_DOT, double(np) * Bp * 8.0, [&] { bk::reduce_partials(partial, np, Bp, hd + sl.rn(Bp), s); });
+        P.run(K_DOT, double(nrp) * Bp * 8.0,
+              [&] { bk::reduce_partials(rp, nrp, Bp, hd + sl.rn(Bp), s, dp<double>(B.rtmp)); });
         HIPCHK(hipMemcpyAsync(host + sl.rn(Bp), hd + sl.rn(Bp), Bp * sizeof(double), hipMemcpyDeviceToHost, s));
         drain();
         for (i64 j = 0; j < nsys; ++j)

@@ -12,7 +12,6 @@ constexpr int kWaves = kThreads / kLanes;
 constexpr int kEntryUnroll = 8;
 constexpr int kRowsPerPart = 256;
 constexpr int kMaxParts = 512;
-constexpr int kSlotsMax = kFusedBasis / kWaves; // basis slots per wave in the update kernel
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
@@ -78,39 +77,39 @@ __device__ __forceinline__ T row_gather(const csr_view<VT> &A, i64 r, const XT *
     return sum;
 }
 
-// grid (row partitions, Bp / 64); wave wv of partition p takes rows p * 4 + wv + k * 4 * np
+// grid (ceil(n / 4), Bp / 64): one state row per wavefront, so the dependent entry -> gather
+// chains of many rows are in flight at once
 template <bmode M, typename VT, typename XT, typename T>
 __global__ void __launch_bounds__(kThreads) k_bop(bop_args<VT, XT, T> a) {
     __shared__ double red[kWaves][kLanes];
     const int lane = threadIdx.x & 63, wave = wave_id();
     const i64 j = static_cast<i64>(blockIdx.y) * kLanes + lane;
     const bool on = !a.active || a.active[j] != 0;
-    T sh = 0;
-    if constexpr (M != bmode::gather_sum) sh = static_cast<T>(a.shifts[j]);
+    const i64 r = static_cast<i64>(blockIdx.x) * kWaves + wave;
     double sq = 0.0;
-    const i64 stride = static_cast<i64>(gridDim.x) * kWaves;
-    for (i64 r = static_cast<i64>(blockIdx.x) * kWaves + wave; r < a.A.n; r += stride) {
+    if (r < a.A.n) {
         T ex = 0;
         const T sum = row_gather<VT, XT, T>(a.A, r, a.x, a.Bp, j, on, ex);
-        if (!on) continue;
-        const i64 o = r * a.Bp + j;
-        if constexpr (M == bmode::gather_sum) {
-            a.out[o] = sum;
-        } else {
-            const T xr = static_cast<T>(a.x[o]);
-            const T d = sh + ex;
-            const T ax = d * xr - sum;
-            if constexpr (M == bmode::shifted) {
-                a.out[o] = ax;
-            } else if constexpr (M == bmode::shifted_residual) {
-                const T res = a.rhs[o] - ax;
-                a.out[o] = res;
-                sq += static_cast<double>(res) * static_cast<double>(res);
-            } else if constexpr (M == bmode::sweep) {
-                const T res = a.rhs[o] - ax;
-                a.out[o] = xr + a.omega * res / d;
+        if (on) {
+            const i64 o = r * a.Bp + j;
+            if constexpr (M == bmode::gather_sum) {
+                a.out[o] = sum;
             } else {
-                a.out[o] = a.rhs[o] - ax;
+                const T xr = static_cast<T>(a.x[o]);
+                const T d = static_cast<T>(a.shifts[j]) + ex;
+                const T ax = d * xr - sum;
+                if constexpr (M == bmode::shifted) {
+                    a.out[o] = ax;
+                } else if constexpr (M == bmode::shifted_residual) {
+                    const T res = a.rhs[o] - ax;
+                    a.out[o] = res;
+                    sq = static_cast<double>(res) * static_cast<double>(res);
+                } else if constexpr (M == bmode::sweep) {
+                    const T res = a.rhs[o] - ax;
+                    a.out[o] = xr + a.omega * res / d;
+                } else {
+                    a.out[o] = a.rhs[o] - ax;
+                }
             }
         }
     }
@@ -123,15 +122,17 @@ __global__ void __launch_bounds__(kThreads) k_bop(bop_args<VT, XT, T> a) {
     }
 }
 
-int op_parts(i64 n) {
-    return static_cast<int>(std::max<i64>(1, std::min<i64>((n + kRowsPerPart - 1) / kRowsPerPart, kMaxParts)));
-}
+i64 row_blocks(i64 n) { return std::max<i64>(1, (n + kWaves - 1) / kWaves); }
 
 template <bmode M, typename VT, typename XT, typename T>
 void launch(const bop_args<VT, XT, T> &a, hipStream_t s) {
     if (a.A.n == 0) return;
-    hipLaunchKernelGGL((k_bop<M, VT, XT, T>), dim3(op_parts(a.A.n), a.Bp / kLanes), dim3(kThreads),
-                       0, s, a);
+    hipLaunchKernelGGL((k_bop<M, VT, XT, T>), dim3(static_cast<u32>(row_blocks(a.A.n)), a.Bp / kLanes),
+                       dim3(kThreads), 0, s, a);
+}
+
+int op_parts(i64 n) {
+    return static_cast<int>(std::max<i64>(1, std::min<i64>((n + kRowsPerPart - 1) / kRowsPerPart, kMaxParts)));
 }
 
 template <typename T>
@@ -144,34 +145,35 @@ int grid_for(i64 n) {
     return static_cast<int>(std::max<i64>(1, std::min<i64>(g, 16384)));
 }
 
+// elementwise kernels: grid (ceil(n / 4), Bp / 64), wavefront = row, lane = system
+__device__ __forceinline__ void row_lane(i64 &r, i64 &j) {
+    r = static_cast<i64>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+    j = static_cast<i64>(blockIdx.y) * kLanes + (threadIdx.x & 63);
+}
+
 template <typename T, typename IN>
 __global__ void __launch_bounds__(kThreads)
     k_jacobi_first(csr_view<T> A, const double *shifts, T omega, const IN *in,
                    const double *scale, T *rhs_out, T *x, i64 Bp, const int *active) {
-    const i64 total = A.n * Bp;
-    for (i64 o = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; o < total;
-         o += static_cast<i64>(gridDim.x) * kThreads) {
-        const i64 r = o / Bp, j = o - r * Bp;
-        if (active && !active[j]) continue;
-        const IN sc = scale ? static_cast<IN>(scale[j]) : IN(1);
-        const T b = static_cast<T>(in[o] * sc);
-        if (rhs_out) rhs_out[o] = b;
-        x[o] = omega * b / (static_cast<T>(shifts[j]) + row_exit(A, r));
-    }
+    i64 r, j;
+    row_lane(r, j);
+    if (r >= A.n || (active && !active[j])) return;
+    const i64 o = r * Bp + j;
+    const IN sc = scale ? static_cast<IN>(scale[j]) : IN(1);
+    const T b = static_cast<T>(in[o] * sc);
+    if (rhs_out) rhs_out[o] = b;
+    x[o] = omega * b / (static_cast<T>(shifts[j]) + row_exit(A, r));
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x, i64 Bp,
                   const int *active) {
-    const i64 total = n * Bp;
-    for (i64 o = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; o < total;
-         o += static_cast<i64>(gridDim.x) * kThreads) {
-        const i64 i = o / Bp, j = o - i * Bp;
-        if (active && !active[j]) continue;
-        const u32 c = coarse_of[i];
-        if (c != 0xFFFFFFFFu) x[o] += weight[i] * coarse[static_cast<i64>(c) * Bp + j];
-    }
+    i64 i, j;
+    row_lane(i, j);
+    if (i >= n || (active && !active[j])) return;
+    const u32 c = coarse_of[i];
+    if (c != 0xFFFFFFFFu) x[i * Bp + j] += weight[i] * coarse[static_cast<i64>(c) * Bp + j];
 }
 
 template <typename T>
@@ -229,6 +231,27 @@ __global__ void __launch_bounds__(kThreads)
             partial[static_cast<i64>(blockIdx.x) * m + (k0 + k) * Bp + j] =
                 ((red[0][k][lane] + red[1][k][lane]) + red[2][k][lane]) + red[3][k][lane];
     }
+}
+
+// first stage of a long reduction: tmp[g * m + o] = sum of partitions g * kGroup ..
+// (g + 1) * kGroup - 1 for output o; grid (ceil(np / kGroup), ceil(m / 64)), waves split the
+// group's partitions, lanes own outputs
+constexpr int kGroup = 256;
+
+__global__ void __launch_bounds__(kThreads)
+    k_reduce_groups(const double *partial, int np, i64 m, double *tmp) {
+    __shared__ double red[kWaves][kLanes];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const i64 o = static_cast<i64>(blockIdx.y) * kLanes + lane;
+    const int p0 = blockIdx.x * kGroup, p1 = min(np, p0 + kGroup);
+    double acc = 0.0;
+    if (o < m)
+        for (int p = p0 + wave; p < p1; p += kWaves) acc += partial[static_cast<i64>(p) * m + o];
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && o < m)
+        tmp[static_cast<i64>(blockIdx.x) * m + o] =
+            ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -325,30 +348,27 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void __launch_bounds__(kThreads)
     k_basis_update(double *w, const double *V, int count, const double *h, i64 n, i64 Bp,
                    const int *active) {
-    const i64 total = n * Bp, ld = n * Bp;
-    for (i64 o = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; o < total;
-         o += static_cast<i64>(gridDim.x) * kThreads) {
-        const i64 j = o % Bp;
-        if (active && !active[j]) continue;
-        double wi = w[o];
-        for (int k = 0; k < count; ++k) wi -= h[static_cast<i64>(k) * Bp + j] * V[k * ld + o];
-        w[o] = wi;
-    }
+    i64 r, j;
+    row_lane(r, j);
+    if (r >= n || (active && !active[j])) return;
+    const i64 o = r * Bp + j, ld = n * Bp;
+    double wi = w[o];
+    for (int k = 0; k < count; ++k) wi -= h[static_cast<i64>(k) * Bp + j] * V[k * ld + o];
+    w[o] = wi;
 }
 
 template <typename ZT>
 __global__ void __launch_bounds__(kThreads)
     k_combine(double *x, const ZT *Z, const int *cnt, const double *y, i64 n, i64 Bp) {
-    const i64 total = n * Bp, ld = n * Bp;
-    for (i64 o = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; o < total;
-         o += static_cast<i64>(gridDim.x) * kThreads) {
-        const i64 j = o % Bp;
-        const int c = cnt[j];
-        if (c <= 0) continue;
-        double s = x[o];
-        for (int k = 0; k < c; ++k) s += y[static_cast<i64>(k) * Bp + j] * static_cast<double>(Z[k * ld + o]);
-        x[o] = s;
-    }
+    i64 r, j;
+    row_lane(r, j);
+    if (r >= n) return;
+    const int c = cnt[j];
+    if (c <= 0) return;
+    const i64 o = r * Bp + j, ld = n * Bp;
+    double s = x[o];
+    for (int k = 0; k < c; ++k) s += y[static_cast<i64>(k) * Bp + j] * static_cast<double>(Z[k * ld + o]);
+    x[o] = s;
 }
 
 // 64 x 64 tiles through LDS; grid (ceil(n / 64), Bp / 64)
@@ -392,6 +412,8 @@ __global__ void __launch_bounds__(kThreads)
 
 int parts(i64 n) { return op_parts(n); }
 
+i64 residual_parts(i64 n) { return row_blocks(n); }
+
 template <typename XT>
 void shifted_apply(const csr_view<double> &A, const double *shifts, const XT *x, const double *b,
                    double *y, double *partial, i64 Bp, const int *active, hipStream_t s) {
@@ -408,7 +430,8 @@ void jacobi_first(const csr_view<T> &A, const double *shifts, double omega, cons
                   const double *scale, T *rhs_out, T *x, i64 Bp, const int *active,
                   hipStream_t s) {
     if (A.n == 0) return;
-    hipLaunchKernelGGL((k_jacobi_first<T, IN>), dim3(grid_for(A.n * Bp)), dim3(kThreads), 0, s, A,
+    hipLaunchKernelGGL((k_jacobi_first<T, IN>), dim3(static_cast<u32>(row_blocks(A.n)), Bp / kLanes),
+                       dim3(kThreads), 0, s, A,
                        shifts, static_cast<T>(omega), in, scale, rhs_out, x, Bp, active);
 }
 
@@ -434,7 +457,8 @@ template <typename T>
 void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x, i64 Bp,
                  const int *active, hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL((k_prolong_add<T>), dim3(grid_for(n * Bp)), dim3(kThreads), 0, s, n,
+    hipLaunchKernelGGL((k_prolong_add<T>), dim3(static_cast<u32>(row_blocks(n)), Bp / kLanes),
+                       dim3(kThreads), 0, s, n,
                        coarse_of, weight, coarse, x, Bp, active);
 }
 
@@ -455,10 +479,20 @@ void multi_dot(const double *V, int count, const double *w, i64 n, i64 Bp, doubl
                        dim3(kThreads), 0, s, V, count, w, n, Bp, partial, np, active);
 }
 
-void reduce_partials(const double *partial, int np, i64 m, double *out, hipStream_t s) {
+void reduce_partials(const double *partial, int np, i64 m, double *out, hipStream_t s,
+                     double *tmp) {
     if (m <= 0) return;
+    if (tmp && np > kGroup) {
+        const int ng = (np + kGroup - 1) / kGroup;
+        hipLaunchKernelGGL(k_reduce_groups, dim3(ng, static_cast<u32>((m + kLanes - 1) / kLanes)),
+                           dim3(kThreads), 0, s, partial, np, m, tmp);
+        hipLaunchKernelGGL(k_reduce, dim3(grid_for(m)), dim3(kThreads), 0, s, tmp, ng, m, out);
+        return;
+    }
     hipLaunchKernelGGL(k_reduce, dim3(grid_for(m)), dim3(kThreads), 0, s, partial, np, m, out);
 }
+
+i64 reduce_tmp_size(int np, i64 m) { return np > kGroup ? (np + kGroup - 1) / kGroup * m : 0; }
 
 void dcgs_update(double *V, int count, const double *sc, const double *tg, const double *beta,
                  const double *w, i64 n, i64 Bp, double *partial, int np, double *eta,
@@ -481,7 +515,8 @@ void dcgs_update(double *V, int count, const double *sc, const double *tg, const
 void basis_update(double *w, const double *V, int count, const double *h, i64 n, i64 Bp,
                   const int *active, hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_basis_update, dim3(grid_for(n * Bp)), dim3(kThreads), 0, s, w, V, count,
+    hipLaunchKernelGGL(k_basis_update, dim3(static_cast<u32>(row_blocks(n)), Bp / kLanes),
+                       dim3(kThreads), 0, s, w, V, count,
                        h, n, Bp, active);
 }
 
@@ -489,7 +524,8 @@ template <typename ZT>
 void combine(double *x, const ZT *Z, const int *cnt, const double *y, i64 n, i64 Bp,
              hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL((k_combine<ZT>), dim3(grid_for(n * Bp)), dim3(kThreads), 0, s, x, Z, cnt, y,
+    hipLaunchKernelGGL((k_combine<ZT>), dim3(static_cast<u32>(row_blocks(n)), Bp / kLanes),
+                       dim3(kThreads), 0, s, x, Z, cnt, y,
                        n, Bp);
 }
 

@@ -21,9 +21,12 @@ constexpr int kLanes = 64; // systems per wavefront
 // row partitions used by the batched reductions for an n-row operand
 int parts(i64 n);
 
+// partial sums of the residual kernel: one per 4-row block
+i64 residual_parts(i64 n);
+
 // y = (s_j + exit) o x - C x                       (mode shifted)
 // y = b - ((s_j + exit) o x - C x), partial sums of y^2 per system (mode shifted_residual):
-//   partial[p * Bp + j] for row partition p < parts(n)
+//   partial[p * Bp + j] for p < residual_parts(n)
 template <typename XT>
 void shifted_apply(const csr_view<double> &A, const double *shifts, const XT *x, const double *b,
                    double *y, double *partial, i64 Bp, const int *active, hipStream_t s);
@@ -57,8 +60,11 @@ void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *
 // partial[p * (count * Bp) + k * Bp + j] = sum over partition p's rows of V_k[i][j] w[i][j]
 void multi_dot(const double *V, int count, const double *w, i64 n, i64 Bp, double *partial,
                int np, const int *active, hipStream_t s);
-// out[o] = sum_p partial[p * m + o] for o < m (fixed order)
-void reduce_partials(const double *partial, int np, i64 m, double *out, hipStream_t s);
+// out[o] = sum_p partial[p * m + o] for o < m (fixed order); with tmp (reduce_tmp_size doubles)
+// a long reduction runs in two stages over groups of 256 partitions
+void reduce_partials(const double *partial, int np, i64 m, double *out, hipStream_t s,
+                     double *tmp = nullptr);
+i64 reduce_tmp_size(int np, i64 m);
 // delayed CGS2 step per system (count < kFusedBasis): V[count] holds u_j, w = A z_j;
 // sc[k * Bp + j] = (Q^T u)_k, tg[k * Bp + j] = (Q^T w)_k for k < count and tg[count * Bp + j] =
 // u^T w; beta[j] = |u - Q s|. Writes q to V[count], eta_j to eta[j], u' to V[count + 1], and

@@ -80,6 +80,60 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
             }
         }
         lv.A = upload_csr<T>(cat);
+        // aggregate-block smoother data of the sectors that are not terminal on this level
+        i64 n_agg = 0;
+        for (i64 k = 0; k < K; ++k)
+            if (offset[l][k] >= 0 && H[k].levels[l].has_blocks)
+                n_agg = std::max<i64>(n_agg, H[k].levels[l].blocks.aggregates);
+        if (n_agg > 0) {
+            std::vector<u32> members, member_block, block_first{0}, rows, jrows;
+            std::vector<u64> inv_off;
+            std::vector<T> inv;
+            lv.h_agg_members.assign(1, 0);
+            lv.h_agg_rows.assign(1, 0);
+            for (i64 a = 0; a < n_agg; ++a) {
+                for (i64 k = 0; k < K; ++k) {
+                    if (offset[l][k] < 0 || !H[k].levels[l].has_blocks) continue;
+                    const auto &bs = H[k].levels[l].blocks;
+                    if (bs.aggregates != n_agg)
+                        throw std::logic_error("sectors of one level disagree on the aggregates");
+                    const u32 base = static_cast<u32>(offset[l][k]);
+                    for (u64 b = bs.agg_blocks[a]; b < bs.agg_blocks[a + 1]; ++b) {
+                        const u64 f = bs.block_first[b], e = bs.block_first[b + 1];
+                        const u64 m = e - f;
+                        const u32 bid = static_cast<u32>(block_first.size() - 1);
+                        for (u64 g = f; g < e; ++g) {
+                            members.push_back(base + bs.members[g]);
+                            member_block.push_back(bid);
+                        }
+                        block_first.push_back(static_cast<u32>(members.size()));
+                        inv_off.push_back(inv.size());
+                        inv.insert(inv.end(), bs.inverse.begin() + bs.inv_offset[b],
+                                   bs.inverse.begin() + bs.inv_offset[b] + m * m);
+                    }
+                    for (u64 q = bs.agg_rows[a]; q < bs.agg_rows[a + 1]; ++q) rows.push_back(base + bs.rows[q]);
+                }
+                lv.h_agg_members.push_back(members.size());
+                lv.h_agg_rows.push_back(rows.size());
+            }
+            for (i64 k = 0; k < K; ++k) {
+                if (offset[l][k] < 0 || H[k].levels[l].has_blocks) continue;
+                for (i64 i = 0; i < H[k].levels[l].rows; ++i) jrows.push_back(static_cast<u32>(offset[l][k] + i));
+            }
+            if (members.size() >= (u64{1} << 32) || rows.size() >= (u64{1} << 32))
+                throw std::overflow_error("aggregate-block smoother: too many block members");
+            lv.n_agg = n_agg;
+            lv.bs_members = upload(members);
+            lv.bs_member_block = upload(member_block);
+            lv.bs_block_first = upload(block_first);
+            lv.bs_inv_off = upload(inv_off);
+            lv.bs_inv = upload(inv);
+            lv.bs_rows = upload(rows);
+            lv.escr = dbuf(static_cast<std::size_t>(std::max<i64>(lv.n, 1)) * sizeof(T));
+            HIPCHK(hipMemset(lv.escr.p, 0, lv.escr.bytes));
+            lv.n_jrows = static_cast<i64>(jrows.size());
+            if (lv.n_jrows > 0) lv.jrows = upload(jrows);
+        }
         lv.n_term = static_cast<i64>(tstart.size());
         lv.h_tstart = tstart;
         lv.h_tsize = tsize;
@@ -125,6 +179,52 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
     }
 }
 
+// aggregate-block smoothing (multigrid.hpp:309-342) of the level's block sectors: block
+// Gauss-Seidel over the aggregates (in reverse for post-smoothing), block Jacobi within one
+// aggregate, the residual kept in lv.res and corrected where each aggregate's update reaches;
+// rows of sectors terminal on this level get Jacobi sweeps instead
+template <typename T>
+void block_smooth(const mk_sector_mg_s &mg, level_dev<T> &lv, const T *rhs, T *x, T *tmp,
+                  int sweeps, bool reverse, bool from_zero) {
+    if (sweeps <= 0) return;
+    auto &P = prof();
+    const hipStream_t s = stream();
+    const double shift = mg.shift, omega = mg.opt.jacobi_weight;
+    const std::size_t ts = sizeof(T);
+    T *r = dp<T>(lv.res);
+    if (from_zero) {
+        P.run(K_OTHER, double(lv.n) * 2 * ts,
+              [&] { HIPCHK(hipMemcpyAsync(r, rhs, lv.n * ts, hipMemcpyDeviceToDevice, s)); });
+    } else {
+        P.run(K_RESIDUAL, op_bytes(lv.A, ts, ts, 2),
+              [&] { gpu::level_residual<T>(lv.A.view, shift, rhs, x, r, s); });
+    }
+    const u32 *mem = dp<u32>(lv.bs_members), *rows = dp<u32>(lv.bs_rows);
+    T *escr = dp<T>(lv.escr);
+    for (int sweep = 0; sweep < sweeps; ++sweep) {
+        for (i64 step = 0; step < lv.n_agg; ++step) {
+            const i64 a = reverse ? lv.n_agg - 1 - step : step;
+            const u64 m0 = lv.h_agg_members[a], m1 = lv.h_agg_members[a + 1];
+            const u64 q0 = lv.h_agg_rows[a], q1 = lv.h_agg_rows[a + 1];
+            P.run(K_SWEEP, double(m1 - m0) * (3 * ts + 12), [&] {
+                gpu::block_solve<T>(mem, dp<u32>(lv.bs_member_block), dp<u32>(lv.bs_block_first),
+                                    dp<u64>(lv.bs_inv_off), dp<T>(lv.bs_inv), m0, m1, r, x, escr, s);
+            });
+            P.run(K_SWEEP, double(q1 - q0) * (2 * ts + 4), [&] {
+                gpu::block_residual<T>(lv.A.view, shift, rows, q0, q1, escr, r, s);
+                gpu::block_zero<T>(mem, m0, m1, escr, s);
+            });
+        }
+        if (lv.n_jrows > 0) {
+            P.run(K_SWEEP, double(lv.n_jrows) * 4 * ts, [&] {
+                gpu::sweep_rows<T>(lv.A.view, shift, omega, rhs, x, tmp, dp<u32>(lv.jrows),
+                                   static_cast<u64>(lv.n_jrows), s);
+                gpu::copy_rows<T>(dp<u32>(lv.jrows), static_cast<u64>(lv.n_jrows), tmp, x, s);
+            });
+        }
+    }
+}
+
 // one cycle of level l from a zero guess (population_multigrid::cycle, multigrid.hpp:201-228)
 // on all active sectors at once. rhs is this level's right-hand side; rhs64 (level 0 only) is an
 // FP64 input that the first sweep multiplies by scale and stores into lv.rhs in precision T. The
@@ -141,7 +241,17 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
     const double sweep_bytes = op_bytes(lv.A, ts, ts, 2);
     T *cur = dp<T>(lv.x);
     T *other = dp<T>(lv.xt);
-    if (rhs64) {
+    const bool blocks = lv.n_agg > 0;
+    if (blocks) {
+        if (rhs64) {
+            T *rb = dp<T>(lv.rhs);
+            P.run(K_OTHER, double(n) * (8 + ts),
+                  [&] { gpu::convert<double, T>(n, rhs64, rb, s, scale); });
+            rhs = rb;
+        }
+        P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
+        block_smooth<T>(mg, lv, rhs, cur, other, lv.pre, false, true);
+    } else if (rhs64) {
         T *rb = dp<T>(lv.rhs);
         if (lv.pre > 0) {
             P.run(K_FIRST, double(n) * (8 + 3 * ts), [&] {
@@ -157,8 +267,8 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             gpu::jacobi_first<T, T>(lv.A.view, shift, omega, rhs, nullptr, cur, s);
         });
     }
-    if (lv.pre == 0) P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
-    for (int k = 1; k < lv.pre; ++k) {
+    if (!blocks && lv.pre == 0) P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
+    for (int k = 1; !blocks && k < lv.pre; ++k) {
         P.run(K_SWEEP, sweep_bytes, [&] {
             gpu::jacobi_sweep<T>(lv.A.view, shift, omega, rhs, cur, other, s);
         });
@@ -207,7 +317,8 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             });
         }
     }
-    for (int k = 0; k < lv.post; ++k) {
+    if (blocks) block_smooth<T>(mg, lv, rhs, cur, other, lv.post, true, false);
+    for (int k = 0; !blocks && k < lv.post; ++k) {
         T *dst = (k == lv.post - 1 && out) ? out : other;
         P.run(K_SWEEP, sweep_bytes, [&] {
             gpu::jacobi_sweep<T>(lv.A.view, shift, omega, rhs, cur, dst, s);
@@ -715,7 +826,8 @@ mk_status mk_sector_mg_create(mk_domain domain, mk_box box, double shift,
         mk_multigrid_options o;
         mk_default_multigrid_options(&o);
         if (options) o = *options;
-        require(o.smoother == MK_JACOBI, "sector_multigrid: only the Jacobi smoother runs on the device");
+        require(o.smoother == MK_JACOBI || o.smoother == MK_AGGREGATE_BLOCK,
+                "sector_multigrid: unknown smoother");
         require(o.precision == MK_FP64 || o.precision == MK_FP32, "sector_multigrid: unknown precision");
         require(o.pre_sweeps >= 0 && o.post_sweeps >= 0 && o.coarse_cycles >= 1,
                 "sector_multigrid: bad sweep or cycle counts");
@@ -731,6 +843,7 @@ mk_status mk_sector_mg_create(mk_domain domain, mk_box box, double shift,
         mg->opt.max_levels = o.max_levels;
         mg->opt.coarse_cycles = o.coarse_cycles;
         mg->opt.sweep_growth = o.sweep_growth;
+        mg->opt.smoother = o.smoother;
         mg->shift = shift;
         mg->precision = o.precision;
         mg->rows = space.size();

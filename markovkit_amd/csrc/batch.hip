@@ -452,6 +452,7 @@ mk_status mk_batch_create(mk_generator gen, mk_sector_mg mg, int64_t n_systems,
     return guarded([&] {
         require(gen && mg && shifts && out && n_systems > 0, "batch: bad arguments");
         require(mg->rows == gen->n, "batch: the preconditioner does not match the operator");
+        require(mg->opt.smoother == MK_JACOBI, "batch: only the Jacobi smoother is batched");
         for (i64 j = 0; j < n_systems; ++j)
             if (!(shifts[j] > 0.0)) throw std::invalid_argument("population_multigrid: shift must be positive");
         auto B = std::make_unique<mk_batch_s>();

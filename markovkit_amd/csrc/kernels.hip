@@ -260,6 +260,83 @@ __global__ void __launch_bounds__(kThreads)
     x[start[b] + i] += s;
 }
 
+// ---- aggregate-block smoother (thread per listed row)
+// sum over row i's entries of rate * x[column], ascending entry order, and row i's exit rate
+template <typename T>
+__device__ __forceinline__ T gather_row(const csr_view<T> &A, u32 i, const T *x, T &exit_i) {
+    T sum = 0;
+    if (A.coded) {
+        const u32 len = A.rlen[i];
+        const u32 base = A.soff[i >> 6] + (i & 63u);
+        exit_i = A.edict[A.rcode[i]];
+        for (u32 k = 0; k < len; ++k) {
+            const u32 e = A.ent[base + k * kSlice];
+            sum += A.vdict[e >> kColBits] * x[e & kColMask];
+        }
+    } else {
+        exit_i = A.exit[i];
+        for (u32 p = A.ptr[i]; p < A.ptr[i + 1]; ++p) sum += A.val[p] * x[A.col[p]];
+    }
+    return sum;
+}
+
+// e = inverse_b r[members of b] for every member of the aggregate's blocks; x += e, escr = e
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_block_solve(const u32 *members, const u32 *member_block, const u32 *block_first,
+                  const u64 *inv_off, const T *inv, u64 m0, u64 m1, const T *r, T *x, T *escr) {
+    const u64 g = m0 + blockIdx.x * static_cast<u64>(kThreads) + threadIdx.x;
+    if (g >= m1) return;
+    const u32 b = member_block[g];
+    const u32 f = block_first[b], m = block_first[b + 1] - f;
+    const T *row = inv + inv_off[b] + static_cast<u64>(g - f) * m;
+    T e = 0;
+    for (u32 c = 0; c < m; ++c) e += row[c] * r[members[f + c]];
+    const u32 i = members[g];
+    x[i] += e;
+    escr[i] = e;
+}
+
+// r -= A e on the rows the aggregate's correction reaches (e = escr, zero off the members)
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_block_residual(csr_view<T> A, T shift, const u32 *rows, u64 q0, u64 q1, const T *escr, T *r) {
+    const u64 q = q0 + blockIdx.x * static_cast<u64>(kThreads) + threadIdx.x;
+    if (q >= q1) return;
+    const u32 i = rows[q];
+    T ex = 0;
+    const T sum = gather_row(A, i, escr, ex);
+    r[i] -= (shift + ex) * escr[i] - sum;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_block_zero(const u32 *members, u64 m0, u64 m1, T *escr) {
+    const u64 g = m0 + blockIdx.x * static_cast<u64>(kThreads) + threadIdx.x;
+    if (g < m1) escr[members[g]] = T(0);
+}
+
+// Jacobi sweep on listed rows: out[i] = x[i] + w (rhs[i] - (A x)[i]) / (shift + exit_i)
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_sweep_rows(csr_view<T> A, T shift, T omega, const T *rhs, const T *x, T *out,
+                 const u32 *rows, u64 nrows) {
+    const u64 q = blockIdx.x * static_cast<u64>(kThreads) + threadIdx.x;
+    if (q >= nrows) return;
+    const u32 i = rows[q];
+    T ex = 0;
+    const T sum = gather_row(A, i, x, ex);
+    const T d = shift + ex;
+    out[i] = x[i] + omega * (rhs[i] - (d * x[i] - sum)) / d;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_copy_rows(const u32 *rows, u64 nrows, const T *src, T *dst) {
+    const u64 q = blockIdx.x * static_cast<u64>(kThreads) + threadIdx.x;
+    if (q < nrows) dst[rows[q]] = src[rows[q]];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_fill(i64 n, T v, T *x) {
     for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
@@ -623,6 +700,49 @@ void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *
                        offset, inverse, max_size, r, x);
 }
 
+namespace {
+u32 blocks_for(u64 n) { return static_cast<u32>((n + kThreads - 1) / kThreads); }
+} // namespace
+
+template <typename T>
+void block_solve(const u32 *members, const u32 *member_block, const u32 *block_first,
+                 const u64 *inv_off, const T *inv, u64 m0, u64 m1, const T *r, T *x, T *escr,
+                 hipStream_t s) {
+    if (m1 <= m0) return;
+    hipLaunchKernelGGL((k_block_solve<T>), dim3(blocks_for(m1 - m0)), dim3(kThreads), 0, s, members,
+                       member_block, block_first, inv_off, inv, m0, m1, r, x, escr);
+}
+
+template <typename T>
+void block_residual(const csr_view<T> &A, double shift, const u32 *rows, u64 q0, u64 q1,
+                    const T *escr, T *r, hipStream_t s) {
+    if (q1 <= q0) return;
+    hipLaunchKernelGGL((k_block_residual<T>), dim3(blocks_for(q1 - q0)), dim3(kThreads), 0, s, A,
+                       static_cast<T>(shift), rows, q0, q1, escr, r);
+}
+
+template <typename T>
+void block_zero(const u32 *members, u64 m0, u64 m1, T *escr, hipStream_t s) {
+    if (m1 <= m0) return;
+    hipLaunchKernelGGL((k_block_zero<T>), dim3(blocks_for(m1 - m0)), dim3(kThreads), 0, s, members,
+                       m0, m1, escr);
+}
+
+template <typename T>
+void sweep_rows(const csr_view<T> &A, double shift, double omega, const T *rhs, const T *x, T *out,
+                const u32 *rows, u64 nrows, hipStream_t s) {
+    if (nrows == 0) return;
+    hipLaunchKernelGGL((k_sweep_rows<T>), dim3(blocks_for(nrows)), dim3(kThreads), 0, s, A,
+                       static_cast<T>(shift), static_cast<T>(omega), rhs, x, out, rows, nrows);
+}
+
+template <typename T>
+void copy_rows(const u32 *rows, u64 nrows, const T *src, T *dst, hipStream_t s) {
+    if (nrows == 0) return;
+    hipLaunchKernelGGL((k_copy_rows<T>), dim3(blocks_for(nrows)), dim3(kThreads), 0, s, rows, nrows,
+                       src, dst);
+}
+
 template <typename T>
 void fill(i64 n, T value, T *x, hipStream_t s) {
     if (n == 0) return;
@@ -725,6 +845,14 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
     template void terminal_solve<T>(i64, const u32 *, const u32 *, const u64 *, const T *, i64, \
                                     const T *, T *, hipStream_t);                               \
     template void fill<T>(i64, T, T *, hipStream_t);                                            \
+    template void block_solve<T>(const u32 *, const u32 *, const u32 *, const u64 *, const T *,   \
+                                 u64, u64, const T *, T *, T *, hipStream_t);                     \
+    template void block_residual<T>(const csr_view<T> &, double, const u32 *, u64, u64,         \
+                                    const T *, T *, hipStream_t);                                 \
+    template void block_zero<T>(const u32 *, u64, u64, T *, hipStream_t);                       \
+    template void sweep_rows<T>(const csr_view<T> &, double, double, const T *, const T *, T *,   \
+                                const u32 *, u64, hipStream_t);                                   \
+    template void copy_rows<T>(const u32 *, u64, const T *, T *, hipStream_t);                  \
     template void axpy<T>(i64, T, const T *, T *, hipStream_t);                                 \
     template void combine<T>(double *, const T *, i64, int, const double *, i64, hipStream_t);
 MK_LEVEL(float)

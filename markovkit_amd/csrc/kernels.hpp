@@ -109,6 +109,26 @@ template <typename T>
 void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
                     const T *inverse, i64 max_size, const T *r, T *x, hipStream_t s);
 
+// aggregate-block smoother (multigrid.hpp:309-342), one aggregate's step at a time:
+//   block_solve: for members g in [m0, m1): e = inverse_b r[members of b], x += e, escr = e
+//   block_residual: r[i] -= (A escr)[i] for listed rows q in [q0, q1) (escr zero off members)
+//   block_zero: escr = 0 on members [m0, m1)
+template <typename T>
+void block_solve(const u32 *members, const u32 *member_block, const u32 *block_first,
+                 const u64 *inv_off, const T *inv, u64 m0, u64 m1, const T *r, T *x, T *escr,
+                 hipStream_t s);
+template <typename T>
+void block_residual(const csr_view<T> &A, double shift, const u32 *rows, u64 q0, u64 q1,
+                    const T *escr, T *r, hipStream_t s);
+template <typename T>
+void block_zero(const u32 *members, u64 m0, u64 m1, T *escr, hipStream_t s);
+// Jacobi sweep on listed rows only (out = x + w (rhs - A x) / (shift + exit)), and copy back
+template <typename T>
+void sweep_rows(const csr_view<T> &A, double shift, double omega, const T *rhs, const T *x, T *out,
+                const u32 *rows, u64 nrows, hipStream_t s);
+template <typename T>
+void copy_rows(const u32 *rows, u64 nrows, const T *src, T *dst, hipStream_t s);
+
 template <typename T>
 void fill(i64 n, T value, T *x, hipStream_t s);
 template <typename A, typename B>

@@ -401,6 +401,14 @@ struct level_dev {
     std::vector<u32> h_tstart, h_tsize;
     std::vector<u64> h_toff;
     std::vector<double> h_tq;
+    // aggregate-block smoother (n_agg > 0): every sector's blocks, aggregate by aggregate, with
+    // member and affected-row ranges per aggregate on the host; rows of sectors that are
+    // terminal on this level keep the Jacobi smoother (jrows)
+    i64 n_agg = 0;
+    std::vector<u64> h_agg_members, h_agg_rows;
+    dbuf bs_members, bs_member_block, bs_block_first, bs_inv_off, bs_inv, bs_rows, escr;
+    i64 n_jrows = 0;
+    dbuf jrows;
     // work vectors
     dbuf rhs, x, xt, res, crhs, cx;
     int pre = 2, post = 2;

@@ -583,6 +583,149 @@ bool invert_dense(std::vector<double> a, i64 n, std::vector<double> &inverse) {
     return true;
 }
 
+namespace {
+
+// multigrid.hpp:268-300 on one sector level; see block_smoother
+block_smoother build_blocks(const population_space &space, const std::vector<i64> &parent,
+                            i64 n_agg, const generator_csr &A, double shift) {
+    block_smoother bs;
+    bs.aggregates = n_agg;
+    const i64 n = space.size();
+    const int S = space.species();
+    const i64 V = space.voxels();
+    std::vector<std::vector<i64>> children(n_agg);
+    for (i64 v = 0; v < V; ++v) children[parent[v]].push_back(v);
+    i64 molecules = 0;
+    for (int s = 0; s < S; ++s) molecules += space.totals(0)[s];
+    const i64 W = std::max<i64>(1, std::min<i64>(molecules, n_agg)); // occupied aggregates per state
+    constexpr u32 none = 0xFFFFFFFFu;
+    std::vector<u32> occ(static_cast<std::size_t>(n * W), none);
+    const i64 chunk = 16384;
+    const i64 tasks = (n + chunk - 1) / chunk;
+    // canonical key of every (aggregate, state with molecules in it), per task and aggregate
+    std::vector<std::vector<std::vector<std::pair<i64, u32>>>> keyed(tasks);
+    run_tasks(tasks, [&](i64 t) {
+        auto &kt = keyed[t];
+        kt.resize(n_agg);
+        std::vector<int> x(V * S), y(V * S), inside(n_agg * S);
+        for (i64 i = t * chunk; i < std::min(n, (t + 1) * chunk); ++i) {
+            space.state_in_sector(0, i, x.data());
+            std::fill(inside.begin(), inside.end(), 0);
+            for (i64 v = 0; v < V; ++v)
+                for (int s = 0; s < S; ++s) inside[parent[v] * S + s] += x[v * S + s];
+            i64 w = 0;
+            for (i64 a = 0; a < n_agg; ++a) {
+                bool any = false;
+                for (int s = 0; s < S; ++s) any = any || inside[a * S + s] > 0;
+                if (!any) continue;
+                occ[i * W + w++] = static_cast<u32>(a);
+                y = x;
+                for (i64 v : children[a])
+                    for (int s = 0; s < S; ++s) y[v * S + s] = 0;
+                for (int s = 0; s < S; ++s) y[children[a][0] * S + s] = inside[a * S + s];
+                kt[a].push_back({space.rank_in_sector(y.data(), 0), static_cast<u32>(i)});
+            }
+        }
+    });
+    // blocks of every aggregate: equal keys, members ascending (stable sort of row-ordered lists)
+    struct agg_part {
+        std::vector<u64> first{0};
+        std::vector<u32> members;
+        std::vector<u64> inv_offset;
+        std::vector<double> inverse;
+    };
+    std::vector<agg_part> parts(n_agg);
+    run_tasks(n_agg, [&](i64 a) {
+        std::vector<std::pair<i64, u32>> all;
+        for (i64 t = 0; t < tasks; ++t) all.insert(all.end(), keyed[t][a].begin(), keyed[t][a].end());
+        std::stable_sort(all.begin(), all.end(),
+                         [](const auto &l, const auto &r) { return l.first < r.first; });
+        auto &P = parts[a];
+        std::size_t b0 = 0;
+        std::vector<double> dense, inv;
+        while (b0 < all.size()) {
+            std::size_t b1 = b0 + 1;
+            while (b1 < all.size() && all[b1].first == all[b0].first) ++b1;
+            const i64 m = static_cast<i64>(b1 - b0);
+            if (m > 1) {
+                const std::size_t m0 = P.members.size();
+                for (std::size_t q = b0; q < b1; ++q) P.members.push_back(all[q].second);
+                const u32 *mem = P.members.data() + m0;
+                dense.assign(static_cast<std::size_t>(m * m), 0.0);
+                for (i64 k = 0; k < m; ++k) {
+                    const u32 i = mem[k];
+                    dense[k * m + k] += shift + A.exit[i];
+                    for (u64 p = A.ptr[i]; p < A.ptr[i + 1]; ++p) {
+                        const u32 *hit = std::lower_bound(mem, mem + m, A.col[p]);
+                        if (hit != mem + m && *hit == A.col[p]) dense[k * m + (hit - mem)] -= A.val[p];
+                    }
+                }
+                if (!invert_dense(dense, m, inv))
+                    throw std::runtime_error("population_multigrid: singular block");
+                P.inv_offset.push_back(P.inverse.size());
+                P.inverse.insert(P.inverse.end(), inv.begin(), inv.end());
+                P.first.push_back(P.members.size());
+            }
+            b0 = b1;
+        }
+        std::vector<std::pair<i64, u32>>().swap(all);
+    });
+    keyed.clear();
+    // rows whose residual an aggregate's correction changes: the members and every row with a
+    // member among its sources
+    std::vector<std::vector<u64>> count(tasks, std::vector<u64>(n_agg, 0));
+    auto affected = [&](i64 i, std::vector<i64> &stamp, auto &&emit) {
+        auto mark = [&](u32 row) {
+            for (i64 w = 0; w < W; ++w) {
+                const u32 a = occ[row * W + w];
+                if (a == none) break;
+                if (stamp[a] != i) {
+                    stamp[a] = i;
+                    emit(a);
+                }
+            }
+        };
+        mark(static_cast<u32>(i));
+        for (u64 p = A.ptr[i]; p < A.ptr[i + 1]; ++p) mark(A.col[p]);
+    };
+    run_tasks(tasks, [&](i64 t) {
+        std::vector<i64> stamp(n_agg, -1);
+        for (i64 i = t * chunk; i < std::min(n, (t + 1) * chunk); ++i)
+            affected(i, stamp, [&](u32 a) { ++count[t][a]; });
+    });
+    bs.agg_rows.assign(n_agg + 1, 0);
+    std::vector<std::vector<u64>> at(tasks, std::vector<u64>(n_agg, 0));
+    for (i64 a = 0; a < n_agg; ++a) {
+        u64 off = bs.agg_rows[a];
+        for (i64 t = 0; t < tasks; ++t) {
+            at[t][a] = off;
+            off += count[t][a];
+        }
+        bs.agg_rows[a + 1] = off;
+    }
+    bs.rows.resize(bs.agg_rows[n_agg]);
+    run_tasks(tasks, [&](i64 t) {
+        std::vector<i64> stamp(n_agg, -1);
+        for (i64 i = t * chunk; i < std::min(n, (t + 1) * chunk); ++i)
+            affected(i, stamp, [&](u32 a) { bs.rows[at[t][a]++] = static_cast<u32>(i); });
+    });
+    // concatenate the aggregates
+    bs.agg_blocks.assign(1, 0);
+    bs.block_first.assign(1, 0);
+    for (i64 a = 0; a < n_agg; ++a) {
+        const auto &P = parts[a];
+        const u64 mbase = bs.members.size(), ibase = bs.inverse.size();
+        bs.members.insert(bs.members.end(), P.members.begin(), P.members.end());
+        for (std::size_t b = 1; b < P.first.size(); ++b) bs.block_first.push_back(mbase + P.first[b]);
+        for (u64 o : P.inv_offset) bs.inv_offset.push_back(ibase + o);
+        bs.inverse.insert(bs.inverse.end(), P.inverse.begin(), P.inverse.end());
+        bs.agg_blocks.push_back(bs.inv_offset.size());
+    }
+    return bs;
+}
+
+} // namespace
+
 sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::vector<int> &totals,
                                         double shift, const multigrid_options &options) {
     if (!(shift > 0.0)) throw std::invalid_argument("population_multigrid: shift must be positive");
@@ -659,6 +802,10 @@ sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::v
         std::vector<double> total(cspace.size(), 0.0);
         for (i64 i = 0; i < level.rows; ++i) total[level.coarse_of[i]] += level.weight[i];
         for (i64 i = 0; i < level.rows; ++i) level.weight[i] /= total[level.coarse_of[i]];
+        if (opt.smoother == 1) {
+            level.blocks = build_blocks(space, parent, coarse->voxels, level.A, shift);
+            level.has_blocks = true;
+        }
         h.levels.push_back(std::move(level));
         dom = std::move(*coarse);
         opt.max_levels -= 1;

@@ -145,8 +145,25 @@ struct multigrid_options {
     int max_levels = 32;
     int coarse_cycles = 1;
     int sweep_growth = 0;
-    int smoother = 0;
+    int smoother = 0; // 0 Jacobi, 1 aggregate blocks
     int precision = 0;
+};
+
+// Aggregate-block smoother of one sector level (multigrid.hpp:268-342): for every coarse voxel
+// a (aggregate), the blocks of two or more states that agree outside a, with the dense inverse
+// of (shift I - Q) restricted to each block. Blocks are ordered by their canonical state (the
+// aggregate's molecules moved into its first fine voxel), members ascending. rows lists, per
+// aggregate, the states whose residual the aggregate's correction changes (its members and the
+// targets of their jumps), ascending.
+struct block_smoother {
+    i64 aggregates = 0;
+    std::vector<u64> agg_blocks;  // aggregates + 1
+    std::vector<u64> block_first; // blocks + 1: first member of each block
+    std::vector<u32> members;     // level row indices
+    std::vector<u64> inv_offset;  // blocks
+    std::vector<double> inverse;  // row-major per block
+    std::vector<u64> agg_rows;    // aggregates + 1
+    std::vector<u32> rows;
 };
 
 // One level of one sector's population_multigrid
@@ -158,6 +175,8 @@ struct sector_level {
     std::vector<double> weight;  // multinomial prolongation weight (non-terminal)
     std::vector<double> inverse; // rows x rows, row-major (terminal): (shift I - Q)^-1
     int pre_sweeps = 2, post_sweeps = 2;
+    bool has_blocks = false;     // aggregate-block smoother (non-terminal levels)
+    block_smoother blocks;
 };
 
 struct sector_hierarchy {

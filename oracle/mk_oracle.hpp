@@ -17,7 +17,8 @@
 //   generator (jump lists, absorbing / reflecting, scatter apply / apply_shifted)
 //                                  include/markovkit/generator.hpp:27-107
 //   population_transfer (multinomial P, aggregation R)   multigrid.hpp:47-118
-//   population_multigrid (V/W cycle, weighted Jacobi)      multigrid.hpp:139-361
+//   population_multigrid (V/W cycle, weighted Jacobi and aggregate-block smoothers)
+//                                                         multigrid.hpp:139-361
 //   sector_multigrid                                       multigrid.hpp:371-408
 //   num::gmres (absent; restated as restarted right-preconditioned GMRES, Saad's
 //       algorithm with modified Gram-Schmidt and Givens rotations, absolute Euclidean
@@ -25,7 +26,7 @@
 //       rdme/cpp/rdme_full_multigrid.cpp:1187-1209
 //   dense partial-pivot LU (num::lu / num::solve, absent)  multigrid.hpp:221, :247-264
 //
-// Pins (tests/test_oracle_*.py, tests/oracle_cases.cpp): the exact values in
+// Pins (tests/test_oracle_pins.py): the exact values in
 // tests/test_rdme.cpp:22-223, the reference's recorded transfer weights
 // (rdme/paper/data/transfer_sparsity/*.csv), the lumped sector-sum closed form that
 // reproduces the recorded masses 0.337423312883 and 0.82128451753, an independent
@@ -659,7 +660,7 @@ struct multigrid_options {
     relaxation smoother = relaxation::jacobi;
 };
 
-// multigrid.hpp:139-361 (weighted Jacobi smoother)
+// multigrid.hpp:139-361 (weighted Jacobi and aggregate-block smoothers)
 class population_multigrid {
   public:
     population_multigrid(const voxel_domain &domain, const std::vector<int> &totals, real shift,
@@ -669,8 +670,6 @@ class population_multigrid {
           rhs_(rows()), x_(rows()), r_(rows()), e_(rows()) {
         if (!(shift > 0.0))
             throw std::invalid_argument("population_multigrid: shift must be positive");
-        if (options.smoother != relaxation::jacobi)
-            throw std::invalid_argument("oracle: only the Jacobi smoother is restated");
         if (!domain.structured) throw std::invalid_argument("population_multigrid needs a grid");
         auto coarse_grid = coarsen(*domain.structured);
         if (rows() <= options.coarsest_size || options.max_levels <= 1 || !coarse_grid) {
@@ -694,6 +693,7 @@ class population_multigrid {
         coarse_x_.assign(n, 0.0);
         coarse_r_.assign(n, 0.0);
         coarse_e_.assign(n, 0.0);
+        if (options.smoother == relaxation::aggregate_block) build_blocks();
     }
 
     [[nodiscard]] idx rows() const { return space_.size(); }
@@ -714,7 +714,7 @@ class population_multigrid {
     void cycle(const real *b, real *x) const {
         std::copy(b, b + rows(), rhs_.begin());
         std::fill(x_.begin(), x_.end(), 0.0);
-        smooth(options_.pre_sweeps);
+        smooth(options_.pre_sweeps, false);
         residual();
         if (coarse_) {
             transfer_->restrict(r_.data(), coarse_rhs_.data());
@@ -730,11 +730,61 @@ class population_multigrid {
             solve(terminal_, r_.data(), e_.data());
         }
         for (idx i = 0; i < rows(); ++i) x_[i] += e_[i];
-        smooth(options_.post_sweeps);
+        smooth(options_.post_sweeps, true);
         std::copy(x_.begin(), x_.end(), x);
     }
 
+    // multigrid.hpp:231-236: `sweeps` smoother sweeps on A x = rhs, starting from x
+    void relax(const real *rhs, real *x, int sweeps = 1, bool reverse = false) const {
+        std::copy(rhs, rhs + rows(), rhs_.begin());
+        std::copy(x, x + rows(), x_.begin());
+        smooth(sweeps, reverse);
+        std::copy(x_.begin(), x_.end(), x);
+    }
+
+    // aggregate a's blocks, in the order the smoother visits them (for tests)
+    [[nodiscard]] idx n_aggregates() const { return blocks_.size(); }
+    [[nodiscard]] idx n_blocks(idx a) const { return blocks_[a].size(); }
+    [[nodiscard]] const std::vector<idx> &block_states(idx a, idx b) const {
+        return blocks_[a][b].states;
+    }
+
   private:
+    struct block {
+        std::vector<idx> states;
+        lu_result lu;
+        mutable std::vector<real> rhs, e;
+    };
+
+    // multigrid.hpp:268-300: group the states that agree outside each coarse voxel; each group
+    // of two or more states is one block. The groups are visited in ascending order of the
+    // outside configuration (the reference's hash-map order is unspecified; only the rounding
+    // of the residual updates depends on it).
+    void build_blocks() {
+        const idx S = space_.n_species();
+        const auto &parent = transfer_->parent();
+        std::vector<idx> position(rows(), rows());
+        blocks_.resize(coarse_->space().n_voxels());
+        for (idx a = 0; a < static_cast<idx>(blocks_.size()); ++a) {
+            std::map<state, std::vector<idx>> groups;
+            for (idx i = 0; i < rows(); ++i) {
+                state outside(space_[i], space_[i] + space_.n_voxels() * S);
+                for (idx v = 0; v < space_.n_voxels(); ++v)
+                    if (parent[v] == a) std::fill_n(outside.begin() + v * S, S, 0);
+                groups[std::move(outside)].push_back(i);
+            }
+            for (auto &[outside, members] : groups) {
+                if (members.size() == 1) continue;
+                for (idx k = 0; k < static_cast<idx>(members.size()); ++k) position[members[k]] = k;
+                auto f = factor(members, position);
+                for (idx i : members) position[i] = rows();
+                const idx n = members.size();
+                blocks_[a].push_back({std::move(members), std::move(f), std::vector<real>(n, 0.0),
+                                      std::vector<real>(n, 0.0)});
+            }
+        }
+    }
+
     [[nodiscard]] lu_result factor(const std::vector<idx> &states,
                                    const std::vector<idx> &position) const {
         const idx n = states.size();
@@ -752,11 +802,36 @@ class population_multigrid {
         apply_system(x_.data(), r_.data());
         for (idx i = 0; i < rows(); ++i) r_[i] = rhs_[i] - r_[i];
     }
-    void smooth(int sweeps) const {
+    // multigrid.hpp:309-342
+    void smooth(int sweeps, bool reverse) const {
+        if (blocks_.empty()) {
+            for (int sweep = 0; sweep < sweeps; ++sweep) {
+                residual();
+                for (idx i = 0; i < rows(); ++i)
+                    x_[i] += options_.jacobi_weight * r_[i] / (shift_ + Q_.exit_rates()[i]);
+            }
+            return;
+        }
+        // block Gauss-Seidel over aggregates, block Jacobi within one aggregate
+        residual();
+        const idx A = blocks_.size();
         for (int sweep = 0; sweep < sweeps; ++sweep) {
-            residual();
-            for (idx i = 0; i < rows(); ++i)
-                x_[i] += options_.jacobi_weight * r_[i] / (shift_ + Q_.exit_rates()[i]);
+            for (idx step = 0; step < A; ++step) {
+                const auto &aggregate = blocks_[reverse ? A - 1 - step : step];
+                for (const block &b : aggregate) {
+                    for (idx k = 0; k < static_cast<idx>(b.states.size()); ++k) b.rhs[k] = r_[b.states[k]];
+                    solve(b.lu, b.rhs.data(), b.e.data());
+                }
+                for (const block &b : aggregate) {
+                    for (idx k = 0; k < static_cast<idx>(b.states.size()); ++k) {
+                        const idx j = b.states[k];
+                        x_[j] += b.e[k];
+                        r_[j] -= (shift_ + Q_.exit_rates()[j]) * b.e[k];
+                        for (const auto *e = Q_.jumps_begin(j); e != Q_.jumps_end(j); ++e)
+                            r_[e->to] += e->rate * b.e[k];
+                    }
+                }
+            }
         }
     }
     [[nodiscard]] idx total_states() const {
@@ -773,6 +848,7 @@ class population_multigrid {
     std::unique_ptr<population_multigrid> coarse_;
     std::unique_ptr<population_transfer> transfer_;
     lu_result terminal_;
+    std::vector<std::vector<block>> blocks_;
     mutable std::vector<real> rhs_, x_, r_, e_;
     mutable std::vector<real> coarse_rhs_, coarse_x_, coarse_r_, coarse_e_;
 };

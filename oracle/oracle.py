@@ -49,7 +49,8 @@ def lib():
         L.orc_apply.argtypes = [P, f64p, f64p]
         L.orc_apply_shifted.argtypes = [P, C.c_double, f64p, f64p]
         L.orc_mg_create.argtypes = [P, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int64,
-                                    C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+                                    C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.orc_mg_relax.argtypes = [P, C.c_int64, f64p, f64p, C.c_int, C.c_int]
         L.orc_mg_destroy.argtypes = [P]
         L.orc_mg_max_levels.argtypes = [P]
         L.orc_mg_max_levels.restype = C.c_int64
@@ -179,14 +180,18 @@ class Problem:
 
 
 class SectorMultigrid:
-    """sector_multigrid on the CPU (Jacobi smoother)."""
+    """sector_multigrid on the CPU (smoother "jacobi" or "aggregate_block")."""
 
     def __init__(self, problem, shift, pre_sweeps=2, post_sweeps=2, jacobi_weight=2.0 / 3.0,
-                 coarsest_size=4, max_levels=32, coarse_cycles=1, sweep_growth=0):
+                 coarsest_size=4, max_levels=32, coarse_cycles=1, sweep_growth=0,
+                 smoother="jacobi"):
+        if smoother not in ("jacobi", "aggregate_block"):
+            raise ValueError("smoother must be 'jacobi' or 'aggregate_block'")
         h = C.c_void_p()
         _check(lib().orc_mg_create(problem.h, float(shift), pre_sweeps, post_sweeps,
                                    jacobi_weight, coarsest_size, max_levels, coarse_cycles,
-                                   sweep_growth, C.byref(h)))
+                                   sweep_growth, 1 if smoother == "aggregate_block" else 0,
+                                   C.byref(h)))
         self.h = h
         self.problem = problem
         self.n = problem.n
@@ -205,6 +210,13 @@ class SectorMultigrid:
         out = np.zeros(k * L, np.int64)
         lib().orc_mg_level_rows(self.h, out)
         return out.reshape(k, L)
+
+    def relax(self, k, rhs, x, sweeps=1, reverse=False):
+        """population_multigrid::relax on sector k's finest level (multigrid.hpp:231-236):
+        `sweeps` smoother sweeps on A x = rhs starting from x (updated in place)"""
+        _check(lib().orc_mg_relax(self.h, k, np.ascontiguousarray(rhs, np.float64), x, sweeps,
+                                  1 if reverse else 0))
+        return x
 
     def apply(self, b):
         x = np.zeros(self.n)

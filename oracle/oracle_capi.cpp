@@ -181,10 +181,11 @@ void orc_apply_shifted(void *h, double shift, const double *x, double *y) {
 }
 
 int orc_mg_create(void *h, double shift, int pre, int post, double omega, int64_t coarsest,
-                  int max_levels, int coarse_cycles, int sweep_growth, void **out) {
+                  int max_levels, int coarse_cycles, int sweep_growth, int smoother, void **out) {
     return guarded([&] {
         auto *p = static_cast<problem *>(h);
         multigrid_options o;
+        o.smoother = smoother == 1 ? relaxation::aggregate_block : relaxation::jacobi;
         o.pre_sweeps = pre;
         o.post_sweeps = post;
         o.jacobi_weight = omega;
@@ -197,6 +198,14 @@ int orc_mg_create(void *h, double shift, int pre, int post, double omega, int64_
 }
 
 void orc_mg_destroy(void *mg) { delete static_cast<sector_multigrid *>(mg); }
+
+int orc_mg_relax(void *mg, int64_t k, const double *rhs, double *x, int sweeps, int reverse) {
+    return guarded([&] {
+        auto *m = static_cast<sector_multigrid *>(mg);
+        if (k < 0 || k >= static_cast<int64_t>(m->n_sectors())) throw std::invalid_argument("sector out of range");
+        m->sector(k).relax(rhs, x, sweeps, reverse != 0);
+    });
+}
 
 int64_t orc_mg_max_levels(void *mg) {
     return static_cast<int64_t>(static_cast<sector_multigrid *>(mg)->max_levels());

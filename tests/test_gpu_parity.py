@@ -83,11 +83,13 @@ def test_mass_balance(case):
 
 
 @pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
-def test_multigrid_cycle(case, precision):
+@pytest.mark.parametrize("smoother", [mk.JACOBI, mk.AGGREGATE_BLOCK], ids=["jacobi", "blocks"])
+def test_multigrid_cycle(case, precision, smoother):
     w, P, box = case["w"], case["P"], case["box"]
     shift = w.shifts[0]
-    M = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=precision))
-    Mo = OracleMG(P, shift)
+    M = mk.SectorMultigrid(w.model.domain, box, shift,
+                           mk.MultigridOptions(precision=precision, smoother=smoother))
+    Mo = OracleMG(P, shift, smoother="jacobi" if smoother == mk.JACOBI else "aggregate_block")
     assert M.max_levels() == Mo.max_levels()
     assert M.n_sectors() == len(P.sector_begins()) - 1
     rng = np.random.default_rng(11)
@@ -147,6 +149,38 @@ def test_dcgs2_matches_cgs2_iterates(case):
         xs[ortho] = x
     scale = np.abs(xs["cgs2"]).sum()
     assert np.abs(xs["dcgs2"] - xs["cgs2"]).sum() <= 1e-9 * scale
+
+
+@pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
+def test_aggregate_block_solve(case, precision):
+    """FGMRES preconditioned by the aggregate-block sector multigrid: converges to the oracle's
+    answer, in no more iterations than with the Jacobi smoother (the reference's "aggregate
+    blocks beat point Jacobi", tests/test_rdme.cpp:207-210, seen on the outer iteration)."""
+    w, P, box, Q = case["w"], case["P"], case["box"], case["Q"]
+    shift = w.shifts[0]
+    b = w.rhs(0, P.n)
+    tol = 1e-10 * np.linalg.norm(b)
+    A = mk.ShiftedGenerator(Q, shift)
+    its = {}
+    xs = {}
+    for name, sm in (("jacobi", mk.JACOBI), ("blocks", mk.AGGREGATE_BLOCK)):
+        M = mk.SectorMultigrid(w.model.domain, box, shift,
+                               mk.MultigridOptions(precision=precision, smoother=sm))
+        x = np.zeros(P.n)
+        res = mk.gmres(A, M, b, x, tolerance=tol, max_iterations=500, restart=50)
+        assert res.converged, (name, res)
+        its[name] = res.iterations
+        xs[name] = x
+    r = b - P.apply_shifted(shift, xs["blocks"])
+    assert np.linalg.norm(r) <= 1.0001 * tol
+    x_ref, info = P.gmres(shift, b, mg=OracleMG(P, shift, smoother="aggregate_block"), tol=tol,
+                          restart=50)
+    assert info["converged"]
+    bound = (np.abs(r).sum() + np.abs(b - P.apply_shifted(shift, x_ref)).sum()) / shift
+    assert np.abs(xs["blocks"] - x_ref).sum() <= bound
+    assert its["blocks"] <= its["jacobi"], its
+    print(f"[{case['name']}] iterations jacobi {its['jacobi']} blocks {its['blocks']} "
+          f"oracle blocks {info['iterations']}")
 
 
 def test_plain_gmres(case):

@@ -351,6 +351,7 @@ void mg_apply_device(mk_sector_mg_s &mg, const double *b, void *z, double scale 
 // ------------------------------------------------------------------ flexible GMRES
 struct gmres_workspace {
     i64 n = 0;
+    i64 ld = 0; // basis stride: n rounded up to 64 elements, so every vector starts on a 512 B boundary
     i64 m = 0;
     std::size_t zs = 0;
     dbuf V, Z, w, partial, hd, yd;
@@ -367,8 +368,9 @@ struct gmres_workspace {
         zs = std::max(zs, zs_);
         V = dbuf();
         Z = dbuf();
-        V = dbuf(static_cast<std::size_t>(n) * (m + 1) * sizeof(double));
-        if (zs) Z = dbuf(static_cast<std::size_t>(n) * m * zs);
+        ld = (n + 63) / 64 * 64;
+        V = dbuf(static_cast<std::size_t>(ld) * (m + 1) * sizeof(double));
+        if (zs) Z = dbuf(static_cast<std::size_t>(ld) * m * zs);
         w = dbuf(n * sizeof(double));
         const int nb = gpu::reduction_blocks(n);
         partial = dbuf(static_cast<std::size_t>(nb) * (m + 2) * sizeof(double));
@@ -418,6 +420,7 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
     double *V = dp<double>(ws.V), *w = dp<double>(ws.w);
     double *partial = dp<double>(ws.partial), *hd = dp<double>(ws.hd);
     ZT *Z = M ? dp<ZT>(ws.Z) : nullptr;
+    const i64 ld = ws.ld;
     const double spmv_bytes = op_bytes(A.A, 8, 8, 1);
     const i64 hs = m + 2; // stride of the device scalar slots in hd
 
@@ -433,7 +436,7 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
     // w = A M (scale v)
     auto operator_image = [&](i64 j, const double *v, double scale) {
         if (M) {
-            ZT *Zj = Z + j * n;
+            ZT *Zj = Z + j * ld;
             mg_apply_device(*M, v, Zj, scale);
             P.run(K_SPMV, op_bytes(A.A, sizeof(ZT), 8, 1), [&] {
                 gpu::shifted_apply<ZT>(A.A.view, shift, Zj, nullptr, w, nullptr, s);
@@ -466,15 +469,15 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
             for (; j < m && res.iterations < o.max_iterations; ++j) {
                 ++res.iterations;
                 const int cnt = static_cast<int>(j);
-                operator_image(j, V + j * n, 1.0 / bj);
+                operator_image(j, V + j * ld, 1.0 / bj);
                 // t = Q^T w and u_j^T w in hd[0 .. j]
                 P.run(K_DOT, double(cnt + 2) * n * 8.0, [&] {
-                    gpu::multi_dot(V, n, cnt + 1, w, n, partial, nb, s);
+                    gpu::multi_dot(V, ld, cnt + 1, w, n, partial, nb, s);
                     gpu::reduce_partials(partial, nb, cnt + 1, hd, s);
                 });
                 // q_j, eta (hd[m + 1]), u_{j+1} and its projections s_{j+1}, |u_{j+1}|^2
                 P.run(K_UPDATE, double(cnt + 4) * n * 8.0, [&] {
-                    gpu::dcgs_update(V, n, cnt, sd, hd, bj, w, n, partial, nb, hd + m + 1, s);
+                    gpu::dcgs_update(V, ld, cnt, sd, hd, bj, w, n, partial, nb, hd + m + 1, s);
                     gpu::reduce_partials(partial, nb, cnt + 2, sd_next, s);
                 });
                 fetch(3 * hs);
@@ -489,12 +492,12 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
                     // the candidate kept too much of span(Q) for the Pythagorean norm: project
                     // it once more explicitly and keep the remaining coefficients for the next
                     // step's finalisation
-                    double *u = V + (j + 1) * n;
+                    double *u = V + (j + 1) * ld;
                     P.run(K_UPDATE, double(cnt + 3) * n * 8.0, [&] {
-                        gpu::basis_update(u, V, n, cnt + 1, sd_next, n, nullptr, nb, s);
+                        gpu::basis_update(u, V, ld, cnt + 1, sd_next, n, nullptr, nb, s);
                     });
                     P.run(K_DOT, double(cnt + 3) * n * 8.0, [&] {
-                        gpu::multi_dot(V, n, cnt + 2, u, n, partial, nb, s);
+                        gpu::multi_dot(V, ld, cnt + 2, u, n, partial, nb, s);
                         gpu::reduce_partials(partial, nb, cnt + 2, sd_next, s);
                     });
                     fetch(3 * hs);
@@ -519,37 +522,37 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
             P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, V, hd, V, s); });
             for (; j < m && res.iterations < o.max_iterations; ++j) {
                 ++res.iterations;
-                double *Vj = V + j * n;
+                double *Vj = V + j * ld;
                 operator_image(j, Vj, 1.0);
                 // classical Gram-Schmidt, twice (CGS2)
                 const int cnt = static_cast<int>(j + 1);
                 double *h1d = hd, *cwd = hd + hs, *bfd = hd + 2 * hs;
-                double *vnext = V + (j + 1) * n;
+                double *vnext = V + (j + 1) * ld;
                 const bool fused = cnt <= gpu::kFusedBasis;
                 if (fused) {
                     // pass 1 projections; update fused with the pass-2 projections and |w|^2;
                     // pass-2 update fused with the normalisation
                     P.run(K_DOT, double(cnt + 1) * n * 8.0, [&] {
-                        gpu::multi_dot(V, n, cnt, w, n, partial, nb, s);
+                        gpu::multi_dot(V, ld, cnt, w, n, partial, nb, s);
                         gpu::reduce_partials(partial, nb, cnt, h1d, s);
                     });
                     P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {
-                        gpu::update_dot(w, V, n, cnt, h1d, n, partial, nb, s);
+                        gpu::update_dot(w, V, ld, cnt, h1d, n, partial, nb, s);
                         gpu::reduce_partials(partial, nb, cnt + 1, cwd, s);
                     });
                     P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {
-                        gpu::update_scale(w, V, n, cnt, cwd, n, vnext, bfd, s);
+                        gpu::update_scale(w, V, ld, cnt, cwd, n, vnext, bfd, s);
                     });
                 } else {
                     for (int pass = 0; pass < 2; ++pass) {
                         double *h = pass == 0 ? h1d : cwd;
                         P.run(K_DOT, double(cnt + 1) * n * 8.0, [&] {
-                            gpu::multi_dot(V, n, cnt, w, n, partial, nb, s);
+                            gpu::multi_dot(V, ld, cnt, w, n, partial, nb, s);
                             gpu::reduce_partials(partial, nb, cnt, h, s);
                         });
                         double *np = pass == 1 ? partial + static_cast<i64>(nb) * (m + 1) : nullptr;
                         P.run(K_UPDATE, double(cnt + 2) * n * 8.0, [&] {
-                            gpu::basis_update(w, V, n, cnt, h, n, np, nb, s);
+                            gpu::basis_update(w, V, ld, cnt, h, n, np, nb, s);
                         });
                     }
                     P.run(K_DOT, nb * 8.0, [&] {
@@ -592,9 +595,9 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
         HIPCHK(hipMemcpyAsync(ws.yd.p, ypin, j * sizeof(double), hipMemcpyHostToDevice, s));
         P.run(K_OTHER, double(j) * n * (M ? sizeof(ZT) : 8) + n * 16.0, [&] {
             if (M) {
-                gpu::combine<ZT>(x, Z, n, static_cast<int>(j), dp<double>(ws.yd), n, s);
+                gpu::combine<ZT>(x, Z, ld, static_cast<int>(j), dp<double>(ws.yd), n, s);
             } else {
-                gpu::combine<double>(x, V, n, static_cast<int>(j), dp<double>(ws.yd), n, s);
+                gpu::combine<double>(x, V, ld, static_cast<int>(j), dp<double>(ws.yd), n, s);
             }
         });
         beta = true_residual();

@@ -25,6 +25,8 @@
 //       tolerance, x in = initial guess)   call sites tests/test_rdme.cpp:238-243,
 //       rdme/cpp/rdme_full_multigrid.cpp:1187-1209
 //   dense partial-pivot LU (num::lu / num::solve, absent)  multigrid.hpp:221, :247-264
+//   num::expv (absent; restated as Sidje's Expokit expv with a degree-6 Pade exponential of
+//       the Hessenberg matrix)   call sites include/markovkit/fsp.hpp:152, :225
 //
 // Pins (tests/test_oracle_pins.py): the exact values in
 // tests/test_rdme.cpp:22-223, the reference's recorded transfer weights
@@ -978,6 +980,191 @@ solver_result gmres(idx n, ApplyA &&apply_a, ApplyM &&apply_m, const real *b, re
             break;
         }
         (void)done;
+    }
+    return res;
+}
+
+// ---------------------------------------------------------------- Krylov expv (num::expv)
+// num::expv(t, A, v, m, tol) is called at include/markovkit/fsp.hpp:152 and :225; its source
+// (third-party numerics @ dc6f1ff) is absent. Restated as Sidje's expv (Expokit, ACM TOMS 24,
+// 1998): Arnoldi (modified Gram-Schmidt) of dimension m on w, the exponential of the
+// (m + 2) x (m + 2) augmented Hessenberg matrix, the a-posteriori error estimate from the
+// two corrected terms, and step-size control with gamma 0.9, delta 1.2, at most 10 rejections.
+
+// exp(X) for a small dense X (row-major n x n): diagonal Pade approximant of degree 6 with
+// scaling and squaring (Golub & Van Loan, Matrix Computations, Algorithm 9.3.1)
+inline std::vector<real> expm_pade6(std::vector<real> X, idx n) {
+    real norm = 0.0;
+    for (idx i = 0; i < n; ++i) {
+        real r = 0.0;
+        for (idx j = 0; j < n; ++j) r += std::abs(X[i * n + j]);
+        norm = std::max(norm, r);
+    }
+    int sq = 0;
+    if (norm > 0.5) sq = static_cast<int>(std::ceil(std::log2(norm / 0.5)));
+    const real scale = std::ldexp(1.0, -sq);
+    for (real &x : X) x *= scale;
+    auto mul = [n](const std::vector<real> &A, const std::vector<real> &B) {
+        std::vector<real> C(n * n, 0.0);
+        for (idx i = 0; i < n; ++i)
+            for (idx k = 0; k < n; ++k) {
+                const real a = A[i * n + k];
+                for (idx j = 0; j < n; ++j) C[i * n + j] += a * B[k * n + j];
+            }
+        return C;
+    };
+    constexpr int q = 6;
+    real c = 1.0;
+    std::vector<real> N(n * n, 0.0), D(n * n, 0.0), P(n * n, 0.0);
+    for (idx i = 0; i < n; ++i) P[i * n + i] = 1.0;
+    for (int k = 0; k <= q; ++k) {
+        if (k > 0) {
+            c *= real(q - k + 1) / real((2 * q - k + 1) * k);
+            P = mul(P, X);
+        }
+        const real sign = (k % 2) ? -1.0 : 1.0;
+        for (idx i = 0; i < n * n; ++i) {
+            N[i] += c * P[i];
+            D[i] += sign * c * P[i];
+        }
+    }
+    // F = D^-1 N
+    auto f = lu(D, n);
+    if (f.singular) throw std::runtime_error("expm: singular Pade denominator");
+    std::vector<real> F(n * n), col(n), out(n);
+    for (idx j = 0; j < n; ++j) {
+        for (idx i = 0; i < n; ++i) col[i] = N[i * n + j];
+        solve(f, col.data(), out.data());
+        for (idx i = 0; i < n; ++i) F[i * n + j] = out[i];
+    }
+    for (int k = 0; k < sq; ++k) F = mul(F, F);
+    return F;
+}
+
+struct expv_options {
+    idx krylov_dim = 30;
+    real tolerance = 1e-8;
+};
+
+struct expv_result {
+    idx steps = 0;
+    idx rejected = 0;
+    real error = 0.0; // accumulated local error estimates
+};
+
+// w = exp(t A) v with y = A x given by apply(x, y); anorm is a norm estimate of A
+template <typename Apply>
+expv_result expv(real t, idx n, const Apply &apply, real anorm, const real *v, real *w,
+                 const expv_options &opt = {}) {
+    expv_result res;
+    const idx m = std::max<idx>(1, std::min<idx>(opt.krylov_dim, n));
+    const real tol = opt.tolerance, btol = 1e-7, gamma = 0.9, delta = 1.2;
+    const int max_reject = 10;
+    const real t_out = std::abs(t), sgn = t >= 0.0 ? 1.0 : -1.0;
+    const real rndoff = anorm * std::numeric_limits<real>::epsilon();
+    auto norm2 = [n](const real *x) {
+        real s = 0.0;
+        for (idx i = 0; i < n; ++i) s += x[i] * x[i];
+        return std::sqrt(s);
+    };
+    std::copy(v, v + n, w);
+    real beta = norm2(w);
+    if (!(beta > 0.0) || t_out == 0.0) return res;
+    auto round_step = [](real step) {
+        const real s = std::pow(10.0, std::floor(std::log10(step)) - 1.0);
+        return std::ceil(step / s) * s;
+    };
+    real xm = 1.0 / real(m);
+    const real fact = std::pow(real(m + 1) / std::exp(1.0), real(m + 1)) *
+                      std::sqrt(2.0 * 3.141592653589793 * real(m + 1));
+    real t_new = (1.0 / anorm) * std::pow((fact * tol) / (4.0 * beta * anorm), xm);
+    t_new = round_step(t_new);
+    real t_now = 0.0;
+    std::vector<real> V((m + 1) * n), p(n);
+    const idx ld = m + 2;
+    while (t_now < t_out) {
+        ++res.steps;
+        real t_step = std::min(t_out - t_now, t_new);
+        std::vector<real> H(ld * ld, 0.0);
+        for (idx i = 0; i < n; ++i) V[i] = w[i] / beta;
+        int k1 = 2;
+        idx mb = m;
+        for (idx j = 0; j < m; ++j) {
+            apply(V.data() + j * n, p.data());
+            for (idx i = 0; i <= j; ++i) {
+                real h = 0.0;
+                const real *Vi = V.data() + i * n;
+                for (idx r = 0; r < n; ++r) h += Vi[r] * p[r];
+                H[i * ld + j] = h;
+                for (idx r = 0; r < n; ++r) p[r] -= h * Vi[r];
+            }
+            const real s = norm2(p.data());
+            if (s < btol) {
+                k1 = 0;
+                mb = j + 1;
+                t_step = t_out - t_now;
+                break;
+            }
+            H[(j + 1) * ld + j] = s;
+            for (idx r = 0; r < n; ++r) V[(j + 1) * n + r] = p[r] / s;
+        }
+        real avnorm = 0.0;
+        if (k1 != 0) {
+            H[(m + 1) * ld + m] = 1.0;
+            apply(V.data() + m * n, p.data());
+            avnorm = norm2(p.data());
+        }
+        std::vector<real> F;
+        real err_loc = btol;
+        for (int rejected = 0;; ++rejected) {
+            const idx mx = mb + k1;
+            std::vector<real> X(mx * mx);
+            for (idx i = 0; i < mx; ++i)
+                for (idx j = 0; j < mx; ++j) X[i * mx + j] = sgn * t_step * H[i * ld + j];
+            F = expm_pade6(std::move(X), mx);
+            if (k1 == 0) {
+                err_loc = btol;
+                // F is mb x mb: widen to the layout below
+                std::vector<real> G(ld * ld, 0.0);
+                for (idx i = 0; i < mb; ++i)
+                    for (idx j = 0; j < mb; ++j) G[i * ld + j] = F[i * mb + j];
+                F = std::move(G);
+                break;
+            }
+            const real phi1 = std::abs(beta * F[m * mx + 0]);
+            const real phi2 = std::abs(beta * F[(m + 1) * mx + 0] * avnorm);
+            if (phi1 > 10.0 * phi2) {
+                err_loc = phi2;
+                xm = 1.0 / real(m);
+            } else if (phi1 > phi2) {
+                err_loc = (phi1 * phi2) / (phi1 - phi2);
+                xm = 1.0 / real(m);
+            } else {
+                err_loc = phi1;
+                xm = 1.0 / real(std::max<idx>(1, m - 1));
+            }
+            if (err_loc <= delta * t_step * tol) {
+                std::vector<real> G(ld * ld, 0.0);
+                for (idx i = 0; i < mx; ++i)
+                    for (idx j = 0; j < mx; ++j) G[i * ld + j] = F[i * mx + j];
+                F = std::move(G);
+                break;
+            }
+            if (rejected == max_reject) throw std::runtime_error("expv: the requested tolerance is too high");
+            t_step = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
+            ++res.rejected;
+        }
+        const idx mx = mb + std::max(0, k1 - 1);
+        std::fill(w, w + n, 0.0);
+        for (idx k = 0; k < mx; ++k) {
+            const real c = beta * F[k * ld + 0];
+            for (idx r = 0; r < n; ++r) w[r] += c * V[k * n + r];
+        }
+        beta = norm2(w);
+        t_now += t_step;
+        t_new = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
+        res.error += std::max(err_loc, rndoff);
+        if (!(beta > 0.0)) break;
     }
     return res;
 }

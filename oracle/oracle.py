@@ -51,6 +51,8 @@ def lib():
         L.orc_mg_create.argtypes = [P, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int64,
                                     C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.orc_mg_relax.argtypes = [P, C.c_int64, f64p, f64p, C.c_int, C.c_int]
+        L.orc_expv_jumps.argtypes = [C.c_int64, i64p, i64p, f64p, f64p, C.c_double, f64p, f64p,
+                                     C.c_int64, C.c_double, i64p, i64p]
         L.orc_mg_destroy.argtypes = [P]
         L.orc_mg_max_levels.argtypes = [P]
         L.orc_mg_max_levels.restype = C.c_int64
@@ -235,3 +237,18 @@ class SectorMultigrid:
         _check(lib().orc_mg_level_apply(self.h, sector, level,
                                         np.ascontiguousarray(x, np.float64), y))
         return y
+
+
+def expv_jumps(first, to, rate, exit_, t, v, krylov_dim=30, tol=1e-8):
+    """w = exp(t Q) v on the CPU (Sidje's expv restated) for a generator given by source-major
+    jump lists; returns (w, steps, rejected)."""
+    first = np.ascontiguousarray(first, np.int64)
+    n = len(first) - 1
+    w = np.zeros(n)
+    steps, rej = np.zeros(1, np.int64), np.zeros(1, np.int64)
+    _check(lib().orc_expv_jumps(n, first, np.ascontiguousarray(to, np.int64),
+                                np.ascontiguousarray(rate, np.float64),
+                                np.ascontiguousarray(exit_, np.float64), float(t),
+                                np.ascontiguousarray(v, np.float64), w, int(krylov_dim), float(tol),
+                                steps, rej))
+    return w, int(steps[0]), int(rej[0])

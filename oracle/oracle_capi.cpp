@@ -275,4 +275,31 @@ int orc_gmres(void *h, void *mg, double shift, const double *b, double *x, doubl
     return r.converged ? 1 : 0;
 }
 
+// w = exp(t Q) v for the generator given by source-major jump lists (first, to, rate) and
+// exit rates: (Q x)_i = -exit_i x_i + sum over jumps j -> i of rate x_j. The norm estimate is
+// ||Q||_1 = max_j (exit_j + sum of j's in-space jump rates).
+int orc_expv_jumps(int64_t n, const int64_t *first, const int64_t *to, const double *rate,
+                   const double *exit, double t, const double *v, double *w, int64_t krylov_dim,
+                   double tol, int64_t *steps, int64_t *rejected) {
+    return guarded([&] {
+        real anorm = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            real c = exit[j];
+            for (int64_t p = first[j]; p < first[j + 1]; ++p) c += rate[p];
+            anorm = std::max(anorm, c);
+        }
+        auto apply = [&](const real *x, real *y) {
+            for (int64_t i = 0; i < n; ++i) y[i] = -exit[i] * x[i];
+            for (int64_t j = 0; j < n; ++j)
+                for (int64_t p = first[j]; p < first[j + 1]; ++p) y[to[p]] += rate[p] * x[j];
+        };
+        expv_options o;
+        o.krylov_dim = static_cast<idx>(krylov_dim);
+        o.tolerance = tol;
+        const auto r = expv(t, static_cast<idx>(n), apply, anorm > 0.0 ? anorm : 1.0, v, w, o);
+        if (steps) *steps = static_cast<int64_t>(r.steps);
+        if (rejected) *rejected = static_cast<int64_t>(r.rejected);
+    });
+}
+
 } // extern "C"

@@ -162,6 +162,29 @@ mk_status mk_gmres_device(mk_generator gen, double shift, mk_sector_mg mg, const
                           double *d_x, const mk_gmres_options *options,
                           mk_solver_result *result);
 
+/* ---- Krylov exponential (finite state projection) -------------------------------------
+ * num::expv(t, Q.matrix(), p, krylov_dim, krylov_tol), called at include/markovkit/fsp.hpp:152
+ * and :225 (numerics source absent): w = exp(t Q) v for the generator, computed as Sidje's
+ * expv (Expokit) with an Arnoldi basis of krylov_dim vectors and local error control at
+ * `tolerance`. Works on any generator handle (mk_generator_create_rdme or _jumps). */
+typedef struct mk_expv_options {
+    int64_t krylov_dim; /* 30 (krylov_fsp_options, fsp.hpp:205) */
+    double tolerance;   /* 1e-8 */
+} mk_expv_options;
+
+typedef struct mk_expv_result {
+    int64_t steps;    /* accepted time steps */
+    int64_t rejected; /* rejected step sizes */
+    double error;     /* sum of the local error estimates */
+} mk_expv_result;
+
+/* host vectors */
+mk_status mk_expv(mk_generator gen, double t, const double *v, double *w,
+                  const mk_expv_options *options, mk_expv_result *result);
+/* v and w in HBM (may alias) */
+mk_status mk_expv_device(mk_generator gen, double t, const double *d_v, double *d_w,
+                         const mk_expv_options *options, mk_expv_result *result);
+
 /* ---- batched resolvent solves ----------------------------------------------------------
  * No single-call counterpart: the reference solves each (shift, rhs) pair with its own
  * num::gmres call and sector_multigrid (examples/rdme_multigrid_benchmark.cpp:92-121). A batch

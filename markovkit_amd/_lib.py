@@ -37,6 +37,14 @@ class SolverResult(C.Structure):
     _fields_ = [("iterations", i64), ("residual", f64), ("converged", i32)]
 
 
+class ExpvOptions(C.Structure):
+    _fields_ = [("krylov_dim", i64), ("tolerance", f64)]
+
+
+class ExpvResult(C.Structure):
+    _fields_ = [("steps", i64), ("rejected", i64), ("error", f64)]
+
+
 SIGNATURES = {
     "mk_last_error": ([], C.c_char_p),
     "mk_default_multigrid_options": ([C.POINTER(MultigridOptions)], None),
@@ -65,6 +73,8 @@ SIGNATURES = {
     "mk_sector_mg_apply_device": ([P, P, P], i32),
     "mk_gmres": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_gmres_device": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
+    "mk_expv": ([P, f64, P, P, C.POINTER(ExpvOptions), C.POINTER(ExpvResult)], i32),
+    "mk_expv_device": ([P, f64, P, P, C.POINTER(ExpvOptions), C.POINTER(ExpvResult)], i32),
     "mk_batch_create": ([P, P, i64, P, C.POINTER(P)], i32),
     "mk_batch_destroy": ([P], i32),
     "mk_batch_solve": ([P, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),

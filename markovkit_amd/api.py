@@ -20,6 +20,8 @@ from typing import Optional, Sequence
 
 import numpy as np
 
+from ._lib import ExpvOptions as _ExpvOptions
+from ._lib import ExpvResult as _ExpvResult
 from ._lib import GmresOptions as _GmresOptions
 from ._lib import MultigridOptions as _MultigridOptions
 from ._lib import SolverResult as _SolverResult
@@ -381,6 +383,35 @@ def gmres(A: ShiftedGenerator, *args, tolerance: float = 1e-10, max_iterations: 
             raise TypeError("x must be a contiguous float64 array")
         check(lib().mk_gmres(A.Q._h, A.shift(), mh, _p(ba), _p(x), C.byref(opts), C.byref(res)))
     return SolverResult(res.iterations, res.residual, bool(res.converged))
+
+
+@dataclass
+class ExpvResult:
+    steps: int
+    rejected: int
+    error: float
+
+
+def expv(Q: Generator, t: float, v, krylov_dim: int = 30, tolerance: float = 1e-8):
+    """num::expv(t, Q.matrix(), v, krylov_dim, tolerance) on the device (fsp.hpp:152, :225):
+    w = exp(t Q) v by Sidje's Krylov expv for a host vector v; returns (w, ExpvResult)."""
+    va = np.ascontiguousarray(v, np.float64)
+    if va.shape != (Q.rows(),):
+        raise ValueError("v must have one entry per state")
+    w = np.zeros_like(va)
+    o = _ExpvOptions(int(krylov_dim), float(tolerance))
+    r = _ExpvResult()
+    check(lib().mk_expv(Q._h, float(t), _p(va), _p(w), C.byref(o), C.byref(r)))
+    return w, ExpvResult(r.steps, r.rejected, r.error)
+
+
+def expv_device(Q: Generator, t: float, v: "DeviceVector", w: "DeviceVector",
+                krylov_dim: int = 30, tolerance: float = 1e-8) -> ExpvResult:
+    """w = exp(t Q) v with both vectors in HBM (they may be the same vector)."""
+    o = _ExpvOptions(int(krylov_dim), float(tolerance))
+    r = _ExpvResult()
+    check(lib().mk_expv_device(Q._h, float(t), v.ptr, w.ptr, C.byref(o), C.byref(r)))
+    return ExpvResult(r.steps, r.rejected, r.error)
 
 
 class BatchSolver:

@@ -382,8 +382,9 @@ struct gmres_workspace {
     }
 };
 
+// one Krylov workspace per host thread
 gmres_workspace &workspace() {
-    static gmres_workspace w;
+    thread_local gmres_workspace w;
     return w;
 }
 
@@ -720,6 +721,11 @@ namespace {
 void finish_generator(mk_generator_s &G, mk::generator_csr &&g) {
     G.n = g.n;
     G.n_jumps = g.n_jumps;
+    // ||Q||_1 = max_j (exit_j + sum of j's in-space jump rates)
+    std::vector<double> out(g.exit.begin(), g.exit.end());
+    for (i64 i = 0; i < g.n; ++i)
+        for (u64 p = g.ptr[i]; p < g.ptr[i + 1]; ++p) out[g.col[p]] += g.val[p];
+    G.norm1 = out.empty() ? 0.0 : *std::max_element(out.begin(), out.end());
     G.A = upload_csr<double>(g);
     G.d_rpart = dbuf(std::max<i64>(G.A.view.n_blocks, 1) * sizeof(double));
     G.exit = std::move(g.exit);

@@ -115,8 +115,11 @@ dbuf upload(const std::vector<T> &v) {
 }
 
 // ------------------------------------------------------------------ library state
+// the library stream of the calling host thread (each thread gets its own, default priority,
+// so solves issued from several threads overlap on the device); streams live until the process
+// ends
 inline hipStream_t stream() {
-    static hipStream_t s = [] {
+    thread_local hipStream_t s = [] {
         hipStream_t t = nullptr;
         HIPCHK(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
         // tuning override of the operator kernels' XCD interleave chunk (0 = identity mapping)
@@ -377,6 +380,7 @@ struct mk_generator_s {
     mk::i64 n = 0;
     mk::i64 n_jumps = 0;
     std::vector<double> exit, leak;
+    double norm1 = 0.0; // ||Q||_1
     mk::rt::dev_csr<double> A;
     mk::rt::dbuf d_rpart;  // one partial residual norm per row block
     mk::rt::dbuf d_x, d_y; // staging for host-buffer calls

@@ -815,6 +815,50 @@ sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::v
     return h;
 }
 
+// ------------------------------------------------------------------ small dense exponential
+std::vector<double> expm_pade(std::vector<double> X, i64 n) {
+    double norm = 0.0;
+    for (i64 i = 0; i < n; ++i) {
+        double r = 0.0;
+        for (i64 j = 0; j < n; ++j) r += std::abs(X[i * n + j]);
+        norm = std::max(norm, r);
+    }
+    const int squarings = norm > 0.5 ? static_cast<int>(std::ceil(std::log2(norm / 0.5))) : 0;
+    const double scale = std::ldexp(1.0, -squarings);
+    for (double &x : X) x *= scale;
+    auto product = [n](const std::vector<double> &A, const std::vector<double> &B) {
+        std::vector<double> C(static_cast<std::size_t>(n * n), 0.0);
+        for (i64 i = 0; i < n; ++i)
+            for (i64 k = 0; k < n; ++k) {
+                const double a = A[i * n + k];
+                if (a == 0.0) continue;
+                for (i64 j = 0; j < n; ++j) C[i * n + j] += a * B[k * n + j];
+            }
+        return C;
+    };
+    // N = sum c_k X^k, D = sum (-1)^k c_k X^k, c_0 = 1, c_k = c_{k-1} (q - k + 1) / ((2q - k + 1) k)
+    constexpr int q = 6;
+    std::vector<double> N(static_cast<std::size_t>(n * n), 0.0), D(N), P(N);
+    for (i64 i = 0; i < n; ++i) P[i * n + i] = 1.0;
+    double c = 1.0;
+    for (int k = 0; k <= q; ++k) {
+        if (k > 0) {
+            c *= double(q - k + 1) / double((2 * q - k + 1) * k);
+            P = product(P, X);
+        }
+        const double sign = (k % 2) ? -1.0 : 1.0;
+        for (std::size_t e = 0; e < N.size(); ++e) {
+            N[e] += c * P[e];
+            D[e] += sign * c * P[e];
+        }
+    }
+    std::vector<double> Dinv;
+    if (!invert_dense(std::move(D), n, Dinv)) throw std::runtime_error("expm: singular Pade denominator");
+    std::vector<double> F = product(Dinv, N);
+    for (int k = 0; k < squarings; ++k) F = product(F, F);
+    return F;
+}
+
 // ------------------------------------------------------------------ coded sliced layout
 namespace {
 

@@ -190,6 +190,10 @@ sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::v
 // dense inverse by Gauss-Jordan with partial pivoting; false when singular
 bool invert_dense(std::vector<double> a, i64 n, std::vector<double> &inverse);
 
+// exp(X) of a small dense matrix (row-major n x n): diagonal degree-6 Pade approximant of
+// X / 2^s with ||X / 2^s||_inf <= 1/2, squared s times
+std::vector<double> expm_pade(std::vector<double> X, i64 n);
+
 // The coded sliced-ELL encoding of a gather-form operator (the device's primary layout):
 // entry k of row 64 s + l is ent[soff[s] + 64 k + l] = column | rate code << 24, with the
 // slice padded to its longest row; rlen / rcode hold each row's length and exit-rate code.

@@ -145,6 +145,11 @@ mk_status mk_shifted_apply_device(mk_generator gen, double shift, const double *
 /* sector_multigrid(domain, box, shift, options): multigrid.hpp:377-384; options may be NULL */
 mk_status mk_sector_mg_create(mk_domain domain, mk_box box, double shift,
                               const mk_multigrid_options *options, mk_sector_mg *out);
+/* Re-target a Jacobi hierarchy to another shift: only the dense terminal inverses depend on it
+ * (multigrid.hpp:247-264), so they are re-inverted and the level operators stay resident. No
+ * reference counterpart (the reference builds a new sector_multigrid per shift). Rejected for
+ * the aggregate-block smoother, whose block inverses depend on the shift. */
+mk_status mk_sector_mg_set_shift(mk_sector_mg mg, double shift);
 mk_status mk_sector_mg_destroy(mk_sector_mg mg);
 /* max_levels(), n_sectors(): multigrid.hpp:388-395 */
 mk_status mk_sector_mg_info(mk_sector_mg mg, int64_t *rows, int64_t *n_sectors,
@@ -205,6 +210,41 @@ mk_status mk_batch_solve(mk_batch batch, const double *B, double *X, const doubl
 mk_status mk_batch_solve_device(mk_batch batch, const double *d_B, double *d_X,
                                 const double *tolerances, const mk_gmres_options *options,
                                 mk_solver_result *results);
+
+/* ---- serving -----------------------------------------------------------------------------
+ * No reference counterpart. A server runs requests on `workers` host threads, each with its own
+ * HIP stream, so requests overlap on the device. Requests are admitted first come first served
+ * when a worker is free, their device footprint (Krylov basis and work vectors) fits
+ * hbm_budget_bytes beside the running ones, and, for a resolvent, a hierarchy of its family is
+ * free. A resolvent family is a generator with a pool of Jacobi sector multigrid handles of the
+ * same box, kept resident and re-targeted to each request's shift (mk_sector_mg_set_shift). The
+ * server does not own the handles; they must outlive it. */
+typedef struct mk_server_s *mk_server;
+enum { MK_REQUEST_RESOLVENT = 0, MK_REQUEST_EXPV = 1 };
+
+typedef struct mk_server_result {
+    int32_t kind;             /* MK_REQUEST_RESOLVENT or MK_REQUEST_EXPV */
+    int32_t status;           /* MK_OK or the failure's status */
+    mk_solver_result solve;   /* resolvent */
+    mk_expv_result expv;      /* exponential */
+    double queued_s;          /* submission to start */
+    double run_s;             /* start to finish */
+} mk_server_result;
+
+mk_status mk_server_create(int32_t workers, int64_t hbm_budget_bytes, mk_server *out);
+/* finishes the queued requests, then joins the workers */
+mk_status mk_server_destroy(mk_server server);
+mk_status mk_server_add_family(mk_server server, mk_generator gen, int32_t n_mg,
+                               const mk_sector_mg *mg, int32_t *family);
+/* (s I - Q) x = b from x = 0 to the absolute tolerance; b is copied at submission */
+mk_status mk_server_submit_resolvent(mk_server server, int32_t family, double shift,
+                                     const double *b, double tolerance,
+                                     const mk_gmres_options *options, int64_t *id);
+/* w = exp(t Q) v; v is copied at submission */
+mk_status mk_server_submit_expv(mk_server server, mk_generator gen, double t, const double *v,
+                                const mk_expv_options *options, int64_t *id);
+/* blocks until request id is done; copies its solution to out (rows of its generator) */
+mk_status mk_server_wait(mk_server server, int64_t id, double *out, mk_server_result *result);
 
 /* ---- device memory and timing (no reference counterpart: HBM-resident callers) ------ */
 mk_status mk_device_alloc(int64_t bytes, void **d_ptr);

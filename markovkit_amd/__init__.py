@@ -7,7 +7,7 @@ in-tree HIP library through its C ABI and has no CPU fallback.
 from .model import (Graph, Grid, LocalReaction, ReactionDiffusion, VoxelDomain, birth_death,
                     box_size, compositions)
 from .api import (ABSORBING, AGGREGATE_BLOCK, FP32, FP64, JACOBI, REFLECTING, BatchSolver,
-                  DeviceVector,
+                  DeviceVector, Server,
                   Generator, MultigridOptions, PopulationBox, SectorMultigrid, ShiftedGenerator,
                   SolverResult, expv, expv_device, gmres, profile_enable, profile_read,
                   profile_reset, synchronize)
@@ -15,7 +15,7 @@ from .api import (ABSORBING, AGGREGATE_BLOCK, FP32, FP64, JACOBI, REFLECTING, Ba
 __all__ = [
     "Graph", "Grid", "LocalReaction", "ReactionDiffusion", "VoxelDomain", "birth_death",
     "box_size", "compositions", "ABSORBING", "AGGREGATE_BLOCK", "FP32", "FP64", "JACOBI",
-    "REFLECTING", "BatchSolver", "DeviceVector", "Generator", "MultigridOptions", "PopulationBox",
+    "REFLECTING", "BatchSolver", "DeviceVector", "Server", "Generator", "MultigridOptions", "PopulationBox",
     "SectorMultigrid", "ShiftedGenerator", "SolverResult", "expv", "expv_device", "gmres",
     "profile_enable",
     "profile_read", "profile_reset", "synchronize",

@@ -45,6 +45,11 @@ class ExpvResult(C.Structure):
     _fields_ = [("steps", i64), ("rejected", i64), ("error", f64)]
 
 
+class ServerResult(C.Structure):
+    _fields_ = [("kind", i32), ("status", i32), ("solve", SolverResult), ("expv", ExpvResult),
+                ("queued_s", f64), ("run_s", f64)]
+
+
 SIGNATURES = {
     "mk_last_error": ([], C.c_char_p),
     "mk_default_multigrid_options": ([C.POINTER(MultigridOptions)], None),
@@ -73,8 +78,15 @@ SIGNATURES = {
     "mk_sector_mg_apply_device": ([P, P, P], i32),
     "mk_gmres": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_gmres_device": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
+    "mk_sector_mg_set_shift": ([P, f64], i32),
     "mk_expv": ([P, f64, P, P, C.POINTER(ExpvOptions), C.POINTER(ExpvResult)], i32),
     "mk_expv_device": ([P, f64, P, P, C.POINTER(ExpvOptions), C.POINTER(ExpvResult)], i32),
+    "mk_server_create": ([i32, i64, C.POINTER(P)], i32),
+    "mk_server_destroy": ([P], i32),
+    "mk_server_add_family": ([P, P, i32, P, C.POINTER(i32)], i32),
+    "mk_server_submit_resolvent": ([P, i32, f64, P, f64, C.POINTER(GmresOptions), C.POINTER(i64)], i32),
+    "mk_server_submit_expv": ([P, P, f64, P, C.POINTER(ExpvOptions), C.POINTER(i64)], i32),
+    "mk_server_wait": ([P, i64, P, C.POINTER(ServerResult)], i32),
     "mk_batch_create": ([P, P, i64, P, C.POINTER(P)], i32),
     "mk_batch_destroy": ([P], i32),
     "mk_batch_solve": ([P, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),

@@ -23,6 +23,7 @@ import numpy as np
 from ._lib import ExpvOptions as _ExpvOptions
 from ._lib import ExpvResult as _ExpvResult
 from ._lib import GmresOptions as _GmresOptions
+from ._lib import ServerResult as _ServerResult
 from ._lib import MultigridOptions as _MultigridOptions
 from ._lib import SolverResult as _SolverResult
 from ._lib import check, lib
@@ -334,6 +335,11 @@ class SectorMultigrid:
     def shift(self) -> float:
         return self._shift
 
+    def set_shift(self, shift: float) -> None:
+        """Re-target the hierarchy to another shift (terminal blocks re-inverted; Jacobi only)."""
+        check(lib().mk_sector_mg_set_shift(self._h, float(shift)))
+        self._shift = float(shift)
+
     def apply(self, b) -> np.ndarray:
         ba = np.ascontiguousarray(b, np.float64)
         x = np.zeros(self._rows)
@@ -460,6 +466,80 @@ class BatchSolver:
         check(lib().mk_batch_solve(self._h, _p(Ba), _p(X), _p(tol) if tol is not None else None,
                                    C.byref(opts), res))
         return X, [SolverResult(r.iterations, r.residual, bool(r.converged)) for r in res]
+
+
+@dataclass
+class ServedResult:
+    kind: str          # "resolvent" or "expv"
+    solve: Optional[SolverResult]
+    expv: Optional[ExpvResult]
+    queued_s: float
+    run_s: float
+
+
+class Server:
+    """Request serving on worker threads with their own HIP streams (mk_server_*): resolvent
+    solves of registered families (a generator and a pool of Jacobi hierarchies, re-targeted
+    to each request's shift) and Krylov exponentials, admitted first come first served by
+    free worker, HBM footprint and free hierarchy."""
+
+    def __init__(self, workers: int = 4, hbm_budget: int = 64 << 30):
+        self._h = C.c_void_p()
+        check(lib().mk_server_create(int(workers), int(hbm_budget), C.byref(self._h)))
+        self._keep = []
+        self._rows = {}
+        self._family_rows = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mk_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_family(self, Q: Generator, hierarchies: Sequence[SectorMultigrid]) -> int:
+        arr = (C.c_void_p * len(hierarchies))(*[m._h.value for m in hierarchies])
+        fam = C.c_int32()
+        check(lib().mk_server_add_family(self._h, Q._h, len(hierarchies), arr, C.byref(fam)))
+        self._keep.append((Q, list(hierarchies)))
+        self._family_rows.append(Q.rows())
+        return fam.value
+
+    def submit_resolvent(self, family: int, shift: float, b, tolerance: float,
+                         restart: int = 50, max_iterations: int = 1000) -> int:
+        ba = np.ascontiguousarray(b, np.float64)
+        if ba.shape != (self._family_rows[family],):
+            raise ValueError("b must have one entry per state of the family")
+        o = _GmresOptions(0.0, int(max_iterations), int(restart), 0)
+        rid = C.c_int64()
+        check(lib().mk_server_submit_resolvent(self._h, int(family), float(shift), _p(ba),
+                                               float(tolerance), C.byref(o), C.byref(rid)))
+        self._rows[rid.value] = len(ba)
+        return rid.value
+
+    def submit_expv(self, Q: Generator, t: float, v, krylov_dim: int = 30,
+                    tolerance: float = 1e-8) -> int:
+        va = np.ascontiguousarray(v, np.float64)
+        o = _ExpvOptions(int(krylov_dim), float(tolerance))
+        rid = C.c_int64()
+        check(lib().mk_server_submit_expv(self._h, Q._h, float(t), _p(va), C.byref(o),
+                                          C.byref(rid)))
+        self._keep.append((Q,))
+        self._rows[rid.value] = len(va)
+        return rid.value
+
+    def wait(self, rid: int):
+        """Block until request rid is done; returns (solution, ServedResult)."""
+        out = np.zeros(self._rows.pop(rid))
+        r = _ServerResult()
+        check(lib().mk_server_wait(self._h, int(rid), _p(out), C.byref(r)))
+        if r.kind == 0:
+            return out, ServedResult("resolvent", SolverResult(r.solve.iterations, r.solve.residual,
+                                                               bool(r.solve.converged)), None,
+                                     r.queued_s, r.run_s)
+        return out, ServedResult("expv", None, ExpvResult(r.expv.steps, r.expv.rejected,
+                                                          r.expv.error), r.queued_s, r.run_s)
 
 
 KERNEL_CLASSES = ("shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict",

@@ -355,6 +355,8 @@ struct gmres_workspace {
     i64 m = 0;
     std::size_t zs = 0;
     dbuf V, Z, w, partial, hd, yd;
+    dbuf rpart; // one partial residual norm per operator row block
+    i64 rpart_n = 0;
     // device scalars hd and their pinned host copy: pass-1 projections at 0, pass-2
     // projections and |w|^2 at m + 2, (beta, flag) at 2 (m + 2); y follows at 3 (m + 2)
     double *host = nullptr;
@@ -415,6 +417,11 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
     const bool delayed = M && o.orthogonalization == MK_DCGS2 && m <= gpu::kFusedBasis;
     auto &ws = workspace();
     ws.ensure(n, m, M ? sizeof(ZT) : 0);
+    if (ws.rpart_n < A.A.view.n_blocks + 1) {
+        ws.rpart = dbuf();
+        ws.rpart_n = A.A.view.n_blocks + 1;
+        ws.rpart = dbuf(static_cast<std::size_t>(ws.rpart_n) * sizeof(double));
+    }
     auto &P = prof();
     const hipStream_t s = stream();
     const int nb = gpu::reduction_blocks(n);
@@ -430,9 +437,9 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
     const int rb = static_cast<int>(std::max<i64>(A.A.view.n_blocks, 1));
     auto true_residual = [&]() {
         P.run(K_SPMV, spmv_bytes + n * 8.0, [&] {
-            gpu::shifted_apply<double>(A.A.view, shift, x, b, V, dp<double>(A.d_rpart), s);
+            gpu::shifted_apply<double>(A.A.view, shift, x, b, V, dp<double>(ws.rpart), s);
         });
-        return std::sqrt(device_norm2(dp<double>(A.d_rpart), rb, hd));
+        return std::sqrt(device_norm2(dp<double>(ws.rpart), rb, hd));
     };
     // w = A M (scale v)
     auto operator_image = [&](i64 j, const double *v, double scale) {
@@ -625,6 +632,30 @@ mk_solver_result gmres_device(mk_generator_s &A, double shift, mk_sector_mg_s *M
 
 } // namespace
 
+namespace {
+// re-invert the terminal blocks of every level for another shift (the level operators and
+// transfers do not depend on it)
+template <typename T>
+void reshift_levels(std::vector<level_dev<T>> &L, double shift) {
+    for (auto &lv : L) {
+        if (lv.n_term == 0) continue;
+        std::vector<T> inv(lv.h_tq.size());
+        for (i64 b = 0; b < lv.n_term; ++b) {
+            const i64 m = lv.h_tsize[b];
+            std::vector<double> a(lv.h_tq.begin() + lv.h_toff[b], lv.h_tq.begin() + lv.h_toff[b] + m * m);
+            for (i64 i = 0; i < m; ++i) a[i * m + i] += shift;
+            std::vector<double> out;
+            if (!mk::invert_dense(std::move(a), m, out))
+                throw std::runtime_error("population_multigrid: singular block");
+            std::copy(out.begin(), out.end(), inv.begin() + lv.h_toff[b]);
+        }
+        HIPCHK(hipMemcpyAsync(lv.tinv.p, inv.data(), inv.size() * sizeof(T), hipMemcpyHostToDevice,
+                              stream()));
+        drain();
+    }
+}
+} // namespace
+
 // ====================================================================== C ABI
 extern "C" {
 
@@ -727,7 +758,6 @@ void finish_generator(mk_generator_s &G, mk::generator_csr &&g) {
         for (u64 p = g.ptr[i]; p < g.ptr[i + 1]; ++p) out[g.col[p]] += g.val[p];
     G.norm1 = out.empty() ? 0.0 : *std::max_element(out.begin(), out.end());
     G.A = upload_csr<double>(g);
-    G.d_rpart = dbuf(std::max<i64>(G.A.view.n_blocks, 1) * sizeof(double));
     G.exit = std::move(g.exit);
     G.leak = std::move(g.leak);
 }
@@ -866,6 +896,21 @@ mk_status mk_sector_mg_create(mk_domain domain, mk_box box, double shift,
             build_levels<double>(*mg, H, mg->L64);
         }
         *out = mg.release();
+    });
+}
+
+mk_status mk_sector_mg_set_shift(mk_sector_mg mg, double shift) {
+    return guarded([&] {
+        require(mg != nullptr, "sector_multigrid: bad arguments");
+        if (!(shift > 0.0)) throw std::invalid_argument("population_multigrid: shift must be positive");
+        require(mg->opt.smoother == MK_JACOBI,
+                "sector_multigrid: the aggregate-block smoother's blocks depend on the shift; build a new one");
+        if (mg->precision == MK_FP32) {
+            reshift_levels<float>(mg->L32, shift);
+        } else {
+            reshift_levels<double>(mg->L64, shift);
+        }
+        mg->shift = shift;
     });
 }
 

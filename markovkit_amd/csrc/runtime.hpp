@@ -382,7 +382,6 @@ struct mk_generator_s {
     std::vector<double> exit, leak;
     double norm1 = 0.0; // ||Q||_1
     mk::rt::dev_csr<double> A;
-    mk::rt::dbuf d_rpart;  // one partial residual norm per row block
     mk::rt::dbuf d_x, d_y; // staging for host-buffer calls
 };
 

@@ -1,0 +1,280 @@
+// Request serving: resolvent solves and Krylov exponentials admitted to worker threads by HBM
+// footprint. No reference counterpart (the reference is a library called one solve at a time).
+//
+// Each worker thread owns a HIP stream (the library stream is per thread) and its Krylov
+// workspaces, so the solves of different workers overlap on the device. A request is admitted,
+// first come first served, once a worker is free, its footprint fits the HBM budget beside the
+// running requests and, for a resolvent, a multigrid hierarchy of its family is free. Resolvent
+// families keep their generator and a pool of hierarchies resident; a hierarchy is re-targeted
+// to each request's shift (terminal inverses only). Results do not depend on what else runs:
+// every request computes on its own stream with fixed-order reductions.
+#include "runtime.hpp"
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+using namespace mk::rt;
+
+namespace {
+
+using clock_type = std::chrono::steady_clock;
+
+struct request {
+    i64 id = 0;
+    int kind = MK_REQUEST_RESOLVENT;
+    int family = -1;
+    mk_generator gen = nullptr;
+    double shift = 0.0, t = 0.0, tolerance = 0.0;
+    mk_gmres_options gopt{};
+    mk_expv_options eopt{};
+    std::vector<double> in, out;
+    i64 footprint = 0;
+    bool done = false;
+    mk_status status = MK_OK;
+    std::string error;
+    mk_solver_result solve{};
+    mk_expv_result expv{};
+    clock_type::time_point submitted, started, finished;
+};
+
+struct family {
+    mk_generator gen = nullptr;
+    std::vector<mk_sector_mg> free;
+    i64 n = 0;
+    std::size_t zs = 8;
+};
+
+// device footprint of a request beyond the resident operators: the Krylov basis (m + 1
+// vectors), Z (m vectors in the preconditioner's precision) or the expv basis, and the work
+// vectors
+i64 resolvent_footprint(i64 n, i64 m, std::size_t zs) {
+    const i64 ld = (n + 63) / 64 * 64;
+    return ld * (8 * (m + 1) + static_cast<i64>(zs) * m + 8 * 4);
+}
+i64 expv_footprint(i64 n, i64 m) {
+    const i64 ld = (n + 63) / 64 * 64;
+    return ld * (8 * (m + 1) + 8 * 4);
+}
+
+// device in/out buffers of one worker thread
+struct worker_buffers {
+    dbuf a, b;
+    void ensure(i64 n) {
+        const std::size_t bytes = static_cast<std::size_t>(std::max<i64>(n, 1)) * sizeof(double);
+        if (a.bytes < bytes) a = dbuf(bytes);
+        if (b.bytes < bytes) b = dbuf(bytes);
+    }
+};
+
+} // namespace
+
+struct mk_server_s {
+    i64 budget = 0;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<request *> queue;
+    std::map<i64, std::unique_ptr<request>> requests;
+    std::vector<family> families;
+    i64 reserved = 0;
+    i64 next_id = 1;
+    bool stopping = false;
+    std::vector<std::thread> threads;
+
+    bool admissible(const request &r) const {
+        if (reserved + r.footprint > budget && reserved > 0) return false;
+        if (r.kind == MK_REQUEST_RESOLVENT && families[r.family].free.empty()) return false;
+        return true;
+    }
+
+    void run(request &r, worker_buffers &buf, mk_sector_mg mg) {
+        const hipStream_t s = stream();
+        const i64 n = static_cast<i64>(r.in.size());
+        buf.ensure(n);
+        double *da = dp<double>(buf.a), *db = dp<double>(buf.b);
+        HIPCHK(hipMemcpyAsync(da, r.in.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+        mk_status st;
+        if (r.kind == MK_REQUEST_RESOLVENT) {
+            HIPCHK(hipMemsetAsync(db, 0, n * sizeof(double), s));
+            st = mk_sector_mg_set_shift(mg, r.shift);
+            if (st == MK_OK) {
+                mk_gmres_options o = r.gopt;
+                o.tolerance = r.tolerance;
+                st = mk_gmres_device(families[r.family].gen, r.shift, mg, da, db, &o, &r.solve);
+            }
+        } else {
+            st = mk_expv_device(r.gen, r.t, da, db, &r.eopt, &r.expv);
+        }
+        if (st != MK_OK) throw std::runtime_error(mk_last_error());
+        r.out.resize(n);
+        HIPCHK(hipMemcpyAsync(r.out.data(), db, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+
+    void worker() {
+        worker_buffers buf;
+        std::unique_lock<std::mutex> lock(mu);
+        while (true) {
+            // a request that is not admissible now becomes so when a running one finishes
+            cv.wait(lock, [&] {
+                return (stopping && queue.empty()) || (!queue.empty() && admissible(*queue.front()));
+            });
+            if (queue.empty()) return;
+            request *r = queue.front();
+            queue.pop_front();
+            reserved += r->footprint;
+            mk_sector_mg mg = nullptr;
+            if (r->kind == MK_REQUEST_RESOLVENT) {
+                mg = families[r->family].free.back();
+                families[r->family].free.pop_back();
+            }
+            r->started = clock_type::now();
+            lock.unlock();
+            mk_status status = MK_OK;
+            std::string error;
+            try {
+                run(*r, buf, mg);
+            } catch (const std::exception &e) {
+                status = MK_ERR_RUNTIME;
+                error = e.what();
+            }
+            lock.lock();
+            r->finished = clock_type::now();
+            r->status = status;
+            r->error = error;
+            r->done = true;
+            reserved -= r->footprint;
+            if (mg) families[r->family].free.push_back(mg);
+            cv.notify_all();
+        }
+    }
+};
+
+namespace {
+
+double seconds(clock_type::duration d) { return std::chrono::duration<double>(d).count(); }
+
+i64 submit(mk_server s, std::unique_ptr<request> r) {
+    std::lock_guard<std::mutex> lock(s->mu);
+    require(!s->stopping, "server: stopping");
+    r->id = s->next_id++;
+    r->submitted = clock_type::now();
+    const i64 id = r->id;
+    s->queue.push_back(r.get());
+    s->requests[id] = std::move(r);
+    s->cv.notify_all();
+    return id;
+}
+
+} // namespace
+
+extern "C" {
+
+mk_status mk_server_create(int32_t workers, int64_t hbm_budget_bytes, mk_server *out) {
+    return guarded([&] {
+        require(out && workers >= 1 && workers <= 64 && hbm_budget_bytes > 0, "server: bad arguments");
+        auto s = std::make_unique<mk_server_s>();
+        s->budget = hbm_budget_bytes;
+        for (int w = 0; w < workers; ++w) s->threads.emplace_back([p = s.get()] { p->worker(); });
+        *out = s.release();
+    });
+}
+
+mk_status mk_server_destroy(mk_server s) {
+    if (!s) return MK_OK;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->stopping = true;
+        s->cv.notify_all();
+    }
+    for (auto &t : s->threads) t.join();
+    delete s;
+    return MK_OK;
+}
+
+mk_status mk_server_add_family(mk_server s, mk_generator gen, int32_t n_mg, const mk_sector_mg *mg,
+                               int32_t *family_out) {
+    return guarded([&] {
+        require(s && gen && mg && family_out && n_mg >= 1, "server: bad family");
+        family f;
+        f.gen = gen;
+        f.n = gen->n;
+        for (int k = 0; k < n_mg; ++k) {
+            require(mg[k] && mg[k]->rows == gen->n, "server: a hierarchy does not match the generator");
+            require(mg[k]->opt.smoother == MK_JACOBI, "server: hierarchies must use the Jacobi smoother");
+            f.zs = mg[k]->precision == MK_FP32 ? sizeof(float) : sizeof(double);
+            f.free.push_back(mg[k]);
+        }
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->families.push_back(std::move(f));
+        *family_out = static_cast<int32_t>(s->families.size() - 1);
+    });
+}
+
+mk_status mk_server_submit_resolvent(mk_server s, int32_t family, double shift, const double *b,
+                                     double tolerance, const mk_gmres_options *options,
+                                     int64_t *id) {
+    return guarded([&] {
+        require(s && b && options && id, "server: bad arguments");
+        if (!(shift > 0.0)) throw std::invalid_argument("population_multigrid: shift must be positive");
+        i64 n;
+        std::size_t zs;
+        {
+            std::lock_guard<std::mutex> lock(s->mu);
+            require(family >= 0 && family < static_cast<int32_t>(s->families.size()), "server: unknown family");
+            n = s->families[family].n;
+            zs = s->families[family].zs;
+        }
+        auto r = std::make_unique<request>();
+        r->kind = MK_REQUEST_RESOLVENT;
+        r->family = family;
+        r->shift = shift;
+        r->tolerance = tolerance;
+        r->gopt = *options;
+        r->in.assign(b, b + n);
+        r->footprint = resolvent_footprint(n, std::max<i64>(1, options->restart), zs);
+        *id = submit(s, std::move(r));
+    });
+}
+
+mk_status mk_server_submit_expv(mk_server s, mk_generator gen, double t, const double *v,
+                                const mk_expv_options *options, int64_t *id) {
+    return guarded([&] {
+        require(s && gen && v && options && id, "server: bad arguments");
+        auto r = std::make_unique<request>();
+        r->kind = MK_REQUEST_EXPV;
+        r->gen = gen;
+        r->t = t;
+        r->eopt = *options;
+        r->in.assign(v, v + gen->n);
+        r->footprint = expv_footprint(gen->n, std::max<i64>(1, options->krylov_dim));
+        *id = submit(s, std::move(r));
+    });
+}
+
+mk_status mk_server_wait(mk_server s, int64_t id, double *out, mk_server_result *result) {
+    return guarded([&] {
+        require(s && result, "server: bad arguments");
+        std::unique_lock<std::mutex> lock(s->mu);
+        auto it = s->requests.find(id);
+        require(it != s->requests.end(), "server: unknown request");
+        request &r = *it->second;
+        s->cv.wait(lock, [&] { return r.done; });
+        result->kind = r.kind;
+        result->status = r.status;
+        result->solve = r.solve;
+        result->expv = r.expv;
+        result->queued_s = seconds(r.started - r.submitted);
+        result->run_s = seconds(r.finished - r.started);
+        if (out && r.status == MK_OK) std::copy(r.out.begin(), r.out.end(), out);
+        const mk_status st = r.status;
+        const std::string err = r.error;
+        s->requests.erase(it);
+        if (st != MK_OK) throw std::runtime_error("server: request failed: " + err);
+    });
+}
+
+} // extern "C"

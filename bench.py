@@ -161,12 +161,106 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16):
     }
 
 
+def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0):
+    """C5: Poisson arrivals at `rate` requests/s (numpy PCG64, seed 1). 70% are RDME resolvent
+    requests of the 1D family (voxels in {16, 32, 48, 64}, caps in {3, 4, 5}, shift ~ U[0.1, 10],
+    b ~ U[0,1) L1-normalised, tolerance 1e-10 relative, FGMRES(50) with an FP32 sector
+    multigrid), 30% toggle-switch FSP exponentials (0..200 x 0..200 = 40,401 states, absorbing
+    box, p0 at a random state, exp(5 Q) p0 at tolerance 1e-8). Families are set up before the
+    arrivals start (generator and two resident hierarchies each); requests are submitted at
+    their arrival times to a server with `workers` worker streams."""
+    from markovkit_amd.fsp import box_generator, toggle_switch
+    rng = np.random.default_rng(seed)
+    kinds = rng.random(n_requests) < 0.7
+    voxels = rng.choice([16, 32, 48, 64], n_requests)
+    caps = rng.choice([3, 4, 5], n_requests)
+    shifts = rng.uniform(0.1, 10.0, n_requests)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    t = time.perf_counter()
+    fams = {}
+    for v, c in sorted({(int(v), int(c)) for v, c, k in zip(voxels, caps, kinds) if k}):
+        w = workloads.birth_death_1d(v, c, 1.0, "random")
+        box = mk.PopulationBox(w.n_voxels, w.max_totals)
+        Q = mk.Generator(box, w.model)
+        mgs = [mk.SectorMultigrid(w.model.domain, box, 1.0, mk.MultigridOptions(precision=mk.FP32))
+               for _ in range(2)]
+        fams[(v, c)] = (Q, mgs)
+    J = box_generator(toggle_switch(), [200, 200], absorbing=True)
+    F = mk.Generator.from_jumps(*J)
+    setup = time.perf_counter() - t
+    log(f"[serving] {len(fams)} resolvent families and the toggle switch set up in {setup:.1f}s")
+    srv = mk.Server(workers=workers, hbm_budget=96 << 30)
+    fid = {key: srv.add_family(Q, mgs) for key, (Q, mgs) in fams.items()}
+    payload = []
+    for i in range(n_requests):
+        if kinds[i]:
+            n = fams[(int(voxels[i]), int(caps[i]))][0].rows()
+            b = rng.random(n)
+            payload.append(b / b.sum())
+        else:
+            v = np.zeros(F.rows())
+            v[int(rng.integers(0, F.rows()))] = 1.0
+            payload.append(v)
+    # warm-up: one request of every kind through the server
+    warm = [srv.submit_resolvent(fid[key], 1.0, np.full(Q.rows(), 1.0 / Q.rows()), 1e-8)
+            for key, (Q, _) in fams.items()]
+    v0 = np.zeros(F.rows())
+    v0[0] = 1.0
+    warm.append(srv.submit_expv(F, 0.1, v0))
+    for r in warm:
+        srv.wait(r)
+    ids = []
+    t0 = time.perf_counter()
+    for i in range(n_requests):
+        delay = t0 + arrivals[i] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        if kinds[i]:
+            b = payload[i]
+            ids.append(srv.submit_resolvent(fid[(int(voxels[i]), int(caps[i]))], float(shifts[i]), b,
+                                            1e-10 * float(np.linalg.norm(b))))
+        else:
+            ids.append(srv.submit_expv(F, 5.0, payload[i], tolerance=1e-8))
+    lat, lat_res, lat_fsp, run = [], [], [], []
+    for i, r in enumerate(ids):
+        _, info = srv.wait(r)
+        if info.kind == "resolvent" and not info.solve.converged:
+            raise RuntimeError("serving: a resolvent request did not converge")
+        latency = info.queued_s + info.run_s
+        lat.append(latency)
+        run.append(info.run_s)
+        (lat_res if info.kind == "resolvent" else lat_fsp).append(latency)
+    total = time.perf_counter() - t0
+    srv.close()
+    return {
+        "workload": (f"{n_requests} requests, Poisson {rate:.0f}/s (seed {seed}): 70% 1D RDME "
+                     f"resolvents (voxels 16-64, caps 3-5, s ~ U[0.1,10], 1e-10 relative), 30% "
+                     f"toggle-switch FSP expv (40,401 states, t = 5, 1e-8); {workers} worker streams"),
+        "value": n_requests / total,
+        "unit": "requests/s",
+        "offered_rate": rate,
+        "p50_ms": 1e3 * percentile(lat, 50),
+        "p99_ms": 1e3 * percentile(lat, 99),
+        "resolvent_p50_ms": 1e3 * percentile(lat_res, 50) if lat_res else None,
+        "resolvent_p99_ms": 1e3 * percentile(lat_res, 99) if lat_res else None,
+        "fsp_p50_ms": 1e3 * percentile(lat_fsp, 50) if lat_fsp else None,
+        "fsp_p99_ms": 1e3 * percentile(lat_fsp, 99) if lat_fsp else None,
+        "mean_run_ms": 1e3 * float(np.mean(run)),
+        "makespan_s": total,
+        "setup_s": setup,
+        "families": len(fams),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="large", choices=["large", "small", "2d", "multishift"])
+    ap.add_argument("--workload", default="large",
+                    choices=["large", "small", "2d", "multishift", "serving"])
+    ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
+    ap.add_argument("--workers", type=int, default=4, help="serving: worker streams")
     ap.add_argument("--restart", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
@@ -181,6 +275,16 @@ def main():
     import markovkit_amd as mk
     from markovkit_amd import workloads
 
+    if args.workload == "serving":
+        sv = serving(mk, workloads, args.requests, args.workers)
+        out = {"metric": "served RDME resolvent + FSP requests/s with p50/p99 latency",
+               "value": sv["value"], "unit": "requests/s", "n_gpus": 1, "steps": args.requests,
+               "warmup": 1, "ms_per_step": 1e3 * sv["makespan_s"] / args.requests,
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f64 (f32 multigrid)", "data": "synthetic",
+               "config": {"workload": sv["workload"]}, "serving": sv}
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == "multishift":
         ms = multishift(mk, workloads, args.steps, args.restart)
         out = {"metric": "batched multi-shift RDME resolvent solves/s at <=1e-10 residual",

@@ -542,6 +542,11 @@ class Server:
                                                           r.expv.error), r.queued_s, r.run_s)
 
 
+def graphs_enable(on: bool = True) -> None:
+    """Replay repeated solver iterations and expv steps as captured HIP graphs (default on)."""
+    check(lib().mk_graphs_enable(1 if on else 0))
+
+
 KERNEL_CLASSES = ("shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict",
                   "prolong", "terminal", "gmres_dot", "gmres_update", "gmres_other")
 

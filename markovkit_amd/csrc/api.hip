@@ -231,7 +231,7 @@ void block_smooth(const mk_sector_mg_s &mg, level_dev<T> &lv, const T *rhs, T *x
 // result is written to out when given, else left in a level buffer whose pointer is returned.
 template <typename T>
 T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t l, const T *rhs,
-             const double *rhs64, T *out, double scale = 1.0) {
+             const double *rhs64, T *out, const double *scale = nullptr) {
     auto &lv = L[l];
     auto &P = prof();
     const hipStream_t s = stream();
@@ -337,11 +337,12 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
     return cur;
 }
 
-// z = M (scale b) on the device: b FP64, z in the hierarchy's precision
-void mg_apply_device(mk_sector_mg_s &mg, const double *b, void *z, double scale = 1.0) {
+// z = M (scale b) on the device: b FP64, z in the hierarchy's precision; scale is read from
+// device memory (1 when null)
+void mg_apply_device(mk_sector_mg_s &mg, const double *b, void *z, const double *scale = nullptr) {
     if (mg.precision == MK_FP32) {
         run_level<float>(mg, mg.L32, 0, nullptr, b, static_cast<float *>(z), scale);
-    } else if (scale == 1.0) {
+    } else if (!scale) {
         run_level<double>(mg, mg.L64, 0, b, nullptr, static_cast<double *>(z));
     } else {
         run_level<double>(mg, mg.L64, 0, nullptr, b, static_cast<double *>(z), scale);
@@ -357,11 +358,16 @@ struct gmres_workspace {
     dbuf V, Z, w, partial, hd, yd;
     dbuf rpart; // one partial residual norm per operator row block
     i64 rpart_n = 0;
+    dbuf scal;                 // per-iteration scalars read by the kernels: 1 / beta_j, beta_j
+    double *hscal = nullptr;   // their pinned host copy
+    graph_cache graphs;        // captured iterations (pointers into this workspace)
     // device scalars hd and their pinned host copy: pass-1 projections at 0, pass-2
     // projections and |w|^2 at m + 2, (beta, flag) at 2 (m + 2); y follows at 3 (m + 2)
     double *host = nullptr;
     ~gmres_workspace() {
+        graphs.clear();
         if (host) (void)hipHostFree(host);
+        if (hscal) (void)hipHostFree(hscal);
     }
     void ensure(i64 n_, i64 m_, std::size_t zs_) {
         if (n_ <= n && m_ <= m && (zs_ == 0 || zs_ <= zs) && V.p) return;
@@ -381,6 +387,9 @@ struct gmres_workspace {
         if (host) (void)hipHostFree(host);
         host = nullptr;
         HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&host), (4 * (m + 2)) * sizeof(double)));
+        if (!hscal) HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&hscal), 2 * sizeof(double)));
+        scal = dbuf(2 * sizeof(double));
+        graphs.clear();
     }
 };
 
@@ -441,8 +450,8 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
         });
         return std::sqrt(device_norm2(dp<double>(ws.rpart), rb, hd));
     };
-    // w = A M (scale v)
-    auto operator_image = [&](i64 j, const double *v, double scale) {
+    // w = A M (scale v), scale read from device memory (1 when null)
+    auto operator_image = [&](i64 j, const double *v, const double *scale) {
         if (M) {
             ZT *Zj = Z + j * ld;
             mg_apply_device(*M, v, Zj, scale);
@@ -477,17 +486,25 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
             for (; j < m && res.iterations < o.max_iterations; ++j) {
                 ++res.iterations;
                 const int cnt = static_cast<int>(j);
-                operator_image(j, V + j * ld, 1.0 / bj);
-                // t = Q^T w and u_j^T w in hd[0 .. j]
-                P.run(K_DOT, double(cnt + 2) * n * 8.0, [&] {
-                    gpu::multi_dot(V, ld, cnt + 1, w, n, partial, nb, s);
-                    gpu::reduce_partials(partial, nb, cnt + 1, hd, s);
-                });
-                // q_j, eta (hd[m + 1]), u_{j+1} and its projections s_{j+1}, |u_{j+1}|^2
-                P.run(K_UPDATE, double(cnt + 4) * n * 8.0, [&] {
-                    gpu::dcgs_update(V, ld, cnt, sd, hd, bj, w, n, partial, nb, hd + m + 1, s);
-                    gpu::reduce_partials(partial, nb, cnt + 2, sd_next, s);
-                });
+                ws.hscal[0] = 1.0 / bj;
+                ws.hscal[1] = bj;
+                double *scal = dp<double>(ws.scal);
+                HIPCHK(hipMemcpyAsync(scal, ws.hscal, 2 * sizeof(double), hipMemcpyHostToDevice, s));
+                graph_key key{A.uid, M ? M->uid : 0, j, static_cast<i64>(sd > sd_next), n, m,
+                              bits(shift)};
+                ws.graphs.run(key, s, [&] {
+                    operator_image(j, V + j * ld, scal);
+                    // t = Q^T w and u_j^T w in hd[0 .. j]
+                    P.run(K_DOT, double(cnt + 2) * n * 8.0, [&] {
+                        gpu::multi_dot(V, ld, cnt + 1, w, n, partial, nb, s);
+                        gpu::reduce_partials(partial, nb, cnt + 1, hd, s);
+                    });
+                    // q_j, eta (hd[m + 1]), u_{j+1} and its projections s_{j+1}, |u_{j+1}|^2
+                    P.run(K_UPDATE, double(cnt + 4) * n * 8.0, [&] {
+                        gpu::dcgs_update(V, ld, cnt, sd, hd, scal + 1, w, n, partial, nb, hd + m + 1, s);
+                        gpu::reduce_partials(partial, nb, cnt + 2, sd_next, s);
+                    });
+                }, n);
                 fetch(3 * hs);
                 const double *t = ws.host, *sn = ws.host + (sd_next - hd);
                 const double eta = ws.host[m + 1];
@@ -531,7 +548,7 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
             for (; j < m && res.iterations < o.max_iterations; ++j) {
                 ++res.iterations;
                 double *Vj = V + j * ld;
-                operator_image(j, Vj, 1.0);
+                operator_image(j, Vj, nullptr);
                 // classical Gram-Schmidt, twice (CGS2)
                 const int cnt = static_cast<int>(j + 1);
                 double *h1d = hd, *cwd = hd + hs, *bfd = hd + 2 * hs;
@@ -660,6 +677,11 @@ void reshift_levels(std::vector<level_dev<T>> &L, double shift) {
 extern "C" {
 
 const char *mk_last_error(void) { return last_error().c_str(); }
+
+mk_status mk_graphs_enable(int32_t on) {
+    graphs_switch().store(on ? 1 : 0);
+    return MK_OK;
+}
 
 void mk_default_multigrid_options(mk_multigrid_options *o) {
     o->pre_sweeps = 2;

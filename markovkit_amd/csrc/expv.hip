@@ -14,9 +14,11 @@ namespace {
 
 struct expv_workspace {
     i64 n = 0, ld = 0, m = 0;
-    dbuf V, p, partial, hd;
+    dbuf V, p, x, partial, hd;
     double *host = nullptr;
+    graph_cache graphs; // one captured Arnoldi per (generator, n, m)
     ~expv_workspace() {
+        graphs.clear();
         if (host) (void)hipHostFree(host);
     }
     // device scalars: h1 and h2 projections ((m + 1) x m each, column-major), |p|^2 per step
@@ -35,6 +37,8 @@ struct expv_workspace {
         V = dbuf();
         V = dbuf(static_cast<std::size_t>(ld) * (m + 1) * sizeof(double));
         p = dbuf(static_cast<std::size_t>(ld) * sizeof(double));
+        x = dbuf(static_cast<std::size_t>(ld) * sizeof(double));
+        graphs.clear();
         partial = dbuf(static_cast<std::size_t>(gpu::reduction_blocks(n)) * (m + 2) * sizeof(double));
         hd = dbuf(total() * sizeof(double));
         if (host) (void)hipHostFree(host);
@@ -53,7 +57,7 @@ double round_step(double step) {
     return std::ceil(step / s) * s;
 }
 
-mk_expv_result expv_device(mk_generator_s &G, double t, const double *v, double *w,
+mk_expv_result expv_device(mk_generator_s &G, double t, const double *v, double *w_out,
                            const mk_expv_options &o) {
     require(o.krylov_dim >= 1 && o.tolerance > 0.0, "expv: bad options");
     mk_expv_result res{0, 0, 0.0};
@@ -86,11 +90,17 @@ mk_expv_result expv_device(mk_generator_s &G, double t, const double *v, double 
         HIPCHK(hipMemcpyAsync(ws.host + off, hd + off, count * sizeof(double), hipMemcpyDeviceToHost, s));
         drain();
     };
-    if (w != v) HIPCHK(hipMemcpyAsync(w, v, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // the iterate lives in the workspace (fixed pointers for the captured Arnoldi)
+    double *w = dp<double>(ws.x);
+    HIPCHK(hipMemcpyAsync(w, v, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    auto finish = [&] {
+        HIPCHK(hipMemcpyAsync(w_out, w, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        return res;
+    };
     norm2_to(w, hd + ws.beta2());
     read(ws.beta2(), 1);
     double beta = std::sqrt(ws.host[ws.beta2()]);
-    if (!(beta > 0.0) || t_out == 0.0) return res;
+    if (!(beta > 0.0) || t_out == 0.0) return finish();
     double xm = 1.0 / double(m);
     const double fact = std::pow(double(m + 1) / std::exp(1.0), double(m + 1)) *
                         std::sqrt(2.0 * 3.141592653589793 * double(m + 1));
@@ -101,6 +111,7 @@ mk_expv_result expv_device(mk_generator_s &G, double t, const double *v, double 
     while (t_now < t_out) {
         ++res.steps;
         double t_step = std::min(t_out - t_now, t_new);
+        ws.graphs.run(graph_key{G.uid, 0, n, m, 0, 0, 0}, s, [&] {
         // V_0 = w / beta (beta^2 still in hd[beta2])
         P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, w, hd + ws.beta2(), V, s); });
         for (i64 j = 0; j < m; ++j) {
@@ -129,6 +140,7 @@ mk_expv_result expv_device(mk_generator_s &G, double t, const double *v, double 
             gpu::shifted_apply<double>(G.A.view, 0.0, V + m * ld, nullptr, pv, nullptr, s);
         });
         norm2_to(pv, hd + ws.nrm() + m);
+        }, n);
         read(0, ws.beta2());
         const double *h1 = ws.host + ws.h1(), *h2 = ws.host + ws.h2(), *nr = ws.host + ws.nrm();
         H.assign(static_cast<std::size_t>(L * L), 0.0);
@@ -196,7 +208,7 @@ mk_expv_result expv_device(mk_generator_s &G, double t, const double *v, double 
         res.error += std::max(err_loc, rndoff);
         if (!(beta > 0.0) || !std::isfinite(beta)) break;
     }
-    return res;
+    return finish();
 }
 
 } // namespace

@@ -212,8 +212,9 @@ __device__ __forceinline__ T row_exit(const csr_view<T> &A, i64 r) {
 
 template <typename T, typename IN>
 __global__ void __launch_bounds__(kThreads)
-    k_jacobi_first(csr_view<T> A, T shift, T omega, const IN *rhs_in, IN scale, T *rhs_out,
-                   T *x) {
+    k_jacobi_first(csr_view<T> A, T shift, T omega, const IN *rhs_in, const double *scale_ptr,
+                   T *rhs_out, T *x) {
+    const IN scale = scale_ptr ? static_cast<IN>(*scale_ptr) : IN(1);
     for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < A.n;
          i += static_cast<i64>(gridDim.x) * kThreads) {
         const T b = static_cast<T>(rhs_in[i] * scale);
@@ -345,7 +346,9 @@ __global__ void __launch_bounds__(kThreads) k_fill(i64 n, T v, T *x) {
 }
 
 template <typename A, typename B>
-__global__ void __launch_bounds__(kThreads) k_convert(i64 n, const A *in, A scale, B *out) {
+__global__ void __launch_bounds__(kThreads)
+    k_convert(i64 n, const A *in, const double *scale_ptr, B *out) {
+    const A scale = scale_ptr ? static_cast<A>(*scale_ptr) : A(1);
     for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
          i += static_cast<i64>(gridDim.x) * kThreads)
         out[i] = static_cast<B>(in[i] * scale);
@@ -542,8 +545,9 @@ __global__ void __launch_bounds__(kThreads)
 //   partial[k nb + block]: Q_k . u' (k < count), q . u' (k = count), u' . u' (k = count + 1)
 template <int PW>
 __global__ void __launch_bounds__(kThreads)
-    k_dcgs_update(double *V, i64 ld, int count, const double *sc, const double *tg, double beta,
-                  const double *w, i64 n, double *partial, int nb, double *eta_out) {
+    k_dcgs_update(double *V, i64 ld, int count, const double *sc, const double *tg,
+                  const double *beta_ptr, const double *w, i64 n, double *partial, int nb,
+                  double *eta_out) {
     __shared__ double part[2][kWaves][kTileRows];
     __shared__ double uw[2][kTileRows];
     __shared__ double2 coef[kFusedBasis]; // (s_k, (Q^T w)_k), zero from count on
@@ -553,6 +557,7 @@ __global__ void __launch_bounds__(kThreads)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int k = threadIdx.x; k < kFusedBasis; k += kThreads)
         coef[k] = k < count ? make_double2(sc[k], tg[k]) : make_double2(0.0, 0.0);
+    const double beta = *beta_ptr;
     if (threadIdx.x == 0) {
         double st = 0.0;
         for (int k = 0; k < count; ++k) st += sc[k] * tg[k];
@@ -649,11 +654,10 @@ void shifted_apply(const csr_view<double> &A, double shift, const XT *x, const d
 
 template <typename T, typename IN>
 void jacobi_first(const csr_view<T> &A, double shift, double omega, const IN *rhs_in, T *rhs_out,
-                  T *x, hipStream_t s, double scale) {
+                  T *x, hipStream_t s, const double *scale) {
     if (A.n == 0) return;
     hipLaunchKernelGGL((k_jacobi_first<T, IN>), dim3(grid_for(A.n)), dim3(kThreads), 0, s, A,
-                       static_cast<T>(shift), static_cast<T>(omega), rhs_in,
-                       static_cast<IN>(scale), rhs_out, x);
+                       static_cast<T>(shift), static_cast<T>(omega), rhs_in, scale, rhs_out, x);
 }
 
 template <typename T>
@@ -750,10 +754,10 @@ void fill(i64 n, T value, T *x, hipStream_t s) {
 }
 
 template <typename A, typename B>
-void convert(i64 n, const A *in, B *out, hipStream_t s, double scale) {
+void convert(i64 n, const A *in, B *out, hipStream_t s, const double *scale) {
     if (n == 0) return;
-    hipLaunchKernelGGL((k_convert<A, B>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, in,
-                       static_cast<A>(scale), out);
+    hipLaunchKernelGGL((k_convert<A, B>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, in, scale,
+                       out);
 }
 
 template <typename T>
@@ -804,9 +808,9 @@ void update_scale(const double *w, const double *V, i64 ld, int count, const dou
                        w, V, ld, count, cw, n, v, beta_flag);
 }
 
-void dcgs_update(double *V, i64 ld, int count, const double *sc, const double *tg, double beta,
-                 const double *w, i64 n, double *partial, int nb, double *eta_out,
-                 hipStream_t s) {
+void dcgs_update(double *V, i64 ld, int count, const double *sc, const double *tg,
+                 const double *beta, const double *w, i64 n, double *partial, int nb,
+                 double *eta_out, hipStream_t s) {
     if (count < 0 || count >= kFusedBasis || n <= 0) return;
     switch ((count + kWaves - 1) / kWaves) {
 #define MK_DCGS(P)                                                                               \
@@ -860,14 +864,15 @@ MK_LEVEL(double)
 #undef MK_LEVEL
 template void jacobi_first<double, double>(const csr_view<double> &, double, double,
                                            const double *, double *, double *, hipStream_t,
-                                           double);
+                                           const double *);
 template void jacobi_first<float, double>(const csr_view<float> &, double, double,
-                                          const double *, float *, float *, hipStream_t, double);
+                                          const double *, float *, float *, hipStream_t,
+                                          const double *);
 template void jacobi_first<float, float>(const csr_view<float> &, double, double, const float *,
-                                         float *, float *, hipStream_t, double);
-template void convert<double, float>(i64, const double *, float *, hipStream_t, double);
-template void convert<float, double>(i64, const float *, double *, hipStream_t, double);
-template void convert<double, double>(i64, const double *, double *, hipStream_t, double);
-template void convert<float, float>(i64, const float *, float *, hipStream_t, double);
+                                         float *, float *, hipStream_t, const double *);
+template void convert<double, float>(i64, const double *, float *, hipStream_t, const double *);
+template void convert<float, double>(i64, const float *, double *, hipStream_t, const double *);
+template void convert<double, double>(i64, const double *, double *, hipStream_t, const double *);
+template void convert<float, float>(i64, const float *, float *, hipStream_t, const double *);
 
 } // namespace mk::gpu

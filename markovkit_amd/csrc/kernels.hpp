@@ -80,10 +80,10 @@ void shifted_apply(const csr_view<double> &A, double shift, const XT *x, const d
 
 // Jacobi smoothing on one multigrid level, working precision T:
 //   first sweep from x = 0:  rhs = (T)(scale * rhs_in), stored to rhs_out if given;
-//   x = w * rhs / (shift + exit)
+//   x = w * rhs / (shift + exit); scale is read from device memory (1 when null)
 template <typename T, typename IN>
 void jacobi_first(const csr_view<T> &A, double shift, double omega, const IN *rhs_in, T *rhs_out,
-                  T *x, hipStream_t s, double scale = 1.0);
+                  T *x, hipStream_t s, const double *scale = nullptr);
 //   sweep: x_out = x + w * (rhs - A x) / (shift + exit)
 template <typename T>
 void jacobi_sweep(const csr_view<T> &A, double shift, double omega, const T *rhs, const T *x,
@@ -131,8 +131,9 @@ void copy_rows(const u32 *rows, u64 nrows, const T *src, T *dst, hipStream_t s);
 
 template <typename T>
 void fill(i64 n, T value, T *x, hipStream_t s);
+// out = (B)(in * scale), scale read from device memory (1 when null)
 template <typename A, typename B>
-void convert(i64 n, const A *in, B *out, hipStream_t s, double scale = 1.0); // out = (B)(in * scale)
+void convert(i64 n, const A *in, B *out, hipStream_t s, const double *scale = nullptr);
 template <typename T>
 void axpy(i64 n, T alpha, const T *x, T *y, hipStream_t s); // y += alpha x
 // plain copy of n 16-byte words (the byte-counter calibration kernel)
@@ -167,13 +168,13 @@ void update_scale(const double *w, const double *V, i64 ld, int count, const dou
 // Delayed CGS2 (one reduction per Arnoldi step; count < kFusedBasis). V[0 .. count) is the
 // final orthonormal basis Q and V[count] the next candidate u, orthogonalised once against Q;
 // sc = Q^T u, tg[0 .. count) = Q^T w and tg[count] = u^T w for the new operator image w, and
-// beta = |u - Q sc| from the Pythagorean identity. Finalises q = (u - Q sc) / beta in V[count],
+// beta = |u - Q sc| (read from device memory). Finalises q = (u - Q sc) / beta in V[count],
 // writes eta = q^T w to eta_out[0] and the new candidate u' = w - Q (Q^T w) - eta q to
 // V[count + 1], with the partial sums of its projections Q_k . u', q . u' and of |u'|^2 at
 // partial[k * nb + block] for k = 0 .. count + 1.
-void dcgs_update(double *V, i64 ld, int count, const double *sc, const double *tg, double beta,
-                 const double *w, i64 n, double *partial, int nb, double *eta_out,
-                 hipStream_t s);
+void dcgs_update(double *V, i64 ld, int count, const double *sc, const double *tg,
+                 const double *beta, const double *w, i64 n, double *partial, int nb,
+                 double *eta_out, hipStream_t s);
 // x += sum_k y[k] Z_k (Z stored in ZT)
 template <typename ZT>
 void combine(double *x, const ZT *Z, i64 ld, int count, const double *y, i64 n, hipStream_t s);

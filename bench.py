@@ -130,8 +130,7 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16):
                 for j in range(len(shifts)))
     its = [r.iterations for r in res]
     # sequential single-system solves on the GPU, a sample spread over the shifts, each system
-    # solved once as a caller would (eager launches: no graph is replayed for a one-off system)
-    mk.graphs_enable(False)
+    # solved once as a caller would
     pick = np.linspace(0, len(shifts) - 1, sequential_sample).round().astype(int)
     seq = 0.0
     for j in pick:
@@ -145,7 +144,6 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16):
         seq += time.perf_counter() - t
         if not r1.converged:
             raise RuntimeError("multishift: a single solve did not converge")
-    mk.graphs_enable(True)
     seq_rate = len(pick) / seq
     return {
         "workload": f"{w.name}: {n} states, {len(shifts)} systems (64 shifts 1e-2..1e2 x 4 rhs), "

@@ -255,10 +255,14 @@ mk_status mk_device_zero(void *d_ptr, int64_t bytes);
 /* d_dst = d_src with the library's plain copy kernel (HBM calibration of byte counters) */
 mk_status mk_device_copy(void *d_dst, const void *d_src, int64_t bytes);
 mk_status mk_synchronize(void);
-/* HIP-graph replay of repeated solver iterations and expv steps (default on; MK_GRAPHS=0 in the
- * environment turns it off at start). Replays run the same kernels with the same arguments as
- * eager launches, so results are identical either way. */
+/* HIP-graph replay of repeated solver iterations and expv steps on operators of at most 262,144
+ * rows (default off; MK_GRAPHS=1 in the environment turns it on at start). Replays run the same
+ * kernels with the same arguments as eager launches, so results are identical either way. */
 mk_status mk_graphs_enable(int32_t on);
+/* Run the multigrid levels of at most 1024 states (and all coarser ones) as one workgroup-wide
+ * kernel instead of one launch per operation (default on; MK_MG_TAIL=0 turns it off at start).
+ * Same formulas and summation orders as the per-level kernels. */
+mk_status mk_fused_tail_enable(int32_t on);
 /* per-kernel-class timing with HIP events on the library stream. Classes, in bit order:
  * "shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict", "prolong",
  * "terminal", "gmres_dot", "gmres_update", "gmres_other". mask: 0 = off, -1 = all, else the

@@ -543,8 +543,13 @@ class Server:
 
 
 def graphs_enable(on: bool = True) -> None:
-    """Replay repeated solver iterations and expv steps as captured HIP graphs (default on)."""
+    """Replay repeated solver iterations and expv steps as captured HIP graphs (default off)."""
     check(lib().mk_graphs_enable(1 if on else 0))
+
+
+def fused_tail_enable(on: bool = True) -> None:
+    """Run the small multigrid levels in one workgroup-wide kernel (default on)."""
+    check(lib().mk_fused_tail_enable(1 if on else 0))
 
 
 KERNEL_CLASSES = ("shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict",

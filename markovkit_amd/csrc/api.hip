@@ -177,6 +177,37 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
             lv.cx = dbuf(vb);
         }
     }
+    // the fused tail: the deepest run of levels that are small, Jacobi-smoothed and V-cycled
+    std::vector<gpu::level_desc<T>> desc(depth);
+    for (i64 l = 0; l < depth; ++l) {
+        auto &lv = L[l];
+        auto &d = desc[l];
+        d.A = lv.A.view;
+        d.R = lv.R.view;
+        d.coarse_of = dp<u32>(lv.coarse_of);
+        d.weight = dp<T>(lv.weight);
+        d.n = lv.n;
+        d.n_next = lv.n_next;
+        d.pre = lv.pre;
+        d.post = lv.post;
+        d.n_term = lv.n_term;
+        d.term_max = lv.term_max;
+        d.tstart = dp<u32>(lv.tstart);
+        d.tsize = dp<u32>(lv.tsize);
+        d.toff = dp<u64>(lv.toff);
+        d.tinv = dp<T>(lv.tinv);
+        d.rhs = dp<T>(lv.rhs);
+        d.x = dp<T>(lv.x);
+        d.xt = dp<T>(lv.xt);
+        d.res = dp<T>(lv.res);
+    }
+    mg.tail_start = -1;
+    if (mg.opt.coarse_cycles == 1) {
+        i64 start = depth;
+        while (start > 0 && L[start - 1].n <= gpu::kTailRows && L[start - 1].n_agg == 0) --start;
+        if (start < depth) mg.tail_start = start;
+    }
+    mg.tail_desc = upload(desc);
 }
 
 // aggregate-block smoothing (multigrid.hpp:309-342) of the level's block sectors: block
@@ -236,6 +267,15 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
     auto &P = prof();
     const hipStream_t s = stream();
     const double shift = mg.shift, omega = mg.opt.jacobi_weight;
+    if (mg.tail_start >= 0 && static_cast<i64>(l) >= mg.tail_start && tail_switch().load() != 0) {
+        // this level and every coarser one in one workgroup
+        const auto *desc = dp<gpu::level_desc<T>>(mg.tail_desc) + l;
+        P.run(K_SWEEP, 0.0, [&] {
+            gpu::vcycle_tail<T>(desc, static_cast<int>(L.size() - l), shift, omega, rhs, rhs64,
+                                scale, out, s);
+        });
+        return out ? out : dp<T>(lv.x);
+    }
     const i64 n = lv.n;
     const std::size_t ts = sizeof(T);
     const double sweep_bytes = op_bytes(lv.A, ts, ts, 2);
@@ -490,7 +530,8 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
                 ws.hscal[1] = bj;
                 double *scal = dp<double>(ws.scal);
                 HIPCHK(hipMemcpyAsync(scal, ws.hscal, 2 * sizeof(double), hipMemcpyHostToDevice, s));
-                graph_key key{A.uid, M ? M->uid : 0, j, static_cast<i64>(sd > sd_next), n, m,
+                graph_key key{A.uid, M ? M->uid : 0, j,
+                              static_cast<i64>(sd > sd_next) + 2 * tail_switch().load(), n, m,
                               bits(shift)};
                 ws.graphs.run(key, s, [&] {
                     operator_image(j, V + j * ld, scal);
@@ -680,6 +721,11 @@ const char *mk_last_error(void) { return last_error().c_str(); }
 
 mk_status mk_graphs_enable(int32_t on) {
     graphs_switch().store(on ? 1 : 0);
+    return MK_OK;
+}
+
+mk_status mk_fused_tail_enable(int32_t on) {
+    tail_switch().store(on ? 1 : 0);
     return MK_OK;
 }
 

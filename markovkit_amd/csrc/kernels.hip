@@ -338,6 +338,119 @@ __global__ void __launch_bounds__(kThreads)
     if (q < nrows) dst[rows[q]] = src[rows[q]];
 }
 
+// fused V-cycle tail: one workgroup runs every level from L[0] down and back up
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads)
+    k_vcycle_tail(const level_desc<T> *L, int levels, T shift, T omega, const T *rhs_in,
+                  const double *rhs64, const double *scale_ptr, T *out) {
+    constexpr int kMaxLevels = 32;
+    T *cur[kMaxLevels];
+    T *oth[kMaxLevels];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const double scale = scale_ptr ? *scale_ptr : 1.0;
+    for (int k = 0; k < levels; ++k) {
+        const level_desc<T> &lv = L[k];
+        const T *rhs = k == 0 ? (rhs64 ? lv.rhs : rhs_in) : lv.rhs;
+        T *x = lv.x, *xt = lv.xt;
+        // first sweep from x = 0 (the FP64 input converted on the top level)
+        for (i64 i = tid; i < lv.n; i += nt) {
+            T b;
+            if (k == 0 && rhs64) {
+                b = static_cast<T>(rhs64[i] * scale);
+                lv.rhs[i] = b;
+            } else {
+                b = rhs[i];
+            }
+            x[i] = lv.pre > 0 ? omega * b / (shift + row_exit(lv.A, i)) : T(0);
+        }
+        __syncthreads();
+        for (int p = 1; p < lv.pre; ++p) {
+            for (i64 i = tid; i < lv.n; i += nt) {
+                T ex = 0;
+                const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
+                const T xr = x[i];
+                const T d = shift + ex;
+                const T ax = d * xr - sum;
+                const T res = rhs[i] - ax;
+                xt[i] = xr + omega * res / d;
+            }
+            __syncthreads();
+            T *t = x;
+            x = xt;
+            xt = t;
+        }
+        for (i64 i = tid; i < lv.n; i += nt) {
+            T ex = 0;
+            const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
+            const T d = shift + ex;
+            lv.res[i] = rhs[i] - (d * x[i] - sum);
+        }
+        __syncthreads();
+        if (lv.n_term > 0) {
+            for (i64 g = tid; g < lv.n_term * lv.term_max; g += nt) {
+                const i64 b = g / lv.term_max, i = g % lv.term_max;
+                const u32 m = lv.tsize[b];
+                if (i >= m) continue;
+                const T *row = lv.tinv + lv.toff[b] + static_cast<u64>(i) * m;
+                const T *rr = lv.res + lv.tstart[b];
+                T acc = 0;
+                for (u32 j = 0; j < m; ++j) acc += row[j] * rr[j];
+                x[lv.tstart[b] + i] += acc;
+            }
+            __syncthreads();
+        }
+        if (lv.n_next > 0) {
+            T *crhs = L[k + 1].rhs;
+            for (i64 c = tid; c < lv.n_next; c += nt) {
+                T ex = 0;
+                crhs[c] = gather_row(lv.R, static_cast<u32>(c), lv.res, ex);
+            }
+            __syncthreads();
+        }
+        cur[k] = x;
+        oth[k] = xt;
+    }
+    for (int k = levels - 1; k >= 0; --k) {
+        const level_desc<T> &lv = L[k];
+        const T *rhs = k == 0 ? (rhs64 ? lv.rhs : rhs_in) : lv.rhs;
+        T *x = cur[k], *xt = oth[k];
+        if (lv.n_next > 0) {
+            const T *cx = cur[k + 1];
+            for (i64 i = tid; i < lv.n; i += nt) {
+                const u32 c = lv.coarse_of[i];
+                if (c != 0xFFFFFFFFu) x[i] += lv.weight[i] * cx[c];
+            }
+            __syncthreads();
+        }
+        for (int p = 0; p < lv.post; ++p) {
+            T *dst = (k == 0 && p == lv.post - 1 && out) ? out : xt;
+            for (i64 i = tid; i < lv.n; i += nt) {
+                T ex = 0;
+                const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
+                const T xr = x[i];
+                const T d = shift + ex;
+                const T ax = d * xr - sum;
+                const T res = rhs[i] - ax;
+                dst[i] = xr + omega * res / d;
+            }
+            __syncthreads();
+            if (dst == xt) {
+                xt = x;
+                x = dst;
+            } else {
+                x = dst;
+            }
+        }
+        cur[k] = x;
+        oth[k] = xt;
+    }
+    // the top level's result in out, else in L[0].x
+    T *dst = out ? out : L[0].x;
+    if (cur[0] != dst) {
+        for (i64 i = tid; i < L[0].n; i += nt) dst[i] = cur[0][i];
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_fill(i64 n, T v, T *x) {
     for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
@@ -748,6 +861,14 @@ void copy_rows(const u32 *rows, u64 nrows, const T *src, T *dst, hipStream_t s) 
 }
 
 template <typename T>
+void vcycle_tail(const level_desc<T> *L, int levels, double shift, double omega, const T *rhs,
+                 const double *rhs64, const double *scale, T *out, hipStream_t s) {
+    if (levels <= 0) return;
+    hipLaunchKernelGGL((k_vcycle_tail<T>), dim3(1), dim3(kTailThreads), 0, s, L, levels,
+                       static_cast<T>(shift), static_cast<T>(omega), rhs, rhs64, scale, out);
+}
+
+template <typename T>
 void fill(i64 n, T value, T *x, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, value, x);
@@ -857,6 +978,8 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
     template void sweep_rows<T>(const csr_view<T> &, double, double, const T *, const T *, T *,   \
                                 const u32 *, u64, hipStream_t);                                   \
     template void copy_rows<T>(const u32 *, u64, const T *, T *, hipStream_t);                  \
+    template void vcycle_tail<T>(const level_desc<T> *, int, double, double, const T *,         \
+                                 const double *, const double *, T *, hipStream_t);              \
     template void axpy<T>(i64, T, const T *, T *, hipStream_t);                                 \
     template void combine<T>(double *, const T *, i64, int, const double *, i64, hipStream_t);
 MK_LEVEL(float)

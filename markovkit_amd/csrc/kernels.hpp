@@ -129,6 +129,32 @@ void sweep_rows(const csr_view<T> &A, double shift, double omega, const T *rhs, 
 template <typename T>
 void copy_rows(const u32 *rows, u64 nrows, const T *src, T *dst, hipStream_t s);
 
+// One multigrid level as the fused tail kernel reads it (all pointers in HBM)
+template <typename T>
+struct level_desc {
+    csr_view<T> A;                     // level operator (shift I - C without the shift)
+    csr_view<T> R;                     // restriction to the next level (n_next rows)
+    const u32 *coarse_of = nullptr;    // prolongation parent (0xFFFFFFFF: none)
+    const T *weight = nullptr;
+    i64 n = 0, n_next = 0;
+    int pre = 0, post = 0;
+    i64 n_term = 0, term_max = 0;      // dense terminal blocks
+    const u32 *tstart = nullptr, *tsize = nullptr;
+    const u64 *toff = nullptr;
+    const T *tinv = nullptr;
+    T *rhs = nullptr, *x = nullptr, *xt = nullptr, *res = nullptr; // work vectors
+};
+constexpr i64 kTailRows = 1024;  // levels of at most this many rows run in the fused tail
+constexpr int kTailThreads = 1024;
+// The V-cycle below and including level L[0] (Jacobi smoother, one coarse cycle) in one
+// workgroup: the same per-row formulas and summation orders as the per-level kernels, phases
+// separated by workgroup barriers. L[0]'s right-hand side is rhs, or (T)(rhs64 * scale) when
+// rhs64 is given (scale from device memory, 1 when null); the result goes to out, else to
+// L[0].x.
+template <typename T>
+void vcycle_tail(const level_desc<T> *L, int levels, double shift, double omega, const T *rhs,
+                 const double *rhs64, const double *scale, T *out, hipStream_t s);
+
 template <typename T>
 void fill(i64 n, T value, T *x, hipStream_t s);
 // out = (B)(in * scale), scale read from device memory (1 when null)

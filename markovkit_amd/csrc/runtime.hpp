@@ -336,7 +336,9 @@ inline u64 next_uid() {
 
 // Captured launch sequences, keyed by everything their kernel arguments depend on. A key is
 // captured on its second use (one-off configurations stay eager) and replayed afterwards.
-// Disabled while profiling (events cannot be recorded inside a capture) and with MK_GRAPHS=0.
+// Off by default (with the fused multigrid tail, replay measured no faster than eager launches);
+// MK_GRAPHS=1 or mk_graphs_enable(1) turns it on; never while profiling (events cannot be
+// recorded inside a capture).
 struct graph_key {
     u64 a = 0, b = 0;
     i64 c = 0, d = 0, e = 0, f = 0;
@@ -349,15 +351,24 @@ struct graph_key {
 inline std::atomic<int> &graphs_switch() {
     static std::atomic<int> on{[] {
         const char *env = std::getenv("MK_GRAPHS");
-        return (env && env[0] == '0') ? 0 : 1;
+        return (env && env[0] == '1') ? 1 : 0;
     }()};
     return on;
 }
 
-// graphs pay off where launch latency is a large share of an iteration: operators of at most
-// kGraphRows rows (measured: -20% per FGMRES iteration at 969 states, -14% at 58,905, +8% at
-// 814,385)
+// graphs are only ever used for operators of at most kGraphRows rows, where launch latency is a
+// large share of an iteration (before the fused tail: -20% per FGMRES iteration at 969 states,
+// -14% at 58,905, +8% at 814,385)
 constexpr i64 kGraphRows = 262144;
+
+// the fused coarse-level tail of the V-cycle (default on; MK_MG_TAIL=0 turns it off at start)
+inline std::atomic<int> &tail_switch() {
+    static std::atomic<int> on{[] {
+        const char *env = std::getenv("MK_MG_TAIL");
+        return (env && env[0] == '0') ? 0 : 1;
+    }()};
+    return on;
+}
 
 inline bool graphs_enabled(i64 rows = 0) {
     return graphs_switch().load() != 0 && prof().mask == 0 && rows <= kGraphRows;
@@ -516,5 +527,8 @@ struct mk_sector_mg_s {
     std::vector<mk::rt::level_dev<double>> L64;
     std::vector<mk::rt::level_dev<float>> L32;
     mk::rt::dbuf d_b, d_x64, d_out; // staging for host-buffer calls
+    // level descriptors of the fused tail (levels tail_start .. end run in one workgroup)
+    mk::rt::dbuf tail_desc;
+    mk::i64 tail_start = -1;
 };
 

@@ -50,7 +50,7 @@ def test_solver_replays_match_eager(wl, precision):
         x, _ = solve(Q, M, s, b)
         assert np.array_equal(x, x_eager)
     finally:
-        mk.graphs_enable(True)
+        mk.graphs_enable(False)
 
 
 def test_expv_replays_match_eager():
@@ -67,4 +67,4 @@ def test_expv_replays_match_eager():
             assert rg.steps == r.steps
             assert np.array_equal(w, w_eager), rep
     finally:
-        mk.graphs_enable(True)
+        mk.graphs_enable(False)

@@ -63,10 +63,15 @@ def run_solves(mk, A, M, b_dev, x_dev, tol, steps, restart, profile_classes=None
     return total, lat, iters, res, prof
 
 
+SMOOTHERS = {"jacobi": "JACOBI", "red_black": "RED_BLACK", "ilu0": "ILU0",
+             "aggregate_block": "AGGREGATE_BLOCK"}
+
+
 def cpu_baseline(w, b, iterations, tol, restart):
-    """The FP64 oracle (1 core) on a bounded sample of the same solve: a 1-iteration and a
-    3-iteration FGMRES run give the fixed and per-iteration costs, scaled to the number of
-    iterations the GPU solve needs."""
+    """The FP64 oracle (1 core) on a bounded sample of the same solve with the reference's
+    method (weighted Jacobi V-cycle): a 1-iteration and a 3-iteration FGMRES run give the fixed
+    and per-iteration costs, scaled to the number of iterations that method needs (measured on
+    the GPU)."""
     from oracle.oracle import Problem, SectorMultigrid
     t = time.perf_counter()
     P = Problem(w.model, w.max_totals)
@@ -263,6 +268,8 @@ def main():
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--workers", type=int, default=4, help="serving: worker streams")
     ap.add_argument("--restart", type=int, default=50)
+    ap.add_argument("--smoother", default="jacobi", choices=list(SMOOTHERS),
+                    help="sector multigrid smoother of the timed solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--no-multishift", action="store_true",
@@ -306,7 +313,9 @@ def main():
     t_gen = time.perf_counter() - t
     log(f"[setup] generator {t_gen:.1f}s, {Q.n_jumps()} jumps")
     t = time.perf_counter()
-    M32 = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=mk.FP32))
+    smoother = getattr(mk, SMOOTHERS[args.smoother])
+    M32 = mk.SectorMultigrid(w.model.domain, box, shift,
+                             mk.MultigridOptions(precision=mk.FP32, smoother=smoother))
     t_mg = time.perf_counter() - t
     log(f"[setup] FP32 multigrid {t_mg:.1f}s, {M32.max_levels()} levels")
     A = mk.ShiftedGenerator(Q, shift)
@@ -337,7 +346,8 @@ def main():
     fp64 = None
     if not args.no_fp64:
         t = time.perf_counter()
-        M64 = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=mk.FP64))
+        M64 = mk.SectorMultigrid(w.model.domain, box, shift,
+                                 mk.MultigridOptions(precision=mk.FP64, smoother=smoother))
         log(f"[setup] FP64 multigrid {time.perf_counter() - t:.1f}s")
         run_solves(mk, A, M64, b_dev, x_dev, tol, max(args.warmup, 1), args.restart)
         total64, lat64, iters64, res64, _ = run_solves(mk, A, M64, b_dev, x_dev, tol, args.steps,
@@ -398,6 +408,7 @@ def main():
             "states": n,
             "jumps": Q.n_jumps(),
             "multigrid_levels": M32.max_levels(),
+            "smoother": args.smoother,
         },
         "p50_ms": 1e3 * percentile(lat, 50),
         "p99_ms": 1e3 * percentile(lat, 99),
@@ -430,8 +441,22 @@ def main():
         log("[multishift] 256-system batch")
         out["multishift"] = multishift(mk, workloads, 3, args.restart, sequential_sample=8)
         log(f"[multishift] {out['multishift']['value']:.1f} solves/s")
+    ref_iters = iters[-1]
+    if args.smoother != "jacobi":
+        # the reference's own method (weighted Jacobi V-cycle) on the GPU, for comparison and
+        # for the CPU baseline's iteration count
+        del M32
+        Mj = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=mk.FP32))
+        run_solves(mk, A, Mj, b_dev, x_dev, tol, 1, args.restart)
+        tj, latj, itj, _, _ = run_solves(mk, A, Mj, b_dev, x_dev, tol, max(3, args.steps // 4),
+                                         args.restart)
+        ref_iters = itj[-1]
+        out["jacobi"] = {"value": len(latj) / tj, "p50_ms": 1e3 * percentile(latj, 50),
+                         "iterations": ref_iters}
+        log(f"[jacobi] {out['jacobi']['value']:.2f} solves/s, {ref_iters} iterations")
+        del Mj
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, b, iters[-1], tol, args.restart)
+        out["cpu_baseline"] = cpu_baseline(w, b, ref_iters, tol, args.restart)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 

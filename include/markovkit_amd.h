@@ -42,8 +42,15 @@ typedef struct mk_batch_s *mk_batch;
 
 /* boundary (generator.hpp:11-13) */
 enum { MK_REFLECTING = 0, MK_ABSORBING = 1 };
-/* relaxation (multigrid.hpp:120-123) */
-enum { MK_JACOBI = 0, MK_AGGREGATE_BLOCK = 1 };
+/* relaxation (multigrid.hpp:120-123), plus two smoothers of the paper driver's family
+ * (rdme/cpp/rdme_full_multigrid.cpp:178-195, :204-296):
+ *   MK_RED_BLACK: Gauss-Seidel in two colours. The fixed-population diffusion graph on a grid is
+ *     bipartite; colour 0 states are updated first (colour 1 first when post-smoothing), each
+ *     colour in parallel. jacobi_weight is not used.
+ *   MK_ILU0: x += (L U)^-1 (rhs - A x) with the ILU(0) factors of the level operator in rank
+ *     order (only the pivots differ from A on a bipartite graph); triangular solves are
+ *     level-scheduled. The pivots depend on the shift (mk_sector_mg_set_shift refuses it). */
+enum { MK_JACOBI = 0, MK_AGGREGATE_BLOCK = 1, MK_RED_BLACK = 2, MK_ILU0 = 3 };
 /* working precision of the multigrid preconditioner (no reference counterpart) */
 enum { MK_FP64 = 0, MK_FP32 = 1 };
 

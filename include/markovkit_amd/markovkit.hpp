@@ -428,7 +428,9 @@ class shifted_generator {
     real shift_;
 };
 
-enum class relaxation : std::uint8_t { jacobi, aggregate_block };
+// jacobi and aggregate_block as at multigrid.hpp:120-123; red_black (multicolour Gauss-Seidel)
+// and ilu0 (level-scheduled ILU(0) in rank order) are the device's extensions
+enum class relaxation : std::uint8_t { jacobi, aggregate_block, red_black, ilu0 };
 
 // precision of the device multigrid (no reference counterpart)
 enum class precision : std::uint8_t { fp64, fp32 };
@@ -464,7 +466,10 @@ class sector_multigrid {
         o.max_levels = options.max_levels;
         o.coarse_cycles = options.coarse_cycles;
         o.sweep_growth = options.sweep_growth;
-        o.smoother = options.smoother == relaxation::jacobi ? MK_JACOBI : MK_AGGREGATE_BLOCK;
+        o.smoother = options.smoother == relaxation::jacobi            ? MK_JACOBI
+                     : options.smoother == relaxation::aggregate_block ? MK_AGGREGATE_BLOCK
+                     : options.smoother == relaxation::red_black       ? MK_RED_BLACK
+                                                                       : MK_ILU0;
         o.precision = options.working_precision == precision::fp32 ? MK_FP32 : MK_FP64;
         auto dom = domain.native();
         mk_sector_mg m = nullptr;

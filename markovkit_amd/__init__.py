@@ -6,7 +6,7 @@ in-tree HIP library through its C ABI and has no CPU fallback.
 """
 from .model import (Graph, Grid, LocalReaction, ReactionDiffusion, VoxelDomain, birth_death,
                     box_size, compositions)
-from .api import (ABSORBING, AGGREGATE_BLOCK, FP32, FP64, JACOBI, REFLECTING, BatchSolver,
+from .api import (ABSORBING, AGGREGATE_BLOCK, FP32, FP64, ILU0, JACOBI, RED_BLACK, REFLECTING, BatchSolver,
                   DeviceVector, Server,
                   Generator, MultigridOptions, PopulationBox, SectorMultigrid, ShiftedGenerator,
                   SolverResult, expv, expv_device, fused_tail_enable, gmres, graphs_enable, profile_enable, profile_read,
@@ -14,7 +14,7 @@ from .api import (ABSORBING, AGGREGATE_BLOCK, FP32, FP64, JACOBI, REFLECTING, Ba
 
 __all__ = [
     "Graph", "Grid", "LocalReaction", "ReactionDiffusion", "VoxelDomain", "birth_death",
-    "box_size", "compositions", "ABSORBING", "AGGREGATE_BLOCK", "FP32", "FP64", "JACOBI",
+    "box_size", "compositions", "ABSORBING", "AGGREGATE_BLOCK", "FP32", "FP64", "ILU0", "JACOBI", "RED_BLACK",
     "REFLECTING", "BatchSolver", "DeviceVector", "Server", "Generator", "MultigridOptions", "PopulationBox",
     "SectorMultigrid", "ShiftedGenerator", "SolverResult", "expv", "expv_device", "fused_tail_enable", "gmres", "graphs_enable",
     "profile_enable",

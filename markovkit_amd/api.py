@@ -33,6 +33,8 @@ REFLECTING = 0
 ABSORBING = 1
 JACOBI = 0
 AGGREGATE_BLOCK = 1
+RED_BLACK = 2  # multicolour Gauss-Seidel
+ILU0 = 3       # level-scheduled ILU(0) in rank order
 FP64 = 0
 FP32 = 1
 

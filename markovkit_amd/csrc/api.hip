@@ -9,6 +9,101 @@ using namespace mk::rt;
 
 namespace {
 
+// red-black smoother: one row-mapped operator per colour (that colour's rows with all their
+// entries, columns unchanged) and the colour of every row
+template <typename T>
+void split_colours(const mk::generator_csr &cat, const std::vector<std::uint8_t> &colour,
+                   level_dev<T> &lv) {
+    for (int c = 0; c < 2; ++c) {
+        mk::generator_csr sub;
+        sub.ptr.assign(1, 0);
+        std::vector<u32> rows;
+        for (i64 i = 0; i < cat.n; ++i) {
+            if (colour[i] != c) continue;
+            rows.push_back(static_cast<u32>(i));
+            sub.col.insert(sub.col.end(), cat.col.begin() + cat.ptr[i], cat.col.begin() + cat.ptr[i + 1]);
+            sub.val.insert(sub.val.end(), cat.val.begin() + cat.ptr[i], cat.val.begin() + cat.ptr[i + 1]);
+            sub.ptr.push_back(sub.col.size());
+            sub.exit.push_back(cat.exit[i]);
+        }
+        sub.n = static_cast<i64>(rows.size());
+        lv.Ac[c] = upload_csr<T>(sub);
+        lv.rowmap[c] = upload(rows);
+        lv.Ac[c].view.rowmap = dp<u32>(lv.rowmap[c]);
+    }
+    lv.colour = upload(colour);
+    lv.rb = true;
+}
+
+// launches of a level schedule: groups of at least kIluWide rows get a grid each, runs of
+// smaller groups share one workgroup
+constexpr i64 kIluWide = 2048;
+
+template <typename T>
+void ilu_schedule(const std::vector<i64> &depth, i64 groups, dbuf &rows_d, dbuf &ptr_d,
+                  std::vector<typename level_dev<T>::ilu_launch> &launches) {
+    std::vector<u32> ptr(groups + 1, 0), rows(depth.size());
+    for (i64 d : depth) ++ptr[d + 1];
+    for (i64 g = 0; g < groups; ++g) ptr[g + 1] += ptr[g];
+    std::vector<u32> fill(ptr.begin(), ptr.end() - 1);
+    for (std::size_t i = 0; i < depth.size(); ++i) rows[fill[depth[i]]++] = static_cast<u32>(i);
+    rows_d = upload(rows);
+    ptr_d = upload(ptr);
+    launches.clear();
+    for (i64 g = 0; g < groups;) {
+        const i64 size = ptr[g + 1] - ptr[g];
+        if (size >= kIluWide) {
+            launches.push_back({static_cast<int>(g), static_cast<int>(g + 1), size});
+            ++g;
+            continue;
+        }
+        i64 e = g;
+        while (e < groups && ptr[e + 1] - ptr[e] < kIluWide) ++e;
+        launches.push_back({static_cast<int>(g), static_cast<int>(e), 0});
+        g = e;
+    }
+}
+
+// ILU(0) of the level operator in rank order (rdme/cpp/rdme_full_multigrid.cpp:208-236, the
+// structured case): the graph is bipartite, so only the pivots change,
+//   pivot_i = shift + exit_i - sum_{j < i} a_ij a_ji / pivot_j  (a_ij a_ji = rate(j->i) rate(i->j));
+// and the dependency depths of the forward (lower neighbours) and backward (upper neighbours)
+// solves. Columns of every row are ascending.
+template <typename T>
+void build_ilu(const mk::generator_csr &cat, double shift, level_dev<T> &lv) {
+    const i64 n = cat.n;
+    std::vector<double> pivot(n);
+    std::vector<T> pinv(n);
+    std::vector<i64> fdepth(n, 0), bdepth(n, 0);
+    i64 fgroups = n > 0 ? 1 : 0, bgroups = n > 0 ? 1 : 0;
+    for (i64 i = 0; i < n; ++i) {
+        double d = shift + cat.exit[i];
+        for (u64 p = cat.ptr[i]; p < cat.ptr[i + 1] && cat.col[p] < i; ++p) {
+            const u32 j = cat.col[p];
+            const auto b = cat.col.begin() + cat.ptr[j], e = cat.col.begin() + cat.ptr[j + 1];
+            const auto it = std::lower_bound(b, e, static_cast<u32>(i));
+            if (it == e || *it != static_cast<u32>(i))
+                throw std::logic_error("ILU(0) smoother: a diffusion jump has no reverse jump");
+            d -= cat.val[p] * cat.val[cat.ptr[j] + (it - b)] / pivot[j];
+            fdepth[i] = std::max(fdepth[i], fdepth[j] + 1);
+        }
+        if (!(d > 0.0) || !std::isfinite(d)) throw std::runtime_error("population_multigrid: ILU(0) pivot");
+        pivot[i] = d;
+        pinv[i] = static_cast<T>(1.0 / d);
+        fgroups = std::max(fgroups, fdepth[i] + 1);
+    }
+    for (i64 i = n; i-- > 0;) {
+        for (u64 p = cat.ptr[i + 1]; p-- > cat.ptr[i] && cat.col[p] > i;)
+            bdepth[i] = std::max(bdepth[i], bdepth[cat.col[p]] + 1);
+        bgroups = std::max(bgroups, bdepth[i] + 1);
+    }
+    if (fgroups >= (i64{1} << 31) || bgroups >= (i64{1} << 31)) throw std::overflow_error("ILU(0) schedule too deep");
+    lv.pinv = upload(pinv);
+    ilu_schedule<T>(fdepth, fgroups, lv.fw_rows, lv.fw_ptr, lv.fw_launch);
+    ilu_schedule<T>(bdepth, bgroups, lv.bw_rows, lv.bw_ptr, lv.bw_launch);
+    lv.ilu = true;
+}
+
 template <typename T>
 void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H,
                   std::vector<level_dev<T>> &L) {
@@ -26,6 +121,27 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
                 level_rows[l] += H[k].levels[l].rows;
             }
     L.resize(depth);
+    // red-black: every level numbers each sector's states colour 0 first (ascending rank within
+    // a colour); ord maps a level row to the sector's natural rank, pos the other way
+    const bool by_colour = mg.opt.smoother == MK_RED_BLACK;
+    std::vector<std::vector<std::vector<u32>>> ord(depth, std::vector<std::vector<u32>>(K)), pos = ord;
+    if (by_colour) {
+        for (i64 l = 0; l < depth; ++l)
+            for (i64 k = 0; k < K; ++k) {
+                if (offset[l][k] < 0) continue;
+                const auto &c = H[k].levels[l].colour;
+                if (static_cast<i64>(c.size()) != H[k].levels[l].rows)
+                    throw std::logic_error("red-black smoother: missing colours");
+                auto &o = ord[l][k];
+                for (int want = 0; want < 2; ++want)
+                    for (std::size_t i = 0; i < c.size(); ++i)
+                        if (c[i] == want) o.push_back(static_cast<u32>(i));
+                pos[l][k].resize(o.size());
+                for (std::size_t i = 0; i < o.size(); ++i) pos[l][k][o[i]] = static_cast<u32>(i);
+            }
+    }
+    auto nat = [&](i64 l, i64 k, i64 i) -> i64 { return by_colour ? ord[l][k][i] : i; };
+    auto neu = [&](i64 l, i64 k, i64 i) -> i64 { return by_colour ? pos[l][k][i] : i; };
     for (i64 l = 0; l < depth; ++l) {
         auto &lv = L[l];
         lv.n = level_rows[l];
@@ -36,6 +152,7 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
         cat.ptr.assign(1, 0);
         std::vector<u32> coarse_of;
         std::vector<double> weight;
+        std::vector<std::uint8_t> colour;
         std::vector<u32> tstart, tsize;
         std::vector<u64> toff;
         std::vector<double> tinv;
@@ -46,40 +163,65 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
             const u32 base = static_cast<u32>(offset[l][k]);
             lv.pre = s.pre_sweeps;
             lv.post = s.post_sweeps;
-            for (i64 i = 0; i < s.rows; ++i) {
-                for (u64 p = s.A.ptr[i]; p < s.A.ptr[i + 1]; ++p) {
-                    cat.col.push_back(base + s.A.col[p]);
-                    cat.val.push_back(s.A.val[p]);
+            std::vector<std::pair<u32, double>> row;
+            for (i64 i2 = 0; i2 < s.rows; ++i2) {
+                const i64 i = nat(l, k, i2);
+                row.clear();
+                for (u64 p = s.A.ptr[i]; p < s.A.ptr[i + 1]; ++p)
+                    row.emplace_back(static_cast<u32>(base + neu(l, k, s.A.col[p])), s.A.val[p]);
+                if (by_colour) std::sort(row.begin(), row.end()); // columns ascending
+                for (const auto &[c, v] : row) {
+                    cat.col.push_back(c);
+                    cat.val.push_back(v);
                 }
                 cat.ptr.push_back(cat.col.size());
+                cat.exit.push_back(s.A.exit[i]);
+                if (!s.colour.empty()) colour.push_back(s.colour[i]);
             }
-            cat.exit.insert(cat.exit.end(), s.A.exit.begin(), s.A.exit.end());
             if (s.terminal) {
+                const i64 m = s.rows;
                 tstart.push_back(base);
-                tsize.push_back(static_cast<u32>(s.rows));
+                tsize.push_back(static_cast<u32>(m));
                 toff.push_back(tinv.size());
-                tinv.insert(tinv.end(), s.inverse.begin(), s.inverse.end());
+                for (i64 i2 = 0; i2 < m; ++i2)
+                    for (i64 j2 = 0; j2 < m; ++j2) tinv.push_back(s.inverse[nat(l, k, i2) * m + nat(l, k, j2)]);
                 const std::size_t q0 = lv.h_tq.size();
-                lv.h_tq.resize(q0 + static_cast<std::size_t>(s.rows * s.rows), 0.0);
-                for (i64 i = 0; i < s.rows; ++i) {
-                    double *row = lv.h_tq.data() + q0 + i * s.rows;
-                    row[i] += s.A.exit[i];
-                    for (u64 p = s.A.ptr[i]; p < s.A.ptr[i + 1]; ++p) row[s.A.col[p]] -= s.A.val[p];
+                lv.h_tq.resize(q0 + static_cast<std::size_t>(m * m), 0.0);
+                for (i64 i2 = 0; i2 < m; ++i2) {
+                    const i64 i = nat(l, k, i2);
+                    double *qrow = lv.h_tq.data() + q0 + i2 * m;
+                    qrow[i2] += s.A.exit[i];
+                    for (u64 p = s.A.ptr[i]; p < s.A.ptr[i + 1]; ++p) qrow[neu(l, k, s.A.col[p])] -= s.A.val[p];
                 }
-                lv.term_max = std::max<i64>(lv.term_max, s.rows);
-                coarse_of.insert(coarse_of.end(), s.rows, 0xFFFFFFFFu);
-                weight.insert(weight.end(), s.rows, 0.0);
+                lv.term_max = std::max<i64>(lv.term_max, m);
+                coarse_of.insert(coarse_of.end(), m, 0xFFFFFFFFu);
+                weight.insert(weight.end(), m, 0.0);
             } else {
                 has_next = true;
                 const i64 cb = offset[l + 1][k];
-                for (i64 i = 0; i < s.rows; ++i) {
-                    coarse_of.push_back(static_cast<u32>(cb + s.coarse_of[i]));
+                for (i64 i2 = 0; i2 < s.rows; ++i2) {
+                    const i64 i = nat(l, k, i2);
+                    coarse_of.push_back(static_cast<u32>(cb + neu(l + 1, k, s.coarse_of[i])));
                     weight.push_back(s.weight[i]);
                 }
                 lv.n_fine_linked += s.rows;
             }
         }
         lv.A = upload_csr<T>(cat);
+        if (mg.opt.smoother == MK_RED_BLACK) {
+            if (static_cast<i64>(colour.size()) != lv.n) throw std::logic_error("red-black smoother: missing colours");
+            split_colours<T>(cat, colour, lv);
+            if (l == 0) {
+                std::vector<u32> p0(lv.n);
+                for (i64 k = 0; k < K; ++k) {
+                    if (offset[0][k] < 0) continue;
+                    for (i64 i2 = 0; i2 < H[k].levels[0].rows; ++i2)
+                        p0[offset[0][k] + i2] = static_cast<u32>(offset[0][k] + nat(0, k, i2));
+                }
+                lv.perm = upload(p0);
+            }
+        }
+        if (mg.opt.smoother == MK_ILU0) build_ilu<T>(cat, mg.shift, lv);
         // aggregate-block smoother data of the sectors that are not terminal on this level
         i64 n_agg = 0;
         for (i64 k = 0; k < K; ++k)
@@ -200,9 +342,12 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
         d.x = dp<T>(lv.x);
         d.xt = dp<T>(lv.xt);
         d.res = dp<T>(lv.res);
+        d.colour = lv.rb ? dp<std::uint8_t>(lv.colour) : nullptr;
+        d.perm = dp<u32>(lv.perm);
     }
     mg.tail_start = -1;
-    if (mg.opt.coarse_cycles == 1) {
+    if (mg.opt.coarse_cycles == 1 && (mg.opt.smoother == MK_JACOBI || mg.opt.smoother == MK_RED_BLACK ||
+                                      mg.opt.smoother == MK_AGGREGATE_BLOCK)) {
         i64 start = depth;
         while (start > 0 && L[start - 1].n <= gpu::kTailRows && L[start - 1].n_agg == 0) --start;
         if (start < depth) mg.tail_start = start;
@@ -281,8 +426,83 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
     const double sweep_bytes = op_bytes(lv.A, ts, ts, 2);
     T *cur = dp<T>(lv.x);
     T *other = dp<T>(lv.xt);
-    const bool blocks = lv.n_agg > 0;
-    if (blocks) {
+    T *res = dp<T>(lv.res);
+    const bool blocks = lv.n_agg > 0, rbgs = lv.rb, ilu = lv.ilu;
+    const bool jacobi = !blocks && !rbgs && !ilu;
+    // a red-black top level orders rows by colour: input and output are permuted
+    const u32 *perm0 = dp<u32>(lv.perm);
+    // the in-place smoothers work in the output when there is one (in the same row order)
+    if ((rbgs || ilu) && out && !perm0) cur = out;
+    bool have_residual = false;
+    // one ILU(0) sweep: x += (L U)^-1 r, with u in `other`
+    auto ilu_sweep = [&](const T *r) {
+        const double fbytes = op_bytes(lv.A, ts, ts, 2) / 2.0 + double(n) * (4 + ts);
+        P.run(K_SWEEP, fbytes, [&] {
+            for (const auto &g : lv.fw_launch)
+                gpu::ilu_forward<T>(lv.A.view, dp<u32>(lv.fw_rows), dp<u32>(lv.fw_ptr), g.g0, g.g1,
+                                    g.wide_rows, dp<T>(lv.pinv), r, other, s);
+        });
+        P.run(K_SWEEP, fbytes + double(n) * 2 * ts, [&] {
+            for (const auto &g : lv.bw_launch)
+                gpu::ilu_backward<T>(lv.A.view, dp<u32>(lv.bw_rows), dp<u32>(lv.bw_ptr), g.g0, g.g1,
+                                     g.wide_rows, dp<T>(lv.pinv), other, cur, s);
+        });
+    };
+    auto half_bytes = [&](int c) { return op_bytes(lv.Ac[c], ts, ts, 2) + 4.0 * double(lv.Ac[c].n); };
+    if (rbgs) {
+        // red-black Gauss-Seidel: colour 0 from x = 0, then colour 1, colour 0, ...; after the last
+        // colour-1 half-sweep only colour-0 rows have a residual
+        if (lv.pre > 0) {
+            T *rb = rhs64 || perm0 ? dp<T>(lv.rhs) : nullptr;
+            P.run(K_FIRST, double(n) * ((rhs64 ? 8.0 + ts : double(ts)) + 2.0 + ts / 2.0 + (rb ? ts : 0)), [&] {
+                if (rhs64) {
+                    gpu::rb_first<T, double>(lv.A.view, dp<std::uint8_t>(lv.colour), shift, rhs64, rb, cur, s,
+                                             scale, perm0);
+                } else {
+                    gpu::rb_first<T, T>(lv.A.view, dp<std::uint8_t>(lv.colour), shift, rhs, rb, cur, s,
+                                        nullptr, perm0);
+                }
+            });
+            if (rb) rhs = rb;
+            for (int half = 1; half < 2 * lv.pre; ++half) {
+                const int c = half & 1;
+                T *rz = half == 2 * lv.pre - 1 ? res : nullptr;
+                P.run(K_SWEEP, half_bytes(c), [&] { gpu::gs_half<T>(lv.Ac[c].view, shift, rhs, cur, rz, s); });
+            }
+            P.run(K_RESIDUAL, half_bytes(0), [&] {
+                gpu::level_residual<T>(lv.Ac[0].view, shift, rhs, cur, res, s);
+            });
+            have_residual = true;
+        } else {
+            if (rhs64 || perm0) {
+                T *rb = dp<T>(lv.rhs);
+                P.run(K_OTHER, double(n) * (8 + ts), [&] {
+                    if (perm0 && rhs64) {
+                        gpu::permute_in<T, double>(n, perm0, rhs64, rb, s, scale);
+                    } else if (perm0) {
+                        gpu::permute_in<T, T>(n, perm0, rhs, rb, s);
+                    } else {
+                        gpu::convert<double, T>(n, rhs64, rb, s, scale);
+                    }
+                });
+                rhs = rb;
+            }
+            P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
+        }
+    } else if (ilu) {
+        if (rhs64) {
+            T *rb = dp<T>(lv.rhs);
+            P.run(K_OTHER, double(n) * (8 + ts), [&] { gpu::convert<double, T>(n, rhs64, rb, s, scale); });
+            rhs = rb;
+        }
+        P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
+        for (int k = 0; k < lv.pre; ++k) {
+            if (k > 0) {
+                P.run(K_RESIDUAL, sweep_bytes, [&] { gpu::level_residual<T>(lv.A.view, shift, rhs, cur, res, s); });
+            }
+            ilu_sweep(k == 0 ? rhs : res); // from x = 0 the residual is rhs
+        }
+    } else if (blocks) {
         if (rhs64) {
             T *rb = dp<T>(lv.rhs);
             P.run(K_OTHER, double(n) * (8 + ts),
@@ -307,17 +527,18 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             gpu::jacobi_first<T, T>(lv.A.view, shift, omega, rhs, nullptr, cur, s);
         });
     }
-    if (!blocks && lv.pre == 0) P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
-    for (int k = 1; !blocks && k < lv.pre; ++k) {
+    if (jacobi && lv.pre == 0) P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
+    for (int k = 1; jacobi && k < lv.pre; ++k) {
         P.run(K_SWEEP, sweep_bytes, [&] {
             gpu::jacobi_sweep<T>(lv.A.view, shift, omega, rhs, cur, other, s);
         });
         std::swap(cur, other);
     }
-    T *res = dp<T>(lv.res);
-    P.run(K_RESIDUAL, sweep_bytes, [&] {
-        gpu::level_residual<T>(lv.A.view, shift, rhs, cur, res, s);
-    });
+    if (!have_residual) {
+        P.run(K_RESIDUAL, sweep_bytes, [&] {
+            gpu::level_residual<T>(lv.A.view, shift, rhs, cur, res, s);
+        });
+    }
     if (lv.n_term > 0) {
         P.run(K_TERMINAL, 0.0, [&] {
             gpu::terminal_solve<T>(lv.n_term, dp<u32>(lv.tstart), dp<u32>(lv.tsize),
@@ -358,7 +579,23 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
         }
     }
     if (blocks) block_smooth<T>(mg, lv, rhs, cur, other, lv.post, true, false);
-    for (int k = 0; !blocks && k < lv.post; ++k) {
+    for (int half = 0; rbgs && half < 2 * lv.post; ++half) {
+        const int c = 1 - (half & 1); // reverse colour order
+        // the last two half-sweeps also store their rows to the output in the caller's order
+        T *nat = perm0 && out && half >= 2 * lv.post - 2 ? out : nullptr;
+        P.run(K_SWEEP, half_bytes(c), [&] {
+            gpu::gs_half<T>(lv.Ac[c].view, shift, rhs, cur, nullptr, s, nat, perm0);
+        });
+    }
+    if (perm0 && out) {
+        if (lv.post == 0) P.run(K_OTHER, double(n) * (2 * ts + 4), [&] { gpu::permute_out<T>(n, perm0, cur, out, s); });
+        return out;
+    }
+    for (int k = 0; ilu && k < lv.post; ++k) {
+        P.run(K_RESIDUAL, sweep_bytes, [&] { gpu::level_residual<T>(lv.A.view, shift, rhs, cur, res, s); });
+        ilu_sweep(res);
+    }
+    for (int k = 0; jacobi && k < lv.post; ++k) {
         T *dst = (k == lv.post - 1 && out) ? out : other;
         P.run(K_SWEEP, sweep_bytes, [&] {
             gpu::jacobi_sweep<T>(lv.A.view, shift, omega, rhs, cur, dst, s);
@@ -933,7 +1170,8 @@ mk_status mk_sector_mg_create(mk_domain domain, mk_box box, double shift,
         mk_multigrid_options o;
         mk_default_multigrid_options(&o);
         if (options) o = *options;
-        require(o.smoother == MK_JACOBI || o.smoother == MK_AGGREGATE_BLOCK,
+        require(o.smoother == MK_JACOBI || o.smoother == MK_AGGREGATE_BLOCK || o.smoother == MK_RED_BLACK ||
+                    o.smoother == MK_ILU0,
                 "sector_multigrid: unknown smoother");
         require(o.precision == MK_FP64 || o.precision == MK_FP32, "sector_multigrid: unknown precision");
         require(o.pre_sweeps >= 0 && o.post_sweeps >= 0 && o.coarse_cycles >= 1,
@@ -971,8 +1209,9 @@ mk_status mk_sector_mg_set_shift(mk_sector_mg mg, double shift) {
     return guarded([&] {
         require(mg != nullptr, "sector_multigrid: bad arguments");
         if (!(shift > 0.0)) throw std::invalid_argument("population_multigrid: shift must be positive");
-        require(mg->opt.smoother == MK_JACOBI,
-                "sector_multigrid: the aggregate-block smoother's blocks depend on the shift; build a new one");
+        require(mg->opt.smoother == MK_JACOBI || mg->opt.smoother == MK_RED_BLACK,
+                "sector_multigrid: the aggregate-block and ILU(0) smoothers' factors depend on the shift; "
+                "build a new one");
         if (mg->precision == MK_FP32) {
             reshift_levels<float>(mg->L32, shift);
         } else {

@@ -51,7 +51,7 @@ u32 padded_grid(i64 nb, u32 chunk) {
     return static_cast<u32>((nb + unit - 1) / unit * unit);
 }
 
-enum class csr_mode { shifted, shifted_residual, sweep, residual, gather_sum };
+enum class csr_mode { shifted, shifted_residual, sweep, residual, gather_sum, gs };
 
 template <typename VT, typename XT, typename T>
 struct csr_args {
@@ -62,13 +62,30 @@ struct csr_args {
     const T *rhs;    // b (shifted_residual) or rhs (sweep, residual)
     T *out;
     double *partial; // shifted_residual: sum of out^2 per row block
+    T *aux;          // gs: residual entries to zero (optional)
+    T *nat;          // gs: also store the new value at nat[perm[row]] (optional)
+    const u32 *perm;
     u32 chunk;       // XCD interleave chunk (set by the launcher)
 };
+
+// vector entry of the operator's local row r
+template <typename VT>
+__device__ __forceinline__ u32 vec_row(const csr_view<VT> &A, u32 r) {
+    return A.rowmap ? A.rowmap[r] : r;
+}
 
 template <csr_mode M, typename VT, typename XT, typename T>
 __device__ __forceinline__ double finish(const csr_args<VT, XT, T> &a, u32 r, T sum, T exit_r) {
     if constexpr (M == csr_mode::gather_sum) {
         a.out[r] = sum;
+        return 0.0;
+    }
+    if constexpr (M == csr_mode::gs) {
+        // in place: the row's entries all have the other colour
+        const T v = (a.rhs[r] + sum) / (static_cast<T>(a.shift) + exit_r);
+        a.out[r] = v;
+        if (a.aux) a.aux[r] = T(0);
+        if (a.nat) a.nat[a.perm[r]] = v;
         return 0.0;
     }
     const T xr = static_cast<T>(a.x[r]);
@@ -132,7 +149,7 @@ __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) 
             if (k0 + u < len) sum += vdict[e[u] >> kColBits] * static_cast<T>(xv[u]);
     }
     double sq = 0.0;
-    if (has_row) sq = finish<M>(a, static_cast<u32>(r), sum, edict[ec]);
+    if (has_row) sq = finish<M>(a, vec_row(a.A, static_cast<u32>(r)), sum, edict[ec]);
     if constexpr (M == csr_mode::shifted_residual) {
         const double t = block_sum(sq, red64);
         if (tid == 0 && a.partial) a.partial[b] = t;
@@ -178,14 +195,14 @@ __global__ void __launch_bounds__(kThreads) k_csr_plain(csr_args<VT, XT, T> a) {
             T sum = 0;
             const u32 q1 = a.A.ptr[r + 1];
             for (u32 q = a.A.ptr[r]; q < q1; ++q) sum += prod[q - p0];
-            sq = finish<M>(a, r, sum, static_cast<T>(a.A.exit[r]));
+            sq = finish<M>(a, vec_row(a.A, r), sum, static_cast<T>(a.A.exit[r]));
         }
     } else {
         T sum = 0;
         for (u32 p = p0 + tid; p < p1; p += kThreads)
             sum += static_cast<T>(a.A.val[p]) * static_cast<T>(a.x[a.A.col[p]]);
         sum = block_sum(sum, red);
-        if (tid == 0) sq = finish<M>(a, r0, sum, static_cast<T>(a.A.exit[r0]));
+        if (tid == 0) sq = finish<M>(a, vec_row(a.A, r0), sum, static_cast<T>(a.A.exit[r0]));
     }
     if constexpr (M == csr_mode::shifted_residual) {
         const double t = block_sum(sq, red64);
@@ -338,6 +355,91 @@ __global__ void __launch_bounds__(kThreads)
     if (q < nrows) dst[rows[q]] = src[rows[q]];
 }
 
+// ---- red-black Gauss-Seidel: the first half-sweep from x = 0
+template <typename T, typename IN>
+__global__ void __launch_bounds__(kThreads)
+    k_rb_first(csr_view<T> A, const u8 *colour, T shift, const IN *rhs_in, const double *scale_ptr,
+               const u32 *perm, T *rhs_out, T *x) {
+    const IN scale = scale_ptr ? static_cast<IN>(*scale_ptr) : IN(1);
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < A.n;
+         i += static_cast<i64>(gridDim.x) * kThreads) {
+        const T b = static_cast<T>(rhs_in[perm ? perm[i] : i] * scale);
+        if (rhs_out) rhs_out[i] = b;
+        if (colour[i] == 0) x[i] = b / (shift + row_exit(A, i));
+    }
+}
+
+// out[i] = (T)(in[perm[i]] * scale) and out[perm[i]] = in[i]: a level's row order to and from
+// the caller's
+template <typename T, typename IN>
+__global__ void __launch_bounds__(kThreads)
+    k_permute_in(i64 n, const u32 *perm, const IN *in, const double *scale_ptr, T *out) {
+    const IN scale = scale_ptr ? static_cast<IN>(*scale_ptr) : IN(1);
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreads)
+        out[i] = static_cast<T>(in[perm[i]] * scale);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_permute_out(i64 n, const u32 *perm, const T *in, T *out) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreads)
+        out[perm[i]] = in[i];
+}
+
+// ---- ILU(0) triangular sweeps over level-scheduled row groups (rows of one group are not
+// neighbours, so they update independently). ONE_WG: one workgroup runs groups [g0, g1) with a
+// barrier after each; else the grid runs the single group g0.
+template <typename T, bool BACK, bool ONE_WG>
+__global__ void __launch_bounds__(ONE_WG ? kTailThreads : kThreads)
+    k_ilu_group(csr_view<T> A, const u32 *rows, const u32 *gptr, int g0, int g1, const T *pinv,
+                const T *r, T *u, T *x) {
+    const u32 first = ONE_WG ? threadIdx.x : blockIdx.x * static_cast<u32>(kThreads) + threadIdx.x;
+    const u32 step = ONE_WG ? blockDim.x : gridDim.x * static_cast<u32>(kThreads);
+    for (int g = g0; g < g1; ++g) {
+        const u32 q1 = gptr[g + 1];
+        for (u32 q = gptr[g] + first; q < q1; q += step) {
+            const u32 i = rows[q];
+            // forward: r_i + sum_{j < i} rate u_j; backward: sum_{j > i} rate e_j (columns ascending)
+            T sum = BACK ? T(0) : r[i];
+            if (A.coded) {
+                const u32 len = A.rlen[i];
+                const u32 base = A.soff[i >> 6] + (i & 63u);
+                for (u32 k = 0; k < len; ++k) {
+                    const u32 e = A.ent[base + k * kSlice];
+                    const u32 c = e & kColMask;
+                    if (BACK ? c > i : c < i) sum += A.vdict[e >> kColBits] * u[c];
+                }
+            } else {
+                for (u32 p = A.ptr[i]; p < A.ptr[i + 1]; ++p) {
+                    const u32 c = A.col[p];
+                    if (BACK ? c > i : c < i) sum += A.val[p] * u[c];
+                }
+            }
+            if constexpr (BACK) {
+                const T e = u[i] + sum * pinv[i];
+                u[i] = e;
+                x[i] += e;
+            } else {
+                u[i] = sum * pinv[i];
+            }
+        }
+        if constexpr (ONE_WG) __syncthreads();
+    }
+}
+
+// one Gauss-Seidel half-sweep of the rows of colour c, in place, by one workgroup
+template <typename T>
+__device__ void tail_gs_half(const csr_view<T> &A, const u8 *colour, u8 c, T shift, const T *rhs,
+                             T *x, i64 n) {
+    for (i64 i = threadIdx.x; i < n; i += blockDim.x) {
+        if (colour[i] != c) continue;
+        T ex = 0;
+        const T sum = gather_row(A, static_cast<u32>(i), x, ex);
+        x[i] = (rhs[i] + sum) / (shift + ex);
+    }
+}
+
 // fused V-cycle tail: one workgroup runs every level from L[0] down and back up
 template <typename T>
 __global__ void __launch_bounds__(kTailThreads)
@@ -348,23 +450,53 @@ __global__ void __launch_bounds__(kTailThreads)
     T *oth[kMaxLevels];
     const int tid = threadIdx.x, nt = blockDim.x;
     const double scale = scale_ptr ? *scale_ptr : 1.0;
+    const u32 *perm0 = L[0].perm; // the top level's row order differs from the caller's
     for (int k = 0; k < levels; ++k) {
         const level_desc<T> &lv = L[k];
-        const T *rhs = k == 0 ? (rhs64 ? lv.rhs : rhs_in) : lv.rhs;
+        const T *rhs = k == 0 ? (rhs64 || perm0 ? lv.rhs : rhs_in) : lv.rhs;
         T *x = lv.x, *xt = lv.xt;
-        // first sweep from x = 0 (the FP64 input converted on the top level)
+        const u8 *colour = lv.colour;
+        // first sweep from x = 0 (the FP64 input converted on the top level); red-black: colour 0
+        // only, colour 1 follows
         for (i64 i = tid; i < lv.n; i += nt) {
             T b;
+            const i64 src = k == 0 && perm0 ? perm0[i] : i;
             if (k == 0 && rhs64) {
-                b = static_cast<T>(rhs64[i] * scale);
+                b = static_cast<T>(rhs64[src] * scale);
+                lv.rhs[i] = b;
+            } else if (k == 0 && perm0) {
+                b = rhs_in[src];
                 lv.rhs[i] = b;
             } else {
                 b = rhs[i];
             }
-            x[i] = lv.pre > 0 ? omega * b / (shift + row_exit(lv.A, i)) : T(0);
+            if (!colour) {
+                x[i] = lv.pre > 0 ? omega * b / (shift + row_exit(lv.A, i)) : T(0);
+            } else if (lv.pre == 0) {
+                x[i] = T(0);
+            } else if (colour[i] == 0) {
+                x[i] = b / (shift + row_exit(lv.A, i));
+            }
         }
         __syncthreads();
-        for (int p = 1; p < lv.pre; ++p) {
+        if (colour) {
+            for (int half = 1; half < 2 * lv.pre; ++half) {
+                tail_gs_half(lv.A, colour, static_cast<u8>(half & 1), shift, rhs, x, lv.n);
+                __syncthreads();
+            }
+            for (i64 i = tid; i < lv.n; i += nt) {
+                if (lv.pre > 0 && colour[i] != 0) {
+                    lv.res[i] = T(0);
+                } else {
+                    T ex = 0;
+                    const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
+                    const T d = shift + ex;
+                    lv.res[i] = rhs[i] - (d * x[i] - sum);
+                }
+            }
+            __syncthreads();
+        }
+        for (int p = 1; !colour && p < lv.pre; ++p) {
             for (i64 i = tid; i < lv.n; i += nt) {
                 T ex = 0;
                 const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
@@ -379,7 +511,7 @@ __global__ void __launch_bounds__(kTailThreads)
             x = xt;
             xt = t;
         }
-        for (i64 i = tid; i < lv.n; i += nt) {
+        for (i64 i = tid; !colour && i < lv.n; i += nt) {
             T ex = 0;
             const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
             const T d = shift + ex;
@@ -412,7 +544,7 @@ __global__ void __launch_bounds__(kTailThreads)
     }
     for (int k = levels - 1; k >= 0; --k) {
         const level_desc<T> &lv = L[k];
-        const T *rhs = k == 0 ? (rhs64 ? lv.rhs : rhs_in) : lv.rhs;
+        const T *rhs = k == 0 ? (rhs64 || perm0 ? lv.rhs : rhs_in) : lv.rhs;
         T *x = cur[k], *xt = oth[k];
         if (lv.n_next > 0) {
             const T *cx = cur[k + 1];
@@ -422,7 +554,12 @@ __global__ void __launch_bounds__(kTailThreads)
             }
             __syncthreads();
         }
-        for (int p = 0; p < lv.post; ++p) {
+        for (int half = 0; lv.colour && half < 2 * lv.post; ++half) {
+            // reverse colour order: 1, 0, 1, 0, ...
+            tail_gs_half(lv.A, lv.colour, static_cast<u8>(1 - (half & 1)), shift, rhs, x, lv.n);
+            __syncthreads();
+        }
+        for (int p = 0; !lv.colour && p < lv.post; ++p) {
             T *dst = (k == 0 && p == lv.post - 1 && out) ? out : xt;
             for (i64 i = tid; i < lv.n; i += nt) {
                 T ex = 0;
@@ -444,9 +581,11 @@ __global__ void __launch_bounds__(kTailThreads)
         cur[k] = x;
         oth[k] = xt;
     }
-    // the top level's result in out, else in L[0].x
+    // the top level's result in out (in the caller's row order), else in L[0].x
     T *dst = out ? out : L[0].x;
-    if (cur[0] != dst) {
+    if (out && perm0) {
+        for (i64 i = tid; i < L[0].n; i += nt) dst[perm0[i]] = cur[0][i];
+    } else if (cur[0] != dst) {
         for (i64 i = tid; i < L[0].n; i += nt) dst[i] = cur[0][i];
     }
 }
@@ -757,7 +896,7 @@ int reduction_blocks(i64 n) {
 template <typename XT>
 void shifted_apply(const csr_view<double> &A, double shift, const XT *x, const double *b,
                    double *y, double *partial, hipStream_t s) {
-    const csr_args<double, XT, double> a{A, shift, 0.0, x, b, y, partial, 0u};
+    const csr_args<double, XT, double> a{A, shift, 0.0, x, b, y, partial, nullptr, nullptr, nullptr, 0u};
     if (b) {
         launch_csr<csr_mode::shifted_residual>(a, s);
     } else {
@@ -777,18 +916,74 @@ template <typename T>
 void jacobi_sweep(const csr_view<T> &A, double shift, double omega, const T *rhs, const T *x,
                   T *x_out, hipStream_t s) {
     launch_csr<csr_mode::sweep>(csr_args<T, T, T>{A, shift, static_cast<T>(omega), x, rhs, x_out,
-                                                  nullptr, 0u}, s);
+                                                  nullptr, nullptr, nullptr, nullptr, 0u}, s);
 }
 
 template <typename T>
 void level_residual(const csr_view<T> &A, double shift, const T *rhs, const T *x, T *r,
                     hipStream_t s) {
-    launch_csr<csr_mode::residual>(csr_args<T, T, T>{A, shift, T(0), x, rhs, r, nullptr, 0u}, s);
+    launch_csr<csr_mode::residual>(
+        csr_args<T, T, T>{A, shift, T(0), x, rhs, r, nullptr, nullptr, nullptr, nullptr, 0u}, s);
+}
+
+template <typename T>
+void gs_half(const csr_view<T> &Ac, double shift, const T *rhs, T *x, T *res_zero, hipStream_t s,
+             T *nat, const u32 *perm) {
+    launch_csr<csr_mode::gs>(csr_args<T, T, T>{Ac, shift, T(0), x, rhs, x, nullptr, res_zero, nat, perm, 0u},
+                             s);
+}
+
+template <typename T, typename IN>
+void permute_in(i64 n, const u32 *perm, const IN *in, T *out, hipStream_t s, const double *scale) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_permute_in<T, IN>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, perm, in, scale,
+                       out);
+}
+
+template <typename T>
+void permute_out(i64 n, const u32 *perm, const T *in, T *out, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_permute_out<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, perm, in, out);
+}
+
+template <typename T, typename IN>
+void rb_first(const csr_view<T> &A, const u8 *colour, double shift, const IN *rhs_in, T *rhs_out,
+              T *x, hipStream_t s, const double *scale, const u32 *perm) {
+    if (A.n == 0) return;
+    hipLaunchKernelGGL((k_rb_first<T, IN>), dim3(grid_for(A.n)), dim3(kThreads), 0, s, A, colour,
+                       static_cast<T>(shift), rhs_in, scale, perm, rhs_out, x);
+}
+
+template <typename T>
+void ilu_forward(const csr_view<T> &A, const u32 *rows, const u32 *gptr, int g0, int g1,
+                 i64 wide_rows, const T *pinv, const T *r, T *u, hipStream_t s) {
+    if (g1 <= g0) return;
+    if (wide_rows > 0) {
+        hipLaunchKernelGGL((k_ilu_group<T, false, false>), dim3(grid_for(wide_rows)), dim3(kThreads), 0, s,
+                           A, rows, gptr, g0, g0 + 1, pinv, r, u, nullptr);
+    } else {
+        hipLaunchKernelGGL((k_ilu_group<T, false, true>), dim3(1), dim3(kTailThreads), 0, s, A, rows,
+                           gptr, g0, g1, pinv, r, u, nullptr);
+    }
+}
+
+template <typename T>
+void ilu_backward(const csr_view<T> &A, const u32 *rows, const u32 *gptr, int g0, int g1,
+                  i64 wide_rows, const T *pinv, T *u, T *x, hipStream_t s) {
+    if (g1 <= g0) return;
+    if (wide_rows > 0) {
+        hipLaunchKernelGGL((k_ilu_group<T, true, false>), dim3(grid_for(wide_rows)), dim3(kThreads), 0, s,
+                           A, rows, gptr, g0, g0 + 1, pinv, nullptr, u, x);
+    } else {
+        hipLaunchKernelGGL((k_ilu_group<T, true, true>), dim3(1), dim3(kTailThreads), 0, s, A, rows,
+                           gptr, g0, g1, pinv, nullptr, u, x);
+    }
 }
 
 template <typename T>
 void gather_sum(const csr_view<T> &R, const T *x, T *y, hipStream_t s) {
-    launch_csr<csr_mode::gather_sum>(csr_args<T, T, T>{R, 0.0, T(0), x, nullptr, y, nullptr, 0u}, s);
+    launch_csr<csr_mode::gather_sum>(
+        csr_args<T, T, T>{R, 0.0, T(0), x, nullptr, y, nullptr, nullptr, nullptr, nullptr, 0u}, s);
 }
 
 template <typename T>
@@ -981,7 +1176,15 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
     template void vcycle_tail<T>(const level_desc<T> *, int, double, double, const T *,         \
                                  const double *, const double *, T *, hipStream_t);              \
     template void axpy<T>(i64, T, const T *, T *, hipStream_t);                                 \
-    template void combine<T>(double *, const T *, i64, int, const double *, i64, hipStream_t);
+    template void combine<T>(double *, const T *, i64, int, const double *, i64, hipStream_t);     \
+    template void gs_half<T>(const csr_view<T> &, double, const T *, T *, T *, hipStream_t, T *,   \
+                             const u32 *);                                                        \
+    template void permute_out<T>(i64, const u32 *, const T *, T *, hipStream_t);                 \
+    template void permute_in<T, T>(i64, const u32 *, const T *, T *, hipStream_t, const double *); \
+    template void ilu_forward<T>(const csr_view<T> &, const u32 *, const u32 *, int, int, i64,    \
+                                 const T *, const T *, T *, hipStream_t);                         \
+    template void ilu_backward<T>(const csr_view<T> &, const u32 *, const u32 *, int, int, i64,   \
+                                  const T *, T *, T *, hipStream_t);
 MK_LEVEL(float)
 MK_LEVEL(double)
 #undef MK_LEVEL
@@ -993,6 +1196,14 @@ template void jacobi_first<float, double>(const csr_view<float> &, double, doubl
                                           const double *);
 template void jacobi_first<float, float>(const csr_view<float> &, double, double, const float *,
                                          float *, float *, hipStream_t, const double *);
+template void rb_first<double, double>(const csr_view<double> &, const u8 *, double, const double *,
+                                       double *, double *, hipStream_t, const double *, const u32 *);
+template void rb_first<float, double>(const csr_view<float> &, const u8 *, double, const double *,
+                                      float *, float *, hipStream_t, const double *, const u32 *);
+template void rb_first<float, float>(const csr_view<float> &, const u8 *, double, const float *,
+                                     float *, float *, hipStream_t, const double *, const u32 *);
+template void permute_in<float, double>(i64, const u32 *, const double *, float *, hipStream_t,
+                                        const double *);
 template void convert<double, float>(i64, const double *, float *, hipStream_t, const double *);
 template void convert<float, double>(i64, const float *, double *, hipStream_t, const double *);
 template void convert<double, double>(i64, const double *, double *, hipStream_t, const double *);

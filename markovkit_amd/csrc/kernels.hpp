@@ -69,6 +69,9 @@ struct csr_view {
     const VT *edict = nullptr;
     int n_vdict = 0;
     int n_edict = 0;
+    // a row subset (one colour of a red-black split): local row r is vector entry rowmap[r];
+    // null for the identity
+    const u32 *rowmap = nullptr;
 };
 
 // y = (shift + exit) o x - C x             (FP64 accumulate; x in XT)
@@ -109,6 +112,48 @@ template <typename T>
 void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
                     const T *inverse, i64 max_size, const T *r, T *x, hipStream_t s);
 
+// Red-black Gauss-Seidel on one level. The level operator is split by colour into two
+// row-mapped operators (rows of one colour, all their entries); a half-sweep updates one colour
+// in place from the other colour's values:
+// The levels of a red-black hierarchy number each sector's states colour 0 first, so a
+// half-sweep streams contiguous row ranges; perm maps the top level's rows to the caller's
+// order (row i is the caller's entry perm[i]).
+//   gs_half: x[i] = (rhs[i] + sum rate x[j]) / (shift + exit_i) for the operator's rows;
+//   res_zero[i] = 0 when res_zero is given (after the last pre-smoothing half-sweep the residual
+//   of the rows just solved is zero by construction); nat[perm[i]] = x[i] when nat is given
+template <typename T>
+void gs_half(const csr_view<T> &Ac, double shift, const T *rhs, T *x, T *res_zero, hipStream_t s,
+             T *nat = nullptr, const u32 *perm = nullptr);
+//   rb_first: the first half-sweep from x = 0 on the full level: rhs[i] = (T)(scale *
+//   rhs_in[perm[i]]) (perm null: identity), stored to rhs_out if given; x[i] = rhs[i] /
+//   (shift + exit_i) for colour[i] == 0 (colour-1 entries of x are left for the following
+//   half-sweep to set)
+template <typename T, typename IN>
+void rb_first(const csr_view<T> &A, const u8 *colour, double shift, const IN *rhs_in, T *rhs_out,
+              T *x, hipStream_t s, const double *scale = nullptr, const u32 *perm = nullptr);
+// out[i] = (T)(scale * in[perm[i]]); out[perm[i]] = in[i]
+template <typename T, typename IN>
+void permute_in(i64 n, const u32 *perm, const IN *in, T *out, hipStream_t s,
+                const double *scale = nullptr);
+template <typename T>
+void permute_out(i64 n, const u32 *perm, const T *in, T *out, hipStream_t s);
+
+// ILU(0) smoothing of one level (rank order; on the bipartite diffusion graph only the pivots
+// differ from A). The triangular solves are level-scheduled: rows are grouped by dependency
+// depth (a row's lower / upper neighbours lie in earlier groups), one group per launch, or a
+// run of small groups in one workgroup with barriers between them. Groups [g0, g1) of rows[]
+// (group k is rows[gptr[k] .. gptr[k + 1])):
+// wide_rows > 0 launches the single group g0 (g1 = g0 + 1) of that many rows over a grid;
+// wide_rows == 0 runs groups [g0, g1) in one workgroup.
+//   ilu_forward:  u[i] = (r[i] + sum_{j < i} rate u[j]) * pinv[i]
+//   ilu_backward: e[i] = u[i] + (sum_{j > i} rate e[j]) * pinv[i] (in place in u); x[i] += e[i]
+template <typename T>
+void ilu_forward(const csr_view<T> &A, const u32 *rows, const u32 *gptr, int g0, int g1,
+                 i64 wide_rows, const T *pinv, const T *r, T *u, hipStream_t s);
+template <typename T>
+void ilu_backward(const csr_view<T> &A, const u32 *rows, const u32 *gptr, int g0, int g1,
+                  i64 wide_rows, const T *pinv, T *u, T *x, hipStream_t s);
+
 // aggregate-block smoother (multigrid.hpp:309-342), one aggregate's step at a time:
 //   block_solve: for members g in [m0, m1): e = inverse_b r[members of b], x += e, escr = e
 //   block_residual: r[i] -= (A escr)[i] for listed rows q in [q0, q1) (escr zero off members)
@@ -143,10 +188,13 @@ struct level_desc {
     const u64 *toff = nullptr;
     const T *tinv = nullptr;
     T *rhs = nullptr, *x = nullptr, *xt = nullptr, *res = nullptr; // work vectors
+    const u8 *colour = nullptr;        // red-black smoother: colour of every row, else null
+    const u32 *perm = nullptr;         // top level of a red-black hierarchy: row -> caller's entry
 };
 constexpr i64 kTailRows = 1024;  // levels of at most this many rows run in the fused tail
 constexpr int kTailThreads = 1024;
-// The V-cycle below and including level L[0] (Jacobi smoother, one coarse cycle) in one
+// The V-cycle below and including level L[0] (Jacobi smoother, or red-black Gauss-Seidel on
+// levels with a colour array; one coarse cycle) in one
 // workgroup: the same per-row formulas and summation orders as the per-level kernels, phases
 // separated by workgroup barriers. L[0]'s right-hand side is rhs, or (T)(rhs64 * scale) when
 // rhs64 is given (scale from device memory, 1 when null); the result goes to out, else to

@@ -511,6 +511,24 @@ struct level_dev {
     dbuf bs_members, bs_member_block, bs_block_first, bs_inv_off, bs_inv, bs_rows, escr;
     i64 n_jrows = 0;
     dbuf jrows;
+    // red-black Gauss-Seidel: the level split by colour into row-mapped operators, and the
+    // colour of every row
+    bool rb = false;
+    dev_csr<T> Ac[2];
+    dbuf rowmap[2], colour;
+    // the level numbers each sector's states colour 0 first; on the top level, perm maps a row
+    // to its index in the caller's (box) order
+    dbuf perm;
+    // ILU(0): reciprocal pivots (for the hierarchy's shift) and the level schedules of the
+    // forward and backward solves: rows grouped by dependency depth (group k is
+    // rows[ptr[k] .. ptr[k + 1])), launched as single wide groups or runs of small groups
+    struct ilu_launch {
+        int g0 = 0, g1 = 0;
+        i64 wide_rows = 0; // > 0: one group of this many rows over a grid
+    };
+    bool ilu = false;
+    dbuf pinv, fw_rows, fw_ptr, bw_rows, bw_ptr;
+    std::vector<ilu_launch> fw_launch, bw_launch;
     // work vectors
     dbuf rhs, x, xt, res, crhs, cx;
     int pre = 2, post = 2;

@@ -204,7 +204,8 @@ mk_status mk_server_add_family(mk_server s, mk_generator gen, int32_t n_mg, cons
         f.n = gen->n;
         for (int k = 0; k < n_mg; ++k) {
             require(mg[k] && mg[k]->rows == gen->n, "server: a hierarchy does not match the generator");
-            require(mg[k]->opt.smoother == MK_JACOBI, "server: hierarchies must use the Jacobi smoother");
+            require(mg[k]->opt.smoother == MK_JACOBI || mg[k]->opt.smoother == MK_RED_BLACK,
+                    "server: hierarchies must use the Jacobi or red-black smoother (re-targeted per shift)");
             f.zs = mg[k]->precision == MK_FP32 ? sizeof(float) : sizeof(double);
             f.free.push_back(mg[k]);
         }

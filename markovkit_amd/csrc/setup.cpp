@@ -724,6 +724,37 @@ block_smoother build_blocks(const population_space &space, const std::vector<i64
     return bs;
 }
 
+// red-black colouring of a sector's diffusion graph (a hop changes one molecule's voxel
+// coordinate sum by one, so it flips the parity); checked on every entry
+std::vector<std::uint8_t> state_colours(const population_space &space, const voxel_domain &dom,
+                                        const generator_csr &A) {
+    const int S = space.species();
+    std::vector<int> parity(dom.voxels);
+    for (i64 v = 0; v < dom.voxels; ++v) {
+        i64 c = 0;
+        for (i64 a : dom.coordinate(v)) c += a;
+        parity[v] = static_cast<int>(c & 1);
+    }
+    const i64 n = space.size();
+    std::vector<std::uint8_t> colour(n);
+    const i64 chunk = 32768;
+    run_tasks((n + chunk - 1) / chunk, [&](i64 t) {
+        std::vector<int> x(dom.voxels * S);
+        for (i64 i = t * chunk; i < std::min(n, (t + 1) * chunk); ++i) {
+            space.state_in_sector(0, i, x.data());
+            int p = 0;
+            for (i64 v = 0; v < dom.voxels; ++v)
+                for (int s = 0; s < S; ++s) p ^= (x[v * S + s] & 1) & parity[v];
+            colour[i] = static_cast<std::uint8_t>(p);
+        }
+    });
+    for (i64 i = 0; i < n; ++i)
+        for (u64 p = A.ptr[i]; p < A.ptr[i + 1]; ++p)
+            if (colour[A.col[p]] == colour[i])
+                throw std::logic_error("red-black smoother: the diffusion graph is not bipartite");
+    return colour;
+}
+
 } // namespace
 
 sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::vector<int> &totals,
@@ -741,6 +772,7 @@ sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::v
         level.pre_sweeps = opt.pre_sweeps;
         level.post_sweeps = opt.post_sweeps;
         level.A = assemble_generator(space, dom, none, true);
+        if (opt.smoother == 2) level.colour = state_colours(space, dom, level.A);
         std::vector<double> doubled = dom.diffusivities;
         for (double &d : doubled) d *= 2.0;
         auto coarse = coarsen(dom, doubled);

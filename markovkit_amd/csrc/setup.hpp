@@ -145,7 +145,7 @@ struct multigrid_options {
     int max_levels = 32;
     int coarse_cycles = 1;
     int sweep_growth = 0;
-    int smoother = 0; // 0 Jacobi, 1 aggregate blocks
+    int smoother = 0; // 0 Jacobi, 1 aggregate blocks, 2 red-black Gauss-Seidel, 3 ILU(0)
     int precision = 0;
 };
 
@@ -177,6 +177,9 @@ struct sector_level {
     int pre_sweeps = 2, post_sweeps = 2;
     bool has_blocks = false;     // aggregate-block smoother (non-terminal levels)
     block_smoother blocks;
+    // red-black smoother: colour of every state, the parity of sum over voxels v and species
+    // s of x[v, s] * (coordinate sum of v); every diffusion jump joins the two colours
+    std::vector<std::uint8_t> colour;
 };
 
 struct sector_hierarchy {

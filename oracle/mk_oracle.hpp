@@ -19,6 +19,8 @@
 //   population_transfer (multinomial P, aggregation R)   multigrid.hpp:47-118
 //   population_multigrid (V/W cycle, weighted Jacobi and aggregate-block smoothers)
 //                                                         multigrid.hpp:139-361
+//       with two more smoothers: red-black Gauss-Seidel and ILU(0) in rank order
+//                                  rdme/cpp/rdme_full_multigrid.cpp:178-195, :204-296
 //   sector_multigrid                                       multigrid.hpp:371-408
 //   num::gmres (absent; restated as restarted right-preconditioned GMRES, Saad's
 //       algorithm with modified Gram-Schmidt and Givens rotations, absolute Euclidean
@@ -648,8 +650,18 @@ class population_transfer {
     idx coarse_size_;
 };
 
-// multigrid.hpp:120-134
-enum class relaxation : std::uint8_t { jacobi, aggregate_block };
+// multigrid.hpp:120-134, plus two smoothers of the paper driver's family:
+//   red_black: Gauss-Seidel in two colours. The fixed-population diffusion graph on a grid is
+//     bipartite (a hop changes the parity of the sum over molecules of their voxels'
+//     coordinate sums; rdme/cpp/rdme_full_multigrid.cpp:218-221), so updating every state of
+//     one colour from the residual taken just before is exact Gauss-Seidel in the order
+//     "colour 0 states, then colour 1 states" (lexicographic GS as at
+//     rdme_full_multigrid.cpp:178-195 in that order). Colour 0 first; reversed for
+//     post-smoothing.
+//   ilu0: x += (L U)^-1 (rhs - A x) with the ILU(0) factors of the level operator in its rank
+//     order (rdme_full_multigrid.cpp:204-296, structured case). On a bipartite graph ILU(0)
+//     changes only the diagonal: pivot_i = a_ii - sum_{j < i} a_ij a_ji / pivot_j.
+enum class relaxation : std::uint8_t { jacobi, aggregate_block, red_black, ilu0 };
 
 struct multigrid_options {
     int pre_sweeps = 2;
@@ -673,6 +685,8 @@ class population_multigrid {
         if (!(shift > 0.0))
             throw std::invalid_argument("population_multigrid: shift must be positive");
         if (!domain.structured) throw std::invalid_argument("population_multigrid needs a grid");
+        if (options.smoother == relaxation::red_black) build_colours(*domain.structured);
+        if (options.smoother == relaxation::ilu0) build_ilu();
         auto coarse_grid = coarsen(*domain.structured);
         if (rows() <= options.coarsest_size || options.max_levels <= 1 || !coarse_grid) {
             std::vector<idx> all(rows());
@@ -750,6 +764,9 @@ class population_multigrid {
     [[nodiscard]] const std::vector<idx> &block_states(idx a, idx b) const {
         return blocks_[a][b].states;
     }
+    // red_black: the colour of every state; ilu0: the ILU(0) pivots (for tests)
+    [[nodiscard]] const std::vector<std::uint8_t> &colours() const { return colour_; }
+    [[nodiscard]] const std::vector<real> &ilu_pivots() const { return pivot_; }
 
   private:
     struct block {
@@ -800,12 +817,95 @@ class population_multigrid {
         if (f.singular) throw std::runtime_error("population_multigrid: singular block");
         return f;
     }
+    // colour of x = parity of sum over voxels v and species s of x[v, s] * (coordinate sum of v);
+    // every jump must join the two colours
+    void build_colours(const grid &g) {
+        const idx S = space_.n_species();
+        std::vector<int> parity(space_.n_voxels());
+        for (idx v = 0; v < space_.n_voxels(); ++v) {
+            idx c = 0;
+            for (idx a : g.coordinate(v)) c += a;
+            parity[v] = static_cast<int>(c & 1);
+        }
+        colour_.assign(rows(), 0);
+        for (idx i = 0; i < rows(); ++i) {
+            int p = 0;
+            for (idx v = 0; v < space_.n_voxels(); ++v)
+                for (idx s = 0; s < S; ++s) p ^= (space_[i][v * S + s] & 1) & parity[v];
+            colour_[i] = static_cast<std::uint8_t>(p);
+        }
+        for (idx j = 0; j < rows(); ++j)
+            for (const auto *e = Q_.jumps_begin(j); e != Q_.jumps_end(j); ++e)
+                if (colour_[e->to] == colour_[j])
+                    throw std::logic_error("red-black smoother: the diffusion graph is not bipartite");
+    }
+    // gather lists (row i: every j with a jump j -> i, ascending) and the ILU(0) pivots
+    void build_ilu() {
+        const idx n = rows();
+        in_first_.assign(n + 1, 0);
+        for (idx j = 0; j < n; ++j)
+            for (const auto *e = Q_.jumps_begin(j); e != Q_.jumps_end(j); ++e) ++in_first_[e->to + 1];
+        for (idx i = 0; i < n; ++i) in_first_[i + 1] += in_first_[i];
+        in_from_.resize(in_first_[n]);
+        in_rate_.resize(in_first_[n]);
+        std::vector<idx> fill(in_first_.begin(), in_first_.end() - 1);
+        for (idx j = 0; j < n; ++j) // sources ascending: each row's list comes out ascending
+            for (const auto *e = Q_.jumps_begin(j); e != Q_.jumps_end(j); ++e) {
+                in_from_[fill[e->to]] = j;
+                in_rate_[fill[e->to]++] = e->rate;
+            }
+        // a_ij a_ji = rate(j -> i) rate(i -> j); the reverse rate is found in row j's list
+        pivot_.assign(n, 0.0);
+        for (idx i = 0; i < n; ++i) {
+            real d = shift_ + Q_.exit_rates()[i];
+            for (idx p = in_first_[i]; p < in_first_[i + 1] && in_from_[p] < i; ++p) {
+                const idx j = in_from_[p];
+                real back = 0.0;
+                for (idx q = in_first_[j]; q < in_first_[j + 1]; ++q)
+                    if (in_from_[q] == i) back = in_rate_[q];
+                d -= in_rate_[p] * back / pivot_[j];
+            }
+            if (!(d > 0.0) || !std::isfinite(d)) throw std::runtime_error("population_multigrid: ILU(0) pivot");
+            pivot_[i] = d;
+        }
+    }
     void residual() const {
         apply_system(x_.data(), r_.data());
         for (idx i = 0; i < rows(); ++i) r_[i] = rhs_[i] - r_[i];
     }
     // multigrid.hpp:309-342
     void smooth(int sweeps, bool reverse) const {
+        if (!colour_.empty()) {
+            for (int sweep = 0; sweep < sweeps; ++sweep)
+                for (int half = 0; half < 2; ++half) {
+                    const std::uint8_t c = static_cast<std::uint8_t>(reverse ? 1 - half : half);
+                    residual();
+                    for (idx i = 0; i < rows(); ++i)
+                        if (colour_[i] == c) x_[i] += r_[i] / (shift_ + Q_.exit_rates()[i]);
+                }
+            return;
+        }
+        if (!pivot_.empty()) {
+            const idx n = rows();
+            for (int sweep = 0; sweep < sweeps; ++sweep) {
+                residual();
+                // forward: (D + L) u = r; backward: (D + U) e = D u; e overwrites u in e_
+                for (idx i = 0; i < n; ++i) {
+                    real s = r_[i];
+                    for (idx p = in_first_[i]; p < in_first_[i + 1] && in_from_[p] < i; ++p)
+                        s += in_rate_[p] * e_[in_from_[p]];
+                    e_[i] = s / pivot_[i];
+                }
+                for (idx i = n; i-- > 0;) {
+                    real s = 0.0;
+                    for (idx p = in_first_[i]; p < in_first_[i + 1]; ++p)
+                        if (in_from_[p] > i) s += in_rate_[p] * e_[in_from_[p]];
+                    e_[i] += s / pivot_[i];
+                }
+                for (idx i = 0; i < n; ++i) x_[i] += e_[i];
+            }
+            return;
+        }
         if (blocks_.empty()) {
             for (int sweep = 0; sweep < sweeps; ++sweep) {
                 residual();
@@ -851,6 +951,9 @@ class population_multigrid {
     std::unique_ptr<population_transfer> transfer_;
     lu_result terminal_;
     std::vector<std::vector<block>> blocks_;
+    std::vector<std::uint8_t> colour_;
+    std::vector<idx> in_first_, in_from_;
+    std::vector<real> in_rate_, pivot_;
     mutable std::vector<real> rhs_, x_, r_, e_;
     mutable std::vector<real> coarse_rhs_, coarse_x_, coarse_r_, coarse_e_;
 };

@@ -51,6 +51,8 @@ def lib():
         L.orc_mg_create.argtypes = [P, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int64,
                                     C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.orc_mg_relax.argtypes = [P, C.c_int64, f64p, f64p, C.c_int, C.c_int]
+        L.orc_mg_colours.argtypes = [P, C.c_int64, np.ctypeslib.ndpointer(np.uint8, flags="C")]
+        L.orc_mg_ilu_pivots.argtypes = [P, C.c_int64, f64p]
         L.orc_expv_jumps.argtypes = [C.c_int64, i64p, i64p, f64p, f64p, C.c_double, f64p, f64p,
                                      C.c_int64, C.c_double, i64p, i64p]
         L.orc_mg_destroy.argtypes = [P]
@@ -182,17 +184,20 @@ class Problem:
 
 
 class SectorMultigrid:
-    """sector_multigrid on the CPU (smoother "jacobi" or "aggregate_block")."""
+    """sector_multigrid on the CPU (smoother "jacobi", "aggregate_block", "red_black" or
+    "ilu0")."""
+
+    SMOOTHERS = ("jacobi", "aggregate_block", "red_black", "ilu0")
 
     def __init__(self, problem, shift, pre_sweeps=2, post_sweeps=2, jacobi_weight=2.0 / 3.0,
                  coarsest_size=4, max_levels=32, coarse_cycles=1, sweep_growth=0,
                  smoother="jacobi"):
-        if smoother not in ("jacobi", "aggregate_block"):
-            raise ValueError("smoother must be 'jacobi' or 'aggregate_block'")
+        if smoother not in self.SMOOTHERS:
+            raise ValueError(f"smoother must be one of {self.SMOOTHERS}")
         h = C.c_void_p()
         _check(lib().orc_mg_create(problem.h, float(shift), pre_sweeps, post_sweeps,
                                    jacobi_weight, coarsest_size, max_levels, coarse_cycles,
-                                   sweep_growth, 1 if smoother == "aggregate_block" else 0,
+                                   sweep_growth, self.SMOOTHERS.index(smoother),
                                    C.byref(h)))
         self.h = h
         self.problem = problem
@@ -219,6 +224,18 @@ class SectorMultigrid:
         _check(lib().orc_mg_relax(self.h, k, np.ascontiguousarray(rhs, np.float64), x, sweeps,
                                   1 if reverse else 0))
         return x
+
+    def colours(self, k):
+        """red-black colours of sector k's finest level"""
+        out = np.zeros(self.level_rows()[k, 0], np.uint8)
+        _check(lib().orc_mg_colours(self.h, k, out))
+        return out
+
+    def ilu_pivots(self, k):
+        """ILU(0) pivots of sector k's finest level (rank order)"""
+        out = np.zeros(self.level_rows()[k, 0])
+        _check(lib().orc_mg_ilu_pivots(self.h, k, out))
+        return out
 
     def apply(self, b):
         x = np.zeros(self.n)

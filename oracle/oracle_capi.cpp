@@ -185,7 +185,8 @@ int orc_mg_create(void *h, double shift, int pre, int post, double omega, int64_
     return guarded([&] {
         auto *p = static_cast<problem *>(h);
         multigrid_options o;
-        o.smoother = smoother == 1 ? relaxation::aggregate_block : relaxation::jacobi;
+        if (smoother < 0 || smoother > 3) throw std::invalid_argument("sector_multigrid: unknown smoother");
+        o.smoother = static_cast<relaxation>(smoother);
         o.pre_sweeps = pre;
         o.post_sweeps = post;
         o.jacobi_weight = omega;
@@ -204,6 +205,24 @@ int orc_mg_relax(void *mg, int64_t k, const double *rhs, double *x, int sweeps, 
         auto *m = static_cast<sector_multigrid *>(mg);
         if (k < 0 || k >= static_cast<int64_t>(m->n_sectors())) throw std::invalid_argument("sector out of range");
         m->sector(k).relax(rhs, x, sweeps, reverse != 0);
+    });
+}
+
+// sector k's finest level: red-black colours (red_black) or ILU(0) pivots (ilu0); -1 when the
+// hierarchy has no such smoother
+int orc_mg_colours(void *mg, int64_t k, uint8_t *out) {
+    return guarded([&] {
+        const auto &c = static_cast<sector_multigrid *>(mg)->sector(k).colours();
+        if (c.empty()) throw std::invalid_argument("no red-black colours");
+        std::copy(c.begin(), c.end(), out);
+    });
+}
+
+int orc_mg_ilu_pivots(void *mg, int64_t k, double *out) {
+    return guarded([&] {
+        const auto &p = static_cast<sector_multigrid *>(mg)->sector(k).ilu_pivots();
+        if (p.empty()) throw std::invalid_argument("no ILU(0) pivots");
+        std::copy(p.begin(), p.end(), out);
     });
 }
 

@@ -82,14 +82,18 @@ def test_mass_balance(case):
     assert abs(lhs - rhs) <= 1e-12 * abs(rhs)
 
 
+SMOOTHERS = {mk.JACOBI: "jacobi", mk.AGGREGATE_BLOCK: "aggregate_block", mk.RED_BLACK: "red_black",
+             mk.ILU0: "ilu0"}
+
+
 @pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
-@pytest.mark.parametrize("smoother", [mk.JACOBI, mk.AGGREGATE_BLOCK], ids=["jacobi", "blocks"])
+@pytest.mark.parametrize("smoother", list(SMOOTHERS), ids=["jacobi", "blocks", "red_black", "ilu0"])
 def test_multigrid_cycle(case, precision, smoother):
     w, P, box = case["w"], case["P"], case["box"]
     shift = w.shifts[0]
     M = mk.SectorMultigrid(w.model.domain, box, shift,
                            mk.MultigridOptions(precision=precision, smoother=smoother))
-    Mo = OracleMG(P, shift, smoother="jacobi" if smoother == mk.JACOBI else "aggregate_block")
+    Mo = OracleMG(P, shift, smoother=SMOOTHERS[smoother])
     assert M.max_levels() == Mo.max_levels()
     assert M.n_sectors() == len(P.sector_begins()) - 1
     rng = np.random.default_rng(11)
@@ -98,7 +102,9 @@ def test_multigrid_cycle(case, precision, smoother):
         z = M.apply(b)
         z_ref = Mo.apply(b)
         err = np.abs(z - z_ref).max() / np.abs(z_ref).max()
-        tol = 1e-12 if precision == mk.FP64 else 2e-5
+        # FP32 ILU(0): the triangular solves carry rounding along dependency chains of up to
+        # total * (voxels - 1) rows (124 at 32 voxels, 4 molecules)
+        tol = 1e-12 if precision == mk.FP64 else (1e-4 if smoother == mk.ILU0 else 2e-5)
         assert err <= tol, (case["name"], precision, err)
 
 
@@ -181,6 +187,35 @@ def test_aggregate_block_solve(case, precision):
     assert its["blocks"] <= its["jacobi"], its
     print(f"[{case['name']}] iterations jacobi {its['jacobi']} blocks {its['blocks']} "
           f"oracle blocks {info['iterations']}")
+
+
+@pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("smoother", [mk.RED_BLACK, mk.ILU0], ids=["red_black", "ilu0"])
+def test_smoother_solve(case, precision, smoother):
+    """FGMRES preconditioned by the red-black Gauss-Seidel or ILU(0) sector multigrid reaches
+    the 1e-10 residual and the oracle's answer (the smoother changes iterates, not the
+    answer), in no more iterations than the oracle's Jacobi-smoothed solve."""
+    w, P, box, Q = case["w"], case["P"], case["box"], case["Q"]
+    shift = w.shifts[0]
+    b = w.rhs(0, P.n)
+    tol = 1e-10 * np.linalg.norm(b)
+    M = mk.SectorMultigrid(w.model.domain, box, shift,
+                           mk.MultigridOptions(precision=precision, smoother=smoother))
+    x = np.zeros(P.n)
+    res = mk.gmres(mk.ShiftedGenerator(Q, shift), M, b, x, tolerance=tol, max_iterations=500,
+                   restart=50)
+    assert res.converged, res
+    r = b - P.apply_shifted(shift, x)
+    assert np.linalg.norm(r) <= 1.0001 * tol
+    x_ref, info = P.gmres(shift, b, mg=OracleMG(P, shift, smoother=SMOOTHERS[smoother]), tol=tol,
+                          restart=50)
+    assert info["converged"]
+    bound = (np.abs(r).sum() + np.abs(b - P.apply_shifted(shift, x_ref)).sum()) / shift
+    assert np.abs(x - x_ref).sum() <= bound
+    _, jac = P.gmres(shift, b, mg=OracleMG(P, shift), tol=tol, restart=50)
+    assert res.iterations <= jac["iterations"] + 1, (res.iterations, jac["iterations"])
+    print(f"[{case['name']}] {SMOOTHERS[smoother]} iterations {res.iterations} "
+          f"(oracle {info['iterations']}, oracle jacobi {jac['iterations']})")
 
 
 def test_plain_gmres(case):

@@ -167,7 +167,7 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16):
     }
 
 
-def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0):
+def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0, smoother_name="red_black"):
     """C5: Poisson arrivals at `rate` requests/s (numpy PCG64, seed 1). 70% are RDME resolvent
     requests of the 1D family (voxels in {16, 32, 48, 64}, caps in {3, 4, 5}, shift ~ U[0.1, 10],
     b ~ U[0,1) L1-normalised, tolerance 1e-10 relative, FGMRES(50) with an FP32 sector
@@ -176,6 +176,7 @@ def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0):
     arrivals start (generator and two resident hierarchies each); requests are submitted at
     their arrival times to a server with `workers` worker streams."""
     from markovkit_amd.fsp import box_generator, toggle_switch
+    smoother = getattr(mk, SMOOTHERS[smoother_name])
     rng = np.random.default_rng(seed)
     kinds = rng.random(n_requests) < 0.7
     voxels = rng.choice([16, 32, 48, 64], n_requests)
@@ -188,7 +189,8 @@ def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0):
         w = workloads.birth_death_1d(v, c, 1.0, "random")
         box = mk.PopulationBox(w.n_voxels, w.max_totals)
         Q = mk.Generator(box, w.model)
-        mgs = [mk.SectorMultigrid(w.model.domain, box, 1.0, mk.MultigridOptions(precision=mk.FP32))
+        mgs = [mk.SectorMultigrid(w.model.domain, box, 1.0,
+                                  mk.MultigridOptions(precision=mk.FP32, smoother=smoother))
                for _ in range(2)]
         fams[(v, c)] = (Q, mgs)
     J = box_generator(toggle_switch(), [200, 200], absorbing=True)
@@ -241,7 +243,8 @@ def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0):
     return {
         "workload": (f"{n_requests} requests, Poisson {rate:.0f}/s (seed {seed}): 70% 1D RDME "
                      f"resolvents (voxels 16-64, caps 3-5, s ~ U[0.1,10], 1e-10 relative), 30% "
-                     f"toggle-switch FSP expv (40,401 states, t = 5, 1e-8); {workers} worker streams"),
+                     f"toggle-switch FSP expv (40,401 states, t = 5, 1e-8); {workers} worker streams; "
+                     f"{smoother_name} smoother"),
         "value": n_requests / total,
         "unit": "requests/s",
         "offered_rate": rate,
@@ -268,10 +271,13 @@ def main():
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--workers", type=int, default=4, help="serving: worker streams")
     ap.add_argument("--restart", type=int, default=50)
-    ap.add_argument("--smoother", default="jacobi", choices=list(SMOOTHERS),
-                    help="sector multigrid smoother of the timed solves")
+    ap.add_argument("--smoother", default="red_black", choices=list(SMOOTHERS),
+                    help="sector multigrid smoother of the timed and served solves (red_black: "
+                         "multicolour Gauss-Seidel; jacobi: the reference's default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--no-jacobi", action="store_true",
+                    help="skip the comparison solve with the reference's Jacobi smoother")
     ap.add_argument("--no-multishift", action="store_true",
                     help="skip the batched multi-shift measurement after the headline")
     args = ap.parse_args()
@@ -284,7 +290,7 @@ def main():
     from markovkit_amd import workloads
 
     if args.workload == "serving":
-        sv = serving(mk, workloads, args.requests, args.workers)
+        sv = serving(mk, workloads, args.requests, args.workers, smoother_name=args.smoother)
         out = {"metric": "served RDME resolvent + FSP requests/s with p50/p99 latency",
                "value": sv["value"], "unit": "requests/s", "n_gpus": 1, "steps": args.requests,
                "warmup": 1, "ms_per_step": 1e3 * sv["makespan_s"] / args.requests,
@@ -442,7 +448,7 @@ def main():
         out["multishift"] = multishift(mk, workloads, 3, args.restart, sequential_sample=8)
         log(f"[multishift] {out['multishift']['value']:.1f} solves/s")
     ref_iters = iters[-1]
-    if args.smoother != "jacobi":
+    if args.smoother != "jacobi" and not args.no_jacobi:
         # the reference's own method (weighted Jacobi V-cycle) on the GPU, for comparison and
         # for the CPU baseline's iteration count
         del M32

@@ -554,8 +554,11 @@ def fused_tail_enable(on: bool = True) -> None:
     check(lib().mk_fused_tail_enable(1 if on else 0))
 
 
-KERNEL_CLASSES = ("shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict",
-                  "prolong", "terminal", "gmres_dot", "gmres_update", "gmres_other")
+# the multigrid classes before gmres_dot count the finest level's launches; coarse_levels
+# counts every launch below it
+KERNEL_CLASSES = ("shifted_spmv", "smooth_first", "smooth_sweep", "residual", "restrict",
+                  "prolong", "terminal", "gmres_dot", "gmres_update", "gmres_other",
+                  "coarse_levels")
 
 
 def profile_enable(on=True):

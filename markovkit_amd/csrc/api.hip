@@ -360,19 +360,20 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
 // aggregate, the residual kept in lv.res and corrected where each aggregate's update reaches;
 // rows of sectors terminal on this level get Jacobi sweeps instead
 template <typename T>
-void block_smooth(const mk_sector_mg_s &mg, level_dev<T> &lv, const T *rhs, T *x, T *tmp,
-                  int sweeps, bool reverse, bool from_zero) {
+void block_smooth(const mk_sector_mg_s &mg, level_dev<T> &lv, std::size_t level, const T *rhs, T *x,
+                  T *tmp, int sweeps, bool reverse, bool from_zero) {
     if (sweeps <= 0) return;
     auto &P = prof();
+    auto cls = [&](kclass c) { return level == 0 ? c : K_COARSE; };
     const hipStream_t s = stream();
     const double shift = mg.shift, omega = mg.opt.jacobi_weight;
     const std::size_t ts = sizeof(T);
     T *r = dp<T>(lv.res);
     if (from_zero) {
-        P.run(K_OTHER, double(lv.n) * 2 * ts,
+        P.run(cls(K_OTHER), double(lv.n) * 2 * ts,
               [&] { HIPCHK(hipMemcpyAsync(r, rhs, lv.n * ts, hipMemcpyDeviceToDevice, s)); });
     } else {
-        P.run(K_RESIDUAL, op_bytes(lv.A, ts, ts, 2),
+        P.run(cls(K_RESIDUAL), op_bytes(lv.A, ts, ts, 2),
               [&] { gpu::level_residual<T>(lv.A.view, shift, rhs, x, r, s); });
     }
     const u32 *mem = dp<u32>(lv.bs_members), *rows = dp<u32>(lv.bs_rows);
@@ -382,17 +383,17 @@ void block_smooth(const mk_sector_mg_s &mg, level_dev<T> &lv, const T *rhs, T *x
             const i64 a = reverse ? lv.n_agg - 1 - step : step;
             const u64 m0 = lv.h_agg_members[a], m1 = lv.h_agg_members[a + 1];
             const u64 q0 = lv.h_agg_rows[a], q1 = lv.h_agg_rows[a + 1];
-            P.run(K_SWEEP, double(m1 - m0) * (3 * ts + 12), [&] {
+            P.run(cls(K_SWEEP), double(m1 - m0) * (3 * ts + 12), [&] {
                 gpu::block_solve<T>(mem, dp<u32>(lv.bs_member_block), dp<u32>(lv.bs_block_first),
                                     dp<u64>(lv.bs_inv_off), dp<T>(lv.bs_inv), m0, m1, r, x, escr, s);
             });
-            P.run(K_SWEEP, double(q1 - q0) * (2 * ts + 4), [&] {
+            P.run(cls(K_SWEEP), double(q1 - q0) * (2 * ts + 4), [&] {
                 gpu::block_residual<T>(lv.A.view, shift, rows, q0, q1, escr, r, s);
                 gpu::block_zero<T>(mem, m0, m1, escr, s);
             });
         }
         if (lv.n_jrows > 0) {
-            P.run(K_SWEEP, double(lv.n_jrows) * 4 * ts, [&] {
+            P.run(cls(K_SWEEP), double(lv.n_jrows) * 4 * ts, [&] {
                 gpu::sweep_rows<T>(lv.A.view, shift, omega, rhs, x, tmp, dp<u32>(lv.jrows),
                                    static_cast<u64>(lv.n_jrows), s);
                 gpu::copy_rows<T>(dp<u32>(lv.jrows), static_cast<u64>(lv.n_jrows), tmp, x, s);
@@ -410,12 +411,13 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
              const double *rhs64, T *out, const double *scale = nullptr) {
     auto &lv = L[l];
     auto &P = prof();
+    auto cls = [&](kclass c) { return l == 0 ? c : K_COARSE; };
     const hipStream_t s = stream();
     const double shift = mg.shift, omega = mg.opt.jacobi_weight;
     if (mg.tail_start >= 0 && static_cast<i64>(l) >= mg.tail_start && tail_switch().load() != 0) {
         // this level and every coarser one in one workgroup
         const auto *desc = dp<gpu::level_desc<T>>(mg.tail_desc) + l;
-        P.run(K_SWEEP, 0.0, [&] {
+        P.run(cls(K_SWEEP), 0.0, [&] {
             gpu::vcycle_tail<T>(desc, static_cast<int>(L.size() - l), shift, omega, rhs, rhs64,
                                 scale, out, s);
         });
@@ -437,12 +439,12 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
     // one ILU(0) sweep: x += (L U)^-1 r, with u in `other`
     auto ilu_sweep = [&](const T *r) {
         const double fbytes = op_bytes(lv.A, ts, ts, 2) / 2.0 + double(n) * (4 + ts);
-        P.run(K_SWEEP, fbytes, [&] {
+        P.run(cls(K_SWEEP), fbytes, [&] {
             for (const auto &g : lv.fw_launch)
                 gpu::ilu_forward<T>(lv.A.view, dp<u32>(lv.fw_rows), dp<u32>(lv.fw_ptr), g.g0, g.g1,
                                     g.wide_rows, dp<T>(lv.pinv), r, other, s);
         });
-        P.run(K_SWEEP, fbytes + double(n) * 2 * ts, [&] {
+        P.run(cls(K_SWEEP), fbytes + double(n) * 2 * ts, [&] {
             for (const auto &g : lv.bw_launch)
                 gpu::ilu_backward<T>(lv.A.view, dp<u32>(lv.bw_rows), dp<u32>(lv.bw_ptr), g.g0, g.g1,
                                      g.wide_rows, dp<T>(lv.pinv), other, cur, s);
@@ -454,7 +456,7 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
         // colour-1 half-sweep only colour-0 rows have a residual
         if (lv.pre > 0) {
             T *rb = rhs64 || perm0 ? dp<T>(lv.rhs) : nullptr;
-            P.run(K_FIRST, double(n) * ((rhs64 ? 8.0 + ts : double(ts)) + 2.0 + ts / 2.0 + (rb ? ts : 0)), [&] {
+            P.run(cls(K_FIRST), double(n) * ((rhs64 ? 8.0 + ts : double(ts)) + 2.0 + ts / 2.0 + (rb ? ts : 0)), [&] {
                 if (rhs64) {
                     gpu::rb_first<T, double>(lv.A.view, dp<std::uint8_t>(lv.colour), shift, rhs64, rb, cur, s,
                                              scale, perm0);
@@ -467,16 +469,16 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             for (int half = 1; half < 2 * lv.pre; ++half) {
                 const int c = half & 1;
                 T *rz = half == 2 * lv.pre - 1 ? res : nullptr;
-                P.run(K_SWEEP, half_bytes(c), [&] { gpu::gs_half<T>(lv.Ac[c].view, shift, rhs, cur, rz, s); });
+                P.run(cls(K_SWEEP), half_bytes(c), [&] { gpu::gs_half<T>(lv.Ac[c].view, shift, rhs, cur, rz, s); });
             }
-            P.run(K_RESIDUAL, half_bytes(0), [&] {
+            P.run(cls(K_RESIDUAL), half_bytes(0), [&] {
                 gpu::level_residual<T>(lv.Ac[0].view, shift, rhs, cur, res, s);
             });
             have_residual = true;
         } else {
             if (rhs64 || perm0) {
                 T *rb = dp<T>(lv.rhs);
-                P.run(K_OTHER, double(n) * (8 + ts), [&] {
+                P.run(cls(K_OTHER), double(n) * (8 + ts), [&] {
                     if (perm0 && rhs64) {
                         gpu::permute_in<T, double>(n, perm0, rhs64, rb, s, scale);
                     } else if (perm0) {
@@ -487,60 +489,60 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
                 });
                 rhs = rb;
             }
-            P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
+            P.run(cls(K_OTHER), double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
         }
     } else if (ilu) {
         if (rhs64) {
             T *rb = dp<T>(lv.rhs);
-            P.run(K_OTHER, double(n) * (8 + ts), [&] { gpu::convert<double, T>(n, rhs64, rb, s, scale); });
+            P.run(cls(K_OTHER), double(n) * (8 + ts), [&] { gpu::convert<double, T>(n, rhs64, rb, s, scale); });
             rhs = rb;
         }
-        P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
+        P.run(cls(K_OTHER), double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
         for (int k = 0; k < lv.pre; ++k) {
             if (k > 0) {
-                P.run(K_RESIDUAL, sweep_bytes, [&] { gpu::level_residual<T>(lv.A.view, shift, rhs, cur, res, s); });
+                P.run(cls(K_RESIDUAL), sweep_bytes, [&] { gpu::level_residual<T>(lv.A.view, shift, rhs, cur, res, s); });
             }
             ilu_sweep(k == 0 ? rhs : res); // from x = 0 the residual is rhs
         }
     } else if (blocks) {
         if (rhs64) {
             T *rb = dp<T>(lv.rhs);
-            P.run(K_OTHER, double(n) * (8 + ts),
+            P.run(cls(K_OTHER), double(n) * (8 + ts),
                   [&] { gpu::convert<double, T>(n, rhs64, rb, s, scale); });
             rhs = rb;
         }
-        P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
-        block_smooth<T>(mg, lv, rhs, cur, other, lv.pre, false, true);
+        P.run(cls(K_OTHER), double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
+        block_smooth<T>(mg, lv, l, rhs, cur, other, lv.pre, false, true);
     } else if (rhs64) {
         T *rb = dp<T>(lv.rhs);
         if (lv.pre > 0) {
-            P.run(K_FIRST, double(n) * (8 + 3 * ts), [&] {
+            P.run(cls(K_FIRST), double(n) * (8 + 3 * ts), [&] {
                 gpu::jacobi_first<T, double>(lv.A.view, shift, omega, rhs64, rb, cur, s, scale);
             });
         } else {
-            P.run(K_OTHER, double(n) * (8 + ts),
+            P.run(cls(K_OTHER), double(n) * (8 + ts),
                   [&] { gpu::convert<double, T>(n, rhs64, rb, s, scale); });
         }
         rhs = rb;
     } else if (lv.pre > 0) {
-        P.run(K_FIRST, double(n) * 3 * ts, [&] {
+        P.run(cls(K_FIRST), double(n) * 3 * ts, [&] {
             gpu::jacobi_first<T, T>(lv.A.view, shift, omega, rhs, nullptr, cur, s);
         });
     }
-    if (jacobi && lv.pre == 0) P.run(K_OTHER, double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
+    if (jacobi && lv.pre == 0) P.run(cls(K_OTHER), double(n) * ts, [&] { gpu::fill<T>(n, T(0), cur, s); });
     for (int k = 1; jacobi && k < lv.pre; ++k) {
-        P.run(K_SWEEP, sweep_bytes, [&] {
+        P.run(cls(K_SWEEP), sweep_bytes, [&] {
             gpu::jacobi_sweep<T>(lv.A.view, shift, omega, rhs, cur, other, s);
         });
         std::swap(cur, other);
     }
     if (!have_residual) {
-        P.run(K_RESIDUAL, sweep_bytes, [&] {
+        P.run(cls(K_RESIDUAL), sweep_bytes, [&] {
             gpu::level_residual<T>(lv.A.view, shift, rhs, cur, res, s);
         });
     }
     if (lv.n_term > 0) {
-        P.run(K_TERMINAL, 0.0, [&] {
+        P.run(cls(K_TERMINAL), 0.0, [&] {
             gpu::terminal_solve<T>(lv.n_term, dp<u32>(lv.tstart), dp<u32>(lv.tsize),
                                    dp<u64>(lv.toff), dp<T>(lv.tinv), lv.term_max, res, cur, s);
         });
@@ -551,53 +553,53 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
         const double rbytes = op_bytes(lv.R, 0, ts, 1) + double(lv.n_fine_linked) * ts;
         const double pbytes = double(lv.n) * (4 + 3 * ts) + double(c.n) * ts;
         if (mg.opt.coarse_cycles <= 1) {
-            P.run(K_RESTRICT, rbytes, [&] {
+            P.run(cls(K_RESTRICT), rbytes, [&] {
                 gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.rhs), s);
             });
             T *cx = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr);
-            P.run(K_PROLONG, pbytes, [&] {
+            P.run(cls(K_PROLONG), pbytes, [&] {
                 gpu::prolong_add<T>(n, dp<u32>(lv.coarse_of), dp<T>(lv.weight), cx, cur, s);
             });
         } else {
-            P.run(K_RESTRICT, rbytes, [&] {
+            P.run(cls(K_RESTRICT), rbytes, [&] {
                 gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.crhs), s);
             });
-            P.run(K_OTHER, double(c.n) * ts, [&] { gpu::fill<T>(c.n, T(0), dp<T>(c.cx), s); });
+            P.run(cls(K_OTHER), double(c.n) * ts, [&] { gpu::fill<T>(c.n, T(0), dp<T>(c.cx), s); });
             const double cbytes = op_bytes(c.A, ts, ts, 2);
             for (int cyc = 0; cyc < mg.opt.coarse_cycles; ++cyc) {
-                P.run(K_RESIDUAL, cbytes, [&] {
+                P.run(cls(K_RESIDUAL), cbytes, [&] {
                     gpu::level_residual<T>(c.A.view, shift, dp<T>(c.crhs), dp<T>(c.cx),
                                            dp<T>(c.rhs), s);
                 });
                 T *ce = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr);
-                P.run(K_OTHER, double(c.n) * 3 * ts, [&] { gpu::axpy<T>(c.n, T(1), ce, dp<T>(c.cx), s); });
+                P.run(cls(K_OTHER), double(c.n) * 3 * ts, [&] { gpu::axpy<T>(c.n, T(1), ce, dp<T>(c.cx), s); });
             }
-            P.run(K_PROLONG, pbytes, [&] {
+            P.run(cls(K_PROLONG), pbytes, [&] {
                 gpu::prolong_add<T>(n, dp<u32>(lv.coarse_of), dp<T>(lv.weight), dp<T>(c.cx),
                                     cur, s);
             });
         }
     }
-    if (blocks) block_smooth<T>(mg, lv, rhs, cur, other, lv.post, true, false);
+    if (blocks) block_smooth<T>(mg, lv, l, rhs, cur, other, lv.post, true, false);
     for (int half = 0; rbgs && half < 2 * lv.post; ++half) {
         const int c = 1 - (half & 1); // reverse colour order
         // the last two half-sweeps also store their rows to the output in the caller's order
         T *nat = perm0 && out && half >= 2 * lv.post - 2 ? out : nullptr;
-        P.run(K_SWEEP, half_bytes(c), [&] {
+        P.run(cls(K_SWEEP), half_bytes(c), [&] {
             gpu::gs_half<T>(lv.Ac[c].view, shift, rhs, cur, nullptr, s, nat, perm0);
         });
     }
     if (perm0 && out) {
-        if (lv.post == 0) P.run(K_OTHER, double(n) * (2 * ts + 4), [&] { gpu::permute_out<T>(n, perm0, cur, out, s); });
+        if (lv.post == 0) P.run(cls(K_OTHER), double(n) * (2 * ts + 4), [&] { gpu::permute_out<T>(n, perm0, cur, out, s); });
         return out;
     }
     for (int k = 0; ilu && k < lv.post; ++k) {
-        P.run(K_RESIDUAL, sweep_bytes, [&] { gpu::level_residual<T>(lv.A.view, shift, rhs, cur, res, s); });
+        P.run(cls(K_RESIDUAL), sweep_bytes, [&] { gpu::level_residual<T>(lv.A.view, shift, rhs, cur, res, s); });
         ilu_sweep(res);
     }
     for (int k = 0; jacobi && k < lv.post; ++k) {
         T *dst = (k == lv.post - 1 && out) ? out : other;
-        P.run(K_SWEEP, sweep_bytes, [&] {
+        P.run(cls(K_SWEEP), sweep_bytes, [&] {
             gpu::jacobi_sweep<T>(lv.A.view, shift, omega, rhs, cur, dst, s);
         });
         if (dst == other) {
@@ -607,7 +609,7 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
         }
     }
     if (out && cur != out) {
-        P.run(K_OTHER, double(n) * 2 * ts,
+        P.run(cls(K_OTHER), double(n) * 2 * ts,
               [&] { HIPCHK(hipMemcpyAsync(out, cur, n * ts, hipMemcpyDeviceToDevice, s)); });
         cur = out;
     }

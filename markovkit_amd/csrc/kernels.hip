@@ -4,6 +4,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace mk::gpu {
 
@@ -135,18 +136,41 @@ __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) 
     }
     __syncthreads();
     T sum = 0;
-    for (u32 k0 = 0; k0 < len; k0 += kSliceUnroll) {
-        u32 e[kSliceUnroll];
+    if constexpr (std::is_same_v<XT, T>) {
+        for (u32 k0 = 0; k0 < len; k0 += kSliceUnroll) {
+            u32 e[kSliceUnroll];
 #pragma unroll
-        for (int u = 0; u < kSliceUnroll; ++u)
-            e[u] = k0 + u < len ? __builtin_nontemporal_load(a.A.ent + base + (k0 + u) * kSlice) : 0u;
-        XT xv[kSliceUnroll];
+            for (int u = 0; u < kSliceUnroll; ++u)
+                e[u] = k0 + u < len ? __builtin_nontemporal_load(a.A.ent + base + (k0 + u) * kSlice) : 0u;
+            XT xv[kSliceUnroll];
 #pragma unroll
-        for (int u = 0; u < kSliceUnroll; ++u)
-            xv[u] = k0 + u < len ? a.x[e[u] & kColMask] : XT(0);
+            for (int u = 0; u < kSliceUnroll; ++u)
+                xv[u] = k0 + u < len ? a.x[e[u] & kColMask] : XT(0);
 #pragma unroll
-        for (int u = 0; u < kSliceUnroll; ++u)
-            if (k0 + u < len) sum += vdict[e[u] >> kColBits] * static_cast<T>(xv[u]);
+            for (int u = 0; u < kSliceUnroll; ++u)
+                if (k0 + u < len) sum += vdict[e[u] >> kColBits] * static_cast<T>(xv[u]);
+        }
+    } else {
+        // x in another precision: with guarded loads the compiler converts each gathered value
+        // inside its guard and waits for it there, one gather at a time. Branch-free instead:
+        // slots past the row's end re-read its last entry (in bounds, same cache line) and get a
+        // zero coefficient, so a chunk's gathers are in flight together; fma(0, x, sum) == sum.
+        for (u32 k0 = 0; k0 < len; k0 += kSliceUnroll) {
+            u32 e[kSliceUnroll];
+#pragma unroll
+            for (int u = 0; u < kSliceUnroll; ++u) {
+                const u32 k = min(k0 + u, len - 1);
+                e[u] = __builtin_nontemporal_load(a.A.ent + base + k * kSlice);
+            }
+            XT xv[kSliceUnroll];
+#pragma unroll
+            for (int u = 0; u < kSliceUnroll; ++u) xv[u] = a.x[e[u] & kColMask];
+#pragma unroll
+            for (int u = 0; u < kSliceUnroll; ++u) {
+                const T c = k0 + u < len ? vdict[e[u] >> kColBits] : T(0);
+                sum += c * static_cast<T>(xv[u]);
+            }
+        }
     }
     double sq = 0.0;
     if (has_row) sq = finish<M>(a, vec_row(a.A, static_cast<u32>(r)), sum, edict[ec]);

@@ -143,12 +143,14 @@ enum kclass {
     K_DOT,
     K_UPDATE,
     K_OTHER,
+    K_COARSE, // every multigrid launch below the finest level (the other multigrid classes are
+              // the finest level's)
     K_COUNT
 };
-const char *const kclass_names[K_COUNT] = {"shifted_spmv", "jacobi_first", "jacobi_sweep",
+const char *const kclass_names[K_COUNT] = {"shifted_spmv", "smooth_first", "smooth_sweep",
                                            "residual",     "restrict",     "prolong",
                                            "terminal",     "gmres_dot",    "gmres_update",
-                                           "gmres_other"};
+                                           "gmres_other",  "coarse_levels"};
 
 struct profiler {
     std::uint32_t mask = 0;

@@ -14,21 +14,22 @@ from markovkit_amd.workloads import birth_death_1d
 pytestmark = pytest.mark.gpu
 
 
-def family(voxels, cap, pool=2):
+def family(voxels, cap, smoother, pool=2):
     w = birth_death_1d(voxels, cap, 0.5, "random")
     box = mk.PopulationBox(w.n_voxels, w.max_totals)
     Q = mk.Generator(box, w.model)
-    mgs = [mk.SectorMultigrid(w.model.domain, box, 1.0, mk.MultigridOptions(precision=mk.FP32))
+    mgs = [mk.SectorMultigrid(w.model.domain, box, 1.0,
+                              mk.MultigridOptions(precision=mk.FP32, smoother=smoother))
            for _ in range(pool)]
     return w, box, Q, mgs
 
 
-@pytest.fixture(scope="module")
-def setup():
-    fams = [family(16, 3), family(32, 3)]
+@pytest.fixture(scope="module", params=[mk.JACOBI, mk.RED_BLACK], ids=["jacobi", "red_black"])
+def setup(request):
+    fams = [family(16, 3, request.param), family(32, 3, request.param)]
     J = box_generator(toggle_switch(), [60, 60], absorbing=True)
     F = mk.Generator.from_jumps(*J)
-    return fams, F
+    return fams, F, request.param
 
 
 def requests(fams, F):
@@ -47,13 +48,14 @@ def requests(fams, F):
     return out
 
 
-def alone(fams, F, req):
+def alone(fams, F, req, smoother):
     kind, f, p, vec = req
     if kind == "expv":
         w, _ = mk.expv(F, p, vec, tolerance=1e-8)
         return w
     w_, box, Q, _ = fams[f]
-    M = mk.SectorMultigrid(w_.model.domain, box, p, mk.MultigridOptions(precision=mk.FP32))
+    M = mk.SectorMultigrid(w_.model.domain, box, p,
+                           mk.MultigridOptions(precision=mk.FP32, smoother=smoother))
     x = np.zeros(Q.rows())
     res = mk.gmres(mk.ShiftedGenerator(Q, p), M, vec, x, tolerance=1e-10 * np.linalg.norm(vec),
                    restart=50)
@@ -63,7 +65,7 @@ def alone(fams, F, req):
 
 @pytest.mark.parametrize("workers,budget", [(3, 64 << 30), (2, 1)], ids=["concurrent", "serialised"])
 def test_results_alone_and_in_the_mix(setup, workers, budget):
-    fams, F = setup
+    fams, F, smoother = setup
     reqs = requests(fams, F)
     srv = mk.Server(workers=workers, hbm_budget=budget)
     ids = [srv.add_family(Q, mgs) for (_, _, Q, mgs) in fams]
@@ -77,7 +79,7 @@ def test_results_alone_and_in_the_mix(setup, workers, budget):
     served = [srv.wait(r) for r in rids]
     srv.close()
     for req, (x, info) in zip(reqs, served):
-        ref = alone(fams, F, req)
+        ref = alone(fams, F, req, smoother)
         assert info.kind == req[0]
         if info.kind == "resolvent":
             assert info.solve.converged
@@ -87,7 +89,7 @@ def test_results_alone_and_in_the_mix(setup, workers, budget):
 
 
 def test_unknown_family_is_rejected(setup):
-    fams, F = setup
+    fams, F, _ = setup
     srv = mk.Server(workers=1)
     srv.add_family(fams[0][2], fams[0][3])
     with pytest.raises(Exception):

@@ -270,6 +270,11 @@ mk_status mk_graphs_enable(int32_t on);
  * kernel instead of one launch per operation (default on; MK_MG_TAIL=0 turns it off at start).
  * Same formulas and summation orders as the per-level kernels. */
 mk_status mk_fused_tail_enable(int32_t on);
+/* Where mk_generator_create_rdme assembles the generator (no reference counterpart): on the
+ * device (default; MK_DEVICE_SETUP=0 turns it off), one thread per state, or on the host. The
+ * operator, exit and leak rates are the same bit for bit. Boxes over 32 molecules per state,
+ * 8 species or 65,535 voxels always assemble on the host. */
+mk_status mk_device_setup_enable(int32_t on);
 /* per-kernel-class timing with HIP events on the library stream. Classes, in bit order:
  * "shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict", "prolong",
  * "terminal", "gmres_dot", "gmres_update", "gmres_other". mask: 0 = off, -1 = all, else the

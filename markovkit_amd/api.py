@@ -549,6 +549,12 @@ def graphs_enable(on: bool = True) -> None:
     check(lib().mk_graphs_enable(1 if on else 0))
 
 
+def device_setup_enable(on: bool = True) -> None:
+    """Assemble RDME generators on the device (default) or on the host; both give the same
+    operator bit for bit."""
+    check(lib().mk_device_setup_enable(1 if on else 0))
+
+
 def fused_tail_enable(on: bool = True) -> None:
     """Run the small multigrid levels in one workgroup-wide kernel (default on)."""
     check(lib().mk_fused_tail_enable(1 if on else 0))

@@ -25,7 +25,8 @@ HIP_FLAGS = ["-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}", "-pthread",
 CXX_FLAGS = ["-O3", "-std=c++20", "-fPIC", "-march=x86-64-v3", "-pthread", "-Wall", "-Wextra",
              f"-I{ROOT}/include"]
 
-HIP_SOURCES = ["kernels.hip", "batch_kernels.hip", "api.hip", "batch.hip", "expv.hip", "server.hip"]
+HIP_SOURCES = ["kernels.hip", "batch_kernels.hip", "api.hip", "batch.hip", "expv.hip", "server.hip",
+               "assemble.hip"]
 CXX_SOURCES = ["setup.cpp"]
 
 

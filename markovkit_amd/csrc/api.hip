@@ -963,6 +963,11 @@ mk_status mk_graphs_enable(int32_t on) {
     return MK_OK;
 }
 
+mk_status mk_device_setup_enable(int32_t on) {
+    device_setup_switch().store(on ? 1 : 0);
+    return MK_OK;
+}
+
 mk_status mk_fused_tail_enable(int32_t on) {
     tail_switch().store(on ? 1 : 0);
     return MK_OK;
@@ -1093,7 +1098,10 @@ mk_status mk_generator_create_rdme(mk_box box, mk_domain domain, int32_t n_react
             rt.rates.assign(rates, rates + static_cast<i64>(n_reactions) * space.voxels() * rt.T);
         }
         auto G = std::make_unique<mk_generator_s>();
-        finish_generator(*G, mk::assemble_generator(space, domain->dom, rt, boundary == MK_ABSORBING));
+        const bool absorbing = boundary == MK_ABSORBING;
+        if (!(device_setup_switch().load() != 0 &&
+              assemble_generator_device(space, domain->dom, rt, absorbing, *G)))
+            finish_generator(*G, mk::assemble_generator(space, domain->dom, rt, absorbing));
         *out = G.release();
     });
 }

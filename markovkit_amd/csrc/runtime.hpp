@@ -364,6 +364,16 @@ inline std::atomic<int> &graphs_switch() {
 constexpr i64 kGraphRows = 262144;
 
 // the fused coarse-level tail of the V-cycle (default on; MK_MG_TAIL=0 turns it off at start)
+// Device-side generator assembly of RDME boxes (default on; MK_DEVICE_SETUP=0 assembles on the
+// host). Both give the same operator, exit and leak arrays bit for bit.
+inline std::atomic<int> &device_setup_switch() {
+    static std::atomic<int> on{[] {
+        const char *env = std::getenv("MK_DEVICE_SETUP");
+        return (env && env[0] == '0') ? 0 : 1;
+    }()};
+    return on;
+}
+
 inline std::atomic<int> &tail_switch() {
     static std::atomic<int> on{[] {
         const char *env = std::getenv("MK_MG_TAIL");
@@ -487,6 +497,14 @@ struct mk_generator_s {
 };
 
 namespace mk::rt {
+
+// The generator of an RDME box assembled on the device (assemble.hip): the same gather-form
+// operator, exit and leak as mk::assemble_generator, encoded in place. Returns false, with G
+// untouched, for boxes outside its limits (more than kMaxMolecules molecules per state, more
+// than kMaxSpecies species, 65,536 voxels or more); the caller then assembles on the host.
+bool assemble_generator_device(const mk::population_space &space, const mk::voxel_domain &domain,
+                               const mk::reaction_tables &reactions, bool absorbing,
+                               mk_generator_s &G);
 
 template <typename T>
 struct level_dev {

@@ -32,6 +32,9 @@ class compositions_table {
     [[nodiscard]] u64 operator()(i64 rest, int rem) const {
         return table_[static_cast<std::size_t>(rest) * stride_ + static_cast<std::size_t>(rem)];
     }
+    // the table, row-major by rest with stride() entries per row (device setup uploads it)
+    [[nodiscard]] const std::vector<u64> &data() const { return table_; }
+    [[nodiscard]] std::size_t stride() const { return stride_; }
 
   private:
     std::vector<u64> table_;
@@ -56,6 +59,9 @@ class population_space {
     [[nodiscard]] i64 end(i64 k) const { return begin_[k + 1]; }
     [[nodiscard]] const std::vector<int> &caps() const { return caps_; }
     [[nodiscard]] bool is_box() const { return box_; }
+    [[nodiscard]] const compositions_table &compositions() const { return comp_; }
+    // per-species state counts of sector k (species 0 most significant in the sector rank)
+    [[nodiscard]] const std::vector<u64> &species_sizes(i64 k) const { return species_sizes_[k]; }
 
     // the sector holding these totals, or -1
     [[nodiscard]] i64 sector_of_totals(const int *totals) const;

@@ -22,8 +22,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}", "-pthread",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-result", f"-I{ROOT}/include"]
-CXX_FLAGS = ["-O3", "-std=c++20", "-fPIC", "-march=x86-64-v3", "-pthread", "-Wall", "-Wextra",
-             f"-I{ROOT}/include"]
+# no contraction of products into sums on the host: setup sums round as the reference's default
+# (FMA-less) build does, and as the device assembly (compiled with contraction off) does
+CXX_FLAGS = ["-O3", "-std=c++20", "-fPIC", "-march=x86-64-v3", "-ffp-contract=off", "-pthread", "-Wall",
+             "-Wextra", f"-I{ROOT}/include"]
 
 HIP_SOURCES = ["kernels.hip", "batch_kernels.hip", "api.hip", "batch.hip", "expv.hip", "server.hip",
                "assemble.hip"]

@@ -21,12 +21,21 @@ void split_colours(const mk::generator_csr &cat, const std::vector<std::uint8_t>
         for (i64 i = 0; i < cat.n; ++i) {
             if (colour[i] != c) continue;
             rows.push_back(static_cast<u32>(i));
-            sub.col.insert(sub.col.end(), cat.col.begin() + cat.ptr[i], cat.col.begin() + cat.ptr[i + 1]);
-            sub.val.insert(sub.val.end(), cat.val.begin() + cat.ptr[i], cat.val.begin() + cat.ptr[i + 1]);
-            sub.ptr.push_back(sub.col.size());
-            sub.exit.push_back(cat.exit[i]);
+            sub.ptr.push_back(sub.ptr.back() + (cat.ptr[i + 1] - cat.ptr[i]));
         }
         sub.n = static_cast<i64>(rows.size());
+        sub.col.resize(sub.ptr.back());
+        sub.val.resize(sub.ptr.back());
+        sub.exit.resize(sub.n);
+        const i64 chunk = 1 << 15;
+        mk::parallel_tasks((sub.n + chunk - 1) / chunk, [&](i64 t) {
+            for (i64 q = t * chunk; q < std::min(sub.n, (t + 1) * chunk); ++q) {
+                const i64 i = rows[q];
+                std::copy(cat.col.begin() + cat.ptr[i], cat.col.begin() + cat.ptr[i + 1], sub.col.begin() + sub.ptr[q]);
+                std::copy(cat.val.begin() + cat.ptr[i], cat.val.begin() + cat.ptr[i + 1], sub.val.begin() + sub.ptr[q]);
+                sub.exit[q] = cat.exit[i];
+            }
+        });
         lv.Ac[c] = upload_csr<T>(sub);
         lv.rowmap[c] = upload(rows);
         lv.Ac[c].view.rowmap = dp<u32>(lv.rowmap[c]);
@@ -163,21 +172,34 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
             const u32 base = static_cast<u32>(offset[l][k]);
             lv.pre = s.pre_sweeps;
             lv.post = s.post_sweeps;
-            std::vector<std::pair<u32, double>> row;
+            // this sector's rows, in the level's order: lengths first, then a parallel fill
+            const std::size_t r0 = cat.ptr.size() - 1;
             for (i64 i2 = 0; i2 < s.rows; ++i2) {
                 const i64 i = nat(l, k, i2);
-                row.clear();
-                for (u64 p = s.A.ptr[i]; p < s.A.ptr[i + 1]; ++p)
-                    row.emplace_back(static_cast<u32>(base + neu(l, k, s.A.col[p])), s.A.val[p]);
-                if (by_colour) std::sort(row.begin(), row.end()); // columns ascending
-                for (const auto &[c, v] : row) {
-                    cat.col.push_back(c);
-                    cat.val.push_back(v);
-                }
-                cat.ptr.push_back(cat.col.size());
-                cat.exit.push_back(s.A.exit[i]);
-                if (!s.colour.empty()) colour.push_back(s.colour[i]);
+                cat.ptr.push_back(cat.ptr.back() + (s.A.ptr[i + 1] - s.A.ptr[i]));
             }
+            cat.col.resize(cat.ptr.back());
+            cat.val.resize(cat.ptr.back());
+            cat.exit.resize(cat.ptr.size() - 1);
+            if (!s.colour.empty()) colour.resize(cat.ptr.size() - 1);
+            const i64 chunk = 1 << 15;
+            mk::parallel_tasks((s.rows + chunk - 1) / chunk, [&](i64 t) {
+                std::vector<std::pair<u32, double>> row;
+                for (i64 i2 = t * chunk; i2 < std::min<i64>(s.rows, (t + 1) * chunk); ++i2) {
+                    const i64 i = nat(l, k, i2);
+                    row.clear();
+                    for (u64 p = s.A.ptr[i]; p < s.A.ptr[i + 1]; ++p)
+                        row.emplace_back(static_cast<u32>(base + neu(l, k, s.A.col[p])), s.A.val[p]);
+                    if (by_colour) std::sort(row.begin(), row.end()); // columns ascending
+                    u64 q = cat.ptr[r0 + i2];
+                    for (const auto &[c, v] : row) {
+                        cat.col[q] = c;
+                        cat.val[q++] = v;
+                    }
+                    cat.exit[r0 + i2] = s.A.exit[i];
+                    if (!s.colour.empty()) colour[r0 + i2] = s.colour[i];
+                }
+            });
             if (s.terminal) {
                 const i64 m = s.rows;
                 tstart.push_back(base);
@@ -1203,9 +1225,22 @@ mk_status mk_sector_mg_create(mk_domain domain, mk_box box, double shift,
         mg->precision = o.precision;
         mg->rows = space.size();
         mg->n_sectors = space.n_sectors();
+        // level operators and transfers of at least 4096 states are built on the device (the
+        // same arrays)
+        mk::level_hooks on_device;
+        if (device_setup_switch().load() != 0) {
+            on_device.assemble = [](const mk::population_space &sp, const mk::voxel_domain &d) {
+                const mk::reaction_tables none;
+                mk::generator_csr g;
+                if (sp.size() < 4096 || !assemble_csr_device(sp, d, none, true, g))
+                    g = mk::assemble_generator(sp, d, none, true);
+                return g;
+            };
+            on_device.transfer = transfer_device;
+        }
         std::vector<mk::sector_hierarchy> H;
         for (i64 k = 0; k < space.n_sectors(); ++k)
-            H.push_back(mk::build_sector_hierarchy(domain->dom, space.totals(k), shift, mg->opt));
+            H.push_back(mk::build_sector_hierarchy(domain->dom, space.totals(k), shift, mg->opt, on_device));
         if (o.precision == MK_FP32) {
             build_levels<float>(*mg, H, mg->L32);
         } else {

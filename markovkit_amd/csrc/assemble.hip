@@ -412,13 +412,26 @@ std::vector<double> table_values(const std::vector<u64> &table) {
 
 } // namespace
 
-bool assemble_generator_device(const mk::population_space &space, const mk::voxel_domain &domain,
-                               const mk::reaction_tables &reactions, bool absorbing,
-                               mk_generator_s &G) {
+namespace {
+
+// the merged rows of a generator, on the device: row i holds len[i] entries at raw_ptr[i]
+struct device_rows {
+    i64 n = 0;
+    u64 raw_total = 0;
+    std::vector<u64> raw_ptr_h;
+    dbuf raw_ptr, col, val, len, exit, leak, jumps, norm;
+};
+
+// passes 1 and 2 of the assembly; false for spaces outside the kernel's limits (a sector
+// space, as the multigrid levels use, is allowed without reactions)
+bool assemble_rows(const mk::population_space &space, const mk::voxel_domain &domain,
+                   const mk::reaction_tables &reactions, bool absorbing, device_rows &out) {
     const int S = space.species();
     const i64 V = space.voxels();
     const i64 n = space.size();
-    if (!space.is_box() || S > kMaxSpecies || V >= 65536 || n >= (i64{1} << 32) || n == 0) return false;
+    if ((!space.is_box() && reactions.n > 0) || S > kMaxSpecies || V >= 65536 || n >= (i64{1} << 32) ||
+        n == 0)
+        return false;
     int total_cap = 0;
     for (int c : space.caps()) total_cap += c;
     if (total_cap > kMaxMolecules) return false;
@@ -505,6 +518,231 @@ bool assemble_generator_device(const mk::population_space &space, const mk::voxe
                        nullptr, dp<u32>(d_col), dp<double>(d_val), dp<u32>(d_len), dp<double>(d_exit),
                        dp<double>(d_leak), dp<u32>(d_jumps), dp<u64>(d_norm));
     HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    out.n = n;
+    out.raw_total = raw_total;
+    out.raw_ptr_h = std::move(raw_ptr);
+    out.raw_ptr = std::move(d_raw_ptr);
+    out.col = std::move(d_col);
+    out.val = std::move(d_val);
+    out.len = std::move(d_len);
+    out.exit = std::move(d_exit);
+    out.leak = std::move(d_leak);
+    out.jumps = std::move(d_jumps);
+    out.norm = std::move(d_norm);
+    return true;
+}
+
+__global__ void __launch_bounds__(kThreadsA)
+    k_compact(i64 n, const u64 *raw_ptr, const u64 *ptr, const u32 *col, const double *val,
+              u32 *col2, double *val2) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreadsA) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kThreadsA) {
+        const u64 a = raw_ptr[i], b = ptr[i], l = ptr[i + 1] - ptr[i];
+        for (u64 p = 0; p < l; ++p) {
+            col2[b + p] = col[a + p];
+            val2[b + p] = val[a + p];
+        }
+    }
+}
+
+} // namespace
+
+namespace {
+
+// population_transfer of one sector level (multigrid.hpp:50-92), per fine state: the rank of its
+// aggregate (counts summed over each coarse voxel's children) in the coarse sector, the log of
+// its unnormalised multinomial weight, summed over occupied voxels in ascending order as the
+// host does, and its red-black colour
+struct transfer_view {
+    const u64 *ccomp = nullptr; // coarse compositions table
+    u32 ccstride = 0;
+    int Vc = 0;
+    const u64 *csizes = nullptr; // coarse sector's per-species sizes
+    const int *parent = nullptr; // V
+    const u8 *parity = nullptr;  // V: coordinate-sum parity
+    const double *log_fact = nullptr, *log_children = nullptr;
+};
+
+__device__ u64 rank_coarse(const transfer_view &t, const unsigned short *p, int tot) {
+    u64 local = 0;
+    int rem = tot;
+    for (int q = 0; q < tot;) {
+        const int v = p[q];
+        int cnt = 1;
+        while (q + cnt < tot && p[q + cnt] == v) ++cnt;
+        if (v + 1 < t.Vc) {
+            const i64 rest = t.Vc - v - 1;
+            local += t.ccomp[static_cast<u64>(rest) * t.ccstride + rem] -
+                     t.ccomp[static_cast<u64>(rest) * t.ccstride + (rem - cnt)];
+        }
+        rem -= cnt;
+        q += cnt;
+    }
+    return local;
+}
+
+// sum of term(v, s, n) over the occupied (v, s) of per-species sorted position lists, v
+// ascending, then s ascending
+template <typename F>
+__device__ void visit_occupied(const unsigned short *pos, const int *off, int S, F &&term) {
+    int head[kMaxSpecies];
+    for (int s = 0; s < S; ++s) head[s] = off[s];
+    while (true) {
+        int best = -1, bv = 0;
+        for (int s = 0; s < S; ++s)
+            if (head[s] < off[s + 1] && (best < 0 || pos[head[s]] < bv)) {
+                best = s;
+                bv = pos[head[s]];
+            }
+        if (best < 0) return;
+        int n = 0;
+        while (head[best] < off[best + 1] && pos[head[best]] == bv) {
+            ++head[best];
+            ++n;
+        }
+        term(bv, best, n);
+    }
+}
+
+__global__ void __launch_bounds__(kThreadsA)
+    k_transfer(box_view b, transfer_view t, i64 *coarse_of, double *log_weight, u8 *colour) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreadsA) + threadIdx.x; i < b.n;
+         i += static_cast<i64>(gridDim.x) * kThreadsA) {
+        state st;
+        unrank(b, 0, static_cast<u64>(i), st);
+        double lw = 0.0;
+        int par = 0;
+        visit_occupied(st.pos, st.off, b.S, [&](int v, int, int n) {
+            lw -= t.log_fact[n];
+            par ^= (n & 1) & t.parity[v];
+        });
+        unsigned short cpos[kMaxMolecules];
+        for (int s = 0; s < b.S; ++s) { // coarse positions, sorted per species
+            int m = st.off[s];
+            for (int q = st.off[s]; q < st.off[s + 1]; ++q) {
+                const unsigned short a = static_cast<unsigned short>(t.parent[st.pos[q]]);
+                int r = m++;
+                while (r > st.off[s] && cpos[r - 1] > a) {
+                    cpos[r] = cpos[r - 1];
+                    --r;
+                }
+                cpos[r] = a;
+            }
+        }
+        visit_occupied(cpos, st.off, b.S, [&](int a, int, int n) {
+            lw += t.log_fact[n] - (n * t.log_children[a]);
+        });
+        u64 r = 0;
+        for (int s = 0; s < b.S; ++s)
+            r = r * t.csizes[s] + rank_coarse(t, cpos + st.off[s], st.tot[s]);
+        coarse_of[i] = static_cast<i64>(r);
+        log_weight[i] = lw;
+        if (colour) colour[i] = static_cast<u8>(par);
+    }
+}
+
+} // namespace
+
+bool transfer_device(const mk::population_space &fine, const mk::population_space &coarse,
+                     const mk::voxel_domain &dom, const std::vector<i64> &parent,
+                     const std::vector<double> &log_fact, const std::vector<double> &log_children,
+                     std::vector<i64> &coarse_of, std::vector<double> &log_weight,
+                     std::vector<std::uint8_t> *colour) {
+    const int S = fine.species();
+    const i64 V = fine.voxels(), n = fine.size();
+    int total = 0;
+    for (int c : fine.totals(0)) total += c;
+    if (fine.is_box() || fine.n_sectors() != 1 || S > kMaxSpecies || V >= 65536 || total > kMaxMolecules ||
+        n >= (i64{1} << 32) || n < 4096 || dom.dim == 0)
+        return false;
+    const hipStream_t s = stream();
+    std::vector<i64> sbegin{0, n};
+    std::vector<int> stot(fine.totals(0).begin(), fine.totals(0).end());
+    std::vector<u64> ssz(fine.species_sizes(0).begin(), fine.species_sizes(0).end());
+    std::vector<int> par32(parent.begin(), parent.end());
+    std::vector<u8> parity(V);
+    for (i64 v = 0; v < V; ++v) {
+        i64 c = 0;
+        for (i64 a : dom.coordinate(v)) c += a;
+        parity[v] = static_cast<u8>(c & 1);
+    }
+    dbuf d_comp = upload(fine.compositions().data()), d_sbegin = upload(sbegin), d_stot = upload(stot),
+         d_ssz = upload(ssz), d_ccomp = upload(coarse.compositions().data()),
+         d_csz = upload(coarse.species_sizes(0)), d_parent = upload(par32), d_parity = upload(parity),
+         d_lf = upload(log_fact), d_lc = upload(log_children);
+    box_view b;
+    b.V = static_cast<int>(V);
+    b.S = S;
+    b.K = 1;
+    b.n = n;
+    b.cstride = static_cast<u32>(fine.compositions().stride());
+    b.comp = dp<u64>(d_comp);
+    b.sbegin = dp<i64>(d_sbegin);
+    b.stotals = dp<int>(d_stot);
+    b.ssizes = dp<u64>(d_ssz);
+    transfer_view t;
+    t.ccomp = dp<u64>(d_ccomp);
+    t.ccstride = static_cast<u32>(coarse.compositions().stride());
+    t.Vc = static_cast<int>(coarse.voxels());
+    t.csizes = dp<u64>(d_csz);
+    t.parent = dp<int>(d_parent);
+    t.parity = dp<u8>(d_parity);
+    t.log_fact = dp<double>(d_lf);
+    t.log_children = dp<double>(d_lc);
+    dbuf d_co(n * sizeof(i64)), d_lw(n * sizeof(double)), d_col(colour ? n : 1);
+    hipLaunchKernelGGL(k_transfer, dim3(grid_of(n)), dim3(kThreadsA), 0, s, b, t, dp<i64>(d_co),
+                       dp<double>(d_lw), colour ? dp<u8>(d_col) : nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    coarse_of = download<i64>(d_co, n);
+    log_weight = download<double>(d_lw, n);
+    if (colour) *colour = download<u8>(d_col, n);
+    return true;
+}
+
+bool assemble_csr_device(const mk::population_space &space, const mk::voxel_domain &domain,
+                         const mk::reaction_tables &reactions, bool absorbing, mk::generator_csr &g) {
+    device_rows r;
+    if (!assemble_rows(space, domain, reactions, absorbing, r)) return false;
+    const hipStream_t s = stream();
+    const i64 n = r.n;
+    const auto len = download<u32>(r.len, n);
+    g = mk::generator_csr();
+    g.n = n;
+    g.ptr.assign(n + 1, 0);
+    for (i64 i = 0; i < n; ++i) g.ptr[i + 1] = g.ptr[i] + len[i];
+    const u64 nnz = g.ptr[n];
+    if (nnz == r.raw_total) { // no merged entries: the raw arrays are the rows
+        g.col = download<u32>(r.col, nnz);
+        g.val = download<double>(r.val, nnz);
+    } else {
+        dbuf d_ptr = upload(g.ptr), c2(std::max<u64>(nnz, 1) * sizeof(u32)), v2(std::max<u64>(nnz, 1) * sizeof(double));
+        hipLaunchKernelGGL(k_compact, dim3(grid_of(n)), dim3(kThreadsA), 0, s, n, dp<u64>(r.raw_ptr),
+                           dp<u64>(d_ptr), dp<u32>(r.col), dp<double>(r.val), dp<u32>(c2), dp<double>(v2));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        g.col = download<u32>(c2, nnz);
+        g.val = download<double>(v2, nnz);
+    }
+    g.exit = download<double>(r.exit, n);
+    g.leak = download<double>(r.leak, n);
+    const auto jumps = download<u32>(r.jumps, n);
+    for (u32 j : jumps) g.n_jumps += j;
+    return true;
+}
+
+bool assemble_generator_device(const mk::population_space &space, const mk::voxel_domain &domain,
+                               const mk::reaction_tables &reactions, bool absorbing,
+                               mk_generator_s &G) {
+    device_rows r;
+    if (!space.is_box() || !assemble_rows(space, domain, reactions, absorbing, r)) return false;
+    const hipStream_t s = stream();
+    const i64 n = r.n;
+    const u64 raw_total = r.raw_total;
+    const auto &raw_ptr = r.raw_ptr_h;
+    dbuf &d_raw_ptr = r.raw_ptr, &d_col = r.col, &d_val = r.val, &d_len = r.len, &d_exit = r.exit,
+         &d_leak = r.leak, &d_jumps = r.jumps, &d_norm = r.norm;
 
     // distinct rates and exit rates, slice widths, longest row and largest column
     dbuf d_vt(kHashSlots * sizeof(u64)), d_et(kHashSlots * sizeof(u64)), d_over(2 * sizeof(int));

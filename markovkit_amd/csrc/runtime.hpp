@@ -506,6 +506,18 @@ bool assemble_generator_device(const mk::population_space &space, const mk::voxe
                                const mk::reaction_tables &reactions, bool absorbing,
                                mk_generator_s &G);
 
+// The same rows as mk::assemble_generator, assembled on the device and copied back to a host
+// CSR (the multigrid levels' operators). False for spaces outside the kernel's limits.
+bool assemble_csr_device(const mk::population_space &space, const mk::voxel_domain &domain,
+                         const mk::reaction_tables &reactions, bool absorbing, mk::generator_csr &g);
+
+// level_hooks::transfer on the device (sector levels of at least 4096 states)
+bool transfer_device(const mk::population_space &fine, const mk::population_space &coarse,
+                     const mk::voxel_domain &dom, const std::vector<i64> &parent,
+                     const std::vector<double> &log_fact, const std::vector<double> &log_children,
+                     std::vector<i64> &coarse_of, std::vector<double> &log_weight,
+                     std::vector<std::uint8_t> *colour);
+
 template <typename T>
 struct level_dev {
     i64 n = 0;

@@ -724,6 +724,18 @@ block_smoother build_blocks(const population_space &space, const std::vector<i64
     return bs;
 }
 
+// every entry of A joins the two colours (the red-black smoother's precondition)
+void check_colours(const std::vector<std::uint8_t> &colour, const generator_csr &A) {
+    const i64 n = A.n, chunk = 1 << 16;
+    std::atomic<bool> bad{false};
+    run_tasks((n + chunk - 1) / chunk, [&](i64 t) {
+        for (i64 i = t * chunk; i < std::min(n, (t + 1) * chunk) && !bad.load(std::memory_order_relaxed); ++i)
+            for (u64 p = A.ptr[i]; p < A.ptr[i + 1]; ++p)
+                if (colour[A.col[p]] == colour[i]) bad = true;
+    });
+    if (bad) throw std::logic_error("red-black smoother: the diffusion graph is not bipartite");
+}
+
 // red-black colouring of a sector's diffusion graph (a hop changes one molecule's voxel
 // coordinate sum by one, so it flips the parity); checked on every entry
 std::vector<std::uint8_t> state_colours(const population_space &space, const voxel_domain &dom,
@@ -748,17 +760,15 @@ std::vector<std::uint8_t> state_colours(const population_space &space, const vox
             colour[i] = static_cast<std::uint8_t>(p);
         }
     });
-    for (i64 i = 0; i < n; ++i)
-        for (u64 p = A.ptr[i]; p < A.ptr[i + 1]; ++p)
-            if (colour[A.col[p]] == colour[i])
-                throw std::logic_error("red-black smoother: the diffusion graph is not bipartite");
+    check_colours(colour, A);
     return colour;
 }
 
 } // namespace
 
 sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::vector<int> &totals,
-                                        double shift, const multigrid_options &options) {
+                                        double shift, const multigrid_options &options,
+                                        const level_hooks &hooks) {
     if (!(shift > 0.0)) throw std::invalid_argument("population_multigrid: shift must be positive");
     if (domain.dim == 0) throw std::invalid_argument("population_multigrid needs a grid domain");
     sector_hierarchy h;
@@ -771,8 +781,7 @@ sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::v
         level.rows = space.size();
         level.pre_sweeps = opt.pre_sweeps;
         level.post_sweeps = opt.post_sweeps;
-        level.A = assemble_generator(space, dom, none, true);
-        if (opt.smoother == 2) level.colour = state_colours(space, dom, level.A);
+        level.A = hooks.assemble ? hooks.assemble(space, dom) : assemble_generator(space, dom, none, true);
         std::vector<double> doubled = dom.diffusivities;
         for (double &d : doubled) d *= 2.0;
         auto coarse = coarsen(dom, doubled);
@@ -788,6 +797,7 @@ sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::v
             }
             if (!invert_dense(std::move(A), n, level.inverse))
                 throw std::runtime_error("population_multigrid: singular block");
+            if (opt.smoother == 2) level.colour = state_colours(space, dom, level.A);
             level.terminal = true;
             h.levels.push_back(std::move(level));
             break;
@@ -806,26 +816,50 @@ sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::v
         }
         std::vector<double> log_children(coarse->voxels);
         for (i64 a = 0; a < coarse->voxels; ++a) log_children[a] = std::log(double(children[a]));
+        // lgamma(n + 1) for every count a state can hold
+        int max_total = 0;
+        for (int t : totals) max_total = std::max(max_total, t);
+        std::vector<double> log_fact(max_total + 1);
+        {
+            int sign = 0;
+            for (int c = 0; c <= max_total; ++c) log_fact[c] = lgamma_r(c + 1.0, &sign);
+        }
         level.coarse_of.resize(level.rows);
         level.weight.resize(level.rows);
         const i64 chunk = 32768;
-        run_tasks((level.rows + chunk - 1) / chunk, [&](i64 t) {
+        std::vector<double> log_weight;
+        const bool by_hook =
+            hooks.transfer && hooks.transfer(space, cspace, dom, parent, log_fact, log_children, level.coarse_of,
+                                             log_weight, opt.smoother == 2 ? &level.colour : nullptr);
+        if (by_hook) {
+            run_tasks((level.rows + chunk - 1) / chunk, [&](i64 t) {
+                for (i64 i = t * chunk; i < std::min(level.rows, (t + 1) * chunk); ++i)
+                    level.weight[i] = std::exp(log_weight[i]);
+            });
+            if (opt.smoother == 2) check_colours(level.colour, level.A);
+        } else {
+            if (opt.smoother == 2) level.colour = state_colours(space, dom, level.A);
+        }
+        if (!by_hook) run_tasks((level.rows + chunk - 1) / chunk, [&](i64 t) {
             std::vector<int> x(dom.voxels * S), c(coarse->voxels * S);
-            int sign = 0;
             for (i64 i = t * chunk; i < std::min(level.rows, (t + 1) * chunk); ++i) {
                 space.state_in_sector(0, i, x.data());
                 std::fill(c.begin(), c.end(), 0);
                 double log_weight = 0.0;
+                // empty voxels add lgamma(1) = 0 and 0 * log(children) = 0, which leave the sums
+                // unchanged, so only occupied ones are visited (the same sums as visiting all)
                 for (i64 v = 0; v < dom.voxels; ++v)
                     for (int s = 0; s < S; ++s) {
                         const int n = x[v * S + s];
+                        if (n == 0) continue;
                         c[parent[v] * S + s] += n;
-                        log_weight -= lgamma_r(n + 1.0, &sign);
+                        log_weight -= log_fact[n];
                     }
                 for (i64 a = 0; a < coarse->voxels; ++a)
                     for (int s = 0; s < S; ++s) {
                         const int n = c[a * S + s];
-                        log_weight += lgamma_r(n + 1.0, &sign) - (n * log_children[a]);
+                        if (n == 0) continue;
+                        log_weight += log_fact[n] - (n * log_children[a]);
                     }
                 level.coarse_of[i] = cspace.rank_in_sector(c.data(), 0);
                 level.weight[i] = std::exp(log_weight);

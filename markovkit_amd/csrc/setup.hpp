@@ -192,9 +192,28 @@ struct sector_hierarchy {
     std::vector<sector_level> levels;
 };
 
+// Optional replacements of the per-state loops of build_sector_hierarchy (the library runs
+// them on the device); each gives exactly what the host loop gives. Empty functions, or a
+// false return, leave the loop on the host.
+struct level_hooks {
+    // one level's diffusion operator (sector space, no reactions, absorbing rule)
+    std::function<generator_csr(const population_space &, const voxel_domain &)> assemble;
+    // for every state i of `fine`: coarse_of[i], the rank of its aggregate in `coarse`;
+    // log_weight[i] = -sum_{v, s} log_fact[x_vs] + sum_{a, s} (log_fact[c_as] - c_as
+    // log_children[a]) over occupied voxels in ascending order; and, when colour is given, the
+    // red-black colour (parity of sum x_vs * (coordinate sum of v))
+    std::function<bool(const population_space &fine, const population_space &coarse,
+                       const voxel_domain &dom, const std::vector<i64> &parent,
+                       const std::vector<double> &log_fact, const std::vector<double> &log_children,
+                       std::vector<i64> &coarse_of, std::vector<double> &log_weight,
+                       std::vector<std::uint8_t> *colour)>
+        transfer;
+};
+
 // population_multigrid's levels for one sector (multigrid.hpp:145-179)
 sector_hierarchy build_sector_hierarchy(const voxel_domain &domain, const std::vector<int> &totals,
-                                        double shift, const multigrid_options &options);
+                                        double shift, const multigrid_options &options,
+                                        const level_hooks &hooks = {});
 
 // dense inverse by Gauss-Jordan with partial pivoting; false when singular
 bool invert_dense(std::vector<double> a, i64 n, std::vector<double> &inverse);

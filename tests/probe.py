@@ -19,8 +19,9 @@ _lib = None
 
 def build():
     tmp = LIB + ".tmp"
+    # the library's host flags: no contraction of products into sums
     subprocess.run(["g++", "-O2", "-std=c++20", "-fPIC", "-shared", "-pthread",
-                    "-march=x86-64-v3", *SRC, "-o", tmp], check=True)
+                    "-march=x86-64-v3", "-ffp-contract=off", *SRC, "-o", tmp], check=True)
     os.replace(tmp, LIB)
 
 

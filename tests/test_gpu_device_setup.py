@@ -1,9 +1,11 @@
-"""Device-side generator assembly (assemble.hip) against the host assembly (setup.cpp), which the
-oracle pins: for each box both are built through the C ABI, and the exit and leak rates, the
-jump count and the operator (a shifted apply of random vectors, which reads every stored entry)
-must be bitwise equal. Cases: 1D birth-death boxes, the 2D two-species annihilation box, a
-reflecting boundary, two reactions with the same change vector (rows with merged entries), an
-explicit voxel graph with uneven volumes, a three-species box, and the 11.2M-state box.
+"""Device-side generator and multigrid-level assembly (assemble.hip) against the host assembly
+(setup.cpp), which the oracle pins: for each box both are built through the C ABI, and the exit
+and leak rates, the jump count and the operator (a shifted apply of random vectors, which reads
+every stored entry) must be bitwise equal. Cases: 1D birth-death boxes, the 2D two-species
+annihilation box, a reflecting boundary, two reactions with the same change vector (rows with
+merged entries), an explicit voxel graph with uneven volumes, a three-species box, and the
+11.2M-state box. Hierarchies whose level operators are assembled on the device give bitwise
+the host hierarchy's cycle, for every smoother and precision.
 """
 import time
 
@@ -81,3 +83,41 @@ def test_full_size_box():
     w = config_large()
     box = mk.PopulationBox(w.n_voxels, w.max_totals)
     compare(box, w.model, trials=1, name=w.name)
+
+
+def hierarchy(w, box, smoother, precision, device):
+    mk.device_setup_enable(device)
+    try:
+        t = time.perf_counter()
+        M = mk.SectorMultigrid(w.model.domain, box, w.shifts[0],
+                               mk.MultigridOptions(precision=precision, smoother=smoother))
+        mk.synchronize()
+        return M, time.perf_counter() - t
+    finally:
+        mk.device_setup_enable(True)
+
+
+@pytest.mark.parametrize("smoother", [mk.JACOBI, mk.RED_BLACK, mk.ILU0], ids=["jacobi", "red_black", "ilu0"])
+@pytest.mark.parametrize("precision", [mk.FP32, mk.FP64], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("wl", [birth_death_1d(32, 4, 0.5, "random"), annihilation_2d(4, (2, 2), 1.0)],
+                         ids=["58905", "2d-4x4"])
+def test_device_hierarchy_matches_host(wl, precision, smoother):
+    """level operators assembled on the device give the same hierarchy: the cycles agree bitwise"""
+    box = mk.PopulationBox(wl.n_voxels, wl.max_totals)
+    Mh, th = hierarchy(wl, box, smoother, precision, False)
+    Md, td = hierarchy(wl, box, smoother, precision, True)
+    b = np.random.default_rng(9).standard_normal(box.size())
+    assert np.array_equal(Md.apply(b), Mh.apply(b))
+    print(f"[{wl.name}] hierarchy host {th:.3f}s device {td:.3f}s")
+
+
+def test_device_hierarchy_full_size():
+    w = config_large()
+    box = mk.PopulationBox(w.n_voxels, w.max_totals)
+    Mh, th = hierarchy(w, box, mk.RED_BLACK, mk.FP32, False)
+    b = np.random.default_rng(9).random(box.size())
+    zh = Mh.apply(b)
+    del Mh
+    Md, td = hierarchy(w, box, mk.RED_BLACK, mk.FP32, True)
+    assert np.array_equal(Md.apply(b), zh)
+    print(f"[{w.name}] red-black FP32 hierarchy host {th:.2f}s device {td:.2f}s")

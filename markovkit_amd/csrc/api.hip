@@ -371,7 +371,7 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
     if (mg.opt.coarse_cycles == 1 && (mg.opt.smoother == MK_JACOBI || mg.opt.smoother == MK_RED_BLACK ||
                                       mg.opt.smoother == MK_AGGREGATE_BLOCK)) {
         i64 start = depth;
-        while (start > 0 && L[start - 1].n <= gpu::kTailRows && L[start - 1].n_agg == 0) --start;
+        while (start > 0 && L[start - 1].n <= tail_rows() && L[start - 1].n_agg == 0) --start;
         if (start < depth) mg.tail_start = start;
     }
     mg.tail_desc = upload(desc);
@@ -573,14 +573,19 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
         auto &c = L[l + 1];
         // restriction reads each linked fine row once (x) plus R's encoding, writes the coarse rows
         const double rbytes = op_bytes(lv.R, 0, ts, 1) + double(lv.n_fine_linked) * ts;
-        const double pbytes = double(lv.n) * (4 + 3 * ts) + double(c.n) * ts;
+        // red-black with post-smoothing: only colour-0 rows need the correction, since the first
+        // post-smoothing half-sweep rewrites colour-1 rows without reading them
+        const bool red_only = rbgs && lv.post > 0;
+        const i64 np = red_only ? lv.Ac[0].n : n;
+        const u32 *prows = red_only ? dp<u32>(lv.rowmap[0]) : nullptr;
+        const double pbytes = double(np) * (4 + 3 * ts + (red_only ? 4 : 0)) + double(c.n) * ts;
         if (mg.opt.coarse_cycles <= 1) {
             P.run(cls(K_RESTRICT), rbytes, [&] {
                 gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.rhs), s);
             });
             T *cx = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr);
             P.run(cls(K_PROLONG), pbytes, [&] {
-                gpu::prolong_add<T>(n, dp<u32>(lv.coarse_of), dp<T>(lv.weight), cx, cur, s);
+                gpu::prolong_add<T>(np, dp<u32>(lv.coarse_of), dp<T>(lv.weight), cx, cur, s, prows);
             });
         } else {
             P.run(cls(K_RESTRICT), rbytes, [&] {
@@ -597,8 +602,8 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
                 P.run(cls(K_OTHER), double(c.n) * 3 * ts, [&] { gpu::axpy<T>(c.n, T(1), ce, dp<T>(c.cx), s); });
             }
             P.run(cls(K_PROLONG), pbytes, [&] {
-                gpu::prolong_add<T>(n, dp<u32>(lv.coarse_of), dp<T>(lv.weight), dp<T>(c.cx),
-                                    cur, s);
+                gpu::prolong_add<T>(np, dp<u32>(lv.coarse_of), dp<T>(lv.weight), dp<T>(c.cx),
+                                    cur, s, prows);
             });
         }
     }

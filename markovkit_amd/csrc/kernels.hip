@@ -278,9 +278,10 @@ __global__ void __launch_bounds__(kThreads)
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x) {
-    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
-         i += static_cast<i64>(gridDim.x) * kThreads) {
+    k_prolong_add(i64 n, const u32 *rows, const u32 *coarse_of, const T *weight, const T *coarse, T *x) {
+    for (i64 q = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; q < n;
+         q += static_cast<i64>(gridDim.x) * kThreads) {
+        const i64 i = rows ? rows[q] : q;
         const u32 c = coarse_of[i];
         if (c != 0xFFFFFFFFu) x[i] += weight[i] * coarse[c];
     }
@@ -573,6 +574,8 @@ __global__ void __launch_bounds__(kTailThreads)
         if (lv.n_next > 0) {
             const T *cx = cur[k + 1];
             for (i64 i = tid; i < lv.n; i += nt) {
+                // red-black with post-smoothing: colour-1 rows are rewritten without being read
+                if (lv.colour && lv.post > 0 && lv.colour[i] != 0) continue;
                 const u32 c = lv.coarse_of[i];
                 if (c != 0xFFFFFFFFu) x[i] += lv.weight[i] * cx[c];
             }
@@ -1020,9 +1023,9 @@ void restrict_children(i64 n_coarse, const u32 *cptr, const u32 *cidx, const T *
 
 template <typename T>
 void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x,
-                 hipStream_t s) {
+                 hipStream_t s, const u32 *rows) {
     if (n == 0) return;
-    hipLaunchKernelGGL((k_prolong_add<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, coarse_of,
+    hipLaunchKernelGGL((k_prolong_add<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, rows, coarse_of,
                        weight, coarse, x);
 }
 
@@ -1185,7 +1188,8 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
     template void restrict_children<T>(i64, const u32 *, const u32 *, const T *, T *,           \
                                        hipStream_t);                                            \
     template void gather_sum<T>(const csr_view<T> &, const T *, T *, hipStream_t);              \
-    template void prolong_add<T>(i64, const u32 *, const T *, const T *, T *, hipStream_t);     \
+    template void prolong_add<T>(i64, const u32 *, const T *, const T *, T *, hipStream_t,       \
+                                 const u32 *);                                                    \
     template void terminal_solve<T>(i64, const u32 *, const u32 *, const u64 *, const T *, i64, \
                                     const T *, T *, hipStream_t);                               \
     template void fill<T>(i64, T, T *, hipStream_t);                                            \

@@ -103,10 +103,11 @@ void gather_sum(const csr_view<T> &R, const T *x, T *y, hipStream_t s);
 template <typename T>
 void restrict_children(i64 n_coarse, const u32 *cptr, const u32 *cidx, const T *fine, T *coarse,
                        hipStream_t s);
-// prolongation fused with the correction: x[i] += weight[i] * coarse[coarse_of[i]]
+// prolongation fused with the correction: x[i] += weight[i] * coarse[coarse_of[i]] for the n
+// rows i = rows[q] (all n rows when rows is null)
 template <typename T>
 void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x,
-                 hipStream_t s);
+                 hipStream_t s, const u32 *rows = nullptr);
 // dense terminal solves: x[start_b + i] += sum_j inv_b[i][j] * r[start_b + j]
 template <typename T>
 void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,

@@ -374,6 +374,16 @@ inline std::atomic<int> &device_setup_switch() {
     return on;
 }
 
+// levels of at most this many rows run in the fused tail (MK_TAIL_ROWS overrides, for tuning)
+inline i64 tail_rows() {
+    static const i64 rows = [] {
+        const char *env = std::getenv("MK_TAIL_ROWS");
+        const long v = env ? std::strtol(env, nullptr, 10) : 0;
+        return v > 0 ? static_cast<i64>(v) : gpu::kTailRows;
+    }();
+    return rows;
+}
+
 inline std::atomic<int> &tail_switch() {
     static std::atomic<int> on{[] {
         const char *env = std::getenv("MK_MG_TAIL");

@@ -100,11 +100,13 @@ def cpu_baseline(w, b, iterations, tol, restart):
     }
 
 
-def multishift(mk, workloads, steps, restart, sequential_sample=16):
+def multishift(mk, workloads, steps, restart, sequential_sample=16, tune=True):
     """C4: 64 log-spaced shifts x 4 random L1-normalised rhs over the 58,905-state 1D box, one
     batch of 256 systems (FP32 sector multigrid, per-system FGMRES with delayed CGS2), each to
-    1e-10 relative residual; beside it the same systems solved one at a time on the GPU (a
-    bounded sample of them, each with its own sector multigrid)."""
+    1e-10 relative residual, with the sweep counts picked by the autotuner (trial batches,
+    untimed); beside it the same systems solved one at a time on the GPU (a bounded sample of
+    them, each with its own red-black sector multigrid)."""
+    from markovkit_amd import autotune
     w = workloads.config_multishift()
     box = mk.PopulationBox(w.n_voxels, w.max_totals)
     Q = mk.Generator(box, w.model)
@@ -116,8 +118,16 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16):
     shifts = [s for s in w.shifts for _ in rhs]
     B = np.array([b for _ in w.shifts for b in rhs])
     tol = 1e-10 * np.linalg.norm(B, axis=1)
+    tuned = None
+    if tune:
+        t = time.perf_counter()
+        tuned = autotune.tune_batch(Q, w.model.domain, box, shifts, B, restart=restart)
+        tuned_s = time.perf_counter() - t
+        log(f"[multishift] autotuned sweeps {tuned.pre}/{tuned.post} in {tuned_s:.1f}s: " +
+            ", ".join(f"{tr.pre}/{tr.post}={tr.seconds * 1e3:.1f}ms" for tr in tuned.trials))
     t = time.perf_counter()
-    M = mk.SectorMultigrid(w.model.domain, box, 1.0, mk.MultigridOptions(precision=mk.FP32))
+    M = mk.SectorMultigrid(w.model.domain, box, 1.0,
+                           tuned.options(precision=mk.FP32) if tuned else mk.MultigridOptions(precision=mk.FP32))
     batch = mk.BatchSolver(Q, M, shifts)
     setup = time.perf_counter() - t
     batch.solve(B, tolerances=tol, restart=restart)  # warm-up
@@ -139,7 +149,8 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16):
     pick = np.linspace(0, len(shifts) - 1, sequential_sample).round().astype(int)
     seq = 0.0
     for j in pick:
-        Ms = mk.SectorMultigrid(w.model.domain, box, shifts[j], mk.MultigridOptions(precision=mk.FP32))
+        Ms = mk.SectorMultigrid(w.model.domain, box, shifts[j],
+                                mk.MultigridOptions(precision=mk.FP32, smoother=mk.RED_BLACK))
         A = mk.ShiftedGenerator(Q, shifts[j])
         x = np.zeros(n)
         mk.gmres(A, Ms, B[j], x, tolerance=tol[j], restart=restart)
@@ -164,6 +175,9 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16):
         "setup_s": setup,
         "sequential_gpu_solves_per_s": seq_rate,
         "batched_over_sequential": (len(shifts) * steps / total) / seq_rate,
+        "autotuned_sweeps": [tuned.pre, tuned.post] if tuned else None,
+        "autotune_trials": ([{"sweeps": [tr.pre, tr.post], "ms": 1e3 * tr.seconds, "max_iterations": tr.iterations}
+                             for tr in tuned.trials] if tuned else None),
     }
 
 
@@ -271,11 +285,15 @@ def main():
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--workers", type=int, default=4, help="serving: worker streams")
     ap.add_argument("--restart", type=int, default=50)
+    ap.add_argument("--pre", type=int, default=2, help="pre-smoothing sweeps (multigrid_options)")
+    ap.add_argument("--post", type=int, default=2, help="post-smoothing sweeps (multigrid_options)")
     ap.add_argument("--smoother", default="red_black", choices=list(SMOOTHERS),
                     help="sector multigrid smoother of the timed and served solves (red_black: "
                          "multicolour Gauss-Seidel; jacobi: the reference's default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched multi-shift solves with the default sweep counts")
     ap.add_argument("--no-jacobi", action="store_true",
                     help="skip the comparison solve with the reference's Jacobi smoother")
     ap.add_argument("--no-multishift", action="store_true",
@@ -300,7 +318,7 @@ def main():
         print(json.dumps(out), flush=True)
         return
     if args.workload == "multishift":
-        ms = multishift(mk, workloads, args.steps, args.restart)
+        ms = multishift(mk, workloads, args.steps, args.restart, tune=not args.no_autotune)
         out = {"metric": "batched multi-shift RDME resolvent solves/s at <=1e-10 residual",
                "value": ms["value"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
                "warmup": 1, "ms_per_step": ms["batch_ms"], "higher_is_better": True,
@@ -321,7 +339,8 @@ def main():
     t = time.perf_counter()
     smoother = getattr(mk, SMOOTHERS[args.smoother])
     M32 = mk.SectorMultigrid(w.model.domain, box, shift,
-                             mk.MultigridOptions(precision=mk.FP32, smoother=smoother))
+                             mk.MultigridOptions(precision=mk.FP32, smoother=smoother,
+                                                 pre_sweeps=args.pre, post_sweeps=args.post))
     t_mg = time.perf_counter() - t
     log(f"[setup] FP32 multigrid {t_mg:.1f}s, {M32.max_levels()} levels")
     A = mk.ShiftedGenerator(Q, shift)
@@ -353,7 +372,8 @@ def main():
     if not args.no_fp64:
         t = time.perf_counter()
         M64 = mk.SectorMultigrid(w.model.domain, box, shift,
-                                 mk.MultigridOptions(precision=mk.FP64, smoother=smoother))
+                                 mk.MultigridOptions(precision=mk.FP64, smoother=smoother,
+                                                     pre_sweeps=args.pre, post_sweeps=args.post))
         log(f"[setup] FP64 multigrid {time.perf_counter() - t:.1f}s")
         run_solves(mk, A, M64, b_dev, x_dev, tol, max(args.warmup, 1), args.restart)
         total64, lat64, iters64, res64, _ = run_solves(mk, A, M64, b_dev, x_dev, tol, args.steps,
@@ -415,6 +435,7 @@ def main():
             "jumps": Q.n_jumps(),
             "multigrid_levels": M32.max_levels(),
             "smoother": args.smoother,
+            "sweeps": [args.pre, args.post],
         },
         "p50_ms": 1e3 * percentile(lat, 50),
         "p99_ms": 1e3 * percentile(lat, 99),
@@ -445,7 +466,8 @@ def main():
     }
     if args.workload == "large" and not args.no_multishift:
         log("[multishift] 256-system batch")
-        out["multishift"] = multishift(mk, workloads, 3, args.restart, sequential_sample=8)
+        out["multishift"] = multishift(mk, workloads, 3, args.restart, sequential_sample=8,
+                                       tune=not args.no_autotune)
         log(f"[multishift] {out['multishift']['value']:.1f} solves/s")
     ref_iters = iters[-1]
     if args.smoother != "jacobi" and not args.no_jacobi:

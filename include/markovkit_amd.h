@@ -275,6 +275,11 @@ mk_status mk_fused_tail_enable(int32_t on);
  * operator, exit and leak rates are the same bit for bit. Boxes over 32 molecules per state,
  * 8 species or 65,535 voxels always assemble on the host. */
 mk_status mk_device_setup_enable(int32_t on);
+/* Workgroup-to-XCD mapping of the operator kernels (no reference counterpart): XCD j takes the
+ * chunks j, j + 8, ... of `chunk` consecutive row blocks; 0 keeps workgroup b on row block b.
+ * Default 16 (MK_XCD_CHUNK overrides). Process-wide; the autotuner sets it. */
+mk_status mk_set_xcd_chunk(int32_t chunk);
+mk_status mk_get_xcd_chunk(int32_t *chunk);
 /* per-kernel-class timing with HIP events on the library stream. Classes, in bit order:
  * "shifted_spmv", "jacobi_first", "jacobi_sweep", "residual", "restrict", "prolong",
  * "terminal", "gmres_dot", "gmres_update", "gmres_other". mask: 0 = off, -1 = all, else the

@@ -990,6 +990,20 @@ mk_status mk_graphs_enable(int32_t on) {
     return MK_OK;
 }
 
+mk_status mk_set_xcd_chunk(int32_t chunk) {
+    return guarded([&] {
+        require(chunk >= 0 && chunk <= 4096, "xcd chunk: 0 to 4096 row blocks");
+        gpu::set_xcd_chunk(chunk);
+    });
+}
+
+mk_status mk_get_xcd_chunk(int32_t *chunk) {
+    return guarded([&] {
+        require(chunk != nullptr, "xcd chunk: bad argument");
+        *chunk = gpu::xcd_chunk();
+    });
+}
+
 mk_status mk_device_setup_enable(int32_t on) {
     device_setup_switch().store(on ? 1 : 0);
     return MK_OK;

@@ -275,6 +275,10 @@ mk_status mk_fused_tail_enable(int32_t on);
  * operator, exit and leak rates are the same bit for bit. Boxes over 32 molecules per state,
  * 8 species or 65,535 voxels always assemble on the host. */
 mk_status mk_device_setup_enable(int32_t on);
+/* The fused tail keeps its levels' operators, transfers and vectors in LDS where they fit in
+ * 150 KB (default on; MK_TAIL_LDS=0 turns it off). Results are bitwise the same either way. No
+ * reference counterpart. */
+mk_status mk_tail_lds_enable(int32_t on);
 /* Workgroup-to-XCD mapping of the operator kernels (no reference counterpart): XCD j takes the
  * chunks j, j + 8, ... of `chunk` consecutive row blocks; 0 keeps workgroup b on row block b.
  * Default 16 (MK_XCD_CHUNK overrides). Process-wide; the autotuner sets it. */

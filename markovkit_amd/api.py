@@ -555,6 +555,11 @@ def device_setup_enable(on: bool = True) -> None:
     check(lib().mk_device_setup_enable(1 if on else 0))
 
 
+def tail_lds_enable(on: bool = True) -> None:
+    """Keep the fused tail's levels in LDS where they fit (default on); bitwise the same."""
+    check(lib().mk_tail_lds_enable(1 if on else 0))
+
+
 def fused_tail_enable(on: bool = True) -> None:
     """Run the small multigrid levels in one workgroup-wide kernel (default on)."""
     check(lib().mk_fused_tail_enable(1 if on else 0))

@@ -113,6 +113,71 @@ void build_ilu(const mk::generator_csr &cat, double shift, level_dev<T> &lv) {
     lv.ilu = true;
 }
 
+// LDS layout of the fused tail: every array of levels tail_start .. end at a 16-byte aligned
+// offset of one arena, the constant ones with a copy from HBM; disabled when an operator is not
+// coded or the arena exceeds the LDS limit
+template <typename T>
+void build_tail_lds(mk_sector_mg_s &mg, const std::vector<level_dev<T>> &L,
+                    const std::vector<gpu::level_desc<T>> &desc) {
+    mg.tail_lds_bytes = 0;
+    mg.tail_plan_n = 0;
+    if (mg.tail_start < 0) return;
+    const i64 depth = static_cast<i64>(L.size());
+    const int levels = static_cast<int>(depth - mg.tail_start);
+    std::vector<gpu::level_desc<T>> ld;
+    std::vector<gpu::lds_copy> plan;
+    u64 off = 0;
+    // offset + 1 as a pointer (0 stays null); src null: a work array, nothing to copy
+    auto place = [&](const void *src, u64 bytes, bool needed) -> std::uintptr_t {
+        if (!needed || bytes == 0) return 0;
+        const u64 o = off;
+        off = (off + bytes + 15) / 16 * 16;
+        if (src) plan.push_back({src, static_cast<u32>(o), static_cast<u32>(bytes)});
+        return static_cast<std::uintptr_t>(o + 1);
+    };
+    auto view = [&](const dev_csr<T> &A, gpu::csr_view<T> &v) {
+        if (!A.coded || A.n == 0) return A.n == 0;
+        const u64 n = static_cast<u64>(A.n);
+        v.soff = reinterpret_cast<const u32 *>(place(A.view.soff, (A.view.n_slices + 1) * 4, true));
+        v.ent = reinterpret_cast<const u32 *>(place(A.view.ent, A.padded * 4, A.padded > 0));
+        v.rlen = reinterpret_cast<const std::uint8_t *>(place(A.view.rlen, n, true));
+        v.rcode = reinterpret_cast<const std::uint8_t *>(place(A.view.rcode, n, true));
+        v.vdict = reinterpret_cast<const T *>(place(A.view.vdict, A.view.n_vdict * sizeof(T), true));
+        v.edict = reinterpret_cast<const T *>(place(A.view.edict, A.view.n_edict * sizeof(T), true));
+        return true;
+    };
+    bool ok = levels <= 12;
+    for (i64 l = mg.tail_start; l < depth && ok; ++l) {
+        const auto &lv = L[l];
+        gpu::level_desc<T> d = desc[l];
+        const u64 n = static_cast<u64>(lv.n), ts = sizeof(T);
+        ok = view(lv.A, d.A) && (lv.n_next == 0 || view(lv.R, d.R));
+        d.coarse_of = reinterpret_cast<const u32 *>(place(desc[l].coarse_of, n * 4, lv.n_next > 0));
+        d.weight = reinterpret_cast<const T *>(place(desc[l].weight, n * ts, lv.n_next > 0));
+        u64 tinv = 0;
+        for (u32 m : lv.h_tsize) tinv += static_cast<u64>(m) * m;
+        const bool term = lv.n_term > 0;
+        d.tstart = reinterpret_cast<const u32 *>(place(desc[l].tstart, lv.n_term * 4, term));
+        d.tsize = reinterpret_cast<const u32 *>(place(desc[l].tsize, lv.n_term * 4, term));
+        d.toff = reinterpret_cast<const u64 *>(place(desc[l].toff, lv.n_term * 8, term));
+        d.tinv = reinterpret_cast<const T *>(place(desc[l].tinv, tinv * ts, term));
+        d.colour = reinterpret_cast<const std::uint8_t *>(place(desc[l].colour, n, desc[l].colour != nullptr));
+        d.perm = reinterpret_cast<const u32 *>(place(desc[l].perm, n * 4, desc[l].perm != nullptr));
+        d.rhs = reinterpret_cast<T *>(place(nullptr, n * ts, true));
+        d.x = reinterpret_cast<T *>(place(nullptr, n * ts, true));
+        d.xt = reinterpret_cast<T *>(place(nullptr, n * ts, true));
+        d.res = reinterpret_cast<T *>(place(nullptr, n * ts, true));
+        ld.push_back(d);
+    }
+    if (!ok || !gpu::vcycle_tail_lds_fits<T>(static_cast<u32>(std::min<u64>(off, ~0u)), levels) ||
+        off > gpu::kTailLdsMax)
+        return;
+    mg.tail_lds_desc = upload(ld);
+    mg.tail_plan = upload(plan);
+    mg.tail_plan_n = static_cast<int>(plan.size());
+    mg.tail_lds_bytes = static_cast<u32>(off);
+}
+
 template <typename T>
 void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H,
                   std::vector<level_dev<T>> &L) {
@@ -375,6 +440,7 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
         if (start < depth) mg.tail_start = start;
     }
     mg.tail_desc = upload(desc);
+    build_tail_lds<T>(mg, L, desc);
 }
 
 // aggregate-block smoothing (multigrid.hpp:309-342) of the level's block sectors: block
@@ -437,11 +503,19 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
     const hipStream_t s = stream();
     const double shift = mg.shift, omega = mg.opt.jacobi_weight;
     if (mg.tail_start >= 0 && static_cast<i64>(l) >= mg.tail_start && tail_switch().load() != 0) {
-        // this level and every coarser one in one workgroup
+        // this level and every coarser one in one workgroup, from LDS where they fit
         const auto *desc = dp<gpu::level_desc<T>>(mg.tail_desc) + l;
+        const bool lds = mg.tail_lds_bytes > 0 && static_cast<i64>(l) == mg.tail_start &&
+                         tail_lds_switch().load() != 0;
         P.run(cls(K_SWEEP), 0.0, [&] {
-            gpu::vcycle_tail<T>(desc, static_cast<int>(L.size() - l), shift, omega, rhs, rhs64,
-                                scale, out, s);
+            if (lds) {
+                gpu::vcycle_tail_lds<T>(dp<gpu::level_desc<T>>(mg.tail_lds_desc), dp<gpu::lds_copy>(mg.tail_plan),
+                                        mg.tail_plan_n, mg.tail_lds_bytes, static_cast<int>(L.size() - l),
+                                        shift, omega, rhs, rhs64, scale, out, dp<T>(lv.x), s);
+            } else {
+                gpu::vcycle_tail<T>(desc, static_cast<int>(L.size() - l), shift, omega, rhs, rhs64,
+                                    scale, out, s);
+            }
         });
         return out ? out : dp<T>(lv.x);
     }
@@ -1006,6 +1080,11 @@ mk_status mk_get_xcd_chunk(int32_t *chunk) {
 
 mk_status mk_device_setup_enable(int32_t on) {
     device_setup_switch().store(on ? 1 : 0);
+    return MK_OK;
+}
+
+mk_status mk_tail_lds_enable(int32_t on) {
+    tail_lds_switch().store(on ? 1 : 0);
     return MK_OK;
 }
 

@@ -466,10 +466,11 @@ __device__ void tail_gs_half(const csr_view<T> &A, const u8 *colour, u8 c, T shi
 }
 
 // fused V-cycle tail: one workgroup runs every level from L[0] down and back up
+// The V-cycle of levels L[0 .. levels) by one workgroup; the top level's result goes to out (in
+// the caller's row order) or, without out, to top_x.
 template <typename T>
-__global__ void __launch_bounds__(kTailThreads)
-    k_vcycle_tail(const level_desc<T> *L, int levels, T shift, T omega, const T *rhs_in,
-                  const double *rhs64, const double *scale_ptr, T *out) {
+__device__ void tail_body(const level_desc<T> *L, int levels, T shift, T omega, const T *rhs_in,
+                          const double *rhs64, const double *scale_ptr, T *out, T *top_x) {
     constexpr int kMaxLevels = 32;
     T *cur[kMaxLevels];
     T *oth[kMaxLevels];
@@ -608,13 +609,82 @@ __global__ void __launch_bounds__(kTailThreads)
         cur[k] = x;
         oth[k] = xt;
     }
-    // the top level's result in out (in the caller's row order), else in L[0].x
-    T *dst = out ? out : L[0].x;
+    // the top level's result in out (in the caller's row order), else in top_x
+    T *dst = out ? out : top_x;
     if (out && perm0) {
         for (i64 i = tid; i < L[0].n; i += nt) dst[perm0[i]] = cur[0][i];
     } else if (cur[0] != dst) {
         for (i64 i = tid; i < L[0].n; i += nt) dst[i] = cur[0][i];
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads)
+    k_vcycle_tail(const level_desc<T> *L, int levels, T shift, T omega, const T *rhs_in,
+                  const double *rhs64, const double *scale_ptr, T *out) {
+    tail_body(L, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, L[0].x);
+}
+
+// pointer fields of an LDS level descriptor hold (offset + 1) into the LDS arena (0: null)
+template <typename P>
+__device__ __forceinline__ P *rebase(P *field, char *base) {
+    const auto off = reinterpret_cast<uintptr_t>(field);
+    return off ? reinterpret_cast<P *>(base + (off - 1)) : nullptr;
+}
+
+template <typename T>
+__device__ void rebase_view(csr_view<T> &v, char *base) {
+    v.soff = rebase(v.soff, base);
+    v.ent = rebase(v.ent, base);
+    v.rlen = rebase(v.rlen, base);
+    v.rcode = rebase(v.rcode, base);
+    v.vdict = rebase(v.vdict, base);
+    v.edict = rebase(v.edict, base);
+}
+
+constexpr int kTailLdsLevels = 12;
+
+// The same V-cycle with every array of the levels staged in LDS first (plan: global source,
+// LDS offset and byte count of each constant array); work vectors live in LDS too.
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads)
+    k_vcycle_tail_lds(const level_desc<T> *L_lds, const lds_copy *plan, int n_plan, int levels,
+                      T shift, T omega, const T *rhs_in, const double *rhs64, const double *scale_ptr,
+                      T *out, T *top_x) {
+    extern __shared__ uint4 arena[];
+    __shared__ uint4 dstore[(kTailLdsLevels * sizeof(level_desc<T>) + 15) / 16];
+    level_desc<T> *d = reinterpret_cast<level_desc<T> *>(dstore);
+    char *base = reinterpret_cast<char *>(arena);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int c = 0; c < n_plan; ++c) {
+        const lds_copy cp = plan[c];
+        const u32 words = cp.bytes / 4;
+        const u32 *src = static_cast<const u32 *>(cp.src);
+        u32 *dst = reinterpret_cast<u32 *>(base + cp.offset);
+        for (u32 w = tid; w < words; w += nt) dst[w] = src[w];
+        for (u32 b = words * 4 + tid; b < cp.bytes; b += nt)
+            base[cp.offset + b] = static_cast<const char *>(cp.src)[b];
+    }
+    if (tid < levels) {
+        level_desc<T> lv = L_lds[tid];
+        rebase_view(lv.A, base);
+        rebase_view(lv.R, base);
+        lv.coarse_of = rebase(lv.coarse_of, base);
+        lv.weight = rebase(lv.weight, base);
+        lv.tstart = rebase(lv.tstart, base);
+        lv.tsize = rebase(lv.tsize, base);
+        lv.toff = rebase(lv.toff, base);
+        lv.tinv = rebase(lv.tinv, base);
+        lv.rhs = rebase(lv.rhs, base);
+        lv.x = rebase(lv.x, base);
+        lv.xt = rebase(lv.xt, base);
+        lv.res = rebase(lv.res, base);
+        lv.colour = rebase(lv.colour, base);
+        lv.perm = rebase(lv.perm, base);
+        __builtin_memcpy(d + tid, &lv, sizeof(lv));
+    }
+    __syncthreads();
+    tail_body(d, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, top_x);
 }
 
 template <typename T>
@@ -1091,6 +1161,27 @@ void vcycle_tail(const level_desc<T> *L, int levels, double shift, double omega,
 }
 
 template <typename T>
+bool vcycle_tail_lds_fits(u32 lds_bytes, int levels) {
+    return levels <= kTailLdsLevels && lds_bytes <= kTailLdsMax;
+}
+
+template <typename T>
+void vcycle_tail_lds(const level_desc<T> *L_lds, const lds_copy *plan, int n_plan, u32 lds_bytes,
+                     int levels, double shift, double omega, const T *rhs, const double *rhs64,
+                     const double *scale, T *out, T *top_x, hipStream_t s) {
+    if (levels <= 0) return;
+    static const bool attr = [] {
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vcycle_tail_lds<T>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(kTailLdsMax)) == hipSuccess;
+    }();
+    (void)attr;
+    hipLaunchKernelGGL((k_vcycle_tail_lds<T>), dim3(1), dim3(kTailThreads), lds_bytes, s, L_lds,
+                       plan, n_plan, levels, static_cast<T>(shift), static_cast<T>(omega), rhs, rhs64,
+                       scale, out, top_x);
+}
+
+template <typename T>
 void fill(i64 n, T value, T *x, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, value, x);
@@ -1203,6 +1294,10 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
     template void copy_rows<T>(const u32 *, u64, const T *, T *, hipStream_t);                  \
     template void vcycle_tail<T>(const level_desc<T> *, int, double, double, const T *,         \
                                  const double *, const double *, T *, hipStream_t);              \
+    template bool vcycle_tail_lds_fits<T>(u32, int);                                              \
+    template void vcycle_tail_lds<T>(const level_desc<T> *, const lds_copy *, int, u32, int,      \
+                                     double, double, const T *, const double *, const double *,  \
+                                     T *, T *, hipStream_t);                                       \
     template void axpy<T>(i64, T, const T *, T *, hipStream_t);                                 \
     template void combine<T>(double *, const T *, i64, int, const double *, i64, hipStream_t);     \
     template void gs_half<T>(const csr_view<T> &, double, const T *, T *, T *, hipStream_t, T *,   \

@@ -203,6 +203,23 @@ constexpr int kTailThreads = 1024;
 template <typename T>
 void vcycle_tail(const level_desc<T> *L, int levels, double shift, double omega, const T *rhs,
                  const double *rhs64, const double *scale, T *out, hipStream_t s);
+// The same tail with every array of its levels in LDS: plan[] copies each constant array
+// (operators, transfers, terminal blocks, colours, top-level permutation) from HBM to its LDS
+// offset, and L_lds holds the levels with every pointer field replaced by (LDS offset + 1)
+// (0 for null); work vectors get LDS offsets too. lds_bytes is the arena size. The result goes
+// to out, else to top_x (HBM).
+struct lds_copy {
+    const void *src = nullptr;
+    u32 offset = 0;
+    u32 bytes = 0;
+};
+constexpr u32 kTailLdsMax = 150 * 1024;
+template <typename T>
+bool vcycle_tail_lds_fits(u32 lds_bytes, int levels);
+template <typename T>
+void vcycle_tail_lds(const level_desc<T> *L_lds, const lds_copy *plan, int n_plan, u32 lds_bytes,
+                     int levels, double shift, double omega, const T *rhs, const double *rhs64,
+                     const double *scale, T *out, T *top_x, hipStream_t s);
 
 template <typename T>
 void fill(i64 n, T value, T *x, hipStream_t s);

@@ -384,6 +384,15 @@ inline i64 tail_rows() {
     return rows;
 }
 
+// LDS-resident fused tail (default on where the levels fit; MK_TAIL_LDS=0 turns it off)
+inline std::atomic<int> &tail_lds_switch() {
+    static std::atomic<int> on{[] {
+        const char *env = std::getenv("MK_TAIL_LDS");
+        return (env && env[0] == '0') ? 0 : 1;
+    }()};
+    return on;
+}
+
 inline std::atomic<int> &tail_switch() {
     static std::atomic<int> on{[] {
         const char *env = std::getenv("MK_MG_TAIL");
@@ -587,8 +596,13 @@ struct mk_sector_mg_s {
     std::vector<mk::rt::level_dev<double>> L64;
     std::vector<mk::rt::level_dev<float>> L32;
     mk::rt::dbuf d_b, d_x64, d_out; // staging for host-buffer calls
-    // level descriptors of the fused tail (levels tail_start .. end run in one workgroup)
+    // level descriptors of the fused tail (levels tail_start .. end run in one workgroup), and
+    // its LDS-resident form when the levels fit (tail_lds_bytes > 0): descriptors with LDS
+    // offsets and the copy plan of their constant arrays
     mk::rt::dbuf tail_desc;
     mk::i64 tail_start = -1;
+    mk::rt::dbuf tail_lds_desc, tail_plan;
+    int tail_plan_n = 0;
+    mk::u32 tail_lds_bytes = 0;
 };
 

@@ -1,6 +1,6 @@
-"""The fused multigrid tail (levels of at most 1024 states in one workgroup-wide kernel) against
-the per-level kernels: same formulas and summation orders, so the cycle and the solves must be
-bitwise identical, in FP32 and FP64, with the Jacobi and red-black Gauss-Seidel smoothers, on
+"""The fused multigrid tail (levels of at most 1024 states in one workgroup-wide kernel), with
+its levels in HBM or staged in LDS, against the per-level kernels: same formulas and summation
+orders, so the cycle and the solves must be bitwise identical, in FP32 and FP64, with the Jacobi and red-black Gauss-Seidel smoothers, on
 1D and 2D boxes, for a whole cycle in the tail (969 states) and for a tail below larger levels
 (58,905 states)."""
 import numpy as np
@@ -26,15 +26,18 @@ def test_tail_matches_per_level_kernels(wl, precision, smoother):
     b = rng.random(box.size())
     try:
         out = {}
-        for on in (False, True):
-            mk.fused_tail_enable(on)
+        for name, tail, lds in (("levels", False, False), ("tail", True, False), ("lds", True, True)):
+            mk.fused_tail_enable(tail)
+            mk.tail_lds_enable(lds)
             z = M.apply(b)
             x = np.zeros(box.size())
             r = mk.gmres(mk.ShiftedGenerator(Q, s), M, b, x, tolerance=1e-10 * np.linalg.norm(b))
             assert r.converged
-            out[on] = (z, x, r.iterations)
-        assert np.array_equal(out[False][0], out[True][0])
-        assert np.array_equal(out[False][1], out[True][1])
-        assert out[False][2] == out[True][2]
+            out[name] = (z, x, r.iterations)
+        for name in ("tail", "lds"):
+            assert np.array_equal(out["levels"][0], out[name][0]), name
+            assert np.array_equal(out["levels"][1], out[name][1]), name
+            assert out["levels"][2] == out[name][2], name
     finally:
         mk.fused_tail_enable(True)
+        mk.tail_lds_enable(True)

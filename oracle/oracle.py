@@ -65,6 +65,29 @@ def lib():
         L.orc_gmres.argtypes = [P, P, C.c_double, f64p, f64p, C.c_double, C.c_int64, C.c_int64,
                                 C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         L.orc_gmres.restype = C.c_int
+        D = C.c_double
+        L.orc_box_create.argtypes = [C.c_int, C.c_int, C.c_int, D, D, D, D, D, D, C.c_int, C.c_int,
+                                     C.c_int, C.c_int, C.c_int64, C.POINTER(C.c_void_p)]
+        L.orc_box_destroy.argtypes = [P]
+        for name in ("orc_box_rows", "orc_box_nnz", "orc_box_voxels"):
+            getattr(L, name).argtypes = [P, C.c_int64]
+            getattr(L, name).restype = C.c_int64
+        L.orc_box_levels.argtypes = [P]
+        L.orc_box_levels.restype = C.c_int64
+        L.orc_box_terminal.argtypes = [P, C.c_int64]
+        L.orc_box_rank.argtypes = [P, i32p]
+        L.orc_box_rank.restype = C.c_int64
+        L.orc_box_initial.argtypes = [P, C.c_int]
+        L.orc_box_initial.restype = C.c_int64
+        L.orc_box_matrix.argtypes = [P, C.c_int64, i64p, i64p, f64p]
+        L.orc_box_leak.argtypes = [P, C.c_int64, f64p]
+        L.orc_box_colours.argtypes = [P, C.c_int64, np.ctypeslib.ndpointer(np.uint8, flags="C")]
+        L.orc_box_transfer.argtypes = [P, C.c_int64, i64p, f64p]
+        L.orc_box_cycle.argtypes = [P, f64p, f64p]
+        L.orc_box_smooth.argtypes = [P, C.c_int64, f64p, f64p, C.c_int]
+        L.orc_box_solve.argtypes = [P, f64p, f64p, D, C.c_int, f64p, C.c_int64,
+                                    C.POINTER(C.c_int), C.POINTER(D)]
+        L.orc_box_solve.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -269,3 +292,128 @@ def expv_jumps(first, to, rate, exit_, t, v, krylov_dim=30, tol=1e-8):
                                 np.ascontiguousarray(v, np.float64), w, int(krylov_dim), float(tol),
                                 steps, rej))
     return w, int(steps[0]), int(rej[0])
+
+
+class BoxMultigrid:
+    """The paper driver's whole-box multigrid (box_mg.hpp) on the CPU (FP64).
+
+    Parameters follow the driver's command line (rdme_full_multigrid.cpp:50-63): Schlogl
+    constants c1..c4, smoother "gs" | "jacobi" | "block-ilu" (the driver's) or "colour-gs"
+    (the GPU library's four-colour order), cycles 1 (V) or 2 (W), kcycle for the K-cycle, and
+    terminal_rows > 0 for an exact dense solve at the first level with at most that many states.
+    ``linear(scale)`` gives the driver's --linear rates (c1 = c2 = 0, c3 = 2 scale, c4 = scale).
+    """
+
+    SMOOTHERS = {"gs": 0, "jacobi": 1, "block-ilu": 2, "colour-gs": 3}
+    DEFAULT_RATES = (0.0133, 0.0006, 0.6924125, 0.114725)
+
+    @staticmethod
+    def linear(scale=1.0):
+        return (0.0, 0.0, 2.0 * scale, scale)
+
+    def __init__(self, dimension=1, cells=4, cap=40, diffusion=1.0, shift=0.05, rates=None,
+                 smoother="gs", sweeps=2, cycles=1, kcycle=False, terminal_rows=0):
+        L = lib()
+        c1, c2, c3, c4 = self.DEFAULT_RATES if rates is None else rates
+        self.shift = shift
+        h = C.c_void_p()
+        _check(L.orc_box_create(dimension, cells, cap, diffusion, shift, c1, c2, c3, c4,
+                                self.SMOOTHERS[smoother], sweeps, cycles, 1 if kcycle else 0,
+                                terminal_rows, C.byref(h)))
+        self.h = h
+        self.n = L.orc_box_rows(h, 0)
+        self.V = L.orc_box_voxels(h, 0)
+        self.cap = cap
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_box_destroy(self.h)
+            self.h = None
+
+    def levels(self):
+        return lib().orc_box_levels(self.h)
+
+    def rows(self, level=0):
+        return lib().orc_box_rows(self.h, level)
+
+    def terminal(self, level):
+        return lib().orc_box_terminal(self.h, level)
+
+    def rank(self, counts):
+        return lib().orc_box_rank(self.h, np.ascontiguousarray(counts, np.int32))
+
+    def initial_index(self, initial):
+        return lib().orc_box_initial(self.h, initial)
+
+    def matrix(self, level=0):
+        """(ptr, col, val) of shift I - Q by destination on that level"""
+        L = lib()
+        n, nnz = L.orc_box_rows(self.h, level), L.orc_box_nnz(self.h, level)
+        ptr = np.zeros(n + 1, np.int64)
+        col = np.zeros(nnz, np.int64)
+        val = np.zeros(nnz, np.float64)
+        L.orc_box_matrix(self.h, level, ptr, col, val)
+        return ptr, col, val
+
+    def dense(self, level=0):
+        ptr, col, val = self.matrix(level)
+        n = len(ptr) - 1
+        A = np.zeros((n, n))
+        for i in range(n):
+            A[i, col[ptr[i]:ptr[i + 1]]] = val[ptr[i]:ptr[i + 1]]
+        return A
+
+    def apply_level(self, level, x):
+        import scipy.sparse as sp
+        ptr, col, val = self.matrix(level)
+        n = len(ptr) - 1
+        return sp.csr_matrix((val, col, ptr), shape=(n, n)) @ np.asarray(x, np.float64)
+
+    def leak(self, level=0):
+        out = np.zeros(self.rows(level))
+        lib().orc_box_leak(self.h, level, out)
+        return out
+
+    def colours(self, level=0):
+        out = np.zeros(self.rows(level), np.uint8)
+        lib().orc_box_colours(self.h, level, out)
+        return out
+
+    def transfer(self, level=0):
+        n = self.rows(level)
+        parent = np.zeros(n, np.int64)
+        weight = np.zeros(n, np.float64)
+        _check(lib().orc_box_transfer(self.h, level, parent, weight))
+        return parent, weight
+
+    def smooth(self, level, b, x, reverse=False):
+        """the level's smoother (all its sweeps) from the guess x (returned)"""
+        x = np.array(x, np.float64)
+        _check(lib().orc_box_smooth(self.h, level, np.ascontiguousarray(b, np.float64), x,
+                                    1 if reverse else 0))
+        return x
+
+    def cycle(self, b, x):
+        """one cycle of A x = b from the guess x (returned; x is not modified)"""
+        x = np.array(x, np.float64)
+        lib().orc_box_cycle(self.h, np.ascontiguousarray(b, np.float64), x)
+        return x
+
+    def apply(self, b):
+        return self.cycle(b, np.zeros(self.n))
+
+    def solve(self, b, x0, tolerance=1e-9, max_iterations=1000):
+        """the driver's standalone iteration: (x, iterations, error, history)"""
+        x = np.array(x0, np.float64)
+        hist = np.zeros(max_iterations + 2)
+        it = C.c_int()
+        err = C.c_double()
+        m = lib().orc_box_solve(self.h, np.ascontiguousarray(b, np.float64), x, tolerance,
+                                max_iterations, hist, len(hist), C.byref(it), C.byref(err))
+        return x, it.value, err.value, hist[:m]
+
+    def initial_problem(self, initial):
+        """the driver's first step: b = shift p0, x0 = p0 with p0 the initial state"""
+        p0 = np.zeros(self.n)
+        p0[self.initial_index(initial)] = 1.0
+        return self.shift * p0, p0

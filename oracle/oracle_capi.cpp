@@ -10,6 +10,7 @@
 // sees only (counts, volume), so the oracle's closures look the table up by volume: the
 // first voxel with that volume supplies the row.
 #include "mk_oracle.hpp"
+#include "box_mg.hpp"
 
 #include <cstring>
 #include <exception>
@@ -319,6 +320,130 @@ int orc_expv_jumps(int64_t n, const int64_t *first, const int64_t *to, const dou
         if (steps) *steps = static_cast<int64_t>(r.steps);
         if (rejected) *rejected = static_cast<int64_t>(r.rejected);
     });
+}
+
+// ---- whole-box multigrid of the paper driver (box_mg.hpp)
+int orc_box_create(int dimension, int cells, int cap, double diffusion, double shift, double c1,
+                   double c2, double c3, double c4, int smoother, int sweeps, int cycles,
+                   int kcycle, int64_t terminal_rows, void **out) {
+    return guarded([&] {
+        orc_box::params p;
+        p.dimension = dimension;
+        p.cells = cells;
+        p.cap = cap;
+        p.diffusion = diffusion;
+        p.shift = shift;
+        p.c1 = c1;
+        p.c2 = c2;
+        p.c3 = c3;
+        p.c4 = c4;
+        if (smoother < 0 || smoother > 3) throw std::invalid_argument("oracle box: unknown smoother");
+        p.smoother = static_cast<orc_box::smoother_kind>(smoother);
+        p.sweeps = sweeps;
+        p.cycles = cycles;
+        p.kcycle = kcycle != 0;
+        p.terminal_rows = static_cast<long>(terminal_rows);
+        if (dimension < 1 || dimension > 3 || cells < 1 || (cells & (cells - 1)) || cap < 0 ||
+            !(shift > 0.0) || sweeps < 1 || cycles < 1 || cycles > 2)
+            throw std::invalid_argument("oracle box: invalid parameters");
+        *out = new orc_box::level(p);
+    });
+}
+
+void orc_box_destroy(void *h) { delete static_cast<orc_box::level *>(h); }
+
+namespace {
+const orc_box::level &box_at(void *h, int64_t l) {
+    return *static_cast<orc_box::level *>(h)->at(static_cast<int>(l));
+}
+} // namespace
+
+int64_t orc_box_levels(void *h) { return static_cast<orc_box::level *>(h)->n_levels(); }
+int64_t orc_box_rows(void *h, int64_t l) { return box_at(h, l).rows(); }
+int64_t orc_box_nnz(void *h, int64_t l) { return static_cast<int64_t>(box_at(h, l).A.val.size()); }
+int64_t orc_box_voxels(void *h, int64_t l) { return box_at(h, l).S.V; }
+int orc_box_terminal(void *h, int64_t l) { return box_at(h, l).terminal; }
+
+int64_t orc_box_rank(void *h, const int *n) {
+    const auto &L = box_at(h, 0);
+    std::vector<int> x(n, n + L.S.V);
+    int total = 0;
+    for (int v : x) total += v;
+    if (total > L.S.cap) return -1;
+    return L.S.rank(x, total);
+}
+
+// the driver's initial state: `initial` molecules dealt round-robin over the voxels
+int64_t orc_box_initial(void *h, int initial) {
+    const auto &L = box_at(h, 0);
+    std::vector<int> x(L.S.V, 0);
+    for (int k = 0; k < initial; ++k) ++x[k % L.S.V];
+    if (initial > L.S.cap) return -1;
+    return L.S.rank(x, initial);
+}
+
+void orc_box_matrix(void *h, int64_t l, int64_t *ptr, int64_t *col, double *val) {
+    const auto &A = box_at(h, l).A;
+    for (std::size_t i = 0; i < A.ptr.size(); ++i) ptr[i] = A.ptr[i];
+    for (std::size_t j = 0; j < A.col.size(); ++j) {
+        col[j] = A.col[j];
+        val[j] = A.val[j];
+    }
+}
+
+void orc_box_leak(void *h, int64_t l, double *out) {
+    const auto &L = box_at(h, l);
+    std::copy(L.leak.begin(), L.leak.end(), out);
+}
+
+void orc_box_colours(void *h, int64_t l, uint8_t *out) {
+    const auto &L = box_at(h, l);
+    std::copy(L.colour.begin(), L.colour.end(), out);
+}
+
+int orc_box_transfer(void *h, int64_t l, int64_t *parent, double *weight) {
+    return guarded([&] {
+        const auto &L = box_at(h, l);
+        if (!L.coarse) throw std::invalid_argument("oracle box: the level has no coarse level");
+        for (std::size_t i = 0; i < L.parent.size(); ++i) {
+            parent[i] = L.parent[i];
+            weight[i] = L.weight[i];
+        }
+    });
+}
+
+// x <- the level's smoother (all its sweeps) on A x = b; reverse = post-smoothing order
+int orc_box_smooth(void *h, int64_t l, const double *b, double *x, int reverse) {
+    return guarded([&] {
+        const auto &L = box_at(h, l);
+        if (L.terminal) throw std::invalid_argument("oracle box: a terminal level has no smoother");
+        orc_box::Vec bb(b, b + L.rows()), xx(x, x + L.rows());
+        L.smooth(bb, xx, reverse != 0);
+        std::copy(xx.begin(), xx.end(), x);
+    });
+}
+
+// x <- one cycle of A x = b from the guess x
+void orc_box_cycle(void *h, const double *b, double *x) {
+    const auto &L = box_at(h, 0);
+    orc_box::Vec bb(b, b + L.rows()), xx(x, x + L.rows());
+    L.cycle(bb, xx);
+    std::copy(xx.begin(), xx.end(), x);
+}
+
+// the standalone iteration from the guess x; the error history (one entry per loop test) goes
+// to history[0 .. returned count) when history_cap allows
+int64_t orc_box_solve(void *h, const double *b, double *x, double tolerance, int max_iterations,
+                      double *history, int64_t history_cap, int *iterations, double *error) {
+    const auto &L = box_at(h, 0);
+    orc_box::Vec bb(b, b + L.rows()), xx(x, x + L.rows()), hist;
+    const auto r = orc_box::standalone(L, bb, xx, tolerance, max_iterations, &hist);
+    std::copy(xx.begin(), xx.end(), x);
+    if (iterations) *iterations = r.iterations;
+    if (error) *error = r.error;
+    const int64_t m = std::min<int64_t>(history_cap, static_cast<int64_t>(hist.size()));
+    for (int64_t k = 0; k < m; ++k) history[k] = hist[k];
+    return static_cast<int64_t>(hist.size());
 }
 
 } // extern "C"

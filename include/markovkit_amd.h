@@ -174,6 +174,83 @@ mk_status mk_gmres_device(mk_generator gen, double shift, mk_sector_mg mg, const
                           double *d_x, const mk_gmres_options *options,
                           mk_solver_result *result);
 
+/* ---- whole-box multigrid (the paper driver, rdme/cpp/rdme_full_multigrid.cpp) ----------
+ * The driver's Level hierarchy (:298-560) for a single species on a Cartesian grid of
+ * cells^dimension voxels (unit box, cell volume 1 / V) with every state of total population
+ * <= cap, in the population box order (total first, voxel 0 most significant; :66-152). Every
+ * level holds the complete operator shift I - Q: diffusion at D cells^2 per molecule and
+ * neighbour, births c3 vol + c1 n (n - 1) / (2 vol), deaths c4 n + c2 n (n - 1) (n - 2) /
+ * (6 vol^2) (:410-412), births at the cap lost (absorbing). Coarse levels halve the cells and
+ * double D (:529-531); transfers are the multinomial weights of :532-557. The one-voxel level is
+ * solved by the Thomas algorithm (:512-527); with terminal_rows > 0 the first level of at most
+ * that many states is solved exactly with a dense inverse instead (no driver counterpart).
+ * Smoothers: MK_BOX_GS is Gauss-Seidel in four-colour order (colour = 2 (total mod 2) + (sum of
+ * the molecules' voxel coordinate sums mod 2); replaces the driver's lexicographic "gs", :178-195,
+ * whose sweep is sequential), MK_BOX_JACOBI the driver's "jacobi" (w = 2/3, :171-175).
+ * Cycles (:817-878): cycles = 1 (V) or 2 (W), cycle_type MK_CYCLE_K for the K-cycle. FP64. */
+typedef struct mk_box_mg_s *mk_box_mg;
+enum { MK_BOX_GS = 0, MK_BOX_JACOBI = 1 };
+enum { MK_CYCLE_STANDARD = 0, MK_CYCLE_K = 1 };
+
+typedef struct mk_box_mg_params {
+    int32_t dimension;     /* 1, 2 or 3 (--dimension) */
+    int32_t cells;         /* per axis, a power of two (--cells) */
+    int32_t cap;           /* total population cap (--cap) */
+    double diffusion;      /* --diffusion */
+    double shift;          /* --shift */
+    double c1, c2, c3, c4; /* Schlogl constants (--c1 .. --c4; --linear s: 0, 0, 2 s, s) */
+    int32_t smoother;      /* MK_BOX_GS or MK_BOX_JACOBI */
+    int32_t sweeps;        /* per smoothing step (--sweeps) */
+    int32_t cycles;        /* 1 or 2 (--cycles) */
+    int32_t cycle_type;    /* MK_CYCLE_STANDARD or MK_CYCLE_K (--cycle-type) */
+    int64_t terminal_rows; /* 0, or at most 8192 */
+} mk_box_mg_params;
+
+typedef struct mk_box_mg_result {
+    int32_t iterations; /* cycles run */
+    int32_t converged;  /* error < tolerance */
+    double error;       /* |b - A x|_1 / shift after the last cycle (:1204-1206) */
+    int64_t history;    /* number of errors the iteration produced (the first before any cycle) */
+} mk_box_mg_result;
+
+/* Level(p) with the hierarchy (rdme_full_multigrid.cpp:298-560) */
+mk_status mk_box_mg_create(const mk_box_mg_params *params, mk_box_mg *out);
+mk_status mk_box_mg_destroy(mk_box_mg mg);
+mk_status mk_box_mg_info(mk_box_mg mg, int64_t *rows, int32_t *levels);
+/* level l's states, voxels, stored off-diagonal entries and terminal kind (0 smoothed, 1
+ * Thomas, 2 dense) */
+mk_status mk_box_mg_level(mk_box_mg mg, int32_t level, int64_t *rows, int64_t *voxels,
+                          int64_t *entries, int32_t *terminal);
+/* the finest operator as a generator (borrowed: valid until mk_box_mg_destroy; do not destroy
+ * it), for mk_gmres_box, mk_expv, mk_shifted_apply */
+mk_status mk_box_mg_generator(mk_box_mg mg, mk_generator *gen);
+/* Level::cycle(b, x) (:817-848): one cycle from the guess x, in place (vectors in HBM) */
+mk_status mk_box_mg_cycle_device(mk_box_mg mg, const double *d_b, double *d_x);
+/* Level::apply(b, x) (:879-880): one cycle from x = 0 */
+mk_status mk_box_mg_apply_device(mk_box_mg mg, const double *d_b, double *d_z);
+/* run() with --method mg (:1180-1212): cycles from the guess x until |b - A x|_1 / shift <
+ * tolerance or max_iterations cycles; history (NULL allowed) receives up to history_cap errors,
+ * the first before any cycle */
+mk_status mk_box_mg_solve(mk_box_mg mg, const double *b, double *x, double tolerance,
+                          int32_t max_iterations, double *history, int64_t history_cap,
+                          mk_box_mg_result *result);
+mk_status mk_box_mg_solve_device(mk_box_mg mg, const double *d_b, double *d_x, double tolerance,
+                                 int32_t max_iterations, double *history, int64_t history_cap,
+                                 mk_box_mg_result *result);
+/* run() with --method mg-gmres (:1200-1209): flexible GMRES on (shift I - Q) x = b preconditioned
+ * by one cycle (standard cycles only; the driver rejects the K-cycle there, :1437-1438). gen is
+ * mk_box_mg_generator's; the driver's tolerance is tolerance * shift / sqrt(rows). */
+mk_status mk_gmres_box(mk_generator gen, double shift, mk_box_mg mg, const double *b, double *x,
+                       const mk_gmres_options *options, mk_solver_result *result);
+mk_status mk_gmres_box_device(mk_generator gen, double shift, mk_box_mg mg, const double *d_b,
+                              double *d_x, const mk_gmres_options *options,
+                              mk_solver_result *result);
+/* The standalone iteration replays one captured cycle (HIP graph) instead of launching its
+ * kernels one by one (default on; MK_BOX_GRAPHS=0 turns it off at start; the K-cycle, which
+ * reads its step lengths on the host, always launches eagerly). Same kernels and arguments:
+ * results are identical. No reference counterpart. */
+mk_status mk_box_mg_graphs_enable(int32_t on);
+
 /* ---- Krylov exponential (finite state projection) -------------------------------------
  * num::expv(t, Q.matrix(), p, krylov_dim, krylov_tol), called at include/markovkit/fsp.hpp:152
  * and :225 (numerics source absent): w = exp(t Q) v for the generator, computed as Sidje's

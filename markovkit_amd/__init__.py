@@ -11,6 +11,8 @@ from .api import (ABSORBING, AGGREGATE_BLOCK, FP32, FP64, ILU0, JACOBI, RED_BLAC
                   Generator, MultigridOptions, PopulationBox, SectorMultigrid, ShiftedGenerator,
                   SolverResult, device_setup_enable, expv, expv_device, fused_tail_enable, tail_lds_enable, gmres, graphs_enable, profile_enable, profile_read,
                   profile_reset, synchronize)
+from . import boxmg
+from .boxmg import BoxMultigrid
 
 __all__ = [
     "Graph", "Grid", "LocalReaction", "ReactionDiffusion", "VoxelDomain", "birth_death",
@@ -18,5 +20,5 @@ __all__ = [
     "REFLECTING", "BatchSolver", "DeviceVector", "Server", "Generator", "MultigridOptions", "PopulationBox",
     "SectorMultigrid", "ShiftedGenerator", "SolverResult", "device_setup_enable", "expv", "expv_device", "fused_tail_enable", "tail_lds_enable", "gmres", "graphs_enable",
     "profile_enable",
-    "profile_read", "profile_reset", "synchronize",
+    "profile_read", "profile_reset", "synchronize", "boxmg", "BoxMultigrid",
 ]

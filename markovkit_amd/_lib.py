@@ -45,6 +45,17 @@ class ExpvResult(C.Structure):
     _fields_ = [("steps", i64), ("rejected", i64), ("error", f64)]
 
 
+class BoxMgParams(C.Structure):
+    _fields_ = [("dimension", i32), ("cells", i32), ("cap", i32), ("diffusion", f64),
+                ("shift", f64), ("c1", f64), ("c2", f64), ("c3", f64), ("c4", f64),
+                ("smoother", i32), ("sweeps", i32), ("cycles", i32), ("cycle_type", i32),
+                ("terminal_rows", i64)]
+
+
+class BoxMgResult(C.Structure):
+    _fields_ = [("iterations", i32), ("converged", i32), ("error", f64), ("history", i64)]
+
+
 class ServerResult(C.Structure):
     _fields_ = [("kind", i32), ("status", i32), ("solve", SolverResult), ("expv", ExpvResult),
                 ("queued_s", f64), ("run_s", f64)]
@@ -91,6 +102,18 @@ SIGNATURES = {
     "mk_batch_destroy": ([P], i32),
     "mk_batch_solve": ([P, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_batch_solve_device": ([P, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
+    "mk_box_mg_create": ([C.POINTER(BoxMgParams), C.POINTER(P)], i32),
+    "mk_box_mg_destroy": ([P], i32),
+    "mk_box_mg_info": ([P, pi64, C.POINTER(i32)], i32),
+    "mk_box_mg_level": ([P, i32, pi64, pi64, pi64, C.POINTER(i32)], i32),
+    "mk_box_mg_generator": ([P, C.POINTER(P)], i32),
+    "mk_box_mg_cycle_device": ([P, P, P], i32),
+    "mk_box_mg_apply_device": ([P, P, P], i32),
+    "mk_box_mg_solve": ([P, P, P, f64, i32, P, i64, C.POINTER(BoxMgResult)], i32),
+    "mk_box_mg_solve_device": ([P, P, P, f64, i32, P, i64, C.POINTER(BoxMgResult)], i32),
+    "mk_gmres_box": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
+    "mk_gmres_box_device": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
+    "mk_box_mg_graphs_enable": ([i32], i32),
     "mk_device_alloc": ([i64, C.POINTER(P)], i32),
     "mk_device_free": ([P], i32),
     "mk_copy_to_device": ([P, P, i64], i32),

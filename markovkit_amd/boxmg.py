@@ -1,0 +1,177 @@
+"""Whole-box multigrid of the reference's paper driver (rdme/cpp/rdme_full_multigrid.cpp) on the
+device: every level holds the complete single-species reaction-diffusion operator of a
+Cartesian grid (Schlogl births and deaths, diffusion, absorbing population cap), coarsened by
+halving the cells and doubling the diffusivity.
+
+Parameters follow the driver's command line (rdme_full_multigrid.cpp:50-63, :1366-1451):
+dimension, cells, cap, diffusion, shift, the Schlogl constants c1..c4 (``linear(s)`` gives the
+driver's --linear s), sweeps, cycles (1 V, 2 W) and kcycle (--cycle-type k). Smoothers:
+"colour-gs" (Gauss-Seidel in four-colour order; the GPU's replacement for the driver's
+sequential lexicographic "gs") and "jacobi" (the driver's, w = 2/3). terminal_rows > 0 solves the
+first level with at most that many states exactly (dense inverse) instead of recursing to the
+one-voxel Thomas solve.
+
+The state order is the population box order (total population first, voxel 0 most
+significant), the driver's Space order: ``markovkit_amd.PopulationBox(cells**dimension, [cap])``
+indexes the same states.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from ._lib import BoxMgParams as _Params
+from ._lib import BoxMgResult as _Result
+from ._lib import GmresOptions as _GmresOptions
+from ._lib import SolverResult as _SolverResult
+from ._lib import check, lib
+from .api import DeviceVector, Generator, SolverResult
+
+SMOOTHERS = {"colour-gs": 0, "jacobi": 1}
+DEFAULT_RATES = (0.0133, 0.0006, 0.6924125, 0.114725)  # rdme_full_multigrid.cpp:57
+
+
+def linear(scale: float = 1.0):
+    """the driver's --linear scale: c1 = c2 = 0, c3 = 2 scale, c4 = scale"""
+    return (0.0, 0.0, 2.0 * scale, scale)
+
+
+class _BorrowedGenerator(Generator):
+    """the hierarchy's finest operator; owned by the hierarchy"""
+
+    def __init__(self, owner, handle):
+        self.box = None
+        self.model = None
+        self._owner = owner
+        self._h = handle
+        r, j = C.c_int64(), C.c_int64()
+        check(lib().mk_generator_size(self._h, C.byref(r), C.byref(j)))
+        self._rows, self._jumps = r.value, j.value
+
+    def __del__(self):
+        self._h = None
+
+
+class BoxMultigrid:
+    def __init__(self, dimension: int = 1, cells: int = 4, cap: int = 40, diffusion: float = 1.0,
+                 shift: float = 0.05, rates=None, smoother: str = "colour-gs", sweeps: int = 2,
+                 cycles: int = 1, kcycle: bool = False, terminal_rows: int = 0):
+        if smoother not in SMOOTHERS:
+            raise ValueError(
+                f"smoother must be one of {sorted(SMOOTHERS)} (the driver's sequential 'gs' and "
+                "'block-ilu' run in four-colour order on the device: use 'colour-gs')")
+        c1, c2, c3, c4 = DEFAULT_RATES if rates is None else rates
+        self._params = _Params(int(dimension), int(cells), int(cap), float(diffusion), float(shift),
+                               float(c1), float(c2), float(c3), float(c4), SMOOTHERS[smoother],
+                               int(sweeps), int(cycles), 1 if kcycle else 0, int(terminal_rows))
+        self._h = C.c_void_p()
+        check(lib().mk_box_mg_create(C.byref(self._params), C.byref(self._h)))
+        rows, levels = C.c_int64(), C.c_int32()
+        check(lib().mk_box_mg_info(self._h, C.byref(rows), C.byref(levels)))
+        self.n = rows.value
+        self._levels = levels.value
+        self.shift = float(shift)
+        self.cap = int(cap)
+        self.voxels = int(cells) ** int(dimension)
+        self.kcycle = bool(kcycle)
+        g = C.c_void_p()
+        check(lib().mk_box_mg_generator(self._h, C.byref(g)))
+        self._gen = _BorrowedGenerator(self, g)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mk_box_mg_destroy(self._h)
+            self._h = None
+
+    def rows(self) -> int:
+        return self.n
+
+    def levels(self) -> int:
+        return self._levels
+
+    def level(self, l: int):
+        """(rows, voxels, stored entries, terminal kind: 0 smoothed, 1 Thomas, 2 dense)"""
+        r, v, e, t = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+        check(lib().mk_box_mg_level(self._h, int(l), C.byref(r), C.byref(v), C.byref(e), C.byref(t)))
+        return r.value, v.value, e.value, t.value
+
+    def generator(self) -> Generator:
+        """the finest operator's generator Q (shift I - Q is the level operator)"""
+        return self._gen
+
+    def initial_problem(self, initial: int):
+        """the driver's first step: b = shift p0, x0 = p0, p0 the state with `initial` molecules
+        dealt round-robin over the voxels (rdme_full_multigrid.cpp:1119-1121)"""
+        from .api import PopulationBox
+        x = np.zeros(self.voxels, np.int32)
+        for k in range(initial):
+            x[k % self.voxels] += 1
+        i = PopulationBox(self.voxels, [self.cap]).find(x)
+        if i is None:
+            raise ValueError("the initial population exceeds the cap")
+        p0 = np.zeros(self.n)
+        p0[i] = 1.0
+        return self.shift * p0, p0
+
+    def cycle(self, b, x):
+        """one cycle of A x = b from the guess x; returns the new x (host arrays)"""
+        db = DeviceVector(self.n, np.ascontiguousarray(b, np.float64))
+        dx = DeviceVector(self.n, np.ascontiguousarray(x, np.float64))
+        self.cycle_device(db, dx)
+        return dx.download()
+
+    def cycle_device(self, b: DeviceVector, x: DeviceVector):
+        check(lib().mk_box_mg_cycle_device(self._h, b.ptr, x.ptr))
+
+    def apply(self, b):
+        """M b: one cycle from x = 0"""
+        db = DeviceVector(self.n, np.ascontiguousarray(b, np.float64))
+        dz = DeviceVector(self.n)
+        check(lib().mk_box_mg_apply_device(self._h, db.ptr, dz.ptr))
+        return dz.download()
+
+    def solve(self, b, x0, tolerance: float = 1e-9, max_iterations: int = 1000):
+        """the driver's standalone iteration: (x, iterations, error, history)"""
+        x = np.array(x0, np.float64)
+        hist = np.zeros(max_iterations + 2)
+        r = _Result()
+        check(lib().mk_box_mg_solve(self._h, np.ascontiguousarray(b, np.float64).ctypes.data_as(C.c_void_p),
+                                    x.ctypes.data_as(C.c_void_p), float(tolerance), int(max_iterations),
+                                    hist.ctypes.data_as(C.c_void_p), len(hist), C.byref(r)))
+        return x, r.iterations, r.error, hist[:min(r.history, len(hist))]
+
+    def solve_device(self, b: DeviceVector, x: DeviceVector, tolerance: float = 1e-9,
+                     max_iterations: int = 1000):
+        """the same with b, x in HBM: (iterations, error, converged)"""
+        r = _Result()
+        check(lib().mk_box_mg_solve_device(self._h, b.ptr, x.ptr, float(tolerance), int(max_iterations),
+                                           None, 0, C.byref(r)))
+        return r.iterations, r.error, bool(r.converged)
+
+    def gmres(self, b, x, tolerance: float, max_iterations: int = 1000, restart: int = 30,
+              orthogonalization: str = "dcgs2") -> SolverResult:
+        """flexible GMRES on (shift I - Q) x = b preconditioned by one cycle (the driver's
+        mg-gmres; its tolerance is tolerance * shift / sqrt(rows)); x (numpy array or
+        DeviceVector) is the guess on input"""
+        from .api import ORTHOGONALIZATION
+        opts = _GmresOptions(float(tolerance), int(max_iterations), int(restart),
+                             ORTHOGONALIZATION[orthogonalization])
+        res = _SolverResult()
+        g = self._gen._h
+        if isinstance(b, DeviceVector):
+            check(lib().mk_gmres_box_device(g, self.shift, self._h, b.ptr, x.ptr, C.byref(opts),
+                                            C.byref(res)))
+        else:
+            ba = np.ascontiguousarray(b, np.float64)
+            if x.dtype != np.float64 or not x.flags.c_contiguous:
+                raise TypeError("x must be a contiguous float64 array")
+            check(lib().mk_gmres_box(g, self.shift, self._h, ba.ctypes.data_as(C.c_void_p),
+                                     x.ctypes.data_as(C.c_void_p), C.byref(opts), C.byref(res)))
+        return SolverResult(res.iterations, res.residual, bool(res.converged))
+
+
+def graphs_enable(on: bool) -> None:
+    """replay the standalone iteration's cycle from a captured HIP graph (default on)"""
+    check(lib().mk_box_mg_graphs_enable(1 if on else 0))

@@ -28,7 +28,7 @@ CXX_FLAGS = ["-O3", "-std=c++20", "-fPIC", "-march=x86-64-v3", "-ffp-contract=of
              "-Wextra", f"-I{ROOT}/include"]
 
 HIP_SOURCES = ["kernels.hip", "batch_kernels.hip", "api.hip", "batch.hip", "expv.hip", "server.hip",
-               "assemble.hip"]
+               "assemble.hip", "boxmg.hip"]
 CXX_SOURCES = ["setup.cpp"]
 
 

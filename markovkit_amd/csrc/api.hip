@@ -796,7 +796,7 @@ double device_norm2(const double *partial, int nb, double *hd) {
 // q_j and forms the next candidate with its projections. Because w_j = A z_j is expanded as
 // Q (t + s_{j+1}) + beta_{j+1} q_{j+1}, column j of H is (t + s_{j+1}, beta_{j+1}) as in CGS2.
 template <typename ZT>
-mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, const double *b,
+mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const double *b,
                         double *x, const mk_gmres_options &o) {
     const i64 n = A.n;
     const i64 m = std::max<i64>(1, std::min<i64>(o.restart, n));
@@ -834,7 +834,7 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
     auto operator_image = [&](i64 j, const double *v, const double *scale) {
         if (M) {
             ZT *Zj = Z + j * ld;
-            mg_apply_device(*M, v, Zj, scale);
+            M->apply(v, Zj, scale);
             P.run(K_SPMV, op_bytes(A.A, sizeof(ZT), 8, 1), [&] {
                 gpu::shifted_apply<ZT>(A.A.view, shift, Zj, nullptr, w, nullptr, s);
             });
@@ -1017,7 +1017,18 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, mk_sector_mg_s *M, cons
     return res;
 }
 
-mk_solver_result gmres_device(mk_generator_s &A, double shift, mk_sector_mg_s *M, const double *b,
+mk_solver_result gmres_device(mk_generator_s &A, double shift, mk_sector_mg_s *mg, const double *b,
+                              double *x, const mk_gmres_options &o) {
+    if (!mg) return mk::rt::gmres_device(A, shift, static_cast<const precond *>(nullptr), b, x, o);
+    precond M{mg->uid, mg->rows, mg->shift, mg->precision,
+              [mg](const double *v, void *z, const double *scale) { mg_apply_device(*mg, v, z, scale); }};
+    return mk::rt::gmres_device(A, shift, &M, b, x, o);
+}
+
+} // namespace
+
+namespace mk::rt {
+mk_solver_result gmres_device(mk_generator_s &A, double shift, const precond *M, const double *b,
                               double *x, const mk_gmres_options &o) {
     if (M) {
         require(M->rows == A.n, "gmres: the preconditioner does not match the operator");
@@ -1027,8 +1038,7 @@ mk_solver_result gmres_device(mk_generator_s &A, double shift, mk_sector_mg_s *M
     }
     return fgmres<double>(A, shift, nullptr, b, x, o);
 }
-
-} // namespace
+} // namespace mk::rt
 
 namespace {
 // re-invert the terminal blocks of every level for another shift (the level operators and
@@ -1053,6 +1063,21 @@ void reshift_levels(std::vector<level_dev<T>> &L, double shift) {
     }
 }
 } // namespace
+
+namespace mk::rt {
+void finish_generator(mk_generator_s &G, mk::generator_csr &&g) {
+    G.n = g.n;
+    G.n_jumps = g.n_jumps;
+    // ||Q||_1 = max_j (exit_j + sum of j's in-space jump rates)
+    std::vector<double> out(g.exit.begin(), g.exit.end());
+    for (i64 i = 0; i < g.n; ++i)
+        for (u64 p = g.ptr[i]; p < g.ptr[i + 1]; ++p) out[g.col[p]] += g.val[p];
+    G.norm1 = out.empty() ? 0.0 : *std::max_element(out.begin(), out.end());
+    G.A = upload_csr<double>(g);
+    G.exit = std::move(g.exit);
+    G.leak = std::move(g.leak);
+}
+} // namespace mk::rt
 
 // ====================================================================== C ABI
 extern "C" {
@@ -1180,20 +1205,7 @@ mk_status mk_domain_destroy(mk_domain d) {
     return MK_OK;
 }
 
-namespace {
-void finish_generator(mk_generator_s &G, mk::generator_csr &&g) {
-    G.n = g.n;
-    G.n_jumps = g.n_jumps;
-    // ||Q||_1 = max_j (exit_j + sum of j's in-space jump rates)
-    std::vector<double> out(g.exit.begin(), g.exit.end());
-    for (i64 i = 0; i < g.n; ++i)
-        for (u64 p = g.ptr[i]; p < g.ptr[i + 1]; ++p) out[g.col[p]] += g.val[p];
-    G.norm1 = out.empty() ? 0.0 : *std::max_element(out.begin(), out.end());
-    G.A = upload_csr<double>(g);
-    G.exit = std::move(g.exit);
-    G.leak = std::move(g.leak);
-}
-} // namespace
+
 
 mk_status mk_generator_create_rdme(mk_box box, mk_domain domain, int32_t n_reactions,
                                    const int32_t *changes, const double *rates, int32_t boundary,

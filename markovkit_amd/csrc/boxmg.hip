@@ -1,0 +1,862 @@
+// Whole-box multigrid on the device: the hierarchy of the reference's paper driver
+// rdme/cpp/rdme_full_multigrid.cpp:298-880, with every level holding the complete
+// reaction-diffusion operator of a single species on a Cartesian grid (births and deaths of the
+// Schlogl family, diffusion, absorbing population cap).
+//
+// Levels. Level l has cells / 2^l cells per axis, diffusivity D 2^l and cell volume 2^(dl) / V;
+// its operator is the RDME generator of that coarse grid with the rates re-evaluated at the
+// coarse volume (assembled by the library's generator assembly, on the device for levels of
+// 4096 states or more). The driver's exact bottom solve is the Thomas algorithm on the one-voxel
+// level; a level with at most terminal_rows states may instead be solved exactly with a dense
+// inverse, which ends the recursion there (the W-cycle's many visits of tiny levels cost launch
+// latency on the GPU, not work).
+//
+// Smoothers. Lexicographic Gauss-Seidel and the population-block ILU of the driver are
+// sequential; the GPU runs Gauss-Seidel in four-colour order instead. The colour of a state is
+// 2 (total mod 2) + (sum over molecules of the coordinate sum of their voxel, mod 2): a diffusion
+// jump changes the second bit and a birth or death the first, so a colour's rows only read the
+// other colours and one launch updates them all. Each level's rows are stored colour by colour
+// (ascending state index within a colour), so a colour pass streams contiguous rows; the finest
+// level's vectors are permuted on entry and exit. Weighted Jacobi (w = 2/3) is the driver's own.
+//
+// Cycle (rdme_full_multigrid.cpp:817-878): pre-smoothing (colours 0..3), residual, restriction
+// by summing children, the coarse correction (V: one coarse cycle, W: two, the exact bottom
+// solve once; or the K-cycle's at most two GCR steps), prolongation with the multinomial
+// weights, post-smoothing (colours 3..0). The standalone iteration of the driver's run()
+// repeats cycles until |b - A x|_1 / shift < tolerance; without the K-cycle one cycle is
+// captured in a HIP graph and replayed.
+#include "runtime.hpp"
+
+#include <cmath>
+#include <numeric>
+
+using namespace mk::rt;
+
+namespace mk::gpu {
+namespace {
+
+constexpr int kBoxThreads = 256;
+constexpr int kDenseChunk = 64; // columns of the dense inverse per workgroup
+
+// the one-voxel level: forward elimination with the stored multipliers, back substitution
+__global__ void k_box_thomas(i64 n, const double *lower, const double *upper, const double *pivot,
+                             const double *b, double *x) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    x[0] = b[0];
+    for (i64 i = 1; i < n; ++i) x[i] = b[i] - lower[i] * x[i - 1];
+    x[n - 1] /= pivot[n - 1];
+    for (i64 i = n - 1; i > 0; --i) x[i - 1] = (x[i - 1] - upper[i - 1] * x[i]) / pivot[i - 1];
+}
+
+// partial[c][i] = sum over the columns j of chunk c of inv[j][i] b[j] (inverse column-major:
+// the lanes of a wavefront read consecutive rows of one column)
+__global__ void __launch_bounds__(kBoxThreads)
+    k_box_dense(i64 n, const double *inv, const double *b, double *partial) {
+    const i64 i = blockIdx.x * static_cast<i64>(kBoxThreads) + threadIdx.x;
+    const i64 j0 = static_cast<i64>(blockIdx.y) * kDenseChunk;
+    const i64 j1 = j0 + kDenseChunk < n ? j0 + kDenseChunk : n;
+    if (i >= n) return;
+    double s = 0.0;
+    for (i64 j = j0; j < j1; ++j) s += inv[j * n + i] * b[j];
+    partial[static_cast<i64>(blockIdx.y) * n + i] = s;
+}
+
+__global__ void __launch_bounds__(kBoxThreads)
+    k_box_dense_sum(i64 n, int chunks, const double *partial, double *x) {
+    const i64 i = blockIdx.x * static_cast<i64>(kBoxThreads) + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < chunks; ++c) s += partial[static_cast<i64>(c) * n + i];
+    x[i] = s;
+}
+
+// partial[block] = sum of |r| over the block's grid-stride share
+__global__ void __launch_bounds__(kBoxThreads) k_box_abs_partial(i64 n, const double *r, double *partial) {
+    __shared__ double red[kBoxThreads / 64];
+    double s = 0.0;
+    for (i64 i = blockIdx.x * static_cast<i64>(kBoxThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kBoxThreads)
+        s += fabs(r[i]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBoxThreads / 64; ++w) t += red[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(kBoxThreads) k_box_copy(i64 n, const double *src, double *dst) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kBoxThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kBoxThreads)
+        dst[i] = src[i];
+}
+
+int grid_of(i64 n) {
+    const i64 g = (n + kBoxThreads - 1) / kBoxThreads;
+    return static_cast<int>(std::max<i64>(1, std::min<i64>(g, 65535)));
+}
+
+} // namespace
+
+void box_thomas(i64 n, const double *lower, const double *upper, const double *pivot,
+                const double *b, double *x, hipStream_t s) {
+    hipLaunchKernelGGL(k_box_thomas, dim3(1), dim3(64), 0, s, n, lower, upper, pivot, b, x);
+}
+
+void box_dense(i64 n, const double *inv, const double *b, double *partial, double *x, hipStream_t s) {
+    const int chunks = static_cast<int>((n + kDenseChunk - 1) / kDenseChunk);
+    const int rows = static_cast<int>((n + kBoxThreads - 1) / kBoxThreads);
+    hipLaunchKernelGGL(k_box_dense, dim3(rows, chunks), dim3(kBoxThreads), 0, s, n, inv, b, partial);
+    hipLaunchKernelGGL(k_box_dense_sum, dim3(rows), dim3(kBoxThreads), 0, s, n, chunks, partial, x);
+}
+
+void box_abs_partial(i64 n, const double *r, double *partial, int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_box_abs_partial, dim3(nb), dim3(kBoxThreads), 0, s, n, r, partial);
+}
+
+void box_copy(i64 n, const double *src, double *dst, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_box_copy, dim3(grid_of(n)), dim3(kBoxThreads), 0, s, n, src, dst);
+}
+
+} // namespace mk::gpu
+
+namespace mk::rt {
+
+// one level of the device hierarchy
+struct box_level {
+    i64 n = 0;
+    i64 voxels = 0;
+    int terminal = 0;  // 0 smoothed, 1 Thomas, 2 dense inverse
+    u32 cb[5] = {};    // colour c holds rows cb[c] .. cb[c + 1]
+    dev_csr<double> A; // the level operator in colour order (residual, Jacobi)
+    dev_csr<double> Ac[4];
+    dbuf rowmap;       // 0 .. n - 1; colour c's operator maps its local rows from rowmap + cb[c]
+    // transfer to the next level, in the two levels' row orders
+    dbuf parent, weight, cptr, cidx;
+    dbuf b, x, r, tmp; // b, x of coarse levels; r the residual; tmp the Jacobi target
+    dbuf lower, upper, pivot, inv, partial;
+    dbuf kv1, kw1, kv2, kw2, kr; // K-cycle vectors (the next level's size)
+    u64 nnz = 0;
+};
+
+} // namespace mk::rt
+
+struct mk_box_mg_s {
+    mk::u64 uid = mk::rt::next_uid();
+    mk_box_mg_params p{};
+    mk::i64 rows = 0;
+    std::vector<mk::rt::box_level> L;
+    mk::rt::dbuf perm0;            // level-0 row r holds the state perm0[r] of the box order
+    mk::rt::dbuf b0, x0;           // the finest level's vectors in its row order
+    mk::rt::dbuf red, scal;        // reduction partials, device scalars
+    double *host = nullptr;        // pinned copy of one scalar
+    std::unique_ptr<mk_generator_s> gen; // the finest operator in the box order
+    mk::rt::graph_cache graphs;
+    ~mk_box_mg_s() {
+        graphs.clear();
+        if (host) (void)hipHostFree(host);
+    }
+};
+
+namespace mk::rt {
+namespace {
+
+// MK_BOX_GRAPHS=0 launches the standalone iteration's cycles eagerly
+std::atomic<int> &box_graph_switch() {
+    static std::atomic<int> on{[] {
+        const char *env = std::getenv("MK_BOX_GRAPHS");
+        return (env && env[0] == '0') ? 0 : 1;
+    }()};
+    return on;
+}
+
+struct host_level {
+    int cells = 0;
+    double diffusion = 0.0;
+    mk::population_space space;
+    mk::voxel_domain dom;
+    mk::generator_csr g;            // box order
+    std::vector<std::uint8_t> colour;
+    std::vector<u32> ord, pos;      // row r = state ord[r]; pos[state] = r
+    std::vector<i64> parent;        // box order of the next level
+    std::vector<double> weight;
+    int terminal = 0;
+};
+
+// the driver's Schlogl rate tables (rdme_full_multigrid.cpp:410-412): births +1, deaths -1,
+// rates[(r V + v) T + m] for m molecules in voxel v
+mk::reaction_tables schlogl_tables(const mk_box_mg_params &p, i64 V) {
+    mk::reaction_tables t;
+    t.n = 2;
+    t.changes = {1, -1};
+    t.stride = {1};
+    t.T = p.cap + 1;
+    t.rates.assign(static_cast<std::size_t>(2 * V * t.T), 0.0);
+    const double vol = 1.0 / static_cast<double>(V);
+    for (i64 v = 0; v < V; ++v)
+        for (i64 m = 0; m <= p.cap; ++m) {
+            const double x = static_cast<double>(m);
+            t.rates[static_cast<std::size_t>(v * t.T + m)] = p.c3 * vol + p.c1 * x * (x - 1) / (2 * vol);
+            t.rates[static_cast<std::size_t>((V + v) * t.T + m)] =
+                p.c4 * x + p.c2 * x * (x - 1) * (x - 2) / (6 * vol * vol);
+        }
+    return t;
+}
+
+i64 voxels_of(const mk_box_mg_params &p, int cells) {
+    i64 v = 1;
+    for (int d = 0; d < p.dimension; ++d) v *= cells;
+    return v;
+}
+
+// sum over axes of the voxel's coordinate (axis 0 fastest)
+std::vector<int> coordinate_sums(int dim, int cells, i64 V) {
+    std::vector<int> out(V, 0);
+    for (i64 v = 0; v < V; ++v) {
+        i64 rest = v;
+        for (int d = 0; d < dim; ++d) {
+            out[v] += static_cast<int>(rest % cells);
+            rest /= cells;
+        }
+    }
+    return out;
+}
+
+void build_level_operator(const mk_box_mg_params &p, host_level &h) {
+    const i64 V = voxels_of(p, h.cells);
+    h.space = mk::population_space::box(V, {p.cap});
+    std::vector<i64> dims(p.dimension, h.cells);
+    std::vector<double> spacing(p.dimension, 1.0 / h.cells);
+    h.dom = mk::voxel_domain::grid(dims, spacing, nullptr, {h.diffusion});
+    const auto tables = schlogl_tables(p, V);
+    if (!(device_setup_switch().load() != 0 && h.space.size() >= 4096 &&
+          assemble_csr_device(h.space, h.dom, tables, true, h.g)))
+        h.g = mk::assemble_generator(h.space, h.dom, tables, true);
+}
+
+// colours of every state and the colour-major order; throws when a stored coupling joins two
+// states of one colour (a reaction that does not change the population by one)
+void colour_order(const mk_box_mg_params &p, host_level &h, bool check) {
+    const i64 n = h.g.n, V = h.space.voxels();
+    const auto csum = coordinate_sums(p.dimension, h.cells, V);
+    h.colour.assign(n, 0);
+    const i64 chunk = 1 << 14;
+    mk::parallel_tasks((n + chunk - 1) / chunk, [&](i64 t) {
+        std::vector<int> x(V);
+        for (i64 i = t * chunk; i < std::min(n, (t + 1) * chunk); ++i) {
+            h.space.state(i, x.data());
+            int total = 0, parity = 0;
+            for (i64 v = 0; v < V; ++v) {
+                total += x[v];
+                parity += x[v] * csum[v];
+            }
+            h.colour[i] = static_cast<std::uint8_t>(((total & 1) << 1) | (parity & 1));
+        }
+    });
+    if (check) {
+        std::atomic<int> bad{0};
+        mk::parallel_tasks((n + chunk - 1) / chunk, [&](i64 t) {
+            for (i64 i = t * chunk; i < std::min(n, (t + 1) * chunk); ++i)
+                for (u64 q = h.g.ptr[i]; q < h.g.ptr[i + 1]; ++q)
+                    if (h.colour[h.g.col[q]] == h.colour[i]) bad.store(1);
+        });
+        if (bad.load())
+            throw std::invalid_argument(
+                "box multigrid: a coupling joins two states of one colour; the four-colour "
+                "Gauss-Seidel needs reactions that change the population by one");
+    }
+    h.ord.resize(n);
+    h.pos.resize(n);
+    u32 at = 0;
+    for (int c = 0; c < 4; ++c)
+        for (i64 i = 0; i < n; ++i)
+            if (h.colour[i] == c) h.ord[at++] = static_cast<u32>(i);
+    for (i64 r = 0; r < n; ++r) h.pos[h.ord[r]] = static_cast<u32>(r);
+}
+
+// the multinomial transfer of fine level f to level c (rdme_full_multigrid.cpp:532-557):
+// the coarse state adds the counts of each 2^d block of voxels; weight = exp(log of the
+// multinomial probability of the fine arrangement), normalised over the coarse state's children
+// in ascending fine order
+void build_transfer(const mk_box_mg_params &p, host_level &f, const host_level &c) {
+    const i64 V = f.space.voxels(), Vc = c.space.voxels(), n = f.g.n;
+    std::vector<i64> vparent(V);
+    for (i64 v = 0; v < V; ++v) {
+        i64 rest = v, stride = 1, cv = 0;
+        for (int d = 0; d < p.dimension; ++d) {
+            cv += ((rest % f.cells) / 2) * stride;
+            rest /= f.cells;
+            stride *= c.cells;
+        }
+        vparent[v] = cv;
+    }
+    std::vector<double> lf(p.cap + 1);
+    for (int k = 0; k <= p.cap; ++k) lf[k] = std::lgamma(k + 1.0);
+    const double log_children = p.dimension * std::log(2.0);
+    f.parent.assign(n, 0);
+    f.weight.assign(n, 0.0);
+    const i64 chunk = 1 << 14;
+    mk::parallel_tasks((n + chunk - 1) / chunk, [&](i64 t) {
+        std::vector<int> x(V), cn(Vc);
+        for (i64 i = t * chunk; i < std::min(n, (t + 1) * chunk); ++i) {
+            f.space.state(i, x.data());
+            std::fill(cn.begin(), cn.end(), 0);
+            int total = 0;
+            double lw = 0.0;
+            for (i64 v = 0; v < V; ++v) total += x[v];
+            lw = -total * log_children;
+            for (i64 v = 0; v < V; ++v) {
+                cn[vparent[v]] += x[v];
+                lw -= lf[x[v]];
+            }
+            for (int k : cn) lw += lf[k];
+            const i64 j = c.space.find(cn.data());
+            if (j < 0) throw std::logic_error("box multigrid: coarse state outside the coarse box");
+            f.parent[i] = j;
+            f.weight[i] = std::exp(lw);
+        }
+    });
+    std::vector<double> sums(c.g.n, 0.0);
+    for (i64 i = 0; i < n; ++i) sums[f.parent[i]] += f.weight[i];
+    for (i64 i = 0; i < n; ++i) f.weight[i] /= sums[f.parent[i]];
+}
+
+// the operator with rows and columns in the level's row order (entries ascending by column)
+mk::generator_csr permuted(const host_level &h) {
+    const i64 n = h.g.n;
+    mk::generator_csr out;
+    out.n = n;
+    out.n_jumps = h.g.n_jumps;
+    out.ptr.assign(n + 1, 0);
+    for (i64 r = 0; r < n; ++r) out.ptr[r + 1] = out.ptr[r] + (h.g.ptr[h.ord[r] + 1] - h.g.ptr[h.ord[r]]);
+    out.col.resize(out.ptr[n]);
+    out.val.resize(out.ptr[n]);
+    out.exit.resize(n);
+    out.leak.resize(n);
+    const i64 chunk = 1 << 14;
+    mk::parallel_tasks((n + chunk - 1) / chunk, [&](i64 t) {
+        std::vector<std::pair<u32, double>> e;
+        for (i64 r = t * chunk; r < std::min(n, (t + 1) * chunk); ++r) {
+            const i64 i = h.ord[r];
+            e.clear();
+            for (u64 q = h.g.ptr[i]; q < h.g.ptr[i + 1]; ++q) e.push_back({h.pos[h.g.col[q]], h.g.val[q]});
+            std::sort(e.begin(), e.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+            u64 at = out.ptr[r];
+            for (auto &[c, v] : e) {
+                out.col[at] = c;
+                out.val[at++] = v;
+            }
+            out.exit[r] = h.g.exit[i];
+            out.leak[r] = h.g.leak[i];
+        }
+    });
+    return out;
+}
+
+// rows [r0, r1) of g (columns unchanged)
+mk::generator_csr row_range(const mk::generator_csr &g, i64 r0, i64 r1) {
+    mk::generator_csr out;
+    out.n = r1 - r0;
+    out.ptr.assign(out.n + 1, 0);
+    for (i64 r = r0; r < r1; ++r) out.ptr[r - r0 + 1] = g.ptr[r + 1] - g.ptr[r0];
+    out.col.assign(g.col.begin() + g.ptr[r0], g.col.begin() + g.ptr[r1]);
+    out.val.assign(g.val.begin() + g.ptr[r0], g.val.begin() + g.ptr[r1]);
+    out.exit.assign(g.exit.begin() + r0, g.exit.begin() + r1);
+    out.leak.assign(g.leak.begin() + r0, g.leak.begin() + r1);
+    return out;
+}
+
+void build(mk_box_mg_s &M) {
+    const auto &p = M.p;
+    std::vector<host_level> H;
+    // levels down to the terminal one
+    int cells = p.cells;
+    double diffusion = p.diffusion;
+    while (true) {
+        host_level h;
+        h.cells = cells;
+        h.diffusion = diffusion;
+        build_level_operator(p, h);
+        const bool dense = p.terminal_rows > 0 && h.g.n <= p.terminal_rows;
+        h.terminal = dense ? 2 : (cells == 1 ? 1 : 0);
+        if (h.terminal) {
+            h.ord.resize(h.g.n);
+            std::iota(h.ord.begin(), h.ord.end(), 0u);
+            h.pos = h.ord;
+        } else {
+            if (cells % 2) throw std::invalid_argument("box multigrid: cells must be a power of two");
+            colour_order(p, h, p.smoother == MK_BOX_GS);
+        }
+        H.push_back(std::move(h));
+        if (H.back().terminal) break;
+        cells /= 2;
+        diffusion *= 2;
+    }
+    for (std::size_t l = 0; l + 1 < H.size(); ++l) build_transfer(p, H[l], H[l + 1]);
+
+    const hipStream_t s = stream();
+    M.rows = H[0].g.n;
+    M.L.resize(H.size());
+    for (std::size_t l = 0; l < H.size(); ++l) {
+        auto &h = H[l];
+        auto &d = M.L[l];
+        const i64 n = h.g.n;
+        d.n = n;
+        d.voxels = h.space.voxels();
+        d.terminal = h.terminal;
+        d.nnz = h.g.ptr.back();
+        if (l > 0) {
+            d.b = dbuf(n * sizeof(double));
+            d.x = dbuf(n * sizeof(double));
+        }
+        if (h.terminal == 1) {
+            // Thomas coefficients of the tridiagonal one-voxel chain (rdme_full_multigrid.cpp:
+            // 512-527) from A = (shift + exit) - rates
+            std::vector<double> lo(n, 0.0), up(n, 0.0), pv(n, 0.0);
+            for (i64 i = 0; i < n; ++i) {
+                pv[i] = p.shift + h.g.exit[i];
+                for (u64 q = h.g.ptr[i]; q < h.g.ptr[i + 1]; ++q) {
+                    if (static_cast<i64>(h.g.col[q]) + 1 == i) lo[i] = -h.g.val[q];
+                    else if (static_cast<i64>(h.g.col[q]) == i + 1) up[i] = -h.g.val[q];
+                    else throw std::logic_error("box multigrid: one-voxel level not tridiagonal");
+                }
+                if (i > 0) {
+                    lo[i] /= pv[i - 1];
+                    pv[i] -= lo[i] * up[i - 1];
+                }
+            }
+            d.lower = upload(lo);
+            d.upper = upload(up);
+            d.pivot = upload(pv);
+            continue;
+        }
+        if (h.terminal == 2) {
+            std::vector<double> a(static_cast<std::size_t>(n) * n, 0.0), inv;
+            for (i64 i = 0; i < n; ++i) {
+                a[static_cast<std::size_t>(i) * n + i] = p.shift + h.g.exit[i];
+                for (u64 q = h.g.ptr[i]; q < h.g.ptr[i + 1]; ++q)
+                    a[static_cast<std::size_t>(i) * n + h.g.col[q]] -= h.g.val[q];
+            }
+            if (!mk::invert_dense(std::move(a), n, inv))
+                throw std::runtime_error("box multigrid: singular terminal level");
+            std::vector<double> colmajor(inv.size());
+            for (i64 i = 0; i < n; ++i)
+                for (i64 j = 0; j < n; ++j)
+                    colmajor[static_cast<std::size_t>(j) * n + i] = inv[static_cast<std::size_t>(i) * n + j];
+            d.inv = upload(colmajor);
+            d.partial = dbuf(static_cast<std::size_t>((n + 63) / 64) * n * sizeof(double));
+            continue;
+        }
+        const auto gp = permuted(h);
+        for (int c = 0; c < 4; ++c) {
+            d.cb[c + 1] = d.cb[c];
+            while (d.cb[c + 1] < n && h.colour[h.ord[d.cb[c + 1]]] == c) ++d.cb[c + 1];
+        }
+        if (d.cb[4] != n) throw std::logic_error("box multigrid: colour order");
+        std::vector<u32> rm(n);
+        std::iota(rm.begin(), rm.end(), 0u);
+        d.rowmap = upload(rm);
+        for (int c = 0; c < 4; ++c) {
+            if (d.cb[c + 1] == d.cb[c]) continue;
+            d.Ac[c] = upload_csr<double>(row_range(gp, d.cb[c], d.cb[c + 1]));
+            d.Ac[c].view.rowmap = dp<u32>(d.rowmap) + d.cb[c];
+        }
+        d.A = upload_csr<double>(gp);
+        d.r = dbuf(n * sizeof(double));
+        if (p.smoother == MK_BOX_JACOBI) d.tmp = dbuf(n * sizeof(double));
+        // transfer in row orders: parent'[pos_f[i]] = pos_c[parent[i]]
+        const auto &hc = H[l + 1];
+        const i64 nc = hc.g.n;
+        std::vector<u32> par(n);
+        std::vector<double> w(n);
+        for (i64 i = 0; i < n; ++i) {
+            par[h.pos[i]] = hc.pos[h.parent[i]];
+            w[h.pos[i]] = h.weight[i];
+        }
+        std::vector<u32> cptr(nc + 1, 0), cidx(n);
+        for (i64 r = 0; r < n; ++r) ++cptr[par[r] + 1];
+        for (i64 c = 0; c < nc; ++c) cptr[c + 1] += cptr[c];
+        {
+            std::vector<u32> fill(cptr.begin(), cptr.end() - 1);
+            for (i64 r = 0; r < n; ++r) cidx[fill[par[r]]++] = static_cast<u32>(r);
+        }
+        d.parent = upload(par);
+        d.weight = upload(w);
+        d.cptr = upload(cptr);
+        d.cidx = upload(cidx);
+        if (p.cycle_type == MK_CYCLE_K && !hc.terminal) {
+            for (dbuf *v : {&d.kv1, &d.kw1, &d.kv2, &d.kw2, &d.kr}) *v = dbuf(nc * sizeof(double));
+        }
+    }
+    M.perm0 = upload(H[0].ord);
+    M.b0 = dbuf(M.rows * sizeof(double));
+    M.x0 = dbuf(M.rows * sizeof(double));
+    M.red = dbuf(static_cast<std::size_t>(gpu::reduction_blocks(M.rows)) * sizeof(double));
+    M.scal = dbuf(8 * sizeof(double));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&M.host), 8 * sizeof(double)));
+    M.gen = std::make_unique<mk_generator_s>();
+    finish_generator(*M.gen, std::move(H[0].g));
+    (void)s;
+}
+
+// ------------------------------------------------------------------ cycle
+int level_class(std::size_t l, int fine) { return l == 0 ? fine : K_COARSE; }
+
+double dot(mk_box_mg_s &M, const double *a, const double *b, i64 n) {
+    const hipStream_t s = stream();
+    const int nb = gpu::reduction_blocks(n);
+    prof().run(K_DOT, 16.0 * n, [&] {
+        gpu::multi_dot(a, n, 1, b, n, dp<double>(M.red), nb, s);
+        gpu::reduce_partials(dp<double>(M.red), nb, 1, dp<double>(M.scal), s);
+    });
+    HIPCHK(hipMemcpyAsync(M.host, M.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    drain();
+    return M.host[0];
+}
+
+void smooth(mk_box_mg_s &M, std::size_t l, const double *b, double *x, bool reverse) {
+    auto &lv = M.L[l];
+    auto &P = prof();
+    const hipStream_t s = stream();
+    const double shift = M.p.shift;
+    for (int sweep = 0; sweep < M.p.sweeps; ++sweep) {
+        if (M.p.smoother == MK_BOX_GS) {
+            for (int k = 0; k < 4; ++k) {
+                const int c = reverse ? 3 - k : k;
+                if (lv.cb[c + 1] == lv.cb[c]) continue;
+                P.run(level_class(l, K_SWEEP), op_bytes(lv.Ac[c], 8, 8, 1) + (lv.cb[c + 1] - lv.cb[c]) * 4.0,
+                      [&] { gpu::gs_half<double>(lv.Ac[c].view, shift, b, x, nullptr, s); });
+            }
+        } else {
+            double *t = dp<double>(lv.tmp);
+            P.run(level_class(l, K_SWEEP), op_bytes(lv.A, 8, 8, 2),
+                  [&] { gpu::jacobi_sweep<double>(lv.A.view, shift, 2.0 / 3.0, b, x, t, s); });
+            P.run(level_class(l, K_OTHER), lv.n * 16.0, [&] { gpu::box_copy(lv.n, t, x, s); });
+        }
+    }
+}
+
+void cycle(mk_box_mg_s &M, std::size_t l, const double *b, double *x);
+
+// K-cycle coarse solve of level l + 1 (rdme_full_multigrid.cpp:856-878)
+void krylov_coarse(mk_box_mg_s &M, std::size_t l) {
+    auto &lv = M.L[l];
+    auto &c = M.L[l + 1];
+    auto &P = prof();
+    const hipStream_t s = stream();
+    const i64 n = c.n;
+    const double shift = M.p.shift;
+    const double *g = dp<double>(c.b);
+    double *xc = dp<double>(c.x);
+    double *v1 = dp<double>(lv.kv1), *w1 = dp<double>(lv.kw1), *v2 = dp<double>(lv.kv2),
+           *w2 = dp<double>(lv.kw2), *r = dp<double>(lv.kr);
+    const double gg = dot(M, g, g, n);
+    if (!(gg > 0.0)) return;
+    auto apply_coarse = [&](const double *in, double *v, double *w) {
+        P.run(K_COARSE, n * 8.0, [&] { gpu::fill<double>(n, 0.0, v, s); });
+        cycle(M, l + 1, in, v);
+        P.run(K_COARSE, op_bytes(c.A, 8, 8, 1),
+              [&] { gpu::shifted_apply<double>(c.A.view, shift, v, nullptr, w, nullptr, s); });
+    };
+    apply_coarse(g, v1, w1);
+    const double ww1 = dot(M, w1, w1, n);
+    if (!(ww1 > 0.0)) return;
+    const double a1 = dot(M, w1, g, n) / ww1;
+    P.run(K_COARSE, n * 48.0, [&] {
+        gpu::axpy<double>(n, a1, v1, xc, s);
+        gpu::box_copy(n, g, r, s);
+        gpu::axpy<double>(n, -a1, w1, r, s);
+    });
+    if (dot(M, r, r, n) <= 0.0625 * gg) return;
+    apply_coarse(r, v2, w2);
+    const double beta = dot(M, w2, w1, n) / ww1;
+    P.run(K_COARSE, n * 48.0, [&] {
+        gpu::axpy<double>(n, -beta, v1, v2, s);
+        gpu::axpy<double>(n, -beta, w1, w2, s);
+    });
+    const double ww2 = dot(M, w2, w2, n);
+    if (!(ww2 > 0.0)) return;
+    const double a2 = dot(M, w2, r, n) / ww2;
+    P.run(K_COARSE, n * 24.0, [&] { gpu::axpy<double>(n, a2, v2, xc, s); });
+}
+
+// one cycle of A_l x = b from the guess x (level l's row order)
+void cycle(mk_box_mg_s &M, std::size_t l, const double *b, double *x) {
+    auto &lv = M.L[l];
+    auto &P = prof();
+    const hipStream_t s = stream();
+    if (lv.terminal == 1) {
+        P.run(K_TERMINAL, lv.n * 40.0, [&] {
+            gpu::box_thomas(lv.n, dp<double>(lv.lower), dp<double>(lv.upper), dp<double>(lv.pivot), b, x, s);
+        });
+        return;
+    }
+    if (lv.terminal == 2) {
+        P.run(K_TERMINAL, double(lv.n) * lv.n * 8.0, [&] {
+            gpu::box_dense(lv.n, dp<double>(lv.inv), b, dp<double>(lv.partial), x, s);
+        });
+        return;
+    }
+    const double shift = M.p.shift;
+    smooth(M, l, b, x, false);
+    P.run(level_class(l, K_RESIDUAL), op_bytes(lv.A, 8, 8, 2),
+          [&] { gpu::level_residual<double>(lv.A.view, shift, b, x, dp<double>(lv.r), s); });
+    auto &c = M.L[l + 1];
+    P.run(level_class(l, K_RESTRICT), lv.n * 12.0 + c.n * 12.0, [&] {
+        gpu::restrict_children<double>(c.n, dp<u32>(lv.cptr), dp<u32>(lv.cidx), dp<double>(lv.r),
+                                       dp<double>(c.b), s);
+        gpu::fill<double>(c.n, 0.0, dp<double>(c.x), s);
+    });
+    if (M.p.cycle_type == MK_CYCLE_K && !c.terminal) {
+        krylov_coarse(M, l);
+    } else {
+        const int visits = c.terminal ? 1 : M.p.cycles;
+        for (int v = 0; v < visits; ++v) cycle(M, l + 1, dp<double>(c.b), dp<double>(c.x));
+    }
+    P.run(level_class(l, K_PROLONG), lv.n * 28.0, [&] {
+        gpu::prolong_add<double>(lv.n, dp<u32>(lv.parent), dp<double>(lv.weight), dp<double>(c.x), x, s);
+    });
+    smooth(M, l, b, x, true);
+}
+
+// |b - A x|_1 / shift on the finest level (row order; the finest level is not terminal)
+double error_bound(mk_box_mg_s &M) {
+    auto &lv = M.L[0];
+    const hipStream_t s = stream();
+    const int nb = gpu::reduction_blocks(M.rows);
+    prof().run(K_RESIDUAL, op_bytes(lv.A, 8, 8, 2), [&] {
+        gpu::level_residual<double>(lv.A.view, M.p.shift, dp<double>(M.b0), dp<double>(M.x0),
+                                    dp<double>(lv.r), s);
+        gpu::box_abs_partial(M.rows, dp<double>(lv.r), dp<double>(M.red), nb, s);
+        gpu::reduce_partials(dp<double>(M.red), nb, 1, dp<double>(M.scal), s);
+    });
+    HIPCHK(hipMemcpyAsync(M.host, M.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    drain();
+    return M.host[0] / M.p.shift;
+}
+
+void run_cycle(mk_box_mg_s &M) {
+    const hipStream_t s = stream();
+    auto body = [&] { cycle(M, 0, dp<double>(M.b0), dp<double>(M.x0)); };
+    if (M.p.cycle_type != MK_CYCLE_K && box_graph_switch().load() != 0 && prof().mask == 0) {
+        graph_key key{M.uid, 1, 0, 0, 0, 0, 0};
+        // graph_cache replays only when graphs are enabled globally; capture here directly
+        auto it = M.graphs.exec.find(key);
+        if (it == M.graphs.exec.end()) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                body();
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(s, &g));
+            hipGraphExec_t e = nullptr;
+            const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(err);
+            it = M.graphs.exec.emplace(key, e).first;
+        }
+        HIPCHK(hipGraphLaunch(it->second, s));
+        return;
+    }
+    body();
+}
+
+void enter(mk_box_mg_s &M, const double *b, const double *x, const double *scale = nullptr) {
+    const hipStream_t s = stream();
+    prof().run(K_OTHER, M.rows * 20.0, [&] {
+        gpu::permute_in<double, double>(M.rows, dp<u32>(M.perm0), b, dp<double>(M.b0), s, scale);
+        if (x) {
+            gpu::permute_in<double, double>(M.rows, dp<u32>(M.perm0), x, dp<double>(M.x0), s);
+        } else {
+            gpu::fill<double>(M.rows, 0.0, dp<double>(M.x0), s);
+        }
+    });
+}
+
+void leave(mk_box_mg_s &M, double *x) {
+    prof().run(K_OTHER, M.rows * 20.0,
+               [&] { gpu::permute_out<double>(M.rows, dp<u32>(M.perm0), dp<double>(M.x0), x, stream()); });
+}
+
+void check_params(const mk_box_mg_params &p) {
+    require(p.dimension >= 1 && p.dimension <= 3, "box multigrid: dimension must be 1, 2 or 3");
+    require(p.cells >= 1 && (p.cells & (p.cells - 1)) == 0, "box multigrid: cells must be a power of two");
+    require(p.cap >= 0, "box multigrid: negative population cap");
+    require(p.shift > 0.0 && std::isfinite(p.shift), "box multigrid: shift must be positive");
+    require(p.diffusion >= 0.0 && p.c1 >= 0.0 && p.c2 >= 0.0 && p.c3 >= 0.0 && p.c4 >= 0.0,
+            "box multigrid: negative rate constant");
+    require(p.smoother == MK_BOX_GS || p.smoother == MK_BOX_JACOBI, "box multigrid: unknown smoother");
+    require(p.sweeps >= 1, "box multigrid: at least one sweep");
+    require(p.cycles == 1 || p.cycles == 2, "box multigrid: cycles must be 1 (V) or 2 (W)");
+    require(p.cycle_type == MK_CYCLE_STANDARD || p.cycle_type == MK_CYCLE_K,
+            "box multigrid: unknown cycle type");
+    require(p.terminal_rows >= 0 && p.terminal_rows <= 8192,
+            "box multigrid: terminal_rows must lie in [0, 8192]");
+    i64 V = 1;
+    for (int d = 0; d < p.dimension; ++d) {
+        require(V <= (i64{1} << 20) / p.cells, "box multigrid: too many voxels");
+        V *= p.cells;
+    }
+}
+
+} // namespace
+} // namespace mk::rt
+
+// ====================================================================== C ABI
+extern "C" {
+
+mk_status mk_box_mg_create(const mk_box_mg_params *params, mk_box_mg *out) {
+    return guarded([&] {
+        require(params && out, "box multigrid: bad arguments");
+        check_params(*params);
+        auto M = std::make_unique<mk_box_mg_s>();
+        M->p = *params;
+        build(*M);
+        *out = M.release();
+    });
+}
+
+mk_status mk_box_mg_destroy(mk_box_mg mg) {
+    delete mg;
+    return MK_OK;
+}
+
+mk_status mk_box_mg_info(mk_box_mg mg, int64_t *rows, int32_t *levels) {
+    return guarded([&] {
+        require(mg, "box multigrid: null handle");
+        if (rows) *rows = mg->rows;
+        if (levels) *levels = static_cast<int32_t>(mg->L.size());
+    });
+}
+
+mk_status mk_box_mg_level(mk_box_mg mg, int32_t level, int64_t *rows, int64_t *voxels,
+                          int64_t *entries, int32_t *terminal) {
+    return guarded([&] {
+        require(mg && level >= 0 && level < static_cast<int32_t>(mg->L.size()),
+                "box multigrid: no such level");
+        const auto &lv = mg->L[level];
+        if (rows) *rows = lv.n;
+        if (voxels) *voxels = lv.voxels;
+        if (entries) *entries = static_cast<int64_t>(lv.nnz);
+        if (terminal) *terminal = lv.terminal;
+    });
+}
+
+mk_status mk_box_mg_generator(mk_box_mg mg, mk_generator *gen) {
+    return guarded([&] {
+        require(mg && gen, "box multigrid: bad arguments");
+        *gen = mg->gen.get();
+    });
+}
+
+mk_status mk_box_mg_cycle_device(mk_box_mg mg, const double *d_b, double *d_x) {
+    return guarded([&] {
+        require(mg && d_b && d_x, "box multigrid: bad arguments");
+        enter(*mg, d_b, d_x);
+        cycle(*mg, 0, dp<double>(mg->b0), dp<double>(mg->x0));
+        leave(*mg, d_x);
+        drain();
+    });
+}
+
+mk_status mk_box_mg_apply_device(mk_box_mg mg, const double *d_b, double *d_z) {
+    return guarded([&] {
+        require(mg && d_b && d_z, "box multigrid: bad arguments");
+        enter(*mg, d_b, nullptr);
+        cycle(*mg, 0, dp<double>(mg->b0), dp<double>(mg->x0));
+        leave(*mg, d_z);
+        drain();
+    });
+}
+
+mk_status mk_box_mg_solve_device(mk_box_mg mg, const double *d_b, double *d_x, double tolerance,
+                                 int32_t max_iterations, double *history, int64_t history_cap,
+                                 mk_box_mg_result *result) {
+    return guarded([&] {
+        require(mg && d_b && d_x && result, "box multigrid: bad arguments");
+        require(max_iterations >= 0, "box multigrid: negative iteration limit");
+        require(!mg->L[0].terminal, "box multigrid: the finest level is solved exactly; use apply");
+        auto &M = *mg;
+        enter(M, d_b, d_x);
+        mk_box_mg_result r{0, 0, 0.0, 0};
+        double err = error_bound(M);
+        int64_t h = 0;
+        while (r.iterations < max_iterations) {
+            if (history && h < history_cap) history[h] = err;
+            ++h;
+            if (err < tolerance || !std::isfinite(err) || err > 1e8) break;
+            run_cycle(M);
+            ++r.iterations;
+            err = error_bound(M);
+        }
+        leave(M, d_x);
+        drain();
+        r.error = err;
+        r.converged = err < tolerance ? 1 : 0;
+        r.history = h;
+        *result = r;
+    });
+}
+
+mk_status mk_box_mg_solve(mk_box_mg mg, const double *b, double *x, double tolerance,
+                          int32_t max_iterations, double *history, int64_t history_cap,
+                          mk_box_mg_result *result) {
+    mk_status st = MK_OK;
+    const mk_status outer = guarded([&] {
+        require(mg && b && x, "box multigrid: bad arguments");
+        const std::size_t bytes = mg->rows * sizeof(double);
+        dbuf db(bytes), dx(bytes);
+        HIPCHK(hipMemcpy(db.p, b, bytes, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dx.p, x, bytes, hipMemcpyHostToDevice));
+        st = mk_box_mg_solve_device(mg, dp<double>(db), dp<double>(dx), tolerance, max_iterations,
+                                    history, history_cap, result);
+        if (st == MK_OK) HIPCHK(hipMemcpy(x, dx.p, bytes, hipMemcpyDeviceToHost));
+    });
+    return outer != MK_OK ? outer : st;
+}
+
+mk_status mk_gmres_box_device(mk_generator gen, double shift, mk_box_mg mg, const double *d_b,
+                              double *d_x, const mk_gmres_options *options,
+                              mk_solver_result *result) {
+    return guarded([&] {
+        require(gen && mg && d_b && d_x && options && result, "gmres: bad arguments");
+        require(mg->p.cycle_type == MK_CYCLE_STANDARD,
+                "gmres: the K-cycle is nonlinear and runs only as a standalone iteration");
+        mk_box_mg_s *m = mg;
+        precond M{m->uid, m->rows, m->p.shift, MK_FP64, [m](const double *v, void *z, const double *scale) {
+                      enter(*m, v, nullptr, scale);
+                      cycle(*m, 0, dp<double>(m->b0), dp<double>(m->x0));
+                      leave(*m, static_cast<double *>(z));
+                  }};
+        *result = gmres_device(*gen, shift, &M, d_b, d_x, *options);
+    });
+}
+
+mk_status mk_gmres_box(mk_generator gen, double shift, mk_box_mg mg, const double *b, double *x,
+                       const mk_gmres_options *options, mk_solver_result *result) {
+    mk_status st = MK_OK;
+    const mk_status outer = guarded([&] {
+        require(gen && mg && b && x, "gmres: bad arguments");
+        const std::size_t bytes = gen->n * sizeof(double);
+        dbuf db(bytes), dx(bytes);
+        HIPCHK(hipMemcpy(db.p, b, bytes, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dx.p, x, bytes, hipMemcpyHostToDevice));
+        st = mk_gmres_box_device(gen, shift, mg, dp<double>(db), dp<double>(dx), options, result);
+        if (st == MK_OK) HIPCHK(hipMemcpy(x, dx.p, bytes, hipMemcpyDeviceToHost));
+    });
+    return outer != MK_OK ? outer : st;
+}
+
+mk_status mk_box_mg_graphs_enable(int32_t on) {
+    box_graph_switch().store(on ? 1 : 0);
+    return MK_OK;
+}
+
+} // extern "C"

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -529,6 +530,24 @@ bool assemble_generator_device(const mk::population_space &space, const mk::voxe
 // CSR (the multigrid levels' operators). False for spaces outside the kernel's limits.
 bool assemble_csr_device(const mk::population_space &space, const mk::voxel_domain &domain,
                          const mk::reaction_tables &reactions, bool absorbing, mk::generator_csr &g);
+
+// G <- the generator g: device upload in the coded or plain layout, exit and leak rates, ||Q||_1
+void finish_generator(mk_generator_s &G, mk::generator_csr &&g);
+
+// A right preconditioner of the flexible GMRES: z = M (scale v), z in the precision's type
+// (FP32 or FP64), scale read from device memory (1 when null). apply launches on the library
+// stream and must not synchronise (GMRES may capture it in a graph).
+struct precond {
+    mk::u64 uid = 0;
+    mk::i64 rows = 0;
+    double shift = 0.0;
+    int precision = MK_FP64;
+    std::function<void(const double *v, void *z, const double *scale)> apply;
+};
+
+// FGMRES on (shift I - Q) x = b with b, x in device memory (M null: unpreconditioned)
+mk_solver_result gmres_device(mk_generator_s &A, double shift, const precond *M, const double *b,
+                              double *x, const mk_gmres_options &o);
 
 // level_hooks::transfer on the device (sector levels of at least 4096 states)
 bool transfer_device(const mk::population_space &fine, const mk::population_space &coarse,

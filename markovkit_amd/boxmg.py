@@ -101,6 +101,10 @@ class BoxMultigrid:
         """the finest operator's generator Q (shift I - Q is the level operator)"""
         return self._gen
 
+    def apply_operator(self, x):
+        """(shift I - Q) x on the finest level (host arrays)"""
+        return self._gen.apply_shifted(self.shift, np.ascontiguousarray(x, np.float64))
+
     def initial_problem(self, initial: int):
         """the driver's first step: b = shift p0, x0 = p0, p0 the state with `initial` molecules
         dealt round-robin over the voxels (rdme_full_multigrid.cpp:1119-1121)"""
@@ -175,3 +179,135 @@ class BoxMultigrid:
 def graphs_enable(on: bool) -> None:
     """replay the standalone iteration's cycle from a captured HIP graph (default on)"""
     check(lib().mk_box_mg_graphs_enable(1 if on else 0))
+
+
+# ---------------------------------------------------------------------- time stepping and FSP
+# The two drivers below follow run() and run_adaptive_fsp() of rdme_full_multigrid.cpp. They only
+# need a solver object with rows(), shift, cap, solve(b, x0, tolerance, max_iterations) ->
+# (x, iterations, error, history) and apply_operator(x) = (shift I - Q) x: the device
+# BoxMultigrid, or the oracle's restatement on the CPU.
+
+HIGH_POPULATION = 21  # the driver's high_mass threshold (rdme_full_multigrid.cpp:1219)
+
+
+def population_offsets(voxels: int, cap: int) -> np.ndarray:
+    """first state index of every total population 0..cap, and the size (box order)"""
+    from math import comb
+    return np.array([comb(k + voxels - 1, voxels) if k > 0 else 0 for k in range(cap + 2)],
+                    np.int64)
+
+
+def marginal(p, voxels: int, cap: int) -> np.ndarray:
+    """probability of each total population 0..cap"""
+    off = population_offsets(voxels, cap)
+    return np.add.reduceat(np.asarray(p), off[:-1]) if len(p) else np.zeros(cap + 1)
+
+
+def time_steps(M, p0, steps: int, integrator: str = "be", tolerance: float = 1e-9,
+               max_iterations: int = 1000, on_step=None):
+    """run() with --steps (rdme_full_multigrid.cpp:1183-1244): backward Euler steps of length
+    1 / shift, (shift I - Q) p_next = shift p; or "cn": two backward-Euler half steps (Rannacher
+    start) and then Crank-Nicolson steps of 2 / shift, (shift I - Q) p_next = 2 shift p -
+    (shift I - Q) p; steps + 1 solves. Each solve starts from the previous distribution. Stops
+    at the first solve that does not reach the tolerance. Returns (p, records); on_step(record,
+    p) is called after every solve."""
+    if integrator not in ("be", "cn"):
+        raise ValueError("integrator must be 'be' or 'cn'")
+    s = M.shift
+    solves = steps + (1 if integrator == "cn" else 0)
+    p = np.array(p0, np.float64)
+    pop = np.repeat(np.arange(M.cap + 1), np.diff(population_offsets(M.voxels, M.cap)))
+    records = []
+    for step in range(1, solves + 1):
+        if integrator == "cn" and step > 2:
+            b = 2 * s * p - M.apply_operator(p)
+        else:
+            b = s * p
+        x, it, err, _ = M.solve(b, p, tolerance, max_iterations)
+        t = ((1.0 if step == 1 else 2.0 * (step - 1)) if integrator == "cn" else float(step)) / s
+        rec = {"step": step, "time": t, "iterations": it, "error": err,
+               "converged": bool(err < tolerance), "mass": float(x.sum()),
+               "high_mass": float(x[pop >= HIGH_POPULATION].sum()), "min_probability": float(x.min())}
+        records.append(rec)
+        if on_step:
+            on_step(rec, x)
+        if not rec["converged"]:
+            break
+        p = x
+    return p, records
+
+
+def adaptive_fsp(make, voxels: int, initial: int, shift: float, steps: int,
+                 fsp_tolerance: float = 1e-6, tolerance: float = 1e-9, cap: Optional[int] = None,
+                 max_iterations: int = 1000, on_attempt=None):
+    """run_adaptive_fsp() (rdme_full_multigrid.cpp:1248-1363): backward-Euler steps of 1 / shift
+    on a total-population box whose cap grows until a step loses little enough probability.
+    make(cap) builds the solver for a cap (the hierarchy is rebuilt on growth: device setup is
+    fast, and the state order keeps every old state's index, so the distribution is extended by
+    zeros). A trial is accepted when missing mass + 2 (accumulated + this solve's l1 error
+    bound) <= fsp_tolerance * step / steps; otherwise the cap grows to max(cap + 4,
+    ceil(1.25 (cap + 1))) and the step is retried from the accepted distribution. Returns (p,
+    cap, attempts)."""
+    import math
+    if cap is None:
+        cap = max(2, initial)
+    solve_tol = min(tolerance, fsp_tolerance / (100.0 * steps))
+    M = None
+    p = None
+    acc = 0.0
+    attempts = []
+    for step in range(1, steps + 1):
+        accepted = False
+        while not accepted:
+            if M is None or M.cap != cap:
+                M = make(cap)
+                q = np.zeros(M.rows())
+                if p is None:
+                    x0 = np.zeros(voxels, np.int64)
+                    for k in range(initial):
+                        x0[k % voxels] += 1
+                    if initial > cap:
+                        raise ValueError("the initial population exceeds the cap")
+                    q[_rank(x0, voxels)] = 1.0
+                else:
+                    q[:len(p)] = p
+                p = q
+            b = shift * p
+            trial, it, err, _ = M.solve(b, p, solve_tol, max_iterations)
+            if not (err <= solve_tol) or not np.isfinite(err):
+                raise RuntimeError("adaptive FSP linear solve failed; no step was accepted")
+            mass = float(trial.sum())
+            negative = float(-trial[trial < 0].sum())
+            if not np.isfinite(mass) or negative > err + 1e-13:
+                raise RuntimeError("adaptive FSP produced invalid probabilities")
+            nxt = acc + err
+            missing = max(0.0, 1.0 - mass)
+            budget = fsp_tolerance * step / steps
+            accepted = missing + 2 * nxt <= budget
+            rec = {"attempt": len(attempts) + 1, "step": step, "time": step / shift, "cap": cap,
+                   "states": M.rows(), "accepted": accepted, "iterations": it, "error": err,
+                   "missing_mass": missing, "error_bound": nxt, "budget": budget}
+            attempts.append(rec)
+            if on_attempt:
+                on_attempt(rec)
+            if accepted:
+                p = trial
+                acc = nxt
+            else:
+                cap = max(cap + 4, math.ceil(1.25 * (cap + 1)))
+                if cap > 10000:
+                    raise RuntimeError("adaptive expansion exceeded the population guard")
+    return p, cap, attempts
+
+
+def _rank(x, voxels: int) -> int:
+    """index of the composition x in the box order (total first, voxel 0 most significant)"""
+    from math import comb
+    total = int(sum(x))
+    r = comb(total + voxels - 1, voxels) if total > 0 else 0
+    left = total
+    for v in range(voxels - 1):
+        rest = voxels - v - 1
+        r += comb(left + rest, rest) - comb(left - int(x[v]) + rest, rest)
+        left -= int(x[v])
+    return r

@@ -36,7 +36,7 @@ namespace mk::gpu {
 namespace {
 
 constexpr int kBoxThreads = 256;
-constexpr int kDenseChunk = 64; // columns of the dense inverse per workgroup
+constexpr int kDenseChunk = 32; // columns of the dense inverse per workgroup
 
 // the one-voxel level: forward elimination with the stored multipliers, back substitution
 __global__ void k_box_thomas(i64 n, const double *lower, const double *upper, const double *pivot,
@@ -49,10 +49,11 @@ __global__ void k_box_thomas(i64 n, const double *lower, const double *upper, co
 }
 
 // partial[c][i] = sum over the columns j of chunk c of inv[j][i] b[j] (inverse column-major:
-// the lanes of a wavefront read consecutive rows of one column)
-__global__ void __launch_bounds__(kBoxThreads)
+// the lanes of a wavefront read consecutive rows of one column); one wavefront per 64 rows and
+// column chunk, so a 561-state level already spreads over 162 workgroups
+__global__ void __launch_bounds__(64)
     k_box_dense(i64 n, const double *inv, const double *b, double *partial) {
-    const i64 i = blockIdx.x * static_cast<i64>(kBoxThreads) + threadIdx.x;
+    const i64 i = blockIdx.x * static_cast<i64>(64) + threadIdx.x;
     const i64 j0 = static_cast<i64>(blockIdx.y) * kDenseChunk;
     const i64 j1 = j0 + kDenseChunk < n ? j0 + kDenseChunk : n;
     if (i >= n) return;
@@ -107,10 +108,14 @@ void box_thomas(i64 n, const double *lower, const double *upper, const double *p
 
 void box_dense(i64 n, const double *inv, const double *b, double *partial, double *x, hipStream_t s) {
     const int chunks = static_cast<int>((n + kDenseChunk - 1) / kDenseChunk);
+    const int tiles = static_cast<int>((n + 63) / 64);
     const int rows = static_cast<int>((n + kBoxThreads - 1) / kBoxThreads);
-    hipLaunchKernelGGL(k_box_dense, dim3(rows, chunks), dim3(kBoxThreads), 0, s, n, inv, b, partial);
+    hipLaunchKernelGGL(k_box_dense, dim3(tiles, chunks), dim3(64), 0, s, n, inv, b, partial);
     hipLaunchKernelGGL(k_box_dense_sum, dim3(rows), dim3(kBoxThreads), 0, s, n, chunks, partial, x);
 }
+
+// doubles of the partial-sum buffer box_dense needs for an n-state level
+i64 box_dense_partials(i64 n) { return (n + kDenseChunk - 1) / kDenseChunk * n; }
 
 void box_abs_partial(i64 n, const double *r, double *partial, int nb, hipStream_t s) {
     hipLaunchKernelGGL(k_box_abs_partial, dim3(nb), dim3(kBoxThreads), 0, s, n, r, partial);
@@ -447,7 +452,7 @@ void build(mk_box_mg_s &M) {
                 for (i64 j = 0; j < n; ++j)
                     colmajor[static_cast<std::size_t>(j) * n + i] = inv[static_cast<std::size_t>(i) * n + j];
             d.inv = upload(colmajor);
-            d.partial = dbuf(static_cast<std::size_t>((n + 63) / 64) * n * sizeof(double));
+            d.partial = dbuf(static_cast<std::size_t>(gpu::box_dense_partials(n)) * sizeof(double));
             continue;
         }
         const auto gp = permuted(h);

@@ -323,6 +323,7 @@ class BoxMultigrid:
         self.h = h
         self.n = L.orc_box_rows(h, 0)
         self.V = L.orc_box_voxels(h, 0)
+        self.voxels = self.V
         self.cap = cap
 
     def __del__(self):
@@ -368,6 +369,10 @@ class BoxMultigrid:
         ptr, col, val = self.matrix(level)
         n = len(ptr) - 1
         return sp.csr_matrix((val, col, ptr), shape=(n, n)) @ np.asarray(x, np.float64)
+
+    def apply_operator(self, x):
+        """(shift I - Q) x on the finest level"""
+        return self.apply_level(0, x)
 
     def leak(self, level=0):
         out = np.zeros(self.rows(level))

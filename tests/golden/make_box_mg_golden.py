@@ -1,5 +1,6 @@
-"""Collects the recorded runs of the reference's whole-box multigrid driver into one fixture,
-tests/golden/box_mg_recorded.json.
+"""Collects the recorded runs of the reference's whole-box multigrid driver into fixtures:
+tests/golden/box_mg_recorded.json (standalone solves), box_mg_steps.json (time-stepping runs:
+per-solve rows and population marginals) and box_mg_adaptive.json (adaptive FSP attempts).
 
 Source: the reference's data files rdme/paper/data/full_multigrid/<name>.json (the command
 line of each run), <name>.csv (its summary row) and, where present, <name>_history.csv (the
@@ -31,6 +32,11 @@ RUNS = [
 
 FLAGS = {"--check", "--export-only", "--profile", "--adaptive-fsp"}
 
+STEP_RUNS = ["time_check_s10", "transient_k80_s20"]
+ADAPTIVE = "/root/reference/rdme/paper/data/adaptive_population_fsp"
+ADAPTIVE_RUNS = ["linear_mid", "linear_tight", "linear_start8", "linear_loose", "cycle_2d",
+                 "cycle_v", "schlogl_refined", "diffusion"]
+
 
 def params(command):
     p = {"dimension": 1, "cells": 4, "cap": 40, "sweeps": 2, "cycles": 1, "initial": 8,
@@ -59,6 +65,12 @@ def params(command):
             p["kcycle"] = v == "k"
         elif key == "method":
             assert v == "mg", v
+        elif key in ("steps",):
+            p[key] = int(v)
+        elif key in ("integrator",):
+            p[key] = v
+        elif key == "fsp_tolerance":
+            p[key] = float(v)
     return p
 
 
@@ -84,6 +96,46 @@ def main():
         out[name] = rec
     with open(os.path.join(HERE, "box_mg_recorded.json"), "w") as f:
         json.dump(out, f, indent=1)
+
+    steps = {}
+    for name in STEP_RUNS:
+        with open(os.path.join(DATA, name + ".json")) as f:
+            cmd = json.load(f)["command"]
+        with open(os.path.join(DATA, name + ".csv")) as f:
+            rows = list(csv.DictReader(f))
+        rec = {"params": params(cmd), "states": int(rows[0]["states"]),
+               "solves": [{"step": int(r["step"]), "iterations": int(r["iterations"]),
+                           "error": float(r["residual_l1_bound"]), "mass": float(r["mass"]),
+                           "high_mass": float(r["high_mass"]),
+                           "solve_s": float(r["solve_s"])} for r in rows],
+               "wall_s": json.load(open(os.path.join(DATA, name + ".json")))["wall_seconds"]}
+        mp = os.path.join(DATA, name + "_marginal.csv")
+        if os.path.exists(mp):
+            with open(mp) as f:
+                m = list(csv.DictReader(f))
+            times = sorted({float(r["time"]) for r in m})
+            rec["marginal_times"] = times
+            rec["marginals"] = [[float(r["probability"]) for r in m if float(r["time"]) == t]
+                                for t in times]
+        steps[name] = rec
+    with open(os.path.join(HERE, "box_mg_steps.json"), "w") as f:
+        json.dump(steps, f)
+
+    adaptive = {}
+    for name in ADAPTIVE_RUNS:
+        with open(os.path.join(ADAPTIVE, name + ".json")) as f:
+            cmd = json.load(f)["command"]
+        with open(os.path.join(ADAPTIVE, name + ".csv")) as f:
+            rows = list(csv.DictReader(f))
+        p = params(cmd)
+        p["cap_given"] = "--cap" in cmd
+        adaptive[name] = {"params": p, "attempts": [
+            {"attempt": int(r["attempt"]), "step": int(r["step"]), "cap": int(r["cap"]),
+             "states": int(r["states"]), "accepted": int(r["accepted"]),
+             "iterations": int(r["iterations"]), "error": float(r["residual_l1_bound"]),
+             "missing_mass": float(r["missing_mass"])} for r in rows]}
+    with open(os.path.join(HERE, "box_mg_adaptive.json"), "w") as f:
+        json.dump(adaptive, f, indent=0)
 
 
 if __name__ == "__main__":

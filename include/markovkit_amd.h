@@ -237,6 +237,14 @@ mk_status mk_box_mg_solve(mk_box_mg mg, const double *b, double *x, double toler
 mk_status mk_box_mg_solve_device(mk_box_mg mg, const double *d_b, double *d_x, double tolerance,
                                  int32_t max_iterations, double *history, int64_t history_cap,
                                  mk_box_mg_result *result);
+/* One time step of run() (:1183-1244) in place on the distribution p (HBM, box order): the
+ * solve of (shift I - Q) p_next = shift p (backward Euler, step 1 / shift; also the two half
+ * steps of the driver's Rannacher start) or, with crank_nicolson != 0, = 2 shift p -
+ * (shift I - Q) p (Crank-Nicolson, step 2 / shift), from the guess p. p is replaced only when
+ * the solve converges. */
+mk_status mk_box_mg_step_device(mk_box_mg mg, double *d_p, int32_t crank_nicolson,
+                                double tolerance, int32_t max_iterations,
+                                mk_box_mg_result *result);
 /* run() with --method mg-gmres (:1200-1209): flexible GMRES on (shift I - Q) x = b preconditioned
  * by one cycle (standard cycles only; the driver rejects the K-cycle there, :1437-1438). gen is
  * mk_box_mg_generator's; the driver's tolerance is tolerance * shift / sqrt(rows). */

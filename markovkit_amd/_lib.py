@@ -111,6 +111,7 @@ SIGNATURES = {
     "mk_box_mg_apply_device": ([P, P, P], i32),
     "mk_box_mg_solve": ([P, P, P, f64, i32, P, i64, C.POINTER(BoxMgResult)], i32),
     "mk_box_mg_solve_device": ([P, P, P, f64, i32, P, i64, C.POINTER(BoxMgResult)], i32),
+    "mk_box_mg_step_device": ([P, P, i32, f64, i32, C.POINTER(BoxMgResult)], i32),
     "mk_gmres_box": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_gmres_box_device": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_box_mg_graphs_enable": ([i32], i32),

@@ -154,6 +154,15 @@ class BoxMultigrid:
                                            None, 0, C.byref(r)))
         return r.iterations, r.error, bool(r.converged)
 
+    def step_device(self, p: DeviceVector, crank_nicolson: bool, tolerance: float = 1e-9,
+                    max_iterations: int = 1000):
+        """one backward-Euler or Crank-Nicolson step in place on p (HBM); (iterations, error,
+        converged); p is left unchanged when the solve does not converge"""
+        r = _Result()
+        check(lib().mk_box_mg_step_device(self._h, p.ptr, 1 if crank_nicolson else 0,
+                                          float(tolerance), int(max_iterations), C.byref(r)))
+        return r.iterations, r.error, bool(r.converged)
+
     def gmres(self, b, x, tolerance: float, max_iterations: int = 1000, restart: int = 30,
               orthogonalization: str = "dcgs2") -> SolverResult:
         """flexible GMRES on (shift I - Q) x = b preconditioned by one cycle (the driver's

@@ -40,7 +40,7 @@ __device__ __forceinline__ T row_gather(const csr_view<VT> &A, i64 r, const XT *
         const i64 slice = r >> 6;
         const u32 len = A.rlen[r];
         const u32 base = A.soff[slice] + static_cast<u32>(r & 63);
-        exit_r = static_cast<T>(A.edict[A.rcode[r]]);
+        exit_r = static_cast<T>(A.exit ? A.exit[r] : A.edict[A.rcode[r]]);
         for (u32 k0 = 0; k0 < len; k0 += kEntryUnroll) {
             u32 e[kEntryUnroll];
 #pragma unroll
@@ -137,7 +137,7 @@ int op_parts(i64 n) {
 
 template <typename T>
 __device__ __forceinline__ T row_exit(const csr_view<T> &A, i64 r) {
-    return A.coded ? A.edict[A.rcode[r]] : A.exit[r];
+    return A.exit ? A.exit[r] : A.edict[A.rcode[r]];
 }
 
 int grid_for(i64 n) {

@@ -38,14 +38,49 @@ namespace {
 constexpr int kBoxThreads = 256;
 constexpr int kDenseChunk = 32; // columns of the dense inverse per workgroup
 
-// the one-voxel level: forward elimination with the stored multipliers, back substitution
-__global__ void k_box_thomas(i64 n, const double *lower, const double *upper, const double *pivot,
-                             const double *b, double *x) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// the one-voxel level: forward elimination with the stored multipliers, back substitution. The
+// workgroup stages the coefficients and b in LDS (n <= kThomasLds), then one lane runs the two
+// recurrences from LDS; longer chains run from global memory.
+constexpr int kThomasLds = 1536;
+__global__ void __launch_bounds__(256) k_box_thomas(i64 n, const double *lower, const double *upper,
+                                                    const double *pivot, const double *b, double *x) {
+    __shared__ double lo[kThomasLds], up[kThomasLds], pv[kThomasLds], xs[kThomasLds];
+    if (n <= kThomasLds) {
+        for (i64 i = threadIdx.x; i < n; i += blockDim.x) {
+            lo[i] = lower[i];
+            up[i] = upper[i];
+            pv[i] = pivot[i];
+            xs[i] = b[i];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (i64 i = 1; i < n; ++i) xs[i] = xs[i] - lo[i] * xs[i - 1];
+            xs[n - 1] /= pv[n - 1];
+            for (i64 i = n - 1; i > 0; --i) xs[i - 1] = (xs[i - 1] - up[i - 1] * xs[i]) / pv[i - 1];
+        }
+        __syncthreads();
+        for (i64 i = threadIdx.x; i < n; i += blockDim.x) x[i] = xs[i];
+        return;
+    }
+    if (threadIdx.x != 0) return;
     x[0] = b[0];
     for (i64 i = 1; i < n; ++i) x[i] = b[i] - lower[i] * x[i - 1];
     x[n - 1] /= pivot[n - 1];
     for (i64 i = n - 1; i > 0; --i) x[i - 1] = (x[i - 1] - upper[i - 1] * x[i]) / pivot[i - 1];
+}
+
+// restriction with one wavefront per coarse state, for levels whose coarse states have many
+// children: lane l sums children l, l + 64, ... in order, then the lanes' sums are added in a
+// fixed tree (deterministic)
+__global__ void __launch_bounds__(kBoxThreads)
+    k_box_restrict_wave(i64 nc, const u32 *cptr, const u32 *cidx, const double *fine, double *coarse) {
+    const i64 c = blockIdx.x * static_cast<i64>(kBoxThreads / 64) + (threadIdx.x >> 6);
+    const u32 lane = threadIdx.x & 63u;
+    if (c >= nc) return;
+    double s = 0.0;
+    for (u32 q = cptr[c] + lane; q < cptr[c + 1]; q += 64) s += fine[cidx[q]];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) coarse[c] = s;
 }
 
 // partial[c][i] = sum over the columns j of chunk c of inv[j][i] b[j] (inverse column-major:
@@ -88,6 +123,16 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_abs_partial(i64 n, const do
     }
 }
 
+// b = s p (backward Euler) or b = 2 s p - A p (Crank-Nicolson, ap = A p), rounded as written
+__global__ void __launch_bounds__(kBoxThreads)
+    k_box_rhs(i64 n, double s, const double *p, const double *ap, double *b) {
+#pragma clang fp contract(off)
+    const double s2 = 2.0 * s;
+    for (i64 i = blockIdx.x * static_cast<i64>(kBoxThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kBoxThreads)
+        b[i] = ap ? s2 * p[i] - ap[i] : s * p[i];
+}
+
 __global__ void __launch_bounds__(kBoxThreads) k_box_copy(i64 n, const double *src, double *dst) {
     for (i64 i = blockIdx.x * static_cast<i64>(kBoxThreads) + threadIdx.x; i < n;
          i += static_cast<i64>(gridDim.x) * kBoxThreads)
@@ -103,7 +148,15 @@ int grid_of(i64 n) {
 
 void box_thomas(i64 n, const double *lower, const double *upper, const double *pivot,
                 const double *b, double *x, hipStream_t s) {
-    hipLaunchKernelGGL(k_box_thomas, dim3(1), dim3(64), 0, s, n, lower, upper, pivot, b, x);
+    hipLaunchKernelGGL(k_box_thomas, dim3(1), dim3(256), 0, s, n, lower, upper, pivot, b, x);
+}
+
+void box_restrict_wave(i64 nc, const u32 *cptr, const u32 *cidx, const double *fine, double *coarse,
+                       hipStream_t s) {
+    if (nc == 0) return;
+    const i64 blocks = (nc + kBoxThreads / 64 - 1) / (kBoxThreads / 64);
+    hipLaunchKernelGGL(k_box_restrict_wave, dim3(static_cast<unsigned>(blocks)), dim3(kBoxThreads), 0, s,
+                       nc, cptr, cidx, fine, coarse);
 }
 
 void box_dense(i64 n, const double *inv, const double *b, double *partial, double *x, hipStream_t s) {
@@ -119,6 +172,11 @@ i64 box_dense_partials(i64 n) { return (n + kDenseChunk - 1) / kDenseChunk * n; 
 
 void box_abs_partial(i64 n, const double *r, double *partial, int nb, hipStream_t s) {
     hipLaunchKernelGGL(k_box_abs_partial, dim3(nb), dim3(kBoxThreads), 0, s, n, r, partial);
+}
+
+void box_rhs(i64 n, double shift, const double *p, const double *ap, double *b, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_box_rhs, dim3(grid_of(n)), dim3(kBoxThreads), 0, s, n, shift, p, ap, b);
 }
 
 void box_copy(i64 n, const double *src, double *dst, hipStream_t s) {
@@ -145,6 +203,7 @@ struct box_level {
     dbuf lower, upper, pivot, inv, partial;
     dbuf kv1, kw1, kv2, kw2, kr; // K-cycle vectors (the next level's size)
     u64 nnz = 0;
+    bool wide_restrict = false;  // coarse states average more than 16 children
 };
 
 } // namespace mk::rt
@@ -157,6 +216,7 @@ struct mk_box_mg_s {
     mk::rt::dbuf perm0;            // level-0 row r holds the state perm0[r] of the box order
     mk::rt::dbuf b0, x0;           // the finest level's vectors in its row order
     mk::rt::dbuf red, scal;        // reduction partials, device scalars
+    mk::rt::dbuf step_b, step_x;   // time-step right-hand side and iterate (box order)
     double *host = nullptr;        // pinned copy of one scalar
     std::unique_ptr<mk_generator_s> gen; // the finest operator in the box order
     mk::rt::graph_cache graphs;
@@ -466,10 +526,10 @@ void build(mk_box_mg_s &M) {
         d.rowmap = upload(rm);
         for (int c = 0; c < 4; ++c) {
             if (d.cb[c + 1] == d.cb[c]) continue;
-            d.Ac[c] = upload_csr<double>(row_range(gp, d.cb[c], d.cb[c + 1]));
+            d.Ac[c] = upload_csr<double>(row_range(gp, d.cb[c], d.cb[c + 1]), true);
             d.Ac[c].view.rowmap = dp<u32>(d.rowmap) + d.cb[c];
         }
-        d.A = upload_csr<double>(gp);
+        d.A = upload_csr<double>(gp, true);
         d.r = dbuf(n * sizeof(double));
         if (p.smoother == MK_BOX_JACOBI) d.tmp = dbuf(n * sizeof(double));
         // transfer in row orders: parent'[pos_f[i]] = pos_c[parent[i]]
@@ -488,6 +548,7 @@ void build(mk_box_mg_s &M) {
             std::vector<u32> fill(cptr.begin(), cptr.end() - 1);
             for (i64 r = 0; r < n; ++r) cidx[fill[par[r]]++] = static_cast<u32>(r);
         }
+        d.wide_restrict = n > 16 * nc;
         d.parent = upload(par);
         d.weight = upload(w);
         d.cptr = upload(cptr);
@@ -611,8 +672,13 @@ void cycle(mk_box_mg_s &M, std::size_t l, const double *b, double *x) {
           [&] { gpu::level_residual<double>(lv.A.view, shift, b, x, dp<double>(lv.r), s); });
     auto &c = M.L[l + 1];
     P.run(level_class(l, K_RESTRICT), lv.n * 12.0 + c.n * 12.0, [&] {
-        gpu::restrict_children<double>(c.n, dp<u32>(lv.cptr), dp<u32>(lv.cidx), dp<double>(lv.r),
-                                       dp<double>(c.b), s);
+        if (lv.wide_restrict) {
+            gpu::box_restrict_wave(c.n, dp<u32>(lv.cptr), dp<u32>(lv.cidx), dp<double>(lv.r),
+                                   dp<double>(c.b), s);
+        } else {
+            gpu::restrict_children<double>(c.n, dp<u32>(lv.cptr), dp<u32>(lv.cidx), dp<double>(lv.r),
+                                           dp<double>(c.b), s);
+        }
         gpu::fill<double>(c.n, 0.0, dp<double>(c.x), s);
     });
     if (M.p.cycle_type == MK_CYCLE_K && !c.terminal) {
@@ -711,6 +777,31 @@ void check_params(const mk_box_mg_params &p) {
     }
 }
 
+// the standalone iteration from the guess x (vectors in the box order)
+mk_box_mg_result solve(mk_box_mg_s &M, const double *b, double *x, double tolerance,
+                       int32_t max_iterations, double *history, int64_t history_cap) {
+    require(max_iterations >= 0, "box multigrid: negative iteration limit");
+    require(!M.L[0].terminal, "box multigrid: the finest level is solved exactly; use apply");
+    enter(M, b, x);
+    mk_box_mg_result r{0, 0, 0.0, 0};
+    double err = error_bound(M);
+    int64_t h = 0;
+    while (r.iterations < max_iterations) {
+        if (history && h < history_cap) history[h] = err;
+        ++h;
+        if (err < tolerance || !std::isfinite(err) || err > 1e8) break;
+        run_cycle(M);
+        ++r.iterations;
+        err = error_bound(M);
+    }
+    leave(M, x);
+    drain();
+    r.error = err;
+    r.converged = err < tolerance ? 1 : 0;
+    r.history = h;
+    return r;
+}
+
 } // namespace
 } // namespace mk::rt
 
@@ -786,27 +877,36 @@ mk_status mk_box_mg_solve_device(mk_box_mg mg, const double *d_b, double *d_x, d
                                  mk_box_mg_result *result) {
     return guarded([&] {
         require(mg && d_b && d_x && result, "box multigrid: bad arguments");
-        require(max_iterations >= 0, "box multigrid: negative iteration limit");
-        require(!mg->L[0].terminal, "box multigrid: the finest level is solved exactly; use apply");
+        *result = solve(*mg, d_b, d_x, tolerance, max_iterations, history, history_cap);
+    });
+}
+
+mk_status mk_box_mg_step_device(mk_box_mg mg, double *d_p, int32_t crank_nicolson,
+                                double tolerance, int32_t max_iterations, mk_box_mg_result *result) {
+    return guarded([&] {
+        require(mg && d_p && result, "box multigrid: bad arguments");
         auto &M = *mg;
-        enter(M, d_b, d_x);
-        mk_box_mg_result r{0, 0, 0.0, 0};
-        double err = error_bound(M);
-        int64_t h = 0;
-        while (r.iterations < max_iterations) {
-            if (history && h < history_cap) history[h] = err;
-            ++h;
-            if (err < tolerance || !std::isfinite(err) || err > 1e8) break;
-            run_cycle(M);
-            ++r.iterations;
-            err = error_bound(M);
+        const i64 n = M.rows;
+        if (!M.step_b.p) {
+            M.step_b = dbuf(n * sizeof(double));
+            M.step_x = dbuf(n * sizeof(double));
         }
-        leave(M, d_x);
-        drain();
-        r.error = err;
-        r.converged = err < tolerance ? 1 : 0;
-        r.history = h;
-        *result = r;
+        const hipStream_t s = stream();
+        double *b = dp<double>(M.step_b), *x = dp<double>(M.step_x);
+        prof().run(K_OTHER, op_bytes(M.gen->A, 8, 8, 2), [&] {
+            if (crank_nicolson) {
+                gpu::shifted_apply<double>(M.gen->A.view, M.p.shift, d_p, nullptr, x, nullptr, s);
+                gpu::box_rhs(n, M.p.shift, d_p, x, b, s);
+            } else {
+                gpu::box_rhs(n, M.p.shift, d_p, nullptr, b, s);
+            }
+            gpu::box_copy(n, d_p, x, s);
+        });
+        *result = solve(M, b, x, tolerance, max_iterations, nullptr, 0);
+        if (result->converged) {
+            prof().run(K_OTHER, n * 16.0, [&] { gpu::box_copy(n, x, d_p, s); });
+            drain();
+        }
     });
 }
 

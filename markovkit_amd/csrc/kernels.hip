@@ -129,9 +129,14 @@ __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) 
     const i64 r = slice * kSlice + lane;
     const bool has_row = slice < a.A.n_slices && r < a.A.n;
     u32 len = 0, ec = 0, base = 0;
+    T ex = 0;
     if (has_row) {
         len = a.A.rlen[r];
-        ec = a.A.rcode[r];
+        if (a.A.exit) {
+            ex = static_cast<T>(a.A.exit[r]);
+        } else {
+            ec = a.A.rcode[r];
+        }
         base = a.A.soff[slice] + lane;
     }
     __syncthreads();
@@ -173,7 +178,7 @@ __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) 
         }
     }
     double sq = 0.0;
-    if (has_row) sq = finish<M>(a, vec_row(a.A, static_cast<u32>(r)), sum, edict[ec]);
+    if (has_row) sq = finish<M>(a, vec_row(a.A, static_cast<u32>(r)), sum, a.A.exit ? ex : edict[ec]);
     if constexpr (M == csr_mode::shifted_residual) {
         const double t = block_sum(sq, red64);
         if (tid == 0 && a.partial) a.partial[b] = t;
@@ -248,7 +253,7 @@ void launch_csr(csr_args<VT, XT, T> a, hipStream_t s) {
 
 template <typename T>
 __device__ __forceinline__ T row_exit(const csr_view<T> &A, i64 r) {
-    return A.coded ? A.edict[A.rcode[r]] : A.exit[r];
+    return A.exit ? A.exit[r] : A.edict[A.rcode[r]];
 }
 
 template <typename T, typename IN>
@@ -311,7 +316,7 @@ __device__ __forceinline__ T gather_row(const csr_view<T> &A, u32 i, const T *x,
     if (A.coded) {
         const u32 len = A.rlen[i];
         const u32 base = A.soff[i >> 6] + (i & 63u);
-        exit_i = A.edict[A.rcode[i]];
+        exit_i = A.exit ? A.exit[i] : A.edict[A.rcode[i]];
         for (u32 k = 0; k < len; ++k) {
             const u32 e = A.ent[base + k * kSlice];
             sum += A.vdict[e >> kColBits] * x[e & kColMask];

@@ -58,7 +58,8 @@ struct csr_view {
     const VT *val = nullptr;
     const VT *exit = nullptr;    // n
     // coded sliced encoding (active when coded != 0; ent may be null when the operator has
-    // no entries)
+    // no entries). A coded operator whose exit rates take more than 256 values may keep them
+    // per row in `exit` instead (rcode and edict then unused, n_edict = 0).
     int coded = 0;
     i64 n_slices = 0;
     const u32 *soff = nullptr;   // n_slices + 1: first entry of each slice

@@ -240,9 +240,9 @@ void check_view(const gpu::csr_view<VT> &v, u64 nnz) {
     if (v.coded) {
         ok = ok && v.n_slices == (v.n + gpu::kSlice - 1) / gpu::kSlice &&
              v.n_blocks * (gpu::kThreads / gpu::kSlice) >= v.n_slices &&
-             (v.n == 0 || (v.soff && v.rlen && v.rcode && v.edict && v.vdict)) &&
+             (v.n == 0 || (v.soff && v.rlen && v.vdict && (v.exit || (v.rcode && v.edict)))) &&
              (nnz == 0 || v.ent) && v.n_vdict > 0 && v.n_vdict <= gpu::kDictMax &&
-             v.n_edict > 0 && v.n_edict <= gpu::kDictMax;
+             (v.exit || (v.n_edict > 0 && v.n_edict <= gpu::kDictMax));
     } else {
         ok = ok && (v.n == 0 || (v.blocks && v.ptr && v.exit)) && (nnz == 0 || (v.col && v.val));
     }
@@ -250,9 +250,11 @@ void check_view(const gpu::csr_view<VT> &v, u64 nnz) {
 }
 
 // upload an operator (gather CSR + exit rates) in the coded sliced layout when it fits, else
-// in the plain CSR-stream layout
+// in the plain CSR-stream layout; with exit_array the coded layout also takes operators whose
+// exit rates overflow the dictionary (kept per row; only the k_csr_sliced passes, row_exit and
+// gather_row read that form)
 template <typename VT>
-dev_csr<VT> upload_csr(const mk::generator_csr &g) {
+dev_csr<VT> upload_csr(const mk::generator_csr &g, bool exit_array = false) {
     dev_csr<VT> d;
     d.n = g.n;
     d.nnz = g.ptr.empty() ? 0 : g.ptr.back();
@@ -265,7 +267,7 @@ dev_csr<VT> upload_csr(const mk::generator_csr &g) {
                   "host and device agree on the coded layout");
     (void)max_len;
     mk::sliced_operator<VT> so;
-    if (mk::encode_sliced<VT>(g, so)) {
+    if (mk::encode_sliced<VT>(g, so, exit_array)) {
         const i64 n_slices = static_cast<i64>(so.soff.size()) - 1;
         d.coded = true;
         d.padded = so.padded;
@@ -285,6 +287,10 @@ dev_csr<VT> upload_csr(const mk::generator_csr &g) {
         d.view.edict = dp<VT>(d.edict);
         d.view.n_vdict = static_cast<int>(so.vdict.size());
         d.view.n_edict = static_cast<int>(so.edict.size());
+        if (!so.exit.empty()) {
+            d.exit = upload(so.exit);
+            d.view.exit = dp<VT>(d.exit);
+        }
         d.view.coded = 1;
         check_view(d.view, d.nnz);
         return d;
@@ -328,6 +334,7 @@ double op_bytes(const dev_csr<VT> &A, std::size_t xs, std::size_t ts, int vector
     const double n = double(A.n), nnz = double(A.nnz);
     double b = A.coded ? double(A.padded) * 4.0 + n * 2.0 + double(A.view.n_slices + 1) * 4.0
                        : nnz * double(sizeof(VT) + 4) + (n + 1) * 4.0 + n * sizeof(VT);
+    if (A.coded && A.view.exit) b += n * (double(sizeof(VT)) - 1.0); // per-row exit, no code
     return b + n * double(xs) + n * double(ts) * vectors;
 }
 

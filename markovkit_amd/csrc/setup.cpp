@@ -972,7 +972,7 @@ u32 code_of(const std::vector<VT> &dict, double v) {
 } // namespace
 
 template <typename VT>
-bool encode_sliced(const generator_csr &g, sliced_operator<VT> &out) {
+bool encode_sliced(const generator_csr &g, sliced_operator<VT> &out, bool exit_array) {
     const i64 n = g.n;
     for (i64 i = 0; i < n; ++i)
         if (g.ptr[i + 1] - g.ptr[i] > 255) return false;
@@ -983,7 +983,10 @@ bool encode_sliced(const generator_csr &g, sliced_operator<VT> &out) {
     out.vdict = dictionary<VT>(g.val);
     if (out.vdict.empty()) return false;
     out.edict = dictionary<VT>(g.exit);
-    if (out.edict.empty()) return false;
+    if (out.edict.empty()) {
+        if (!exit_array) return false;
+        out.exit.assign(g.exit.begin(), g.exit.end());
+    }
     const i64 n_slices = (n + kSliceRows - 1) / kSliceRows;
     out.soff.assign(n_slices + 1, 0);
     u64 total = 0;
@@ -1006,7 +1009,8 @@ bool encode_sliced(const generator_csr &g, sliced_operator<VT> &out) {
                 const i64 i = sl * kSliceRows + l;
                 if (i >= n) break;
                 out.rlen[i] = static_cast<std::uint8_t>(g.ptr[i + 1] - g.ptr[i]);
-                out.rcode[i] = static_cast<std::uint8_t>(code_of(out.edict, g.exit[i]));
+                if (!out.edict.empty())
+                    out.rcode[i] = static_cast<std::uint8_t>(code_of(out.edict, g.exit[i]));
                 u64 k = 0;
                 for (u64 p = g.ptr[i]; p < g.ptr[i + 1]; ++p, ++k)
                     out.ent[out.soff[sl] + k * kSliceRows + l] =
@@ -1017,7 +1021,7 @@ bool encode_sliced(const generator_csr &g, sliced_operator<VT> &out) {
     return true;
 }
 
-template bool encode_sliced<double>(const generator_csr &, sliced_operator<double> &);
-template bool encode_sliced<float>(const generator_csr &, sliced_operator<float> &);
+template bool encode_sliced<double>(const generator_csr &, sliced_operator<double> &, bool);
+template bool encode_sliced<float>(const generator_csr &, sliced_operator<float> &, bool);
 
 } // namespace mk

@@ -233,13 +233,15 @@ struct sliced_operator {
     std::vector<u32> ent;
     std::vector<std::uint8_t> rlen, rcode;
     std::vector<VT> vdict, edict;
+    std::vector<VT> exit; // per-row exit rates when edict is empty (exit_array encodings)
     u64 padded = 0;
 };
 
 // false when the operator does not fit the coded layout (a column at or above 2^24, more than 256
 // distinct rates or exit rates, rows longer than 255 entries, or 2^32 stored entries)
+// With exit_array, exit rates that do not fit the 256-entry dictionary are kept per row instead.
 template <typename VT>
-bool encode_sliced(const generator_csr &g, sliced_operator<VT> &out);
+bool encode_sliced(const generator_csr &g, sliced_operator<VT> &out, bool exit_array = false);
 
 unsigned worker_threads();
 

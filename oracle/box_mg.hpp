@@ -215,6 +215,8 @@ struct level {
     Vec leak;
     std::vector<int> population;
     std::vector<std::uint8_t> colour;
+    std::vector<Index> by_colour, colour_begin; // rows of colour c: by_colour[colour_begin[c] ..
+                                                // colour_begin[c + 1]), ascending
     std::vector<Index> block_offsets;
     std::vector<block_ilu> ilu;
     std::unique_ptr<level> coarse;
@@ -289,11 +291,21 @@ struct level {
             }
             A.ptr.push_back(static_cast<Index>(A.val.size()));
         });
-        if (p.smoother == smoother_kind::colour_gs)
+        if (p.smoother == smoother_kind::colour_gs) {
             for (Index r = 0; r < n; ++r)
                 for (Index j = A.ptr[r]; j < A.ptr[r + 1]; ++j)
                     if (A.col[j] != r && colour[A.col[j]] == colour[r])
                         throw std::invalid_argument("oracle box: a coupling joins two states of one colour");
+            colour_begin.assign(5, 0);
+            for (int c = 0; c < 4; ++c) {
+                colour_begin[c + 1] = colour_begin[c];
+                for (Index r = 0; r < n; ++r)
+                    if (colour[r] == c) {
+                        by_colour.push_back(r);
+                        ++colour_begin[c + 1];
+                    }
+            }
+        }
         residual.assign(n, 0.0);
         scratch.assign(n, 0.0);
         const bool dense = p.terminal_rows > 0 && static_cast<long>(n) <= p.terminal_rows;
@@ -430,8 +442,7 @@ struct level {
             case smoother_kind::colour_gs:
                 for (int k = 0; k < 4; ++k) {
                     const int c = reverse ? 3 - k : k;
-                    for (Index r = 0; r < n; ++r)
-                        if (colour[r] == c) A.gs_row(r, b, x);
+                    for (Index q = colour_begin[c]; q < colour_begin[c + 1]; ++q) A.gs_row(by_colour[q], b, x);
                 }
                 break;
             case smoother_kind::block_ilu:

@@ -180,14 +180,14 @@ mk_status mk_gmres_device(mk_generator gen, double shift, mk_sector_mg mg, const
  * <= cap, in the population box order (total first, voxel 0 most significant; :66-152). Every
  * level holds the complete operator shift I - Q: diffusion at D cells^2 per molecule and
  * neighbour, births c3 vol + c1 n (n - 1) / (2 vol), deaths c4 n + c2 n (n - 1) (n - 2) /
- * (6 vol^2) (:410-412), births at the cap lost (absorbing). Coarse levels halve the cells and
- * double D (:529-531); transfers are the multinomial weights of :532-557. The one-voxel level is
- * solved by the Thomas algorithm (:512-527); with terminal_rows > 0 the first level of at most
+ * (6 vol^2) (:400-402), births at the cap lost (absorbing). Coarse levels halve the cells and
+ * double D (:538-541); transfers are the multinomial weights of :555-572. The one-voxel level is
+ * solved by the Thomas algorithm (:519-535); with terminal_rows > 0 the first level of at most
  * that many states is solved exactly with a dense inverse instead (no driver counterpart).
  * Smoothers: MK_BOX_GS is Gauss-Seidel in four-colour order (colour = 2 (total mod 2) + (sum of
  * the molecules' voxel coordinate sums mod 2); replaces the driver's lexicographic "gs", :178-195,
- * whose sweep is sequential), MK_BOX_JACOBI the driver's "jacobi" (w = 2/3, :171-175).
- * Cycles (:817-878): cycles = 1 (V) or 2 (W), cycle_type MK_CYCLE_K for the K-cycle. FP64. */
+ * whose sweep is sequential), MK_BOX_JACOBI the driver's "jacobi" (w = 2/3, :180-186).
+ * Cycles (:819-878): cycles = 1 (V) or 2 (W), cycle_type MK_CYCLE_K for the K-cycle. FP64. */
 typedef struct mk_box_mg_s *mk_box_mg;
 enum { MK_BOX_GS = 0, MK_BOX_JACOBI = 1 };
 enum { MK_CYCLE_STANDARD = 0, MK_CYCLE_K = 1 };
@@ -209,11 +209,11 @@ typedef struct mk_box_mg_params {
 typedef struct mk_box_mg_result {
     int32_t iterations; /* cycles run */
     int32_t converged;  /* error < tolerance */
-    double error;       /* |b - A x|_1 / shift after the last cycle (:1204-1206) */
+    double error;       /* |b - A x|_1 / shift after the last cycle (:884-887, :1215) */
     int64_t history;    /* number of errors the iteration produced (the first before any cycle) */
 } mk_box_mg_result;
 
-/* Level(p) with the hierarchy (rdme_full_multigrid.cpp:298-560) */
+/* Level(p) with the hierarchy (rdme_full_multigrid.cpp:298-573) */
 mk_status mk_box_mg_create(const mk_box_mg_params *params, mk_box_mg *out);
 mk_status mk_box_mg_destroy(mk_box_mg mg);
 mk_status mk_box_mg_info(mk_box_mg mg, int64_t *rows, int32_t *levels);
@@ -224,11 +224,11 @@ mk_status mk_box_mg_level(mk_box_mg mg, int32_t level, int64_t *rows, int64_t *v
 /* the finest operator as a generator (borrowed: valid until mk_box_mg_destroy; do not destroy
  * it), for mk_gmres_box, mk_expv, mk_shifted_apply */
 mk_status mk_box_mg_generator(mk_box_mg mg, mk_generator *gen);
-/* Level::cycle(b, x) (:817-848): one cycle from the guess x, in place (vectors in HBM) */
+/* Level::cycle(b, x) (:819-848): one cycle from the guess x, in place (vectors in HBM) */
 mk_status mk_box_mg_cycle_device(mk_box_mg mg, const double *d_b, double *d_x);
-/* Level::apply(b, x) (:879-880): one cycle from x = 0 */
+/* Level::apply(b, x) (:883): one cycle from x = 0 */
 mk_status mk_box_mg_apply_device(mk_box_mg mg, const double *d_b, double *d_z);
-/* run() with --method mg (:1180-1212): cycles from the guess x until |b - A x|_1 / shift <
+/* run() with --method mg (:1186-1212): cycles from the guess x until |b - A x|_1 / shift <
  * tolerance or max_iterations cycles; history (NULL allowed) receives up to history_cap errors,
  * the first before any cycle */
 mk_status mk_box_mg_solve(mk_box_mg mg, const double *b, double *x, double tolerance,
@@ -237,7 +237,7 @@ mk_status mk_box_mg_solve(mk_box_mg mg, const double *b, double *x, double toler
 mk_status mk_box_mg_solve_device(mk_box_mg mg, const double *d_b, double *d_x, double tolerance,
                                  int32_t max_iterations, double *history, int64_t history_cap,
                                  mk_box_mg_result *result);
-/* One time step of run() (:1183-1244) in place on the distribution p (HBM, box order): the
+/* One time step of run() (:1174-1245) in place on the distribution p (HBM, box order): the
  * solve of (shift I - Q) p_next = shift p (backward Euler, step 1 / shift; also the two half
  * steps of the driver's Rannacher start) or, with crank_nicolson != 0, = 2 shift p -
  * (shift I - Q) p (Crank-Nicolson, step 2 / shift), from the guess p. p is replaced only when
@@ -245,8 +245,8 @@ mk_status mk_box_mg_solve_device(mk_box_mg mg, const double *d_b, double *d_x, d
 mk_status mk_box_mg_step_device(mk_box_mg mg, double *d_p, int32_t crank_nicolson,
                                 double tolerance, int32_t max_iterations,
                                 mk_box_mg_result *result);
-/* run() with --method mg-gmres (:1200-1209): flexible GMRES on (shift I - Q) x = b preconditioned
- * by one cycle (standard cycles only; the driver rejects the K-cycle there, :1437-1438). gen is
+/* run() with --method mg-gmres (:1200-1213): flexible GMRES on (shift I - Q) x = b preconditioned
+ * by one cycle (standard cycles only; the driver rejects the K-cycle there, :1441-1442). gen is
  * mk_box_mg_generator's; the driver's tolerance is tolerance * shift / sqrt(rows). */
 mk_status mk_gmres_box(mk_generator gen, double shift, mk_box_mg mg, const double *b, double *x,
                        const mk_gmres_options *options, mk_solver_result *result);

@@ -3,7 +3,7 @@ device: every level holds the complete single-species reaction-diffusion operato
 Cartesian grid (Schlogl births and deaths, diffusion, absorbing population cap), coarsened by
 halving the cells and doubling the diffusivity.
 
-Parameters follow the driver's command line (rdme_full_multigrid.cpp:50-63, :1366-1451):
+Parameters follow the driver's command line (rdme_full_multigrid.cpp:49-63, :1366-1451):
 dimension, cells, cap, diffusion, shift, the Schlogl constants c1..c4 (``linear(s)`` gives the
 driver's --linear s), sweeps, cycles (1 V, 2 W) and kcycle (--cycle-type k). Smoothers:
 "colour-gs" (Gauss-Seidel in four-colour order; the GPU's replacement for the driver's
@@ -107,7 +107,7 @@ class BoxMultigrid:
 
     def initial_problem(self, initial: int):
         """the driver's first step: b = shift p0, x0 = p0, p0 the state with `initial` molecules
-        dealt round-robin over the voxels (rdme_full_multigrid.cpp:1119-1121)"""
+        dealt round-robin over the voxels (rdme_full_multigrid.cpp:1121-1124)"""
         from .api import PopulationBox
         x = np.zeros(self.voxels, np.int32)
         for k in range(initial):
@@ -196,7 +196,7 @@ def graphs_enable(on: bool) -> None:
 # (x, iterations, error, history) and apply_operator(x) = (shift I - Q) x: the device
 # BoxMultigrid, or the oracle's restatement on the CPU.
 
-HIGH_POPULATION = 21  # the driver's high_mass threshold (rdme_full_multigrid.cpp:1219)
+HIGH_POPULATION = 21  # the driver's high_mass threshold (rdme_full_multigrid.cpp:1227)
 
 
 def population_offsets(voxels: int, cap: int) -> np.ndarray:
@@ -214,7 +214,7 @@ def marginal(p, voxels: int, cap: int) -> np.ndarray:
 
 def time_steps(M, p0, steps: int, integrator: str = "be", tolerance: float = 1e-9,
                max_iterations: int = 1000, on_step=None):
-    """run() with --steps (rdme_full_multigrid.cpp:1183-1244): backward Euler steps of length
+    """run() with --steps (rdme_full_multigrid.cpp:1174-1245): backward Euler steps of length
     1 / shift, (shift I - Q) p_next = shift p; or "cn": two backward-Euler half steps (Rannacher
     start) and then Crank-Nicolson steps of 2 / shift, (shift I - Q) p_next = 2 shift p -
     (shift I - Q) p; steps + 1 solves. Each solve starts from the previous distribution. Stops

@@ -19,7 +19,7 @@
 // (ascending state index within a colour), so a colour pass streams contiguous rows; the finest
 // level's vectors are permuted on entry and exit. Weighted Jacobi (w = 2/3) is the driver's own.
 //
-// Cycle (rdme_full_multigrid.cpp:817-878): pre-smoothing (colours 0..3), residual, restriction
+// Cycle (rdme_full_multigrid.cpp:819-878): pre-smoothing (colours 0..3), residual, restriction
 // by summing children, the coarse correction (V: one coarse cycle, W: two, the exact bottom
 // solve once; or the K-cycle's at most two GCR steps), prolongation with the multinomial
 // weights, post-smoothing (colours 3..0). The standalone iteration of the driver's run()
@@ -251,7 +251,7 @@ struct host_level {
     int terminal = 0;
 };
 
-// the driver's Schlogl rate tables (rdme_full_multigrid.cpp:410-412): births +1, deaths -1,
+// the driver's Schlogl rate tables (rdme_full_multigrid.cpp:400-402): births +1, deaths -1,
 // rates[(r V + v) T + m] for m molecules in voxel v
 mk::reaction_tables schlogl_tables(const mk_box_mg_params &p, i64 V) {
     mk::reaction_tables t;
@@ -342,7 +342,7 @@ void colour_order(const mk_box_mg_params &p, host_level &h, bool check) {
     for (i64 r = 0; r < n; ++r) h.pos[h.ord[r]] = static_cast<u32>(r);
 }
 
-// the multinomial transfer of fine level f to level c (rdme_full_multigrid.cpp:532-557):
+// the multinomial transfer of fine level f to level c (rdme_full_multigrid.cpp:555-572):
 // the coarse state adds the counts of each 2^d block of voxels; weight = exp(log of the
 // multinomial probability of the fine arrangement), normalised over the coarse state's children
 // in ascending fine order
@@ -478,8 +478,8 @@ void build(mk_box_mg_s &M) {
             d.x = dbuf(n * sizeof(double));
         }
         if (h.terminal == 1) {
-            // Thomas coefficients of the tridiagonal one-voxel chain (rdme_full_multigrid.cpp:
-            // 512-527) from A = (shift + exit) - rates
+            // Thomas coefficients of the tridiagonal one-voxel chain
+            // (rdme_full_multigrid.cpp:519-535) from A = (shift + exit) - rates
             std::vector<double> lo(n, 0.0), up(n, 0.0), pv(n, 0.0);
             for (i64 i = 0; i < n; ++i) {
                 pv[i] = p.shift + h.g.exit[i];

@@ -16,11 +16,11 @@
 //       coarse level: cells / 2, D * 2; multinomial transfer weights normalised
 //       per parent; Thomas solve of the one-voxel level                          :298-560
 //   smooth dispatch (point smoothers, population-block sweeps forward / reverse)  :768-816
-//   cycle (V for cycles = 1, W for cycles = 2; the exact bottom solve visited once) :817-848
+//   cycle (V for cycles = 1, W for cycles = 2; the exact bottom solve visited once) :819-848
 //   K-cycle coarse solve (at most two GCR steps, the second skipped when the
 //       first reduces |g|^2 by 1/16)                                              :849-878
 //   the standalone iteration of run(): x0 = the initial distribution,
-//       error = |b - A x|_1 / shift, cycles until error < tolerance              :1180-1212
+//       error = |b - A x|_1 / shift, cycles until error < tolerance              :1186-1212
 //
 // Two options of the GPU library that the driver does not have are restated here as well, so
 // that the library's hierarchy has an exact CPU counterpart:
@@ -455,7 +455,7 @@ struct level {
         }
     }
 
-    // one cycle on A x = b from the guess x (rdme_full_multigrid.cpp:817-848)
+    // one cycle on A x = b from the guess x (rdme_full_multigrid.cpp:819-848)
     void cycle(const Vec &b, Vec &x) const {
         if (terminal) {
             solve_exact(b, x);
@@ -525,7 +525,7 @@ struct level {
     int n_levels() const { return 1 + (coarse ? coarse->n_levels() : 0); }
 };
 
-// run() of the driver with --method mg (rdme_full_multigrid.cpp:1180-1212): x holds the guess;
+// run() of the driver with --method mg (rdme_full_multigrid.cpp:1186-1212): x holds the guess;
 // history receives the error before every cycle and after the last
 struct solve_result {
     int iterations = 0;

@@ -297,7 +297,7 @@ def expv_jumps(first, to, rate, exit_, t, v, krylov_dim=30, tol=1e-8):
 class BoxMultigrid:
     """The paper driver's whole-box multigrid (box_mg.hpp) on the CPU (FP64).
 
-    Parameters follow the driver's command line (rdme_full_multigrid.cpp:50-63): Schlogl
+    Parameters follow the driver's command line (rdme_full_multigrid.cpp:49-63): Schlogl
     constants c1..c4, smoother "gs" | "jacobi" | "block-ilu" (the driver's) or "colour-gs"
     (the GPU library's four-colour order), cycles 1 (V) or 2 (W), kcycle for the K-cycle, and
     terminal_rows > 0 for an exact dense solve at the first level with at most that many states.

@@ -187,9 +187,14 @@ mk_status mk_gmres_device(mk_generator gen, double shift, mk_sector_mg mg, const
  * Smoothers: MK_BOX_GS is Gauss-Seidel in four-colour order (colour = 2 (total mod 2) + (sum of
  * the molecules' voxel coordinate sums mod 2); replaces the driver's lexicographic "gs", :178-195,
  * whose sweep is sequential), MK_BOX_JACOBI the driver's "jacobi" (w = 2/3, :180-186).
+ * The driver's sequential smoothers themselves run as level-scheduled triangular solves (rows in
+ * the box order, long dependency chains: slower per cycle, the driver's cycle counts):
+ * MK_BOX_GS_LEX is its "gs" (x += (D + L)^-1 (b - A x), backward (D + U) after the correction),
+ * MK_BOX_BLOCK_ILU its structured "block-ilu" (population blocks in order, each the block's
+ * residual and its ILU(0) forward and backward solves, :204-296, :795-815).
  * Cycles (:819-878): cycles = 1 (V) or 2 (W), cycle_type MK_CYCLE_K for the K-cycle. FP64. */
 typedef struct mk_box_mg_s *mk_box_mg;
-enum { MK_BOX_GS = 0, MK_BOX_JACOBI = 1 };
+enum { MK_BOX_GS = 0, MK_BOX_JACOBI = 1, MK_BOX_GS_LEX = 2, MK_BOX_BLOCK_ILU = 3 };
 enum { MK_CYCLE_STANDARD = 0, MK_CYCLE_K = 1 };
 
 typedef struct mk_box_mg_params {

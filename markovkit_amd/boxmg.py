@@ -6,8 +6,10 @@ halving the cells and doubling the diffusivity.
 Parameters follow the driver's command line (rdme_full_multigrid.cpp:49-63, :1366-1451):
 dimension, cells, cap, diffusion, shift, the Schlogl constants c1..c4 (``linear(s)`` gives the
 driver's --linear s), sweeps, cycles (1 V, 2 W) and kcycle (--cycle-type k). Smoothers:
-"colour-gs" (Gauss-Seidel in four-colour order; the GPU's replacement for the driver's
-sequential lexicographic "gs") and "jacobi" (the driver's, w = 2/3). terminal_rows > 0 solves the
+"colour-gs" (Gauss-Seidel in four-colour order: one launch per colour), "jacobi" (the driver's,
+w = 2/3), and the driver's sequential "gs" (lexicographic) and "block-ilu" (population-block
+ILU(0)), run as level-scheduled triangular solves: the driver's cycle counts, but long
+dependency chains per sweep. terminal_rows > 0 solves the
 first level with at most that many states exactly (dense inverse) instead of recursing to the
 one-voxel Thomas solve.
 
@@ -29,7 +31,7 @@ from ._lib import SolverResult as _SolverResult
 from ._lib import check, lib
 from .api import DeviceVector, Generator, SolverResult
 
-SMOOTHERS = {"colour-gs": 0, "jacobi": 1}
+SMOOTHERS = {"colour-gs": 0, "jacobi": 1, "gs": 2, "block-ilu": 3}
 DEFAULT_RATES = (0.0133, 0.0006, 0.6924125, 0.114725)  # rdme_full_multigrid.cpp:57
 
 
@@ -59,9 +61,7 @@ class BoxMultigrid:
                  shift: float = 0.05, rates=None, smoother: str = "colour-gs", sweeps: int = 2,
                  cycles: int = 1, kcycle: bool = False, terminal_rows: int = 0):
         if smoother not in SMOOTHERS:
-            raise ValueError(
-                f"smoother must be one of {sorted(SMOOTHERS)} (the driver's sequential 'gs' and "
-                "'block-ilu' run in four-colour order on the device: use 'colour-gs')")
+            raise ValueError(f"smoother must be one of {sorted(SMOOTHERS)}")
         c1, c2, c3, c4 = DEFAULT_RATES if rates is None else rates
         self._params = _Params(int(dimension), int(cells), int(cap), float(diffusion), float(shift),
                                float(c1), float(c2), float(c3), float(c4), SMOOTHERS[smoother],

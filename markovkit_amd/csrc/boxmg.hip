@@ -133,6 +133,14 @@ __global__ void __launch_bounds__(kBoxThreads)
         b[i] = ap ? s2 * p[i] - ap[i] : s * p[i];
 }
 
+// u = r * pinv (the diagonal solve that starts a backward Gauss-Seidel sweep)
+__global__ void __launch_bounds__(kBoxThreads)
+    k_box_scale(i64 n, const double *r, const double *pinv, double *u) {
+    for (i64 i = blockIdx.x * static_cast<i64>(kBoxThreads) + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * kBoxThreads)
+        u[i] = r[i] * pinv[i];
+}
+
 __global__ void __launch_bounds__(kBoxThreads) k_box_copy(i64 n, const double *src, double *dst) {
     for (i64 i = blockIdx.x * static_cast<i64>(kBoxThreads) + threadIdx.x; i < n;
          i += static_cast<i64>(gridDim.x) * kBoxThreads)
@@ -179,6 +187,11 @@ void box_rhs(i64 n, double shift, const double *p, const double *ap, double *b, 
     hipLaunchKernelGGL(k_box_rhs, dim3(grid_of(n)), dim3(kBoxThreads), 0, s, n, shift, p, ap, b);
 }
 
+void box_scale(i64 n, const double *r, const double *pinv, double *u, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_box_scale, dim3(grid_of(n)), dim3(kBoxThreads), 0, s, n, r, pinv, u);
+}
+
 void box_copy(i64 n, const double *src, double *dst, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_box_copy, dim3(grid_of(n)), dim3(kBoxThreads), 0, s, n, src, dst);
@@ -204,6 +217,18 @@ struct box_level {
     dbuf kv1, kw1, kv2, kw2, kr; // K-cycle vectors (the next level's size)
     u64 nnz = 0;
     bool wide_restrict = false;  // coarse states average more than 16 children
+    // the driver's sequential smoothers (rows in the box order): level-scheduled triangular
+    // solves. Lexicographic Gauss-Seidel: one schedule over the whole level on A, pinv = 1 /
+    // diagonal. Population-block ILU: the in-block operator Ain, pinv = 1 / ILU pivot, one
+    // schedule per population block, and one row-range operator per block for its residual.
+    struct tri_launch {
+        int g0 = 0, g1 = 0;
+        i64 wide_rows = 0; // > 0: one group of this many rows over a grid
+    };
+    dev_csr<double> Ain;
+    std::vector<dev_csr<double>> Ablock;
+    std::vector<std::vector<tri_launch>> fw, bw; // per block (one entry for Gauss-Seidel)
+    dbuf pinv, u, fw_rows, fw_ptr, bw_rows, bw_ptr;
 };
 
 } // namespace mk::rt
@@ -236,6 +261,10 @@ std::atomic<int> &box_graph_switch() {
         return (env && env[0] == '0') ? 0 : 1;
     }()};
     return on;
+}
+
+bool sequential(const mk_box_mg_params &p) {
+    return p.smoother == MK_BOX_GS_LEX || p.smoother == MK_BOX_BLOCK_ILU;
 }
 
 struct host_level {
@@ -434,6 +463,139 @@ mk::generator_csr row_range(const mk::generator_csr &g, i64 r0, i64 r1) {
     return out;
 }
 
+
+// Level schedule of triangular solves: rows grouped by (block, dependency depth); per block the
+// launches cover its groups: groups of at least kWideGroup rows get a grid each, runs of smaller
+// groups run in one workgroup with a barrier between groups
+constexpr i64 kWideGroup = 2048;
+
+void tri_schedule(const std::vector<i64> &depth, const std::vector<i64> &block_begin, dbuf &rows_d,
+                  dbuf &ptr_d, std::vector<std::vector<box_level::tri_launch>> &launches) {
+    const i64 nb = static_cast<i64>(block_begin.size()) - 1;
+    std::vector<i64> gbase(nb + 1, 0);
+    for (i64 k = 0; k < nb; ++k) {
+        i64 deepest = -1;
+        for (i64 i = block_begin[k]; i < block_begin[k + 1]; ++i) deepest = std::max(deepest, depth[i]);
+        gbase[k + 1] = gbase[k] + deepest + 1;
+    }
+    const i64 groups = gbase[nb];
+    if (groups >= (i64{1} << 31)) throw std::overflow_error("box multigrid: schedule too deep");
+    std::vector<u32> ptr(groups + 1, 0), rows(depth.size());
+    std::vector<i64> group_of(depth.size());
+    for (i64 k = 0; k < nb; ++k)
+        for (i64 i = block_begin[k]; i < block_begin[k + 1]; ++i) group_of[i] = gbase[k] + depth[i];
+    for (i64 g : group_of) ++ptr[g + 1];
+    for (i64 g = 0; g < groups; ++g) ptr[g + 1] += ptr[g];
+    std::vector<u32> fill(ptr.begin(), ptr.end() - 1);
+    for (std::size_t i = 0; i < group_of.size(); ++i) rows[fill[group_of[i]]++] = static_cast<u32>(i);
+    rows_d = upload(rows);
+    ptr_d = upload(ptr);
+    launches.assign(nb, {});
+    for (i64 k = 0; k < nb; ++k) {
+        for (i64 g = gbase[k]; g < gbase[k + 1];) {
+            const i64 size = ptr[g + 1] - ptr[g];
+            if (size >= kWideGroup) {
+                launches[k].push_back({static_cast<int>(g), static_cast<int>(g + 1), size});
+                ++g;
+                continue;
+            }
+            i64 e = g;
+            while (e < gbase[k + 1] && ptr[e + 1] - ptr[e] < kWideGroup) ++e;
+            launches[k].push_back({static_cast<int>(g), static_cast<int>(e), 0});
+            g = e;
+        }
+    }
+}
+
+// the two sequential smoothers of the driver on level h (box order), device data into d
+void build_sequential(const mk_box_mg_params &p, const host_level &h, box_level &d) {
+    const i64 n = h.g.n;
+    const auto &g = h.g;
+    std::vector<double> pinv(n);
+    std::vector<i64> fdepth(n, 0), bdepth(n, 0);
+    std::vector<i64> block_begin;
+    if (p.smoother == MK_BOX_GS_LEX) {
+        // (D + L) and (D + U) of the whole level: every lower (upper) column is a dependency
+        for (i64 i = 0; i < n; ++i) {
+            pinv[i] = 1.0 / (p.shift + g.exit[i]);
+            for (u64 q = g.ptr[i]; q < g.ptr[i + 1] && g.col[q] < i; ++q)
+                fdepth[i] = std::max(fdepth[i], fdepth[g.col[q]] + 1);
+        }
+        for (i64 i = n; i-- > 0;)
+            for (u64 q = g.ptr[i + 1]; q-- > g.ptr[i] && g.col[q] > i;)
+                bdepth[i] = std::max(bdepth[i], bdepth[g.col[q]] + 1);
+        block_begin = {0, n};
+        tri_schedule(fdepth, block_begin, d.fw_rows, d.fw_ptr, d.fw);
+        tri_schedule(bdepth, block_begin, d.bw_rows, d.bw_ptr, d.bw);
+        d.pinv = upload(pinv);
+        d.u = dbuf(n * sizeof(double));
+        return;
+    }
+    // population blocks: rows of total k are [offset(k), offset(k + 1)) in the box order
+    block_begin.assign(p.cap + 2, 0);
+    for (int k = 0; k <= p.cap; ++k) block_begin[k] = h.space.begin(k);
+    block_begin[p.cap + 1] = n;
+    std::vector<int> block_of(n);
+    for (int k = 0; k <= p.cap; ++k)
+        for (i64 i = block_begin[k]; i < block_begin[k + 1]; ++i) block_of[i] = k;
+    // the in-block operator: the diffusion entries (same population); exit rates stay the
+    // level's, so that its diagonal is the level operator's
+    mk::generator_csr in;
+    in.n = n;
+    in.ptr.assign(n + 1, 0);
+    for (i64 i = 0; i < n; ++i) {
+        u64 c = 0;
+        for (u64 q = g.ptr[i]; q < g.ptr[i + 1]; ++q) c += block_of[g.col[q]] == block_of[i];
+        in.ptr[i + 1] = in.ptr[i] + c;
+    }
+    in.col.reserve(in.ptr[n]);
+    in.val.reserve(in.ptr[n]);
+    for (i64 i = 0; i < n; ++i)
+        for (u64 q = g.ptr[i]; q < g.ptr[i + 1]; ++q)
+            if (block_of[g.col[q]] == block_of[i]) {
+                in.col.push_back(g.col[q]);
+                in.val.push_back(g.val[q]);
+            }
+    in.exit = g.exit;
+    in.leak = g.leak;
+    // structured ILU(0) pivots (rdme_full_multigrid.cpp:210-233): the block graph is bipartite,
+    // so only the diagonal changes: p_i = a_ii - sum_{j < i in the block} (a_ij / p_j) a_ji
+    std::vector<double> pivot(n);
+    for (i64 i = 0; i < n; ++i) {
+        double dd = p.shift + in.exit[i];
+        for (u64 q = in.ptr[i]; q < in.ptr[i + 1] && in.col[q] < i; ++q) {
+            const u32 j = in.col[q];
+            const auto b = in.col.begin() + in.ptr[j], e = in.col.begin() + in.ptr[j + 1];
+            const auto it = std::lower_bound(b, e, static_cast<u32>(i));
+            if (it == e || *it != static_cast<u32>(i))
+                throw std::logic_error("box multigrid: a diffusion jump has no reverse jump");
+            // a_ij = -val(q), a_ji = -val(j -> i entry): (a_ij / p_j) a_ji = val val' / p_j
+            dd -= in.val[q] / pivot[j] * in.val[in.ptr[j] + (it - b)];
+            fdepth[i] = std::max(fdepth[i], fdepth[j] + 1);
+        }
+        if (!(dd > 0.0) || !std::isfinite(dd)) throw std::runtime_error("box multigrid: ILU pivot");
+        pivot[i] = dd;
+        pinv[i] = 1.0 / dd;
+    }
+    for (i64 i = n; i-- > 0;)
+        for (u64 q = in.ptr[i + 1]; q-- > in.ptr[i] && in.col[q] > i;)
+            bdepth[i] = std::max(bdepth[i], bdepth[in.col[q]] + 1);
+    tri_schedule(fdepth, block_begin, d.fw_rows, d.fw_ptr, d.fw);
+    tri_schedule(bdepth, block_begin, d.bw_rows, d.bw_ptr, d.bw);
+    d.pinv = upload(pinv);
+    d.u = dbuf(n * sizeof(double));
+    d.Ain = upload_csr<double>(in, true);
+    std::vector<u32> rm(n);
+    std::iota(rm.begin(), rm.end(), 0u);
+    d.rowmap = upload(rm);
+    d.Ablock.resize(p.cap + 1);
+    for (int k = 0; k <= p.cap; ++k) {
+        if (block_begin[k + 1] == block_begin[k]) continue;
+        d.Ablock[k] = upload_csr<double>(row_range(g, block_begin[k], block_begin[k + 1]), true);
+        d.Ablock[k].view.rowmap = dp<u32>(d.rowmap) + block_begin[k];
+    }
+}
+
 void build(mk_box_mg_s &M) {
     const auto &p = M.p;
     std::vector<host_level> H;
@@ -447,7 +609,9 @@ void build(mk_box_mg_s &M) {
         build_level_operator(p, h);
         const bool dense = p.terminal_rows > 0 && h.g.n <= p.terminal_rows;
         h.terminal = dense ? 2 : (cells == 1 ? 1 : 0);
-        if (h.terminal) {
+        if (h.terminal || sequential(p)) {
+            // terminal levels and the driver's sequential smoothers keep the box order
+            if (!h.terminal && cells % 2) throw std::invalid_argument("box multigrid: cells must be a power of two");
             h.ord.resize(h.g.n);
             std::iota(h.ord.begin(), h.ord.end(), 0u);
             h.pos = h.ord;
@@ -515,16 +679,19 @@ void build(mk_box_mg_s &M) {
             d.partial = dbuf(static_cast<std::size_t>(gpu::box_dense_partials(n)) * sizeof(double));
             continue;
         }
-        const auto gp = permuted(h);
-        for (int c = 0; c < 4; ++c) {
+        const auto gp = sequential(p) ? h.g : permuted(h);
+        if (sequential(p)) build_sequential(p, h, d);
+        for (int c = 0; c < 4 && !sequential(p); ++c) {
             d.cb[c + 1] = d.cb[c];
             while (d.cb[c + 1] < n && h.colour[h.ord[d.cb[c + 1]]] == c) ++d.cb[c + 1];
         }
-        if (d.cb[4] != n) throw std::logic_error("box multigrid: colour order");
-        std::vector<u32> rm(n);
-        std::iota(rm.begin(), rm.end(), 0u);
-        d.rowmap = upload(rm);
-        for (int c = 0; c < 4; ++c) {
+        if (!sequential(p)) {
+            if (d.cb[4] != n) throw std::logic_error("box multigrid: colour order");
+            std::vector<u32> rm(n);
+            std::iota(rm.begin(), rm.end(), 0u);
+            d.rowmap = upload(rm);
+        }
+        for (int c = 0; c < 4 && !sequential(p); ++c) {
             if (d.cb[c + 1] == d.cb[c]) continue;
             d.Ac[c] = upload_csr<double>(row_range(gp, d.cb[c], d.cb[c + 1]), true);
             d.Ac[c].view.rowmap = dp<u32>(d.rowmap) + d.cb[c];
@@ -588,8 +755,43 @@ void smooth(mk_box_mg_s &M, std::size_t l, const double *b, double *x, bool reve
     auto &P = prof();
     const hipStream_t s = stream();
     const double shift = M.p.shift;
+    auto forward = [&](const dev_csr<double> &A, const box_level::tri_launch &g, const double *r) {
+        gpu::ilu_forward<double>(A.view, dp<u32>(lv.fw_rows), dp<u32>(lv.fw_ptr), g.g0, g.g1, g.wide_rows,
+                                 dp<double>(lv.pinv), r, dp<double>(lv.u), s);
+    };
+    auto backward = [&](const dev_csr<double> &A, const box_level::tri_launch &g) {
+        gpu::ilu_backward<double>(A.view, dp<u32>(lv.bw_rows), dp<u32>(lv.bw_ptr), g.g0, g.g1, g.wide_rows,
+                                  dp<double>(lv.pinv), dp<double>(lv.u), x, s);
+    };
     for (int sweep = 0; sweep < M.p.sweeps; ++sweep) {
-        if (M.p.smoother == MK_BOX_GS) {
+        if (M.p.smoother == MK_BOX_GS_LEX) {
+            // x += (D + L)^-1 (b - A x) forward, x += (D + U)^-1 (b - A x) backward
+            double *r = dp<double>(lv.r), *u = dp<double>(lv.u);
+            P.run(level_class(l, K_SWEEP), op_bytes(lv.A, 8, 8, 2), [&] {
+                gpu::level_residual<double>(lv.A.view, shift, b, x, r, s);
+                if (!reverse) {
+                    for (const auto &g : lv.fw[0]) forward(lv.A, g, r);
+                    gpu::axpy<double>(lv.n, 1.0, u, x, s);
+                } else {
+                    gpu::box_scale(lv.n, r, dp<double>(lv.pinv), u, s);
+                    for (const auto &g : lv.bw[0]) backward(lv.A, g);
+                }
+            });
+        } else if (M.p.smoother == MK_BOX_BLOCK_ILU) {
+            // population blocks in order (reversed after the coarse correction): the block's
+            // residual with the current x, then its ILU(0) solve added to x
+            double *r = dp<double>(lv.r);
+            P.run(level_class(l, K_SWEEP), op_bytes(lv.A, 8, 8, 2) + op_bytes(lv.Ain, 8, 8, 2), [&] {
+                const int nb = static_cast<int>(lv.Ablock.size());
+                for (int v = 0; v < nb; ++v) {
+                    const int k = reverse ? nb - 1 - v : v;
+                    if (lv.Ablock[k].n == 0) continue;
+                    gpu::level_residual<double>(lv.Ablock[k].view, shift, b, x, r, s);
+                    for (const auto &g : lv.fw[k]) forward(lv.Ain, g, r);
+                    for (const auto &g : lv.bw[k]) backward(lv.Ain, g);
+                }
+            });
+        } else if (M.p.smoother == MK_BOX_GS) {
             for (int k = 0; k < 4; ++k) {
                 const int c = reverse ? 3 - k : k;
                 if (lv.cb[c + 1] == lv.cb[c]) continue;
@@ -763,7 +965,9 @@ void check_params(const mk_box_mg_params &p) {
     require(p.shift > 0.0 && std::isfinite(p.shift), "box multigrid: shift must be positive");
     require(p.diffusion >= 0.0 && p.c1 >= 0.0 && p.c2 >= 0.0 && p.c3 >= 0.0 && p.c4 >= 0.0,
             "box multigrid: negative rate constant");
-    require(p.smoother == MK_BOX_GS || p.smoother == MK_BOX_JACOBI, "box multigrid: unknown smoother");
+    require(p.smoother == MK_BOX_GS || p.smoother == MK_BOX_JACOBI || p.smoother == MK_BOX_GS_LEX ||
+                p.smoother == MK_BOX_BLOCK_ILU,
+            "box multigrid: unknown smoother");
     require(p.sweeps >= 1, "box multigrid: at least one sweep");
     require(p.cycles == 1 || p.cycles == 2, "box multigrid: cycles must be 1 (V) or 2 (W)");
     require(p.cycle_type == MK_CYCLE_STANDARD || p.cycle_type == MK_CYCLE_K,

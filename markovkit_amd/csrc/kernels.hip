@@ -115,7 +115,8 @@ __device__ __forceinline__ double finish(const csr_args<VT, XT, T> &a, u32 r, T 
 // coded sliced operator: one wavefront per 64-row slice, four slices per workgroup
 constexpr int kSliceUnroll = 16;
 
-template <csr_mode M, typename VT, typename XT, typename T>
+// EX: exit rates per row (a.A.exit) instead of the exit dictionary
+template <csr_mode M, typename VT, typename XT, typename T, bool EX>
 __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) {
     __shared__ T vdict[kDictMax];
     __shared__ T edict[kDictMax];
@@ -132,7 +133,7 @@ __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) 
     T ex = 0;
     if (has_row) {
         len = a.A.rlen[r];
-        if (a.A.exit) {
+        if constexpr (EX) {
             ex = static_cast<T>(a.A.exit[r]);
         } else {
             ec = a.A.rcode[r];
@@ -178,7 +179,13 @@ __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) 
         }
     }
     double sq = 0.0;
-    if (has_row) sq = finish<M>(a, vec_row(a.A, static_cast<u32>(r)), sum, a.A.exit ? ex : edict[ec]);
+    if (has_row) {
+        if constexpr (EX) {
+            sq = finish<M>(a, vec_row(a.A, static_cast<u32>(r)), sum, ex);
+        } else {
+            sq = finish<M>(a, vec_row(a.A, static_cast<u32>(r)), sum, edict[ec]);
+        }
+    }
     if constexpr (M == csr_mode::shifted_residual) {
         const double t = block_sum(sq, red64);
         if (tid == 0 && a.partial) a.partial[b] = t;
@@ -244,8 +251,10 @@ void launch_csr(csr_args<VT, XT, T> a, hipStream_t s) {
     if (a.A.n_blocks == 0) return;
     a.chunk = static_cast<u32>(g_xcd_chunk);
     const dim3 grid(padded_grid(a.A.n_blocks, a.chunk));
-    if (a.A.coded) {
-        hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T>), grid, dim3(kThreads), 0, s, a);
+    if (a.A.coded && a.A.exit) {
+        hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, true>), grid, dim3(kThreads), 0, s, a);
+    } else if (a.A.coded) {
+        hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, false>), grid, dim3(kThreads), 0, s, a);
     } else {
         hipLaunchKernelGGL((k_csr_plain<M, VT, XT, T>), grid, dim3(kThreads), 0, s, a);
     }

@@ -41,7 +41,7 @@ CASES = {
     "1d-4-cap40-schlogl": dict(dimension=1, cells=4, cap=40, shift=0.05, rates=SCHLOGL),
 }
 
-ORACLE_SMOOTHER = {"colour-gs": "colour-gs", "jacobi": "jacobi"}
+ORACLE_SMOOTHER = {"colour-gs": "colour-gs", "jacobi": "jacobi", "gs": "gs", "block-ilu": "block-ilu"}
 
 
 def pair(case, smoother="colour-gs", sweeps=2, cycles=2, kcycle=False, terminal_rows=0):
@@ -69,7 +69,7 @@ def test_finest_operator_matches_oracle(case):
 
 @pytest.mark.parametrize("terminal_rows", [0, 200], ids=["thomas", "dense200"])
 @pytest.mark.parametrize("cycle", ["V", "W", "K"])
-@pytest.mark.parametrize("smoother", ["colour-gs", "jacobi"])
+@pytest.mark.parametrize("smoother", ["colour-gs", "jacobi", "gs", "block-ilu"])
 @pytest.mark.parametrize("case", list(CASES))
 def test_cycle_matches_oracle(case, smoother, cycle, terminal_rows):
     G, O = pair(case, smoother, sweeps=2, cycles=1 if cycle == "V" else 2, kcycle=cycle == "K",
@@ -86,7 +86,7 @@ def test_cycle_matches_oracle(case, smoother, cycle, terminal_rows):
 
 
 @pytest.mark.parametrize("cycle", ["V", "W", "K"])
-@pytest.mark.parametrize("smoother", ["colour-gs", "jacobi"])
+@pytest.mark.parametrize("smoother", ["colour-gs", "jacobi", "gs", "block-ilu"])
 @pytest.mark.parametrize("case", ["1d-64-cap2", "2d-4-cap4-schlogl", "1d-4-cap40-schlogl"])
 def test_standalone_iteration_matches_oracle(case, smoother, cycle):
     G, O = pair(case, smoother, sweeps=2, cycles=1 if cycle == "V" else 2, kcycle=cycle == "K")
@@ -126,8 +126,34 @@ def test_recorded_answer(name, terminal_rows):
           f"lexicographic-GS cycles)")
 
 
-def test_graph_replay_is_bitwise_eager():
-    G = mk.BoxMultigrid(1, 64, 3, 1.0, 0.5, LIN, "colour-gs", 2, 2)
+@pytest.mark.parametrize("name", ["graph_gap_cap2_w", "voxels_1d_64_mg", "voxels_1d_64_kcycle",
+                                  "voxels_2d_8_mg", "voxels_3d_4_mg", "block_v16_k4_gs_s1",
+                                  "block_v16_k4_block-ilu_s1", "block_v16_k4_block-ilu_s2",
+                                  "block_schlogl_v4_k40_block-ilu_s1_kcycle",
+                                  "block_v64_k4_block-ilu_s1"])
+def test_driver_smoothers_reproduce_recorded_cycles(name):
+    """with the driver's own smoother (lexicographic GS or population-block ILU) the device
+    takes the recorded number of cycles (one more or fewer where the last cycle lands at the
+    tolerance) and ends with the recorded mass"""
+    rec = RECORDED[name]
+    p = rec["params"]
+    G = mk.BoxMultigrid(p["dimension"], p["cells"], p["cap"], p["diffusion"], p["shift"],
+                        tuple(p["rates"]), p["smoother"], p["sweeps"], p["cycles"], p["kcycle"])
+    b, x0 = G.initial_problem(p["initial"])
+    x, it, err, hist = G.solve(b, x0, p["tolerance"], p["max_iterations"])
+    assert err < p["tolerance"]
+    assert abs(it - rec["iterations"]) <= 1, (it, rec["iterations"])
+    assert abs(x.sum() - rec["mass"]) <= 2 * p["tolerance"] + 1e-11
+    if "history" in rec:
+        H = np.array(rec["history"])
+        m = min(len(H), len(hist))
+        assert np.all(np.abs(hist[:5] - H[:5]) <= 1e-9 * H[:5])
+    print(f"[{name}] {G.rows()} states, {p['smoother']}: {it} cycles (driver {rec['iterations']})")
+
+
+@pytest.mark.parametrize("smoother", ["colour-gs", "gs", "block-ilu"])
+def test_graph_replay_is_bitwise_eager(smoother):
+    G = mk.BoxMultigrid(1, 64, 3, 1.0, 0.5, LIN, smoother, 2, 2)
     b, x0 = G.initial_problem(2)
     try:
         boxmg.graphs_enable(False)
@@ -174,4 +200,4 @@ def test_rejects_bad_parameters():
         with pytest.raises(ValueError):
             mk.BoxMultigrid(**{**dict(dimension=1, cells=4, cap=2, shift=0.5), **kw})
     with pytest.raises(ValueError):
-        mk.BoxMultigrid(smoother="gs")
+        mk.BoxMultigrid(smoother="lexicographic")

@@ -263,6 +263,12 @@ mk_status mk_gmres_box_device(mk_generator gen, double shift, mk_box_mg mg, cons
  * reads its step lengths on the host, always launches eagerly). Same kernels and arguments:
  * results are identical. No reference counterpart. */
 mk_status mk_box_mg_graphs_enable(int32_t on);
+/* Levels of at most 2,048 states run a visit's pre-smoothing, residual and restriction in one
+ * single-workgroup kernel, and its prolongation and post-smoothing in another, instead of about
+ * 20 launches (four-colour Gauss-Seidel and Jacobi; default on, MK_BOX_SMALL=0 turns it off at
+ * start). Same formulas and summation orders as the per-operation kernels. No reference
+ * counterpart. */
+mk_status mk_box_mg_small_enable(int32_t on);
 
 /* ---- Krylov exponential (finite state projection) -------------------------------------
  * num::expv(t, Q.matrix(), p, krylov_dim, krylov_tol), called at include/markovkit/fsp.hpp:152

@@ -115,6 +115,7 @@ SIGNATURES = {
     "mk_gmres_box": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_gmres_box_device": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_box_mg_graphs_enable": ([i32], i32),
+    "mk_box_mg_small_enable": ([i32], i32),
     "mk_device_alloc": ([i64, C.POINTER(P)], i32),
     "mk_device_free": ([P], i32),
     "mk_copy_to_device": ([P, P, i64], i32),

@@ -185,6 +185,12 @@ class BoxMultigrid:
         return SolverResult(res.iterations, res.residual, bool(res.converged))
 
 
+def small_levels_enable(on: bool) -> None:
+    """run small levels' smoothing and transfers in two single-workgroup kernels per visit
+    (default on)"""
+    check(lib().mk_box_mg_small_enable(1 if on else 0))
+
+
 def graphs_enable(on: bool) -> None:
     """replay the standalone iteration's cycle from a captured HIP graph (default on)"""
     check(lib().mk_box_mg_graphs_enable(1 if on else 0))

@@ -83,6 +83,113 @@ __global__ void __launch_bounds__(kBoxThreads)
     if (lane == 0) coarse[c] = s;
 }
 
+// ---- small levels: one workgroup runs a whole visit's smoothing and transfer, with a barrier
+// between the phases (about 20 launches per visit become 2)
+constexpr int kSmallThreads = 1024;
+
+struct small_level {
+    csr_view<double> A; // the level operator, rows in the level's order
+    i64 n = 0;
+    u32 cb[5] = {};     // colour c holds rows cb[c] .. cb[c + 1]
+    double shift = 0.0;
+    int sweeps = 0, jacobi = 0;
+    const double *b = nullptr;
+    double *x = nullptr, *r = nullptr, *tmp = nullptr;
+    i64 nc = 0;         // the next level: rows, children lists, parents and weights
+    int wide = 0;       // restriction by wavefronts (as k_box_restrict_wave)
+    const u32 *cptr = nullptr, *cidx = nullptr, *parent = nullptr;
+    const double *weight = nullptr;
+    double *coarse_b = nullptr, *coarse_x = nullptr;
+};
+
+// sum over row i's entries of rate * x[column] in entry order, and row i's exit rate
+__device__ __forceinline__ double row_sum(const csr_view<double> &A, u32 i, const double *x, double &ex) {
+    double sum = 0.0;
+    if (A.coded) {
+        const u32 len = A.rlen[i];
+        const u32 base = A.soff[i >> 6] + (i & 63u);
+        ex = A.exit ? A.exit[i] : A.edict[A.rcode[i]];
+        for (u32 k = 0; k < len; ++k) {
+            const u32 e = A.ent[base + k * kSlice];
+            sum += A.vdict[e >> kColBits] * x[e & kColMask];
+        }
+    } else {
+        ex = A.exit[i];
+        for (u32 q = A.ptr[i]; q < A.ptr[i + 1]; ++q) sum += A.val[q] * x[A.col[q]];
+    }
+    return sum;
+}
+
+// the level's smoother (all sweeps), with the formulas of the per-level kernels
+__device__ void small_smooth(const small_level &L, bool reverse) {
+    const u32 t = threadIdx.x, step = blockDim.x;
+    for (int sweep = 0; sweep < L.sweeps; ++sweep) {
+        if (L.jacobi) {
+            const double omega = 2.0 / 3.0;
+            for (u32 i = t; i < L.n; i += step) {
+                double ex = 0.0;
+                const double sum = row_sum(L.A, i, L.x, ex);
+                const double xr = L.x[i], d = L.shift + ex;
+                const double res = L.b[i] - (d * xr - sum);
+                L.tmp[i] = xr + omega * res / d;
+            }
+            __syncthreads();
+            for (u32 i = t; i < L.n; i += step) L.x[i] = L.tmp[i];
+            __syncthreads();
+        } else {
+            for (int k = 0; k < 4; ++k) {
+                const int c = reverse ? 3 - k : k;
+                for (u32 i = L.cb[c] + t; i < L.cb[c + 1]; i += step) {
+                    double ex = 0.0;
+                    const double sum = row_sum(L.A, i, L.x, ex);
+                    L.x[i] = (L.b[i] + sum) / (L.shift + ex);
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// pre-smoothing, residual, restriction to the next level and its zero iterate
+__global__ void __launch_bounds__(kSmallThreads) k_box_small_down(small_level L) {
+    small_smooth(L, false);
+    const u32 t = threadIdx.x, step = blockDim.x;
+    for (u32 i = t; i < L.n; i += step) {
+        double ex = 0.0;
+        const double sum = row_sum(L.A, i, L.x, ex);
+        L.r[i] = L.b[i] - ((L.shift + ex) * L.x[i] - sum);
+    }
+    __syncthreads();
+    if (L.wide) {
+        // one wavefront per coarse state, the summation order of k_box_restrict_wave
+        const u32 lane = t & 63u, wave = t >> 6, waves = step >> 6;
+        for (u32 c = wave; c < L.nc; c += waves) {
+            double sum = 0.0;
+            for (u32 q = L.cptr[c] + lane; q < L.cptr[c + 1]; q += 64) sum += L.r[L.cidx[q]];
+            for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o, 64);
+            if (lane == 0) {
+                L.coarse_b[c] = sum;
+                L.coarse_x[c] = 0.0;
+            }
+        }
+        return;
+    }
+    for (u32 c = t; c < L.nc; c += step) {
+        double sum = 0.0;
+        for (u32 q = L.cptr[c]; q < L.cptr[c + 1]; ++q) sum += L.r[L.cidx[q]];
+        L.coarse_b[c] = sum;
+        L.coarse_x[c] = 0.0;
+    }
+}
+
+// prolongation of the next level's correction and post-smoothing
+__global__ void __launch_bounds__(kSmallThreads) k_box_small_up(small_level L) {
+    const u32 t = threadIdx.x, step = blockDim.x;
+    for (u32 i = t; i < L.n; i += step) L.x[i] += L.weight[i] * L.coarse_x[L.parent[i]];
+    __syncthreads();
+    small_smooth(L, true);
+}
+
 // partial[c][i] = sum over the columns j of chunk c of inv[j][i] b[j] (inverse column-major:
 // the lanes of a wavefront read consecutive rows of one column); one wavefront per 64 rows and
 // column chunk, so a 561-state level already spreads over 162 workgroups
@@ -187,6 +294,14 @@ void box_rhs(i64 n, double shift, const double *p, const double *ap, double *b, 
     hipLaunchKernelGGL(k_box_rhs, dim3(grid_of(n)), dim3(kBoxThreads), 0, s, n, shift, p, ap, b);
 }
 
+void box_small_down(const small_level &L, hipStream_t s) {
+    hipLaunchKernelGGL(k_box_small_down, dim3(1), dim3(kSmallThreads), 0, s, L);
+}
+
+void box_small_up(const small_level &L, hipStream_t s) {
+    hipLaunchKernelGGL(k_box_small_up, dim3(1), dim3(kSmallThreads), 0, s, L);
+}
+
 void box_scale(i64 n, const double *r, const double *pinv, double *u, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_box_scale, dim3(grid_of(n)), dim3(kBoxThreads), 0, s, n, r, pinv, u);
@@ -229,6 +344,7 @@ struct box_level {
     std::vector<dev_csr<double>> Ablock;
     std::vector<std::vector<tri_launch>> fw, bw; // per block (one entry for Gauss-Seidel)
     dbuf pinv, u, fw_rows, fw_ptr, bw_rows, bw_ptr;
+    bool small = false; // smoothing and transfers of a visit run in one workgroup each way
 };
 
 } // namespace mk::rt
@@ -253,6 +369,26 @@ struct mk_box_mg_s {
 
 namespace mk::rt {
 namespace {
+
+// levels of at most this many states run a visit's smoothing and transfers in two
+// single-workgroup kernels (four-colour Gauss-Seidel and Jacobi; MK_BOX_SMALL=0 turns it off at
+// start, mk_box_mg_small_enable at run time)
+constexpr i64 kSmallRows = 2048;
+i64 small_rows() {
+    static const i64 rows = [] {
+        const char *env = std::getenv("MK_BOX_SMALL_ROWS"); // tuning override
+        const long v = env ? std::strtol(env, nullptr, 10) : 0;
+        return v > 0 ? static_cast<i64>(v) : kSmallRows;
+    }();
+    return rows;
+}
+std::atomic<int> &box_small_switch() {
+    static std::atomic<int> on{[] {
+        const char *env = std::getenv("MK_BOX_SMALL");
+        return (env && env[0] == '0') ? 0 : 1;
+    }()};
+    return on;
+}
 
 // MK_BOX_GRAPHS=0 launches the standalone iteration's cycles eagerly
 std::atomic<int> &box_graph_switch() {
@@ -697,6 +833,7 @@ void build(mk_box_mg_s &M) {
             d.Ac[c].view.rowmap = dp<u32>(d.rowmap) + d.cb[c];
         }
         d.A = upload_csr<double>(gp, true);
+        d.small = n <= small_rows() && (p.smoother == MK_BOX_GS || p.smoother == MK_BOX_JACOBI);
         d.r = dbuf(n * sizeof(double));
         if (p.smoother == MK_BOX_JACOBI) d.tmp = dbuf(n * sizeof(double));
         // transfer in row orders: parent'[pos_f[i]] = pos_c[parent[i]]
@@ -851,11 +988,48 @@ void krylov_coarse(mk_box_mg_s &M, std::size_t l) {
     P.run(K_COARSE, n * 24.0, [&] { gpu::axpy<double>(n, a2, v2, xc, s); });
 }
 
+// the descriptor of a small level's fused kernels
+gpu::small_level small_desc(mk_box_mg_s &M, std::size_t l, const double *b, double *x) {
+    auto &lv = M.L[l];
+    auto &c = M.L[l + 1];
+    gpu::small_level S;
+    S.A = lv.A.view;
+    S.n = lv.n;
+    for (int k = 0; k < 5; ++k) S.cb[k] = lv.cb[k];
+    S.shift = M.p.shift;
+    S.sweeps = M.p.sweeps;
+    S.jacobi = M.p.smoother == MK_BOX_JACOBI ? 1 : 0;
+    S.b = b;
+    S.x = x;
+    S.r = dp<double>(lv.r);
+    S.tmp = dp<double>(lv.tmp);
+    S.nc = c.n;
+    S.wide = lv.wide_restrict ? 1 : 0;
+    S.cptr = dp<u32>(lv.cptr);
+    S.cidx = dp<u32>(lv.cidx);
+    S.parent = dp<u32>(lv.parent);
+    S.weight = dp<double>(lv.weight);
+    S.coarse_b = dp<double>(c.b);
+    S.coarse_x = dp<double>(c.x);
+    return S;
+}
+
+void coarse_correction(mk_box_mg_s &M, std::size_t l);
+
 // one cycle of A_l x = b from the guess x (level l's row order)
 void cycle(mk_box_mg_s &M, std::size_t l, const double *b, double *x) {
     auto &lv = M.L[l];
     auto &P = prof();
     const hipStream_t s = stream();
+    if (lv.small && !lv.terminal && box_small_switch().load() != 0) {
+        const auto S = small_desc(M, l, b, x);
+        const double bytes = 2.0 * M.p.sweeps * op_bytes(lv.A, 8, 8, 1);
+        P.run(level_class(l, K_SWEEP), bytes + op_bytes(lv.A, 8, 8, 2),
+              [&] { gpu::box_small_down(S, s); });
+        coarse_correction(M, l);
+        P.run(level_class(l, K_SWEEP), bytes + lv.n * 28.0, [&] { gpu::box_small_up(S, s); });
+        return;
+    }
     if (lv.terminal == 1) {
         P.run(K_TERMINAL, lv.n * 40.0, [&] {
             gpu::box_thomas(lv.n, dp<double>(lv.lower), dp<double>(lv.upper), dp<double>(lv.pivot), b, x, s);
@@ -883,16 +1057,23 @@ void cycle(mk_box_mg_s &M, std::size_t l, const double *b, double *x) {
         }
         gpu::fill<double>(c.n, 0.0, dp<double>(c.x), s);
     });
+    coarse_correction(M, l);
+    P.run(level_class(l, K_PROLONG), lv.n * 28.0, [&] {
+        gpu::prolong_add<double>(lv.n, dp<u32>(lv.parent), dp<double>(lv.weight), dp<double>(c.x), x, s);
+    });
+    smooth(M, l, b, x, true);
+}
+
+// the correction of level l from level l + 1 (whose b is set and x zero): the exact bottom
+// solve once, the K-cycle's GCR steps, or `cycles` coarse cycles
+void coarse_correction(mk_box_mg_s &M, std::size_t l) {
+    auto &c = M.L[l + 1];
     if (M.p.cycle_type == MK_CYCLE_K && !c.terminal) {
         krylov_coarse(M, l);
     } else {
         const int visits = c.terminal ? 1 : M.p.cycles;
         for (int v = 0; v < visits; ++v) cycle(M, l + 1, dp<double>(c.b), dp<double>(c.x));
     }
-    P.run(level_class(l, K_PROLONG), lv.n * 28.0, [&] {
-        gpu::prolong_add<double>(lv.n, dp<u32>(lv.parent), dp<double>(lv.weight), dp<double>(c.x), x, s);
-    });
-    smooth(M, l, b, x, true);
 }
 
 // |b - A x|_1 / shift on the finest level (row order; the finest level is not terminal)
@@ -1161,6 +1342,11 @@ mk_status mk_gmres_box(mk_generator gen, double shift, mk_box_mg mg, const doubl
         if (st == MK_OK) HIPCHK(hipMemcpy(x, dx.p, bytes, hipMemcpyDeviceToHost));
     });
     return outer != MK_OK ? outer : st;
+}
+
+mk_status mk_box_mg_small_enable(int32_t on) {
+    box_small_switch().store(on ? 1 : 0);
+    return MK_OK;
 }
 
 mk_status mk_box_mg_graphs_enable(int32_t on) {

@@ -151,6 +151,28 @@ def test_driver_smoothers_reproduce_recorded_cycles(name):
     print(f"[{name}] {G.rows()} states, {p['smoother']}: {it} cycles (driver {rec['iterations']})")
 
 
+@pytest.mark.parametrize("cycle", ["W", "K"])
+@pytest.mark.parametrize("smoother", ["colour-gs", "jacobi"])
+@pytest.mark.parametrize("case", ["1d-64-cap2", "2d-4-cap4-schlogl", "1d-4-cap40-schlogl", "3d-4-cap2"])
+def test_small_level_kernels_match_per_operation_kernels(case, smoother, cycle):
+    """levels of at most 2,048 states run each visit as two single-workgroup kernels; the
+    cycle equals the per-operation kernels' (same formulas and summation orders)"""
+    G, _ = pair(case, smoother, cycles=2, kcycle=cycle == "K")
+    rng = np.random.default_rng(17)
+    b = rng.random(G.rows())
+    x0 = rng.random(G.rows()) * 0.1
+    try:
+        boxmg.small_levels_enable(False)
+        z0 = G.cycle(b, x0)
+        boxmg.small_levels_enable(True)
+        z1 = G.cycle(b, x0)
+    finally:
+        boxmg.small_levels_enable(True)
+    diff = np.max(np.abs(z1 - z0))
+    print(f"[{case} {smoother} {cycle}] fused vs per-operation max diff {diff:.3e}")
+    assert np.array_equal(z1, z0)
+
+
 @pytest.mark.parametrize("smoother", ["colour-gs", "gs", "block-ilu"])
 def test_graph_replay_is_bitwise_eager(smoother):
     G = mk.BoxMultigrid(1, 64, 3, 1.0, 0.5, LIN, smoother, 2, 2)

@@ -65,6 +65,9 @@ typedef struct mk_multigrid_options {
     int32_t sweep_growth;   /* 0 */
     int32_t smoother;       /* MK_JACOBI */
     int32_t precision;      /* MK_FP64 or MK_FP32 (device-only extension) */
+    int32_t cycle_levels;   /* 0: every level takes coarse_cycles visits of its coarse level (the
+                               reference); k > 0: only the top k levels do, the ones below one
+                               (a W-cycle at the top over a V-cycle tail; extension) */
 } mk_multigrid_options;
 
 /* Arnoldi orthogonalisation of the device solver (extension; the reference's field set ends at

@@ -446,6 +446,7 @@ struct multigrid_options {
     int sweep_growth = 0;
     relaxation smoother = relaxation::jacobi;
     precision working_precision = precision::fp64;
+    int cycle_levels = 0; // extension: coarse_cycles on the top cycle_levels levels only
 };
 
 // sector_multigrid (multigrid.hpp:371-408)
@@ -471,6 +472,7 @@ class sector_multigrid {
                      : options.smoother == relaxation::red_black       ? MK_RED_BLACK
                                                                        : MK_ILU0;
         o.precision = options.working_precision == precision::fp32 ? MK_FP32 : MK_FP64;
+        o.cycle_levels = options.cycle_levels;
         auto dom = domain.native();
         mk_sector_mg m = nullptr;
         detail::check(mk_sector_mg_create(dom->h, box.native(), shift, &o, &m));

@@ -25,7 +25,8 @@ pf64 = C.POINTER(C.c_double)
 class MultigridOptions(C.Structure):
     _fields_ = [("pre_sweeps", i32), ("post_sweeps", i32), ("jacobi_weight", f64),
                 ("coarsest_size", i64), ("max_levels", i32), ("coarse_cycles", i32),
-                ("sweep_growth", i32), ("smoother", i32), ("precision", i32)]
+                ("sweep_growth", i32), ("smoother", i32), ("precision", i32),
+                ("cycle_levels", i32)]
 
 
 class GmresOptions(C.Structure):

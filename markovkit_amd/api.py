@@ -292,11 +292,13 @@ class MultigridOptions:
     sweep_growth: int = 0
     smoother: int = JACOBI
     precision: int = FP64
+    cycle_levels: int = 0  # 0: coarse_cycles on every level; k: on the top k levels only
 
     def to_c(self) -> _MultigridOptions:
         return _MultigridOptions(self.pre_sweeps, self.post_sweeps, self.jacobi_weight,
                                  self.coarsest_size, self.max_levels, self.coarse_cycles,
-                                 self.sweep_growth, self.smoother, self.precision)
+                                 self.sweep_growth, self.smoother, self.precision,
+                                 self.cycle_levels)
 
 
 class SectorMultigrid:

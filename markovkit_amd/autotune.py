@@ -23,6 +23,8 @@ import numpy as np
 from ._lib import check, lib
 
 SWEEPS: Tuple[Tuple[int, int], ...] = ((2, 2), (1, 2), (2, 1), (1, 1), (3, 3))
+# (coarse_cycles, cycle_levels): the V-cycle, and W-cycles on the top levels over a V-cycle tail
+CYCLES: Tuple[Tuple[int, int], ...] = ((1, 0), (2, 2), (2, 3))
 XCD_CHUNKS: Tuple[int, ...] = (16, 0, 8, 32)
 
 
@@ -43,6 +45,8 @@ class Trial:
     xcd_chunk: int
     seconds: float
     iterations: int
+    coarse_cycles: int = 1
+    cycle_levels: int = 0
 
 
 @dataclass
@@ -52,19 +56,24 @@ class TuneResult:
     xcd_chunk: int
     seconds: float
     trials: List[Trial] = field(default_factory=list)
+    coarse_cycles: int = 1
+    cycle_levels: int = 0
 
     def options(self, **kw):
-        """MultigridOptions with the tuned sweep counts (other fields from kw)"""
+        """MultigridOptions with the tuned sweep counts and cycle shape (other fields from kw)"""
         from .api import MultigridOptions
-        return MultigridOptions(pre_sweeps=self.pre, post_sweeps=self.post, **kw)
+        return MultigridOptions(pre_sweeps=self.pre, post_sweeps=self.post,
+                                coarse_cycles=self.coarse_cycles, cycle_levels=self.cycle_levels, **kw)
 
 
 def tune_solve(Q, domain, box, shift: float, b, *, smoother=None, precision=None,
                sweeps: Sequence[Tuple[int, int]] = SWEEPS, chunks: Sequence[int] = XCD_CHUNKS,
+               cycles: Sequence[Tuple[int, int]] = ((1, 0),),
                rel_tolerance: float = 1e-10, restart: int = 50, repeats: int = 2,
                apply: bool = True) -> TuneResult:
-    """Time FGMRES(restart) + sector multigrid on (shift I - Q) x = b for every shape and return
-    the fastest. With apply=True the XCD chunk of the result is left set (the sweep counts go
+    """Time FGMRES(restart) + sector multigrid on (shift I - Q) x = b for every shape (sweep
+    counts x cycle shapes (coarse_cycles, cycle_levels) x XCD chunks) and return the fastest.
+    With apply=True the XCD chunk of the result is left set (the sweep counts and cycle shape go
     into the hierarchy options the caller builds)."""
     from . import api as mk
     smoother = mk.RED_BLACK if smoother is None else smoother
@@ -78,10 +87,11 @@ def tune_solve(Q, domain, box, shift: float, b, *, smoother=None, precision=None
     start_chunk = xcd_chunk()
     trials: List[Trial] = []
     try:
-        for pre, post in sweeps:
+        for (pre, post), (cyc, lev) in [(sw, cy) for sw in sweeps for cy in cycles]:
             M = mk.SectorMultigrid(domain, box, shift,
                                    mk.MultigridOptions(precision=precision, smoother=smoother,
-                                                       pre_sweeps=pre, post_sweeps=post))
+                                                       pre_sweeps=pre, post_sweeps=post,
+                                                       coarse_cycles=cyc, cycle_levels=lev))
             for chunk in chunks:
                 set_xcd_chunk(chunk)
                 x_dev.zero()
@@ -99,14 +109,15 @@ def tune_solve(Q, domain, box, shift: float, b, *, smoother=None, precision=None
                         best = float("inf")
                         break
                     its = r.iterations
-                trials.append(Trial(pre, post, chunk, best, its))
+                trials.append(Trial(pre, post, chunk, best, its, cyc, lev))
             del M
     finally:
         set_xcd_chunk(start_chunk)
     win = min(trials, key=lambda t: t.seconds)  # first minimum: earlier candidates win ties
     if apply:
         set_xcd_chunk(win.xcd_chunk)
-    return TuneResult(win.pre, win.post, win.xcd_chunk, win.seconds, trials)
+    return TuneResult(win.pre, win.post, win.xcd_chunk, win.seconds, trials, win.coarse_cycles,
+                      win.cycle_levels)
 
 
 def tune_batch(Q, domain, box, shifts, B, *, precision=None,

@@ -9,6 +9,12 @@ using namespace mk::rt;
 
 namespace {
 
+// the levels whose coarse correction takes coarse_cycles visits: the top cycle_levels of them
+// (all when cycle_levels is 0); the ones below visit their coarse level once
+i64 cycle_levels(const mk::multigrid_options &o) {
+    return o.cycle_levels > 0 ? static_cast<i64>(o.cycle_levels) : i64{1} << 30;
+}
+
 // red-black smoother: one row-mapped operator per colour (that colour's rows with all their
 // entries, columns unchanged) and the colour of every row
 template <typename T>
@@ -433,10 +439,12 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
         d.perm = dp<u32>(lv.perm);
     }
     mg.tail_start = -1;
-    if (mg.opt.coarse_cycles == 1 && (mg.opt.smoother == MK_JACOBI || mg.opt.smoother == MK_RED_BLACK ||
-                                      mg.opt.smoother == MK_AGGREGATE_BLOCK)) {
+    if (mg.opt.smoother == MK_JACOBI || mg.opt.smoother == MK_RED_BLACK ||
+        mg.opt.smoother == MK_AGGREGATE_BLOCK) {
+        // the tail's levels must each visit their coarse level once
+        const i64 first_v = mg.opt.coarse_cycles == 1 ? 0 : cycle_levels(mg.opt);
         i64 start = depth;
-        while (start > 0 && L[start - 1].n <= tail_rows() && L[start - 1].n_agg == 0) --start;
+        while (start > first_v && L[start - 1].n <= tail_rows() && L[start - 1].n_agg == 0) --start;
         if (start < depth) mg.tail_start = start;
     }
     mg.tail_desc = upload(desc);
@@ -653,7 +661,7 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
         const i64 np = red_only ? lv.Ac[0].n : n;
         const u32 *prows = red_only ? dp<u32>(lv.rowmap[0]) : nullptr;
         const double pbytes = double(np) * (4 + 3 * ts + (red_only ? 4 : 0)) + double(c.n) * ts;
-        if (mg.opt.coarse_cycles <= 1) {
+        if (mg.opt.coarse_cycles <= 1 || l >= cycle_levels(mg.opt)) {
             P.run(cls(K_RESTRICT), rbytes, [&] {
                 gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.rhs), s);
             });
@@ -1128,6 +1136,7 @@ void mk_default_multigrid_options(mk_multigrid_options *o) {
     o->sweep_growth = 0;
     o->smoother = MK_JACOBI;
     o->precision = MK_FP64;
+    o->cycle_levels = 0;
 }
 
 mk_status mk_box_create(int64_t n_voxels, int32_t n_species, const int32_t *max_totals,
@@ -1316,7 +1325,7 @@ mk_status mk_sector_mg_create(mk_domain domain, mk_box box, double shift,
                     o.smoother == MK_ILU0,
                 "sector_multigrid: unknown smoother");
         require(o.precision == MK_FP64 || o.precision == MK_FP32, "sector_multigrid: unknown precision");
-        require(o.pre_sweeps >= 0 && o.post_sweeps >= 0 && o.coarse_cycles >= 1,
+        require(o.pre_sweeps >= 0 && o.post_sweeps >= 0 && o.coarse_cycles >= 1 && o.cycle_levels >= 0,
                 "sector_multigrid: bad sweep or cycle counts");
         if (!(shift > 0.0)) throw std::invalid_argument("population_multigrid: shift must be positive");
         const auto &space = box->space;
@@ -1329,6 +1338,7 @@ mk_status mk_sector_mg_create(mk_domain domain, mk_box box, double shift,
         mg->opt.coarsest_size = o.coarsest_size;
         mg->opt.max_levels = o.max_levels;
         mg->opt.coarse_cycles = o.coarse_cycles;
+        mg->opt.cycle_levels = o.cycle_levels;
         mg->opt.sweep_growth = o.sweep_growth;
         mg->opt.smoother = o.smoother;
         mg->shift = shift;

@@ -153,6 +153,7 @@ struct multigrid_options {
     int sweep_growth = 0;
     int smoother = 0; // 0 Jacobi, 1 aggregate blocks, 2 red-black Gauss-Seidel, 3 ILU(0)
     int precision = 0;
+    int cycle_levels = 0; // 0: every level takes coarse_cycles visits; k: the top k levels only
 };
 
 // Aggregate-block smoother of one sector level (multigrid.hpp:268-342): for every coarse voxel

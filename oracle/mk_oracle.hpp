@@ -672,6 +672,9 @@ struct multigrid_options {
     int coarse_cycles = 1;
     int sweep_growth = 0;
     relaxation smoother = relaxation::jacobi;
+    // the library's extension: only the top cycle_levels levels take coarse_cycles visits of
+    // their coarse level, the ones below take one (0: every level, the reference's semantics)
+    int cycle_levels = 0;
 };
 
 // multigrid.hpp:139-361 (weighted Jacobi and aggregate-block smoothers)
@@ -700,6 +703,8 @@ class population_multigrid {
         coarse_options.max_levels -= 1;
         coarse_options.pre_sweeps += options.sweep_growth;
         coarse_options.post_sweeps += options.sweep_growth;
+        if (options.cycle_levels == 1) coarse_options.coarse_cycles = 1;
+        if (options.cycle_levels > 1) coarse_options.cycle_levels = options.cycle_levels - 1;
         coarse_ = std::make_unique<population_multigrid>(voxel_domain(*coarse_grid, doubled),
                                                          totals, shift, coarse_options);
         transfer_ = std::make_unique<population_transfer>(*domain.structured, *coarse_grid,

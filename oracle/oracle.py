@@ -49,7 +49,8 @@ def lib():
         L.orc_apply.argtypes = [P, f64p, f64p]
         L.orc_apply_shifted.argtypes = [P, C.c_double, f64p, f64p]
         L.orc_mg_create.argtypes = [P, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int64,
-                                    C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+                                    C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(C.c_void_p)]
         L.orc_mg_relax.argtypes = [P, C.c_int64, f64p, f64p, C.c_int, C.c_int]
         L.orc_mg_colours.argtypes = [P, C.c_int64, np.ctypeslib.ndpointer(np.uint8, flags="C")]
         L.orc_mg_ilu_pivots.argtypes = [P, C.c_int64, f64p]
@@ -214,13 +215,13 @@ class SectorMultigrid:
 
     def __init__(self, problem, shift, pre_sweeps=2, post_sweeps=2, jacobi_weight=2.0 / 3.0,
                  coarsest_size=4, max_levels=32, coarse_cycles=1, sweep_growth=0,
-                 smoother="jacobi"):
+                 smoother="jacobi", cycle_levels=0):
         if smoother not in self.SMOOTHERS:
             raise ValueError(f"smoother must be one of {self.SMOOTHERS}")
         h = C.c_void_p()
         _check(lib().orc_mg_create(problem.h, float(shift), pre_sweeps, post_sweeps,
                                    jacobi_weight, coarsest_size, max_levels, coarse_cycles,
-                                   sweep_growth, self.SMOOTHERS.index(smoother),
+                                   sweep_growth, self.SMOOTHERS.index(smoother), cycle_levels,
                                    C.byref(h)))
         self.h = h
         self.problem = problem

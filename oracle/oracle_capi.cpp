@@ -182,7 +182,8 @@ void orc_apply_shifted(void *h, double shift, const double *x, double *y) {
 }
 
 int orc_mg_create(void *h, double shift, int pre, int post, double omega, int64_t coarsest,
-                  int max_levels, int coarse_cycles, int sweep_growth, int smoother, void **out) {
+                  int max_levels, int coarse_cycles, int sweep_growth, int smoother, int cycle_levels,
+                  void **out) {
     return guarded([&] {
         auto *p = static_cast<problem *>(h);
         multigrid_options o;
@@ -195,6 +196,8 @@ int orc_mg_create(void *h, double shift, int pre, int post, double omega, int64_
         o.max_levels = max_levels;
         o.coarse_cycles = coarse_cycles;
         o.sweep_growth = sweep_growth;
+        if (cycle_levels < 0) throw std::invalid_argument("sector_multigrid: negative cycle_levels");
+        o.cycle_levels = cycle_levels;
         *out = new sector_multigrid(*p->domain, *p->box, shift, o);
     });
 }

@@ -109,6 +109,53 @@ def test_multigrid_cycle(case, precision, smoother):
 
 
 @pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("smoother", [mk.JACOBI, mk.RED_BLACK], ids=["jacobi", "red_black"])
+@pytest.mark.parametrize("cycles,levels", [(2, 0), (2, 1), (2, 2), (3, 1)],
+                         ids=["W", "W-top1", "W-top2", "3-top1"])
+def test_multigrid_cycle_coarse_visits(case, precision, smoother, cycles, levels):
+    """W-cycles (coarse_cycles visits of every coarse level, or of the top cycle_levels levels
+    over a V-cycle below, where the fused tail may take over) against the oracle's cycle"""
+    w, P, box = case["w"], case["P"], case["box"]
+    shift = w.shifts[0]
+    M = mk.SectorMultigrid(w.model.domain, box, shift,
+                           mk.MultigridOptions(precision=precision, smoother=smoother,
+                                               coarse_cycles=cycles, cycle_levels=levels))
+    Mo = OracleMG(P, shift, smoother=SMOOTHERS[smoother], coarse_cycles=cycles,
+                  cycle_levels=levels)
+    rng = np.random.default_rng(13)
+    b = rng.standard_normal(P.n)
+    z = M.apply(b)
+    z_ref = Mo.apply(b)
+    err = np.abs(z - z_ref).max() / np.abs(z_ref).max()
+    assert err <= (1e-12 if precision == mk.FP64 else 2e-5), (case["name"], err)
+
+
+@pytest.mark.parametrize("levels", [1, 2, 3])
+def test_hybrid_cycle_solve(case, levels):
+    """FGMRES with the FP32 red-black preconditioner W-cycled on the top levels reaches the
+    1e-10 residual and the oracle's answer"""
+    w, P, box, Q = case["w"], case["P"], case["box"], case["Q"]
+    shift = w.shifts[0]
+    b = w.rhs(0, P.n)
+    tol = 1e-10 * np.linalg.norm(b)
+    M = mk.SectorMultigrid(w.model.domain, box, shift,
+                           mk.MultigridOptions(precision=mk.FP32, smoother=mk.RED_BLACK,
+                                               coarse_cycles=2, cycle_levels=levels))
+    x = np.zeros(P.n)
+    res = mk.gmres(mk.ShiftedGenerator(Q, shift), M, b, x, tolerance=tol, max_iterations=500,
+                   restart=50)
+    assert res.converged, res
+    r = b - P.apply_shifted(shift, x)
+    assert np.linalg.norm(r) <= 1.0001 * tol
+    x_ref, info = P.gmres(shift, b, mg=OracleMG(P, shift, smoother="red_black", coarse_cycles=2,
+                                                cycle_levels=levels), tol=tol, restart=50)
+    assert info["converged"]
+    bound = (np.abs(r).sum() + np.abs(b - P.apply_shifted(shift, x_ref)).sum()) / shift
+    assert np.abs(x - x_ref).sum() <= bound
+    assert abs(res.iterations - info["iterations"]) <= 2, (res.iterations, info["iterations"])
+
+
+@pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("ortho,restart", [("dcgs2", 50), ("cgs2", 50), ("dcgs2", 6)],
                          ids=["dcgs2", "cgs2", "dcgs2_restart6"])
 def test_resolvent_solve(case, precision, ortho, restart):

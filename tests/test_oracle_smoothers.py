@@ -158,3 +158,26 @@ def test_every_smoother_gives_the_direct_answer(wl):
         its[sm] = info["iterations"]
     # the stronger smoothers need no more outer iterations than weighted Jacobi here
     assert its["red_black"] <= its["jacobi"] and its["ilu0"] <= its["jacobi"], its
+
+
+def test_cycle_levels_semantics():
+    """cycle_levels: 0 and any count at or beyond the hierarchy's depth give the full W-cycle,
+    one coarse visit gives the V-cycle whatever cycle_levels, and on a two-level hierarchy
+    cycle_levels = 1 is the full W-cycle (the oracle cycle's results, bitwise)"""
+    from oracle.oracle import Problem, SectorMultigrid as OracleMG
+    from markovkit_amd.workloads import birth_death_1d
+    w = birth_death_1d(16, 3, 0.5, "random")
+    P = Problem(w.model, w.max_totals)
+    s = w.shifts[0]
+    b = np.random.default_rng(2).standard_normal(P.n)
+    for sm in ("jacobi", "red_black"):
+        full = OracleMG(P, s, smoother=sm, coarse_cycles=2).apply(b)
+        assert np.array_equal(OracleMG(P, s, smoother=sm, coarse_cycles=2, cycle_levels=32).apply(b), full)
+        v = OracleMG(P, s, smoother=sm).apply(b)
+        for k in (1, 2, 5):
+            assert np.array_equal(OracleMG(P, s, smoother=sm, cycle_levels=k).apply(b), v)
+        two = OracleMG(P, s, smoother=sm, coarse_cycles=2, max_levels=2)
+        assert np.array_equal(OracleMG(P, s, smoother=sm, coarse_cycles=2, max_levels=2,
+                                       cycle_levels=1).apply(b), two.apply(b))
+        top1 = OracleMG(P, s, smoother=sm, coarse_cycles=2, cycle_levels=1).apply(b)
+        assert not np.array_equal(top1, full) and not np.array_equal(top1, v)

@@ -409,8 +409,18 @@ inline std::atomic<int> &tail_switch() {
     return on;
 }
 
+// the row limit of graph replay (MK_GRAPH_ROWS overrides, for tuning)
+inline i64 graph_rows() {
+    static const i64 rows = [] {
+        const char *env = std::getenv("MK_GRAPH_ROWS");
+        const long long v = env ? std::strtoll(env, nullptr, 10) : 0;
+        return v > 0 ? static_cast<i64>(v) : kGraphRows;
+    }();
+    return rows;
+}
+
 inline bool graphs_enabled(i64 rows = 0) {
-    return graphs_switch().load() != 0 && prof().mask == 0 && rows <= kGraphRows;
+    return graphs_switch().load() != 0 && prof().mask == 0 && rows <= graph_rows();
 }
 
 struct graph_cache {

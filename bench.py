@@ -382,8 +382,11 @@ def main():
     ap.add_argument("--post", type=int, default=2, help="post-smoothing sweeps (multigrid_options)")
     ap.add_argument("--cycles", type=int, default=2,
                     help="coarse_cycles of the timed hierarchy (1 V-cycle, 2 W-cycle)")
-    ap.add_argument("--cycle-levels", type=int, default=3,
+    ap.add_argument("--cycle-levels", type=int, default=2,
                     help="levels that take --cycles coarse visits (0: all; the ones below take one)")
+    ap.add_argument("--coarsest", type=int, default=100,
+                    help="coarsest_size of the timed hierarchy (sectors of at most this many "
+                         "states are solved densely)")
     ap.add_argument("--smoother", default="red_black", choices=list(SMOOTHERS),
                     help="sector multigrid smoother of the timed and served solves (red_black: "
                          "multicolour Gauss-Seidel; jacobi: the reference's default)")
@@ -451,7 +454,7 @@ def main():
     t = time.perf_counter()
     smoother = getattr(mk, SMOOTHERS[args.smoother])
     opts = dict(pre_sweeps=args.pre, post_sweeps=args.post, coarse_cycles=args.cycles,
-                cycle_levels=args.cycle_levels)
+                cycle_levels=args.cycle_levels, coarsest_size=args.coarsest)
     M32 = mk.SectorMultigrid(w.model.domain, box, shift,
                              mk.MultigridOptions(precision=mk.FP32, smoother=smoother, **opts))
     t_mg = time.perf_counter() - t
@@ -554,6 +557,7 @@ def main():
             "sweeps": [args.pre, args.post],
             "coarse_cycles": args.cycles,
             "cycle_levels": args.cycle_levels,
+            "coarsest_size": args.coarsest,
         },
         "p50_ms": 1e3 * percentile(lat, 50),
         "p99_ms": 1e3 * percentile(lat, 99),

@@ -110,18 +110,21 @@ def test_multigrid_cycle(case, precision, smoother):
 
 @pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("smoother", [mk.JACOBI, mk.RED_BLACK], ids=["jacobi", "red_black"])
-@pytest.mark.parametrize("cycles,levels", [(2, 0), (2, 1), (2, 2), (3, 1)],
-                         ids=["W", "W-top1", "W-top2", "3-top1"])
-def test_multigrid_cycle_coarse_visits(case, precision, smoother, cycles, levels):
+@pytest.mark.parametrize("cycles,levels,coarsest", [(2, 0, 4), (2, 1, 4), (2, 2, 4), (3, 1, 4),
+                                                    (2, 2, 100), (2, 0, 36)],
+                         ids=["W", "W-top1", "W-top2", "3-top1", "W-top2-c100", "W-c36"])
+def test_multigrid_cycle_coarse_visits(case, precision, smoother, cycles, levels, coarsest):
     """W-cycles (coarse_cycles visits of every coarse level, or of the top cycle_levels levels
-    over a V-cycle below, where the fused tail may take over) against the oracle's cycle"""
+    over a V-cycle below, where the fused tail may take over), with dense bottom blocks of up to
+    coarsest states, against the oracle's cycle"""
     w, P, box = case["w"], case["P"], case["box"]
     shift = w.shifts[0]
     M = mk.SectorMultigrid(w.model.domain, box, shift,
                            mk.MultigridOptions(precision=precision, smoother=smoother,
-                                               coarse_cycles=cycles, cycle_levels=levels))
+                                               coarse_cycles=cycles, cycle_levels=levels,
+                                               coarsest_size=coarsest))
     Mo = OracleMG(P, shift, smoother=SMOOTHERS[smoother], coarse_cycles=cycles,
-                  cycle_levels=levels)
+                  cycle_levels=levels, coarsest_size=coarsest)
     rng = np.random.default_rng(13)
     b = rng.standard_normal(P.n)
     z = M.apply(b)
@@ -132,15 +135,17 @@ def test_multigrid_cycle_coarse_visits(case, precision, smoother, cycles, levels
 
 @pytest.mark.parametrize("levels", [1, 2, 3])
 def test_hybrid_cycle_solve(case, levels):
-    """FGMRES with the FP32 red-black preconditioner W-cycled on the top levels reaches the
-    1e-10 residual and the oracle's answer"""
+    """FGMRES with the FP32 red-black preconditioner W-cycled on the top levels (dense bottom
+    blocks of up to 100 states, the bench's hierarchy) reaches the 1e-10 residual and the
+    oracle's answer"""
     w, P, box, Q = case["w"], case["P"], case["box"], case["Q"]
     shift = w.shifts[0]
     b = w.rhs(0, P.n)
     tol = 1e-10 * np.linalg.norm(b)
     M = mk.SectorMultigrid(w.model.domain, box, shift,
                            mk.MultigridOptions(precision=mk.FP32, smoother=mk.RED_BLACK,
-                                               coarse_cycles=2, cycle_levels=levels))
+                                               coarse_cycles=2, cycle_levels=levels,
+                                               coarsest_size=100))
     x = np.zeros(P.n)
     res = mk.gmres(mk.ShiftedGenerator(Q, shift), M, b, x, tolerance=tol, max_iterations=500,
                    restart=50)
@@ -148,7 +153,8 @@ def test_hybrid_cycle_solve(case, levels):
     r = b - P.apply_shifted(shift, x)
     assert np.linalg.norm(r) <= 1.0001 * tol
     x_ref, info = P.gmres(shift, b, mg=OracleMG(P, shift, smoother="red_black", coarse_cycles=2,
-                                                cycle_levels=levels), tol=tol, restart=50)
+                                                cycle_levels=levels, coarsest_size=100),
+                          tol=tol, restart=50)
     assert info["converged"]
     bound = (np.abs(r).sum() + np.abs(b - P.apply_shifted(shift, x_ref)).sum()) / shift
     assert np.abs(x - x_ref).sum() <= bound

@@ -380,13 +380,16 @@ def main():
     ap.add_argument("--restart", type=int, default=50)
     ap.add_argument("--pre", type=int, default=2, help="pre-smoothing sweeps (multigrid_options)")
     ap.add_argument("--post", type=int, default=2, help="post-smoothing sweeps (multigrid_options)")
-    ap.add_argument("--cycles", type=int, default=2,
-                    help="coarse_cycles of the timed hierarchy (1 V-cycle, 2 W-cycle)")
-    ap.add_argument("--cycle-levels", type=int, default=2,
-                    help="levels that take --cycles coarse visits (0: all; the ones below take one)")
-    ap.add_argument("--coarsest", type=int, default=100,
+    ap.add_argument("--cycles", type=int, default=None,
+                    help="coarse_cycles of the timed hierarchy (1 V-cycle, 2 W-cycle; default: 2 "
+                         "on the large workload, the reference's 1 on the others)")
+    ap.add_argument("--cycle-levels", type=int, default=None,
+                    help="levels that take --cycles coarse visits (0: all; the ones below take "
+                         "one; default: 3 on the large workload, else 0)")
+    ap.add_argument("--coarsest", type=int, default=None,
                     help="coarsest_size of the timed hierarchy (sectors of at most this many "
-                         "states are solved densely)")
+                         "states are solved densely; default: 100 on the large workload, else "
+                         "the reference's default)")
     ap.add_argument("--smoother", default="red_black", choices=list(SMOOTHERS),
                     help="sector multigrid smoother of the timed and served solves (red_black: "
                          "multicolour Gauss-Seidel; jacobi: the reference's default)")
@@ -453,6 +456,15 @@ def main():
     log(f"[setup] generator {t_gen:.1f}s, {Q.n_jumps()} jumps")
     t = time.perf_counter()
     smoother = getattr(mk, SMOOTHERS[args.smoother])
+    # the large workload's hierarchy: W-cycles on the top 3 levels, dense sectors of <= 100 states
+    ref = mk.MultigridOptions()
+    tuned_large = args.workload == "large"
+    if args.cycles is None:
+        args.cycles = 2 if tuned_large else ref.coarse_cycles
+    if args.cycle_levels is None:
+        args.cycle_levels = 3 if tuned_large else ref.cycle_levels
+    if args.coarsest is None:
+        args.coarsest = 100 if tuned_large else ref.coarsest_size
     opts = dict(pre_sweeps=args.pre, post_sweeps=args.post, coarse_cycles=args.cycles,
                 cycle_levels=args.cycle_levels, coarsest_size=args.coarsest)
     M32 = mk.SectorMultigrid(w.model.domain, box, shift,

@@ -365,7 +365,7 @@ mk_status mk_synchronize(void);
  * rows (default off; MK_GRAPHS=1 in the environment turns it on at start). Replays run the same
  * kernels with the same arguments as eager launches, so results are identical either way. */
 mk_status mk_graphs_enable(int32_t on);
-/* Run the multigrid levels of at most 1024 states (and all coarser ones) as one workgroup-wide
+/* Run the multigrid levels of at most 2048 states (and all coarser ones) as one workgroup-wide
  * kernel instead of one launch per operation (default on; MK_MG_TAIL=0 turns it off at start).
  * Same formulas and summation orders as the per-level kernels. */
 mk_status mk_fused_tail_enable(int32_t on);

@@ -9,12 +9,6 @@ using namespace mk::rt;
 
 namespace {
 
-// the levels whose coarse correction takes coarse_cycles visits: the top cycle_levels of them
-// (all when cycle_levels is 0); the ones below visit their coarse level once
-i64 cycle_levels(const mk::multigrid_options &o) {
-    return o.cycle_levels > 0 ? static_cast<i64>(o.cycle_levels) : i64{1} << 30;
-}
-
 // red-black smoother: one row-mapped operator per colour (that colour's rows with all their
 // entries, columns unchanged) and the colour of every row
 template <typename T>
@@ -119,14 +113,25 @@ void build_ilu(const mk::generator_csr &cat, double shift, level_dev<T> &lv) {
     lv.ilu = true;
 }
 
+// the tail's LDS image in HBM: every constant array copied to its arena offset (again after the
+// terminal blocks change with the shift)
+void refresh_tail_image(mk_sector_mg_s &mg) {
+    if (mg.tail_lds_bytes == 0) return;
+    for (const auto &cp : mg.tail_copies)
+        HIPCHK(hipMemcpyAsync(static_cast<char *>(mg.tail_image.p) + cp.offset, cp.src, cp.bytes,
+                              hipMemcpyDeviceToDevice, stream()));
+    drain();
+}
+
 // LDS layout of the fused tail: every array of levels tail_start .. end at a 16-byte aligned
-// offset of one arena, the constant ones with a copy from HBM; disabled when an operator is not
-// coded or the arena exceeds the LDS limit
+// offset of one arena, the constant ones first (their image is assembled in HBM) and the work
+// vectors after them; disabled when an operator is not coded or the arena exceeds the LDS limit
 template <typename T>
 void build_tail_lds(mk_sector_mg_s &mg, const std::vector<level_dev<T>> &L,
                     const std::vector<gpu::level_desc<T>> &desc) {
     mg.tail_lds_bytes = 0;
-    mg.tail_plan_n = 0;
+    mg.tail_image_bytes = 0;
+    mg.tail_copies.clear();
     if (mg.tail_start < 0) return;
     const i64 depth = static_cast<i64>(L.size());
     const int levels = static_cast<int>(depth - mg.tail_start);
@@ -168,20 +173,29 @@ void build_tail_lds(mk_sector_mg_s &mg, const std::vector<level_dev<T>> &L,
         d.toff = reinterpret_cast<const u64 *>(place(desc[l].toff, lv.n_term * 8, term));
         d.tinv = reinterpret_cast<const T *>(place(desc[l].tinv, tinv * ts, term));
         d.colour = reinterpret_cast<const std::uint8_t *>(place(desc[l].colour, n, desc[l].colour != nullptr));
+        for (int c = 0; c < 2; ++c)
+            d.crows[c] = reinterpret_cast<const u32 *>(place(desc[l].crows[c], static_cast<u64>(desc[l].ncrows[c]) * 4,
+                                                             desc[l].crows[c] != nullptr));
         d.perm = reinterpret_cast<const u32 *>(place(desc[l].perm, n * 4, desc[l].perm != nullptr));
-        d.rhs = reinterpret_cast<T *>(place(nullptr, n * ts, true));
-        d.x = reinterpret_cast<T *>(place(nullptr, n * ts, true));
-        d.xt = reinterpret_cast<T *>(place(nullptr, n * ts, true));
-        d.res = reinterpret_cast<T *>(place(nullptr, n * ts, true));
         ld.push_back(d);
+    }
+    const u64 image_bytes = off;
+    for (std::size_t q = 0; q < ld.size() && ok; ++q) {
+        const u64 n = static_cast<u64>(ld[q].n), ts = sizeof(T);
+        ld[q].rhs = reinterpret_cast<T *>(place(nullptr, n * ts, true));
+        ld[q].x = reinterpret_cast<T *>(place(nullptr, n * ts, true));
+        ld[q].xt = reinterpret_cast<T *>(place(nullptr, n * ts, true));
+        ld[q].res = reinterpret_cast<T *>(place(nullptr, n * ts, true));
     }
     if (!ok || !gpu::vcycle_tail_lds_fits<T>(static_cast<u32>(std::min<u64>(off, ~0u)), levels) ||
         off > gpu::kTailLdsMax)
         return;
     mg.tail_lds_desc = upload(ld);
-    mg.tail_plan = upload(plan);
-    mg.tail_plan_n = static_cast<int>(plan.size());
+    mg.tail_copies = std::move(plan);
+    mg.tail_image = dbuf(std::max<u64>(image_bytes, 16));
+    mg.tail_image_bytes = static_cast<u32>(image_bytes);
     mg.tail_lds_bytes = static_cast<u32>(off);
+    refresh_tail_image(mg);
 }
 
 template <typename T>
@@ -370,6 +384,9 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
             if (lv.n_jrows > 0) lv.jrows = upload(jrows);
         }
         lv.n_term = static_cast<i64>(tstart.size());
+        u64 term_rows = 0;
+        for (u32 m : tsize) term_rows += m;
+        lv.direct = l > 0 && !has_next && static_cast<i64>(term_rows) == lv.n;
         lv.h_tstart = tstart;
         lv.h_tsize = tsize;
         lv.h_toff = toff;
@@ -437,6 +454,11 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
         d.res = dp<T>(lv.res);
         d.colour = lv.rb ? dp<std::uint8_t>(lv.colour) : nullptr;
         d.perm = dp<u32>(lv.perm);
+        d.direct = lv.direct ? 1 : 0;
+        for (int c = 0; c < 2 && lv.rb; ++c) {
+            d.crows[c] = dp<u32>(lv.rowmap[c]);
+            d.ncrows[c] = lv.Ac[c].n;
+        }
     }
     mg.tail_start = -1;
     if (mg.opt.smoother == MK_JACOBI || mg.opt.smoother == MK_RED_BLACK ||
@@ -445,7 +467,10 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
         const i64 first_v = mg.opt.coarse_cycles == 1 ? 0 : cycle_levels(mg.opt);
         i64 start = depth;
         while (start > first_v && L[start - 1].n <= tail_rows() && L[start - 1].n_agg == 0) --start;
-        if (start < depth) mg.tail_start = start;
+        // a lone direct level is one terminal-solve launch without the tail
+        // (the tail tracks at most 32 levels)
+        if (start < depth && depth - start <= 32 && !(start == depth - 1 && L[start].direct))
+            mg.tail_start = start;
     }
     mg.tail_desc = upload(desc);
     build_tail_lds<T>(mg, L, desc);
@@ -517,8 +542,8 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
                          tail_lds_switch().load() != 0;
         P.run(cls(K_SWEEP), 0.0, [&] {
             if (lds) {
-                gpu::vcycle_tail_lds<T>(dp<gpu::level_desc<T>>(mg.tail_lds_desc), dp<gpu::lds_copy>(mg.tail_plan),
-                                        mg.tail_plan_n, mg.tail_lds_bytes, static_cast<int>(L.size() - l),
+                gpu::vcycle_tail_lds<T>(dp<gpu::level_desc<T>>(mg.tail_lds_desc), mg.tail_image.p,
+                                        mg.tail_image_bytes, mg.tail_lds_bytes, static_cast<int>(L.size() - l),
                                         shift, omega, rhs, rhs64, scale, out, dp<T>(lv.x), s);
             } else {
                 gpu::vcycle_tail<T>(desc, static_cast<int>(L.size() - l), shift, omega, rhs, rhs64,
@@ -526,6 +551,14 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             }
         });
         return out ? out : dp<T>(lv.x);
+    }
+    if (lv.direct) {
+        T *dst = out ? out : dp<T>(lv.x);
+        P.run(cls(K_TERMINAL), 0.0, [&] {
+            gpu::terminal_solve<T>(lv.n_term, dp<u32>(lv.tstart), dp<u32>(lv.tsize), dp<u64>(lv.toff),
+                                   dp<T>(lv.tinv), lv.term_max, rhs, dst, s, true);
+        });
+        return dst;
     }
     const i64 n = lv.n;
     const std::size_t ts = sizeof(T);
@@ -673,9 +706,11 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             P.run(cls(K_RESTRICT), rbytes, [&] {
                 gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.crhs), s);
             });
-            P.run(cls(K_OTHER), double(c.n) * ts, [&] { gpu::fill<T>(c.n, T(0), dp<T>(c.cx), s); });
+            // from cx = 0 the first coarse residual is crhs itself and the first correction
+            // is cx: that cycle writes cx directly
+            run_level<T>(mg, L, l + 1, dp<T>(c.crhs), nullptr, dp<T>(c.cx));
             const double cbytes = op_bytes(c.A, ts, ts, 2);
-            for (int cyc = 0; cyc < mg.opt.coarse_cycles; ++cyc) {
+            for (int cyc = 1; cyc < mg.opt.coarse_cycles; ++cyc) {
                 P.run(cls(K_RESIDUAL), cbytes, [&] {
                     gpu::level_residual<T>(c.A.view, shift, dp<T>(c.crhs), dp<T>(c.cx),
                                            dp<T>(c.rhs), s);
@@ -1382,6 +1417,7 @@ mk_status mk_sector_mg_set_shift(mk_sector_mg mg, double shift) {
         } else {
             reshift_levels<double>(mg->L64, shift);
         }
+        refresh_tail_image(*mg);
         mg->shift = shift;
     });
 }

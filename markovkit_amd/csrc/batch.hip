@@ -153,7 +153,7 @@ T *brun_level(mk_batch_s &B, std::vector<level_dev<T>> &L, std::vector<blevel<T>
         const double cn = double(c.n) * double(Bp);
         const double rbytes = op_bytes(lv.R, 0, 0, 0) + double(lv.n_fine_linked) * Bp * ts + cn * ts;
         const double pbytes = double(lv.n) * 4 + nb * 2 * ts + cn * ts;
-        if (mg.opt.coarse_cycles <= 1) {
+        if (mg.opt.coarse_cycles <= 1 || static_cast<i64>(l) >= cycle_levels(mg.opt)) {
             P.run(K_RESTRICT, rbytes, [&] { bk::gather_sum<T>(lv.R.view, res, dp<T>(cb.rhs), Bp, act, s); });
             T *cx = brun_level<T>(B, L, BL, l + 1, dp<T>(cb.rhs), nullptr, nullptr, nullptr, act);
             P.run(K_PROLONG, pbytes, [&] {

@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(kThreads)
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_terminal(i64 nb, const u32 *start, const u32 *size, const u64 *offset, const T *inv,
-               i64 max_size, const T *r, T *x) {
+               i64 max_size, const T *r, T *x, int assign) {
     const i64 g = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x;
     if (g >= nb * max_size) return;
     const i64 b = g / max_size, i = g % max_size;
@@ -313,8 +313,13 @@ __global__ void __launch_bounds__(kThreads)
     const T *row = inv + offset[b] + static_cast<u64>(i) * m;
     const T *rr = r + start[b];
     T s = 0;
+#pragma unroll 8
     for (u32 j = 0; j < m; ++j) s += row[j] * rr[j];
-    x[start[b] + i] += s;
+    if (assign) {
+        x[start[b] + i] = s;
+    } else {
+        x[start[b] + i] += s;
+    }
 }
 
 // ---- aggregate-block smoother (thread per listed row)
@@ -326,6 +331,8 @@ __device__ __forceinline__ T gather_row(const csr_view<T> &A, u32 i, const T *x,
         const u32 len = A.rlen[i];
         const u32 base = A.soff[i >> 6] + (i & 63u);
         exit_i = A.exit ? A.exit[i] : A.edict[A.rcode[i]];
+        // unrolled so that several entries' loads are in flight; the sum keeps entry order
+#pragma unroll 4
         for (u32 k = 0; k < len; ++k) {
             const u32 e = A.ent[base + k * kSlice];
             sum += A.vdict[e >> kColBits] * x[e & kColMask];
@@ -467,71 +474,151 @@ __global__ void __launch_bounds__(ONE_WG ? kTailThreads : kThreads)
     }
 }
 
-// one Gauss-Seidel half-sweep of the rows of colour c, in place, by one workgroup
-template <typename T>
-__device__ void tail_gs_half(const csr_view<T> &A, const u8 *colour, u8 c, T shift, const T *rhs,
-                             T *x, i64 n) {
-    for (i64 i = threadIdx.x; i < n; i += blockDim.x) {
-        if (colour[i] != c) continue;
-        T ex = 0;
-        const T sum = gather_row(A, static_cast<u32>(i), x, ex);
-        x[i] = (rhs[i] + sum) / (shift + ex);
+// The fused tail runs from HBM (LDS = false) or with every level array staged in LDS
+// (LDS = true). Pointers reach the tail through descriptors, so the compiler cannot see their
+// address space; lds() states it where an array is known to live in LDS, which turns flat
+// accesses into LDS instructions.
+template <bool LDS, typename P>
+__device__ __forceinline__ P *lds(P *p) {
+#if __HIP_DEVICE_COMPILE__
+    if constexpr (LDS) __builtin_assume(__builtin_amdgcn_is_shared(reinterpret_cast<const void *>(p)));
+#endif
+    return p;
+}
+
+// gather_row and row_exit for the tail: the same sums in the same order (an LDS tail holds
+// coded operators only)
+template <bool LDS, typename T>
+__device__ __forceinline__ T tail_exit(const csr_view<T> &A, i64 r) {
+    if constexpr (!LDS) {
+        return row_exit(A, r);
+    } else {
+        return A.exit ? lds<LDS>(A.exit)[r] : lds<LDS>(A.edict)[lds<LDS>(A.rcode)[r]];
     }
 }
 
-// fused V-cycle tail: one workgroup runs every level from L[0] down and back up
+template <bool LDS, typename T>
+__device__ __forceinline__ T tail_gather(const csr_view<T> &A, u32 i, const T *x, T &exit_i) {
+    if constexpr (!LDS) {
+        return gather_row(A, i, x, exit_i);
+    } else {
+        const u32 len = lds<LDS>(A.rlen)[i];
+        const u32 base = lds<LDS>(A.soff)[i >> 6] + (i & 63u);
+        exit_i = tail_exit<LDS>(A, i);
+        const u32 *ent = lds<LDS>(A.ent);
+        const T *vdict = lds<LDS>(A.vdict);
+        const T *xs = lds<LDS>(x);
+        T sum = 0;
+#pragma unroll 4
+        for (u32 k = 0; k < len; ++k) {
+            const u32 e = ent[base + k * kSlice];
+            sum += vdict[e >> kColBits] * xs[e & kColMask];
+        }
+        return sum;
+    }
+}
+
+// one Gauss-Seidel half-sweep of the rows of one colour (listed in rows), in place, by one
+// workgroup
+template <bool LDS, typename T>
+__device__ __forceinline__ void tail_gs_half(const csr_view<T> &A, const u32 *rows, i64 nrows, T shift,
+                                             const T *rhs, T *x) {
+    for (i64 q = threadIdx.x; q < nrows; q += blockDim.x) {
+        const u32 i = lds<LDS>(rows)[q];
+        T ex = 0;
+        const T sum = tail_gather<LDS>(A, i, x, ex);
+        lds<LDS>(x)[i] = (lds<LDS>(rhs)[i] + sum) / (shift + ex);
+    }
+}
+
+// dense terminal blocks: x[block rows] (+)= inv_b r[block rows]
+template <bool LDS, bool ASSIGN, typename T>
+__device__ __forceinline__ void tail_terminal(const level_desc<T> &lv, const T *r, T *x) {
+    for (i64 g = threadIdx.x; g < lv.n_term * lv.term_max; g += blockDim.x) {
+        const i64 b = g / lv.term_max, i = g % lv.term_max;
+        const u32 m = lds<LDS>(lv.tsize)[b];
+        if (i >= m) continue;
+        const u32 st = lds<LDS>(lv.tstart)[b];
+        const T *row = lds<LDS>(lv.tinv) + lds<LDS>(lv.toff)[b] + static_cast<u64>(i) * m;
+        const T *rr = lds<LDS>(r) + st;
+        T acc = 0;
+#pragma unroll 8
+        for (u32 j = 0; j < m; ++j) acc += row[j] * rr[j];
+        if constexpr (ASSIGN) {
+            lds<LDS>(x)[st + i] = acc;
+        } else {
+            lds<LDS>(x)[st + i] += acc;
+        }
+    }
+}
+
 // The V-cycle of levels L[0 .. levels) by one workgroup; the top level's result goes to out (in
-// the caller's row order) or, without out, to top_x.
-template <typename T>
-__device__ void tail_body(const level_desc<T> *L, int levels, T shift, T omega, const T *rhs_in,
-                          const double *rhs64, const double *scale_ptr, T *out, T *top_x) {
-    constexpr int kMaxLevels = 32;
-    T *cur[kMaxLevels];
-    T *oth[kMaxLevels];
+// the caller's row order) or, without out, to top_x. rhs_in, rhs64, out and top_x are in HBM;
+// the top level's right-hand side is first copied to L[0].rhs.
+template <bool LDS, typename T>
+__device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T shift, T omega,
+                                          const T *rhs_in, const double *rhs64, const double *scale_ptr,
+                                          T *out, T *top_x) {
+    // bit k: level k's current iterate is in xt (the Jacobi sweeps alternate x and xt); levels
+    // are at most 32
+    u32 flip = 0;
+    T *top = nullptr; // the top level's result
     const int tid = threadIdx.x, nt = blockDim.x;
     const double scale = scale_ptr ? *scale_ptr : 1.0;
     const u32 *perm0 = L[0].perm; // the top level's row order differs from the caller's
     for (int k = 0; k < levels; ++k) {
         const level_desc<T> &lv = L[k];
-        const T *rhs = k == 0 ? (rhs64 || perm0 ? lv.rhs : rhs_in) : lv.rhs;
+        const T *rhs = lv.rhs;
         T *x = lv.x, *xt = lv.xt;
         const u8 *colour = lv.colour;
+        if (lv.direct && k > 0) {
+            // the cycle from zero on a level of terminal blocks only is their dense solve
+            tail_terminal<LDS, true>(lv, rhs, x);
+            __syncthreads();
+            continue;
+        }
         // first sweep from x = 0 (the FP64 input converted on the top level); red-black: colour 0
         // only, colour 1 follows
         for (i64 i = tid; i < lv.n; i += nt) {
             T b;
-            const i64 src = k == 0 && perm0 ? perm0[i] : i;
-            if (k == 0 && rhs64) {
-                b = static_cast<T>(rhs64[src] * scale);
-                lv.rhs[i] = b;
-            } else if (k == 0 && perm0) {
-                b = rhs_in[src];
-                lv.rhs[i] = b;
+            if (k == 0) {
+                const i64 src = perm0 ? lds<LDS>(perm0)[i] : i;
+                b = rhs64 ? static_cast<T>(rhs64[src] * scale) : rhs_in[src];
+                lds<LDS>(lv.rhs)[i] = b;
             } else {
-                b = rhs[i];
+                b = lds<LDS>(rhs)[i];
             }
             if (!colour) {
-                x[i] = lv.pre > 0 ? omega * b / (shift + row_exit(lv.A, i)) : T(0);
+                lds<LDS>(x)[i] = lv.pre > 0 ? omega * b / (shift + tail_exit<LDS>(lv.A, i)) : T(0);
             } else if (lv.pre == 0) {
-                x[i] = T(0);
-            } else if (colour[i] == 0) {
-                x[i] = b / (shift + row_exit(lv.A, i));
+                lds<LDS>(x)[i] = T(0);
+            } else if (lds<LDS>(colour)[i] == 0) {
+                lds<LDS>(x)[i] = b / (shift + tail_exit<LDS>(lv.A, i));
             }
         }
         __syncthreads();
         if (colour) {
             for (int half = 1; half < 2 * lv.pre; ++half) {
-                tail_gs_half(lv.A, colour, static_cast<u8>(half & 1), shift, rhs, x, lv.n);
+                const int c = half & 1;
+                tail_gs_half<LDS>(lv.A, lv.crows[c], lv.ncrows[c], shift, rhs, x);
                 __syncthreads();
             }
-            for (i64 i = tid; i < lv.n; i += nt) {
-                if (lv.pre > 0 && colour[i] != 0) {
-                    lv.res[i] = T(0);
-                } else {
+            if (lv.pre > 0) {
+                // after the last colour-1 half-sweep only colour-0 rows have a residual
+                for (i64 q = tid; q < lv.ncrows[0]; q += nt) {
+                    const u32 i = lds<LDS>(lv.crows[0])[q];
                     T ex = 0;
-                    const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
+                    const T sum = tail_gather<LDS>(lv.A, i, x, ex);
                     const T d = shift + ex;
-                    lv.res[i] = rhs[i] - (d * x[i] - sum);
+                    lds<LDS>(lv.res)[i] = lds<LDS>(rhs)[i] - (d * lds<LDS>(x)[i] - sum);
+                }
+                for (i64 q = tid; q < lv.ncrows[1]; q += nt) lds<LDS>(lv.res)[lds<LDS>(lv.crows[1])[q]] = T(0);
+            } else {
+                for (i64 i = tid; i < lv.n; i += nt) {
+                    T ex = 0;
+                    const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
+                    const T d = shift + ex;
+                    lds<LDS>(lv.res)[i] = lds<LDS>(rhs)[i] - (d * lds<LDS>(x)[i] - sum);
                 }
             }
             __syncthreads();
@@ -539,12 +626,12 @@ __device__ void tail_body(const level_desc<T> *L, int levels, T shift, T omega, 
         for (int p = 1; !colour && p < lv.pre; ++p) {
             for (i64 i = tid; i < lv.n; i += nt) {
                 T ex = 0;
-                const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
-                const T xr = x[i];
+                const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
+                const T xr = lds<LDS>(x)[i];
                 const T d = shift + ex;
                 const T ax = d * xr - sum;
-                const T res = rhs[i] - ax;
-                xt[i] = xr + omega * res / d;
+                const T res = lds<LDS>(rhs)[i] - ax;
+                lds<LDS>(xt)[i] = xr + omega * res / d;
             }
             __syncthreads();
             T *t = x;
@@ -553,63 +640,59 @@ __device__ void tail_body(const level_desc<T> *L, int levels, T shift, T omega, 
         }
         for (i64 i = tid; !colour && i < lv.n; i += nt) {
             T ex = 0;
-            const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
+            const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
             const T d = shift + ex;
-            lv.res[i] = rhs[i] - (d * x[i] - sum);
+            lds<LDS>(lv.res)[i] = lds<LDS>(rhs)[i] - (d * lds<LDS>(x)[i] - sum);
         }
         __syncthreads();
         if (lv.n_term > 0) {
-            for (i64 g = tid; g < lv.n_term * lv.term_max; g += nt) {
-                const i64 b = g / lv.term_max, i = g % lv.term_max;
-                const u32 m = lv.tsize[b];
-                if (i >= m) continue;
-                const T *row = lv.tinv + lv.toff[b] + static_cast<u64>(i) * m;
-                const T *rr = lv.res + lv.tstart[b];
-                T acc = 0;
-                for (u32 j = 0; j < m; ++j) acc += row[j] * rr[j];
-                x[lv.tstart[b] + i] += acc;
-            }
+            tail_terminal<LDS, false>(lv, lv.res, x);
             __syncthreads();
         }
         if (lv.n_next > 0) {
             T *crhs = L[k + 1].rhs;
             for (i64 c = tid; c < lv.n_next; c += nt) {
                 T ex = 0;
-                crhs[c] = gather_row(lv.R, static_cast<u32>(c), lv.res, ex);
+                lds<LDS>(crhs)[c] = tail_gather<LDS>(lv.R, static_cast<u32>(c), lv.res, ex);
             }
             __syncthreads();
         }
-        cur[k] = x;
-        oth[k] = xt;
+        if (x == lv.xt) flip |= 1u << k;
     }
     for (int k = levels - 1; k >= 0; --k) {
         const level_desc<T> &lv = L[k];
-        const T *rhs = k == 0 ? (rhs64 || perm0 ? lv.rhs : rhs_in) : lv.rhs;
-        T *x = cur[k], *xt = oth[k];
+        const T *rhs = lv.rhs;
+        const bool f = (flip >> k) & 1u;
+        T *x = f ? lv.xt : lv.x, *xt = f ? lv.x : lv.xt;
+        if (lv.direct && k > 0) continue;
         if (lv.n_next > 0) {
-            const T *cx = cur[k + 1];
-            for (i64 i = tid; i < lv.n; i += nt) {
-                // red-black with post-smoothing: colour-1 rows are rewritten without being read
-                if (lv.colour && lv.post > 0 && lv.colour[i] != 0) continue;
-                const u32 c = lv.coarse_of[i];
-                if (c != 0xFFFFFFFFu) x[i] += lv.weight[i] * cx[c];
+            const T *cx = ((flip >> (k + 1)) & 1u) ? L[k + 1].xt : L[k + 1].x;
+            // red-black with post-smoothing: colour-1 rows are rewritten without being read
+            const bool red_only = lv.colour && lv.post > 0;
+            const i64 np = red_only ? lv.ncrows[0] : lv.n;
+            for (i64 q = tid; q < np; q += nt) {
+                const i64 i = red_only ? lds<LDS>(lv.crows[0])[q] : q;
+                const u32 c = lds<LDS>(lv.coarse_of)[i];
+                if (c != 0xFFFFFFFFu) lds<LDS>(x)[i] += lds<LDS>(lv.weight)[i] * lds<LDS>(cx)[c];
             }
             __syncthreads();
         }
         for (int half = 0; lv.colour && half < 2 * lv.post; ++half) {
             // reverse colour order: 1, 0, 1, 0, ...
-            tail_gs_half(lv.A, lv.colour, static_cast<u8>(1 - (half & 1)), shift, rhs, x, lv.n);
+            const int c = 1 - (half & 1);
+            tail_gs_half<LDS>(lv.A, lv.crows[c], lv.ncrows[c], shift, rhs, x);
             __syncthreads();
         }
         for (int p = 0; !lv.colour && p < lv.post; ++p) {
+            // the top level's last sweep may write out (HBM): dst is accessed as generic memory
             T *dst = (k == 0 && p == lv.post - 1 && out) ? out : xt;
             for (i64 i = tid; i < lv.n; i += nt) {
                 T ex = 0;
-                const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
-                const T xr = x[i];
+                const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
+                const T xr = lds<LDS>(x)[i];
                 const T d = shift + ex;
                 const T ax = d * xr - sum;
-                const T res = rhs[i] - ax;
+                const T res = lds<LDS>(rhs)[i] - ax;
                 dst[i] = xr + omega * res / d;
             }
             __syncthreads();
@@ -620,15 +703,18 @@ __device__ void tail_body(const level_desc<T> *L, int levels, T shift, T omega, 
                 x = dst;
             }
         }
-        cur[k] = x;
-        oth[k] = xt;
+        if (k > 0) {
+            flip = (flip & ~(1u << k)) | (x == lv.xt ? 1u << k : 0u);
+        } else {
+            top = x;
+        }
     }
     // the top level's result in out (in the caller's row order), else in top_x
     T *dst = out ? out : top_x;
     if (out && perm0) {
-        for (i64 i = tid; i < L[0].n; i += nt) dst[perm0[i]] = cur[0][i];
-    } else if (cur[0] != dst) {
-        for (i64 i = tid; i < L[0].n; i += nt) dst[i] = cur[0][i];
+        for (i64 i = tid; i < L[0].n; i += nt) dst[perm0[i]] = top[i];
+    } else if (top != dst) {
+        for (i64 i = tid; i < L[0].n; i += nt) dst[i] = top[i];
     }
 }
 
@@ -636,7 +722,7 @@ template <typename T>
 __global__ void __launch_bounds__(kTailThreads)
     k_vcycle_tail(const level_desc<T> *L, int levels, T shift, T omega, const T *rhs_in,
                   const double *rhs64, const double *scale_ptr, T *out) {
-    tail_body(L, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, L[0].x);
+    tail_body<false>(L, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, L[0].x);
 }
 
 // pointer fields of an LDS level descriptor hold (offset + 1) into the LDS arena (0: null)
@@ -658,26 +744,37 @@ __device__ void rebase_view(csr_view<T> &v, char *base) {
 
 constexpr int kTailLdsLevels = 12;
 
-// The same V-cycle with every array of the levels staged in LDS first (plan: global source,
-// LDS offset and byte count of each constant array); work vectors live in LDS too.
+// The same V-cycle with every array of the levels staged in LDS first (one contiguous read of
+// the image of the constant arrays); work vectors live in LDS too.
 template <typename T>
 __global__ void __launch_bounds__(kTailThreads)
-    k_vcycle_tail_lds(const level_desc<T> *L_lds, const lds_copy *plan, int n_plan, int levels,
+    k_vcycle_tail_lds(const level_desc<T> *L_lds, const uint4 *image, u32 image_bytes, int levels,
                       T shift, T omega, const T *rhs_in, const double *rhs64, const double *scale_ptr,
                       T *out, T *top_x) {
     extern __shared__ uint4 arena[];
     __shared__ uint4 dstore[(kTailLdsLevels * sizeof(level_desc<T>) + 15) / 16];
     level_desc<T> *d = reinterpret_cast<level_desc<T> *>(dstore);
     char *base = reinterpret_cast<char *>(arena);
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int c = 0; c < n_plan; ++c) {
-        const lds_copy cp = plan[c];
-        const u32 words = cp.bytes / 4;
-        const u32 *src = static_cast<const u32 *>(cp.src);
-        u32 *dst = reinterpret_cast<u32 *>(base + cp.offset);
-        for (u32 w = tid; w < words; w += nt) dst[w] = src[w];
-        for (u32 b = words * 4 + tid; b < cp.bytes; b += nt)
-            base[cp.offset + b] = static_cast<const char *>(cp.src)[b];
+    const int tid = threadIdx.x;
+    const u32 nt = blockDim.x;
+    // stage the image: eight 16-byte loads per thread in flight before the LDS stores
+    const u32 n16 = image_bytes / 16;
+    const uint4 zero{0u, 0u, 0u, 0u};
+    auto ld = [&](u32 k) { return k < n16 ? image[k] : zero; };
+    auto st = [&](u32 k, const uint4 &v) {
+        if (k < n16) arena[k] = v;
+    };
+    for (u32 b0 = tid; b0 < n16; b0 += 8 * nt) {
+        const uint4 v0 = ld(b0), v1 = ld(b0 + nt), v2 = ld(b0 + 2 * nt), v3 = ld(b0 + 3 * nt);
+        const uint4 v4 = ld(b0 + 4 * nt), v5 = ld(b0 + 5 * nt), v6 = ld(b0 + 6 * nt), v7 = ld(b0 + 7 * nt);
+        st(b0, v0);
+        st(b0 + nt, v1);
+        st(b0 + 2 * nt, v2);
+        st(b0 + 3 * nt, v3);
+        st(b0 + 4 * nt, v4);
+        st(b0 + 5 * nt, v5);
+        st(b0 + 6 * nt, v6);
+        st(b0 + 7 * nt, v7);
     }
     if (tid < levels) {
         level_desc<T> lv = L_lds[tid];
@@ -694,11 +791,13 @@ __global__ void __launch_bounds__(kTailThreads)
         lv.xt = rebase(lv.xt, base);
         lv.res = rebase(lv.res, base);
         lv.colour = rebase(lv.colour, base);
+        lv.crows[0] = rebase(lv.crows[0], base);
+        lv.crows[1] = rebase(lv.crows[1], base);
         lv.perm = rebase(lv.perm, base);
         __builtin_memcpy(d + tid, &lv, sizeof(lv));
     }
     __syncthreads();
-    tail_body(d, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, top_x);
+    tail_body<true>(d, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, top_x);
 }
 
 template <typename T>
@@ -1115,12 +1214,12 @@ void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, 
 
 template <typename T>
 void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
-                    const T *inverse, i64 max_size, const T *r, T *x, hipStream_t s) {
+                    const T *inverse, i64 max_size, const T *r, T *x, hipStream_t s, bool assign) {
     if (n_blocks == 0) return;
     const i64 threads = n_blocks * max_size;
     const int grid = static_cast<int>((threads + kThreads - 1) / kThreads);
     hipLaunchKernelGGL((k_terminal<T>), dim3(grid), dim3(kThreads), 0, s, n_blocks, start, size,
-                       offset, inverse, max_size, r, x);
+                       offset, inverse, max_size, r, x, static_cast<int>(assign));
 }
 
 namespace {
@@ -1180,7 +1279,7 @@ bool vcycle_tail_lds_fits(u32 lds_bytes, int levels) {
 }
 
 template <typename T>
-void vcycle_tail_lds(const level_desc<T> *L_lds, const lds_copy *plan, int n_plan, u32 lds_bytes,
+void vcycle_tail_lds(const level_desc<T> *L_lds, const void *image, u32 image_bytes, u32 lds_bytes,
                      int levels, double shift, double omega, const T *rhs, const double *rhs64,
                      const double *scale, T *out, T *top_x, hipStream_t s) {
     if (levels <= 0) return;
@@ -1191,7 +1290,7 @@ void vcycle_tail_lds(const level_desc<T> *L_lds, const lds_copy *plan, int n_pla
     }();
     (void)attr;
     hipLaunchKernelGGL((k_vcycle_tail_lds<T>), dim3(1), dim3(kTailThreads), lds_bytes, s, L_lds,
-                       plan, n_plan, levels, static_cast<T>(shift), static_cast<T>(omega), rhs, rhs64,
+                       static_cast<const uint4 *>(image), image_bytes, levels, static_cast<T>(shift), static_cast<T>(omega), rhs, rhs64,
                        scale, out, top_x);
 }
 
@@ -1296,7 +1395,7 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
     template void prolong_add<T>(i64, const u32 *, const T *, const T *, T *, hipStream_t,       \
                                  const u32 *);                                                    \
     template void terminal_solve<T>(i64, const u32 *, const u32 *, const u64 *, const T *, i64, \
-                                    const T *, T *, hipStream_t);                               \
+                                    const T *, T *, hipStream_t, bool);                         \
     template void fill<T>(i64, T, T *, hipStream_t);                                            \
     template void block_solve<T>(const u32 *, const u32 *, const u32 *, const u64 *, const T *,   \
                                  u64, u64, const T *, T *, T *, hipStream_t);                     \
@@ -1309,7 +1408,7 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
     template void vcycle_tail<T>(const level_desc<T> *, int, double, double, const T *,         \
                                  const double *, const double *, T *, hipStream_t);              \
     template bool vcycle_tail_lds_fits<T>(u32, int);                                              \
-    template void vcycle_tail_lds<T>(const level_desc<T> *, const lds_copy *, int, u32, int,      \
+    template void vcycle_tail_lds<T>(const level_desc<T> *, const void *, u32, u32, int,      \
                                      double, double, const T *, const double *, const double *,  \
                                      T *, T *, hipStream_t);                                       \
     template void axpy<T>(i64, T, const T *, T *, hipStream_t);                                 \

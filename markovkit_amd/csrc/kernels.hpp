@@ -109,10 +109,11 @@ void restrict_children(i64 n_coarse, const u32 *cptr, const u32 *cidx, const T *
 template <typename T>
 void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x,
                  hipStream_t s, const u32 *rows = nullptr);
-// dense terminal solves: x[start_b + i] += sum_j inv_b[i][j] * r[start_b + j]
+// dense terminal solves: x[start_b + i] += sum_j inv_b[i][j] * r[start_b + j] (assign: x[...] =)
 template <typename T>
 void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
-                    const T *inverse, i64 max_size, const T *r, T *x, hipStream_t s);
+                    const T *inverse, i64 max_size, const T *r, T *x, hipStream_t s,
+                    bool assign = false);
 
 // Red-black Gauss-Seidel on one level. The level operator is split by colour into two
 // row-mapped operators (rows of one colour, all their entries); a half-sweep updates one colour
@@ -192,8 +193,11 @@ struct level_desc {
     T *rhs = nullptr, *x = nullptr, *xt = nullptr, *res = nullptr; // work vectors
     const u8 *colour = nullptr;        // red-black smoother: colour of every row, else null
     const u32 *perm = nullptr;         // top level of a red-black hierarchy: row -> caller's entry
+    int direct = 0;                    // every row in a terminal block, no coarser level
+    const u32 *crows[2] = {nullptr, nullptr}; // red-black: the rows of each colour, ascending
+    i64 ncrows[2] = {0, 0};
 };
-constexpr i64 kTailRows = 1024;  // levels of at most this many rows run in the fused tail
+constexpr i64 kTailRows = 2048;  // levels of at most this many rows run in the fused tail
 constexpr int kTailThreads = 1024;
 // The V-cycle below and including level L[0] (Jacobi smoother, or red-black Gauss-Seidel on
 // levels with a colour array; one coarse cycle) in one
@@ -204,12 +208,12 @@ constexpr int kTailThreads = 1024;
 template <typename T>
 void vcycle_tail(const level_desc<T> *L, int levels, double shift, double omega, const T *rhs,
                  const double *rhs64, const double *scale, T *out, hipStream_t s);
-// The same tail with every array of its levels in LDS: plan[] copies each constant array
-// (operators, transfers, terminal blocks, colours, top-level permutation) from HBM to its LDS
-// offset, and L_lds holds the levels with every pointer field replaced by (LDS offset + 1)
-// (0 for null); work vectors get LDS offsets too. lds_bytes is the arena size. The result goes
-// to out, else to top_x (HBM).
-struct lds_copy {
+// The same tail with every array of its levels in LDS. image holds the arena's first
+// image_bytes (a multiple of 16): every constant array (operators, transfers, terminal blocks,
+// colours and colour row lists, top-level permutation) at its LDS offset; the work vectors
+// follow in the arena up to lds_bytes. L_lds holds the levels with every pointer field replaced
+// by (LDS offset + 1) (0 for null). The result goes to out, else to top_x (HBM).
+struct lds_copy { // a constant array and its place in the image
     const void *src = nullptr;
     u32 offset = 0;
     u32 bytes = 0;
@@ -218,7 +222,7 @@ constexpr u32 kTailLdsMax = 150 * 1024;
 template <typename T>
 bool vcycle_tail_lds_fits(u32 lds_bytes, int levels);
 template <typename T>
-void vcycle_tail_lds(const level_desc<T> *L_lds, const lds_copy *plan, int n_plan, u32 lds_bytes,
+void vcycle_tail_lds(const level_desc<T> *L_lds, const void *image, u32 image_bytes, u32 lds_bytes,
                      int levels, double shift, double omega, const T *rhs, const double *rhs64,
                      const double *scale, T *out, T *top_x, hipStream_t s);
 

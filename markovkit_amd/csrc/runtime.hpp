@@ -383,6 +383,12 @@ inline std::atomic<int> &device_setup_switch() {
 }
 
 // levels of at most this many rows run in the fused tail (MK_TAIL_ROWS overrides, for tuning)
+// the levels whose coarse correction takes coarse_cycles visits: the top cycle_levels of them
+// (all when cycle_levels is 0); the ones below visit their coarse level once
+inline i64 cycle_levels(const mk::multigrid_options &o) {
+    return o.cycle_levels > 0 ? static_cast<i64>(o.cycle_levels) : i64{1} << 30;
+}
+
 inline i64 tail_rows() {
     static const i64 rows = [] {
         const char *env = std::getenv("MK_TAIL_ROWS");
@@ -585,6 +591,10 @@ struct level_dev {
     // terminal blocks on this level
     i64 n_term = 0, term_max = 0;
     dbuf tstart, tsize, toff, tinv;
+    // a coarse level whose rows all lie in terminal blocks (and no coarser level): its cycle
+    // from zero is x = T^-1 rhs, the value the reference's smooth / solve / smooth sequence
+    // reaches up to rounding, since the block solve is exact
+    bool direct = false;
     // host copies: block starts, sizes, offsets and the dense blocks of (diag(exit) - C)
     // without the shift, laid out like tinv (re-inverted per shift by batched solves)
     std::vector<u32> h_tstart, h_tsize;
@@ -634,11 +644,14 @@ struct mk_sector_mg_s {
     mk::rt::dbuf d_b, d_x64, d_out; // staging for host-buffer calls
     // level descriptors of the fused tail (levels tail_start .. end run in one workgroup), and
     // its LDS-resident form when the levels fit (tail_lds_bytes > 0): descriptors with LDS
-    // offsets and the copy plan of their constant arrays
+    // offsets, and tail_image, the arena's constant part (its first tail_image_bytes, every
+    // constant array at its LDS offset) assembled in HBM by tail_copies, so that the kernel
+    // stages it with one contiguous read
     mk::rt::dbuf tail_desc;
     mk::i64 tail_start = -1;
-    mk::rt::dbuf tail_lds_desc, tail_plan;
-    int tail_plan_n = 0;
+    mk::rt::dbuf tail_lds_desc, tail_image;
+    std::vector<mk::gpu::lds_copy> tail_copies;
+    mk::u32 tail_image_bytes = 0;
     mk::u32 tail_lds_bytes = 0;
 };
 

@@ -73,6 +73,30 @@ def test_batch_matches_single_solves(small, precision, restart):
         print(f"[batch m={restart}] s={s} its={res[j].iterations} single its={r1.iterations}")
 
 
+@pytest.mark.parametrize("cycles,levels", [(2, 1), (2, 0)], ids=["W-top1", "W"])
+def test_batch_w_cycle_levels(small, cycles, levels):
+    """A batch takes its hierarchy's cycle shape: W-cycles on every level, or on the top
+    cycle_levels only. Three FGMRES iterations from zero (not converged) give the single-system
+    solver's iterate up to rounding, which a different cycle shape would not."""
+    w, box, Q = small
+    n = box.size()
+    shifts = [0.5, 5.0]
+    B = np.array(rhs_set(n, [2, 3]))
+    opts = dict(precision=mk.FP64, coarse_cycles=cycles, cycle_levels=levels)
+    M = mk.SectorMultigrid(w.model.domain, box, 0.5, mk.MultigridOptions(**opts))
+    assert M.max_levels() >= 3
+    X, res = mk.BatchSolver(Q, M, shifts).solve(B, tolerances=[1e-30] * 2, restart=50,
+                                                max_iterations=3)
+    for j, s in enumerate(shifts):
+        assert res[j].iterations == 3 and not res[j].converged, res[j]
+        Ms = mk.SectorMultigrid(w.model.domain, box, s, mk.MultigridOptions(**opts))
+        x1 = np.zeros(n)
+        mk.gmres(mk.ShiftedGenerator(Q, s), Ms, B[j], x1, tolerance=1e-30, restart=50,
+                 max_iterations=3)
+        err = np.abs(X[j] - x1).max() / np.abs(x1).max()
+        assert err <= 1e-9, (j, err)
+
+
 def test_batch_against_oracle_and_lumped(small):
     w, box, Q = small
     n = box.size()

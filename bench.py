@@ -378,8 +378,11 @@ def main():
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--workers", type=int, default=4, help="serving: worker streams")
     ap.add_argument("--restart", type=int, default=50)
-    ap.add_argument("--pre", type=int, default=2, help="pre-smoothing sweeps (multigrid_options)")
-    ap.add_argument("--post", type=int, default=2, help="post-smoothing sweeps (multigrid_options)")
+    ap.add_argument("--pre", type=int, default=None,
+                    help="pre-smoothing sweeps (multigrid_options; default: 1 on the large "
+                         "workload, else the reference's 2)")
+    ap.add_argument("--post", type=int, default=None,
+                    help="post-smoothing sweeps (multigrid_options; default: the reference's 2)")
     ap.add_argument("--cycles", type=int, default=None,
                     help="coarse_cycles of the timed hierarchy (1 V-cycle, 2 W-cycle; default: 2 "
                          "on the large workload, the reference's 1 on the others)")
@@ -456,9 +459,14 @@ def main():
     log(f"[setup] generator {t_gen:.1f}s, {Q.n_jumps()} jumps")
     t = time.perf_counter()
     smoother = getattr(mk, SMOOTHERS[args.smoother])
-    # the large workload's hierarchy: W-cycles on the top 3 levels, dense sectors of <= 100 states
+    # the large workload's hierarchy: 1 + 2 sweeps, W-cycles on the top 3 levels, dense sectors
+    # of <= 100 states
     ref = mk.MultigridOptions()
     tuned_large = args.workload == "large"
+    if args.pre is None:
+        args.pre = 1 if tuned_large else ref.pre_sweeps
+    if args.post is None:
+        args.post = ref.post_sweeps
     if args.cycles is None:
         args.cycles = 2 if tuned_large else ref.coarse_cycles
     if args.cycle_levels is None:

@@ -710,17 +710,24 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             // is cx: that cycle writes cx directly
             run_level<T>(mg, L, l + 1, dp<T>(c.crhs), nullptr, dp<T>(c.cx));
             const double cbytes = op_bytes(c.A, ts, ts, 2);
+            // the last cycle's correction is added inside the prolongation (cx + ce, then the
+            // weighted add, as the separate axpy and prolongation round)
+            T *last = nullptr;
             for (int cyc = 1; cyc < mg.opt.coarse_cycles; ++cyc) {
                 P.run(cls(K_RESIDUAL), cbytes, [&] {
                     gpu::level_residual<T>(c.A.view, shift, dp<T>(c.crhs), dp<T>(c.cx),
                                            dp<T>(c.rhs), s);
                 });
                 T *ce = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr);
-                P.run(cls(K_OTHER), double(c.n) * 3 * ts, [&] { gpu::axpy<T>(c.n, T(1), ce, dp<T>(c.cx), s); });
+                if (cyc + 1 < mg.opt.coarse_cycles) {
+                    P.run(cls(K_OTHER), double(c.n) * 3 * ts, [&] { gpu::axpy<T>(c.n, T(1), ce, dp<T>(c.cx), s); });
+                } else {
+                    last = ce;
+                }
             }
             P.run(cls(K_PROLONG), pbytes, [&] {
                 gpu::prolong_add<T>(np, dp<u32>(lv.coarse_of), dp<T>(lv.weight), dp<T>(c.cx),
-                                    cur, s, prows);
+                                    cur, s, prows, last);
             });
         }
     }

@@ -292,12 +292,16 @@ __global__ void __launch_bounds__(kThreads)
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_prolong_add(i64 n, const u32 *rows, const u32 *coarse_of, const T *weight, const T *coarse, T *x) {
+    k_prolong_add(i64 n, const u32 *rows, const u32 *coarse_of, const T *weight, const T *coarse,
+                  const T *coarse2, T *x) {
     for (i64 q = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; q < n;
          q += static_cast<i64>(gridDim.x) * kThreads) {
         const i64 i = rows ? rows[q] : q;
         const u32 c = coarse_of[i];
-        if (c != 0xFFFFFFFFu) x[i] += weight[i] * coarse[c];
+        if (c == 0xFFFFFFFFu) continue;
+        T cv = coarse[c];
+        if (coarse2) cv += coarse2[c]; // the coarse iterate's last correction, added first
+        x[i] += weight[i] * cv;
     }
 }
 
@@ -1206,10 +1210,10 @@ void restrict_children(i64 n_coarse, const u32 *cptr, const u32 *cidx, const T *
 
 template <typename T>
 void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x,
-                 hipStream_t s, const u32 *rows) {
+                 hipStream_t s, const u32 *rows, const T *coarse2) {
     if (n == 0) return;
     hipLaunchKernelGGL((k_prolong_add<T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, rows, coarse_of,
-                       weight, coarse, x);
+                       weight, coarse, coarse2, x);
 }
 
 template <typename T>
@@ -1393,7 +1397,7 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
                                        hipStream_t);                                            \
     template void gather_sum<T>(const csr_view<T> &, const T *, T *, hipStream_t);              \
     template void prolong_add<T>(i64, const u32 *, const T *, const T *, T *, hipStream_t,       \
-                                 const u32 *);                                                    \
+                                 const u32 *, const T *);                                         \
     template void terminal_solve<T>(i64, const u32 *, const u32 *, const u64 *, const T *, i64, \
                                     const T *, T *, hipStream_t, bool);                         \
     template void fill<T>(i64, T, T *, hipStream_t);                                            \

@@ -108,7 +108,7 @@ void restrict_children(i64 n_coarse, const u32 *cptr, const u32 *cidx, const T *
 // rows i = rows[q] (all n rows when rows is null)
 template <typename T>
 void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x,
-                 hipStream_t s, const u32 *rows = nullptr);
+                 hipStream_t s, const u32 *rows = nullptr, const T *coarse2 = nullptr);
 // dense terminal solves: x[start_b + i] += sum_j inv_b[i][j] * r[start_b + j] (assign: x[...] =)
 template <typename T>
 void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,

@@ -1,8 +1,10 @@
-"""The fused multigrid tail (levels of at most 1024 states in one workgroup-wide kernel), with
+"""The fused multigrid tail (levels of at most 2048 states in one workgroup-wide kernel), with
 its levels in HBM or staged in LDS, against the per-level kernels: same formulas and summation
-orders, so the cycle and the solves must be bitwise identical, in FP32 and FP64, with the Jacobi and red-black Gauss-Seidel smoothers, on
-1D and 2D boxes, for a whole cycle in the tail (969 states) and for a tail below larger levels
-(58,905 states)."""
+orders, so the cycle and the solves must be bitwise identical, in FP32 and FP64, with the
+Jacobi and red-black Gauss-Seidel smoothers, on 1D and 2D boxes, for a whole cycle in the tail
+(969 states) and for a tail below larger levels (58,905 states), with V-cycles and with
+W-cycles on the top levels over dense bottom blocks (the bench's hierarchy shape, whose
+bottom level is solved directly); and the LDS image after a shift change."""
 import numpy as np
 import pytest
 
@@ -12,16 +14,22 @@ from markovkit_amd.workloads import annihilation_2d, birth_death_1d, config_smal
 pytestmark = pytest.mark.gpu
 
 
+SHAPES = {"V": {}, "W3-c100": dict(pre_sweeps=1, post_sweeps=2, coarse_cycles=2, cycle_levels=3,
+                                   coarsest_size=100)}
+
+
+@pytest.mark.parametrize("shape", list(SHAPES))
 @pytest.mark.parametrize("smoother", [mk.JACOBI, mk.RED_BLACK], ids=["jacobi", "red_black"])
 @pytest.mark.parametrize("precision", [mk.FP32, mk.FP64], ids=["fp32", "fp64"])
 @pytest.mark.parametrize("wl", [config_small(), annihilation_2d(4, (2, 2), 1.0),
                                 birth_death_1d(32, 4, 0.7, "random")], ids=["969", "2d", "58905"])
-def test_tail_matches_per_level_kernels(wl, precision, smoother):
+def test_tail_matches_per_level_kernels(wl, precision, smoother, shape):
     box = mk.PopulationBox(wl.n_voxels, wl.max_totals)
     Q = mk.Generator(box, wl.model)
     s = wl.shifts[0]
     M = mk.SectorMultigrid(wl.model.domain, box, s,
-                           mk.MultigridOptions(precision=precision, smoother=smoother))
+                           mk.MultigridOptions(precision=precision, smoother=smoother,
+                                               **SHAPES[shape]))
     rng = np.random.default_rng(3)
     b = rng.random(box.size())
     try:
@@ -41,3 +49,24 @@ def test_tail_matches_per_level_kernels(wl, precision, smoother):
     finally:
         mk.fused_tail_enable(True)
         mk.tail_lds_enable(True)
+
+
+@pytest.mark.parametrize("precision", [mk.FP32, mk.FP64], ids=["fp32", "fp64"])
+def test_tail_image_follows_set_shift(precision):
+    """The LDS tail stages an image of its constant arrays assembled at setup; the terminal
+    inverses change with the shift, so after set_shift the image must be rebuilt: a retargeted
+    hierarchy gives the cycle of one built at the new shift (up to the rounding of the two
+    inversions; a stale image would be off at the size of the change)."""
+    wl = birth_death_1d(32, 4, 0.7, "random")
+    box = mk.PopulationBox(wl.n_voxels, wl.max_totals)
+    opts = mk.MultigridOptions(precision=precision, smoother=mk.RED_BLACK, pre_sweeps=1,
+                               coarse_cycles=2, cycle_levels=2, coarsest_size=100)
+    b = np.random.default_rng(5).random(box.size())
+    M = mk.SectorMultigrid(wl.model.domain, box, 0.7, opts)
+    z_old = M.apply(b)
+    M.set_shift(3.0)
+    z = M.apply(b)
+    z_ref = mk.SectorMultigrid(wl.model.domain, box, 3.0, opts).apply(b)
+    err = np.abs(z - z_ref).max() / np.abs(z_ref).max()
+    assert err <= (1e-10 if precision == mk.FP64 else 1e-5), err
+    assert np.abs(z_old - z_ref).max() / np.abs(z_ref).max() > 1e-2

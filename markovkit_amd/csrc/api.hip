@@ -137,7 +137,9 @@ void build_tail_lds(mk_sector_mg_s &mg, const std::vector<level_dev<T>> &L,
     const int levels = static_cast<int>(depth - mg.tail_start);
     std::vector<gpu::level_desc<T>> ld;
     std::vector<gpu::lds_copy> plan;
-    u64 off = 0;
+    // the descriptors first
+    const u64 dbytes = (static_cast<u64>(levels) * sizeof(gpu::level_desc<T>) + 15) / 16 * 16;
+    u64 off = dbytes;
     // offset + 1 as a pointer (0 stays null); src null: a work array, nothing to copy
     auto place = [&](const void *src, u64 bytes, bool needed) -> std::uintptr_t {
         if (!needed || bytes == 0) return 0;
@@ -190,9 +192,10 @@ void build_tail_lds(mk_sector_mg_s &mg, const std::vector<level_dev<T>> &L,
     if (!ok || !gpu::vcycle_tail_lds_fits<T>(static_cast<u32>(std::min<u64>(off, ~0u)), levels) ||
         off > gpu::kTailLdsMax)
         return;
-    mg.tail_lds_desc = upload(ld);
     mg.tail_copies = std::move(plan);
     mg.tail_image = dbuf(std::max<u64>(image_bytes, 16));
+    HIPCHK(hipMemcpy(mg.tail_image.p, ld.data(), ld.size() * sizeof(gpu::level_desc<T>),
+                     hipMemcpyHostToDevice));
     mg.tail_image_bytes = static_cast<u32>(image_bytes);
     mg.tail_lds_bytes = static_cast<u32>(off);
     refresh_tail_image(mg);
@@ -542,8 +545,7 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
                          tail_lds_switch().load() != 0;
         P.run(cls(K_SWEEP), 0.0, [&] {
             if (lds) {
-                gpu::vcycle_tail_lds<T>(dp<gpu::level_desc<T>>(mg.tail_lds_desc), mg.tail_image.p,
-                                        mg.tail_image_bytes, mg.tail_lds_bytes, static_cast<int>(L.size() - l),
+                gpu::vcycle_tail_lds<T>(mg.tail_image.p, mg.tail_image_bytes, mg.tail_lds_bytes, static_cast<int>(L.size() - l),
                                         shift, omega, rhs, rhs64, scale, out, dp<T>(lv.x), s);
             } else {
                 gpu::vcycle_tail<T>(desc, static_cast<int>(L.size() - l), shift, omega, rhs, rhs64,

@@ -752,12 +752,12 @@ constexpr int kTailLdsLevels = 12;
 // the image of the constant arrays); work vectors live in LDS too.
 template <typename T>
 __global__ void __launch_bounds__(kTailThreads)
-    k_vcycle_tail_lds(const level_desc<T> *L_lds, const uint4 *image, u32 image_bytes, int levels,
+    k_vcycle_tail_lds(const uint4 *image, u32 image_bytes, int levels,
                       T shift, T omega, const T *rhs_in, const double *rhs64, const double *scale_ptr,
                       T *out, T *top_x) {
     extern __shared__ uint4 arena[];
-    __shared__ uint4 dstore[(kTailLdsLevels * sizeof(level_desc<T>) + 15) / 16];
-    level_desc<T> *d = reinterpret_cast<level_desc<T> *>(dstore);
+    // the image starts with the level descriptors, pointer fields holding LDS offset + 1
+    level_desc<T> *d = reinterpret_cast<level_desc<T> *>(arena);
     char *base = reinterpret_cast<char *>(arena);
     const int tid = threadIdx.x;
     const u32 nt = blockDim.x;
@@ -780,8 +780,9 @@ __global__ void __launch_bounds__(kTailThreads)
         st(b0 + 6 * nt, v6);
         st(b0 + 7 * nt, v7);
     }
+    __syncthreads();
     if (tid < levels) {
-        level_desc<T> lv = L_lds[tid];
+        level_desc<T> lv = d[tid];
         rebase_view(lv.A, base);
         rebase_view(lv.R, base);
         lv.coarse_of = rebase(lv.coarse_of, base);
@@ -1283,7 +1284,7 @@ bool vcycle_tail_lds_fits(u32 lds_bytes, int levels) {
 }
 
 template <typename T>
-void vcycle_tail_lds(const level_desc<T> *L_lds, const void *image, u32 image_bytes, u32 lds_bytes,
+void vcycle_tail_lds(const void *image, u32 image_bytes, u32 lds_bytes,
                      int levels, double shift, double omega, const T *rhs, const double *rhs64,
                      const double *scale, T *out, T *top_x, hipStream_t s) {
     if (levels <= 0) return;
@@ -1293,9 +1294,9 @@ void vcycle_tail_lds(const level_desc<T> *L_lds, const void *image, u32 image_by
                                    static_cast<int>(kTailLdsMax)) == hipSuccess;
     }();
     (void)attr;
-    hipLaunchKernelGGL((k_vcycle_tail_lds<T>), dim3(1), dim3(kTailThreads), lds_bytes, s, L_lds,
-                       static_cast<const uint4 *>(image), image_bytes, levels, static_cast<T>(shift), static_cast<T>(omega), rhs, rhs64,
-                       scale, out, top_x);
+    hipLaunchKernelGGL((k_vcycle_tail_lds<T>), dim3(1), dim3(kTailThreads), lds_bytes, s,
+                       static_cast<const uint4 *>(image), image_bytes, levels, static_cast<T>(shift),
+                       static_cast<T>(omega), rhs, rhs64, scale, out, top_x);
 }
 
 template <typename T>
@@ -1412,7 +1413,7 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
     template void vcycle_tail<T>(const level_desc<T> *, int, double, double, const T *,         \
                                  const double *, const double *, T *, hipStream_t);              \
     template bool vcycle_tail_lds_fits<T>(u32, int);                                              \
-    template void vcycle_tail_lds<T>(const level_desc<T> *, const void *, u32, u32, int,      \
+    template void vcycle_tail_lds<T>(const void *, u32, u32, int,      \
                                      double, double, const T *, const double *, const double *,  \
                                      T *, T *, hipStream_t);                                       \
     template void axpy<T>(i64, T, const T *, T *, hipStream_t);                                 \

@@ -209,10 +209,10 @@ template <typename T>
 void vcycle_tail(const level_desc<T> *L, int levels, double shift, double omega, const T *rhs,
                  const double *rhs64, const double *scale, T *out, hipStream_t s);
 // The same tail with every array of its levels in LDS. image holds the arena's first
-// image_bytes (a multiple of 16): every constant array (operators, transfers, terminal blocks,
-// colours and colour row lists, top-level permutation) at its LDS offset; the work vectors
-// follow in the arena up to lds_bytes. L_lds holds the levels with every pointer field replaced
-// by (LDS offset + 1) (0 for null). The result goes to out, else to top_x (HBM).
+// image_bytes (a multiple of 16): the level descriptors, with every pointer field replaced by
+// (LDS offset + 1) (0 for null), then every constant array (operators, transfers, terminal
+// blocks, colours and colour row lists, top-level permutation) at its LDS offset; the work
+// vectors follow in the arena up to lds_bytes. The result goes to out, else to top_x (HBM).
 struct lds_copy { // a constant array and its place in the image
     const void *src = nullptr;
     u32 offset = 0;
@@ -222,7 +222,7 @@ constexpr u32 kTailLdsMax = 150 * 1024;
 template <typename T>
 bool vcycle_tail_lds_fits(u32 lds_bytes, int levels);
 template <typename T>
-void vcycle_tail_lds(const level_desc<T> *L_lds, const void *image, u32 image_bytes, u32 lds_bytes,
+void vcycle_tail_lds(const void *image, u32 image_bytes, u32 lds_bytes,
                      int levels, double shift, double omega, const T *rhs, const double *rhs64,
                      const double *scale, T *out, T *top_x, hipStream_t s);
 

@@ -643,13 +643,13 @@ struct mk_sector_mg_s {
     std::vector<mk::rt::level_dev<float>> L32;
     mk::rt::dbuf d_b, d_x64, d_out; // staging for host-buffer calls
     // level descriptors of the fused tail (levels tail_start .. end run in one workgroup), and
-    // its LDS-resident form when the levels fit (tail_lds_bytes > 0): descriptors with LDS
-    // offsets, and tail_image, the arena's constant part (its first tail_image_bytes, every
-    // constant array at its LDS offset) assembled in HBM by tail_copies, so that the kernel
-    // stages it with one contiguous read
+    // its LDS-resident form when the levels fit (tail_lds_bytes > 0): tail_image, the arena's
+    // constant part (its first tail_image_bytes: the descriptors with LDS offsets, then every
+    // constant array at its LDS offset, copied there by tail_copies) assembled in HBM, so that
+    // the kernel stages it with one contiguous read
     mk::rt::dbuf tail_desc;
     mk::i64 tail_start = -1;
-    mk::rt::dbuf tail_lds_desc, tail_image;
+    mk::rt::dbuf tail_image;
     std::vector<mk::gpu::lds_copy> tail_copies;
     mk::u32 tail_image_bytes = 0;
     mk::u32 tail_lds_bytes = 0;

@@ -184,6 +184,7 @@ void build_tail_lds(mk_sector_mg_s &mg, const std::vector<level_dev<T>> &L,
     const u64 image_bytes = off;
     for (std::size_t q = 0; q < ld.size() && ok; ++q) {
         const u64 n = static_cast<u64>(ld[q].n), ts = sizeof(T);
+        ld[q].acc = reinterpret_cast<T *>(place(nullptr, n * ts, ld[q].acc != nullptr));
         ld[q].rhs = reinterpret_cast<T *>(place(nullptr, n * ts, true));
         ld[q].x = reinterpret_cast<T *>(place(nullptr, n * ts, true));
         ld[q].xt = reinterpret_cast<T *>(place(nullptr, n * ts, true));
@@ -475,6 +476,10 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
         if (start < depth && depth - start <= 32 && !(start == depth - 1 && L[start].direct))
             mg.tail_start = start;
     }
+    // a W visit from the level above the tail runs its coarse cycles inside the tail, which
+    // accumulates the iterate in that level's cx
+    mg.tail_w = mg.tail_start > 0 && mg.opt.coarse_cycles > 1 && mg.tail_start - 1 < cycle_levels(mg.opt);
+    if (mg.tail_w) desc[mg.tail_start].acc = dp<T>(L[mg.tail_start].cx);
     mg.tail_desc = upload(desc);
     build_tail_lds<T>(mg, L, desc);
 }
@@ -532,7 +537,7 @@ void block_smooth(const mk_sector_mg_s &mg, level_dev<T> &lv, std::size_t level,
 // result is written to out when given, else left in a level buffer whose pointer is returned.
 template <typename T>
 T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t l, const T *rhs,
-             const double *rhs64, T *out, const double *scale = nullptr) {
+             const double *rhs64, T *out, const double *scale = nullptr, int cycles = 1) {
     auto &lv = L[l];
     auto &P = prof();
     auto cls = [&](kclass c) { return l == 0 ? c : K_COARSE; };
@@ -546,14 +551,15 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
         P.run(cls(K_SWEEP), 0.0, [&] {
             if (lds) {
                 gpu::vcycle_tail_lds<T>(mg.tail_image.p, mg.tail_image_bytes, mg.tail_lds_bytes, static_cast<int>(L.size() - l),
-                                        shift, omega, rhs, rhs64, scale, out, dp<T>(lv.x), s);
+                                        shift, omega, rhs, rhs64, scale, out, dp<T>(lv.x), s, cycles);
             } else {
                 gpu::vcycle_tail<T>(desc, static_cast<int>(L.size() - l), shift, omega, rhs, rhs64,
-                                    scale, out, s);
+                                    scale, out, s, cycles);
             }
         });
         return out ? out : dp<T>(lv.x);
     }
+    if (cycles != 1) throw std::logic_error("sector_multigrid: coarse cycles inside a level outside the tail");
     if (lv.direct) {
         T *dst = out ? out : dp<T>(lv.x);
         P.run(cls(K_TERMINAL), 0.0, [&] {
@@ -708,14 +714,18 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             P.run(cls(K_RESTRICT), rbytes, [&] {
                 gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.crhs), s);
             });
+            // the tail runs all of the coarse cycles in one launch
+            const bool in_tail = mg.tail_w && mg.tail_start == static_cast<i64>(l + 1) &&
+                                 tail_switch().load() != 0;
             // from cx = 0 the first coarse residual is crhs itself and the first correction
             // is cx: that cycle writes cx directly
-            run_level<T>(mg, L, l + 1, dp<T>(c.crhs), nullptr, dp<T>(c.cx));
+            run_level<T>(mg, L, l + 1, dp<T>(c.crhs), nullptr, dp<T>(c.cx), nullptr,
+                         in_tail ? mg.opt.coarse_cycles : 1);
             const double cbytes = op_bytes(c.A, ts, ts, 2);
             // the last cycle's correction is added inside the prolongation (cx + ce, then the
             // weighted add, as the separate axpy and prolongation round)
             T *last = nullptr;
-            for (int cyc = 1; cyc < mg.opt.coarse_cycles; ++cyc) {
+            for (int cyc = 1; !in_tail && cyc < mg.opt.coarse_cycles; ++cyc) {
                 P.run(cls(K_RESIDUAL), cbytes, [&] {
                     gpu::level_residual<T>(c.A.view, shift, dp<T>(c.crhs), dp<T>(c.cx),
                                            dp<T>(c.rhs), s);

@@ -562,14 +562,18 @@ __device__ __forceinline__ void tail_terminal(const level_desc<T> &lv, const T *
 template <bool LDS, typename T>
 __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T shift, T omega,
                                           const T *rhs_in, const double *rhs64, const double *scale_ptr,
-                                          T *out, T *top_x) {
-    // bit k: level k's current iterate is in xt (the Jacobi sweeps alternate x and xt); levels
-    // are at most 32
-    u32 flip = 0;
+                                          T *out, T *top_x, int cycles) {
     T *top = nullptr; // the top level's result
     const int tid = threadIdx.x, nt = blockDim.x;
     const double scale = scale_ptr ? *scale_ptr : 1.0;
     const u32 *perm0 = L[0].perm; // the top level's row order differs from the caller's
+    T *acc = L[0].acc;            // cycles > 1: the top level's accumulated iterate
+    for (int cyc = 0; cyc < cycles; ++cyc) {
+    // the top level's last Jacobi sweep writes out directly in a single cycle
+    T *vout = cycles == 1 ? out : nullptr;
+    // bit k: level k's current iterate is in xt (the Jacobi sweeps alternate x and xt); levels
+    // are at most 32
+    u32 flip = 0;
     for (int k = 0; k < levels; ++k) {
         const level_desc<T> &lv = L[k];
         const T *rhs = lv.rhs;
@@ -585,7 +589,7 @@ __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T 
         // only, colour 1 follows
         for (i64 i = tid; i < lv.n; i += nt) {
             T b;
-            if (k == 0) {
+            if (k == 0 && cyc == 0) {
                 const i64 src = perm0 ? lds<LDS>(perm0)[i] : i;
                 b = rhs64 ? static_cast<T>(rhs64[src] * scale) : rhs_in[src];
                 lds<LDS>(lv.rhs)[i] = b;
@@ -689,7 +693,7 @@ __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T 
         }
         for (int p = 0; !lv.colour && p < lv.post; ++p) {
             // the top level's last sweep may write out (HBM): dst is accessed as generic memory
-            T *dst = (k == 0 && p == lv.post - 1 && out) ? out : xt;
+            T *dst = (k == 0 && p == lv.post - 1 && vout) ? vout : xt;
             for (i64 i = tid; i < lv.n; i += nt) {
                 T ex = 0;
                 const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
@@ -713,6 +717,37 @@ __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T 
             top = x;
         }
     }
+    if (cycles > 1) {
+        // coarse cycles of the caller's W visit (no permutation: the tail starts below the top
+        // level): acc = the first result, acc += each later correction (as an axpy), and the
+        // last correction added on the way out; every later cycle starts from zero on the
+        // top level's residual rhs_in - A acc
+        const level_desc<T> &lv = L[0];
+        const bool last = cyc + 1 == cycles;
+        T *fin = out ? out : top_x;
+        for (i64 i = tid; i < lv.n; i += nt) {
+            const T t = lds<LDS>(top)[i];
+            if (cyc == 0) {
+                lds<LDS>(acc)[i] = t;
+            } else if (!last) {
+                lds<LDS>(acc)[i] += t;
+            } else {
+                fin[i] = lds<LDS>(acc)[i] + t;
+            }
+        }
+        __syncthreads();
+        if (!last) {
+            for (i64 i = tid; i < lv.n; i += nt) {
+                T ex = 0;
+                const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), acc, ex);
+                const T d = shift + ex;
+                lds<LDS>(lv.rhs)[i] = rhs_in[i] - (d * lds<LDS>(acc)[i] - sum);
+            }
+            __syncthreads();
+        }
+    }
+    }
+    if (cycles > 1) return;
     // the top level's result in out (in the caller's row order), else in top_x
     T *dst = out ? out : top_x;
     if (out && perm0) {
@@ -725,8 +760,8 @@ __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T 
 template <typename T>
 __global__ void __launch_bounds__(kTailThreads)
     k_vcycle_tail(const level_desc<T> *L, int levels, T shift, T omega, const T *rhs_in,
-                  const double *rhs64, const double *scale_ptr, T *out) {
-    tail_body<false>(L, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, L[0].x);
+                  const double *rhs64, const double *scale_ptr, T *out, int cycles) {
+    tail_body<false>(L, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, L[0].x, cycles);
 }
 
 // pointer fields of an LDS level descriptor hold (offset + 1) into the LDS arena (0: null)
@@ -754,7 +789,7 @@ template <typename T>
 __global__ void __launch_bounds__(kTailThreads)
     k_vcycle_tail_lds(const uint4 *image, u32 image_bytes, int levels,
                       T shift, T omega, const T *rhs_in, const double *rhs64, const double *scale_ptr,
-                      T *out, T *top_x) {
+                      T *out, T *top_x, int cycles) {
     extern __shared__ uint4 arena[];
     // the image starts with the level descriptors, pointer fields holding LDS offset + 1
     level_desc<T> *d = reinterpret_cast<level_desc<T> *>(arena);
@@ -799,10 +834,11 @@ __global__ void __launch_bounds__(kTailThreads)
         lv.crows[0] = rebase(lv.crows[0], base);
         lv.crows[1] = rebase(lv.crows[1], base);
         lv.perm = rebase(lv.perm, base);
+        lv.acc = rebase(lv.acc, base);
         __builtin_memcpy(d + tid, &lv, sizeof(lv));
     }
     __syncthreads();
-    tail_body<true>(d, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, top_x);
+    tail_body<true>(d, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, top_x, cycles);
 }
 
 template <typename T>
@@ -1272,10 +1308,10 @@ void copy_rows(const u32 *rows, u64 nrows, const T *src, T *dst, hipStream_t s) 
 
 template <typename T>
 void vcycle_tail(const level_desc<T> *L, int levels, double shift, double omega, const T *rhs,
-                 const double *rhs64, const double *scale, T *out, hipStream_t s) {
+                 const double *rhs64, const double *scale, T *out, hipStream_t s, int cycles) {
     if (levels <= 0) return;
     hipLaunchKernelGGL((k_vcycle_tail<T>), dim3(1), dim3(kTailThreads), 0, s, L, levels,
-                       static_cast<T>(shift), static_cast<T>(omega), rhs, rhs64, scale, out);
+                       static_cast<T>(shift), static_cast<T>(omega), rhs, rhs64, scale, out, cycles);
 }
 
 template <typename T>
@@ -1286,7 +1322,7 @@ bool vcycle_tail_lds_fits(u32 lds_bytes, int levels) {
 template <typename T>
 void vcycle_tail_lds(const void *image, u32 image_bytes, u32 lds_bytes,
                      int levels, double shift, double omega, const T *rhs, const double *rhs64,
-                     const double *scale, T *out, T *top_x, hipStream_t s) {
+                     const double *scale, T *out, T *top_x, hipStream_t s, int cycles) {
     if (levels <= 0) return;
     static const bool attr = [] {
         return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vcycle_tail_lds<T>),
@@ -1296,7 +1332,7 @@ void vcycle_tail_lds(const void *image, u32 image_bytes, u32 lds_bytes,
     (void)attr;
     hipLaunchKernelGGL((k_vcycle_tail_lds<T>), dim3(1), dim3(kTailThreads), lds_bytes, s,
                        static_cast<const uint4 *>(image), image_bytes, levels, static_cast<T>(shift),
-                       static_cast<T>(omega), rhs, rhs64, scale, out, top_x);
+                       static_cast<T>(omega), rhs, rhs64, scale, out, top_x, cycles);
 }
 
 template <typename T>
@@ -1411,11 +1447,11 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
                                 const u32 *, u64, hipStream_t);                                   \
     template void copy_rows<T>(const u32 *, u64, const T *, T *, hipStream_t);                  \
     template void vcycle_tail<T>(const level_desc<T> *, int, double, double, const T *,         \
-                                 const double *, const double *, T *, hipStream_t);              \
+                                 const double *, const double *, T *, hipStream_t, int);         \
     template bool vcycle_tail_lds_fits<T>(u32, int);                                              \
     template void vcycle_tail_lds<T>(const void *, u32, u32, int,      \
                                      double, double, const T *, const double *, const double *,  \
-                                     T *, T *, hipStream_t);                                       \
+                                     T *, T *, hipStream_t, int);                                  \
     template void axpy<T>(i64, T, const T *, T *, hipStream_t);                                 \
     template void combine<T>(double *, const T *, i64, int, const double *, i64, hipStream_t);     \
     template void gs_half<T>(const csr_view<T> &, double, const T *, T *, T *, hipStream_t, T *,   \

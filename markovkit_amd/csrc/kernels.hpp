@@ -196,6 +196,7 @@ struct level_desc {
     int direct = 0;                    // every row in a terminal block, no coarser level
     const u32 *crows[2] = {nullptr, nullptr}; // red-black: the rows of each colour, ascending
     i64 ncrows[2] = {0, 0};
+    T *acc = nullptr;                  // the tail's top level under a W visit: its iterate
 };
 constexpr i64 kTailRows = 2048;  // levels of at most this many rows run in the fused tail
 constexpr int kTailThreads = 1024;
@@ -204,10 +205,12 @@ constexpr int kTailThreads = 1024;
 // workgroup: the same per-row formulas and summation orders as the per-level kernels, phases
 // separated by workgroup barriers. L[0]'s right-hand side is rhs, or (T)(rhs64 * scale) when
 // rhs64 is given (scale from device memory, 1 when null); the result goes to out, else to
-// L[0].x.
+// L[0].x. cycles > 1 (a W visit from the level above; L[0] is then not the hierarchy's top):
+// that many cycles on L[0], each later one from zero on the residual of the accumulated
+// iterate in L[0].acc, as the caller's cycle loop does with the same formulas.
 template <typename T>
 void vcycle_tail(const level_desc<T> *L, int levels, double shift, double omega, const T *rhs,
-                 const double *rhs64, const double *scale, T *out, hipStream_t s);
+                 const double *rhs64, const double *scale, T *out, hipStream_t s, int cycles = 1);
 // The same tail with every array of its levels in LDS. image holds the arena's first
 // image_bytes (a multiple of 16): the level descriptors, with every pointer field replaced by
 // (LDS offset + 1) (0 for null), then every constant array (operators, transfers, terminal
@@ -224,7 +227,7 @@ bool vcycle_tail_lds_fits(u32 lds_bytes, int levels);
 template <typename T>
 void vcycle_tail_lds(const void *image, u32 image_bytes, u32 lds_bytes,
                      int levels, double shift, double omega, const T *rhs, const double *rhs64,
-                     const double *scale, T *out, T *top_x, hipStream_t s);
+                     const double *scale, T *out, T *top_x, hipStream_t s, int cycles = 1);
 
 template <typename T>
 void fill(i64 n, T value, T *x, hipStream_t s);

@@ -649,6 +649,7 @@ struct mk_sector_mg_s {
     // the kernel stages it with one contiguous read
     mk::rt::dbuf tail_desc;
     mk::i64 tail_start = -1;
+    bool tail_w = false; // the level above the tail takes coarse_cycles visits: the tail runs them
     mk::rt::dbuf tail_image;
     std::vector<mk::gpu::lds_copy> tail_copies;
     mk::u32 tail_image_bytes = 0;

@@ -569,183 +569,183 @@ __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T 
     const u32 *perm0 = L[0].perm; // the top level's row order differs from the caller's
     T *acc = L[0].acc;            // cycles > 1: the top level's accumulated iterate
     for (int cyc = 0; cyc < cycles; ++cyc) {
-    // the top level's last Jacobi sweep writes out directly in a single cycle
-    T *vout = cycles == 1 ? out : nullptr;
-    // bit k: level k's current iterate is in xt (the Jacobi sweeps alternate x and xt); levels
-    // are at most 32
-    u32 flip = 0;
-    for (int k = 0; k < levels; ++k) {
-        const level_desc<T> &lv = L[k];
-        const T *rhs = lv.rhs;
-        T *x = lv.x, *xt = lv.xt;
-        const u8 *colour = lv.colour;
-        if (lv.direct && k > 0) {
-            // the cycle from zero on a level of terminal blocks only is their dense solve
-            tail_terminal<LDS, true>(lv, rhs, x);
+        // the top level's last Jacobi sweep writes out directly in a single cycle
+        T *vout = cycles == 1 ? out : nullptr;
+        // bit k: level k's current iterate is in xt (the Jacobi sweeps alternate x and xt); levels
+        // are at most 32
+        u32 flip = 0;
+        for (int k = 0; k < levels; ++k) {
+            const level_desc<T> &lv = L[k];
+            const T *rhs = lv.rhs;
+            T *x = lv.x, *xt = lv.xt;
+            const u8 *colour = lv.colour;
+            if (lv.direct && k > 0) {
+                // the cycle from zero on a level of terminal blocks only is their dense solve
+                tail_terminal<LDS, true>(lv, rhs, x);
+                __syncthreads();
+                continue;
+            }
+            // first sweep from x = 0 (the FP64 input converted on the top level); red-black: colour 0
+            // only, colour 1 follows
+            for (i64 i = tid; i < lv.n; i += nt) {
+                T b;
+                if (k == 0 && cyc == 0) {
+                    const i64 src = perm0 ? lds<LDS>(perm0)[i] : i;
+                    b = rhs64 ? static_cast<T>(rhs64[src] * scale) : rhs_in[src];
+                    lds<LDS>(lv.rhs)[i] = b;
+                } else {
+                    b = lds<LDS>(rhs)[i];
+                }
+                if (!colour) {
+                    lds<LDS>(x)[i] = lv.pre > 0 ? omega * b / (shift + tail_exit<LDS>(lv.A, i)) : T(0);
+                } else if (lv.pre == 0) {
+                    lds<LDS>(x)[i] = T(0);
+                } else if (lds<LDS>(colour)[i] == 0) {
+                    lds<LDS>(x)[i] = b / (shift + tail_exit<LDS>(lv.A, i));
+                }
+            }
             __syncthreads();
-            continue;
-        }
-        // first sweep from x = 0 (the FP64 input converted on the top level); red-black: colour 0
-        // only, colour 1 follows
-        for (i64 i = tid; i < lv.n; i += nt) {
-            T b;
-            if (k == 0 && cyc == 0) {
-                const i64 src = perm0 ? lds<LDS>(perm0)[i] : i;
-                b = rhs64 ? static_cast<T>(rhs64[src] * scale) : rhs_in[src];
-                lds<LDS>(lv.rhs)[i] = b;
-            } else {
-                b = lds<LDS>(rhs)[i];
-            }
-            if (!colour) {
-                lds<LDS>(x)[i] = lv.pre > 0 ? omega * b / (shift + tail_exit<LDS>(lv.A, i)) : T(0);
-            } else if (lv.pre == 0) {
-                lds<LDS>(x)[i] = T(0);
-            } else if (lds<LDS>(colour)[i] == 0) {
-                lds<LDS>(x)[i] = b / (shift + tail_exit<LDS>(lv.A, i));
-            }
-        }
-        __syncthreads();
-        if (colour) {
-            for (int half = 1; half < 2 * lv.pre; ++half) {
-                const int c = half & 1;
-                tail_gs_half<LDS>(lv.A, lv.crows[c], lv.ncrows[c], shift, rhs, x);
+            if (colour) {
+                for (int half = 1; half < 2 * lv.pre; ++half) {
+                    const int c = half & 1;
+                    tail_gs_half<LDS>(lv.A, lv.crows[c], lv.ncrows[c], shift, rhs, x);
+                    __syncthreads();
+                }
+                if (lv.pre > 0) {
+                    // after the last colour-1 half-sweep only colour-0 rows have a residual
+                    for (i64 q = tid; q < lv.ncrows[0]; q += nt) {
+                        const u32 i = lds<LDS>(lv.crows[0])[q];
+                        T ex = 0;
+                        const T sum = tail_gather<LDS>(lv.A, i, x, ex);
+                        const T d = shift + ex;
+                        lds<LDS>(lv.res)[i] = lds<LDS>(rhs)[i] - (d * lds<LDS>(x)[i] - sum);
+                    }
+                    for (i64 q = tid; q < lv.ncrows[1]; q += nt) lds<LDS>(lv.res)[lds<LDS>(lv.crows[1])[q]] = T(0);
+                } else {
+                    for (i64 i = tid; i < lv.n; i += nt) {
+                        T ex = 0;
+                        const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
+                        const T d = shift + ex;
+                        lds<LDS>(lv.res)[i] = lds<LDS>(rhs)[i] - (d * lds<LDS>(x)[i] - sum);
+                    }
+                }
                 __syncthreads();
             }
-            if (lv.pre > 0) {
-                // after the last colour-1 half-sweep only colour-0 rows have a residual
-                for (i64 q = tid; q < lv.ncrows[0]; q += nt) {
-                    const u32 i = lds<LDS>(lv.crows[0])[q];
-                    T ex = 0;
-                    const T sum = tail_gather<LDS>(lv.A, i, x, ex);
-                    const T d = shift + ex;
-                    lds<LDS>(lv.res)[i] = lds<LDS>(rhs)[i] - (d * lds<LDS>(x)[i] - sum);
-                }
-                for (i64 q = tid; q < lv.ncrows[1]; q += nt) lds<LDS>(lv.res)[lds<LDS>(lv.crows[1])[q]] = T(0);
-            } else {
+            for (int p = 1; !colour && p < lv.pre; ++p) {
                 for (i64 i = tid; i < lv.n; i += nt) {
                     T ex = 0;
                     const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
+                    const T xr = lds<LDS>(x)[i];
                     const T d = shift + ex;
-                    lds<LDS>(lv.res)[i] = lds<LDS>(rhs)[i] - (d * lds<LDS>(x)[i] - sum);
+                    const T ax = d * xr - sum;
+                    const T res = lds<LDS>(rhs)[i] - ax;
+                    lds<LDS>(xt)[i] = xr + omega * res / d;
+                }
+                __syncthreads();
+                T *t = x;
+                x = xt;
+                xt = t;
+            }
+            for (i64 i = tid; !colour && i < lv.n; i += nt) {
+                T ex = 0;
+                const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
+                const T d = shift + ex;
+                lds<LDS>(lv.res)[i] = lds<LDS>(rhs)[i] - (d * lds<LDS>(x)[i] - sum);
+            }
+            __syncthreads();
+            if (lv.n_term > 0) {
+                tail_terminal<LDS, false>(lv, lv.res, x);
+                __syncthreads();
+            }
+            if (lv.n_next > 0) {
+                T *crhs = L[k + 1].rhs;
+                for (i64 c = tid; c < lv.n_next; c += nt) {
+                    T ex = 0;
+                    lds<LDS>(crhs)[c] = tail_gather<LDS>(lv.R, static_cast<u32>(c), lv.res, ex);
+                }
+                __syncthreads();
+            }
+            if (x == lv.xt) flip |= 1u << k;
+        }
+        for (int k = levels - 1; k >= 0; --k) {
+            const level_desc<T> &lv = L[k];
+            const T *rhs = lv.rhs;
+            const bool f = (flip >> k) & 1u;
+            T *x = f ? lv.xt : lv.x, *xt = f ? lv.x : lv.xt;
+            if (lv.direct && k > 0) continue;
+            if (lv.n_next > 0) {
+                const T *cx = ((flip >> (k + 1)) & 1u) ? L[k + 1].xt : L[k + 1].x;
+                // red-black with post-smoothing: colour-1 rows are rewritten without being read
+                const bool red_only = lv.colour && lv.post > 0;
+                const i64 np = red_only ? lv.ncrows[0] : lv.n;
+                for (i64 q = tid; q < np; q += nt) {
+                    const i64 i = red_only ? lds<LDS>(lv.crows[0])[q] : q;
+                    const u32 c = lds<LDS>(lv.coarse_of)[i];
+                    if (c != 0xFFFFFFFFu) lds<LDS>(x)[i] += lds<LDS>(lv.weight)[i] * lds<LDS>(cx)[c];
+                }
+                __syncthreads();
+            }
+            for (int half = 0; lv.colour && half < 2 * lv.post; ++half) {
+                // reverse colour order: 1, 0, 1, 0, ...
+                const int c = 1 - (half & 1);
+                tail_gs_half<LDS>(lv.A, lv.crows[c], lv.ncrows[c], shift, rhs, x);
+                __syncthreads();
+            }
+            for (int p = 0; !lv.colour && p < lv.post; ++p) {
+                // the top level's last sweep may write out (HBM): dst is accessed as generic memory
+                T *dst = (k == 0 && p == lv.post - 1 && vout) ? vout : xt;
+                for (i64 i = tid; i < lv.n; i += nt) {
+                    T ex = 0;
+                    const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
+                    const T xr = lds<LDS>(x)[i];
+                    const T d = shift + ex;
+                    const T ax = d * xr - sum;
+                    const T res = lds<LDS>(rhs)[i] - ax;
+                    dst[i] = xr + omega * res / d;
+                }
+                __syncthreads();
+                if (dst == xt) {
+                    xt = x;
+                    x = dst;
+                } else {
+                    x = dst;
+                }
+            }
+            if (k > 0) {
+                flip = (flip & ~(1u << k)) | (x == lv.xt ? 1u << k : 0u);
+            } else {
+                top = x;
+            }
+        }
+        if (cycles > 1) {
+            // coarse cycles of the caller's W visit (no permutation: the tail starts below the top
+            // level): acc = the first result, acc += each later correction (as an axpy), and the
+            // last correction added on the way out; every later cycle starts from zero on the
+            // top level's residual rhs_in - A acc
+            const level_desc<T> &lv = L[0];
+            const bool last = cyc + 1 == cycles;
+            T *fin = out ? out : top_x;
+            for (i64 i = tid; i < lv.n; i += nt) {
+                const T t = lds<LDS>(top)[i];
+                if (cyc == 0) {
+                    lds<LDS>(acc)[i] = t;
+                } else if (!last) {
+                    lds<LDS>(acc)[i] += t;
+                } else {
+                    fin[i] = lds<LDS>(acc)[i] + t;
                 }
             }
             __syncthreads();
-        }
-        for (int p = 1; !colour && p < lv.pre; ++p) {
-            for (i64 i = tid; i < lv.n; i += nt) {
-                T ex = 0;
-                const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
-                const T xr = lds<LDS>(x)[i];
-                const T d = shift + ex;
-                const T ax = d * xr - sum;
-                const T res = lds<LDS>(rhs)[i] - ax;
-                lds<LDS>(xt)[i] = xr + omega * res / d;
-            }
-            __syncthreads();
-            T *t = x;
-            x = xt;
-            xt = t;
-        }
-        for (i64 i = tid; !colour && i < lv.n; i += nt) {
-            T ex = 0;
-            const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
-            const T d = shift + ex;
-            lds<LDS>(lv.res)[i] = lds<LDS>(rhs)[i] - (d * lds<LDS>(x)[i] - sum);
-        }
-        __syncthreads();
-        if (lv.n_term > 0) {
-            tail_terminal<LDS, false>(lv, lv.res, x);
-            __syncthreads();
-        }
-        if (lv.n_next > 0) {
-            T *crhs = L[k + 1].rhs;
-            for (i64 c = tid; c < lv.n_next; c += nt) {
-                T ex = 0;
-                lds<LDS>(crhs)[c] = tail_gather<LDS>(lv.R, static_cast<u32>(c), lv.res, ex);
-            }
-            __syncthreads();
-        }
-        if (x == lv.xt) flip |= 1u << k;
-    }
-    for (int k = levels - 1; k >= 0; --k) {
-        const level_desc<T> &lv = L[k];
-        const T *rhs = lv.rhs;
-        const bool f = (flip >> k) & 1u;
-        T *x = f ? lv.xt : lv.x, *xt = f ? lv.x : lv.xt;
-        if (lv.direct && k > 0) continue;
-        if (lv.n_next > 0) {
-            const T *cx = ((flip >> (k + 1)) & 1u) ? L[k + 1].xt : L[k + 1].x;
-            // red-black with post-smoothing: colour-1 rows are rewritten without being read
-            const bool red_only = lv.colour && lv.post > 0;
-            const i64 np = red_only ? lv.ncrows[0] : lv.n;
-            for (i64 q = tid; q < np; q += nt) {
-                const i64 i = red_only ? lds<LDS>(lv.crows[0])[q] : q;
-                const u32 c = lds<LDS>(lv.coarse_of)[i];
-                if (c != 0xFFFFFFFFu) lds<LDS>(x)[i] += lds<LDS>(lv.weight)[i] * lds<LDS>(cx)[c];
-            }
-            __syncthreads();
-        }
-        for (int half = 0; lv.colour && half < 2 * lv.post; ++half) {
-            // reverse colour order: 1, 0, 1, 0, ...
-            const int c = 1 - (half & 1);
-            tail_gs_half<LDS>(lv.A, lv.crows[c], lv.ncrows[c], shift, rhs, x);
-            __syncthreads();
-        }
-        for (int p = 0; !lv.colour && p < lv.post; ++p) {
-            // the top level's last sweep may write out (HBM): dst is accessed as generic memory
-            T *dst = (k == 0 && p == lv.post - 1 && vout) ? vout : xt;
-            for (i64 i = tid; i < lv.n; i += nt) {
-                T ex = 0;
-                const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
-                const T xr = lds<LDS>(x)[i];
-                const T d = shift + ex;
-                const T ax = d * xr - sum;
-                const T res = lds<LDS>(rhs)[i] - ax;
-                dst[i] = xr + omega * res / d;
-            }
-            __syncthreads();
-            if (dst == xt) {
-                xt = x;
-                x = dst;
-            } else {
-                x = dst;
+            if (!last) {
+                for (i64 i = tid; i < lv.n; i += nt) {
+                    T ex = 0;
+                    const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), acc, ex);
+                    const T d = shift + ex;
+                    lds<LDS>(lv.rhs)[i] = rhs_in[i] - (d * lds<LDS>(acc)[i] - sum);
+                }
+                __syncthreads();
             }
         }
-        if (k > 0) {
-            flip = (flip & ~(1u << k)) | (x == lv.xt ? 1u << k : 0u);
-        } else {
-            top = x;
-        }
-    }
-    if (cycles > 1) {
-        // coarse cycles of the caller's W visit (no permutation: the tail starts below the top
-        // level): acc = the first result, acc += each later correction (as an axpy), and the
-        // last correction added on the way out; every later cycle starts from zero on the
-        // top level's residual rhs_in - A acc
-        const level_desc<T> &lv = L[0];
-        const bool last = cyc + 1 == cycles;
-        T *fin = out ? out : top_x;
-        for (i64 i = tid; i < lv.n; i += nt) {
-            const T t = lds<LDS>(top)[i];
-            if (cyc == 0) {
-                lds<LDS>(acc)[i] = t;
-            } else if (!last) {
-                lds<LDS>(acc)[i] += t;
-            } else {
-                fin[i] = lds<LDS>(acc)[i] + t;
-            }
-        }
-        __syncthreads();
-        if (!last) {
-            for (i64 i = tid; i < lv.n; i += nt) {
-                T ex = 0;
-                const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), acc, ex);
-                const T d = shift + ex;
-                lds<LDS>(lv.rhs)[i] = rhs_in[i] - (d * lds<LDS>(acc)[i] - sum);
-            }
-            __syncthreads();
-        }
-    }
     }
     if (cycles > 1) return;
     // the top level's result in out (in the caller's row order), else in top_x

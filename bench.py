@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+EPS = float(np.finfo(np.float64).eps)
 METRIC = "RDME resolvent solves/s + p99 latency at ≤1e-10 residual, mixed vs FP64"
 
 
@@ -67,37 +68,70 @@ SMOOTHERS = {"jacobi": "JACOBI", "red_black": "RED_BLACK", "ilu0": "ILU0",
              "aggregate_block": "AGGREGATE_BLOCK"}
 
 
-def cpu_baseline(w, b, iterations, tol, restart):
-    """The FP64 oracle (1 core) on a bounded sample of the same solve with the reference's
-    method (weighted Jacobi V-cycle): a 1-iteration and a 3-iteration FGMRES run give the fixed
-    and per-iteration costs, scaled to the number of iterations that method needs (measured on
-    the GPU)."""
+def host_info():
+    """CPU model and core counts of this host (the CPU baseline's hardware)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cores": usable}
+
+
+def cpu_baseline(w, b, x_gpu, Q, tol, restart):
+    """One complete FP64 CPU oracle solve of the same system with the reference's method
+    (scatter apply_shifted, per-sector weighted-Jacobi V-cycle, restarted GMRES), on one core,
+    timed; the oracle's setup (state space, generator, hierarchy) is timed separately and
+    excluded. Its solution also gives the per-state parity of the GPU solution:
+    ||x_gpu - x_oracle||_1 <= (||r_gpu||_1 + ||r_oracle||_1 + rounding) / s, where both
+    residuals are evaluated in FP64 and rounding = 16 eps sum_j (s + 2 exit_j - leak_j)
+    (|x_gpu,j| + |x_oracle,j|) bounds their evaluation error."""
     from oracle.oracle import Problem, SectorMultigrid
+    shift = w.shifts[0]
     t = time.perf_counter()
     P = Problem(w.model, w.max_totals)
-    Mo = SectorMultigrid(P, w.shifts[0])
+    Mo = SectorMultigrid(P, shift)
     setup = time.perf_counter() - t
-    log(f"[cpu] oracle setup {setup:.1f}s")
-    times = {}
-    for k in (1, 3):
-        t = time.perf_counter()
-        P.gmres(w.shifts[0], b, mg=Mo, tol=tol, max_iterations=k, restart=restart)
-        times[k] = time.perf_counter() - t
-        log(f"[cpu] {k} FGMRES iteration(s): {times[k]:.2f}s")
-    per_iter = (times[3] - times[1]) / 2.0
-    fixed = max(times[1] - per_iter, 0.0)
-    solve_s = fixed + iterations * per_iter
-    return {
+    log(f"[cpu] oracle setup {setup:.1f}s; one complete FP64 solve on one core")
+    t = time.perf_counter()
+    x_ref, info = P.gmres(shift, b, mg=Mo, tol=tol, max_iterations=1000, restart=restart)
+    solve_s = time.perf_counter() - t
+    log(f"[cpu] oracle solve {solve_s:.1f}s, {info['iterations']} iterations, "
+        f"converged {info['converged']}")
+    r_gpu = b - P.apply_shifted(shift, x_gpu)
+    r_ref = b - P.apply_shifted(shift, x_ref)
+    absA = shift + 2 * Q.exit_rates() - Q.leak_rates()
+    rounding = 16 * EPS * float((absA * (np.abs(x_gpu) + np.abs(x_ref))).sum())
+    l1 = float(np.abs(x_gpu - x_ref).sum())
+    bound = float((np.abs(r_gpu).sum() + np.abs(r_ref).sum() + rounding) / shift)
+    hi = host_info()
+    out = {
         "value": 1.0 / solve_s,
         "unit": "solves/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"FP64 CPU oracle (scatter apply_shifted, per-sector Jacobi V-cycle, restarted "
-                   f"GMRES) on the same {P.n}-state solve: 1- and 3-iteration runs "
-                   f"({times[1]:.2f}s, {times[3]:.2f}s) scaled to {iterations} iterations; "
-                   f"setup {setup:.1f}s excluded"),
+        "sample": (f"one complete FP64 CPU oracle solve (scatter apply_shifted, per-sector "
+                   f"weighted-Jacobi V-cycle 2+2, GMRES({restart}) with MGS) of the same "
+                   f"{P.n}-state system to the same tolerance: {info['iterations']} iterations, "
+                   f"{solve_s:.1f}s on one core of {hi['cpu_model']}; setup {setup:.1f}s excluded"),
         "seconds_per_solve": solve_s,
+        "setup_s": setup,
+        "iterations": info["iterations"],
+        "converged": bool(info["converged"]),
+        "relative_residual": float(np.linalg.norm(r_ref) / np.linalg.norm(b)),
+        **hi,
     }
+    parity = {"l1_vs_oracle": l1, "l1_vs_oracle_bound": bound, "within_bound": l1 <= bound,
+              "oracle_relative_residual": out["relative_residual"]}
+    return out, parity, P
 
 
 def box_mg(mk, cpu=True, solves=None):
@@ -368,6 +402,67 @@ def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0, smooth
     }
 
 
+def c3(mk, workloads, steps, restart):
+    """2D two-species annihilation on 8 x 8 voxels (h = 1/8, D = 1, births 0.1 * volume each,
+    A + B -> 0 at n_A n_B / volume), caps {2, 2}: 4,601,025 states, s = 1, b = e_0. FGMRES(restart)
+    preconditioned by the red-black sector multigrid (V-cycle, 2 + 2 sweeps), mixed (FP32
+    multigrid) and plain FP64, each to 1e-10 relative residual. Checked in the run: true
+    residual, mass balance 1^T b = sum (s + leak) x, x >= -bound, and the mixed solution within
+    the L1 bound of the FP64 one."""
+    w = workloads.config_2d()
+    shift = w.shifts[0]
+    t = time.perf_counter()
+    box = mk.PopulationBox(w.n_voxels, w.max_totals)
+    Q = mk.Generator(box, w.model)
+    n = box.size()
+    mk.synchronize()
+    t_gen = time.perf_counter() - t
+    b = w.rhs(0, n)
+    tol = 1e-10 * float(np.linalg.norm(b))
+    A = mk.ShiftedGenerator(Q, shift)
+    b_dev, x_dev = mk.DeviceVector(n, b), mk.DeviceVector(n)
+    abs_A = shift + 2 * Q.exit_rates() - Q.leak_rates()
+    out = {"workload": f"{w.name}: 2D two-species annihilation, {n} states, s={shift}, b=e_0, "
+                       f"FGMRES({restart}) + red-black sector multigrid (V, 2+2), tol 1e-10 relative",
+           "states": n, "setup_s": {"generator": t_gen}}
+    xs = {}
+    for name, prec in (("mixed", mk.FP32), ("fp64", mk.FP64)):
+        t = time.perf_counter()
+        M = mk.SectorMultigrid(w.model.domain, box, shift,
+                               mk.MultigridOptions(precision=prec, smoother=mk.RED_BLACK))
+        mk.synchronize()
+        out["setup_s"][f"multigrid_{name}"] = time.perf_counter() - t
+        run_solves(mk, A, M, b_dev, x_dev, tol, 2, restart)
+        total, lat, iters, res, _ = run_solves(mk, A, M, b_dev, x_dev, tol, steps, restart)
+        x = x_dev.download()
+        r = b - Q.apply_shifted(shift, x)
+        bound = float((np.abs(r).sum() + 16 * EPS * (abs_A * np.abs(x)).sum()) / shift)
+        # 1^T b - sum (s + leak) x = 1^T r exactly, so it is at most ||r||_1 plus rounding
+        mass = abs(float(b.sum()) - float(((shift + Q.leak_rates()) * x).sum()))
+        mass_bound = float(np.abs(r).sum() + 64 * EPS * ((shift + Q.leak_rates()) * np.abs(x)).sum())
+        rel = float(np.linalg.norm(r) / np.linalg.norm(b))
+        if not (rel <= 1.0001e-10 and float(x.min()) >= -bound and mass <= mass_bound):
+            raise RuntimeError(f"C3 {name}: residual {rel:.3e}, min x {x.min():.3e}, mass {mass:.3e}")
+        xs[name] = (x, r)
+        rec = {"value": steps / total, "p50_ms": 1e3 * percentile(lat, 50),
+               "p99_ms": 1e3 * percentile(lat, 99), "iterations": iters[-1],
+               "relative_residual": rel, "l1_bound": bound, "mass_balance_error": mass,
+               "min_x": float(x.min())}
+        if name == "mixed":
+            out.update(rec)
+        else:
+            out["fp64"] = rec
+        del M
+    (xm, rm), (x64, r64) = xs["mixed"], xs["fp64"]
+    out["l1_mixed_vs_fp64"] = float(np.abs(xm - x64).sum())
+    out["l1_mixed_vs_fp64_bound"] = float((np.abs(rm).sum() + np.abs(r64).sum() + 16 * EPS *
+                                           (abs_A * (np.abs(xm) + np.abs(x64))).sum()) / shift)
+    if out["l1_mixed_vs_fp64"] > out["l1_mixed_vs_fp64_bound"]:
+        raise RuntimeError(f"C3: mixed and FP64 solutions differ by {out['l1_mixed_vs_fp64']:.3e}")
+    out["mixed_over_fp64"] = out["value"] / out["fp64"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -408,6 +503,8 @@ def main():
                     help="box workload: time only the first N solves of the transient (profiling)")
     ap.add_argument("--no-box", action="store_true",
                     help="skip the whole-box multigrid transient after the headline")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the 2D two-species annihilation solves after the headline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (each solve "
@@ -505,7 +602,10 @@ def main():
         f"residual {res.residual:.3e}")
     r = b - Q.apply_shifted(shift, x_mixed)
     rel_res = float(np.linalg.norm(r) / np.linalg.norm(b))
-    l1_bound = float(np.abs(r).sum() / shift)
+    # ||x - x*||_1 <= ||b - A x||_1 / s, with the FP64 evaluation error of b - A x (at most
+    # 16 eps || |A| |x| ||_1, where || |A| |x| ||_1 = sum_j (s + 2 exit_j - leak_j) |x_j|)
+    abs_A = shift + 2 * Q.exit_rates() - Q.leak_rates()
+    l1_bound = float((np.abs(r).sum() + 16 * EPS * (abs_A * np.abs(x_mixed)).sum()) / shift)
 
     fp64 = None
     if not args.no_fp64:
@@ -527,7 +627,8 @@ def main():
             "iterations": iters64[-1],
             "relative_residual": float(np.linalg.norm(r64) / np.linalg.norm(b)),
             "l1_mixed_vs_fp64": float(np.abs(x_mixed - x64).sum()),
-            "l1_bound": float((np.abs(r).sum() + np.abs(r64).sum()) / shift),
+            "l1_bound": float((np.abs(r).sum() + np.abs(r64).sum() +
+                               16 * EPS * (abs_A * (np.abs(x_mixed) + np.abs(x64))).sum()) / shift),
         }
         del M64
 
@@ -540,6 +641,9 @@ def main():
         Xk = np.array([x_mixed[begins[k]:begins[k + 1]].sum() for k in range(K)])
         X = workloads.lumped_sector_sums(shift, 0.05, 0.02, w.max_totals[0], Bk)
         lumped = float(np.abs(Xk - X).sum())
+        if lumped > l1_bound:
+            raise RuntimeError(f"sector sums off the closed form by {lumped:.3e}, above the L1 "
+                               f"bound {l1_bound:.3e} of the solution")
 
     d = prof[dominant]
     launches = d["launches"]
@@ -616,8 +720,8 @@ def main():
         out["box_mg"] = box_mg(mk, cpu=False)
     ref_iters = iters[-1]
     if args.smoother != "jacobi" and not args.no_jacobi:
-        # the reference's own method (weighted Jacobi V-cycle) on the GPU, for comparison and
-        # for the CPU baseline's iteration count
+        # the reference's own method (weighted Jacobi V-cycle) on the GPU, the method the CPU
+        # oracle solve below runs
         del M32
         Mj = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=mk.FP32))
         run_solves(mk, A, Mj, b_dev, x_dev, tol, 1, args.restart)
@@ -628,9 +732,20 @@ def main():
                          "iterations": ref_iters}
         log(f"[jacobi] {out['jacobi']['value']:.2f} solves/s, {ref_iters} iterations")
         del Mj
+    if args.workload == "large" and not args.no_c3:
+        log("[c3] 2D two-species annihilation, 4,601,025 states")
+        out["c3"] = c3(mk, workloads, max(10, args.steps), args.restart)
+        log(f"[c3] {out['c3']['value']:.1f} solves/s mixed, {out['c3']['fp64']['value']:.1f} FP64")
+    from markovkit_amd.calibrate import copy_bandwidth
+    out["stream_copy_gbs"] = copy_bandwidth()
+    out["roofline"]["frac_of_stream_copy"] = achieved / out["stream_copy_gbs"]
+    log(f"[copy] plain copy kernel {out['stream_copy_gbs']:.0f} GB/s read+write")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, b, ref_iters, tol, args.restart)
+        out["cpu_baseline"], out["parity_vs_oracle"], _ = cpu_baseline(
+            w, b, x_mixed, Q, tol, args.restart)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        if not out["parity_vs_oracle"]["within_bound"]:
+            raise RuntimeError(f"GPU solution off the oracle's: {out['parity_vs_oracle']}")
     print(json.dumps(out), flush=True)
 
 

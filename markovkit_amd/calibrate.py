@@ -11,6 +11,29 @@ import time
 from ._lib import check, lib
 
 
+def copy_bandwidth(nbytes: int = 1 << 30, repeats: int = 5) -> float:
+    """Best read+write bandwidth (GB/s, 2 * nbytes per copy) of the library's plain copy kernel
+    between two device buffers of nbytes each."""
+    L = lib()
+    a, b = C.c_void_p(), C.c_void_p()
+    check(L.mk_device_alloc(nbytes, C.byref(a)))
+    check(L.mk_device_alloc(nbytes, C.byref(b)))
+    try:
+        check(L.mk_device_zero(a, nbytes))
+        check(L.mk_device_copy(b, a, nbytes))  # warm-up
+        check(L.mk_synchronize())
+        best = float("inf")
+        for _ in range(repeats):
+            t = time.perf_counter()
+            check(L.mk_device_copy(b, a, nbytes))
+            check(L.mk_synchronize())
+            best = min(best, time.perf_counter() - t)
+    finally:
+        check(L.mk_device_free(a))
+        check(L.mk_device_free(b))
+    return 2 * nbytes / best / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--bytes", type=int, default=1 << 30)

@@ -932,9 +932,11 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
                 ws.hscal[1] = bj;
                 double *scal = dp<double>(ws.scal);
                 HIPCHK(hipMemcpyAsync(scal, ws.hscal, 2 * sizeof(double), hipMemcpyHostToDevice, s));
+                // the fused-tail switches choose the launches captured: both are part of the key
                 graph_key key{A.uid, M ? M->uid : 0, j,
-                              static_cast<i64>(sd > sd_next) + 2 * tail_switch().load(), n, m,
-                              bits(shift)};
+                              static_cast<i64>(sd > sd_next) + 2 * tail_switch().load() +
+                                  4 * tail_lds_switch().load(),
+                              n, m, bits(shift)};
                 ws.graphs.run(key, s, [&] {
                     operator_image(j, V + j * ld, scal);
                     // t = Q^T w and u_j^T w in hd[0 .. j]

@@ -1096,7 +1096,8 @@ void run_cycle(mk_box_mg_s &M) {
     const hipStream_t s = stream();
     auto body = [&] { cycle(M, 0, dp<double>(M.b0), dp<double>(M.x0)); };
     if (M.p.cycle_type != MK_CYCLE_K && box_graph_switch().load() != 0 && prof().mask == 0) {
-        graph_key key{M.uid, 1, 0, 0, 0, 0, 0};
+        // the small-level switch selects the kernels captured: it is part of the key
+        graph_key key{M.uid, 1, box_small_switch().load(), 0, 0, 0, 0};
         // graph_cache replays only when graphs are enabled globally; capture here directly
         auto it = M.graphs.exec.find(key);
         if (it == M.graphs.exec.end()) {

@@ -78,7 +78,9 @@ struct mk_server_s {
     std::condition_variable cv;
     std::deque<request *> queue;
     std::map<i64, std::unique_ptr<request>> requests;
-    std::vector<family> families;
+    // a deque keeps every family at its address while mk_server_add_family appends (workers hold
+    // no reference into it outside the lock either: run() gets the generator handle by value)
+    std::deque<family> families;
     i64 reserved = 0;
     i64 next_id = 1;
     bool stopping = false;
@@ -90,7 +92,7 @@ struct mk_server_s {
         return true;
     }
 
-    void run(request &r, worker_buffers &buf, mk_sector_mg mg) {
+    void run(request &r, worker_buffers &buf, mk_generator gen, mk_sector_mg mg) {
         const hipStream_t s = stream();
         const i64 n = static_cast<i64>(r.in.size());
         buf.ensure(n);
@@ -103,10 +105,10 @@ struct mk_server_s {
             if (st == MK_OK) {
                 mk_gmres_options o = r.gopt;
                 o.tolerance = r.tolerance;
-                st = mk_gmres_device(families[r.family].gen, r.shift, mg, da, db, &o, &r.solve);
+                st = mk_gmres_device(gen, r.shift, mg, da, db, &o, &r.solve);
             }
         } else {
-            st = mk_expv_device(r.gen, r.t, da, db, &r.eopt, &r.expv);
+            st = mk_expv_device(gen, r.t, da, db, &r.eopt, &r.expv);
         }
         if (st != MK_OK) throw std::runtime_error(mk_last_error());
         r.out.resize(n);
@@ -127,16 +129,19 @@ struct mk_server_s {
             queue.pop_front();
             reserved += r->footprint;
             mk_sector_mg mg = nullptr;
+            mk_generator gen = r->gen;
             if (r->kind == MK_REQUEST_RESOLVENT) {
-                mg = families[r->family].free.back();
-                families[r->family].free.pop_back();
+                family &f = families[r->family];
+                mg = f.free.back();
+                f.free.pop_back();
+                gen = f.gen;
             }
             r->started = clock_type::now();
             lock.unlock();
             mk_status status = MK_OK;
             std::string error;
             try {
-                run(*r, buf, mg);
+                run(*r, buf, gen, mg);
             } catch (const std::exception &e) {
                 status = MK_ERR_RUNTIME;
                 error = e.what();

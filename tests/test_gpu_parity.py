@@ -356,3 +356,25 @@ def test_plain_layout_generator():
     assert res.converged
     r = b - ((0.5 + exit_) * xs - C @ xs)
     assert np.linalg.norm(r) <= 1.0001e-10 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf], ids=["nan", "inf"])
+def test_non_finite_rhs_is_reported(bad):
+    """A NaN or infinite entry in b makes the initial residual non-finite: the solve stops at
+    once and reports converged = False (non-convergence is not an error in the reference);
+    the library stays usable for the next solve."""
+    w = config_small()
+    box = mk.PopulationBox(w.n_voxels, w.max_totals)
+    Q = mk.Generator(box, w.model)
+    shift = w.shifts[0]
+    M = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=mk.FP32))
+    b = w.rhs(0, box.size())
+    b_bad = b.copy()
+    b_bad[box.size() // 2] = bad
+    x = np.zeros(box.size())
+    res = mk.gmres(mk.ShiftedGenerator(Q, shift), M, b_bad, x, tolerance=1e-10, restart=50)
+    assert not res.converged and not np.isfinite(res.residual)
+    assert res.iterations == 0
+    x = np.zeros(box.size())
+    res = mk.gmres(mk.ShiftedGenerator(Q, shift), M, b, x, tolerance=1e-10, restart=50)
+    assert res.converged

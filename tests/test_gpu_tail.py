@@ -2,9 +2,10 @@
 its levels in HBM or staged in LDS, against the per-level kernels: same formulas and summation
 orders, so the cycle and the solves must be bitwise identical, in FP32 and FP64, with the
 Jacobi and red-black Gauss-Seidel smoothers, on 1D and 2D boxes, for a whole cycle in the tail
-(969 states) and for a tail below larger levels (58,905 states), with V-cycles and with
+(969 states) and for a tail below larger levels (58,905 states), with V-cycles, with
 W-cycles on the top levels over dense bottom blocks (the bench's hierarchy shape, whose
-bottom level is solved directly); and the LDS image after a shift change."""
+bottom level is solved directly) and with three coarse cycles per visit; and the LDS image
+after a shift change."""
 import numpy as np
 import pytest
 
@@ -15,7 +16,11 @@ pytestmark = pytest.mark.gpu
 
 
 SHAPES = {"V": {}, "W3-c100": dict(pre_sweeps=1, post_sweeps=2, coarse_cycles=2, cycle_levels=3,
-                                   coarsest_size=100)}
+                                   coarsest_size=100),
+          # three coarse cycles per visit: the tail's middle cycles (accumulate, then a fresh
+          # residual) run only with coarse_cycles >= 3
+          "C3-L3": dict(pre_sweeps=1, post_sweeps=2, coarse_cycles=3, cycle_levels=3,
+                        coarsest_size=100)}
 
 
 @pytest.mark.parametrize("shape", list(SHAPES))

@@ -912,7 +912,8 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
     };
     double beta = true_residual();
     res.residual = beta;
-    if (!(beta > o.tolerance)) {
+    // converged already, or a non-finite right-hand side or guess: nothing to iterate on
+    if (!(beta > o.tolerance) || !std::isfinite(beta)) {
         res.converged = std::isfinite(beta) ? 1 : 0;
         return res;
     }

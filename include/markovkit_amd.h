@@ -176,6 +176,21 @@ mk_status mk_gmres(mk_generator gen, double shift, mk_sector_mg mg, const double
 mk_status mk_gmres_device(mk_generator gen, double shift, mk_sector_mg mg, const double *d_b,
                           double *d_x, const mk_gmres_options *options,
                           mk_solver_result *result);
+/* mk_gmres_device without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under their own HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The Krylov basis, the preconditioned vectors
+ * and the work arrays live in one 64 MiB device array of the library (one per GPU, held until
+ * the process ends); the generator and the hierarchy are created beforehand, as for any solve.
+ * Limits: restart <= 64, and (restart + 1) basis vectors of 8 bytes plus restart preconditioned
+ * vectors (4 bytes each with an FP32 hierarchy, else 8) per row, rounded to 64 rows, and the
+ * reduction arrays must fit 64 MiB (about 58,000 rows at restart 50 with an FP32 hierarchy);
+ * larger requests fail with MK_ERR_INVALID_ARGUMENT. Calls on one GPU share its array, so calls
+ * that overlap on one GPU (from several host threads) race: serialise them. Launches are eager
+ * (no graph replay); the kernels and the arithmetic are mk_gmres_device's, so the results are
+ * the same bit for bit. */
+mk_status mk_gmres_device_noalloc(mk_generator gen, double shift, mk_sector_mg mg,
+                                  const double *d_b, double *d_x, const mk_gmres_options *options,
+                                  mk_solver_result *result);
 
 /* ---- whole-box multigrid (the paper driver, rdme/cpp/rdme_full_multigrid.cpp) ----------
  * The driver's Level hierarchy (:298-560) for a single species on a Cartesian grid of

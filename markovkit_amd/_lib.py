@@ -90,6 +90,7 @@ SIGNATURES = {
     "mk_sector_mg_apply_device": ([P, P, P], i32),
     "mk_gmres": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_gmres_device": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
+    "mk_gmres_device_noalloc": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_sector_mg_set_shift": ([P, f64], i32),
     "mk_expv": ([P, f64, P, P, C.POINTER(ExpvOptions), C.POINTER(ExpvResult)], i32),
     "mk_expv_device": ([P, f64, P, P, C.POINTER(ExpvOptions), C.POINTER(ExpvResult)], i32),

@@ -4,6 +4,8 @@
 #include "runtime.hpp"
 
 #include <map>
+#include <mutex>
+#include <string>
 
 using namespace mk::rt;
 
@@ -841,6 +843,57 @@ gmres_workspace &workspace() {
     return w;
 }
 
+// The device and host arrays one FGMRES call works in: the thread's cached workspace (pinned
+// host scalars, graph replay) or a view into the allocation-free scratch array (pageable host
+// scalars on the caller's stack, per-iteration scalars stored by kernel arguments, eager launches)
+struct krylov_space {
+    i64 ld = 0;                // basis stride (elements)
+    double *V = nullptr;       // m + 1 basis vectors
+    void *Z = nullptr;         // m preconditioned vectors
+    double *w = nullptr, *partial = nullptr, *hd = nullptr, *yd = nullptr, *rpart = nullptr,
+           *scal = nullptr;
+    double *host = nullptr;    // 4 (m + 2) doubles of host memory the device scalars are read into
+    double *hscal = nullptr;   // 2 pinned doubles the per-iteration scalars are copied from; null:
+                               // they are stored by a kernel's arguments
+    graph_cache *graphs = nullptr; // null: eager launches only
+};
+
+krylov_space cached_space(gmres_workspace &ws) {
+    krylov_space K;
+    K.ld = ws.ld;
+    K.V = dp<double>(ws.V);
+    K.Z = ws.Z.p;
+    K.w = dp<double>(ws.w);
+    K.partial = dp<double>(ws.partial);
+    K.hd = dp<double>(ws.hd);
+    K.yd = dp<double>(ws.yd);
+    K.rpart = dp<double>(ws.rpart);
+    K.scal = dp<double>(ws.scal);
+    K.host = ws.host;
+    K.hscal = ws.hscal;
+    K.graphs = &ws.graphs;
+    return K;
+}
+
+// up to kFusedBasis doubles passed by value into device memory (no host staging buffer)
+struct value_pack {
+    double v[gpu::kFusedBasis];
+};
+
+__global__ void k_store_values(value_pack p, int count, double *dst) {
+    const int i = static_cast<int>(threadIdx.x);
+    if (i < count) dst[i] = p.v[i];
+}
+
+void store_values(const double *vals, int count, double *dst, hipStream_t s) {
+    if (count <= 0) return;
+    if (count > gpu::kFusedBasis) throw std::logic_error("store_values: too many values");
+    value_pack p{};
+    std::copy(vals, vals + count, p.v);
+    hipLaunchKernelGGL(k_store_values, dim3(1), dim3(gpu::kFusedBasis), 0, s, p, count, dst);
+    check_launch("store_values");
+}
+
 double device_norm2(const double *partial, int nb, double *hd) {
     auto &P = prof();
     P.run(K_DOT, nb * 8.0, [&] { gpu::reduce_partials(partial, nb, 1, hd, stream()); });
@@ -857,29 +910,26 @@ double device_norm2(const double *partial, int nb, double *hd) {
 // by 1 / beta_j), and one reduction gives Q^T w and u_j^T w; the update kernel then finalises
 // q_j and forms the next candidate with its projections. Because w_j = A z_j is expanded as
 // Q (t + s_{j+1}) + beta_{j+1} q_{j+1}, column j of H is (t + s_{j+1}, beta_{j+1}) as in CGS2.
+i64 gmres_restart(const mk_gmres_options &o, i64 n) {
+    return std::max<i64>(1, std::min<i64>(o.restart, n));
+}
+
 template <typename ZT>
 mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const double *b,
-                        double *x, const mk_gmres_options &o) {
+                        double *x, const mk_gmres_options &o, const krylov_space &K) {
     const i64 n = A.n;
-    const i64 m = std::max<i64>(1, std::min<i64>(o.restart, n));
+    const i64 m = gmres_restart(o, n);
     require(m + 1 <= 512, "gmres: restart above 511 is not supported on the device");
     require(o.orthogonalization == MK_DCGS2 || o.orthogonalization == MK_CGS2,
             "gmres: unknown orthogonalization");
     const bool delayed = M && o.orthogonalization == MK_DCGS2 && m <= gpu::kFusedBasis;
-    auto &ws = workspace();
-    ws.ensure(n, m, M ? sizeof(ZT) : 0);
-    if (ws.rpart_n < A.A.view.n_blocks + 1) {
-        ws.rpart = dbuf();
-        ws.rpart_n = A.A.view.n_blocks + 1;
-        ws.rpart = dbuf(static_cast<std::size_t>(ws.rpart_n) * sizeof(double));
-    }
     auto &P = prof();
     const hipStream_t s = stream();
     const int nb = gpu::reduction_blocks(n);
-    double *V = dp<double>(ws.V), *w = dp<double>(ws.w);
-    double *partial = dp<double>(ws.partial), *hd = dp<double>(ws.hd);
-    ZT *Z = M ? dp<ZT>(ws.Z) : nullptr;
-    const i64 ld = ws.ld;
+    double *V = K.V, *w = K.w;
+    double *partial = K.partial, *hd = K.hd;
+    ZT *Z = M ? static_cast<ZT *>(K.Z) : nullptr;
+    const i64 ld = K.ld;
     const double spmv_bytes = op_bytes(A.A, 8, 8, 1);
     const i64 hs = m + 2; // stride of the device scalar slots in hd
 
@@ -888,9 +938,9 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
     const int rb = static_cast<int>(std::max<i64>(A.A.view.n_blocks, 1));
     auto true_residual = [&]() {
         P.run(K_SPMV, spmv_bytes + n * 8.0, [&] {
-            gpu::shifted_apply<double>(A.A.view, shift, x, b, V, dp<double>(ws.rpart), s);
+            gpu::shifted_apply<double>(A.A.view, shift, x, b, V, K.rpart, s);
         });
-        return std::sqrt(device_norm2(dp<double>(ws.rpart), rb, hd));
+        return std::sqrt(device_norm2(K.rpart, rb, hd));
     };
     // w = A M (scale v), scale read from device memory (1 when null)
     auto operator_image = [&](i64 j, const double *v, const double *scale) {
@@ -907,8 +957,26 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
         }
     };
     auto fetch = [&](std::size_t count) {
-        HIPCHK(hipMemcpyAsync(ws.host, hd, count * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(K.host, hd, count * sizeof(double), hipMemcpyDeviceToHost, s));
         drain();
+    };
+    // device copies of host values the next kernels read: from the pinned staging array, or
+    // through a kernel's arguments
+    auto put = [&](const double *vals, int count, double *dst, double *pinned) {
+        if (pinned) {
+            std::copy(vals, vals + count, pinned);
+            HIPCHK(hipMemcpyAsync(dst, pinned, count * sizeof(double), hipMemcpyHostToDevice, s));
+        } else {
+            store_values(vals, count, dst, s);
+        }
+    };
+    // the body of one Arnoldi step: replayed from a captured graph, or launched eagerly
+    auto step = [&](const graph_key &key, auto &&body) {
+        if (K.graphs) {
+            K.graphs->run(key, s, body, n);
+        } else {
+            body();
+        }
     };
     double beta = true_residual();
     res.residual = beta;
@@ -929,16 +997,15 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
             for (; j < m && res.iterations < o.max_iterations; ++j) {
                 ++res.iterations;
                 const int cnt = static_cast<int>(j);
-                ws.hscal[0] = 1.0 / bj;
-                ws.hscal[1] = bj;
-                double *scal = dp<double>(ws.scal);
-                HIPCHK(hipMemcpyAsync(scal, ws.hscal, 2 * sizeof(double), hipMemcpyHostToDevice, s));
+                const double sv[2] = {1.0 / bj, bj};
+                double *scal = K.scal;
+                put(sv, 2, scal, K.hscal);
                 // the fused-tail switches choose the launches captured: both are part of the key
                 graph_key key{A.uid, M ? M->uid : 0, j,
                               static_cast<i64>(sd > sd_next) + 2 * tail_switch().load() +
                                   4 * tail_lds_switch().load(),
                               n, m, bits(shift)};
-                ws.graphs.run(key, s, [&] {
+                step(key, [&] {
                     operator_image(j, V + j * ld, scal);
                     // t = Q^T w and u_j^T w in hd[0 .. j]
                     P.run(K_DOT, double(cnt + 2) * n * 8.0, [&] {
@@ -950,10 +1017,10 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
                         gpu::dcgs_update(V, ld, cnt, sd, hd, scal + 1, w, n, partial, nb, hd + m + 1, s);
                         gpu::reduce_partials(partial, nb, cnt + 2, sd_next, s);
                     });
-                }, n);
+                });
                 fetch(3 * hs);
-                const double *t = ws.host, *sn = ws.host + (sd_next - hd);
-                const double eta = ws.host[m + 1];
+                const double *t = K.host, *sn = K.host + (sd_next - hd);
+                const double eta = K.host[m + 1];
                 for (i64 k = 0; k < j; ++k) col[k] = t[k] + sn[k];
                 col[j] = eta + sn[j];
                 double ss = 0.0;
@@ -1033,11 +1100,11 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
                     P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, w, bfd + 2, vnext, s); });
                 }
                 fetch(3 * hs);
-                const double *h1 = ws.host, *h2 = ws.host + hs;
+                const double *h1 = K.host, *h2 = K.host + hs;
                 double hn;
                 if (fused) {
-                    hn = ws.host[2 * hs];
-                    if (ws.host[2 * hs + 1] != 0.0) {
+                    hn = K.host[2 * hs];
+                    if (K.host[2 * hs + 1] != 0.0) {
                         // the second pass removed too much for the Pythagorean norm: v_{j+1}
                         // holds the unnormalised vector; normalise it explicitly
                         P.run(K_DOT, n * 8.0, [&] {
@@ -1047,10 +1114,10 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
                         P.run(K_OTHER, n * 16.0,
                               [&] { gpu::scale_by_norm(n, vnext, bfd + 2, vnext, s); });
                         fetch(3 * hs);
-                        hn = std::sqrt(ws.host[2 * hs + 2]);
+                        hn = std::sqrt(K.host[2 * hs + 2]);
                     }
                 } else {
-                    hn = std::sqrt(ws.host[2 * hs + 2]);
+                    hn = std::sqrt(K.host[2 * hs + 2]);
                 }
                 for (i64 k = 0; k <= j; ++k) col[k] = h1[k] + h2[k];
                 const double est = ls.add_column(j, col.data(), hn);
@@ -1061,14 +1128,15 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
             }
         }
         ls.solve(j);
-        double *ypin = ws.host + 3 * hs;
-        std::copy(ls.y.begin(), ls.y.begin() + j, ypin);
-        HIPCHK(hipMemcpyAsync(ws.yd.p, ypin, j * sizeof(double), hipMemcpyHostToDevice, s));
+        for (i64 c = 0; c < j; c += gpu::kFusedBasis) {
+            const int cnt = static_cast<int>(std::min<i64>(gpu::kFusedBasis, j - c));
+            put(ls.y.data() + c, cnt, K.yd + c, K.hscal ? K.host + 3 * hs + c : nullptr);
+        }
         P.run(K_OTHER, double(j) * n * (M ? sizeof(ZT) : 8) + n * 16.0, [&] {
             if (M) {
-                gpu::combine<ZT>(x, Z, ld, static_cast<int>(j), dp<double>(ws.yd), n, s);
+                gpu::combine<ZT>(x, Z, ld, static_cast<int>(j), K.yd, n, s);
             } else {
-                gpu::combine<double>(x, V, ld, static_cast<int>(j), dp<double>(ws.yd), n, s);
+                gpu::combine<double>(x, V, ld, static_cast<int>(j), K.yd, n, s);
             }
         });
         beta = true_residual();
@@ -1082,26 +1150,136 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
     return res;
 }
 
+precond sector_precond(mk_sector_mg_s *mg) {
+    return precond{mg->uid, mg->rows, mg->shift, mg->precision,
+                   [mg](const double *v, void *z, const double *scale) { mg_apply_device(*mg, v, z, scale); }};
+}
+
 mk_solver_result gmres_device(mk_generator_s &A, double shift, mk_sector_mg_s *mg, const double *b,
                               double *x, const mk_gmres_options &o) {
     if (!mg) return mk::rt::gmres_device(A, shift, static_cast<const precond *>(nullptr), b, x, o);
-    precond M{mg->uid, mg->rows, mg->shift, mg->precision,
-              [mg](const double *v, void *z, const double *scale) { mg_apply_device(*mg, v, z, scale); }};
+    const precond M = sector_precond(mg);
     return mk::rt::gmres_device(A, shift, &M, b, x, o);
 }
 
 } // namespace
 
+namespace {
+// checks shared by both workspaces; the size of a preconditioned vector (0 without one)
+std::size_t check_gmres(const mk_generator_s &A, double shift, const precond *M,
+                        const mk_gmres_options &o) {
+    require(gmres_restart(o, A.n) + 1 <= 512, "gmres: restart above 511 is not supported on the device");
+    require(o.orthogonalization == MK_DCGS2 || o.orthogonalization == MK_CGS2,
+            "gmres: unknown orthogonalization");
+    if (!M) return 0;
+    require(M->rows == A.n, "gmres: the preconditioner does not match the operator");
+    require(M->shift == shift, "gmres: the preconditioner was built for another shift");
+    return M->precision == MK_FP32 ? sizeof(float) : sizeof(double);
+}
+
+mk_solver_result run_fgmres(mk_generator_s &A, double shift, const precond *M, const double *b,
+                            double *x, const mk_gmres_options &o, const krylov_space &K) {
+    if (M && M->precision == MK_FP32) return fgmres<float>(A, shift, M, b, x, o, K);
+    return fgmres<double>(A, shift, M, b, x, o, K);
+}
+
+// ---- the allocation-free workspace: one file-scope device array per GPU ----------------------
+// Calls on one GPU share its array (the last call's contents stay in it): two calls that overlap
+// on one GPU, from two host threads, race. Every call clears the small scalar arrays it uses.
+constexpr std::size_t kScratchBytes = std::size_t{64} << 20;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// the device address of the current GPU's scratch array, looked up once per device (with that
+// device current) and kept; the host-side name `scratch` is only a handle for the lookup
+unsigned char *scratch_address() {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices) throw device_error("scratch: device index out of range");
+    static std::mutex mu;
+    static void *addr[kScratchDevices] = {};
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw device_error("scratch: hipGetSymbolAddress returned no address");
+        addr[dev] = p;
+    }
+    return static_cast<unsigned char *>(addr[dev]);
+}
+
+// carves 256-byte aligned arrays out of a base address (null base: sizes only)
+struct carver {
+    unsigned char *base = nullptr;
+    std::size_t off = 0;
+    template <typename T>
+    T *take(std::size_t count) {
+        off = (off + 255) / 256 * 256;
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+krylov_space carve_space(unsigned char *base, const mk_generator_s &A, i64 m, std::size_t zs,
+                         std::size_t &bytes) {
+    const i64 n = A.n;
+    carver c{base};
+    krylov_space K;
+    K.ld = (n + 63) / 64 * 64;
+    K.V = c.take<double>(static_cast<std::size_t>(K.ld) * (m + 1));
+    K.Z = zs ? static_cast<void *>(c.take<unsigned char>(static_cast<std::size_t>(K.ld) * m * zs)) : nullptr;
+    K.w = c.take<double>(n);
+    K.partial = c.take<double>(static_cast<std::size_t>(gpu::reduction_blocks(n)) * (m + 2));
+    K.hd = c.take<double>(3 * (m + 2));
+    K.yd = c.take<double>(m + 1);
+    K.rpart = c.take<double>(A.A.view.n_blocks + 1);
+    K.scal = c.take<double>(2);
+    bytes = c.off;
+    return K;
+}
+
+mk_solver_result gmres_device_noalloc(mk_generator_s &A, double shift, const precond *M,
+                                      const double *b, double *x, const mk_gmres_options &o) {
+    const std::size_t zs = check_gmres(A, shift, M, o);
+    const i64 m = gmres_restart(o, A.n);
+    require(m <= gpu::kFusedBasis, "gmres (no allocation): restart must be at most 64");
+    std::size_t bytes = 0;
+    carve_space(nullptr, A, m, zs, bytes);
+    if (bytes > kScratchBytes)
+        throw std::invalid_argument("gmres (no allocation): the Krylov workspace needs " +
+                                    std::to_string(bytes) + " bytes, above the " +
+                                    std::to_string(kScratchBytes) + "-byte scratch array");
+    krylov_space K = carve_space(scratch_address(), A, m, zs, bytes);
+    // the scalar arrays start at zero on every call (the vectors are written before they are read)
+    const hipStream_t s = stream();
+    const i64 nb = gpu::reduction_blocks(A.n);
+    gpu::fill<double>(nb * (m + 2), 0.0, K.partial, s);
+    gpu::fill<double>(3 * (m + 2), 0.0, K.hd, s);
+    gpu::fill<double>(m + 1, 0.0, K.yd, s);
+    gpu::fill<double>(A.A.view.n_blocks + 1, 0.0, K.rpart, s);
+    gpu::fill<double>(2, 0.0, K.scal, s);
+    check_launch("scratch clear");
+    double host[4 * (gpu::kFusedBasis + 2)] = {};
+    K.host = host;
+    K.hscal = nullptr;  // per-iteration scalars go by kernel arguments
+    K.graphs = nullptr; // eager launches
+    return run_fgmres(A, shift, M, b, x, o, K);
+}
+} // namespace
+
 namespace mk::rt {
 mk_solver_result gmres_device(mk_generator_s &A, double shift, const precond *M, const double *b,
                               double *x, const mk_gmres_options &o) {
-    if (M) {
-        require(M->rows == A.n, "gmres: the preconditioner does not match the operator");
-        require(M->shift == shift, "gmres: the preconditioner was built for another shift");
-        if (M->precision == MK_FP32) return fgmres<float>(A, shift, M, b, x, o);
-        return fgmres<double>(A, shift, M, b, x, o);
+    const std::size_t zs = check_gmres(A, shift, M, o);
+    auto &ws = workspace();
+    ws.ensure(A.n, gmres_restart(o, A.n), zs);
+    if (ws.rpart_n < A.A.view.n_blocks + 1) {
+        ws.rpart = dbuf();
+        ws.rpart_n = A.A.view.n_blocks + 1;
+        ws.rpart = dbuf(static_cast<std::size_t>(ws.rpart_n) * sizeof(double));
     }
-    return fgmres<double>(A, shift, nullptr, b, x, o);
+    return run_fgmres(A, shift, M, b, x, o, cached_space(ws));
 }
 } // namespace mk::rt
 
@@ -1498,6 +1676,22 @@ mk_status mk_gmres_device(mk_generator gen, double shift, mk_sector_mg mg, const
         require(gen && options && result, "gmres: bad arguments");
         require(options->max_iterations >= 0 && options->restart >= 1, "gmres: bad options");
         *result = gmres_device(*gen, shift, mg, d_b, d_x, *options);
+        drain();
+    });
+}
+
+mk_status mk_gmres_device_noalloc(mk_generator gen, double shift, mk_sector_mg mg,
+                                  const double *d_b, double *d_x, const mk_gmres_options *options,
+                                  mk_solver_result *result) {
+    return guarded([&] {
+        require(gen && d_b && d_x && options && result, "gmres: bad arguments");
+        require(options->max_iterations >= 0 && options->restart >= 1, "gmres: bad options");
+        if (mg) {
+            const precond M = sector_precond(mg);
+            *result = gmres_device_noalloc(*gen, shift, &M, d_b, d_x, *options);
+        } else {
+            *result = gmres_device_noalloc(*gen, shift, nullptr, d_b, d_x, *options);
+        }
         drain();
     });
 }

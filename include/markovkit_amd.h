@@ -112,6 +112,13 @@ mk_status mk_box_state(mk_box box, int64_t i, int32_t *state);
 /* find(x): population_space.hpp:194-202; *index = -1 when x is outside the box */
 mk_status mk_box_find(mk_box box, const int32_t *state, int64_t *index);
 
+/* population_sector(n_voxels, totals): population_space.hpp:103-127, every state with totals[s]
+ * molecules of species s, in rank order. The handle is a space with one sector: every mk_box_*
+ * query, mk_generator_create_rdme and mk_sector_mg_create take it (a sector multigrid over it is
+ * the reference's population_multigrid). */
+mk_status mk_sector_space_create(int64_t n_voxels, int32_t n_species, const int32_t *totals,
+                                 mk_box *out);
+
 /* ---- voxel_domain (rdme/reaction_diffusion.hpp:23-56) over a grid<D> ---------------- */
 /* voxel_domain(grid<D>(dims, spacing), [volumes,] diffusivities): reaction_diffusion.hpp:29-47.
  * volumes may be NULL (uniform cell volume). */
@@ -143,6 +150,12 @@ mk_status mk_generator_destroy(mk_generator gen);
 mk_status mk_generator_size(mk_generator gen, int64_t *rows, int64_t *n_jumps);
 /* exit_rates(), leak_rates(): generator.hpp:55-59 */
 mk_status mk_generator_rates(mk_generator gen, double *exit, double *leak);
+/* The stored jumps, for generator::jumps(j) and matrix() (generator.hpp:51-53, :84-89): row i
+ * (a destination) lists (column j, rate of j -> i), columns ascending, with jumps of one source
+ * to one destination merged (summed). *entries receives the count; ptr[rows + 1], col[entries]
+ * and rate[entries] may be NULL (size query). Decoded on the host from the device layout. */
+mk_status mk_generator_entries(mk_generator gen, int64_t *entries, int64_t *ptr, int64_t *col,
+                               double *rate);
 /* apply(x, y): y = Q x, generator.hpp:62-70 */
 mk_status mk_generator_apply(mk_generator gen, const double *x, double *y);
 /* shifted_generator::apply -> apply_shifted(shift, x, y): y = shift x - Q x,
@@ -164,6 +177,22 @@ mk_status mk_sector_mg_destroy(mk_sector_mg mg);
 /* max_levels(), n_sectors(): multigrid.hpp:388-395 */
 mk_status mk_sector_mg_info(mk_sector_mg mg, int64_t *rows, int64_t *n_sectors,
                             int64_t *max_levels);
+/* population_multigrid (multigrid.hpp:139-236) = a sector multigrid over a space made by
+ * mk_sector_space_create (one sector). For such a handle:
+ * level l's states, jumps (the diffusion generator's, before merging) and terminal flag
+ * (n_levels, grid_complexity, operator_complexity, multigrid.hpp:188-193); */
+mk_status mk_sector_mg_level(mk_sector_mg mg, int32_t level, int64_t *rows, int64_t *jumps,
+                             int32_t *terminal);
+/* the transfer from level l to level l + 1 (population_transfer, multigrid.hpp:47-118): for each
+ * level-l state its coarse state and its multinomial prolongation weight (FP64, rank order) */
+mk_status mk_sector_mg_transfer(mk_sector_mg mg, int32_t level, int64_t *coarse_of, double *weight);
+/* relax(rhs, x, sweeps, reverse) (multigrid.hpp:231-236): `sweeps` sweeps of the hierarchy's
+ * smoother on its finest level from the guess x, on the device in the hierarchy's working
+ * precision (any sector multigrid handle; the finest level of every sector). */
+mk_status mk_sector_mg_relax(mk_sector_mg mg, const double *rhs, double *x, int32_t sweeps,
+                             int32_t reverse);
+mk_status mk_sector_mg_relax_device(mk_sector_mg mg, const double *d_rhs, double *d_x,
+                                    int32_t sweeps, int32_t reverse);
 /* apply(b, x): one cycle per sector from a zero guess, multigrid.hpp:397-403 */
 mk_status mk_sector_mg_apply(mk_sector_mg mg, const double *b, double *x);
 mk_status mk_sector_mg_apply_device(mk_sector_mg mg, const double *d_b, double *d_x);

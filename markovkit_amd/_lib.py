@@ -72,6 +72,7 @@ SIGNATURES = {
     "mk_box_sector": ([P, i64, pi64, pi64, P], i32),
     "mk_box_state": ([P, i64, P], i32),
     "mk_box_find": ([P, P, pi64], i32),
+    "mk_sector_space_create": ([i64, i32, P, C.POINTER(P)], i32),
     "mk_domain_create_grid": ([i32, P, P, P, i32, P, C.POINTER(P)], i32),
     "mk_domain_create_graph": ([i64, P, P, P, P, i32, P, C.POINTER(P)], i32),
     "mk_domain_destroy": ([P], i32),
@@ -80,6 +81,7 @@ SIGNATURES = {
     "mk_generator_destroy": ([P], i32),
     "mk_generator_size": ([P, pi64, pi64], i32),
     "mk_generator_rates": ([P, P, P], i32),
+    "mk_generator_entries": ([P, pi64, P, P, P], i32),
     "mk_generator_apply": ([P, P, P], i32),
     "mk_shifted_apply": ([P, f64, P, P], i32),
     "mk_shifted_apply_device": ([P, f64, P, P], i32),
@@ -87,6 +89,10 @@ SIGNATURES = {
     "mk_sector_mg_destroy": ([P], i32),
     "mk_sector_mg_info": ([P, pi64, pi64, pi64], i32),
     "mk_sector_mg_apply": ([P, P, P], i32),
+    "mk_sector_mg_level": ([P, i32, pi64, pi64, C.POINTER(i32)], i32),
+    "mk_sector_mg_transfer": ([P, i32, P, P], i32),
+    "mk_sector_mg_relax": ([P, P, P, i32, i32], i32),
+    "mk_sector_mg_relax_device": ([P, P, P, i32, i32], i32),
     "mk_sector_mg_apply_device": ([P, P, P], i32),
     "mk_gmres": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_gmres_device": ([P, f64, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),

@@ -124,6 +124,19 @@ class PopulationBox:
         return k
 
 
+class PopulationSector(PopulationBox):
+    """population_sector(n_voxels, totals): the states with exactly totals[s] molecules of each
+    species, in rank order (a space of one sector; every box query applies)."""
+
+    def __init__(self, n_voxels: int, totals: Sequence[int]):
+        self.max_totals = np.ascontiguousarray(totals, np.int32)
+        self.totals = [int(t) for t in totals]
+        self._h = C.c_void_p()
+        check(lib().mk_sector_space_create(int(n_voxels), len(self.max_totals), _p(self.max_totals),
+                                           C.byref(self._h)))
+        self.voxels = int(n_voxels)
+
+
 class _Domain:
     """voxel_domain handle on the library side."""
 
@@ -237,6 +250,17 @@ class Generator:
         l = np.zeros(self._rows)
         check(lib().mk_generator_rates(self._h, _p(e), _p(l)))
         return e, l
+
+    def entries(self):
+        """(ptr, col, rate): the stored jumps by destination (row i: sources j and rates of
+        j -> i, sources ascending, jumps of one source to one destination summed)."""
+        e = C.c_int64()
+        check(lib().mk_generator_entries(self._h, C.byref(e), None, None, None))
+        ptr = np.zeros(self.rows() + 1, np.int64)
+        col = np.zeros(e.value, np.int64)
+        rate = np.zeros(e.value)
+        check(lib().mk_generator_entries(self._h, C.byref(e), _p(ptr), _p(col), _p(rate)))
+        return ptr, col, rate
 
     def exit_rates(self) -> np.ndarray:
         return self._rates()[0]
@@ -352,6 +376,28 @@ class SectorMultigrid:
 
     def apply_device(self, b: DeviceVector, x: DeviceVector):
         check(lib().mk_sector_mg_apply_device(self._h, b.ptr, x.ptr))
+
+    def relax(self, rhs, x, sweeps: int = 1, reverse: bool = False) -> np.ndarray:
+        """`sweeps` smoother sweeps on the finest level from the guess x (returned as a new
+        array), on the device in the hierarchy's precision."""
+        ra = np.ascontiguousarray(rhs, np.float64)
+        xa = np.array(x, np.float64, copy=True)
+        check(lib().mk_sector_mg_relax(self._h, _p(ra), _p(xa), int(sweeps), int(bool(reverse))))
+        return xa
+
+    def level(self, level: int):
+        """(rows, jumps, terminal) of a level of a hierarchy over one population_sector."""
+        r, j, t = C.c_int64(), C.c_int64(), C.c_int32()
+        check(lib().mk_sector_mg_level(self._h, int(level), C.byref(r), C.byref(j), C.byref(t)))
+        return r.value, j.value, bool(t.value)
+
+    def transfer(self, level: int = 0):
+        """(coarse_of, weight) from level `level` to the next of a hierarchy over one sector."""
+        n = self.level(level)[0]
+        co = np.zeros(n, np.int64)
+        w = np.zeros(n)
+        check(lib().mk_sector_mg_transfer(self._h, int(level), _p(co), _p(w)))
+        return co, w
 
 
 @dataclass

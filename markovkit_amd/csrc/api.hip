@@ -1307,6 +1307,103 @@ void reshift_levels(std::vector<level_dev<T>> &L, double shift) {
 }
 } // namespace
 
+namespace {
+// `sweeps` smoother sweeps on the finest level from the caller's x (population_multigrid::relax,
+// multigrid.hpp:231-236), in the hierarchy's working precision; rhs and x in HBM, caller order
+template <typename T>
+void relax_top(mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, const double *rhs64, double *x64,
+               int sweeps, bool reverse) {
+    auto &lv = L[0];
+    require(!lv.direct && lv.A.n > 0, "relax: the level is solved directly, it has no smoother");
+    const hipStream_t s = stream();
+    const i64 n = lv.n;
+    const double shift = mg.shift, omega = mg.opt.jacobi_weight;
+    T *rhs = dp<T>(lv.rhs), *x = dp<T>(lv.x), *tmp = dp<T>(lv.xt), *res = dp<T>(lv.res);
+    const u32 *perm = dp<u32>(lv.perm);
+    if (perm) {
+        gpu::permute_in<T, double>(n, perm, rhs64, rhs, s);
+        gpu::permute_in<T, double>(n, perm, x64, x, s);
+    } else {
+        gpu::convert<double, T>(n, rhs64, rhs, s);
+        gpu::convert<double, T>(n, x64, x, s);
+    }
+    check_launch("relax: input");
+    if (lv.rb) {
+        for (int k = 0; k < sweeps; ++k)
+            for (int half = 0; half < 2; ++half) {
+                const int c = reverse ? 1 - half : half;
+                gpu::gs_half<T>(lv.Ac[c].view, shift, rhs, x, nullptr, s);
+            }
+    } else if (lv.ilu) {
+        for (int k = 0; k < sweeps; ++k) {
+            gpu::level_residual<T>(lv.A.view, shift, rhs, x, res, s);
+            for (const auto &g : lv.fw_launch)
+                gpu::ilu_forward<T>(lv.A.view, dp<u32>(lv.fw_rows), dp<u32>(lv.fw_ptr), g.g0, g.g1,
+                                    g.wide_rows, dp<T>(lv.pinv), res, tmp, s);
+            for (const auto &g : lv.bw_launch)
+                gpu::ilu_backward<T>(lv.A.view, dp<u32>(lv.bw_rows), dp<u32>(lv.bw_ptr), g.g0, g.g1,
+                                     g.wide_rows, dp<T>(lv.pinv), tmp, x, s);
+        }
+    } else if (lv.n_agg > 0) {
+        block_smooth<T>(mg, lv, 0, rhs, x, tmp, sweeps, reverse, false);
+    } else {
+        for (int k = 0; k < sweeps; ++k) {
+            gpu::jacobi_sweep<T>(lv.A.view, shift, omega, rhs, x, tmp, s);
+            std::swap(x, tmp);
+        }
+    }
+    check_launch("relax: sweeps");
+    const T *nat = x;
+    if (perm) {
+        T *other = x == res ? tmp : res;
+        gpu::permute_out<T>(n, perm, x, other, s);
+        nat = other;
+    }
+    gpu::convert<T, double>(n, nat, x64, s);
+    check_launch("relax: output");
+}
+
+// the stored entries of a generator in gather form, decoded on the host from its device arrays:
+// row i lists (column j, rate of j -> i), columns ascending
+void download_gather(const mk_generator_s &G, std::vector<int64_t> &ptr, std::vector<int64_t> &col,
+                     std::vector<double> &val) {
+    const auto &A = G.A;
+    const i64 n = A.n;
+    ptr.assign(n + 1, 0);
+    col.clear();
+    val.clear();
+    auto fetch = [](const dbuf &b, auto &v) {
+        v.resize(b.bytes / sizeof(v[0]));
+        if (b.bytes) HIPCHK(hipMemcpy(v.data(), b.p, v.size() * sizeof(v[0]), hipMemcpyDeviceToHost));
+    };
+    if (A.coded) {
+        std::vector<u32> soff, ent;
+        std::vector<std::uint8_t> rlen;
+        std::vector<double> vdict;
+        fetch(A.soff, soff);
+        fetch(A.ent, ent);
+        fetch(A.rlen, rlen);
+        fetch(A.vdict, vdict);
+        for (i64 i = 0; i < n; ++i) {
+            const u64 base = soff[i >> 6] + static_cast<u64>(i & 63);
+            for (u32 k = 0; k < rlen[i]; ++k) {
+                const u32 e = ent[base + static_cast<u64>(k) * gpu::kSlice];
+                col.push_back(e & gpu::kColMask);
+                val.push_back(vdict[e >> gpu::kColBits]);
+            }
+            ptr[i + 1] = static_cast<int64_t>(col.size());
+        }
+    } else {
+        std::vector<u32> p, c;
+        fetch(A.ptr, p);
+        fetch(A.col, c);
+        fetch(A.val, val);
+        for (i64 i = 0; i <= n; ++i) ptr[i] = p[i];
+        col.assign(c.begin(), c.end());
+    }
+}
+} // namespace
+
 namespace mk::rt {
 void finish_generator(mk_generator_s &G, mk::generator_csr &&g) {
     G.n = g.n;
@@ -1418,6 +1515,16 @@ mk_status mk_box_find(mk_box box, const int32_t *state, int64_t *index) {
     return guarded([&] { *index = box->space.find(state); });
 }
 
+mk_status mk_sector_space_create(int64_t n_voxels, int32_t n_species, const int32_t *totals,
+                                mk_box *out) {
+    return guarded([&] {
+        require(out && totals && n_species > 0, "population_sector: bad arguments");
+        auto b = std::make_unique<mk_box_s>();
+        b->space = mk::population_space::sector(n_voxels, std::vector<int>(totals, totals + n_species));
+        *out = b.release();
+    });
+}
+
 mk_status mk_domain_create_grid(int32_t dim, const int64_t *dims, const double *spacing,
                                 const double *volumes, int32_t n_species,
                                 const double *diffusivities, mk_domain *out) {
@@ -1502,6 +1609,20 @@ mk_status mk_generator_size(mk_generator g, int64_t *rows, int64_t *n_jumps) {
     return guarded([&] {
         if (rows) *rows = g->n;
         if (n_jumps) *n_jumps = g->n_jumps;
+    });
+}
+
+mk_status mk_generator_entries(mk_generator g, int64_t *entries, int64_t *ptr, int64_t *col,
+                               double *rate) {
+    return guarded([&] {
+        require(g && entries, "generator: bad arguments");
+        std::vector<int64_t> p, c;
+        std::vector<double> v;
+        download_gather(*g, p, c, v);
+        *entries = static_cast<int64_t>(c.size());
+        if (ptr) std::copy(p.begin(), p.end(), ptr);
+        if (col) std::copy(c.begin(), c.end(), col);
+        if (rate) std::copy(v.begin(), v.end(), rate);
     });
 }
 
@@ -1601,6 +1722,8 @@ mk_status mk_sector_mg_create(mk_domain domain, mk_box box, double shift,
         } else {
             build_levels<double>(*mg, H, mg->L64);
         }
+        if (!space.is_box() && space.n_sectors() == 1)
+            mg->host_levels = std::make_shared<mk::sector_hierarchy>(std::move(H[0]));
         *out = mg.release();
     });
 }
@@ -1619,6 +1742,60 @@ mk_status mk_sector_mg_set_shift(mk_sector_mg mg, double shift) {
         }
         refresh_tail_image(*mg);
         mg->shift = shift;
+    });
+}
+
+mk_status mk_sector_mg_level(mk_sector_mg mg, int32_t level, int64_t *rows, int64_t *jumps,
+                             int32_t *terminal) {
+    return guarded([&] {
+        require(mg && mg->host_levels, "population_multigrid: the hierarchy is not over one population_sector");
+        const auto &H = mg->host_levels->levels;
+        require(level >= 0 && level < static_cast<int32_t>(H.size()), "population_multigrid: no such level");
+        const auto &lv = H[level];
+        if (rows) *rows = lv.rows;
+        if (jumps) *jumps = lv.A.n_jumps;
+        if (terminal) *terminal = lv.terminal ? 1 : 0;
+    });
+}
+
+mk_status mk_sector_mg_transfer(mk_sector_mg mg, int32_t level, int64_t *coarse_of, double *weight) {
+    return guarded([&] {
+        require(mg && mg->host_levels && coarse_of && weight,
+                "population_multigrid: the hierarchy is not over one population_sector");
+        const auto &H = mg->host_levels->levels;
+        require(level >= 0 && level + 1 < static_cast<int32_t>(H.size()) && !H[level].terminal,
+                "population_multigrid: the level has no coarse level");
+        std::copy(H[level].coarse_of.begin(), H[level].coarse_of.end(), coarse_of);
+        std::copy(H[level].weight.begin(), H[level].weight.end(), weight);
+    });
+}
+
+mk_status mk_sector_mg_relax_device(mk_sector_mg mg, const double *d_rhs, double *d_x,
+                                    int32_t sweeps, int32_t reverse) {
+    return guarded([&] {
+        require(mg && d_rhs && d_x && sweeps >= 0, "population_multigrid: bad arguments");
+        if (mg->precision == MK_FP32) {
+            relax_top<float>(*mg, mg->L32, d_rhs, d_x, sweeps, reverse != 0);
+        } else {
+            relax_top<double>(*mg, mg->L64, d_rhs, d_x, sweeps, reverse != 0);
+        }
+        drain();
+    });
+}
+
+mk_status mk_sector_mg_relax(mk_sector_mg mg, const double *rhs, double *x, int32_t sweeps,
+                             int32_t reverse) {
+    return guarded([&] {
+        require(mg && rhs && x && sweeps >= 0, "population_multigrid: bad arguments");
+        const std::size_t bytes = static_cast<std::size_t>(mg->rows) * sizeof(double);
+        if (!mg->d_b.p) mg->d_b = dbuf(std::max<std::size_t>(bytes, 8));
+        if (!mg->d_x64.p) mg->d_x64 = dbuf(std::max<std::size_t>(bytes, 8));
+        HIPCHK(hipMemcpy(mg->d_b.p, rhs, bytes, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(mg->d_x64.p, x, bytes, hipMemcpyHostToDevice));
+        const mk_status st = mk_sector_mg_relax_device(mg, dp<double>(mg->d_b), dp<double>(mg->d_x64),
+                                                       sweeps, reverse);
+        if (st != MK_OK) throw std::runtime_error(mk_last_error());
+        HIPCHK(hipMemcpy(x, mg->d_x64.p, bytes, hipMemcpyDeviceToHost));
     });
 }
 

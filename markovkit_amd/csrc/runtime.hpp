@@ -654,5 +654,8 @@ struct mk_sector_mg_s {
     std::vector<mk::gpu::lds_copy> tail_copies;
     mk::u32 tail_image_bytes = 0;
     mk::u32 tail_lds_bytes = 0;
+    // a hierarchy over one population_sector (population_multigrid) keeps the host form of its
+    // levels: rows, entries and the transfers, in rank order and FP64
+    std::shared_ptr<mk::sector_hierarchy> host_levels;
 };
 

@@ -1,0 +1,109 @@
+"""The reference's per-sector surface on the GPU, through the C ABI, against the oracle:
+- population_sector (rdme/population_space.hpp:103-127): the states of a sector in the order of
+  that sector inside a population_box;
+- generator::jumps / matrix (generator.hpp:51-53, :84-89): the stored entries read back from the
+  device equal the oracle's jump lists (merged per source and target) bit for bit;
+- population_multigrid (multigrid.hpp:139-236) as a sector multigrid over one sector: its level
+  sizes, transfer (coarse state and multinomial weight) and cycle equal the oracle's for that
+  sector of a box, and relax (multigrid.hpp:231-236) from a nonzero guess equals the oracle's
+  relax for every smoother (FP64; 1e-12 relative, the summation order differs: gather on the
+  device, scatter in the oracle).
+"""
+import numpy as np
+import pytest
+
+import markovkit_amd as mk
+from markovkit_amd.model import Grid, ReactionDiffusion, VoxelDomain
+from markovkit_amd.workloads import annihilation_2d, birth_death_1d
+from oracle.oracle import Problem, SectorMultigrid as OracleMG
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "1d_v8_total3": (birth_death_1d(8, 3, 0.7, "random"), [3]),
+    "2d_4x4_totals21": (annihilation_2d(4, (2, 1), 0.9), [2, 1]),
+}
+SMOOTHERS = {mk.JACOBI: "jacobi", mk.RED_BLACK: "red_black", mk.AGGREGATE_BLOCK: "aggregate_block",
+             mk.ILU0: "ilu0"}
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def case(request):
+    w, totals = CASES[request.param]
+    P = Problem(w.model, w.max_totals)
+    box = mk.PopulationBox(w.n_voxels, w.max_totals)
+    k = box.find_sector(totals)
+    sector = mk.PopulationSector(w.n_voxels, totals)
+    return dict(name=request.param, w=w, P=P, box=box, k=k, sector=sector, totals=totals)
+
+
+def test_sector_states_follow_the_box_order(case):
+    box, k, sector = case["box"], case["k"], case["sector"]
+    lo, hi = box.sector_begin(k), box.sector_end(k)
+    assert sector.size() == hi - lo and sector.n_sectors() == 1
+    for i in range(0, sector.size(), max(1, sector.size() // 97)):
+        assert sector[i] == box[lo + i]
+        assert sector.find(sector[i]) == i
+    other = list(box[hi]) if hi < box.size() else None
+    if other is not None:
+        assert sector.find(other) is None
+
+
+def gather_from_oracle(P):
+    first, to, rate = P.jumps()
+    src = np.repeat(np.arange(P.n), np.diff(first))
+    # merge jumps of one source to one target (in generation order), then sort by destination
+    key = to * P.n + src
+    order = np.argsort(key, kind="stable")
+    key, rate = key[order], rate[order]
+    uniq, start = np.unique(key, return_index=True)
+    merged = np.add.reduceat(rate, start) if len(rate) else rate
+    dest, source = uniq // P.n, uniq % P.n
+    ptr = np.zeros(P.n + 1, np.int64)
+    np.add.at(ptr, dest + 1, 1)
+    return np.cumsum(ptr), source, merged
+
+
+def test_generator_entries_equal_the_oracle_jumps(case):
+    w, P = case["w"], case["P"]
+    Q = mk.Generator(case["box"], w.model)
+    ptr, col, rate = Q.entries()
+    optr, ocol, orate = gather_from_oracle(P)
+    assert np.array_equal(ptr, optr)
+    assert np.array_equal(col, ocol)
+    assert np.array_equal(rate, orate)  # one merge per (source, target): same sums, same order
+
+
+@pytest.mark.parametrize("smoother", list(SMOOTHERS), ids=list(SMOOTHERS.values()))
+def test_population_multigrid_against_the_oracle_sector(case, smoother):
+    w, P, k, sector = case["w"], case["P"], case["k"], case["sector"]
+    shift = w.shifts[0]
+    opts = mk.MultigridOptions(precision=mk.FP64, smoother=smoother)
+    M = mk.SectorMultigrid(w.model.domain, sector, shift, opts)
+    O = OracleMG(P, shift, smoother=SMOOTHERS[smoother])
+    rows = O.level_rows()[k]
+    levels = [int(r) for r in rows if r > 0]
+    assert M.max_levels() == len(levels)
+    for l, r in enumerate(levels):
+        assert M.level(l)[0] == r
+    for l in range(len(levels) - 1):
+        co, wt = M.transfer(l)
+        oco, owt = O.transfer(k, l)
+        assert np.array_equal(co, oco)
+        np.testing.assert_allclose(wt, owt, rtol=1e-14, atol=0)
+    n = sector.size()
+    lo = case["box"].sector_begin(k)
+    rng = np.random.default_rng(4)
+    rhs = rng.random(n)
+    x0 = rng.random(n) - 0.5
+    # the cycle of the one-sector hierarchy is the oracle's cycle of that sector
+    b = np.zeros(P.n)
+    b[lo:lo + n] = rhs
+    z = M.apply(rhs)
+    zo = O.apply(b)[lo:lo + n]
+    assert np.abs(z - zo).max() <= 1e-12 * np.abs(zo).max()
+    for reverse in (False, True):
+        xr = M.relax(rhs, x0, sweeps=2, reverse=reverse)
+        xo = O.relax(k, rhs, x0.copy(), sweeps=2, reverse=reverse)
+        assert np.abs(xr - xo).max() <= 1e-12 * np.abs(xo).max(), (SMOOTHERS[smoother], reverse)
+        assert not np.array_equal(xr, x0)

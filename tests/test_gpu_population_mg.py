@@ -13,7 +13,6 @@ import numpy as np
 import pytest
 
 import markovkit_amd as mk
-from markovkit_amd.model import Grid, ReactionDiffusion, VoxelDomain
 from markovkit_amd.workloads import annihilation_2d, birth_death_1d
 from oracle.oracle import Problem, SectorMultigrid as OracleMG
 

@@ -344,13 +344,27 @@ mk_status mk_expv_device(mk_generator gen, double t, const double *d_v, double *
  * No single-call counterpart: the reference solves each (shift, rhs) pair with its own
  * num::gmres call and sector_multigrid (examples/rdme_multigrid_benchmark.cpp:92-121). A batch
  * solves (s_j I - Q) x_j = b_j for j < n_systems together: gen is the shared generator, mg a
- * sector multigrid of the same box (its level operators and transfers are shared, its own
- * shift is not used: the terminal blocks are re-inverted for every distinct s_j, and its
- * precision selects the FP32 or FP64 preconditioner). gen and mg must outlive the batch. Every
- * system runs flexible GMRES(restart <= 64) with delayed CGS2; systems stop individually. */
+ * sector multigrid of the same box with the Jacobi or red-black smoother (red-black with at
+ * least one pre- and one post-smoothing sweep); its level operators and transfers are shared,
+ * its own shift is not used: every system holds the terminal inverses of its s_j, and the
+ * hierarchy's precision selects the FP32 or FP64 preconditioner. gen and mg must outlive the
+ * batch. Every system runs flexible GMRES(restart <= 64) with delayed CGS2 and stops
+ * individually; the steps of an Arnoldi iteration are replayed from captured HIP graphs. */
 mk_status mk_batch_create(mk_generator gen, mk_sector_mg mg, int64_t n_systems,
                           const double *shifts, mk_batch *out);
+/* Systems of different sizes in one batch: system j is the leading rows[j] x rows[j] block of
+ * s_j I - Q, rows[j] the end of a sector of the box (NULL: every system has all rows). For an
+ * absorbing generator of a one-species box that block is the generator of the box of a smaller
+ * cap (births out of its top sector are absorbed), and the leading sectors of the sector
+ * multigrid are that box's hierarchy: systems of caps 3, 4 and 5 share one launch on the cap-5
+ * box. B and X keep `rows` values per system (the box's row count); entries past rows[j] are
+ * ignored on input and zero on output. */
+mk_status mk_batch_create_sized(mk_generator gen, mk_sector_mg mg, int64_t n_systems,
+                                const double *shifts, const int64_t *rows, mk_batch *out);
 mk_status mk_batch_destroy(mk_batch batch);
+/* graph replay of the batched Arnoldi steps (default on; MK_BATCH_GRAPHS=0 turns it off at
+ * start); replays run the same kernels with the same arguments */
+mk_status mk_batch_graphs_enable(int32_t on);
 /* B and X are system-major (system j's vector at j * rows), host buffers; X holds the initial
  * guesses on input. tolerances: n_systems absolute residual norms, or NULL for
  * options->tolerance; results: n_systems entries (orthogonalization is ignored). */

@@ -6,7 +6,7 @@ in-tree HIP library through its C ABI and has no CPU fallback.
 """
 from .model import (Graph, Grid, LocalReaction, ReactionDiffusion, VoxelDomain, birth_death,
                     box_size, compositions)
-from .api import (ABSORBING, AGGREGATE_BLOCK, FP32, FP64, ILU0, JACOBI, RED_BLACK, REFLECTING, BatchSolver,
+from .api import (ABSORBING, AGGREGATE_BLOCK, FP32, FP64, ILU0, JACOBI, RED_BLACK, REFLECTING, BatchSolver, batch_graphs_enable,
                   DeviceVector, Server,
                   Generator, MultigridOptions, PopulationBox, PopulationSector, SectorMultigrid, ShiftedGenerator,
                   SolverResult, device_setup_enable, expv, expv_device, fused_tail_enable, tail_lds_enable, gmres, graphs_enable, profile_enable, profile_read,
@@ -17,7 +17,7 @@ from .boxmg import BoxMultigrid
 __all__ = [
     "Graph", "Grid", "LocalReaction", "ReactionDiffusion", "VoxelDomain", "birth_death",
     "box_size", "compositions", "ABSORBING", "AGGREGATE_BLOCK", "FP32", "FP64", "ILU0", "JACOBI", "RED_BLACK",
-    "REFLECTING", "BatchSolver", "DeviceVector", "Server", "Generator", "MultigridOptions", "PopulationBox", "PopulationSector",
+    "REFLECTING", "BatchSolver", "batch_graphs_enable", "DeviceVector", "Server", "Generator", "MultigridOptions", "PopulationBox", "PopulationSector",
     "SectorMultigrid", "ShiftedGenerator", "SolverResult", "device_setup_enable", "expv", "expv_device", "fused_tail_enable", "tail_lds_enable", "gmres", "graphs_enable",
     "profile_enable",
     "profile_read", "profile_reset", "synchronize", "boxmg", "BoxMultigrid",

@@ -107,6 +107,8 @@ SIGNATURES = {
     "mk_server_submit_expv": ([P, P, f64, P, C.POINTER(ExpvOptions), C.POINTER(i64)], i32),
     "mk_server_wait": ([P, i64, P, C.POINTER(ServerResult)], i32),
     "mk_batch_create": ([P, P, i64, P, C.POINTER(P)], i32),
+    "mk_batch_create_sized": ([P, P, i64, P, P, C.POINTER(P)], i32),
+    "mk_batch_graphs_enable": ([i32], i32),
     "mk_batch_destroy": ([P], i32),
     "mk_batch_solve": ([P, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_batch_solve_device": ([P, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),

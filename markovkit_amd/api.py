@@ -477,12 +477,18 @@ class BatchSolver:
     stopping on its own tolerance (mk_batch_create / mk_batch_solve). The multigrid's
     precision selects the preconditioner; its own shift is not used."""
 
-    def __init__(self, Q: Generator, M: SectorMultigrid, shifts: Sequence[float]):
+    def __init__(self, Q: Generator, M: SectorMultigrid, shifts: Sequence[float], rows=None):
+        """rows (optional): per system, the leading rows it has (the end of a sector of the
+        box; for an absorbing one-species box, the size of the box of a smaller cap)."""
         self.shifts = np.ascontiguousarray(shifts, np.float64)
         self._Q, self._M = Q, M  # the batch reads both handles
         self._h = C.c_void_p()
-        check(lib().mk_batch_create(Q._h, M._h, len(self.shifts), _p(self.shifts),
-                                    C.byref(self._h)))
+        self.rows = None if rows is None else np.ascontiguousarray(rows, np.int64)
+        if self.rows is not None and self.rows.shape != self.shifts.shape:
+            raise ValueError("one row count per system")
+        check(lib().mk_batch_create_sized(Q._h, M._h, len(self.shifts), _p(self.shifts),
+                                          _p(self.rows) if self.rows is not None else None,
+                                          C.byref(self._h)))
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -590,6 +596,11 @@ class Server:
                                      r.queued_s, r.run_s)
         return out, ServedResult("expv", None, ExpvResult(r.expv.steps, r.expv.rejected,
                                                           r.expv.error), r.queued_s, r.run_s)
+
+
+def batch_graphs_enable(on: bool = True) -> None:
+    """Graph replay of the batched Arnoldi steps (default on)."""
+    check(lib().mk_batch_graphs_enable(1 if on else 0))
 
 
 def graphs_enable(on: bool = True) -> None:

@@ -214,11 +214,13 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
     // row offset of every active sector on every level
     std::vector<std::vector<i64>> offset(depth, std::vector<i64>(K, -1));
     std::vector<i64> level_rows(depth, 0);
+    mg.sector_rows.assign(depth, std::vector<i64>(K, 0));
     for (i64 l = 0; l < depth; ++l)
         for (i64 k = 0; k < K; ++k)
             if (static_cast<i64>(H[k].levels.size()) > l) {
                 offset[l][k] = level_rows[l];
                 level_rows[l] += H[k].levels[l].rows;
+                mg.sector_rows[l][k] = H[k].levels[l].rows;
             }
     L.resize(depth);
     // red-black: every level numbers each sector's states colour 0 first (ascending rank within

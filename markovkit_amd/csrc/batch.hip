@@ -5,13 +5,28 @@
 //
 // The shift enters the multigrid only through the diagonal of every level operator and the
 // dense terminal inverses, so one hierarchy serves all systems: level operators and transfers
-// are shared (read once per row for all systems of a wavefront) and the terminal blocks are
-// re-inverted once per distinct shift. Each system runs flexible GMRES(m) with delayed CGS2
-// and its own Givens least-squares problem; a system leaves the lockstep iteration when its
-// residual estimate reaches its tolerance (its lanes then load and store nothing).
+// are shared (read once per row for all systems of a wavefront) and every system slot holds the
+// terminal inverses of its own shift. Each system runs flexible GMRES(m) with delayed CGS2 and its
+// own Givens least-squares problem.
+//
+// Systems of different sizes share the launch: system j may be the leading rows[j] x rows[j]
+// block of s_j I - Q, rows[j] a boundary between whole sectors. For an absorbing generator of a
+// one-species box, the leading block of sectors 0..k is exactly the generator of the box of cap
+// k (births out of sector k are absorbed in both), and the sector multigrid is block-diagonal by
+// sector, so its leading sectors are that box's hierarchy. Lane j works on the rows below its
+// limit on every level; the vectors it gathers past them stay zero.
+//
+// The batch is an engine of slots: systems are admitted to free slots and retired at restart
+// boundaries (continuous batching, used by the request server); mk_batch_solve admits all its
+// systems and steps until they are done. A lane's arithmetic does not depend on the other lanes,
+// so a system's result is the same alone and in any mix. The operations of one Arnoldi step
+// (preconditioner cycle, operator, projections, update) are captured in a HIP graph per step
+// index and replayed.
 #include "batch_kernels.hpp"
 #include "runtime.hpp"
 
+#include <cstring>
+#include <map>
 #include <type_traits>
 
 using namespace mk::rt;
@@ -22,29 +37,62 @@ namespace {
 template <typename T>
 struct blevel {
     dbuf rhs, x, xt, res, crhs, cx; // n_level x Bp work vectors
-    dbuf tinv;                      // terminal inverses, one set of per_shift values per shift
+    dbuf tinv;                      // terminal inverses: per_shift values for every slot
     u64 per_shift = 0;
 };
+
+// the FGMRES state of one slot
+struct slot_state {
+    i64 rows = 0;        // rows of the system; 0: a free slot
+    double shift = 1.0, tol = 0.0;
+    i64 max_iterations = 0;
+    bool fresh = false;  // admitted; its initial residual is not taken yet
+    bool active = false; // iterating
+    bool done = false;   // finished (converged, non-finite or out of iterations)
+    mk_solver_result res{};
+    double beta = 0.0, bj = 0.0;
+    i64 cols = 0;
+    char flag = 0;
+};
+
+// the batched step graphs are replayed by default (MK_BATCH_GRAPHS=0 launches eagerly)
+std::atomic<int> &batch_graph_switch() {
+    static std::atomic<int> on{[] {
+        const char *env = std::getenv("MK_BATCH_GRAPHS");
+        return (env && env[0] == '0') ? 0 : 1;
+    }()};
+    return on;
+}
 
 } // namespace
 
 struct mk_batch_s {
+    mk::u64 uid = next_uid();
     mk_generator_s *gen = nullptr;
     mk_sector_mg_s *mg = nullptr;
-    i64 n = 0, nsys = 0, Bp = 0, n_shift = 0;
-    std::vector<double> shifts; // per padded system (1 for padding)
-    dbuf d_shift, d_sidx;
+    i64 n = 0, Bp = 0, nsys = 0;
+    int levels = 0;
+    std::vector<slot_state> slot;
+    std::vector<hessenberg_ls> ls;
+    std::vector<std::vector<int>> lim;  // [level][slot]: the slot's rows on the level
+    std::vector<i64> sector_end;        // level-0 row where each sector ends (admissible sizes)
+    dbuf d_shift, d_sidx, d_lim, d_rows, d_sel;
+    int *h_lim = nullptr;               // pinned: the levels x Bp limits of the current step
+    std::map<mk::u64, std::vector<std::vector<double>>> inverses; // terminal inverses per shift
     std::vector<blevel<float>> B32;
     std::vector<blevel<double>> B64;
     // Krylov workspace for restart m
     i64 m = 0;
     int np = 1;
-    dbuf V, Z, w, partial, rpart, rtmp, hd, d_b, d_x, d_io, d_scale, d_beta, d_mask, d_active, d_cnt, d_y;
+    dbuf V, Z, w, partial, rpart, rtmp, hd, d_b, d_x, d_io, d_scale, d_beta, d_cnt, d_y;
     double *host = nullptr; // pinned: hd mirror, y, scale, beta
-    int *ihost = nullptr;   // pinned: mask, active, cnt
+    int *ihost = nullptr;   // pinned: cnt, rows, sel
+    graph_cache graphs;
     ~mk_batch_s() {
+        graphs.clear();
         if (host) (void)hipHostFree(host);
         if (ihost) (void)hipHostFree(ihost);
+        if (h_lim) (void)hipHostFree(h_lim);
     }
 };
 
@@ -62,11 +110,25 @@ struct slots {
 };
 
 template <typename T>
-void build_blevels(mk_batch_s &B, const std::vector<level_dev<T>> &L, std::vector<blevel<T>> &BL,
-                   const std::vector<double> &distinct) {
+std::vector<level_dev<T>> &levels_of(mk_batch_s &B);
+template <>
+std::vector<level_dev<float>> &levels_of<float>(mk_batch_s &B) { return B.mg->L32; }
+template <>
+std::vector<level_dev<double>> &levels_of<double>(mk_batch_s &B) { return B.mg->L64; }
+template <typename T>
+std::vector<blevel<T>> &blevels_of(mk_batch_s &B);
+template <>
+std::vector<blevel<float>> &blevels_of<float>(mk_batch_s &B) { return B.B32; }
+template <>
+std::vector<blevel<double>> &blevels_of<double>(mk_batch_s &B) { return B.B64; }
+
+template <typename T>
+void build_blevels(mk_batch_s &B) {
+    auto &L = levels_of<T>(B);
+    auto &BL = blevels_of<T>(B);
     BL.clear();
     BL.resize(L.size());
-    const int cycles = B.mg->opt.coarse_cycles;
+    const bool cycles = B.mg->opt.coarse_cycles > 1;
     for (std::size_t l = 0; l < L.size(); ++l) {
         const auto &lv = L[l];
         auto &bl = BL[l];
@@ -75,36 +137,92 @@ void build_blevels(mk_batch_s &B, const std::vector<level_dev<T>> &L, std::vecto
         bl.x = dbuf(vb);
         bl.xt = dbuf(vb);
         bl.res = dbuf(vb);
-        if (cycles > 1) {
+        if (cycles) {
             bl.crhs = dbuf(vb);
             bl.cx = dbuf(vb);
         }
         if (lv.n_term > 0) {
             bl.per_shift = lv.h_tq.size();
-            std::vector<T> inv(bl.per_shift * distinct.size());
-            for (std::size_t d = 0; d < distinct.size(); ++d) {
-                for (i64 b = 0; b < lv.n_term; ++b) {
-                    const i64 m = lv.h_tsize[b];
-                    std::vector<double> a(lv.h_tq.begin() + lv.h_toff[b],
-                                          lv.h_tq.begin() + lv.h_toff[b] + m * m);
-                    for (i64 i = 0; i < m; ++i) a[i * m + i] += distinct[d];
-                    std::vector<double> out;
-                    if (!mk::invert_dense(std::move(a), m, out))
-                        throw std::runtime_error("population_multigrid: singular block");
-                    std::copy(out.begin(), out.end(), inv.begin() + d * bl.per_shift + lv.h_toff[b]);
-                }
-            }
-            bl.tinv = upload(inv);
+            bl.tinv = dbuf(bl.per_shift * B.Bp * sizeof(T));
         }
     }
 }
 
-// one batched cycle of level l from a zero guess (population_multigrid::cycle,
-// multigrid.hpp:201-228), the per-system form of run_level in api.hip
+// the terminal inverses of every level for one shift (cached by the shift's bits)
+const std::vector<std::vector<double>> &inverses_for(mk_batch_s &B, double shift) {
+    auto it = B.inverses.find(bits(shift));
+    if (it != B.inverses.end()) return it->second;
+    std::vector<std::vector<double>> per_level(B.levels);
+    auto fill = [&](const auto &L) {
+        for (int l = 0; l < B.levels; ++l) {
+            const auto &lv = L[l];
+            if (lv.n_term == 0) continue;
+            auto &inv = per_level[l];
+            inv.assign(lv.h_tq.size(), 0.0);
+            for (i64 b = 0; b < lv.n_term; ++b) {
+                const i64 m = lv.h_tsize[b];
+                std::vector<double> a(lv.h_tq.begin() + lv.h_toff[b], lv.h_tq.begin() + lv.h_toff[b] + m * m);
+                for (i64 i = 0; i < m; ++i) a[i * m + i] += shift;
+                std::vector<double> out;
+                if (!mk::invert_dense(std::move(a), m, out))
+                    throw std::runtime_error("population_multigrid: singular block");
+                std::copy(out.begin(), out.end(), inv.begin() + lv.h_toff[b]);
+            }
+        }
+    };
+    if (B.mg->precision == MK_FP32) {
+        fill(B.mg->L32);
+    } else {
+        fill(B.mg->L64);
+    }
+    if (B.inverses.size() >= 4096) B.inverses.clear();
+    return B.inverses.emplace(bits(shift), std::move(per_level)).first->second;
+}
+
+// slot j takes shift s and the leading `rows` rows: limits per level and terminal inverses
 template <typename T>
-T *brun_level(mk_batch_s &B, std::vector<level_dev<T>> &L, std::vector<blevel<T>> &BL,
-              std::size_t l, const T *rhs, const double *rhs64, const double *scale, T *out,
-              const int *act) {
+void put_slot_inverses(mk_batch_s &B, i64 j, const std::vector<std::vector<double>> &inv) {
+    auto &BL = blevels_of<T>(B);
+    for (int l = 0; l < B.levels; ++l) {
+        if (BL[l].per_shift == 0) continue;
+        std::vector<T> v(inv[l].begin(), inv[l].end());
+        HIPCHK(hipMemcpy(static_cast<T *>(BL[l].tinv.p) + j * BL[l].per_shift, v.data(),
+                         v.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+}
+
+void set_slot(mk_batch_s &B, i64 j, double shift, i64 rows) {
+    if (!(shift > 0.0) || !std::isfinite(shift))
+        throw std::invalid_argument("population_multigrid: shift must be positive");
+    const auto k = std::find(B.sector_end.begin(), B.sector_end.end(), rows);
+    require(k != B.sector_end.end(),
+            "batch: a system's rows must end at a sector boundary of the box");
+    const i64 sectors = (k - B.sector_end.begin()) + 1;
+    for (int l = 0; l < B.levels; ++l) {
+        i64 r = 0;
+        for (i64 q = 0; q < sectors; ++q) r += B.mg->sector_rows[l][q];
+        B.lim[l][j] = static_cast<int>(r);
+    }
+    auto &S = B.slot[j];
+    S.rows = rows;
+    S.shift = shift;
+    const auto &inv = inverses_for(B, shift);
+    if (B.mg->precision == MK_FP32) {
+        put_slot_inverses<float>(B, j, inv);
+    } else {
+        put_slot_inverses<double>(B, j, inv);
+    }
+    HIPCHK(hipMemcpy(dp<double>(B.d_shift) + j, &shift, sizeof(double), hipMemcpyHostToDevice));
+}
+
+// one batched cycle of level l from a zero guess (population_multigrid::cycle,
+// multigrid.hpp:201-228), the per-system form of run_level in api.hip; lims holds the row limits
+// of every level (levels x Bp, 0 for masked lanes)
+template <typename T>
+T *brun_level(mk_batch_s &B, std::size_t l, const T *rhs, const double *rhs64, const double *scale,
+              T *out, const int *lims) {
+    auto &L = levels_of<T>(B);
+    auto &BL = blevels_of<T>(B);
     auto &lv = L[l];
     auto &bl = BL[l];
     auto &P = prof();
@@ -112,83 +230,139 @@ T *brun_level(mk_batch_s &B, std::vector<level_dev<T>> &L, std::vector<blevel<T>
     const auto &mg = *B.mg;
     const double omega = mg.opt.jacobi_weight;
     const double *sh = dp<double>(B.d_shift);
+    const int *sidx = dp<int>(B.d_sidx);
     const i64 n = lv.n, Bp = B.Bp;
+    const int *lim = lims + static_cast<i64>(l) * Bp;
     const double ts = double(sizeof(T)), nb = double(n) * double(Bp);
     const double sweep_bytes = op_bytes(lv.A, 0, 0, 0) + nb * 4.0 * ts;
     T *cur = dp<T>(bl.x);
     T *other = dp<T>(bl.xt);
-    if (rhs64) {
+    T *res = dp<T>(bl.res);
+    auto terminal = [&](const T *r, T *x) {
+        P.run(K_TERMINAL, 0.0, [&] {
+            bk::terminal_solve<T>(lv.n_term, dp<u32>(lv.tstart), dp<u32>(lv.tsize), dp<u64>(lv.toff),
+                                  dp<T>(bl.tinv), bl.per_shift, sidx, lv.term_max, r, x, Bp, lim, s);
+        });
+    };
+    if (lv.direct) {
+        // a coarse level of terminal blocks only: its cycle from zero is their dense solve
+        T *dst = out ? out : cur;
+        P.run(K_OTHER, nb * ts, [&] { gpu::fill<T>(n * Bp, T(0), dst, s); });
+        terminal(rhs, dst);
+        return dst;
+    }
+    const bool rbgs = lv.rb;
+    const u32 *perm0 = dp<u32>(lv.perm);
+    if (rbgs && out && !perm0) cur = out; // the in-place smoother works in the output
+    bool have_residual = false;
+    if (rbgs) {
+        // red-black Gauss-Seidel: colour 0 from x = 0, then colour 1, colour 0, ...; after the
+        // last colour-1 half-sweep only colour-0 rows have a residual
+        T *rb = rhs64 || perm0 ? dp<T>(bl.rhs) : nullptr;
+        P.run(K_FIRST, nb * (rhs64 ? 8.0 + ts : ts) + nb * ts, [&] {
+            if (rhs64) {
+                bk::rb_first<T, double>(lv.A.view, dp<std::uint8_t>(lv.colour), sh, rhs64, scale, rb, cur,
+                                        perm0, Bp, lim, s);
+            } else {
+                bk::rb_first<T, T>(lv.A.view, dp<std::uint8_t>(lv.colour), sh, rhs, nullptr, rb, cur,
+                                   perm0, Bp, lim, s);
+            }
+        });
+        if (rb) rhs = rb;
+        auto half_bytes = [&](int c) {
+            return op_bytes(lv.Ac[c], 0, 0, 0) + double(lv.Ac[c].n) * Bp * 3.0 * ts;
+        };
+        for (int half = 1; half < 2 * lv.pre; ++half) {
+            const int c = half & 1;
+            T *rz = half == 2 * lv.pre - 1 ? res : nullptr;
+            P.run(K_SWEEP, half_bytes(c), [&] { bk::gs_half<T>(lv.Ac[c].view, sh, rhs, cur, rz, Bp, lim, s); });
+        }
+        P.run(K_RESIDUAL, half_bytes(0), [&] { bk::level_residual<T>(lv.Ac[0].view, sh, rhs, cur, res, Bp, lim, s); });
+        have_residual = true;
+    } else if (rhs64) {
         T *rb = dp<T>(bl.rhs);
         P.run(K_FIRST, nb * (8 + 3 * ts), [&] {
             bk::jacobi_first<T, double>(lv.A.view, sh, lv.pre > 0 ? omega : 0.0, rhs64, scale, rb,
-                                        cur, Bp, act, s);
+                                        cur, Bp, lim, s);
         });
         rhs = rb;
     } else {
         P.run(K_FIRST, nb * 3 * ts, [&] {
             bk::jacobi_first<T, T>(lv.A.view, sh, lv.pre > 0 ? omega : 0.0, rhs, nullptr, nullptr,
-                                   cur, Bp, act, s);
+                                   cur, Bp, lim, s);
         });
     }
-    for (int k = 1; k < lv.pre; ++k) {
+    for (int k = 1; !rbgs && k < lv.pre; ++k) {
         P.run(K_SWEEP, sweep_bytes, [&] {
-            bk::jacobi_sweep<T>(lv.A.view, sh, omega, rhs, cur, other, Bp, act, s);
+            bk::jacobi_sweep<T>(lv.A.view, sh, omega, rhs, cur, other, Bp, lim, s);
         });
         std::swap(cur, other);
     }
-    T *res = dp<T>(bl.res);
-    P.run(K_RESIDUAL, sweep_bytes, [&] {
-        bk::level_residual<T>(lv.A.view, sh, rhs, cur, res, Bp, act, s);
-    });
-    if (lv.n_term > 0) {
-        P.run(K_TERMINAL, 0.0, [&] {
-            bk::terminal_solve<T>(lv.n_term, dp<u32>(lv.tstart), dp<u32>(lv.tsize), dp<u64>(lv.toff),
-                                  dp<T>(bl.tinv), bl.per_shift, dp<int>(B.d_sidx), lv.term_max, res,
-                                  cur, Bp, act, s);
+    if (!have_residual) {
+        P.run(K_RESIDUAL, sweep_bytes, [&] {
+            bk::level_residual<T>(lv.A.view, sh, rhs, cur, res, Bp, lim, s);
         });
     }
+    if (lv.n_term > 0) terminal(res, cur);
     if (lv.n_next > 0) {
         auto &c = L[l + 1];
         auto &cb = BL[l + 1];
         const double cn = double(c.n) * double(Bp);
         const double rbytes = op_bytes(lv.R, 0, 0, 0) + double(lv.n_fine_linked) * Bp * ts + cn * ts;
-        const double pbytes = double(lv.n) * 4 + nb * 2 * ts + cn * ts;
+        // red-black with post-smoothing: only colour-0 rows need the correction (the first
+        // post-smoothing half-sweep rewrites colour-1 rows without reading them)
+        const bool red_only = rbgs && lv.post > 0;
+        const i64 np = red_only ? lv.Ac[0].n : n;
+        const u32 *prows = red_only ? dp<u32>(lv.rowmap[0]) : nullptr;
+        const double pbytes = double(np) * (4 + 2 * Bp * ts) + cn * ts;
+        const int *clim = lims + static_cast<i64>(l + 1) * Bp;
         if (mg.opt.coarse_cycles <= 1 || static_cast<i64>(l) >= cycle_levels(mg.opt)) {
-            P.run(K_RESTRICT, rbytes, [&] { bk::gather_sum<T>(lv.R.view, res, dp<T>(cb.rhs), Bp, act, s); });
-            T *cx = brun_level<T>(B, L, BL, l + 1, dp<T>(cb.rhs), nullptr, nullptr, nullptr, act);
+            P.run(K_RESTRICT, rbytes, [&] { bk::gather_sum<T>(lv.R.view, res, dp<T>(cb.rhs), Bp, clim, s); });
+            T *cx = brun_level<T>(B, l + 1, dp<T>(cb.rhs), nullptr, nullptr, nullptr, lims);
             P.run(K_PROLONG, pbytes, [&] {
-                bk::prolong_add<T>(n, dp<u32>(lv.coarse_of), dp<T>(lv.weight), cx, cur, Bp, act, s);
+                bk::prolong_add<T>(np, dp<u32>(lv.coarse_of), dp<T>(lv.weight), cx, cur, Bp, lim, s, prows);
             });
         } else {
-            P.run(K_RESTRICT, rbytes, [&] { bk::gather_sum<T>(lv.R.view, res, dp<T>(cb.crhs), Bp, act, s); });
+            P.run(K_RESTRICT, rbytes, [&] { bk::gather_sum<T>(lv.R.view, res, dp<T>(cb.crhs), Bp, clim, s); });
             P.run(K_OTHER, cn * ts, [&] { gpu::fill<T>(c.n * Bp, T(0), dp<T>(cb.cx), s); });
             const double cbytes = op_bytes(c.A, 0, 0, 0) + cn * 3 * ts;
             for (int cyc = 0; cyc < mg.opt.coarse_cycles; ++cyc) {
                 P.run(K_RESIDUAL, cbytes, [&] {
-                    bk::level_residual<T>(c.A.view, sh, dp<T>(cb.crhs), dp<T>(cb.cx), dp<T>(cb.rhs), Bp, act, s);
+                    bk::level_residual<T>(c.A.view, sh, dp<T>(cb.crhs), dp<T>(cb.cx), dp<T>(cb.rhs), Bp, clim, s);
                 });
-                T *ce = brun_level<T>(B, L, BL, l + 1, dp<T>(cb.rhs), nullptr, nullptr, nullptr, act);
+                T *ce = brun_level<T>(B, l + 1, dp<T>(cb.rhs), nullptr, nullptr, nullptr, lims);
                 P.run(K_OTHER, cn * 3 * ts, [&] { gpu::axpy<T>(c.n * Bp, T(1), ce, dp<T>(cb.cx), s); });
             }
             P.run(K_PROLONG, pbytes, [&] {
-                bk::prolong_add<T>(n, dp<u32>(lv.coarse_of), dp<T>(lv.weight), dp<T>(cb.cx), cur, Bp, act, s);
+                bk::prolong_add<T>(np, dp<u32>(lv.coarse_of), dp<T>(lv.weight), dp<T>(cb.cx), cur, Bp, lim, s, prows);
             });
         }
     }
-    for (int k = 0; k < lv.post; ++k) {
-        T *dst = (k == lv.post - 1 && out) ? out : other;
-        P.run(K_SWEEP, sweep_bytes, [&] {
-            bk::jacobi_sweep<T>(lv.A.view, sh, omega, rhs, cur, dst, Bp, act, s);
-        });
-        if (dst == other) {
-            std::swap(cur, other);
-        } else {
-            cur = dst;
+    if (rbgs) {
+        for (int half = 0; half < 2 * lv.post; ++half) {
+            const int c = 1 - (half & 1); // reverse colour order
+            // the last two half-sweeps also store their rows to the output in the caller's order
+            T *nat = perm0 && out && half >= 2 * lv.post - 2 ? out : nullptr;
+            P.run(K_SWEEP, op_bytes(lv.Ac[c], 0, 0, 0) + double(lv.Ac[c].n) * Bp * 3.0 * ts, [&] {
+                bk::gs_half<T>(lv.Ac[c].view, sh, rhs, cur, nullptr, Bp, lim, s, nat, perm0);
+            });
+        }
+        if (perm0 && out) return out;
+    } else {
+        for (int k = 0; k < lv.post; ++k) {
+            T *dst = (k == lv.post - 1 && out) ? out : other;
+            P.run(K_SWEEP, sweep_bytes, [&] {
+                bk::jacobi_sweep<T>(lv.A.view, sh, omega, rhs, cur, dst, Bp, lim, s);
+            });
+            if (dst == other) {
+                std::swap(cur, other);
+            } else {
+                cur = dst;
+            }
         }
     }
     if (out && cur != out) {
-        P.run(K_OTHER, nb * 2 * ts,
-              [&] { HIPCHK(hipMemcpyAsync(out, cur, n * Bp * sizeof(T), hipMemcpyDeviceToDevice, s)); });
+        P.run(K_OTHER, nb * 2 * ts, [&] { gpu::copy16(n * Bp * sizeof(T) / 16, cur, out, s); });
         cur = out;
     }
     return cur;
@@ -199,6 +373,7 @@ void ensure_workspace(mk_batch_s &B, i64 m) {
     const i64 n = B.n, Bp = B.Bp;
     const std::size_t vec = static_cast<std::size_t>(std::max<i64>(n, 1)) * Bp;
     const std::size_t zs = B.mg->precision == MK_FP32 ? sizeof(float) : sizeof(double);
+    B.graphs.clear();
     B.V = dbuf();
     B.Z = dbuf();
     B.V = dbuf(vec * (m + 1) * sizeof(double));
@@ -213,28 +388,67 @@ void ensure_workspace(mk_batch_s &B, i64 m) {
     B.hd = dbuf(sl.total(Bp) * sizeof(double));
     B.d_scale = dbuf(Bp * sizeof(double));
     B.d_beta = dbuf(Bp * sizeof(double));
-    B.d_mask = dbuf(Bp * sizeof(int));
-    B.d_active = dbuf(Bp * sizeof(int));
     B.d_cnt = dbuf(Bp * sizeof(int));
     B.d_y = dbuf(m * Bp * sizeof(double));
     if (B.host) (void)hipHostFree(B.host);
-    if (B.ihost) (void)hipHostFree(B.ihost);
     B.host = nullptr;
-    B.ihost = nullptr;
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&B.host),
                          (sl.total(Bp) + m * Bp + 2 * Bp) * sizeof(double)));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&B.ihost), 3 * Bp * sizeof(int)));
     B.m = m;
 }
 
-// the batched FGMRES(m) with delayed CGS2 (the per-system algorithm of fgmres in api.hip)
+// the limits of the current step: lane j's rows on every level where mask[j], else 0
+void push_limits(mk_batch_s &B, const char *mask) {
+    for (int l = 0; l < B.levels; ++l)
+        for (i64 j = 0; j < B.Bp; ++j) B.h_lim[l * B.Bp + j] = mask[j] ? B.lim[l][j] : 0;
+    HIPCHK(hipMemcpyAsync(B.d_lim.p, B.h_lim, static_cast<std::size_t>(B.levels) * B.Bp * sizeof(int),
+                          hipMemcpyHostToDevice, stream()));
+}
+
+// V_0 = b - A x and beta_j = |r_j| for the lanes in mask
+void true_residual(mk_batch_s &B, const char *mask) {
+    auto &P = prof();
+    const hipStream_t s = stream();
+    const i64 n = B.n, Bp = B.Bp;
+    const slots sl{(B.m + 2) * Bp};
+    push_limits(B, mask);
+    double *rp = dp<double>(B.rpart), *hd = dp<double>(B.hd);
+    const int nrp = static_cast<int>(bk::residual_parts(n));
+    const double nb = double(n) * double(Bp);
+    P.run(K_SPMV, op_bytes(B.gen->A, 0, 0, 0) + nb * 24.0, [&] {
+        bk::shifted_apply<double>(B.gen->A.view, dp<double>(B.d_shift), dp<double>(B.d_x), dp<double>(B.d_b),
+                                  dp<double>(B.V), rp, Bp, dp<int>(B.d_lim), s);
+    });
+    P.run(K_DOT, double(nrp) * Bp * 8.0,
+          [&] { bk::reduce_partials(rp, nrp, Bp, hd + sl.rn(Bp), s, dp<double>(B.rtmp)); });
+    HIPCHK(hipMemcpyAsync(B.host + sl.rn(Bp), hd + sl.rn(Bp), Bp * sizeof(double), hipMemcpyDeviceToHost, s));
+    drain();
+    for (i64 j = 0; j < Bp; ++j)
+        if (mask[j]) B.slot[j].beta = std::sqrt(B.host[sl.rn(Bp) + j]);
+}
+
+// after a residual: converged, non-finite or out of iterations ends the slot's solve
+void settle(slot_state &S) {
+    S.res.residual = S.beta;
+    if (!std::isfinite(S.beta)) {
+        S.active = false;
+    } else if (!(S.beta > S.tol)) {
+        S.res.converged = 1;
+        S.active = false;
+    } else if (S.res.iterations >= S.max_iterations) {
+        S.active = false;
+    } else {
+        S.active = true;
+    }
+    S.done = !S.active;
+}
+
+// One restart cycle of FGMRES(m) with delayed CGS2 for the active slots (the per-system
+// algorithm of fgmres in api.hip), then x_j += Z y_j and the true residual. Fresh slots take their
+// initial residual first. Returns whether any slot is still active.
 template <typename ZT>
-void bgmres(mk_batch_s &B, const double *b, double *x, const std::vector<double> &tol,
-            const mk_gmres_options &o, mk_solver_result *out) {
-    const i64 n = B.n, Bp = B.Bp, nsys = B.nsys;
-    const i64 m = std::max<i64>(1, std::min<i64>(o.restart, n));
-    require(m <= gpu::kFusedBasis, "batch gmres: restart above 64 is not supported");
-    ensure_workspace(B, m);
+bool step(mk_batch_s &B) {
+    const i64 n = B.n, Bp = B.Bp, m = B.m;
     auto &P = prof();
     const hipStream_t s = stream();
     auto &A = *B.gen;
@@ -247,242 +461,343 @@ void bgmres(mk_batch_s &B, const double *b, double *x, const std::vector<double>
     const slots sl{(m + 2) * Bp};
     double *host = B.host;
     double *yh = host + sl.total(Bp), *scale_h = yh + m * Bp, *beta_h = scale_h + Bp;
-    int *mask_h = B.ihost, *active_h = mask_h + Bp, *cnt_h = active_h + Bp;
-    int *d_mask = dp<int>(B.d_mask), *d_active = dp<int>(B.d_active), *d_cnt = dp<int>(B.d_cnt);
+    int *cnt_h = B.ihost;
     const double nb = double(vec);
-    const double spmv_bytes = op_bytes(A.A, 0, 0, 0) + nb * 16.0;
+    std::vector<char> mask(Bp, 0);
 
-    auto push_ints = [&](const int *h, int *d) {
-        HIPCHK(hipMemcpyAsync(d, h, Bp * sizeof(int), hipMemcpyHostToDevice, s));
-    };
-    // V_0 = b - A x and |r_j|^2 for the active systems
-    std::vector<double> beta(Bp, 0.0);
-    auto true_residual = [&]() {
-        push_ints(active_h, d_active);
-        double *rp = dp<double>(B.rpart);
-        const int nrp = static_cast<int>(bk::residual_parts(n));
-        P.run(K_SPMV, spmv_bytes + nb * 8.0, [&] {
-            bk::shifted_apply<double>(A.A.view, sh, x, b, V, rp, Bp, d_active, s);
-        });
-        P.run(K_DOT, double(nrp) * Bp * 8.0,
-              [&] { bk::reduce_partials(rp, nrp, Bp, hd + sl.rn(Bp), s, dp<double>(B.rtmp)); });
-        HIPCHK(hipMemcpyAsync(host + sl.rn(Bp), hd + sl.rn(Bp), Bp * sizeof(double), hipMemcpyDeviceToHost, s));
-        drain();
-        for (i64 j = 0; j < nsys; ++j)
-            if (active_h[j]) beta[j] = std::sqrt(host[sl.rn(Bp) + j]);
-    };
-
-    for (i64 j = 0; j < Bp; ++j) active_h[j] = j < nsys ? 1 : 0;
-    for (i64 j = 0; j < nsys; ++j) out[j] = mk_solver_result{0, 0.0, 0};
-    true_residual();
-    for (i64 j = 0; j < nsys; ++j) {
-        out[j].residual = beta[j];
-        if (!(beta[j] > tol[j])) {
-            out[j].converged = std::isfinite(beta[j]) ? 1 : 0;
-            active_h[j] = 0;
-        }
-        if (o.max_iterations <= 0) active_h[j] = 0;
+    // initial residuals of the slots admitted since the last step
+    bool any_fresh = false;
+    for (i64 j = 0; j < Bp; ++j) {
+        mask[j] = B.slot[j].fresh ? 1 : 0;
+        any_fresh = any_fresh || mask[j];
     }
-    std::vector<hessenberg_ls> ls(nsys, hessenberg_ls(m));
-    std::vector<i64> cols(nsys, 0);
-    std::vector<double> bj(nsys, 0.0), col(m + 1);
-    std::vector<char> flag(nsys, 0);
-    auto any = [&](const int *a) {
-        for (i64 j = 0; j < nsys; ++j)
-            if (a[j]) return true;
+    if (any_fresh) {
+        true_residual(B, mask.data());
+        for (i64 j = 0; j < Bp; ++j) {
+            auto &S = B.slot[j];
+            if (!S.fresh) continue;
+            S.fresh = false;
+            S.res = mk_solver_result{0, 0.0, 0};
+            settle(S);
+        }
+    }
+    bool any = false;
+    for (i64 j = 0; j < Bp; ++j) {
+        mask[j] = B.slot[j].active ? 1 : 0;
+        any = any || mask[j];
+    }
+    if (!any) return false;
+    std::vector<char> iterated(mask);
+    std::vector<double> col(m + 1);
+    for (i64 j = 0; j < Bp; ++j) {
+        auto &S = B.slot[j];
+        S.cols = 0;
+        if (!mask[j]) continue;
+        B.ls[j].start(S.beta);
+        S.bj = S.beta;
+    }
+    auto any_mask = [&] {
+        for (i64 j = 0; j < Bp; ++j)
+            if (mask[j]) return true;
         return false;
     };
-    while (any(active_h)) {
-        for (i64 j = 0; j < Bp; ++j) mask_h[j] = j < nsys ? active_h[j] : 0;
-        for (i64 j = 0; j < nsys; ++j) {
-            cols[j] = 0;
-            if (!mask_h[j]) continue;
-            ls[j].start(beta[j]);
-            bj[j] = beta[j];
+    int which = 0; // s buffer the next update kernel reads
+    const bool graphs = batch_graph_switch().load() != 0;
+    for (i64 c = 0; c < m && any_mask(); ++c) {
+        const int cnt = static_cast<int>(c);
+        for (i64 j = 0; j < Bp; ++j) {
+            scale_h[j] = mask[j] ? 1.0 / B.slot[j].bj : 0.0;
+            beta_h[j] = mask[j] ? B.slot[j].bj : 1.0;
         }
-        int which = 0; // s buffer the next update kernel reads
-        for (i64 c = 0; c < m && any(mask_h); ++c) {
-            const int cnt = static_cast<int>(c);
-            for (i64 j = 0; j < Bp; ++j) {
-                const bool on = j < nsys && mask_h[j];
-                scale_h[j] = on ? 1.0 / bj[j] : 0.0;
-                beta_h[j] = on ? bj[j] : 1.0;
-            }
-            HIPCHK(hipMemcpyAsync(B.d_scale.p, scale_h, Bp * sizeof(double), hipMemcpyHostToDevice, s));
-            HIPCHK(hipMemcpyAsync(B.d_beta.p, beta_h, Bp * sizeof(double), hipMemcpyHostToDevice, s));
-            push_ints(mask_h, d_mask);
-            // z = M (u / beta), w = A z
+        HIPCHK(hipMemcpyAsync(B.d_scale.p, scale_h, Bp * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(B.d_beta.p, beta_h, Bp * sizeof(double), hipMemcpyHostToDevice, s));
+        push_limits(B, mask.data());
+        const int *lims = dp<int>(B.d_lim);
+        double *sd = hd + sl.s(which), *sd_next = hd + sl.s(1 - which);
+        // z = M (u / beta), w = A z, projections, delayed CGS2 update: one captured graph per step
+        const graph_key key{B.uid, static_cast<u64>(sizeof(ZT)), c, which, n, m, 0};
+        B.graphs.run(key, s, [&] {
             ZT *Zc = Z + c * vec;
-            if constexpr (std::is_same_v<ZT, float>) {
-                brun_level<float>(B, B.mg->L32, B.B32, 0, nullptr, V + c * vec, dp<double>(B.d_scale), Zc, d_mask);
-            } else {
-                brun_level<double>(B, B.mg->L64, B.B64, 0, nullptr, V + c * vec, dp<double>(B.d_scale), Zc, d_mask);
-            }
+            brun_level<ZT>(B, 0, nullptr, V + c * vec, dp<double>(B.d_scale), Zc, lims);
             P.run(K_SPMV, op_bytes(A.A, 0, 0, 0) + nb * (sizeof(ZT) + 8.0), [&] {
-                bk::shifted_apply<ZT>(A.A.view, sh, Zc, nullptr, w, nullptr, Bp, d_mask, s);
+                bk::shifted_apply<ZT>(A.A.view, sh, Zc, nullptr, w, nullptr, Bp, lims, s);
             });
-            double *sd = hd + sl.s(which), *sd_next = hd + sl.s(1 - which);
             P.run(K_DOT, double(cnt + 2) * nb * 8.0, [&] {
-                bk::multi_dot(V, cnt + 1, w, n, Bp, partial, np, d_mask, s);
+                bk::multi_dot(V, cnt + 1, w, n, Bp, partial, np, lims, s);
                 bk::reduce_partials(partial, np, (cnt + 1) * Bp, hd + sl.t(), s);
             });
             P.run(K_UPDATE, double(cnt + 4) * nb * 8.0, [&] {
                 bk::dcgs_update(V, cnt, sd, hd + sl.t(), dp<double>(B.d_beta), w, n, Bp, partial, np,
-                                hd + sl.eta(), d_mask, s);
+                                hd + sl.eta(), lims, s);
+                bk::reduce_partials(partial, np, (cnt + 2) * Bp, sd_next, s);
+            });
+        }, n, graphs);
+        HIPCHK(hipMemcpyAsync(host, hd, sl.total(Bp) * sizeof(double), hipMemcpyDeviceToHost, s));
+        drain();
+        const double *t = host + sl.t(), *eta = host + sl.eta();
+        double *sn = host + sl.s(1 - which);
+        bool any_flag = false;
+        std::vector<char> flag(Bp, 0);
+        for (i64 j = 0; j < Bp; ++j) {
+            if (!mask[j]) continue;
+            double ss = 0.0;
+            for (i64 k = 0; k <= c; ++k) ss += sn[k * Bp + j] * sn[k * Bp + j];
+            if (!(ss <= 0.25 * sn[(c + 1) * Bp + j])) {
+                flag[j] = 1;
+                any_flag = true;
+            }
+        }
+        std::vector<double> first;
+        if (any_flag) {
+            // explicit second projection of the flagged systems' candidates; the other systems
+            // keep their coefficients
+            first.assign(sn, sn + (c + 2) * Bp);
+            push_limits(B, flag.data());
+            double *u = V + (c + 1) * vec;
+            P.run(K_UPDATE, double(cnt + 3) * nb * 8.0, [&] {
+                bk::basis_update(u, V, cnt + 1, sd_next, n, Bp, lims, s);
+            });
+            P.run(K_DOT, double(cnt + 3) * nb * 8.0, [&] {
+                bk::multi_dot(V, cnt + 2, u, n, Bp, partial, np, lims, s);
                 bk::reduce_partials(partial, np, (cnt + 2) * Bp, sd_next, s);
             });
             HIPCHK(hipMemcpyAsync(host, hd, sl.total(Bp) * sizeof(double), hipMemcpyDeviceToHost, s));
             drain();
-            const double *t = host + sl.t(), *eta = host + sl.eta();
-            double *sn = host + sl.s(1 - which);
-            bool any_flag = false;
-            for (i64 j = 0; j < nsys; ++j) {
-                flag[j] = 0;
-                if (!mask_h[j]) continue;
-                double ss = 0.0;
-                for (i64 k = 0; k <= c; ++k) ss += sn[k * Bp + j] * sn[k * Bp + j];
-                if (!(ss <= 0.25 * sn[(c + 1) * Bp + j])) {
-                    flag[j] = 1;
-                    any_flag = true;
-                }
-            }
-            std::vector<double> first;
-            if (any_flag) {
-                // explicit second projection of the flagged systems' candidates; the other
-                // systems keep their coefficients
-                first.assign(sn, sn + (c + 2) * Bp);
-                for (i64 j = 0; j < Bp; ++j) cnt_h[j] = (j < nsys && flag[j]) ? 1 : 0;
-                push_ints(cnt_h, d_cnt);
-                double *u = V + (c + 1) * vec;
-                P.run(K_UPDATE, double(cnt + 3) * nb * 8.0, [&] {
-                    bk::basis_update(u, V, cnt + 1, sd_next, n, Bp, d_cnt, s);
-                });
-                P.run(K_DOT, double(cnt + 3) * nb * 8.0, [&] {
-                    bk::multi_dot(V, cnt + 2, u, n, Bp, partial, np, d_cnt, s);
-                    bk::reduce_partials(partial, np, (cnt + 2) * Bp, sd_next, s);
-                });
-                HIPCHK(hipMemcpyAsync(host, hd, sl.total(Bp) * sizeof(double), hipMemcpyDeviceToHost, s));
-                drain();
-                // keep the unflagged systems' first coefficients in the buffer the next step
-                // reads
-                for (i64 j = 0; j < Bp; ++j)
-                    if (!(j < nsys && flag[j]))
-                        for (i64 k = 0; k <= c + 1; ++k) sn[k * Bp + j] = first[k * Bp + j];
-                HIPCHK(hipMemcpyAsync(sd_next, sn, (c + 2) * Bp * sizeof(double), hipMemcpyHostToDevice, s));
-            }
-            for (i64 j = 0; j < nsys; ++j) {
-                if (!mask_h[j]) continue;
-                ++out[j].iterations;
-                for (i64 k = 0; k < c; ++k) col[k] = t[k * Bp + j] + sn[k * Bp + j];
-                col[c] = eta[j] + sn[c * Bp + j];
-                double ss = 0.0;
-                for (i64 k = 0; k <= c; ++k) ss += sn[k * Bp + j] * sn[k * Bp + j];
-                if (flag[j])
-                    for (i64 k = 0; k <= c; ++k) col[k] += first[k * Bp + j];
-                const double hn = std::sqrt(std::max(sn[(c + 1) * Bp + j] - ss, 0.0));
-                bj[j] = hn;
-                cols[j] = c + 1;
-                const double est = ls[j].add_column(c, col.data(), hn);
-                if (!(est > tol[j]) || !(hn > 0.0) || !std::isfinite(hn) ||
-                    out[j].iterations >= o.max_iterations)
-                    mask_h[j] = 0;
-            }
-            which = 1 - which;
+            for (i64 j = 0; j < Bp; ++j)
+                if (!flag[j])
+                    for (i64 k = 0; k <= c + 1; ++k) sn[k * Bp + j] = first[k * Bp + j];
+            HIPCHK(hipMemcpyAsync(sd_next, sn, (c + 2) * Bp * sizeof(double), hipMemcpyHostToDevice, s));
         }
-        // x_j += Z y_j over each system's columns
-        std::fill(yh, yh + m * Bp, 0.0);
-        for (i64 j = 0; j < Bp; ++j) cnt_h[j] = 0;
-        for (i64 j = 0; j < nsys; ++j) {
-            if (!active_h[j] || cols[j] == 0) continue;
-            ls[j].solve(cols[j]);
-            for (i64 k = 0; k < cols[j]; ++k) yh[k * Bp + j] = ls[j].y[k];
-            cnt_h[j] = static_cast<int>(cols[j]);
+        for (i64 j = 0; j < Bp; ++j) {
+            if (!mask[j]) continue;
+            auto &S = B.slot[j];
+            ++S.res.iterations;
+            for (i64 k = 0; k < c; ++k) col[k] = t[k * Bp + j] + sn[k * Bp + j];
+            col[c] = eta[j] + sn[c * Bp + j];
+            double ss = 0.0;
+            for (i64 k = 0; k <= c; ++k) ss += sn[k * Bp + j] * sn[k * Bp + j];
+            if (flag[j])
+                for (i64 k = 0; k <= c; ++k) col[k] += first[k * Bp + j];
+            const double hn = std::sqrt(std::max(sn[(c + 1) * Bp + j] - ss, 0.0));
+            S.bj = hn;
+            S.cols = c + 1;
+            const double est = B.ls[j].add_column(c, col.data(), hn);
+            if (!(est > S.tol) || !(hn > 0.0) || !std::isfinite(hn) || S.res.iterations >= S.max_iterations)
+                mask[j] = 0;
         }
-        HIPCHK(hipMemcpyAsync(B.d_y.p, yh, m * Bp * sizeof(double), hipMemcpyHostToDevice, s));
-        push_ints(cnt_h, d_cnt);
-        P.run(K_OTHER, nb * (m * sizeof(ZT) + 16.0), [&] {
-            bk::combine<ZT>(x, Z, d_cnt, dp<double>(B.d_y), n, Bp, s);
-        });
-        true_residual();
-        for (i64 j = 0; j < nsys; ++j) {
-            if (!active_h[j]) continue;
-            out[j].residual = beta[j];
-            if (!std::isfinite(beta[j])) {
-                active_h[j] = 0;
-            } else if (!(beta[j] > tol[j])) {
-                out[j].converged = 1;
-                active_h[j] = 0;
-            } else if (out[j].iterations >= o.max_iterations) {
-                active_h[j] = 0;
-            }
-        }
+        which = 1 - which;
     }
+    // x_j += Z y_j over each system's columns
+    std::fill(yh, yh + m * Bp, 0.0);
+    for (i64 j = 0; j < Bp; ++j) {
+        cnt_h[j] = 0;
+        auto &S = B.slot[j];
+        if (!iterated[j] || S.cols == 0) continue;
+        B.ls[j].solve(S.cols);
+        for (i64 k = 0; k < S.cols; ++k) yh[k * Bp + j] = B.ls[j].y[k];
+        cnt_h[j] = static_cast<int>(S.cols);
+    }
+    HIPCHK(hipMemcpyAsync(B.d_y.p, yh, m * Bp * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(B.d_cnt.p, cnt_h, Bp * sizeof(int), hipMemcpyHostToDevice, s));
+    P.run(K_OTHER, nb * (m * sizeof(ZT) + 16.0), [&] {
+        bk::combine<ZT>(dp<double>(B.d_x), Z, dp<int>(B.d_cnt), dp<double>(B.d_y), n, Bp, s);
+    });
+    true_residual(B, iterated.data());
+    bool still = false;
+    for (i64 j = 0; j < Bp; ++j) {
+        if (!iterated[j]) continue;
+        settle(B.slot[j]);
+        still = still || B.slot[j].active;
+    }
+    return still;
+}
+
+bool engine_step(mk_batch_s &B) {
+    if (B.mg->precision == MK_FP32) return step<float>(B);
+    return step<double>(B);
+}
+
+// zero the tails (rows past each lane's rows) of the solution, the right-hand side and every
+// preconditioned vector on the lanes with want[j] whose system is shorter than the box, so the
+// operator's gathers across a lane's last sector read zeros (the kernels never write there)
+void clear_lane_tails(mk_batch_s &B, const std::vector<char> &want) {
+    const hipStream_t s = stream();
+    const i64 n = B.n, Bp = B.Bp;
+    int *rows_h = B.ihost + Bp, *sel_h = B.ihost + 2 * Bp;
+    bool any = false;
+    for (i64 j = 0; j < Bp; ++j) {
+        rows_h[j] = static_cast<int>(B.slot[j].rows);
+        sel_h[j] = want[j] && B.slot[j].rows > 0 && B.slot[j].rows < n ? 1 : 0;
+        any = any || sel_h[j];
+    }
+    if (!any) return;
+    HIPCHK(hipMemcpyAsync(B.d_rows.p, rows_h, Bp * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(B.d_sel.p, sel_h, Bp * sizeof(int), hipMemcpyHostToDevice, s));
+    const int *rows = dp<int>(B.d_rows), *sel = dp<int>(B.d_sel);
+    bk::clear_tails<double>(dp<double>(B.d_x), 1, n * Bp, n, Bp, rows, sel, s);
+    bk::clear_tails<double>(dp<double>(B.d_b), 1, n * Bp, n, Bp, rows, sel, s);
+    if (B.mg->precision == MK_FP32) {
+        bk::clear_tails<float>(dp<float>(B.Z), static_cast<int>(B.m), n * Bp, n, Bp, rows, sel, s);
+    } else {
+        bk::clear_tails<double>(dp<double>(B.Z), static_cast<int>(B.m), n * Bp, n, Bp, rows, sel, s);
+    }
+    check_launch("batch: lane tails");
+    drain();
+}
+
+mk_batch_s *create_engine(mk_generator_s *gen, mk_sector_mg_s *mg, i64 slots) {
+    require(gen && mg && slots > 0, "batch: bad arguments");
+    require(mg->rows == gen->n, "batch: the preconditioner does not match the operator");
+    require(mg->opt.smoother == MK_JACOBI || mg->opt.smoother == MK_RED_BLACK,
+            "batch: the Jacobi and red-black smoothers are batched");
+    const bool fp32 = mg->precision == MK_FP32;
+    const auto &L0 = fp32 ? mg->L32[0].rb : mg->L64[0].rb;
+    if (L0 && (mg->opt.pre_sweeps < 1 || mg->opt.post_sweeps < 1))
+        throw std::invalid_argument("batch: red-black batches need pre- and post-smoothing sweeps");
+    auto B = std::make_unique<mk_batch_s>();
+    B->gen = gen;
+    B->mg = mg;
+    B->n = gen->n;
+    B->Bp = (slots + bk::kLanes - 1) / bk::kLanes * bk::kLanes;
+    B->levels = static_cast<int>(mg->max_levels);
+    B->slot.assign(B->Bp, slot_state{});
+    B->ls.assign(B->Bp, hessenberg_ls(1));
+    B->lim.assign(B->levels, std::vector<int>(B->Bp, 0));
+    i64 end = 0;
+    for (i64 k = 0; k < static_cast<i64>(mg->sector_rows[0].size()); ++k) {
+        end += mg->sector_rows[0][k];
+        B->sector_end.push_back(end);
+    }
+    require(end == B->n, "batch: the hierarchy does not cover the box");
+    B->d_shift = upload(std::vector<double>(B->Bp, 1.0));
+    std::vector<int> sidx(B->Bp);
+    for (i64 j = 0; j < B->Bp; ++j) sidx[j] = static_cast<int>(j);
+    B->d_sidx = upload(sidx);
+    B->d_lim = dbuf(static_cast<std::size_t>(std::max(B->levels, 1)) * B->Bp * sizeof(int));
+    B->d_rows = dbuf(B->Bp * sizeof(int));
+    B->d_sel = dbuf(B->Bp * sizeof(int));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&B->h_lim),
+                         static_cast<std::size_t>(std::max(B->levels, 1)) * B->Bp * sizeof(int)));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&B->ihost), 3 * B->Bp * sizeof(int)));
+    const std::size_t vb = static_cast<std::size_t>(std::max<i64>(B->n, 1)) * B->Bp * sizeof(double);
+    B->d_b = dbuf(vb);
+    B->d_x = dbuf(vb);
+    if (fp32) {
+        build_blevels<float>(*B);
+    } else {
+        build_blevels<double>(*B);
+    }
+    return B.release();
 }
 
 void batch_solve_device(mk_batch_s &B, const double *d_B, double *d_X, const double *tolerances,
                         const mk_gmres_options &o, mk_solver_result *results) {
     require(o.max_iterations >= 0 && o.restart >= 1, "batch gmres: bad options");
-    const i64 n = B.n, Bp = B.Bp;
-    const std::size_t vb = static_cast<std::size_t>(std::max<i64>(n, 1)) * Bp * sizeof(double);
-    if (!B.d_b.p || B.d_b.bytes < vb) B.d_b = dbuf(vb);
-    if (!B.d_x.p || B.d_x.bytes < vb) B.d_x = dbuf(vb);
-    std::vector<double> tol(B.nsys, o.tolerance);
-    if (tolerances) std::copy(tolerances, tolerances + B.nsys, tol.begin());
-    auto &P = prof();
-    P.run(K_OTHER, double(n) * B.nsys * 16.0, [&] {
-        bk::pack(d_B, n, B.nsys, dp<double>(B.d_b), Bp, stream());
-        bk::pack(d_X, n, B.nsys, dp<double>(B.d_x), Bp, stream());
-    });
-    if (B.mg->precision == MK_FP32) {
-        bgmres<float>(B, dp<double>(B.d_b), dp<double>(B.d_x), tol, o, results);
-    } else {
-        bgmres<double>(B, dp<double>(B.d_b), dp<double>(B.d_x), tol, o, results);
+    const i64 m = std::max<i64>(1, std::min<i64>(o.restart, B.n));
+    require(m <= gpu::kFusedBasis, "batch gmres: restart above 64 is not supported");
+    ensure_workspace(B, m);
+    for (i64 j = 0; j < B.Bp; ++j) {
+        B.ls[j] = hessenberg_ls(m);
+        auto &S = B.slot[j];
+        S.tol = j < B.nsys ? (tolerances ? tolerances[j] : o.tolerance) : 0.0;
+        S.max_iterations = o.max_iterations;
+        S.fresh = j < B.nsys;
+        S.active = S.done = false;
     }
-    P.run(K_OTHER, double(n) * B.nsys * 16.0,
-          [&] { bk::unpack(dp<double>(B.d_x), n, B.nsys, d_X, Bp, stream()); });
+    auto &P = prof();
+    P.run(K_OTHER, double(B.n) * B.nsys * 16.0, [&] {
+        bk::pack(d_B, B.n, B.nsys, dp<double>(B.d_b), B.Bp, stream());
+        bk::pack(d_X, B.n, B.nsys, dp<double>(B.d_x), B.Bp, stream());
+    });
+    clear_lane_tails(B, std::vector<char>(B.Bp, 1));
+    while (engine_step(B)) {
+    }
+    for (i64 j = 0; j < B.nsys; ++j) results[j] = B.slot[j].res;
+    P.run(K_OTHER, double(B.n) * B.nsys * 16.0,
+          [&] { bk::unpack(dp<double>(B.d_x), B.n, B.nsys, d_X, B.Bp, stream()); });
     drain();
 }
 
 } // namespace
 
+// ---- continuous batching for the request server (server.hip) -----------------------------
+namespace mk::rt {
+
+mk_batch_s *batch_engine_create(mk_generator_s *gen, mk_sector_mg_s *mg, i64 slots, i64 restart) {
+    auto *B = create_engine(gen, mg, slots);
+    try {
+        const i64 m = std::max<i64>(1, std::min<i64>(restart, B->n));
+        require(m <= gpu::kFusedBasis, "batch gmres: restart above 64 is not supported");
+        ensure_workspace(*B, m);
+        for (auto &l : B->ls) l = hessenberg_ls(m);
+    } catch (...) {
+        delete B;
+        throw;
+    }
+    return B;
+}
+
+void batch_engine_destroy(mk_batch_s *B) { delete B; }
+
+i64 batch_engine_slots(const mk_batch_s &B) { return B.Bp; }
+
+// admit a system to a free slot: b (host, `rows` values), x = 0, its shift, tolerance and
+// iteration limit; it takes its initial residual at the next step
+void batch_engine_admit(mk_batch_s &B, i64 j, const double *b, i64 rows, double shift, double tol,
+                        i64 max_iterations) {
+    require(j >= 0 && j < B.Bp && B.slot[j].rows == 0, "batch: the slot is not free");
+    set_slot(B, j, shift, rows);
+    auto &S = B.slot[j];
+    S.tol = tol;
+    S.max_iterations = max_iterations;
+    S.fresh = true;
+    S.active = S.done = false;
+    S.res = mk_solver_result{0, 0.0, 0};
+    const hipStream_t s = stream();
+    if (!B.d_io.p || B.d_io.bytes < static_cast<std::size_t>(B.n) * sizeof(double))
+        B.d_io = dbuf(static_cast<std::size_t>(std::max<i64>(B.n, 1)) * sizeof(double));
+    HIPCHK(hipMemcpyAsync(B.d_io.p, b, rows * sizeof(double), hipMemcpyHostToDevice, s));
+    bk::put_lane(dp<double>(B.d_io), rows, dp<double>(B.d_b), B.n, B.Bp, j, s);
+    bk::put_lane(dp<double>(B.d_io), 0, dp<double>(B.d_x), B.n, B.Bp, j, s); // x = 0
+    std::vector<char> want(B.Bp, 0);
+    want[j] = 1;
+    clear_lane_tails(B, want);
+    drain();
+}
+
+bool batch_engine_step(mk_batch_s &B) { return engine_step(B); }
+
+// a finished slot's result and solution (`rows` values, host); the slot becomes free
+bool batch_engine_take(mk_batch_s &B, i64 j, double *x, mk_solver_result *res) {
+    auto &S = B.slot[j];
+    if (S.rows == 0 || !S.done) return false;
+    const hipStream_t s = stream();
+    bk::get_lane(dp<double>(B.d_x), S.rows, B.Bp, j, dp<double>(B.d_io), s);
+    HIPCHK(hipMemcpyAsync(x, B.d_io.p, S.rows * sizeof(double), hipMemcpyDeviceToHost, s));
+    drain();
+    *res = S.res;
+    S = slot_state{};
+    return true;
+}
+
+} // namespace mk::rt
+
 extern "C" {
+
+mk_status mk_batch_create_sized(mk_generator gen, mk_sector_mg mg, int64_t n_systems,
+                                const double *shifts, const int64_t *rows, mk_batch *out) {
+    return guarded([&] {
+        require(gen && mg && shifts && out && n_systems > 0, "batch: bad arguments");
+        std::unique_ptr<mk_batch_s> B(create_engine(gen, mg, n_systems));
+        B->nsys = n_systems;
+        for (i64 j = 0; j < n_systems; ++j) set_slot(*B, j, shifts[j], rows ? rows[j] : B->n);
+        *out = B.release();
+    });
+}
 
 mk_status mk_batch_create(mk_generator gen, mk_sector_mg mg, int64_t n_systems,
                           const double *shifts, mk_batch *out) {
-    return guarded([&] {
-        require(gen && mg && shifts && out && n_systems > 0, "batch: bad arguments");
-        require(mg->rows == gen->n, "batch: the preconditioner does not match the operator");
-        require(mg->opt.smoother == MK_JACOBI, "batch: only the Jacobi smoother is batched");
-        for (i64 j = 0; j < n_systems; ++j)
-            if (!(shifts[j] > 0.0)) throw std::invalid_argument("population_multigrid: shift must be positive");
-        auto B = std::make_unique<mk_batch_s>();
-        B->gen = gen;
-        B->mg = mg;
-        B->n = gen->n;
-        B->nsys = n_systems;
-        B->Bp = (n_systems + bk::kLanes - 1) / bk::kLanes * bk::kLanes;
-        B->shifts.assign(B->Bp, 1.0);
-        std::copy(shifts, shifts + n_systems, B->shifts.begin());
-        std::vector<double> distinct;
-        std::vector<int> sidx(B->Bp, 0);
-        for (i64 j = 0; j < n_systems; ++j) {
-            auto it = std::find(distinct.begin(), distinct.end(), shifts[j]);
-            if (it == distinct.end()) {
-                distinct.push_back(shifts[j]);
-                it = distinct.end() - 1;
-            }
-            sidx[j] = static_cast<int>(it - distinct.begin());
-        }
-        B->n_shift = static_cast<i64>(distinct.size());
-        B->d_shift = upload(B->shifts);
-        B->d_sidx = upload(sidx);
-        if (mg->precision == MK_FP32) {
-            build_blevels<float>(*B, mg->L32, B->B32, distinct);
-        } else {
-            build_blevels<double>(*B, mg->L64, B->B64, distinct);
-        }
-        *out = B.release();
-    });
+    return mk_batch_create_sized(gen, mg, n_systems, shifts, nullptr, out);
 }
 
 mk_status mk_batch_destroy(mk_batch b) {
@@ -511,6 +826,11 @@ mk_status mk_batch_solve(mk_batch b, const double *B, double *X, const double *t
         batch_solve_device(*b, dB, dX, tolerances, *options, results);
         HIPCHK(hipMemcpy(X, dX, bytes, hipMemcpyDeviceToHost));
     });
+}
+
+mk_status mk_batch_graphs_enable(int32_t on) {
+    batch_graph_switch().store(on ? 1 : 0);
+    return MK_OK;
 }
 
 } // extern "C"

@@ -15,8 +15,11 @@ constexpr int kMaxParts = 512;
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
-enum class bmode { shifted, shifted_residual, sweep, residual, gather_sum };
+enum class bmode { shifted, shifted_residual, sweep, residual, gather_sum, gs };
 
+// limit (when not null): lane j works on the rows r < limit[j] of the operator's level (0: an
+// inactive system). Rows are the level's rows: a row-mapped operator (A.rowmap) maps its row q
+// to level row rowmap[q], which indexes the vectors and the limit.
 template <typename VT, typename XT, typename T>
 struct bop_args {
     csr_view<VT> A;
@@ -27,7 +30,11 @@ struct bop_args {
     T *out;
     double *partial;
     i64 Bp;
-    const int *active;
+    const int *limit;
+    T *zero = nullptr;      // gs: rows also cleared in this vector (the residual after the last
+                            // pre-smoothing half-sweep)
+    T *nat = nullptr;       // gs: rows also stored to nat[perm[row]] (the caller's order)
+    const u32 *perm = nullptr;
 };
 
 // sum over the row's entries of rate * x[column][j], ascending entry order; row r is
@@ -84,16 +91,23 @@ __global__ void __launch_bounds__(kThreads) k_bop(bop_args<VT, XT, T> a) {
     __shared__ double red[kWaves][kLanes];
     const int lane = threadIdx.x & 63, wave = wave_id();
     const i64 j = static_cast<i64>(blockIdx.y) * kLanes + lane;
-    const bool on = !a.active || a.active[j] != 0;
     const i64 r = static_cast<i64>(blockIdx.x) * kWaves + wave;
     double sq = 0.0;
     if (r < a.A.n) {
+        const i64 row = a.A.rowmap ? static_cast<i64>(a.A.rowmap[r]) : r;
+        const bool on = !a.limit || row < a.limit[j];
         T ex = 0;
         const T sum = row_gather<VT, XT, T>(a.A, r, a.x, a.Bp, j, on, ex);
         if (on) {
-            const i64 o = r * a.Bp + j;
+            const i64 o = row * a.Bp + j;
             if constexpr (M == bmode::gather_sum) {
                 a.out[o] = sum;
+            } else if constexpr (M == bmode::gs) {
+                // x_row = (rhs + sum over the other colour) / (s_j + exit), in place
+                const T v = (a.rhs[o] + sum) / (static_cast<T>(a.shifts[j]) + ex);
+                a.out[o] = v;
+                if (a.zero) a.zero[o] = T(0);
+                if (a.nat) a.nat[static_cast<i64>(a.perm[row]) * a.Bp + j] = v;
             } else {
                 const T xr = static_cast<T>(a.x[o]);
                 const T d = static_cast<T>(a.shifts[j]) + ex;
@@ -154,10 +168,10 @@ __device__ __forceinline__ void row_lane(i64 &r, i64 &j) {
 template <typename T, typename IN>
 __global__ void __launch_bounds__(kThreads)
     k_jacobi_first(csr_view<T> A, const double *shifts, T omega, const IN *in,
-                   const double *scale, T *rhs_out, T *x, i64 Bp, const int *active) {
+                   const double *scale, T *rhs_out, T *x, i64 Bp, const int *limit) {
     i64 r, j;
     row_lane(r, j);
-    if (r >= A.n || (active && !active[j])) return;
+    if (r >= A.n || (limit && r >= limit[j])) return;
     const i64 o = r * Bp + j;
     const IN sc = scale ? static_cast<IN>(scale[j]) : IN(1);
     const T b = static_cast<T>(in[o] * sc);
@@ -165,13 +179,34 @@ __global__ void __launch_bounds__(kThreads)
     x[o] = omega * b / (static_cast<T>(shifts[j]) + row_exit(A, r));
 }
 
+// the first red-black half-sweep from x = 0: rhs[i] = (T)(scale_j in[perm[i]]) (perm null:
+// identity), stored to rhs_out if given; x[i] = rhs[i] / (s_j + exit_i) on colour 0 (colour-1
+// entries are set by the next half-sweep)
+template <typename T, typename IN>
+__global__ void __launch_bounds__(kThreads)
+    k_rb_first(csr_view<T> A, const u8 *colour, const double *shifts, const IN *in,
+               const double *scale, T *rhs_out, T *x, const u32 *perm, i64 Bp, const int *limit) {
+    i64 r, j;
+    row_lane(r, j);
+    if (r >= A.n || (limit && r >= limit[j])) return;
+    const i64 o = r * Bp + j;
+    const i64 src = perm ? static_cast<i64>(perm[r]) * Bp + j : o;
+    const IN sc = scale ? static_cast<IN>(scale[j]) : IN(1);
+    const T b = static_cast<T>(in[src] * sc);
+    if (rhs_out) rhs_out[o] = b;
+    if (colour[r] == 0) x[o] = b / (static_cast<T>(shifts[j]) + row_exit(A, r));
+}
+
+// x[i] += weight[i] coarse[coarse_of[i]] for the n rows i = rows[q] (all rows when rows is null)
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x, i64 Bp,
-                  const int *active) {
-    i64 i, j;
-    row_lane(i, j);
-    if (i >= n || (active && !active[j])) return;
+                  const int *limit, const u32 *rows) {
+    i64 q, j;
+    row_lane(q, j);
+    if (q >= n) return;
+    const i64 i = rows ? static_cast<i64>(rows[q]) : q;
+    if (limit && i >= limit[j]) return;
     const u32 c = coarse_of[i];
     if (c != 0xFFFFFFFFu) x[i * Bp + j] += weight[i] * coarse[static_cast<i64>(c) * Bp + j];
 }
@@ -180,17 +215,18 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_terminal(i64 nb, const u32 *start, const u32 *size, const u64 *offset, const T *inv,
                u64 per_shift, const int *shift_index, i64 max_size, const T *r, T *x, i64 Bp,
-               const int *active) {
+               const int *limit) {
     const i64 total = nb * max_size * Bp;
     for (i64 g = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; g < total;
          g += static_cast<i64>(gridDim.x) * kThreads) {
         const i64 j = g % Bp, bi = g / Bp;
         const i64 b = bi / max_size, i = bi % max_size;
         const u32 m = size[b];
-        if (i >= m || (active && !active[j])) continue;
+        const i64 s0 = start[b];
+        // a block lies in one sector: it is inside a lane's rows or past them
+        if (i >= m || (limit && s0 >= limit[j])) continue;
         const T *row = inv + static_cast<u64>(shift_index[j]) * per_shift + offset[b] +
                        static_cast<u64>(i) * m;
-        const i64 s0 = start[b];
         T acc = 0;
         for (u32 c = 0; c < m; ++c) acc += row[c] * r[(s0 + c) * Bp + j];
         x[(s0 + i) * Bp + j] += acc;
@@ -201,20 +237,20 @@ __global__ void __launch_bounds__(kThreads)
 // grid (np, chunks of 8 basis vectors, Bp / 64)
 __global__ void __launch_bounds__(kThreads)
     k_multi_dot(const double *V, int count, const double *w, i64 n, i64 Bp, double *partial,
-                int np, const int *active) {
+                int np, const int *limit) {
     __shared__ double red[kWaves][kDotChunk][kLanes];
     const int lane = threadIdx.x & 63, wave = wave_id();
     const i64 j = static_cast<i64>(blockIdx.z) * kLanes + lane;
-    const bool on = !active || active[j] != 0;
+    const i64 rows = limit ? min(n, static_cast<i64>(limit[j])) : n;
     const int k0 = blockIdx.y * kDotChunk;
     const int kc = min(kDotChunk, count - k0);
     const i64 ld = n * Bp;
     double acc[kDotChunk];
 #pragma unroll
     for (int k = 0; k < kDotChunk; ++k) acc[k] = 0.0;
-    if (on) {
+    {
         const i64 stride = static_cast<i64>(np) * kWaves;
-        for (i64 r = static_cast<i64>(blockIdx.x) * kWaves + wave; r < n; r += stride) {
+        for (i64 r = static_cast<i64>(blockIdx.x) * kWaves + wave; r < rows; r += stride) {
             const i64 o = r * Bp + j;
             const double wr = w[o];
 #pragma unroll
@@ -270,14 +306,15 @@ template <int PW>
 __global__ void __launch_bounds__(kThreads)
     k_dcgs_update(double *V, int count, const double *sc, const double *tg, const double *beta,
                   const double *w, i64 n, i64 Bp, double *partial, int np, double *eta_out,
-                  const int *active) {
+                  const int *limit) {
     __shared__ double part[2][kWaves][kLanes];
     __shared__ double uw[2][kLanes];
     __shared__ double red[2][kLanes];
     constexpr int kSlots = PW > 0 ? PW : 1;
     const int lane = threadIdx.x & 63, wave = wave_id();
     const i64 j = static_cast<i64>(blockIdx.y) * kLanes + lane;
-    const bool on = !active || active[j] != 0;
+    const i64 rows = limit ? min(n, static_cast<i64>(limit[j])) : n;
+    const bool on = rows > 0;
     const i64 ld = n * Bp;
     double cs[kSlots], ct[kSlots], acc[kSlots];
     const double *src[kSlots];
@@ -301,11 +338,13 @@ __global__ void __launch_bounds__(kThreads)
     double *const q_out = V + static_cast<i64>(count) * ld;
     double *const u_out = V + static_cast<i64>(count + 1) * ld;
     double own = 0.0; // wave 2: q . u', wave 3: u' . u'
+    // every wave runs every row (the loop holds barriers); lanes past their rows compute zeros
     for (i64 r = blockIdx.x; r < n; r += np) {
         const i64 o = r * Bp + j;
+        const bool live = r < rows;
         double v[kSlots];
 #pragma unroll
-        for (int kk = 0; kk < PW; ++kk) v[kk] = on ? src[kk][o] : 0.0;
+        for (int kk = 0; kk < PW; ++kk) v[kk] = live ? src[kk][o] : 0.0;
         double a = 0.0, b = 0.0;
 #pragma unroll
         for (int kk = 0; kk < PW; ++kk) {
@@ -314,14 +353,14 @@ __global__ void __launch_bounds__(kThreads)
         }
         part[0][wave][lane] = a;
         part[1][wave][lane] = b;
-        if (wave < 2) uw[wave][lane] = on ? (wave == 0 ? q_out[o] : w[o]) : 0.0;
+        if (wave < 2) uw[wave][lane] = live ? (wave == 0 ? q_out[o] : w[o]) : 0.0;
         __syncthreads();
         const double S = ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) + part[0][3][lane];
         const double T = ((part[1][0][lane] + part[1][1][lane]) + part[1][2][lane]) + part[1][3][lane];
-        const double q = on ? (uw[0][lane] - S) * ib : 0.0;
-        const double up = on ? (uw[1][lane] - T) - eta * q : 0.0;
+        const double q = live ? (uw[0][lane] - S) * ib : 0.0;
+        const double up = live ? (uw[1][lane] - T) - eta * q : 0.0;
         __syncthreads();
-        if (on) {
+        if (live) {
             if (wave == 0) q_out[o] = q;
             if (wave == 1) u_out[o] = up;
         }
@@ -347,10 +386,10 @@ __global__ void __launch_bounds__(kThreads)
 
 __global__ void __launch_bounds__(kThreads)
     k_basis_update(double *w, const double *V, int count, const double *h, i64 n, i64 Bp,
-                   const int *active) {
+                   const int *limit) {
     i64 r, j;
     row_lane(r, j);
-    if (r >= n || (active && !active[j])) return;
+    if (r >= n || (limit && r >= limit[j])) return;
     const i64 o = r * Bp + j, ld = n * Bp;
     double wi = w[o];
     for (int k = 0; k < count; ++k) wi -= h[static_cast<i64>(k) * Bp + j] * V[k * ld + o];
@@ -408,7 +447,53 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// dst[r][j] = r < rows ? src[r] : 0 for one lane j
+__global__ void __launch_bounds__(kThreads)
+    k_put_lane(const double *src, i64 rows, double *dst, i64 n, i64 Bp, i64 j) {
+    for (i64 r = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; r < n;
+         r += static_cast<i64>(gridDim.x) * kThreads)
+        dst[r * Bp + j] = r < rows ? src[r] : 0.0;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_get_lane(const double *src, i64 rows, i64 Bp, i64 j, double *dst) {
+    for (i64 r = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; r < rows;
+         r += static_cast<i64>(gridDim.x) * kThreads)
+        dst[r] = src[r * Bp + j];
+}
+
+// X_k[r][j] = 0 for r >= rows[j], k < count (vectors ld apart), lanes with sel[j] != 0 (all when
+// sel is null); grid (ceil(n / 4), Bp / 64)
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_clear_tails(T *X, int count, i64 ld, i64 n, i64 Bp, const int *rows, const int *sel) {
+    i64 r, j;
+    row_lane(r, j);
+    if (r >= n || r < rows[j] || (sel && !sel[j])) return;
+    for (int k = 0; k < count; ++k) X[k * ld + r * Bp + j] = T(0);
+}
+
 } // namespace
+
+void put_lane(const double *src, i64 rows, double *dst, i64 n, i64 Bp, i64 j, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_put_lane, dim3(grid_for(n)), dim3(kThreads), 0, s, src, rows, dst, n, Bp, j);
+}
+
+void get_lane(const double *src, i64 rows, i64 Bp, i64 j, double *dst, hipStream_t s) {
+    if (rows == 0) return;
+    hipLaunchKernelGGL(k_get_lane, dim3(grid_for(rows)), dim3(kThreads), 0, s, src, rows, Bp, j, dst);
+}
+
+template <typename T>
+void clear_tails(T *X, int count, i64 ld, i64 n, i64 Bp, const int *rows, const int *sel,
+                 hipStream_t s) {
+    if (n == 0 || count <= 0) return;
+    hipLaunchKernelGGL((k_clear_tails<T>), dim3(static_cast<u32>(row_blocks(n)), Bp / kLanes),
+                       dim3(kThreads), 0, s, X, count, ld, n, Bp, rows, sel);
+}
+template void clear_tails<float>(float *, int, i64, i64, i64, const int *, const int *, hipStream_t);
+template void clear_tails<double>(double *, int, i64, i64, i64, const int *, const int *, hipStream_t);
 
 int parts(i64 n) { return op_parts(n); }
 
@@ -416,8 +501,8 @@ i64 residual_parts(i64 n) { return row_blocks(n); }
 
 template <typename XT>
 void shifted_apply(const csr_view<double> &A, const double *shifts, const XT *x, const double *b,
-                   double *y, double *partial, i64 Bp, const int *active, hipStream_t s) {
-    const bop_args<double, XT, double> a{A, shifts, 0.0, x, b, y, partial, Bp, active};
+                   double *y, double *partial, i64 Bp, const int *limit, hipStream_t s) {
+    const bop_args<double, XT, double> a{A, shifts, 0.0, x, b, y, partial, Bp, limit};
     if (b) {
         launch<bmode::shifted_residual>(a, s);
     } else {
@@ -427,56 +512,75 @@ void shifted_apply(const csr_view<double> &A, const double *shifts, const XT *x,
 
 template <typename T, typename IN>
 void jacobi_first(const csr_view<T> &A, const double *shifts, double omega, const IN *in,
-                  const double *scale, T *rhs_out, T *x, i64 Bp, const int *active,
+                  const double *scale, T *rhs_out, T *x, i64 Bp, const int *limit,
                   hipStream_t s) {
     if (A.n == 0) return;
     hipLaunchKernelGGL((k_jacobi_first<T, IN>), dim3(static_cast<u32>(row_blocks(A.n)), Bp / kLanes),
                        dim3(kThreads), 0, s, A,
-                       shifts, static_cast<T>(omega), in, scale, rhs_out, x, Bp, active);
+                       shifts, static_cast<T>(omega), in, scale, rhs_out, x, Bp, limit);
 }
 
 template <typename T>
 void jacobi_sweep(const csr_view<T> &A, const double *shifts, double omega, const T *rhs,
-                  const T *x, T *x_out, i64 Bp, const int *active, hipStream_t s) {
+                  const T *x, T *x_out, i64 Bp, const int *limit, hipStream_t s) {
     launch<bmode::sweep>(bop_args<T, T, T>{A, shifts, static_cast<T>(omega), x, rhs, x_out, nullptr,
-                                           Bp, active}, s);
+                                           Bp, limit}, s);
 }
 
 template <typename T>
 void level_residual(const csr_view<T> &A, const double *shifts, const T *rhs, const T *x, T *r,
-                    i64 Bp, const int *active, hipStream_t s) {
-    launch<bmode::residual>(bop_args<T, T, T>{A, shifts, T(0), x, rhs, r, nullptr, Bp, active}, s);
+                    i64 Bp, const int *limit, hipStream_t s) {
+    launch<bmode::residual>(bop_args<T, T, T>{A, shifts, T(0), x, rhs, r, nullptr, Bp, limit}, s);
 }
 
 template <typename T>
-void gather_sum(const csr_view<T> &R, const T *x, T *y, i64 Bp, const int *active, hipStream_t s) {
-    launch<bmode::gather_sum>(bop_args<T, T, T>{R, nullptr, T(0), x, nullptr, y, nullptr, Bp, active}, s);
+void gather_sum(const csr_view<T> &R, const T *x, T *y, i64 Bp, const int *limit, hipStream_t s) {
+    launch<bmode::gather_sum>(bop_args<T, T, T>{R, nullptr, T(0), x, nullptr, y, nullptr, Bp, limit}, s);
 }
 
 template <typename T>
 void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x, i64 Bp,
-                 const int *active, hipStream_t s) {
+                 const int *limit, hipStream_t s, const u32 *rows) {
     if (n == 0) return;
     hipLaunchKernelGGL((k_prolong_add<T>), dim3(static_cast<u32>(row_blocks(n)), Bp / kLanes),
                        dim3(kThreads), 0, s, n,
-                       coarse_of, weight, coarse, x, Bp, active);
+                       coarse_of, weight, coarse, x, Bp, limit, rows);
+}
+
+template <typename T, typename IN>
+void rb_first(const csr_view<T> &A, const u8 *colour, const double *shifts, const IN *in,
+              const double *scale, T *rhs_out, T *x, const u32 *perm, i64 Bp, const int *limit,
+              hipStream_t s) {
+    if (A.n == 0) return;
+    hipLaunchKernelGGL((k_rb_first<T, IN>), dim3(static_cast<u32>(row_blocks(A.n)), Bp / kLanes),
+                       dim3(kThreads), 0, s, A, colour, shifts, in, scale, rhs_out, x, perm, Bp, limit);
+}
+
+template <typename T>
+void gs_half(const csr_view<T> &Ac, const double *shifts, const T *rhs, T *x, T *res_zero, i64 Bp,
+             const int *limit, hipStream_t s, T *nat, const u32 *perm) {
+    bop_args<T, T, T> a{Ac, shifts, T(0), x, rhs, x, nullptr, Bp, limit};
+    a.zero = res_zero;
+    a.nat = nat;
+    a.perm = perm;
+    launch<bmode::gs>(a, s);
 }
 
 template <typename T>
 void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
                     const T *inverse, u64 per_shift, const int *shift_index, i64 max_size,
-                    const T *r, T *x, i64 Bp, const int *active, hipStream_t s) {
+                    const T *r, T *x, i64 Bp, const int *limit, hipStream_t s) {
     if (n_blocks == 0) return;
     hipLaunchKernelGGL((k_terminal<T>), dim3(grid_for(n_blocks * max_size * Bp)), dim3(kThreads), 0,
                        s, n_blocks, start, size, offset, inverse, per_shift, shift_index, max_size,
-                       r, x, Bp, active);
+                       r, x, Bp, limit);
 }
 
 void multi_dot(const double *V, int count, const double *w, i64 n, i64 Bp, double *partial,
-               int np, const int *active, hipStream_t s) {
+               int np, const int *limit, hipStream_t s) {
     if (count <= 0 || n == 0) return;
     hipLaunchKernelGGL(k_multi_dot, dim3(np, (count + kDotChunk - 1) / kDotChunk, Bp / kLanes),
-                       dim3(kThreads), 0, s, V, count, w, n, Bp, partial, np, active);
+                       dim3(kThreads), 0, s, V, count, w, n, Bp, partial, np, limit);
 }
 
 void reduce_partials(const double *partial, int np, i64 m, double *out, hipStream_t s,
@@ -496,14 +600,14 @@ i64 reduce_tmp_size(int np, i64 m) { return np > kGroup ? (np + kGroup - 1) / kG
 
 void dcgs_update(double *V, int count, const double *sc, const double *tg, const double *beta,
                  const double *w, i64 n, i64 Bp, double *partial, int np, double *eta,
-                 const int *active, hipStream_t s) {
+                 const int *limit, hipStream_t s) {
     if (count < 0 || count >= kFusedBasis || n == 0) return;
     const dim3 grid(np, Bp / kLanes);
     switch ((count + kWaves - 1) / kWaves) {
 #define MK_BDCGS(P)                                                                              \
     case P:                                                                                      \
         hipLaunchKernelGGL(k_dcgs_update<P>, grid, dim3(kThreads), 0, s, V, count, sc, tg, beta, \
-                           w, n, Bp, partial, np, eta, active);                                  \
+                           w, n, Bp, partial, np, eta, limit);                                  \
         break;
         MK_BDCGS(0) MK_BDCGS(1) MK_BDCGS(2) MK_BDCGS(3) MK_BDCGS(4) MK_BDCGS(5) MK_BDCGS(6)
         MK_BDCGS(7) MK_BDCGS(8) MK_BDCGS(9) MK_BDCGS(10) MK_BDCGS(11) MK_BDCGS(12)
@@ -513,11 +617,11 @@ void dcgs_update(double *V, int count, const double *sc, const double *tg, const
 }
 
 void basis_update(double *w, const double *V, int count, const double *h, i64 n, i64 Bp,
-                  const int *active, hipStream_t s) {
+                  const int *limit, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_basis_update, dim3(static_cast<u32>(row_blocks(n)), Bp / kLanes),
                        dim3(kThreads), 0, s, w, V, count,
-                       h, n, Bp, active);
+                       h, n, Bp, limit);
 }
 
 template <typename ZT>
@@ -558,7 +662,12 @@ template void shifted_apply<float>(const csr_view<double> &, const double *, con
     template void gather_sum<T>(const csr_view<T> &, const T *, T *, i64, const int *,            \
                                 hipStream_t);                                                     \
     template void prolong_add<T>(i64, const u32 *, const T *, const T *, T *, i64, const int *,   \
-                                 hipStream_t);                                                    \
+                                 hipStream_t, const u32 *);                                       \
+    template void rb_first<T, double>(const csr_view<T> &, const u8 *, const double *,            \
+                                      const double *, const double *, T *, T *, const u32 *, i64, \
+                                      const int *, hipStream_t);                                  \
+    template void gs_half<T>(const csr_view<T> &, const double *, const T *, T *, T *, i64,       \
+                             const int *, hipStream_t, T *, const u32 *);                         \
     template void terminal_solve<T>(i64, const u32 *, const u32 *, const u64 *, const T *, u64,   \
                                     const int *, i64, const T *, T *, i64, const int *,          \
                                     hipStream_t);                                                 \
@@ -567,6 +676,9 @@ template void shifted_apply<float>(const csr_view<double> &, const double *, con
 MK_BLEVEL(float)
 MK_BLEVEL(double)
 #undef MK_BLEVEL
+template void rb_first<float, float>(const csr_view<float> &, const u8 *, const double *,
+                                     const float *, const double *, float *, float *, const u32 *,
+                                     i64, const int *, hipStream_t);
 template void jacobi_first<float, float>(const csr_view<float> &, const double *, double,
                                          const float *, const double *, float *, float *, i64,
                                          const int *, hipStream_t);

@@ -8,8 +8,10 @@
 // access. The operators are the single-system ones (coded sliced or plain, see kernels.hpp);
 // only the diagonal shift varies with the system.
 //
-// active (when not null) holds one int per system: lanes of inactive systems load and store
-// nothing and contribute zero partial sums. Reductions run in a fixed order.
+// limit (when not null) holds one int per system: lane j works on the rows r < limit[j] of the
+// level the kernel runs on and loads and stores nothing past them (0: an inactive system), with
+// zero partial sums there. A system of fewer rows (a leading block of whole sectors) thus shares
+// the launch with larger ones. Reductions run in a fixed order.
 #pragma once
 
 #include "kernels.hpp"
@@ -29,37 +31,50 @@ i64 residual_parts(i64 n);
 //   partial[p * Bp + j] for p < residual_parts(n)
 template <typename XT>
 void shifted_apply(const csr_view<double> &A, const double *shifts, const XT *x, const double *b,
-                   double *y, double *partial, i64 Bp, const int *active, hipStream_t s);
+                   double *y, double *partial, i64 Bp, const int *limit, hipStream_t s);
 
 // multigrid level operations in precision T (see kernels.hpp for the single-system forms)
 //   first sweep from zero: rhs = (T)(in * scale_j) (stored to rhs_out if given),
 //   x = w rhs / (s_j + exit)
 template <typename T, typename IN>
 void jacobi_first(const csr_view<T> &A, const double *shifts, double omega, const IN *in,
-                  const double *scale, T *rhs_out, T *x, i64 Bp, const int *active,
+                  const double *scale, T *rhs_out, T *x, i64 Bp, const int *limit,
                   hipStream_t s);
 template <typename T>
 void jacobi_sweep(const csr_view<T> &A, const double *shifts, double omega, const T *rhs,
-                  const T *x, T *x_out, i64 Bp, const int *active, hipStream_t s);
+                  const T *x, T *x_out, i64 Bp, const int *limit, hipStream_t s);
 template <typename T>
 void level_residual(const csr_view<T> &A, const double *shifts, const T *rhs, const T *x, T *r,
-                    i64 Bp, const int *active, hipStream_t s);
+                    i64 Bp, const int *limit, hipStream_t s);
 template <typename T>
-void gather_sum(const csr_view<T> &R, const T *x, T *y, i64 Bp, const int *active, hipStream_t s);
+void gather_sum(const csr_view<T> &R, const T *x, T *y, i64 Bp, const int *limit, hipStream_t s);
+// x[i] += weight[i] coarse[coarse_of[i]] for the rows i = rows[q], q < n (all n rows when rows
+// is null)
 template <typename T>
 void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x, i64 Bp,
-                 const int *active, hipStream_t s);
+                 const int *limit, hipStream_t s, const u32 *rows = nullptr);
+// red-black Gauss-Seidel on a level numbered colour 0 first (see gs_half and rb_first in
+// kernels.hpp): the first half-sweep from x = 0, rhs taken from in[perm[i]] (scaled per system)
+template <typename T, typename IN>
+void rb_first(const csr_view<T> &A, const u8 *colour, const double *shifts, const IN *in,
+              const double *scale, T *rhs_out, T *x, const u32 *perm, i64 Bp, const int *limit,
+              hipStream_t s);
+// one colour's half-sweep in place over its row-mapped operator; res_zero rows cleared; nat
+// (with perm) receives the rows in the caller's order
+template <typename T>
+void gs_half(const csr_view<T> &Ac, const double *shifts, const T *rhs, T *x, T *res_zero, i64 Bp,
+             const int *limit, hipStream_t s, T *nat = nullptr, const u32 *perm = nullptr);
 // dense terminal blocks with one inverse set per distinct shift: system j uses
 // inverse + shift_index[j] * per_shift
 template <typename T>
 void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
                     const T *inverse, u64 per_shift, const int *shift_index, i64 max_size,
-                    const T *r, T *x, i64 Bp, const int *active, hipStream_t s);
+                    const T *r, T *x, i64 Bp, const int *limit, hipStream_t s);
 
 // ---- Krylov (FP64), count basis vectors V_k = V + k * n * Bp
 // partial[p * (count * Bp) + k * Bp + j] = sum over partition p's rows of V_k[i][j] w[i][j]
 void multi_dot(const double *V, int count, const double *w, i64 n, i64 Bp, double *partial,
-               int np, const int *active, hipStream_t s);
+               int np, const int *limit, hipStream_t s);
 // out[o] = sum_p partial[p * m + o] for o < m (fixed order); with tmp (reduce_tmp_size doubles)
 // a long reduction runs in two stages over groups of 256 partitions
 void reduce_partials(const double *partial, int np, i64 m, double *out, hipStream_t s,
@@ -71,14 +86,23 @@ i64 reduce_tmp_size(int np, i64 m);
 // partial[p * ((count + 2) * Bp) + k * Bp + j] for k <= count + 1 as in the single-system form
 void dcgs_update(double *V, int count, const double *sc, const double *tg, const double *beta,
                  const double *w, i64 n, i64 Bp, double *partial, int np, double *eta,
-                 const int *active, hipStream_t s);
+                 const int *limit, hipStream_t s);
 // w[i][j] -= sum_{k < count} h[k * Bp + j] V_k[i][j]
 void basis_update(double *w, const double *V, int count, const double *h, i64 n, i64 Bp,
-                  const int *active, hipStream_t s);
+                  const int *limit, hipStream_t s);
 // x[i][j] += sum_{k < cnt[j]} y[k * Bp + j] Z_k[i][j]
 template <typename ZT>
 void combine(double *x, const ZT *Z, const int *cnt, const double *y, i64 n, i64 Bp,
              hipStream_t s);
+// one lane j of a state-major vector: dst[r * Bp + j] = src[r] for r < rows, 0 for rows <= r < n
+void put_lane(const double *src, i64 rows, double *dst, i64 n, i64 Bp, i64 j, hipStream_t s);
+// dst[r] = src[r * Bp + j] for r < rows
+void get_lane(const double *src, i64 rows, i64 Bp, i64 j, double *dst, hipStream_t s);
+// X_k[r * Bp + j] = 0 for rows[j] <= r < n and k < count (vectors ld elements apart), on the
+// lanes with sel[j] != 0 (every lane when sel is null)
+template <typename T>
+void clear_tails(T *X, int count, i64 ld, i64 n, i64 Bp, const int *rows, const int *sel,
+                 hipStream_t s);
 // system-major (src[j * n + i], j < nsys) to state-major (dst[i * Bp + j], zero for j >= nsys)
 void pack(const double *src, i64 n, i64 nsys, double *dst, i64 Bp, hipStream_t s);
 // state-major to system-major for j < nsys

@@ -439,9 +439,10 @@ struct graph_cache {
         uses.clear();
     }
     // run body on s: replay its graph, capture it, or launch it eagerly
+    // force: capture whatever the process-wide switch says (still never while profiling)
     template <typename F>
-    void run(const graph_key &k, hipStream_t s, F &&body, i64 rows) {
-        if (!graphs_enabled(rows)) {
+    void run(const graph_key &k, hipStream_t s, F &&body, i64 rows, bool force = false) {
+        if (force ? prof().mask != 0 : !graphs_enabled(rows)) {
             body();
             return;
         }
@@ -657,5 +658,8 @@ struct mk_sector_mg_s {
     // a hierarchy over one population_sector (population_multigrid) keeps the host form of its
     // levels: rows, entries and the transfers, in rank order and FP64
     std::shared_ptr<mk::sector_hierarchy> host_levels;
+    // rows of every sector on every level ([level][sector]; 0 where a sector has no such level):
+    // a leading block of whole sectors is a prefix of every level
+    std::vector<std::vector<mk::i64>> sector_rows;
 };
 

@@ -210,3 +210,106 @@ def test_multishift_256_solves():
         # (about eps * sum_j (s + 2 exit_j - leak_j) |x_j|)
         bound = (np.abs(r).sum() + 16 * EPS * ((s + absA) * np.abs(X[j])).sum()) / s
         assert np.abs(Xk - X_closed).sum() <= bound, (j, np.abs(Xk - X_closed).sum(), bound)
+
+
+@pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
+def test_batch_red_black(small, precision):
+    """Red-black Gauss-Seidel hierarchies in a batch (1 + 2 sweeps, W-cycles on the top level):
+    every system reaches its tolerance and lies within the L1 bound of the single-system
+    solver's solution with the same hierarchy shape and of the oracle's (Jacobi) solution; FP64
+    iteration counts equal the single-system ones up to rounding."""
+    w, box, Q = small
+    n = box.size()
+    shifts = [0.05, 0.5, 5.0, 50.0]
+    B = np.array(rhs_set(n, [10, 11, 12, 13]))
+    tol = 1e-10 * np.linalg.norm(B, axis=1)
+    opts = dict(precision=precision, smoother=mk.RED_BLACK, pre_sweeps=1, post_sweeps=2,
+                coarse_cycles=2, cycle_levels=1)
+    M = mk.SectorMultigrid(w.model.domain, box, 0.5, mk.MultigridOptions(**opts))
+    X, res = mk.BatchSolver(Q, M, shifts).solve(B, tolerances=tol)
+    P = Problem(w.model, w.max_totals)
+    for j, s in enumerate(shifts):
+        assert res[j].converged, (j, res[j])
+        assert np.linalg.norm(B[j] - Q.apply_shifted(s, X[j])) <= 1.0001 * tol[j]
+        Ms = mk.SectorMultigrid(w.model.domain, box, s, mk.MultigridOptions(**opts))
+        x1 = np.zeros(n)
+        r1 = mk.gmres(mk.ShiftedGenerator(Q, s), Ms, B[j], x1, tolerance=tol[j])
+        assert r1.converged
+        bound = (l1_residual(Q, s, B[j], X[j]) + l1_residual(Q, s, B[j], x1)) / s
+        assert np.abs(X[j] - x1).sum() <= bound
+        if precision == mk.FP64:
+            assert abs(res[j].iterations - r1.iterations) <= 1, (j, res[j].iterations, r1.iterations)
+        x_ref, _ = P.gmres(s, B[j], mg=OracleMG(P, s), tol=tol[j], restart=50)
+        bound = (np.abs(B[j] - P.apply_shifted(s, X[j])).sum() +
+                 np.abs(B[j] - P.apply_shifted(s, x_ref)).sum()) / s
+        assert np.abs(X[j] - x_ref).sum() <= bound
+        print(f"[batch red-black] s={s} its={res[j].iterations} single {r1.iterations}")
+
+
+@pytest.mark.parametrize("smoother", [mk.JACOBI, mk.RED_BLACK], ids=["jacobi", "red_black"])
+def test_batch_uneven_sizes_in_one_launch(smoother):
+    """Systems of the boxes of caps 3, 4 and 5 on 16 voxels (969, 4,845 and 20,349 states) in
+    one batch over the cap-5 box: system j is the leading rows[j] block, which for the absorbing
+    generator is the cap-k box's generator. Each solution equals (within the L1 bound) the
+    single-system solve on its own box, its sector sums the lumped closed form of cap k, and
+    its entries past its rows are zero."""
+    big = birth_death_1d(16, 5, 0.5, "random")
+    box, Q = build(big)
+    caps = [3, 4, 5, 3, 5, 4]
+    shifts = [0.3, 0.3, 0.3, 4.0, 4.0, 1.0]
+    rows = [mk.PopulationBox(16, [c]).size() for c in caps]
+    assert rows[:3] == [969, 4845, 20349]
+    n = box.size()
+    B = np.zeros((len(caps), n))
+    for j, r in enumerate(rows):
+        b = np.random.default_rng(20 + j).random(r)
+        B[j, :r] = b / b.sum()
+    tol = 1e-10 * np.linalg.norm(B, axis=1)
+    opts = mk.MultigridOptions(precision=mk.FP32, smoother=smoother)
+    M = mk.SectorMultigrid(big.model.domain, box, 1.0, opts)
+    X, res = mk.BatchSolver(Q, M, shifts, rows=rows).solve(B, tolerances=tol)
+    for j, (c, s, r) in enumerate(zip(caps, shifts, rows)):
+        assert res[j].converged, (j, res[j])
+        assert not np.any(X[j, r:]), "entries past the system's rows stay zero"
+        w = birth_death_1d(16, c, s, "random")
+        bj, Qj = build(w)
+        assert bj.size() == r
+        x = X[j, :r]
+        rj = B[j, :r] - Qj.apply_shifted(s, x)
+        assert np.linalg.norm(rj) <= 1.0001 * tol[j], (j, np.linalg.norm(rj), tol[j])
+        Mj = mk.SectorMultigrid(w.model.domain, bj, s, opts)
+        x1 = np.zeros(r)
+        assert mk.gmres(mk.ShiftedGenerator(Qj, s), Mj, B[j, :r], x1, tolerance=tol[j]).converged
+        bound = (np.abs(rj).sum() + l1_residual(Qj, s, B[j, :r], x1)) / s
+        assert np.abs(x - x1).sum() <= bound
+        begins = [bj.sector_begin(k) for k in range(c + 1)] + [r]
+        Bk = np.array([B[j][begins[k]:begins[k + 1]].sum() for k in range(c + 1)])
+        Xk = np.array([x[begins[k]:begins[k + 1]].sum() for k in range(c + 1)])
+        absA = s + 2 * Qj.exit_rates() - Qj.leak_rates()
+        bound = (np.abs(rj).sum() + 16 * EPS * (absA * np.abs(x)).sum()) / s
+        assert np.abs(Xk - lumped_sector_sums(s, 0.05, 0.02, c, Bk)).sum() <= bound
+        print(f"[batch uneven] cap {c} rows {r} s={s} its={res[j].iterations}")
+
+
+def test_batch_graphs_and_mix_are_bitwise(small):
+    """A system's result does not depend on graph replay or on the other systems of its batch:
+    the same system alone, in a mix of 70 systems at another slot, and with eager launches
+    gives the same solution and iteration count bit for bit."""
+    w, box, Q = small
+    n = box.size()
+    M = mk.SectorMultigrid(w.model.domain, box, 1.0,
+                           mk.MultigridOptions(precision=mk.FP32, smoother=mk.RED_BLACK))
+    shifts = list(np.geomspace(0.05, 20.0, 70))
+    B = np.array(rhs_set(n, range(70)))
+    tol = 1e-10 * np.linalg.norm(B, axis=1)
+    X, res = mk.BatchSolver(Q, M, shifts).solve(B, tolerances=tol)
+    for j in (0, 33, 69):
+        Xa, ra = mk.BatchSolver(Q, M, [shifts[j]]).solve(B[j:j + 1].copy(), tolerances=tol[j:j + 1].copy())
+        assert np.array_equal(Xa[0], X[j]) and ra[0].iterations == res[j].iterations, j
+    try:
+        mk.batch_graphs_enable(False)
+        Xe, re = mk.BatchSolver(Q, M, shifts).solve(B, tolerances=tol)
+    finally:
+        mk.batch_graphs_enable(True)
+    assert np.array_equal(Xe, X)
+    assert [r.iterations for r in re] == [r.iterations for r in res]

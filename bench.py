@@ -250,8 +250,10 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16, tune=True):
         t = time.perf_counter()
         tuned = autotune.tune_batch(Q, w.model.domain, box, shifts, B, restart=restart)
         tuned_s = time.perf_counter() - t
-        log(f"[multishift] autotuned sweeps {tuned.pre}/{tuned.post} in {tuned_s:.1f}s: " +
-            ", ".join(f"{tr.pre}/{tr.post}={tr.seconds * 1e3:.1f}ms" for tr in tuned.trials))
+        names = {mk.JACOBI: "jacobi", mk.RED_BLACK: "red_black"}
+        log(f"[multishift] autotuned {names.get(tuned.smoother)} {tuned.pre}/{tuned.post} in "
+            f"{tuned_s:.1f}s: " + ", ".join(f"{names.get(tr.smoother)} {tr.pre}/{tr.post}="
+                                            f"{tr.seconds * 1e3:.1f}ms" for tr in tuned.trials))
     t = time.perf_counter()
     M = mk.SectorMultigrid(w.model.domain, box, 1.0,
                            tuned.options(precision=mk.FP32) if tuned else mk.MultigridOptions(precision=mk.FP32))
@@ -303,7 +305,11 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16, tune=True):
         "sequential_gpu_solves_per_s": seq_rate,
         "batched_over_sequential": (len(shifts) * steps / total) / seq_rate,
         "autotuned_sweeps": [tuned.pre, tuned.post] if tuned else None,
-        "autotune_trials": ([{"sweeps": [tr.pre, tr.post], "ms": 1e3 * tr.seconds, "max_iterations": tr.iterations}
+        "autotuned_smoother": ({mk.JACOBI: "jacobi", mk.RED_BLACK: "red_black"}.get(tuned.smoother)
+                               if tuned else None),
+        "autotune_trials": ([{"smoother": {mk.JACOBI: "jacobi", mk.RED_BLACK: "red_black"}.get(tr.smoother),
+                              "sweeps": [tr.pre, tr.post], "ms": 1e3 * tr.seconds,
+                              "max_iterations": tr.iterations}
                              for tr in tuned.trials] if tuned else None),
     }
 

@@ -362,8 +362,10 @@ mk_status mk_batch_create(mk_generator gen, mk_sector_mg mg, int64_t n_systems,
 mk_status mk_batch_create_sized(mk_generator gen, mk_sector_mg mg, int64_t n_systems,
                                 const double *shifts, const int64_t *rows, mk_batch *out);
 mk_status mk_batch_destroy(mk_batch batch);
-/* graph replay of the batched Arnoldi steps (default on; MK_BATCH_GRAPHS=0 turns it off at
- * start); replays run the same kernels with the same arguments */
+/* graph replay of the batched Arnoldi steps of mk_batch_solve (default on; MK_BATCH_GRAPHS=0
+ * turns it off at start); replays run the same kernels with the same arguments. While a step is
+ * captured the runtime rejects synchronous (legacy-stream) HIP calls of other host threads:
+ * turn replay off when other threads do device work during mk_batch_solve. */
 mk_status mk_batch_graphs_enable(int32_t on);
 /* B and X are system-major (system j's vector at j * rows), host buffers; X holds the initial
  * guesses on input. tolerances: n_systems absolute residual norms, or NULL for
@@ -376,13 +378,18 @@ mk_status mk_batch_solve_device(mk_batch batch, const double *d_B, double *d_X,
                                 mk_solver_result *results);
 
 /* ---- serving -----------------------------------------------------------------------------
- * No reference counterpart. A server runs requests on `workers` host threads, each with its own
- * HIP stream, so requests overlap on the device. Requests are admitted first come first served
- * when a worker is free, their device footprint (Krylov basis and work vectors) fits
- * hbm_budget_bytes beside the running ones, and, for a resolvent, a hierarchy of its family is
- * free. A resolvent family is a generator with a pool of Jacobi sector multigrid handles of the
- * same box, kept resident and re-targeted to each request's shift (mk_sector_mg_set_shift). The
- * server does not own the handles; they must outlive it. */
+ * No reference counterpart. A resolvent family is a generator with a pool of sector multigrid
+ * handles (Jacobi or red-black) of the same box, kept resident.
+ * Continuous batching: a family of at most the batching row limit gets a batch engine (one slot
+ * per system; the generator, the first hierarchy and the slots' Krylov bases resident in HBM) run
+ * by its own thread and HIP stream: at every FGMRES restart boundary it retires finished systems
+ * and admits queued requests of the family into free slots (restart 50; the request's
+ * max_iterations applies). Other families, and exponentials, run on `workers` host threads with
+ * their own HIP streams, admitted first come first served when a worker is free, their device
+ * footprint (Krylov basis and work vectors) fits hbm_budget_bytes beside the running ones and,
+ * for a resolvent, a hierarchy of its family is free (re-targeted to the request's shift with
+ * mk_sector_mg_set_shift). A request's result does not depend on the other requests running.
+ * The server does not own the handles; they must outlive it. */
 typedef struct mk_server_s *mk_server;
 enum { MK_REQUEST_RESOLVENT = 0, MK_REQUEST_EXPV = 1 };
 
@@ -404,6 +411,15 @@ mk_status mk_server_add_family(mk_server server, mk_generator gen, int32_t n_mg,
 mk_status mk_server_submit_resolvent(mk_server server, int32_t family, double shift,
                                      const double *b, double tolerance,
                                      const mk_gmres_options *options, int64_t *id);
+/* the same for the leading `rows` rows of the family's box (a sector boundary; for an absorbing
+ * one-species box, the box of a smaller cap); b has `rows` values, and so has the solution.
+ * Batched families only (rows < the box's size) */
+mk_status mk_server_submit_resolvent_sized(mk_server server, int32_t family, double shift,
+                                           const double *b, int64_t rows, double tolerance,
+                                           const mk_gmres_options *options, int64_t *id);
+/* families added after this call with at most max_rows states (default 524,288) are served by
+ * a batch engine of `slots` systems (default 64; 0 disables batching) */
+mk_status mk_server_batching(mk_server server, int64_t max_rows, int32_t slots);
 /* w = exp(t Q) v; v is copied at submission */
 mk_status mk_server_submit_expv(mk_server server, mk_generator gen, double t, const double *v,
                                 const mk_expv_options *options, int64_t *id);

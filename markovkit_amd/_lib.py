@@ -104,6 +104,8 @@ SIGNATURES = {
     "mk_server_destroy": ([P], i32),
     "mk_server_add_family": ([P, P, i32, P, C.POINTER(i32)], i32),
     "mk_server_submit_resolvent": ([P, i32, f64, P, f64, C.POINTER(GmresOptions), C.POINTER(i64)], i32),
+    "mk_server_submit_resolvent_sized": ([P, i32, f64, P, i64, f64, C.POINTER(GmresOptions), pi64], i32),
+    "mk_server_batching": ([P, i64, i32], i32),
     "mk_server_submit_expv": ([P, P, f64, P, C.POINTER(ExpvOptions), C.POINTER(i64)], i32),
     "mk_server_wait": ([P, i64, P, C.POINTER(ServerResult)], i32),
     "mk_batch_create": ([P, P, i64, P, C.POINTER(P)], i32),

@@ -534,14 +534,19 @@ class ServedResult:
 
 
 class Server:
-    """Request serving on worker threads with their own HIP streams (mk_server_*): resolvent
-    solves of registered families (a generator and a pool of Jacobi hierarchies, re-targeted
-    to each request's shift) and Krylov exponentials, admitted first come first served by
-    free worker, HBM footprint and free hierarchy."""
+    """Request serving (mk_server_*). Resolvent families of at most batch_rows states are
+    served by continuous batching: a resident batch engine of batch_slots systems admits queued
+    requests at every restart boundary. Larger families (a generator and a pool of Jacobi or
+    red-black hierarchies, re-targeted to each request's shift) and Krylov exponentials run on
+    worker threads with their own HIP streams, admitted first come first served by free worker,
+    HBM footprint and free hierarchy. batch_rows = 0 turns batching off."""
 
-    def __init__(self, workers: int = 4, hbm_budget: int = 64 << 30):
+    def __init__(self, workers: int = 4, hbm_budget: int = 64 << 30, batch_rows: int = None,
+                 batch_slots: int = 64):
         self._h = C.c_void_p()
         check(lib().mk_server_create(int(workers), int(hbm_budget), C.byref(self._h)))
+        if batch_rows is not None:
+            check(lib().mk_server_batching(self._h, int(batch_rows), int(batch_slots)))
         self._keep = []
         self._rows = {}
         self._family_rows = []
@@ -564,13 +569,17 @@ class Server:
 
     def submit_resolvent(self, family: int, shift: float, b, tolerance: float,
                          restart: int = 50, max_iterations: int = 1000) -> int:
+        """(shift I - Q) x = b; b has one entry per state of the family's box, or fewer: the
+        leading rows of whole sectors (for an absorbing one-species box, a box of a smaller
+        cap; batched families only)."""
         ba = np.ascontiguousarray(b, np.float64)
-        if ba.shape != (self._family_rows[family],):
-            raise ValueError("b must have one entry per state of the family")
+        if ba.ndim != 1 or not 0 < len(ba) <= self._family_rows[family]:
+            raise ValueError("b must have at most one entry per state of the family")
         o = _GmresOptions(0.0, int(max_iterations), int(restart), 0)
         rid = C.c_int64()
-        check(lib().mk_server_submit_resolvent(self._h, int(family), float(shift), _p(ba),
-                                               float(tolerance), C.byref(o), C.byref(rid)))
+        check(lib().mk_server_submit_resolvent_sized(self._h, int(family), float(shift), _p(ba),
+                                                     len(ba), float(tolerance), C.byref(o),
+                                                     C.byref(rid)))
         self._rows[rid.value] = len(ba)
         return rid.value
 

@@ -47,6 +47,7 @@ class Trial:
     iterations: int
     coarse_cycles: int = 1
     cycle_levels: int = 0
+    smoother: int = None
 
 
 @dataclass
@@ -58,10 +59,14 @@ class TuneResult:
     trials: List[Trial] = field(default_factory=list)
     coarse_cycles: int = 1
     cycle_levels: int = 0
+    smoother: int = None
 
     def options(self, **kw):
-        """MultigridOptions with the tuned sweep counts and cycle shape (other fields from kw)"""
+        """MultigridOptions with the tuned sweep counts, cycle shape and (when tuned) smoother;
+        other fields from kw"""
         from .api import MultigridOptions
+        if self.smoother is not None:
+            kw.setdefault("smoother", self.smoother)
         return MultigridOptions(pre_sweeps=self.pre, post_sweeps=self.post,
                                 coarse_cycles=self.coarse_cycles, cycle_levels=self.cycle_levels, **kw)
 
@@ -120,44 +125,47 @@ def tune_solve(Q, domain, box, shift: float, b, *, smoother=None, precision=None
                       win.cycle_levels)
 
 
-def tune_batch(Q, domain, box, shifts, B, *, precision=None,
+def tune_batch(Q, domain, box, shifts, B, *, precision=None, smoothers=None,
                sweeps: Sequence[Tuple[int, int]] = SWEEPS, chunks: Sequence[int] = (16,),
                rel_tolerance: float = 1e-10, restart: int = 50, repeats: int = 1,
                apply: bool = True) -> TuneResult:
-    """The same search for a batched multi-shift solve (BatchSolver, Jacobi hierarchy): every
-    system to rel_tolerance * ||b_j||."""
+    """The same search for a batched multi-shift solve (BatchSolver) over smoothers (Jacobi and
+    red-black Gauss-Seidel by default) and sweep counts: every system to
+    rel_tolerance * ||b_j||."""
     from . import api as mk
     precision = mk.FP32 if precision is None else precision
+    smoothers = (mk.JACOBI, mk.RED_BLACK) if smoothers is None else tuple(smoothers)
     B = np.ascontiguousarray(B, np.float64)
     tols = rel_tolerance * np.linalg.norm(B, axis=1)
     start_chunk = xcd_chunk()
     trials: List[Trial] = []
     try:
-        for pre, post in sweeps:
-            M = mk.SectorMultigrid(domain, box, 1.0,
-                                   mk.MultigridOptions(precision=precision, pre_sweeps=pre,
-                                                       post_sweeps=post))
-            solver = mk.BatchSolver(Q, M, shifts)
-            for chunk in chunks:
-                set_xcd_chunk(chunk)
-                solver.solve(B, tolerances=tols, restart=restart)  # warm-up
-                best = float("inf")
-                its = 0
-                for _ in range(repeats):
-                    mk.synchronize()
-                    t = time.perf_counter()
-                    _, res = solver.solve(B, tolerances=tols, restart=restart)
-                    mk.synchronize()
-                    best = min(best, time.perf_counter() - t)
-                    if not all(r.converged for r in res):
-                        best = float("inf")
-                        break
-                    its = max(r.iterations for r in res)
-                trials.append(Trial(pre, post, chunk, best, its))
-            del solver, M
+        for smoother in smoothers:
+            for pre, post in sweeps:
+                M = mk.SectorMultigrid(domain, box, 1.0,
+                                       mk.MultigridOptions(precision=precision, smoother=smoother,
+                                                           pre_sweeps=pre, post_sweeps=post))
+                solver = mk.BatchSolver(Q, M, shifts)
+                for chunk in chunks:
+                    set_xcd_chunk(chunk)
+                    solver.solve(B, tolerances=tols, restart=restart)  # warm-up
+                    best = float("inf")
+                    its = 0
+                    for _ in range(repeats):
+                        mk.synchronize()
+                        t = time.perf_counter()
+                        _, res = solver.solve(B, tolerances=tols, restart=restart)
+                        mk.synchronize()
+                        best = min(best, time.perf_counter() - t)
+                        if not all(r.converged for r in res):
+                            best = float("inf")
+                            break
+                        its = max(r.iterations for r in res)
+                    trials.append(Trial(pre, post, chunk, best, its, smoother=smoother))
+                del solver, M
     finally:
         set_xcd_chunk(start_chunk)
     win = min(trials, key=lambda t: t.seconds)
     if apply:
         set_xcd_chunk(win.xcd_chunk)
-    return TuneResult(win.pre, win.post, win.xcd_chunk, win.seconds, trials)
+    return TuneResult(win.pre, win.post, win.xcd_chunk, win.seconds, trials, smoother=win.smoother)

@@ -19,9 +19,9 @@
 // The batch is an engine of slots: systems are admitted to free slots and retired at restart
 // boundaries (continuous batching, used by the request server); mk_batch_solve admits all its
 // systems and steps until they are done. A lane's arithmetic does not depend on the other lanes,
-// so a system's result is the same alone and in any mix. The operations of one Arnoldi step
-// (preconditioner cycle, operator, projections, update) are captured in a HIP graph per step
-// index and replayed.
+// so a system's result is the same alone and in any mix. In mk_batch_solve the operations of one
+// Arnoldi step (preconditioner cycle, operator, projections, update) are captured in a HIP graph
+// per step index and replayed; the server's engines launch them eagerly.
 #include "batch_kernels.hpp"
 #include "runtime.hpp"
 
@@ -88,6 +88,10 @@ struct mk_batch_s {
     double *host = nullptr; // pinned: hd mirror, y, scale, beta
     int *ihost = nullptr;   // pinned: cnt, rows, sel
     graph_cache graphs;
+    // step graphs are captured only where no other host thread of the process runs device work:
+    // during a capture the runtime rejects synchronous (legacy-stream) calls of other threads,
+    // so the server's engines, which run beside other threads, launch eagerly
+    bool capture = true;
     ~mk_batch_s() {
         graphs.clear();
         if (host) (void)hipHostFree(host);
@@ -502,7 +506,7 @@ bool step(mk_batch_s &B) {
         return false;
     };
     int which = 0; // s buffer the next update kernel reads
-    const bool graphs = batch_graph_switch().load() != 0;
+    const bool graphs = B.capture && batch_graph_switch().load() != 0;
     for (i64 c = 0; c < m && any_mask(); ++c) {
         const int cnt = static_cast<int>(c);
         for (i64 j = 0; j < Bp; ++j) {
@@ -725,6 +729,7 @@ namespace mk::rt {
 
 mk_batch_s *batch_engine_create(mk_generator_s *gen, mk_sector_mg_s *mg, i64 slots, i64 restart) {
     auto *B = create_engine(gen, mg, slots);
+    B->capture = false; // the engine runs beside other host threads
     try {
         const i64 m = std::max<i64>(1, std::min<i64>(restart, B->n));
         require(m <= gpu::kFusedBasis, "batch gmres: restart above 64 is not supported");
@@ -778,6 +783,11 @@ bool batch_engine_take(mk_batch_s &B, i64 j, double *x, mk_solver_result *res) {
     *res = S.res;
     S = slot_state{};
     return true;
+}
+
+// free a slot whatever its state (after a failure)
+void batch_engine_release(mk_batch_s &B, i64 j) {
+    if (j >= 0 && j < B.Bp) B.slot[j] = slot_state{};
 }
 
 } // namespace mk::rt

@@ -1,13 +1,22 @@
-// Request serving: resolvent solves and Krylov exponentials admitted to worker threads by HBM
-// footprint. No reference counterpart (the reference is a library called one solve at a time).
+// Request serving: resolvent solves and Krylov exponentials. No reference counterpart (the
+// reference is a library called one solve at a time).
 //
-// Each worker thread owns a HIP stream (the library stream is per thread) and its Krylov
-// workspaces, so the solves of different workers overlap on the device. A request is admitted,
-// first come first served, once a worker is free, its footprint fits the HBM budget beside the
-// running requests and, for a resolvent, a multigrid hierarchy of its family is free. Resolvent
-// families keep their generator and a pool of hierarchies resident; a hierarchy is re-targeted
-// to each request's shift (terminal inverses only). Results do not depend on what else runs:
-// every request computes on its own stream with fixed-order reductions.
+// Continuous batching: a resolvent family of at most batch_rows states keeps a batch engine
+// resident (batch.hip: one slot per system, the generator and one hierarchy shared, the Krylov
+// bases of all slots in HBM) driven by its own thread and HIP stream. At every restart boundary
+// the engine retires the finished systems and admits queued requests of its family into the
+// free slots, so requests that arrive while others run join them on the device as slots free
+// up. A request may be the leading block of whole sectors of its family's box (for an absorbing
+// one-species box, the box of a smaller cap), so the caps of one grid share an engine.
+//
+// Larger resolvent families and exponentials run on worker threads, each with its own HIP
+// stream and Krylov workspaces, so their solves overlap on the device too. Such a request is
+// admitted first come first served once a worker is free, its footprint fits the HBM budget
+// beside the running requests and, for a resolvent, a multigrid hierarchy of its family is
+// free; the hierarchy is re-targeted to the request's shift (terminal inverses only).
+//
+// Results do not depend on what else runs: every slot and every request computes with
+// fixed-order reductions of its own, so a request's result is the same alone and in any mix.
 #include "runtime.hpp"
 
 #include <chrono>
@@ -19,6 +28,18 @@
 
 using namespace mk::rt;
 
+namespace mk::rt {
+// the continuous-batching engine of batch.hip
+mk_batch_s *batch_engine_create(mk_generator_s *gen, mk_sector_mg_s *mg, i64 slots, i64 restart);
+void batch_engine_destroy(mk_batch_s *B);
+i64 batch_engine_slots(const mk_batch_s &B);
+void batch_engine_admit(mk_batch_s &B, i64 j, const double *b, i64 rows, double shift, double tol,
+                        i64 max_iterations);
+bool batch_engine_step(mk_batch_s &B);
+bool batch_engine_take(mk_batch_s &B, i64 j, double *x, mk_solver_result *res);
+void batch_engine_release(mk_batch_s &B, i64 j);
+} // namespace mk::rt
+
 namespace {
 
 using clock_type = std::chrono::steady_clock;
@@ -27,6 +48,7 @@ struct request {
     i64 id = 0;
     int kind = MK_REQUEST_RESOLVENT;
     int family = -1;
+    i64 rows = 0; // resolvent: the leading rows of the family's box the system has
     mk_generator gen = nullptr;
     double shift = 0.0, t = 0.0, tolerance = 0.0;
     mk_gmres_options gopt{};
@@ -46,6 +68,10 @@ struct family {
     std::vector<mk_sector_mg> free;
     i64 n = 0;
     std::size_t zs = 8;
+    // continuous batching: the engine, its queue and its thread (null engine: worker path)
+    mk_batch_s *engine = nullptr;
+    std::deque<request *> queue;
+    std::thread thread;
 };
 
 // device footprint of a request beyond the resident operators: the Krylov basis (m + 1
@@ -74,6 +100,8 @@ struct worker_buffers {
 
 struct mk_server_s {
     i64 budget = 0;
+    i64 batch_rows = i64{1} << 19; // families of at most this many states are batched
+    i64 batch_slots = 64;
     std::mutex mu;
     std::condition_variable cv;
     std::deque<request *> queue;
@@ -114,6 +142,80 @@ struct mk_server_s {
         r.out.resize(n);
         HIPCHK(hipMemcpyAsync(r.out.data(), db, n * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+    }
+
+    // the engine thread of a batched family: admit queued requests into free slots, run one
+    // restart cycle of every active slot, hand out the finished ones; repeat
+    void engine_loop(family &f) {
+        mk_batch_s &B = *f.engine;
+        const i64 slots = batch_engine_slots(B);
+        std::vector<request *> in_slot(slots, nullptr);
+        i64 busy = 0;
+        std::unique_lock<std::mutex> lock(mu);
+        while (true) {
+            cv.wait(lock, [&] { return stopping || !f.queue.empty() || busy > 0; });
+            if (stopping && f.queue.empty() && busy == 0) return;
+            std::vector<std::pair<i64, request *>> admit;
+            for (i64 j = 0; j < slots && !f.queue.empty(); ++j) {
+                if (in_slot[j]) continue;
+                request *r = f.queue.front();
+                f.queue.pop_front();
+                r->started = clock_type::now();
+                in_slot[j] = r;
+                ++busy;
+                admit.emplace_back(j, r);
+            }
+            lock.unlock();
+            std::vector<std::pair<request *, std::string>> failed;
+            for (auto &[j, r] : admit) {
+                try {
+                    batch_engine_admit(B, j, r->in.data(), r->rows, r->shift, r->tolerance,
+                                       r->gopt.max_iterations);
+                } catch (const std::exception &e) {
+                    failed.emplace_back(r, e.what());
+                    in_slot[j] = nullptr;
+                    batch_engine_release(B, j);
+                }
+            }
+            std::vector<request *> finished;
+            std::string step_error;
+            try {
+                batch_engine_step(B);
+                for (i64 j = 0; j < slots; ++j) {
+                    request *r = in_slot[j];
+                    if (!r) continue;
+                    r->out.resize(r->rows);
+                    if (batch_engine_take(B, j, r->out.data(), &r->solve)) {
+                        finished.push_back(r);
+                        in_slot[j] = nullptr;
+                    }
+                }
+            } catch (const std::exception &e) {
+                // a device failure ends every request of the engine
+                step_error = e.what();
+                for (i64 j = 0; j < slots; ++j)
+                    if (in_slot[j]) {
+                        failed.emplace_back(in_slot[j], step_error);
+                        in_slot[j] = nullptr;
+                        batch_engine_release(B, j);
+                    }
+            }
+            lock.lock();
+            for (request *r : finished) {
+                r->finished = clock_type::now();
+                r->status = MK_OK;
+                r->done = true;
+                --busy;
+            }
+            for (auto &[r, e] : failed) {
+                r->finished = clock_type::now();
+                r->status = MK_ERR_RUNTIME;
+                r->error = e;
+                r->done = true;
+                --busy;
+            }
+            cv.notify_all();
+        }
     }
 
     void worker() {
@@ -168,7 +270,11 @@ i64 submit(mk_server s, std::unique_ptr<request> r) {
     r->id = s->next_id++;
     r->submitted = clock_type::now();
     const i64 id = r->id;
-    s->queue.push_back(r.get());
+    if (r->kind == MK_REQUEST_RESOLVENT && s->families[r->family].engine) {
+        s->families[r->family].queue.push_back(r.get());
+    } else {
+        s->queue.push_back(r.get());
+    }
     s->requests[id] = std::move(r);
     s->cv.notify_all();
     return id;
@@ -196,6 +302,10 @@ mk_status mk_server_destroy(mk_server s) {
         s->cv.notify_all();
     }
     for (auto &t : s->threads) t.join();
+    for (auto &f : s->families) {
+        if (f.thread.joinable()) f.thread.join();
+        if (f.engine) batch_engine_destroy(f.engine);
+    }
     delete s;
     return MK_OK;
 }
@@ -204,45 +314,79 @@ mk_status mk_server_add_family(mk_server s, mk_generator gen, int32_t n_mg, cons
                                int32_t *family_out) {
     return guarded([&] {
         require(s && gen && mg && family_out && n_mg >= 1, "server: bad family");
-        family f;
-        f.gen = gen;
-        f.n = gen->n;
         for (int k = 0; k < n_mg; ++k) {
             require(mg[k] && mg[k]->rows == gen->n, "server: a hierarchy does not match the generator");
             require(mg[k]->opt.smoother == MK_JACOBI || mg[k]->opt.smoother == MK_RED_BLACK,
                     "server: hierarchies must use the Jacobi or red-black smoother (re-targeted per shift)");
+        }
+        std::lock_guard<std::mutex> lock(s->mu);
+        require(!s->stopping, "server: stopping");
+        family &f = s->families.emplace_back();
+        f.gen = gen;
+        f.n = gen->n;
+        for (int k = 0; k < n_mg; ++k) {
             f.zs = mg[k]->precision == MK_FP32 ? sizeof(float) : sizeof(double);
             f.free.push_back(mg[k]);
         }
-        std::lock_guard<std::mutex> lock(s->mu);
-        s->families.push_back(std::move(f));
+        const bool rb_ok = mg[0]->opt.smoother != MK_RED_BLACK ||
+                           (mg[0]->opt.pre_sweeps >= 1 && mg[0]->opt.post_sweeps >= 1);
+        if (gen->n <= s->batch_rows && s->batch_slots > 0 && rb_ok) {
+            try {
+                f.engine = batch_engine_create(gen, mg[0], s->batch_slots, 50);
+            } catch (...) {
+                s->families.pop_back();
+                throw;
+            }
+            f.thread = std::thread([s, &f] { s->engine_loop(f); });
+        }
         *family_out = static_cast<int32_t>(s->families.size() - 1);
+    });
+}
+
+mk_status mk_server_submit_resolvent_sized(mk_server s, int32_t family, double shift,
+                                           const double *b, int64_t rows, double tolerance,
+                                           const mk_gmres_options *options, int64_t *id) {
+    return guarded([&] {
+        require(s && b && options && id, "server: bad arguments");
+        if (!(shift > 0.0)) throw std::invalid_argument("population_multigrid: shift must be positive");
+        i64 n;
+        std::size_t zs;
+        bool batched;
+        {
+            std::lock_guard<std::mutex> lock(s->mu);
+            require(family >= 0 && family < static_cast<int32_t>(s->families.size()), "server: unknown family");
+            n = s->families[family].n;
+            zs = s->families[family].zs;
+            batched = s->families[family].engine != nullptr;
+        }
+        if (rows < 0) rows = n;
+        require(rows >= 1 && rows <= n, "server: bad row count");
+        require(rows == n || batched, "server: only batched families take systems of fewer rows");
+        auto r = std::make_unique<request>();
+        r->kind = MK_REQUEST_RESOLVENT;
+        r->family = family;
+        r->rows = rows;
+        r->shift = shift;
+        r->tolerance = tolerance;
+        r->gopt = *options;
+        r->in.assign(b, b + rows);
+        r->footprint = batched ? 0 : resolvent_footprint(n, std::max<i64>(1, options->restart), zs);
+        *id = submit(s, std::move(r));
     });
 }
 
 mk_status mk_server_submit_resolvent(mk_server s, int32_t family, double shift, const double *b,
                                      double tolerance, const mk_gmres_options *options,
                                      int64_t *id) {
+    return mk_server_submit_resolvent_sized(s, family, shift, b, -1, tolerance, options, id);
+}
+
+mk_status mk_server_batching(mk_server s, int64_t max_rows, int32_t slots) {
     return guarded([&] {
-        require(s && b && options && id, "server: bad arguments");
-        if (!(shift > 0.0)) throw std::invalid_argument("population_multigrid: shift must be positive");
-        i64 n;
-        std::size_t zs;
-        {
-            std::lock_guard<std::mutex> lock(s->mu);
-            require(family >= 0 && family < static_cast<int32_t>(s->families.size()), "server: unknown family");
-            n = s->families[family].n;
-            zs = s->families[family].zs;
-        }
-        auto r = std::make_unique<request>();
-        r->kind = MK_REQUEST_RESOLVENT;
-        r->family = family;
-        r->shift = shift;
-        r->tolerance = tolerance;
-        r->gopt = *options;
-        r->in.assign(b, b + n);
-        r->footprint = resolvent_footprint(n, std::max<i64>(1, options->restart), zs);
-        *id = submit(s, std::move(r));
+        require(s && max_rows >= 0 && slots >= 0 && slots <= 4096, "server: bad batching arguments");
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->batch_rows = max_rows;
+        s->batch_slots = slots;
     });
 }
 

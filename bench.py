@@ -290,6 +290,29 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16, tune=True):
         if not r1.converged:
             raise RuntimeError("multishift: a single solve did not converge")
     seq_rate = len(pick) / seq
+    # block Krylov: the 4 right-hand sides of every shift share one basis for their first restart
+    # cycle, block Gram-Schmidt projections on FP64 matrix cores
+    bsolver = mk.BatchSolver(Q, M, shifts, block=4)
+    bsolver.solve(B, tolerances=tol, restart=restart)  # warm-up
+    mk.synchronize()
+    blat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t = time.perf_counter()
+        Xb, resb = bsolver.solve(B, tolerances=tol, restart=restart)
+        blat.append(time.perf_counter() - t)
+    btotal = time.perf_counter() - t0
+    if not all(r.converged for r in resb):
+        raise RuntimeError("multishift block batch: a system did not converge")
+    bworst = max(float(np.linalg.norm(B[j] - Q.apply_shifted(shifts[j], Xb[j])) / np.linalg.norm(B[j]))
+                 for j in range(len(shifts)))
+    bits = [r.iterations for r in resb]
+    block = {"value": len(shifts) * steps / btotal, "batch_ms": 1e3 * btotal / steps,
+             "batch_p50_ms": 1e3 * percentile(blat, 50), "iterations_min": min(bits),
+             "iterations_max": max(bits), "worst_relative_residual": bworst,
+             "method": "block FGMRES(restart) per shift (4 rhs, shared basis, block CGS2 with "
+                       "V^T W on v_mfma_f64_16x16x4f64, Cholesky QR), then per-system restarts"}
+    log(f"[multishift] block: {block['value']:.1f} solves/s, iterations {min(bits)}-{max(bits)}")
     return {
         "workload": f"{w.name}: {n} states, {len(shifts)} systems (64 shifts 1e-2..1e2 x 4 rhs), "
                     f"FP32 sector multigrid, FGMRES({restart}) + delayed CGS2, tol 1e-10 relative",
@@ -304,6 +327,7 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16, tune=True):
         "setup_s": setup,
         "sequential_gpu_solves_per_s": seq_rate,
         "batched_over_sequential": (len(shifts) * steps / total) / seq_rate,
+        "block": block,
         "autotuned_sweeps": [tuned.pre, tuned.post] if tuned else None,
         "autotuned_smoother": ({mk.JACOBI: "jacobi", mk.RED_BLACK: "red_black"}.get(tuned.smoother)
                                if tuned else None),
@@ -320,9 +344,14 @@ def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0, smooth
     b ~ U[0,1) L1-normalised, tolerance 1e-10 relative, FGMRES(50) with an FP32 sector
     multigrid), 30% toggle-switch FSP exponentials (0..200 x 0..200 = 40,401 states, absorbing
     box, p0 at a random state, exp(5 Q) p0 at tolerance 1e-8). Families are set up before the
-    arrivals start (generator and two resident hierarchies each); requests are submitted at
-    their arrival times to a server with `workers` worker streams."""
+    arrivals start and stay resident. Continuous batching: on 16 and 32 voxels one family on the
+    cap-5 box serves all three caps (a request of cap k is the box's leading sectors 0..k), and
+    every family of at most 524,288 states runs in a batch engine that admits requests at restart
+    boundaries; the larger families run on `workers` worker streams with two hierarchies each
+    (the headline W-cycle hierarchy above a million states). Requests are submitted at their
+    arrival times."""
     from markovkit_amd.fsp import box_generator, toggle_switch
+    from markovkit_amd.model import box_size
     smoother = getattr(mk, SMOOTHERS[smoother_name])
     rng = np.random.default_rng(seed)
     kinds = rng.random(n_requests) < 0.7
@@ -330,41 +359,52 @@ def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0, smooth
     caps = rng.choice([3, 4, 5], n_requests)
     shifts = rng.uniform(0.1, 10.0, n_requests)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    shared = (16, 32)  # grids whose cap-5 box serves every cap
     t = time.perf_counter()
-    fams = {}
-    for v, c in sorted({(int(v), int(c)) for v, c, k in zip(voxels, caps, kinds) if k}):
+    built = {}
+    for v, c in sorted({(int(v), 5 if int(v) in shared else int(c))
+                        for v, c, k in zip(voxels, caps, kinds) if k}):
         w = workloads.birth_death_1d(v, c, 1.0, "random")
         box = mk.PopulationBox(w.n_voxels, w.max_totals)
         Q = mk.Generator(box, w.model)
-        mgs = [mk.SectorMultigrid(w.model.domain, box, 1.0,
-                                  mk.MultigridOptions(precision=mk.FP32, smoother=smoother))
-               for _ in range(2)]
-        fams[(v, c)] = (Q, mgs)
+        big = box.size() > 1_000_000
+        opts = mk.MultigridOptions(precision=mk.FP32, smoother=smoother,
+                                   **(dict(pre_sweeps=1, post_sweeps=2, coarse_cycles=2,
+                                           cycle_levels=3, coarsest_size=100) if big else {}))
+        mgs = [mk.SectorMultigrid(w.model.domain, box, 1.0, opts) for _ in range(2)]
+        built[(v, c)] = (Q, mgs)
     J = box_generator(toggle_switch(), [200, 200], absorbing=True)
     F = mk.Generator.from_jumps(*J)
     setup = time.perf_counter() - t
-    log(f"[serving] {len(fams)} resolvent families and the toggle switch set up in {setup:.1f}s")
     srv = mk.Server(workers=workers, hbm_budget=96 << 30)
-    fid = {key: srv.add_family(Q, mgs) for key, (Q, mgs) in fams.items()}
+    fid = {key: srv.add_family(Q, mgs) for key, (Q, mgs) in built.items()}
+    batched = {key for key, (Q, _) in built.items() if Q.rows() <= 524288}
+    log(f"[serving] {len(built)} resolvent families ({len(batched)} batched) and the toggle switch "
+        f"set up in {setup:.1f}s")
+
+    def route(v, c):
+        key = (v, 5) if v in shared else (v, c)
+        return key, box_size(v, [c])
+
     payload = []
     for i in range(n_requests):
         if kinds[i]:
-            n = fams[(int(voxels[i]), int(caps[i]))][0].rows()
-            b = rng.random(n)
+            _, rows = route(int(voxels[i]), int(caps[i]))
+            b = rng.random(rows)
             payload.append(b / b.sum())
         else:
             v = np.zeros(F.rows())
             v[int(rng.integers(0, F.rows()))] = 1.0
             payload.append(v)
-    # warm-up: one request of every kind through the server
+    # warm-up: one request of every family and an exponential
     warm = [srv.submit_resolvent(fid[key], 1.0, np.full(Q.rows(), 1.0 / Q.rows()), 1e-8)
-            for key, (Q, _) in fams.items()]
+            for key, (Q, _) in built.items()]
     v0 = np.zeros(F.rows())
     v0[0] = 1.0
     warm.append(srv.submit_expv(F, 0.1, v0))
     for r in warm:
         srv.wait(r)
-    ids = []
+    ids, cls = [], []
     t0 = time.perf_counter()
     for i in range(n_requests):
         delay = t0 + arrivals[i] - time.perf_counter()
@@ -372,11 +412,14 @@ def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0, smooth
             time.sleep(delay)
         if kinds[i]:
             b = payload[i]
-            ids.append(srv.submit_resolvent(fid[(int(voxels[i]), int(caps[i]))], float(shifts[i]), b,
+            key, _ = route(int(voxels[i]), int(caps[i]))
+            ids.append(srv.submit_resolvent(fid[key], float(shifts[i]), b,
                                             1e-10 * float(np.linalg.norm(b))))
+            cls.append("resolvent_batched" if key in batched else "resolvent_worker")
         else:
             ids.append(srv.submit_expv(F, 5.0, payload[i], tolerance=1e-8))
-    lat, lat_res, lat_fsp, run = [], [], [], []
+            cls.append("expv")
+    lat, run, by = [], [], {}
     for i, r in enumerate(ids):
         _, info = srv.wait(r)
         if info.kind == "resolvent" and not info.solve.converged:
@@ -384,89 +427,30 @@ def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0, smooth
         latency = info.queued_s + info.run_s
         lat.append(latency)
         run.append(info.run_s)
-        (lat_res if info.kind == "resolvent" else lat_fsp).append(latency)
+        by.setdefault(cls[i], []).append((latency, info.run_s))
     total = time.perf_counter() - t0
     srv.close()
+    per_kind = {k: {"requests": len(v), "p50_ms": 1e3 * percentile([a for a, _ in v], 50),
+                    "p99_ms": 1e3 * percentile([a for a, _ in v], 99),
+                    "mean_run_ms": 1e3 * float(np.mean([b for _, b in v]))} for k, v in by.items()}
     return {
         "workload": (f"{n_requests} requests, Poisson {rate:.0f}/s (seed {seed}): 70% 1D RDME "
                      f"resolvents (voxels 16-64, caps 3-5, s ~ U[0.1,10], 1e-10 relative), 30% "
-                     f"toggle-switch FSP expv (40,401 states, t = 5, 1e-8); {workers} worker streams; "
+                     f"toggle-switch FSP expv (40,401 states, t = 5, 1e-8); continuous batching for "
+                     f"families of <= 524,288 states, {workers} worker streams for the rest; "
                      f"{smoother_name} smoother"),
         "value": n_requests / total,
         "unit": "requests/s",
         "offered_rate": rate,
         "p50_ms": 1e3 * percentile(lat, 50),
         "p99_ms": 1e3 * percentile(lat, 99),
-        "resolvent_p50_ms": 1e3 * percentile(lat_res, 50) if lat_res else None,
-        "resolvent_p99_ms": 1e3 * percentile(lat_res, 99) if lat_res else None,
-        "fsp_p50_ms": 1e3 * percentile(lat_fsp, 50) if lat_fsp else None,
-        "fsp_p99_ms": 1e3 * percentile(lat_fsp, 99) if lat_fsp else None,
         "mean_run_ms": 1e3 * float(np.mean(run)),
+        "per_kind": per_kind,
         "makespan_s": total,
         "setup_s": setup,
-        "families": len(fams),
+        "families": len(built),
+        "batched_families": len(batched),
     }
-
-
-def c3(mk, workloads, steps, restart):
-    """2D two-species annihilation on 8 x 8 voxels (h = 1/8, D = 1, births 0.1 * volume each,
-    A + B -> 0 at n_A n_B / volume), caps {2, 2}: 4,601,025 states, s = 1, b = e_0. FGMRES(restart)
-    preconditioned by the red-black sector multigrid (V-cycle, 2 + 2 sweeps), mixed (FP32
-    multigrid) and plain FP64, each to 1e-10 relative residual. Checked in the run: true
-    residual, mass balance 1^T b = sum (s + leak) x, x >= -bound, and the mixed solution within
-    the L1 bound of the FP64 one."""
-    w = workloads.config_2d()
-    shift = w.shifts[0]
-    t = time.perf_counter()
-    box = mk.PopulationBox(w.n_voxels, w.max_totals)
-    Q = mk.Generator(box, w.model)
-    n = box.size()
-    mk.synchronize()
-    t_gen = time.perf_counter() - t
-    b = w.rhs(0, n)
-    tol = 1e-10 * float(np.linalg.norm(b))
-    A = mk.ShiftedGenerator(Q, shift)
-    b_dev, x_dev = mk.DeviceVector(n, b), mk.DeviceVector(n)
-    abs_A = shift + 2 * Q.exit_rates() - Q.leak_rates()
-    out = {"workload": f"{w.name}: 2D two-species annihilation, {n} states, s={shift}, b=e_0, "
-                       f"FGMRES({restart}) + red-black sector multigrid (V, 2+2), tol 1e-10 relative",
-           "states": n, "setup_s": {"generator": t_gen}}
-    xs = {}
-    for name, prec in (("mixed", mk.FP32), ("fp64", mk.FP64)):
-        t = time.perf_counter()
-        M = mk.SectorMultigrid(w.model.domain, box, shift,
-                               mk.MultigridOptions(precision=prec, smoother=mk.RED_BLACK))
-        mk.synchronize()
-        out["setup_s"][f"multigrid_{name}"] = time.perf_counter() - t
-        run_solves(mk, A, M, b_dev, x_dev, tol, 2, restart)
-        total, lat, iters, res, _ = run_solves(mk, A, M, b_dev, x_dev, tol, steps, restart)
-        x = x_dev.download()
-        r = b - Q.apply_shifted(shift, x)
-        bound = float((np.abs(r).sum() + 16 * EPS * (abs_A * np.abs(x)).sum()) / shift)
-        # 1^T b - sum (s + leak) x = 1^T r exactly, so it is at most ||r||_1 plus rounding
-        mass = abs(float(b.sum()) - float(((shift + Q.leak_rates()) * x).sum()))
-        mass_bound = float(np.abs(r).sum() + 64 * EPS * ((shift + Q.leak_rates()) * np.abs(x)).sum())
-        rel = float(np.linalg.norm(r) / np.linalg.norm(b))
-        if not (rel <= 1.0001e-10 and float(x.min()) >= -bound and mass <= mass_bound):
-            raise RuntimeError(f"C3 {name}: residual {rel:.3e}, min x {x.min():.3e}, mass {mass:.3e}")
-        xs[name] = (x, r)
-        rec = {"value": steps / total, "p50_ms": 1e3 * percentile(lat, 50),
-               "p99_ms": 1e3 * percentile(lat, 99), "iterations": iters[-1],
-               "relative_residual": rel, "l1_bound": bound, "mass_balance_error": mass,
-               "min_x": float(x.min())}
-        if name == "mixed":
-            out.update(rec)
-        else:
-            out["fp64"] = rec
-        del M
-    (xm, rm), (x64, r64) = xs["mixed"], xs["fp64"]
-    out["l1_mixed_vs_fp64"] = float(np.abs(xm - x64).sum())
-    out["l1_mixed_vs_fp64_bound"] = float((np.abs(rm).sum() + np.abs(r64).sum() + 16 * EPS *
-                                           (abs_A * (np.abs(xm) + np.abs(x64))).sum()) / shift)
-    if out["l1_mixed_vs_fp64"] > out["l1_mixed_vs_fp64_bound"]:
-        raise RuntimeError(f"C3: mixed and FP64 solutions differ by {out['l1_mixed_vs_fp64']:.3e}")
-    out["mixed_over_fp64"] = out["value"] / out["fp64"]["value"]
-    return out
 
 
 def main():
@@ -509,6 +493,8 @@ def main():
                     help="box workload: time only the first N solves of the transient (profiling)")
     ap.add_argument("--no-box", action="store_true",
                     help="skip the whole-box multigrid transient after the headline")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving mix after the headline")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the 2D two-species annihilation solves after the headline")
     args = ap.parse_args()
@@ -742,6 +728,11 @@ def main():
         log("[c3] 2D two-species annihilation, 4,601,025 states")
         out["c3"] = c3(mk, workloads, max(10, args.steps), args.restart)
         log(f"[c3] {out['c3']['value']:.1f} solves/s mixed, {out['c3']['fp64']['value']:.1f} FP64")
+    if args.workload == "large" and not args.no_serving:
+        log("[serving] serving mix")
+        out["serving"] = serving(mk, workloads, args.requests, args.workers, smoother_name=args.smoother)
+        log(f"[serving] {out['serving']['value']:.1f} requests/s, p50 {out['serving']['p50_ms']:.0f} ms, "
+            f"p99 {out['serving']['p99_ms']:.0f} ms")
     from markovkit_amd.calibrate import copy_bandwidth
     out["stream_copy_gbs"] = copy_bandwidth()
     out["roofline"]["frac_of_stream_copy"] = achieved / out["stream_copy_gbs"]

@@ -362,6 +362,14 @@ mk_status mk_batch_create(mk_generator gen, mk_sector_mg mg, int64_t n_systems,
 mk_status mk_batch_create_sized(mk_generator gen, mk_sector_mg mg, int64_t n_systems,
                                 const double *shifts, const int64_t *rows, mk_batch *out);
 mk_status mk_batch_destroy(mk_batch batch);
+/* Block Krylov for right-hand sides sharing a shift (block = 4; 1 restores per-system FGMRES):
+ * every 4 consecutive systems (one shift, one size) run their first restart cycle as one block
+ * FGMRES: a shared basis of 4-vector blocks, block classical Gram-Schmidt twice whose projections
+ * V^T W (the group-diagonal 4 x 4 blocks) run on FP64 matrix cores (v_mfma_f64_16x16x4f64),
+ * Cholesky QR of each new block and Givens rotations on the block Hessenberg least-squares
+ * problem. A system the block cycle leaves unconverged (or a group whose residual block is rank
+ * deficient) continues with its own FGMRES restarts from the block cycle's iterate. */
+mk_status mk_batch_set_block(mk_batch batch, int32_t block);
 /* graph replay of the batched Arnoldi steps of mk_batch_solve (default on; MK_BATCH_GRAPHS=0
  * turns it off at start); replays run the same kernels with the same arguments. While a step is
  * captured the runtime rejects synchronous (legacy-stream) HIP calls of other host threads:

@@ -111,6 +111,7 @@ SIGNATURES = {
     "mk_batch_create": ([P, P, i64, P, C.POINTER(P)], i32),
     "mk_batch_create_sized": ([P, P, i64, P, P, C.POINTER(P)], i32),
     "mk_batch_graphs_enable": ([i32], i32),
+    "mk_batch_set_block": ([P, i32], i32),
     "mk_batch_destroy": ([P], i32),
     "mk_batch_solve": ([P, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),
     "mk_batch_solve_device": ([P, P, P, P, C.POINTER(GmresOptions), C.POINTER(SolverResult)], i32),

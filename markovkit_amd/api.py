@@ -477,9 +477,12 @@ class BatchSolver:
     stopping on its own tolerance (mk_batch_create / mk_batch_solve). The multigrid's
     precision selects the preconditioner; its own shift is not used."""
 
-    def __init__(self, Q: Generator, M: SectorMultigrid, shifts: Sequence[float], rows=None):
+    def __init__(self, Q: Generator, M: SectorMultigrid, shifts: Sequence[float], rows=None,
+                 block: int = 1):
         """rows (optional): per system, the leading rows it has (the end of a sector of the
-        box; for an absorbing one-species box, the size of the box of a smaller cap)."""
+        box; for an absorbing one-species box, the size of the box of a smaller cap).
+        block = 4: every 4 consecutive systems (one shift, one size) share a block Krylov basis
+        for their first restart cycle, projected by block Gram-Schmidt on matrix cores."""
         self.shifts = np.ascontiguousarray(shifts, np.float64)
         self._Q, self._M = Q, M  # the batch reads both handles
         self._h = C.c_void_p()
@@ -489,6 +492,8 @@ class BatchSolver:
         check(lib().mk_batch_create_sized(Q._h, M._h, len(self.shifts), _p(self.shifts),
                                           _p(self.rows) if self.rows is not None else None,
                                           C.byref(self._h)))
+        if block != 1:
+            check(lib().mk_batch_set_block(self._h, int(block)))
 
     def __del__(self):
         if getattr(self, "_h", None):

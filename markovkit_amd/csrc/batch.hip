@@ -88,6 +88,15 @@ struct mk_batch_s {
     double *host = nullptr; // pinned: hd mirror, y, scale, beta
     int *ihost = nullptr;   // pinned: cnt, rows, sel
     graph_cache graphs;
+    // block Krylov: with block = 4, every group of 4 consecutive slots with one shift and one row
+    // count runs its first restart cycle as one block FGMRES (shared basis, block Gram-Schmidt on
+    // matrix cores); lanes it leaves unconverged continue with per-slot restarts
+    int block = 1;
+    std::vector<char> block_pending; // per group: the block cycle is still to run
+    int npb = 1;                     // row partitions of the block projections
+    dbuf d_bh, bpartial;             // block coefficients (H1, H2, Gram, C) and partial sums
+    double *bh = nullptr;            // pinned mirror of H1, H2 and the Gram blocks
+    dbuf d_bcnt;
     // step graphs are captured only where no other host thread of the process runs device work:
     // during a capture the runtime rejects synchronous (legacy-stream) calls of other threads,
     // so the server's engines, which run beside other threads, launch eagerly
@@ -97,6 +106,7 @@ struct mk_batch_s {
         if (host) (void)hipHostFree(host);
         if (ihost) (void)hipHostFree(ihost);
         if (h_lim) (void)hipHostFree(h_lim);
+        if (bh) (void)hipHostFree(bh);
     }
 };
 
@@ -410,7 +420,7 @@ void push_limits(mk_batch_s &B, const char *mask) {
 }
 
 // V_0 = b - A x and beta_j = |r_j| for the lanes in mask
-void true_residual(mk_batch_s &B, const char *mask) {
+void true_residual(mk_batch_s &B, const char *mask, double *dst = nullptr) {
     auto &P = prof();
     const hipStream_t s = stream();
     const i64 n = B.n, Bp = B.Bp;
@@ -421,7 +431,7 @@ void true_residual(mk_batch_s &B, const char *mask) {
     const double nb = double(n) * double(Bp);
     P.run(K_SPMV, op_bytes(B.gen->A, 0, 0, 0) + nb * 24.0, [&] {
         bk::shifted_apply<double>(B.gen->A.view, dp<double>(B.d_shift), dp<double>(B.d_x), dp<double>(B.d_b),
-                                  dp<double>(B.V), rp, Bp, dp<int>(B.d_lim), s);
+                                  dst ? dst : dp<double>(B.V), rp, Bp, dp<int>(B.d_lim), s);
     });
     P.run(K_DOT, double(nrp) * Bp * 8.0,
           [&] { bk::reduce_partials(rp, nrp, Bp, hd + sl.rn(Bp), s, dp<double>(B.rtmp)); });
@@ -447,9 +457,28 @@ void settle(slot_state &S) {
     S.done = !S.active;
 }
 
+// the initial residuals of the slots admitted since the last step
+void take_initial_residuals(mk_batch_s &B) {
+    std::vector<char> mask(B.Bp, 0);
+    bool any_fresh = false;
+    for (i64 j = 0; j < B.Bp; ++j) {
+        mask[j] = B.slot[j].fresh ? 1 : 0;
+        any_fresh = any_fresh || mask[j];
+    }
+    if (!any_fresh) return;
+    true_residual(B, mask.data());
+    for (i64 j = 0; j < B.Bp; ++j) {
+        auto &S = B.slot[j];
+        if (!S.fresh) continue;
+        S.fresh = false;
+        S.res = mk_solver_result{0, 0.0, 0};
+        settle(S);
+    }
+}
+
 // One restart cycle of FGMRES(m) with delayed CGS2 for the active slots (the per-system
-// algorithm of fgmres in api.hip), then x_j += Z y_j and the true residual. Fresh slots take their
-// initial residual first. Returns whether any slot is still active.
+// algorithm of fgmres in api.hip), then x_j += Z y_j and the true residual. Returns whether any
+// slot is still active.
 template <typename ZT>
 bool step(mk_batch_s &B) {
     const i64 n = B.n, Bp = B.Bp, m = B.m;
@@ -468,23 +497,6 @@ bool step(mk_batch_s &B) {
     int *cnt_h = B.ihost;
     const double nb = double(vec);
     std::vector<char> mask(Bp, 0);
-
-    // initial residuals of the slots admitted since the last step
-    bool any_fresh = false;
-    for (i64 j = 0; j < Bp; ++j) {
-        mask[j] = B.slot[j].fresh ? 1 : 0;
-        any_fresh = any_fresh || mask[j];
-    }
-    if (any_fresh) {
-        true_residual(B, mask.data());
-        for (i64 j = 0; j < Bp; ++j) {
-            auto &S = B.slot[j];
-            if (!S.fresh) continue;
-            S.fresh = false;
-            S.res = mk_solver_result{0, 0.0, 0};
-            settle(S);
-        }
-    }
     bool any = false;
     for (i64 j = 0; j < Bp; ++j) {
         mask[j] = B.slot[j].active ? 1 : 0;
@@ -616,9 +628,316 @@ bool step(mk_batch_s &B) {
     return still;
 }
 
+// ---- block FGMRES cycle of the pending groups ----------------------------------------------
+constexpr int kP = bk::kBlock;
+
+// G = R^T R with R upper triangular (kP x kP, row-major); false when G is not numerically
+// positive definite (the block is rank deficient)
+bool block_cholesky(const double *G, double *R) {
+    for (int i = 0; i < kP * kP; ++i) R[i] = 0.0;
+    for (int j = 0; j < kP; ++j) {
+        double d = G[j * kP + j];
+        for (int k = 0; k < j; ++k) d -= R[k * kP + j] * R[k * kP + j];
+        if (!(d > 1e-24 * std::max(G[j * kP + j], 1e-300)) || !std::isfinite(d)) return false;
+        R[j * kP + j] = std::sqrt(d);
+        for (int i = j + 1; i < kP; ++i) {
+            double v = G[j * kP + i];
+            for (int k = 0; k < j; ++k) v -= R[k * kP + j] * R[k * kP + i];
+            R[j * kP + i] = v / R[j * kP + j];
+        }
+    }
+    return true;
+}
+
+// the inverse of an upper triangular kP x kP matrix
+void upper_inverse(const double *R, double *Ri) {
+    for (int i = 0; i < kP * kP; ++i) Ri[i] = 0.0;
+    for (int c = 0; c < kP; ++c) {
+        for (int r = c; r >= 0; --r) {
+            double v = r == c ? 1.0 : 0.0;
+            for (int k = r + 1; k <= c; ++k) v -= R[r * kP + k] * Ri[k * kP + c];
+            Ri[r * kP + c] = v / R[r * kP + r];
+        }
+    }
+}
+
+// the block least-squares problem of one group: the block Hessenberg matrix ((m + 1) kP x m kP,
+// row-major), its right-hand side (m + 1) kP x kP, and the Givens rotations applied so far
+struct block_ls {
+    i64 m = 0, cols = 0;
+    std::vector<double> H, g;
+    struct rot {
+        i64 i, j;
+        double c, s;
+    };
+    std::vector<rot> rots;
+    void start(i64 m_, const double *S0) {
+        m = m_;
+        cols = 0;
+        H.assign((m + 1) * kP * m * kP, 0.0);
+        g.assign((m + 1) * kP * kP, 0.0);
+        for (int a = 0; a < kP; ++a)
+            for (int b = 0; b < kP; ++b) g[a * kP + b] = S0[a * kP + b];
+        rots.clear();
+    }
+    double &h(i64 r, i64 c) { return H[r * m * kP + c]; }
+    // block column c: projections Hc[(i kP + a) kP + b] for i <= c and the new block's R; returns
+    // the residual norm estimate of each right-hand side (column of g)
+    void add(i64 c, const double *Hc, const double *R, double *est) {
+        const i64 c0 = c * kP, nc = (c + 1) * kP;
+        for (i64 i = 0; i <= c; ++i)
+            for (int a = 0; a < kP; ++a)
+                for (int b = 0; b < kP; ++b) h(i * kP + a, c0 + b) = Hc[(i * kP + a) * kP + b];
+        for (int a = 0; a < kP; ++a)
+            for (int b = 0; b < kP; ++b) h(nc + a, c0 + b) = R[a * kP + b];
+        for (const auto &q : rots)
+            for (int b = 0; b < kP; ++b) {
+                const double x = h(q.i, c0 + b), y = h(q.j, c0 + b);
+                h(q.i, c0 + b) = q.c * x + q.s * y;
+                h(q.j, c0 + b) = -q.s * x + q.c * y;
+            }
+        for (int b = 0; b < kP; ++b) {
+            const i64 col = c0 + b;
+            for (i64 r = col + 1; r <= col + kP; ++r) {
+                const double x = h(col, col), y = h(r, col);
+                if (y == 0.0) continue;
+                const double d = std::hypot(x, y);
+                const rot q{col, r, x / d, y / d};
+                for (i64 cc = col; cc < nc; ++cc) {
+                    const double u = h(col, cc), v = h(r, cc);
+                    h(col, cc) = q.c * u + q.s * v;
+                    h(r, cc) = -q.s * u + q.c * v;
+                }
+                for (int k = 0; k < kP; ++k) {
+                    const double u = g[col * kP + k], v = g[r * kP + k];
+                    g[col * kP + k] = q.c * u + q.s * v;
+                    g[r * kP + k] = -q.s * u + q.c * v;
+                }
+                rots.push_back(q);
+            }
+        }
+        cols = c + 1;
+        for (int k = 0; k < kP; ++k) {
+            double ss = 0.0;
+            for (int a = 0; a < kP; ++a) ss += g[(nc + a) * kP + k] * g[(nc + a) * kP + k];
+            est[k] = std::sqrt(ss);
+        }
+    }
+    // Y = R^-1 g over the first cols block columns (row-major, cols kP x kP)
+    void solve(std::vector<double> &Y) {
+        const i64 N = cols * kP;
+        Y.assign(N * kP, 0.0);
+        for (int k = 0; k < kP; ++k)
+            for (i64 i = N; i-- > 0;) {
+                double v = g[i * kP + k];
+                for (i64 j = i + 1; j < N; ++j) v -= h(i, j) * Y[j * kP + k];
+                Y[i * kP + k] = v / h(i, i);
+            }
+    }
+};
+
+// One block FGMRES(m) cycle for every pending group with an active slot: the residual block is
+// orthonormalised (Cholesky QR), and each step applies the preconditioner to the kP basis vectors
+// (one batched cycle), projects the operator's image on the block basis twice (block CGS2, the
+// projections on matrix cores), orthonormalises the remainder by Cholesky QR and extends the
+// block Hessenberg least-squares problem. Then x += Z Y and the true residual per slot; the slots
+// left unconverged continue with per-slot restarts.
+template <typename ZT>
+void block_cycle(mk_batch_s &B) {
+    const i64 n = B.n, Bp = B.Bp, m = B.m, G = Bp / kP;
+    auto &P = prof();
+    const hipStream_t s = stream();
+    auto &A = *B.gen;
+    const i64 vec = n * Bp;
+    const double nb = double(vec);
+    double *V = dp<double>(B.V), *w = dp<double>(B.w), *part = dp<double>(B.bpartial);
+    ZT *Z = dp<ZT>(B.Z);
+    const double *sh = dp<double>(B.d_shift);
+    const i64 hsz = (m + 1) * Bp * kP;
+    double *H1 = dp<double>(B.d_bh), *H2 = H1 + hsz, *Gd = H2 + hsz, *Cd = Gd + Bp * kP;
+    const int *lims = dp<int>(B.d_lim);
+    std::vector<char> run(G, 0), mask(Bp, 0);
+    bool any = false;
+    for (i64 g = 0; g < G; ++g) {
+        if (!B.block_pending[g]) continue;
+        B.block_pending[g] = 0;
+        bool on = false;
+        for (int b = 0; b < kP; ++b) on = on || B.slot[g * kP + b].active;
+        if (!on) continue;
+        run[g] = 1;
+        any = true;
+        for (int b = 0; b < kP; ++b) mask[g * kP + b] = B.slot[g * kP + b].rows > 0 ? 1 : 0;
+    }
+    if (!any) return;
+    // the projections on `count` block vectors of v, reduced into out
+    auto project = [&](const double *v, int count, const double *wv, double *out) {
+        P.run(K_DOT, double(count + 1) * nb * 8.0, [&] {
+            bk::block_dot(v, count, wv, n, Bp, part, B.npb, lims, s);
+            bk::reduce_partials(part, B.npb, static_cast<i64>(count) * Bp * kP, out, s);
+        });
+    };
+    auto fetch = [&](const double *src, i64 count, double *dst) {
+        HIPCHK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, s));
+    };
+    std::vector<double> C(Bp * kP, 0.0);
+    auto put_coefficients = [&](i64 count) {
+        HIPCHK(hipMemcpy(Cd, C.data(), count * sizeof(double), hipMemcpyHostToDevice));
+    };
+    // the residual block: Gram matrix, Cholesky, V_0 = R S0^-1 (through w)
+    push_limits(B, mask.data());
+    project(V, 1, V, Gd);
+    fetch(Gd, Bp * kP, B.bh);
+    drain();
+    std::vector<block_ls> ls(G);
+    double Gm[kP * kP], R[kP * kP], Ri[kP * kP];
+    for (i64 g = 0; g < G; ++g) {
+        if (!run[g]) continue;
+        for (int a = 0; a < kP; ++a)
+            for (int b = 0; b < kP; ++b) Gm[a * kP + b] = B.bh[(g * kP + b) * kP + a];
+        if (!block_cholesky(Gm, R)) {
+            run[g] = 0; // a rank-deficient residual block: per-slot restarts take over
+            for (int b = 0; b < kP; ++b) mask[g * kP + b] = 0;
+            continue;
+        }
+        ls[g].start(m, R);
+        upper_inverse(R, Ri);
+        for (int a = 0; a < kP; ++a)
+            for (int b = 0; b < kP; ++b) C[(g * kP + b) * kP + a] = Ri[a * kP + b];
+    }
+    push_limits(B, mask.data());
+    put_coefficients(Bp * kP);
+    P.run(K_OTHER, nb * 40.0, [&] { bk::block_apply<double>(w, V, 1, Cd, n, Bp, 1.0, true, lims, s); });
+    std::fill(C.begin(), C.end(), 0.0);
+    for (i64 j = 0; j < Bp; ++j) C[j * kP + (j % kP)] = 1.0; // identity: V_0 = w
+    put_coefficients(Bp * kP);
+    P.run(K_OTHER, nb * 40.0, [&] { bk::block_apply<double>(V, w, 1, Cd, n, Bp, 1.0, true, lims, s); });
+    std::vector<char> iterated(mask);
+    std::vector<double> Hc((m + 1) * kP * kP);
+    double est[kP];
+    auto any_run = [&] {
+        for (i64 g = 0; g < G; ++g)
+            if (run[g]) return true;
+        return false;
+    };
+    for (i64 c = 0; c < m && any_run(); ++c) {
+        const int cnt = static_cast<int>(c + 1);
+        push_limits(B, mask.data());
+        ZT *Zc = Z + c * vec;
+        brun_level<ZT>(B, 0, nullptr, V + c * vec, nullptr, Zc, lims);
+        P.run(K_SPMV, op_bytes(A.A, 0, 0, 0) + nb * (sizeof(ZT) + 8.0), [&] {
+            bk::shifted_apply<ZT>(A.A.view, sh, Zc, nullptr, w, nullptr, Bp, lims, s);
+        });
+        // block CGS2: two projections of w on the basis blocks, then its Gram matrix
+        project(V, cnt, w, H1);
+        P.run(K_UPDATE, double(cnt + 2) * nb * 8.0, [&] {
+            bk::block_apply<double>(w, V, cnt, H1, n, Bp, -1.0, false, lims, s);
+        });
+        project(V, cnt, w, H2);
+        P.run(K_UPDATE, double(cnt + 2) * nb * 8.0, [&] {
+            bk::block_apply<double>(w, V, cnt, H2, n, Bp, -1.0, false, lims, s);
+        });
+        project(w, 1, w, Gd);
+        fetch(H1, cnt * Bp * kP, B.bh);
+        fetch(H2, cnt * Bp * kP, B.bh + hsz);
+        fetch(Gd, Bp * kP, B.bh + 2 * hsz);
+        drain();
+        std::fill(C.begin(), C.end(), 0.0);
+        for (i64 g = 0; g < G; ++g) {
+            if (!run[g]) continue;
+            for (i64 i = 0; i < cnt; ++i)
+                for (int a = 0; a < kP; ++a)
+                    for (int b = 0; b < kP; ++b) {
+                        const i64 o = (i * Bp + g * kP + b) * kP + a;
+                        Hc[(i * kP + a) * kP + b] = B.bh[o] + B.bh[hsz + o];
+                    }
+            for (int a = 0; a < kP; ++a)
+                for (int b = 0; b < kP; ++b) Gm[a * kP + b] = B.bh[2 * hsz + (g * kP + b) * kP + a];
+            const bool full = block_cholesky(Gm, R);
+            if (!full) std::fill(R, R + kP * kP, 0.0);
+            ls[g].add(c, Hc.data(), R, est);
+            bool done = true, out_of_iterations = false;
+            for (int b = 0; b < kP; ++b) {
+                auto &S = B.slot[g * kP + b];
+                if (S.rows == 0) continue;
+                ++S.res.iterations;
+                done = done && !(est[b] > S.tol);
+                out_of_iterations = out_of_iterations || S.res.iterations >= S.max_iterations;
+            }
+            if (done || !full || out_of_iterations) {
+                run[g] = 0;
+                for (int b = 0; b < kP; ++b) mask[g * kP + b] = 0;
+                continue;
+            }
+            upper_inverse(R, Ri);
+            for (int a = 0; a < kP; ++a)
+                for (int b = 0; b < kP; ++b) C[(g * kP + b) * kP + a] = Ri[a * kP + b];
+        }
+        if (c + 1 < m && any_run()) {
+            push_limits(B, mask.data());
+            put_coefficients(Bp * kP);
+            P.run(K_OTHER, nb * 40.0, [&] {
+                bk::block_apply<double>(V + (c + 1) * vec, w, 1, Cd, n, Bp, 1.0, true, lims, s);
+            });
+        }
+    }
+    // x += Z Y per group: coefficients (i Bp + j) kP + a = Y[i kP + a][j mod kP]
+    std::vector<double> Yc(m * Bp * kP, 0.0), Y;
+    std::vector<int> counts(Bp, 0);
+    i64 most = 0;
+    for (i64 g = 0; g < G; ++g) {
+        if (ls[g].cols == 0) continue;
+        ls[g].solve(Y);
+        for (i64 i = 0; i < ls[g].cols; ++i)
+            for (int a = 0; a < kP; ++a)
+                for (int b = 0; b < kP; ++b) Yc[(i * Bp + g * kP + b) * kP + a] = Y[(i * kP + a) * kP + b];
+        for (int b = 0; b < kP; ++b) counts[g * kP + b] = static_cast<int>(ls[g].cols);
+        most = std::max(most, ls[g].cols);
+    }
+    std::vector<char> updated(Bp, 0);
+    for (i64 j = 0; j < Bp; ++j) updated[j] = counts[j] > 0 && B.slot[j].rows > 0 ? 1 : 0;
+    HIPCHK(hipMemcpy(Cd, Yc.data(), m * Bp * kP * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(B.d_bcnt.p, counts.data(), Bp * sizeof(int), hipMemcpyHostToDevice));
+    push_limits(B, updated.data());
+    P.run(K_OTHER, nb * (most * sizeof(ZT) * kP + 16.0), [&] {
+        bk::block_apply<ZT>(dp<double>(B.d_x), Z, static_cast<int>(most), Cd, n, Bp, 1.0, false, lims, s,
+                            dp<int>(B.d_bcnt));
+    });
+    true_residual(B, iterated.data());
+    for (i64 j = 0; j < Bp; ++j)
+        if (iterated[j] && B.slot[j].active) settle(B.slot[j]);
+}
+
 bool engine_step(mk_batch_s &B) {
+    take_initial_residuals(B);
+    // groups still owed their block cycle run it before any per-slot restart
+    bool pending = false;
+    for (char p : B.block_pending) pending = pending || p;
+    if (pending) {
+        if (B.mg->precision == MK_FP32) {
+            block_cycle<float>(B);
+        } else {
+            block_cycle<double>(B);
+        }
+        for (const auto &S : B.slot)
+            if (S.active) return true;
+        return false;
+    }
     if (B.mg->precision == MK_FP32) return step<float>(B);
     return step<double>(B);
+}
+
+// the block workspace for restart m
+void ensure_block(mk_batch_s &B) {
+    const i64 m = B.m, Bp = B.Bp;
+    const std::size_t coeff = static_cast<std::size_t>((2 * (m + 1) + 1 + m) * Bp * kP);
+    if (B.d_bh.bytes >= coeff * sizeof(double)) return;
+    B.npb = std::min(bk::parts(B.n), 128);
+    B.d_bh = dbuf(coeff * sizeof(double));
+    B.bpartial = dbuf(static_cast<std::size_t>(B.npb) * (m + 1) * Bp * kP * sizeof(double));
+    B.d_bcnt = dbuf(Bp * sizeof(int));
+    if (B.bh) (void)hipHostFree(B.bh);
+    B.bh = nullptr;
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&B.bh), (2 * (m + 1) + 1) * Bp * kP * sizeof(double)));
 }
 
 // zero the tails (rows past each lane's rows) of the solution, the right-hand side and every
@@ -700,6 +1019,11 @@ void batch_solve_device(mk_batch_s &B, const double *d_B, double *d_X, const dou
     const i64 m = std::max<i64>(1, std::min<i64>(o.restart, B.n));
     require(m <= gpu::kFusedBasis, "batch gmres: restart above 64 is not supported");
     ensure_workspace(B, m);
+    B.block_pending.assign(B.Bp / kP, 0);
+    if (B.block > 1) {
+        ensure_block(B);
+        for (i64 g = 0; g * kP < B.nsys; ++g) B.block_pending[g] = 1;
+    }
     for (i64 j = 0; j < B.Bp; ++j) {
         B.ls[j] = hessenberg_ls(m);
         auto &S = B.slot[j];
@@ -835,6 +1159,22 @@ mk_status mk_batch_solve(mk_batch b, const double *B, double *X, const double *t
         HIPCHK(hipMemcpy(dX, X, bytes, hipMemcpyHostToDevice));
         batch_solve_device(*b, dB, dX, tolerances, *options, results);
         HIPCHK(hipMemcpy(X, dX, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+mk_status mk_batch_set_block(mk_batch b, int32_t block) {
+    return guarded([&] {
+        require(b && (block == 1 || block == kP), "batch: block must be 1 or 4");
+        if (block > 1) {
+            require(b->nsys % kP == 0, "batch: block solves take whole groups of 4 systems");
+            for (i64 g = 0; g * kP < b->nsys; ++g)
+                for (int q = 1; q < kP; ++q) {
+                    const auto &a = b->slot[g * kP], &c = b->slot[g * kP + q];
+                    require(a.shift == c.shift && a.rows == c.rows,
+                            "batch: the 4 systems of a block group need one shift and one size");
+                }
+        }
+        b->block = block;
     });
 }
 

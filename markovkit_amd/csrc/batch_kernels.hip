@@ -447,6 +447,80 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// ---- block Krylov (groups of kBlock consecutive lanes share one basis) ----
+// Projections of a block basis on a block: H[i][j][a] = sum_r V_i[r][4 g + a] W[r][j] for every
+// lane j of group g = j / 4, i < count, on FP64 matrix cores. A wave takes 16 lanes (4 groups)
+// and 4 rows per v_mfma_f64_16x16x4f64 step, the rows being the reduction dimension:
+//   A[m][k] = V_i[r + k][16 s + m] (lane l: m = l % 16, k = l / 16),
+//   B[k][n] = W[r + k][16 s + n]   (the same element position),
+//   D[m][n] = sum_k A[m][k] B[k][n] (lane l, register q: m = 4 q + l / 16, n = l % 16),
+// so lane l's register q = (l % 16) / 4 holds the group-diagonal entry m = 4 q + l / 16 of column n:
+// H[i][16 s + n][a = l / 16]. The off-diagonal 4 x 4 blocks (lanes of different groups) are
+// computed and dropped. grid (np partitions of the rows, basis chunks of kDotChunk, Bp / 64).
+typedef double mk_double4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double pick4(const mk_double4 &v, int q) {
+    return q == 0 ? v[0] : q == 1 ? v[1] : q == 2 ? v[2] : v[3];
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_block_dot(const double *V, int count, const double *W, i64 n, i64 Bp, double *partial, int np,
+                const int *limit) {
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const i64 col = static_cast<i64>(blockIdx.z) * kLanes + wave * 16 + (lane & 15);
+    const int kr = lane >> 4;
+    const i64 lim = limit ? min(n, static_cast<i64>(limit[col])) : n;
+    const int k0 = blockIdx.y * kDotChunk;
+    const int kc = min(kDotChunk, count - k0);
+    const i64 ld = n * Bp;
+    const i64 per = ((n + np - 1) / np + 3) / 4 * 4;
+    const i64 r0 = static_cast<i64>(blockIdx.x) * per, r1 = min(n, r0 + per);
+    mk_double4 acc[kDotChunk];
+#pragma unroll
+    for (int k = 0; k < kDotChunk; ++k) acc[k] = mk_double4{0.0, 0.0, 0.0, 0.0};
+    for (i64 r = r0; r < r1; r += 4) {
+        const i64 row = r + kr;
+        const bool ok = row < r1 && row < lim;
+        const i64 o = row * Bp + col;
+        const double w = ok ? W[o] : 0.0;
+        double v[kDotChunk];
+#pragma unroll
+        for (int k = 0; k < kDotChunk; ++k) v[k] = (ok && k < kc) ? V[(k0 + k) * ld + o] : 0.0;
+#pragma unroll
+        for (int k = 0; k < kDotChunk; ++k)
+            acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[k], w, acc[k], 0, 0, 0);
+    }
+    const int q = (lane & 15) >> 2;
+    const i64 m = static_cast<i64>(count) * Bp * kBlock;
+    double *out = partial + static_cast<i64>(blockIdx.x) * m;
+#pragma unroll
+    for (int k = 0; k < kDotChunk; ++k)
+        if (k < kc) out[((k0 + k) * Bp + col) * kBlock + kr] = pick4(acc[k], q);
+}
+
+// out[r][j] = sign sum_{i < count} sum_a V_i[r][4 g + a] C[i][j][a] (+ out[r][j] unless assign), for
+// the rows r < limit[j] of every lane j (group g = j / 4): the block updates W -= V H, the scaling
+// V_{k+1} = W R^-1 and the solution update X += Z Y
+template <typename IN>
+__global__ void __launch_bounds__(kThreads)
+    k_block_apply(double *out, const IN *V, int count, const double *C, i64 n, i64 Bp, double sign,
+                  int assign, const int *limit, const int *counts) {
+    i64 r, j;
+    row_lane(r, j);
+    if (r >= n || (limit && r >= limit[j])) return;
+    const i64 ld = n * Bp, base = r * Bp + (j & ~static_cast<i64>(kBlock - 1));
+    const int cnt = counts ? counts[j] : count;
+    double s = 0.0;
+    for (int i = 0; i < cnt; ++i) {
+        const IN *v = V + i * ld + base;
+        const double *c = C + (static_cast<i64>(i) * Bp + j) * kBlock;
+        s += ((static_cast<double>(v[0]) * c[0] + static_cast<double>(v[1]) * c[1]) +
+              static_cast<double>(v[2]) * c[2]) + static_cast<double>(v[3]) * c[3];
+    }
+    const i64 o = r * Bp + j;
+    out[o] = assign ? sign * s : out[o] + sign * s;
+}
+
 // dst[r][j] = r < rows ? src[r] : 0 for one lane j
 __global__ void __launch_bounds__(kThreads)
     k_put_lane(const double *src, i64 rows, double *dst, i64 n, i64 Bp, i64 j) {
@@ -494,6 +568,26 @@ void clear_tails(T *X, int count, i64 ld, i64 n, i64 Bp, const int *rows, const 
 }
 template void clear_tails<float>(float *, int, i64, i64, i64, const int *, const int *, hipStream_t);
 template void clear_tails<double>(double *, int, i64, i64, i64, const int *, const int *, hipStream_t);
+
+void block_dot(const double *V, int count, const double *W, i64 n, i64 Bp, double *partial,
+               int np, const int *limit, hipStream_t s) {
+    if (count <= 0 || n == 0) return;
+    hipLaunchKernelGGL(k_block_dot, dim3(np, (count + kDotChunk - 1) / kDotChunk, Bp / kLanes),
+                       dim3(kThreads), 0, s, V, count, W, n, Bp, partial, np, limit);
+}
+
+template <typename IN>
+void block_apply(double *out, const IN *V, int count, const double *C, i64 n, i64 Bp, double sign,
+                 bool assign, const int *limit, hipStream_t s, const int *counts) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_block_apply<IN>), dim3(static_cast<u32>(row_blocks(n)), Bp / kLanes),
+                       dim3(kThreads), 0, s, out, V, count, C, n, Bp, sign, assign ? 1 : 0, limit,
+                       counts);
+}
+template void block_apply<double>(double *, const double *, int, const double *, i64, i64, double,
+                                  bool, const int *, hipStream_t, const int *);
+template void block_apply<float>(double *, const float *, int, const double *, i64, i64, double,
+                                 bool, const int *, hipStream_t, const int *);
 
 int parts(i64 n) { return op_parts(n); }
 

@@ -19,6 +19,7 @@
 namespace mk::gpu::batch {
 
 constexpr int kLanes = 64; // systems per wavefront
+constexpr int kBlock = 4;  // systems per block of the block Krylov method (one shift, 4 rhs)
 
 // row partitions used by the batched reductions for an n-row operand
 int parts(i64 n);
@@ -94,6 +95,17 @@ void basis_update(double *w, const double *V, int count, const double *h, i64 n,
 template <typename ZT>
 void combine(double *x, const ZT *Z, const int *cnt, const double *y, i64 n, i64 Bp,
              hipStream_t s);
+// ---- block Krylov: lanes 4 g .. 4 g + 3 form group g, which shares one basis
+// partial[p * (count Bp 4) + (i Bp + j) 4 + a] = sum over partition p's rows of V_i[r][4 g + a]
+// W[r][j], i < count, g = j / 4: the group-diagonal blocks of V^T W on FP64 matrix cores
+// (v_mfma_f64_16x16x4f64)
+void block_dot(const double *V, int count, const double *W, i64 n, i64 Bp, double *partial,
+               int np, const int *limit, hipStream_t s);
+// out[r][j] = sign sum_{i < count, a < 4} V_i[r][4 g + a] C[(i Bp + j) 4 + a], added to out unless
+// assign; rows r < limit[j]; counts (when given) replaces count per lane
+template <typename IN>
+void block_apply(double *out, const IN *V, int count, const double *C, i64 n, i64 Bp, double sign,
+                 bool assign, const int *limit, hipStream_t s, const int *counts = nullptr);
 // one lane j of a state-major vector: dst[r * Bp + j] = src[r] for r < rows, 0 for rows <= r < n
 void put_lane(const double *src, i64 rows, double *dst, i64 n, i64 Bp, i64 j, hipStream_t s);
 // dst[r] = src[r * Bp + j] for r < rows

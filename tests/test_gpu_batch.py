@@ -313,3 +313,64 @@ def test_batch_graphs_and_mix_are_bitwise(small):
         mk.batch_graphs_enable(True)
     assert np.array_equal(Xe, X)
     assert [r.iterations for r in re] == [r.iterations for r in res]
+
+
+@pytest.mark.parametrize("smoother", [mk.JACOBI, mk.RED_BLACK], ids=["jacobi", "red_black"])
+def test_block_krylov_groups(small, smoother):
+    """Block FGMRES (groups of 4 right-hand sides per shift sharing a basis, block Gram-Schmidt
+    on matrix cores): every system reaches its tolerance, lies within the L1 bound of the
+    per-system batch's solution and of the oracle's, and its sector sums match the lumped closed
+    form; a group takes no more block steps than its slowest system takes per-system
+    iterations."""
+    w, box, Q = small
+    n = box.size()
+    group_shifts = [0.05, 0.5, 5.0, 50.0, 0.3]
+    shifts = [s for s in group_shifts for _ in range(4)]
+    rhs = rhs_set(n, [30, 31, 32, 33])
+    B = np.array([b for _ in group_shifts for b in rhs])
+    tol = 1e-10 * np.linalg.norm(B, axis=1)
+    M = mk.SectorMultigrid(w.model.domain, box, 1.0,
+                           mk.MultigridOptions(precision=mk.FP32, smoother=smoother))
+    Xb, rb = mk.BatchSolver(Q, M, shifts, block=4).solve(B, tolerances=tol)
+    Xs, rs = mk.BatchSolver(Q, M, shifts).solve(B, tolerances=tol)
+    P = Problem(w.model, w.max_totals)
+    begins = [box.sector_begin(k) for k in range(box.n_sectors())] + [n]
+    absA = 2 * Q.exit_rates() - Q.leak_rates()
+    for j, s in enumerate(shifts):
+        assert rb[j].converged, (j, rb[j])
+        r = B[j] - Q.apply_shifted(s, Xb[j])
+        assert np.linalg.norm(r) <= 1.0001 * tol[j]
+        bound = (np.abs(r).sum() + l1_residual(Q, s, B[j], Xs[j])) / s
+        assert np.abs(Xb[j] - Xs[j]).sum() <= bound
+        Bk = np.array([B[j][begins[k]:begins[k + 1]].sum() for k in range(4)])
+        Xk = np.array([Xb[j][begins[k]:begins[k + 1]].sum() for k in range(4)])
+        lb = (np.abs(r).sum() + 16 * EPS * ((s + absA) * np.abs(Xb[j])).sum()) / s
+        assert np.abs(Xk - lumped_sector_sums(s, 0.05, 0.02, 3, Bk)).sum() <= lb
+    for g in range(len(group_shifts)):
+        blk = [rb[4 * g + q].iterations for q in range(4)]
+        per = [rs[4 * g + q].iterations for q in range(4)]
+        assert max(blk) <= max(per), (g, blk, per)
+        print(f"[block] s={group_shifts[g]} block steps {blk} per-system iterations {per}")
+    x_ref, _ = P.gmres(shifts[0], B[0], mg=OracleMG(P, shifts[0]), tol=tol[0], restart=50)
+    bound = (np.abs(B[0] - P.apply_shifted(shifts[0], Xb[0])).sum() +
+             np.abs(B[0] - P.apply_shifted(shifts[0], x_ref)).sum()) / shifts[0]
+    assert np.abs(Xb[0] - x_ref).sum() <= bound
+
+
+def test_block_krylov_multishift_256():
+    """The 256-system multi-shift configuration (64 shifts x 4 rhs, 58,905 states) with block
+    groups: every system reaches 1e-10 relative residual."""
+    w = config_multishift()
+    box, Q = build(w)
+    n = box.size()
+    rhs = rhs_set(n, w.seeds)
+    shifts = [s for s in w.shifts for _ in rhs]
+    B = np.array([b for _ in w.shifts for b in rhs])
+    tol = 1e-10 * np.linalg.norm(B, axis=1)
+    M = mk.SectorMultigrid(w.model.domain, box, 1.0, mk.MultigridOptions(precision=mk.FP32))
+    X, res = mk.BatchSolver(Q, M, shifts, block=4).solve(B, tolerances=tol)
+    its = [r.iterations for r in res]
+    print(f"[block multishift] iterations min {min(its)} max {max(its)}")
+    for j, s in enumerate(shifts):
+        assert res[j].converged, (j, res[j])
+        assert np.linalg.norm(B[j] - Q.apply_shifted(s, X[j])) <= 1.0001 * tol[j]

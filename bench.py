@@ -311,7 +311,7 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16, tune=True):
              "batch_p50_ms": 1e3 * percentile(blat, 50), "iterations_min": min(bits),
              "iterations_max": max(bits), "worst_relative_residual": bworst,
              "method": "block FGMRES(restart) per shift (4 rhs, shared basis, block CGS2 with "
-                       "V^T W on v_mfma_f64_16x16x4f64, Cholesky QR), then per-system restarts"}
+                       "V^T W on v_mfma_f64_16x16x4f64, Cholesky QR twice), then per-system restarts"}
     log(f"[multishift] block: {block['value']:.1f} solves/s, iterations {min(bits)}-{max(bits)}")
     return {
         "workload": f"{w.name}: {n} states, {len(shifts)} systems (64 shifts 1e-2..1e2 x 4 rhs), "

@@ -26,6 +26,7 @@
 #include "runtime.hpp"
 
 #include <cstring>
+#include <limits>
 #include <map>
 #include <type_traits>
 
@@ -95,6 +96,7 @@ struct mk_batch_s {
     std::vector<char> block_pending; // per group: the block cycle is still to run
     int npb = 1;                     // row partitions of the block projections
     dbuf d_bh, bpartial;             // block coefficients (H1, H2, Gram, C) and partial sums
+    dbuf bscratch, bscratch2;        // block vectors of the Cholesky QR passes
     double *bh = nullptr;            // pinned mirror of H1, H2 and the Gram blocks
     dbuf d_bcnt;
     // step graphs are captured only where no other host thread of the process runs device work:
@@ -661,6 +663,27 @@ void upper_inverse(const double *R, double *Ri) {
     }
 }
 
+// max |R_ii| / min |R_ii| of an upper triangular kP x kP matrix: a lower bound of its condition
+// number, used to end a block cycle before the block's columns become dependent
+double diag_ratio(const double *R) {
+    double lo = std::abs(R[0]), hi = lo;
+    for (int i = 1; i < kP; ++i) {
+        lo = std::min(lo, std::abs(R[i * kP + i]));
+        hi = std::max(hi, std::abs(R[i * kP + i]));
+    }
+    return lo > 0.0 ? hi / lo : std::numeric_limits<double>::infinity();
+}
+
+// R = A B for upper triangular kP x kP matrices
+void upper_product(const double *A, const double *B, double *R) {
+    for (int i = 0; i < kP; ++i)
+        for (int j = 0; j < kP; ++j) {
+            double v = 0.0;
+            for (int k = i; k <= j; ++k) v += A[i * kP + k] * B[k * kP + j];
+            R[i * kP + j] = i <= j ? v : 0.0;
+        }
+}
+
 // the block least-squares problem of one group: the block Hessenberg matrix ((m + 1) kP x m kP,
 // row-major), its right-hand side (m + 1) kP x kP, and the Givens rotations applied so far
 struct block_ls {
@@ -737,11 +760,12 @@ struct block_ls {
 };
 
 // One block FGMRES(m) cycle for every pending group with an active slot: the residual block is
-// orthonormalised (Cholesky QR), and each step applies the preconditioner to the kP basis vectors
-// (one batched cycle), projects the operator's image on the block basis twice (block CGS2, the
-// projections on matrix cores), orthonormalises the remainder by Cholesky QR and extends the
-// block Hessenberg least-squares problem. Then x += Z Y and the true residual per slot; the slots
-// left unconverged continue with per-slot restarts.
+// orthonormalised (Cholesky QR twice), and each step applies the preconditioner to the kP basis
+// vectors (one batched cycle), projects the operator's image on the block basis twice (block CGS2,
+// the projections on matrix cores), orthonormalises the remainder by Cholesky QR twice and extends
+// the block Hessenberg least-squares problem. A group whose new block is rank deficient or
+// ill-conditioned (diagonal ratio of R above 1e7) ends its cycle there. Then x += Z Y and the true
+// residual per slot; the slots left unconverged continue with per-slot restarts.
 template <typename ZT>
 void block_cycle(mk_batch_s &B) {
     const i64 n = B.n, Bp = B.Bp, m = B.m, G = Bp / kP;
@@ -783,32 +807,90 @@ void block_cycle(mk_batch_s &B) {
     auto put_coefficients = [&](i64 count) {
         HIPCHK(hipMemcpy(Cd, C.data(), count * sizeof(double), hipMemcpyHostToDevice));
     };
-    // the residual block: Gram matrix, Cholesky, V_0 = R S0^-1 (through w)
-    push_limits(B, mask.data());
-    project(V, 1, V, Gd);
-    fetch(Gd, Bp * kP, B.bh);
-    drain();
+    // Cholesky QR twice of the block src (src = dst R with dst orthonormal, R upper triangular),
+    // for the groups in `on`; dst != src; R per group into Rg; groups whose block is rank deficient
+    // or too ill-conditioned come back false in ok
+    std::vector<double> Rg(G * kP * kP, 0.0);
+    std::vector<char> ok(G, 0);
+    double Gm[kP * kP], R1[kP * kP], R2[kP * kP], Ri[kP * kP];
+    auto gram_of = [&](i64 g, const double *h) {
+        for (int a = 0; a < kP; ++a)
+            for (int b = 0; b < kP; ++b) Gm[a * kP + b] = h[(g * kP + b) * kP + a];
+    };
+    auto scale_by = [&](const std::vector<char> &on, double *dst, const double *src,
+                        const std::vector<double> &Rs) {
+        std::fill(C.begin(), C.end(), 0.0);
+        std::vector<char> lanes(Bp, 0);
+        for (i64 g = 0; g < G; ++g) {
+            if (!on[g]) continue;
+            upper_inverse(&Rs[g * kP * kP], Ri);
+            for (int a = 0; a < kP; ++a)
+                for (int b = 0; b < kP; ++b) C[(g * kP + b) * kP + a] = Ri[a * kP + b];
+            for (int b = 0; b < kP; ++b) lanes[g * kP + b] = B.slot[g * kP + b].rows > 0 ? 1 : 0;
+        }
+        push_limits(B, lanes.data());
+        put_coefficients(Bp * kP);
+        P.run(K_OTHER, nb * 40.0, [&] { bk::block_apply<double>(dst, src, 1, Cd, n, Bp, 1.0, true, lims, s); });
+    };
+    auto cholqr2 = [&](const std::vector<char> &on, const double *src, double *dst, double *tmp) {
+        std::vector<char> lanes(Bp, 0);
+        for (i64 g = 0; g < G; ++g)
+            if (on[g])
+                for (int b = 0; b < kP; ++b) lanes[g * kP + b] = B.slot[g * kP + b].rows > 0 ? 1 : 0;
+        push_limits(B, lanes.data());
+        project(src, 1, src, Gd);
+        fetch(Gd, Bp * kP, B.bh + 2 * hsz);
+        drain();
+        std::vector<char> live(G, 0);
+        std::vector<double> Rfirst(G * kP * kP, 0.0);
+        for (i64 g = 0; g < G; ++g) {
+            ok[g] = 0;
+            std::fill(&Rg[g * kP * kP], &Rg[(g + 1) * kP * kP], 0.0);
+            if (!on[g]) continue;
+            gram_of(g, B.bh + 2 * hsz);
+            if (!block_cholesky(Gm, R1)) continue;
+            // an ill-conditioned block keeps its R for the least-squares problem, but the group
+            // ends its cycle here
+            std::copy(R1, R1 + kP * kP, &Rg[g * kP * kP]);
+            if (diag_ratio(R1) > 1e7) continue;
+            std::copy(R1, R1 + kP * kP, &Rfirst[g * kP * kP]);
+            live[g] = 1;
+        }
+        scale_by(live, tmp, src, Rfirst);
+        push_limits(B, lanes.data());
+        project(tmp, 1, tmp, Gd);
+        fetch(Gd, Bp * kP, B.bh + 2 * hsz);
+        drain();
+        std::vector<double> Rsecond(G * kP * kP, 0.0);
+        for (i64 g = 0; g < G; ++g) {
+            if (!live[g]) continue;
+            gram_of(g, B.bh + 2 * hsz);
+            if (!block_cholesky(Gm, R2)) {
+                live[g] = 0;
+                continue;
+            }
+            std::copy(R2, R2 + kP * kP, &Rsecond[g * kP * kP]);
+            upper_product(R2, &Rfirst[g * kP * kP], &Rg[g * kP * kP]);
+            ok[g] = 1;
+        }
+        scale_by(live, dst, tmp, Rsecond);
+    };
+    // the residual block V_0 = R S0^-1 (through w and Z's last column as scratch)
+    double *scratch = reinterpret_cast<double *>(B.bscratch.p);
+    cholqr2(run, V, w, scratch);
     std::vector<block_ls> ls(G);
-    double Gm[kP * kP], R[kP * kP], Ri[kP * kP];
     for (i64 g = 0; g < G; ++g) {
         if (!run[g]) continue;
-        for (int a = 0; a < kP; ++a)
-            for (int b = 0; b < kP; ++b) Gm[a * kP + b] = B.bh[(g * kP + b) * kP + a];
-        if (!block_cholesky(Gm, R)) {
+        if (!ok[g]) {
             run[g] = 0; // a rank-deficient residual block: per-slot restarts take over
             for (int b = 0; b < kP; ++b) mask[g * kP + b] = 0;
             continue;
         }
-        ls[g].start(m, R);
-        upper_inverse(R, Ri);
-        for (int a = 0; a < kP; ++a)
-            for (int b = 0; b < kP; ++b) C[(g * kP + b) * kP + a] = Ri[a * kP + b];
+        ls[g].start(m, &Rg[g * kP * kP]);
     }
-    push_limits(B, mask.data());
-    put_coefficients(Bp * kP);
-    P.run(K_OTHER, nb * 40.0, [&] { bk::block_apply<double>(w, V, 1, Cd, n, Bp, 1.0, true, lims, s); });
     std::fill(C.begin(), C.end(), 0.0);
     for (i64 j = 0; j < Bp; ++j) C[j * kP + (j % kP)] = 1.0; // identity: V_0 = w
+    push_limits(B, mask.data());
     put_coefficients(Bp * kP);
     P.run(K_OTHER, nb * 40.0, [&] { bk::block_apply<double>(V, w, 1, Cd, n, Bp, 1.0, true, lims, s); });
     std::vector<char> iterated(mask);
@@ -827,7 +909,7 @@ void block_cycle(mk_batch_s &B) {
         P.run(K_SPMV, op_bytes(A.A, 0, 0, 0) + nb * (sizeof(ZT) + 8.0), [&] {
             bk::shifted_apply<ZT>(A.A.view, sh, Zc, nullptr, w, nullptr, Bp, lims, s);
         });
-        // block CGS2: two projections of w on the basis blocks, then its Gram matrix
+        // block CGS2: two projections of w on the basis blocks
         project(V, cnt, w, H1);
         P.run(K_UPDATE, double(cnt + 2) * nb * 8.0, [&] {
             bk::block_apply<double>(w, V, cnt, H1, n, Bp, -1.0, false, lims, s);
@@ -836,12 +918,13 @@ void block_cycle(mk_batch_s &B) {
         P.run(K_UPDATE, double(cnt + 2) * nb * 8.0, [&] {
             bk::block_apply<double>(w, V, cnt, H2, n, Bp, -1.0, false, lims, s);
         });
-        project(w, 1, w, Gd);
         fetch(H1, cnt * Bp * kP, B.bh);
         fetch(H2, cnt * Bp * kP, B.bh + hsz);
-        fetch(Gd, Bp * kP, B.bh + 2 * hsz);
         drain();
-        std::fill(C.begin(), C.end(), 0.0);
+        // the next block: V_{c+1} R = w by Cholesky QR twice (the last step's block is not kept)
+        const bool next = c + 1 < m;
+        double *vnext = next ? V + (c + 1) * vec : scratch;
+        cholqr2(run, w, vnext, scratch == vnext ? dp<double>(B.bscratch2) : scratch);
         for (i64 g = 0; g < G; ++g) {
             if (!run[g]) continue;
             for (i64 i = 0; i < cnt; ++i)
@@ -850,11 +933,7 @@ void block_cycle(mk_batch_s &B) {
                         const i64 o = (i * Bp + g * kP + b) * kP + a;
                         Hc[(i * kP + a) * kP + b] = B.bh[o] + B.bh[hsz + o];
                     }
-            for (int a = 0; a < kP; ++a)
-                for (int b = 0; b < kP; ++b) Gm[a * kP + b] = B.bh[2 * hsz + (g * kP + b) * kP + a];
-            const bool full = block_cholesky(Gm, R);
-            if (!full) std::fill(R, R + kP * kP, 0.0);
-            ls[g].add(c, Hc.data(), R, est);
+            ls[g].add(c, Hc.data(), &Rg[g * kP * kP], est);
             bool done = true, out_of_iterations = false;
             for (int b = 0; b < kP; ++b) {
                 auto &S = B.slot[g * kP + b];
@@ -863,21 +942,10 @@ void block_cycle(mk_batch_s &B) {
                 done = done && !(est[b] > S.tol);
                 out_of_iterations = out_of_iterations || S.res.iterations >= S.max_iterations;
             }
-            if (done || !full || out_of_iterations) {
+            if (done || !ok[g] || out_of_iterations) {
                 run[g] = 0;
                 for (int b = 0; b < kP; ++b) mask[g * kP + b] = 0;
-                continue;
             }
-            upper_inverse(R, Ri);
-            for (int a = 0; a < kP; ++a)
-                for (int b = 0; b < kP; ++b) C[(g * kP + b) * kP + a] = Ri[a * kP + b];
-        }
-        if (c + 1 < m && any_run()) {
-            push_limits(B, mask.data());
-            put_coefficients(Bp * kP);
-            P.run(K_OTHER, nb * 40.0, [&] {
-                bk::block_apply<double>(V + (c + 1) * vec, w, 1, Cd, n, Bp, 1.0, true, lims, s);
-            });
         }
     }
     // x += Z Y per group: coefficients (i Bp + j) kP + a = Y[i kP + a][j mod kP]
@@ -935,6 +1003,8 @@ void ensure_block(mk_batch_s &B) {
     B.d_bh = dbuf(coeff * sizeof(double));
     B.bpartial = dbuf(static_cast<std::size_t>(B.npb) * (m + 1) * Bp * kP * sizeof(double));
     B.d_bcnt = dbuf(Bp * sizeof(int));
+    B.bscratch = dbuf(static_cast<std::size_t>(std::max<i64>(B.n, 1)) * Bp * sizeof(double));
+    B.bscratch2 = dbuf(static_cast<std::size_t>(std::max<i64>(B.n, 1)) * Bp * sizeof(double));
     if (B.bh) (void)hipHostFree(B.bh);
     B.bh = nullptr;
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&B.bh), (2 * (m + 1) + 1) * Bp * kP * sizeof(double)));

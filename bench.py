@@ -348,8 +348,9 @@ def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0, smooth
     cap-5 box serves all three caps (a request of cap k is the box's leading sectors 0..k), and
     every family of at most 524,288 states runs in a batch engine that admits requests at restart
     boundaries; the larger families run on `workers` worker streams with two hierarchies each
-    (the headline W-cycle hierarchy above a million states). Requests are submitted at their
-    arrival times."""
+    (above a million states: 1 + 2 sweeps, W-cycles on the top 3 levels, dense sectors of at most
+    100 states, whose terminal inverses a new shift recomputes quickly). Requests are submitted
+    at their arrival times."""
     from markovkit_amd.fsp import box_generator, toggle_switch
     from markovkit_amd.model import box_size
     smoother = getattr(mk, SMOOTHERS[smoother_name])
@@ -473,10 +474,10 @@ def main():
                          "on the large workload, the reference's 1 on the others)")
     ap.add_argument("--cycle-levels", type=int, default=None,
                     help="levels that take --cycles coarse visits (0: all; the ones below take "
-                         "one; default: 3 on the large workload, else 0)")
+                         "one; default: 2 on the large workload, else 0)")
     ap.add_argument("--coarsest", type=int, default=None,
                     help="coarsest_size of the timed hierarchy (sectors of at most this many "
-                         "states are solved densely; default: 100 on the large workload, else "
+                         "states are solved densely; default: 800 on the large workload, else "
                          "the reference's default)")
     ap.add_argument("--smoother", default="red_black", choices=list(SMOOTHERS),
                     help="sector multigrid smoother of the timed and served solves (red_black: "
@@ -548,8 +549,8 @@ def main():
     log(f"[setup] generator {t_gen:.1f}s, {Q.n_jumps()} jumps")
     t = time.perf_counter()
     smoother = getattr(mk, SMOOTHERS[args.smoother])
-    # the large workload's hierarchy: 1 + 2 sweeps, W-cycles on the top 3 levels, dense sectors
-    # of <= 100 states
+    # the large workload's hierarchy: 1 + 2 sweeps, W-cycles on the top 2 levels, dense sectors
+    # of <= 800 states (the 8-voxel level: blocks of up to 792 states, a wave per row)
     ref = mk.MultigridOptions()
     tuned_large = args.workload == "large"
     if args.pre is None:
@@ -559,9 +560,9 @@ def main():
     if args.cycles is None:
         args.cycles = 2 if tuned_large else ref.coarse_cycles
     if args.cycle_levels is None:
-        args.cycle_levels = 3 if tuned_large else ref.cycle_levels
+        args.cycle_levels = 2 if tuned_large else ref.cycle_levels
     if args.coarsest is None:
-        args.coarsest = 100 if tuned_large else ref.coarsest_size
+        args.coarsest = 800 if tuned_large else ref.coarsest_size
     opts = dict(pre_sweeps=args.pre, post_sweeps=args.post, coarse_cycles=args.cycles,
                 cycle_levels=args.cycle_levels, coarsest_size=args.coarsest)
     M32 = mk.SectorMultigrid(w.model.domain, box, shift,

@@ -474,7 +474,10 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
         // the tail's levels must each visit their coarse level once
         const i64 first_v = mg.opt.coarse_cycles == 1 ? 0 : cycle_levels(mg.opt);
         i64 start = depth;
-        while (start > first_v && L[start - 1].n <= tail_rows() && L[start - 1].n_agg == 0) --start;
+        // (dense blocks of more than kWideTerminal rows take a wave per row: not in the tail)
+        while (start > first_v && L[start - 1].n <= tail_rows() && L[start - 1].n_agg == 0 &&
+               L[start - 1].term_max <= gpu::kWideTerminal)
+            --start;
         // a lone direct level is one terminal-solve launch without the tail
         // (the tail tracks at most 32 levels)
         if (start < depth && depth - start <= 32 && !(start == depth - 1 && L[start].direct))

@@ -326,6 +326,34 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// the same product for blocks of more than kWideTerminal rows: one wave per row, lanes over
+// columns (coalesced reads of the row), lane partial sums over j = lane, lane + 64, ... and a
+// fixed shuffle tree; grid nb * max_size waves
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_terminal_wide(i64 nb, const u32 *start, const u32 *size, const u64 *offset, const T *inv,
+                    i64 max_size, const T *r, T *x, int assign) {
+    const i64 g = (blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (g >= nb * max_size) return;
+    const i64 b = g / max_size, i = g % max_size;
+    const u32 m = size[b];
+    if (i >= m) return;
+    const T *row = inv + offset[b] + static_cast<u64>(i) * m;
+    const T *rr = r + start[b];
+    T s = 0;
+#pragma unroll 4
+    for (u32 j = lane; j < m; j += 64) s += row[j] * rr[j];
+    s = wave_sum(s);
+    if (lane == 0) {
+        if (assign) {
+            x[start[b] + i] = s;
+        } else {
+            x[start[b] + i] += s;
+        }
+    }
+}
+
 // ---- aggregate-block smoother (thread per listed row)
 // sum over row i's entries of rate * x[column], ascending entry order, and row i's exit rate
 template <typename T>
@@ -1257,6 +1285,13 @@ template <typename T>
 void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
                     const T *inverse, i64 max_size, const T *r, T *x, hipStream_t s, bool assign) {
     if (n_blocks == 0) return;
+    if (max_size > kWideTerminal) {
+        const i64 threads = n_blocks * max_size * 64;
+        const int grid = static_cast<int>((threads + kThreads - 1) / kThreads);
+        hipLaunchKernelGGL((k_terminal_wide<T>), dim3(grid), dim3(kThreads), 0, s, n_blocks, start,
+                           size, offset, inverse, max_size, r, x, static_cast<int>(assign));
+        return;
+    }
     const i64 threads = n_blocks * max_size;
     const int grid = static_cast<int>((threads + kThreads - 1) / kThreads);
     hipLaunchKernelGGL((k_terminal<T>), dim3(grid), dim3(kThreads), 0, s, n_blocks, start, size,

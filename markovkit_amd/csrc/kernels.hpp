@@ -109,7 +109,9 @@ void restrict_children(i64 n_coarse, const u32 *cptr, const u32 *cidx, const T *
 template <typename T>
 void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x,
                  hipStream_t s, const u32 *rows = nullptr, const T *coarse2 = nullptr);
-// dense terminal solves: x[start_b + i] += sum_j inv_b[i][j] * r[start_b + j] (assign: x[...] =)
+// dense terminal solves: x[start_b + i] += sum_j inv_b[i][j] * r[start_b + j] (assign: x[...] =);
+// a thread per row, or a wave per row (lanes over columns) when a block exceeds kWideTerminal
+constexpr i64 kWideTerminal = 128;
 template <typename T>
 void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
                     const T *inverse, i64 max_size, const T *r, T *x, hipStream_t s,

@@ -111,12 +111,14 @@ def test_multigrid_cycle(case, precision, smoother):
 @pytest.mark.parametrize("precision", [mk.FP64, mk.FP32], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("smoother", [mk.JACOBI, mk.RED_BLACK], ids=["jacobi", "red_black"])
 @pytest.mark.parametrize("cycles,levels,coarsest", [(2, 0, 4), (2, 1, 4), (2, 2, 4), (3, 1, 4),
-                                                    (2, 2, 100), (2, 0, 36)],
-                         ids=["W", "W-top1", "W-top2", "3-top1", "W-top2-c100", "W-c36"])
+                                                    (2, 2, 100), (2, 0, 36), (2, 2, 800)],
+                         ids=["W", "W-top1", "W-top2", "3-top1", "W-top2-c100", "W-c36",
+                              "W-top2-c800"])
 def test_multigrid_cycle_coarse_visits(case, precision, smoother, cycles, levels, coarsest):
     """W-cycles (coarse_cycles visits of every coarse level, or of the top cycle_levels levels
     over a V-cycle below, where the fused tail may take over), with dense bottom blocks of up to
-    coarsest states, against the oracle's cycle"""
+    coarsest states, against the oracle's cycle; blocks of more than 128 states (330 on 32
+    voxels with coarsest 800) take the wave-per-row dense kernel"""
     w, P, box = case["w"], case["P"], case["box"]
     shift = w.shifts[0]
     M = mk.SectorMultigrid(w.model.domain, box, shift,

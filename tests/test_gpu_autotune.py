@@ -43,5 +43,11 @@ def test_tune_batch():
     shifts = [0.1, 1.0, 10.0]
     B = np.random.default_rng(2).random((3, box.size()))
     res = autotune.tune_batch(Q, w.model.domain, box, shifts, B, sweeps=((2, 2), (1, 1)))
-    assert len(res.trials) == 2 and (res.pre, res.post) in ((2, 2), (1, 1))
+    # both smoothers (Jacobi, red-black) times both sweep pairs
+    assert len(res.trials) == 4 and (res.pre, res.post) in ((2, 2), (1, 1))
+    assert {t.smoother for t in res.trials} == {mk.JACOBI, mk.RED_BLACK}
+    assert res.smoother in (mk.JACOBI, mk.RED_BLACK)
     assert all(np.isfinite(t.seconds) for t in res.trials)
+    only = autotune.tune_batch(Q, w.model.domain, box, shifts, B, smoothers=(mk.JACOBI,),
+                               sweeps=((2, 2), (1, 1)))
+    assert len(only.trials) == 2 and only.smoother == mk.JACOBI

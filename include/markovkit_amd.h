@@ -340,6 +340,16 @@ mk_status mk_expv(mk_generator gen, double t, const double *v, double *w,
 mk_status mk_expv_device(mk_generator gen, double t, const double *d_v, double *d_w,
                          const mk_expv_options *options, mk_expv_result *result);
 
+/* Batched exponentials w_j = exp(t_j Q) v_j of `count` vectors over one generator (v and w
+ * host buffers, system-major: vector j at j * rows), each to options->tolerance with
+ * options->krylov_dim basis vectors; results: count entries. No single-call counterpart (the
+ * reference calls num::expv once per vector). Up to 256 vectors advance together: one Arnoldi
+ * process per substep for all of them (the operator read once per row for 64 vectors) and
+ * Expokit's step control per vector; a vector's result does not depend on the others in the
+ * call (not bit for bit mk_expv's: the kernels differ). */
+mk_status mk_expv_batch(mk_generator gen, int64_t count, const double *t, const double *v, double *w,
+                        const mk_expv_options *options, mk_expv_result *results);
+
 /* ---- batched resolvent solves ----------------------------------------------------------
  * No single-call counterpart: the reference solves each (shift, rhs) pair with its own
  * num::gmres call and sector_multigrid (examples/rdme_multigrid_benchmark.cpp:92-121). A batch

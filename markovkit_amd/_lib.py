@@ -100,6 +100,7 @@ SIGNATURES = {
     "mk_sector_mg_set_shift": ([P, f64], i32),
     "mk_expv": ([P, f64, P, P, C.POINTER(ExpvOptions), C.POINTER(ExpvResult)], i32),
     "mk_expv_device": ([P, f64, P, P, C.POINTER(ExpvOptions), C.POINTER(ExpvResult)], i32),
+    "mk_expv_batch": ([P, i64, P, P, P, C.POINTER(ExpvOptions), P], i32),
     "mk_server_create": ([i32, i64, C.POINTER(P)], i32),
     "mk_server_destroy": ([P], i32),
     "mk_server_add_family": ([P, P, i32, P, C.POINTER(i32)], i32),

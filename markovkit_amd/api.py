@@ -470,6 +470,21 @@ def expv_device(Q: Generator, t: float, v: "DeviceVector", w: "DeviceVector",
     return ExpvResult(r.steps, r.rejected, r.error)
 
 
+def expv_batch(Q: Generator, ts, V, krylov_dim: int = 30, tolerance: float = 1e-8):
+    """w_j = exp(t_j Q) v_j for the rows v_j of V (count x states) advanced together on the
+    device (mk_expv_batch): one Arnoldi process per substep for all vectors, Expokit's step
+    control per vector. Returns (W, [ExpvResult])."""
+    Va = np.ascontiguousarray(np.atleast_2d(V), np.float64)
+    ta = np.ascontiguousarray(np.atleast_1d(ts), np.float64)
+    if Va.shape[1] != Q.rows() or ta.shape != (Va.shape[0],):
+        raise ValueError("V must hold one vector per time, each with one entry per state")
+    W = np.zeros_like(Va)
+    o = _ExpvOptions(int(krylov_dim), float(tolerance))
+    r = (_ExpvResult * max(1, len(ta)))()
+    check(lib().mk_expv_batch(Q._h, len(ta), _p(ta), _p(Va), _p(W), C.byref(o), r))
+    return W, [ExpvResult(x.steps, x.rejected, x.error) for x in r[:len(ta)]]
+
+
 class BatchSolver:
     """Batched resolvent solves (s_j I - Q) x_j = b_j, j < len(shifts), over one population box:
     the generator and the sector multigrid hierarchy are shared, the terminal blocks are
@@ -541,10 +556,12 @@ class ServedResult:
 class Server:
     """Request serving (mk_server_*). Resolvent families of at most batch_rows states are
     served by continuous batching: a resident batch engine of batch_slots systems admits queued
-    requests at every restart boundary. Larger families (a generator and a pool of Jacobi or
-    red-black hierarchies, re-targeted to each request's shift) and Krylov exponentials run on
-    worker threads with their own HIP streams, admitted first come first served by free worker,
-    HBM footprint and free hierarchy. batch_rows = 0 turns batching off."""
+    requests at every restart boundary; Krylov exponentials over a generator of at most
+    batch_rows states join a batched expv engine of that generator at substep boundaries.
+    Larger families (a generator and a pool of Jacobi or red-black hierarchies, re-targeted to
+    each request's shift) and larger exponentials run on worker threads with their own HIP
+    streams, admitted first come first served by free worker, HBM footprint and free hierarchy.
+    batch_rows = 0 turns batching off."""
 
     def __init__(self, workers: int = 4, hbm_budget: int = 64 << 30, batch_rows: int = None,
                  batch_slots: int = 64):

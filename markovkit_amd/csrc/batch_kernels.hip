@@ -249,8 +249,26 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int k = 0; k < kDotChunk; ++k) acc[k] = 0.0;
     {
+        // rows r, r + stride, ... in order; four rows' loads in flight at a time (the sums
+        // keep the row order)
         const i64 stride = static_cast<i64>(np) * kWaves;
-        for (i64 r = static_cast<i64>(blockIdx.x) * kWaves + wave; r < rows; r += stride) {
+        i64 r = static_cast<i64>(blockIdx.x) * kWaves + wave;
+        for (; r + 3 * stride < rows; r += 4 * stride) {
+            double wr[4], v[4][kDotChunk];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const i64 o = (r + u * stride) * Bp + j;
+                wr[u] = w[o];
+#pragma unroll
+                for (int k = 0; k < kDotChunk; ++k) v[u][k] = k < kc ? V[(k0 + k) * ld + o] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int k = 0; k < kDotChunk; ++k)
+                    if (k < kc) acc[k] += v[u][k] * wr[u];
+        }
+        for (; r < rows; r += stride) {
             const i64 o = r * Bp + j;
             const double wr = w[o];
 #pragma unroll
@@ -281,8 +299,18 @@ __global__ void __launch_bounds__(kThreads)
     const i64 o = static_cast<i64>(blockIdx.y) * kLanes + lane;
     const int p0 = blockIdx.x * kGroup, p1 = min(np, p0 + kGroup);
     double acc = 0.0;
-    if (o < m)
-        for (int p = p0 + wave; p < p1; p += kWaves) acc += partial[static_cast<i64>(p) * m + o];
+    if (o < m) {
+        // this wave's partitions in order, four loads in flight at a time
+        int p = p0 + wave;
+        for (; p + 3 * kWaves < p1; p += 4 * kWaves) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = partial[static_cast<i64>(p + u * kWaves) * m + o];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += v[u];
+        }
+        for (; p < p1; p += kWaves) acc += partial[static_cast<i64>(p) * m + o];
+    }
     red[wave][lane] = acc;
     __syncthreads();
     if (wave == 0 && o < m)
@@ -294,8 +322,17 @@ __global__ void __launch_bounds__(kThreads)
     k_reduce(const double *partial, int np, i64 m, double *out) {
     for (i64 o = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; o < m;
          o += static_cast<i64>(gridDim.x) * kThreads) {
+        // partitions in order, eight loads in flight at a time
         double s = 0.0;
-        for (int p = 0; p < np; ++p) s += partial[static_cast<i64>(p) * m + o];
+        int p = 0;
+        for (; p + 8 <= np; p += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = partial[static_cast<i64>(p + u) * m + o];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        for (; p < np; ++p) s += partial[static_cast<i64>(p) * m + o];
         out[o] = s;
     }
 }
@@ -307,8 +344,9 @@ __global__ void __launch_bounds__(kThreads)
     k_dcgs_update(double *V, int count, const double *sc, const double *tg, const double *beta,
                   const double *w, i64 n, i64 Bp, double *partial, int np, double *eta_out,
                   const int *limit) {
-    __shared__ double part[2][kWaves][kLanes];
-    __shared__ double uw[2][kLanes];
+    constexpr int kRows = PW <= 8 ? 4 : 2;
+    __shared__ double part[kRows][2][kWaves][kLanes];
+    __shared__ double uw[kRows][2][kLanes];
     __shared__ double red[2][kLanes];
     constexpr int kSlots = PW > 0 ? PW : 1;
     const int lane = threadIdx.x & 63, wave = wave_id();
@@ -338,36 +376,51 @@ __global__ void __launch_bounds__(kThreads)
     double *const q_out = V + static_cast<i64>(count) * ld;
     double *const u_out = V + static_cast<i64>(count + 1) * ld;
     double own = 0.0; // wave 2: q . u', wave 3: u' . u'
-    // every wave runs every row (the loop holds barriers); lanes past their rows compute zeros
-    for (i64 r = blockIdx.x; r < n; r += np) {
-        const i64 o = r * Bp + j;
-        const bool live = r < rows;
-        double v[kSlots];
+    // every wave runs every row (the loop holds barriers); lanes past their rows compute zeros.
+    // RB rows per pair of barriers (rows r0, r0 + np, ...; the sums keep the row order)
+    constexpr int RB = kRows;
+    for (i64 r0 = blockIdx.x; r0 < n; r0 += RB * static_cast<i64>(np)) {
+        double v[RB][kSlots];
+        bool live[RB];
 #pragma unroll
-        for (int kk = 0; kk < PW; ++kk) v[kk] = live ? src[kk][o] : 0.0;
-        double a = 0.0, b = 0.0;
+        for (int u = 0; u < RB; ++u) {
+            const i64 r = r0 + u * static_cast<i64>(np);
+            const i64 o = r * Bp + j;
+            live[u] = r < rows;
 #pragma unroll
-        for (int kk = 0; kk < PW; ++kk) {
-            a += cs[kk] * v[kk];
-            b += ct[kk] * v[kk];
+            for (int kk = 0; kk < PW; ++kk) v[u][kk] = live[u] ? src[kk][o] : 0.0;
+            double a = 0.0, b = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < PW; ++kk) {
+                a += cs[kk] * v[u][kk];
+                b += ct[kk] * v[u][kk];
+            }
+            part[u][0][wave][lane] = a;
+            part[u][1][wave][lane] = b;
+            if (wave < 2) uw[u][wave][lane] = live[u] ? (wave == 0 ? q_out[o] : w[o]) : 0.0;
         }
-        part[0][wave][lane] = a;
-        part[1][wave][lane] = b;
-        if (wave < 2) uw[wave][lane] = live ? (wave == 0 ? q_out[o] : w[o]) : 0.0;
         __syncthreads();
-        const double S = ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) + part[0][3][lane];
-        const double T = ((part[1][0][lane] + part[1][1][lane]) + part[1][2][lane]) + part[1][3][lane];
-        const double q = live ? (uw[0][lane] - S) * ib : 0.0;
-        const double up = live ? (uw[1][lane] - T) - eta * q : 0.0;
-        __syncthreads();
-        if (live) {
-            if (wave == 0) q_out[o] = q;
-            if (wave == 1) u_out[o] = up;
-        }
-        if (wave == 2) own += q * up;
-        if (wave == 3) own += up * up;
+        double q[RB], up[RB];
 #pragma unroll
-        for (int kk = 0; kk < PW; ++kk) acc[kk] += v[kk] * up;
+        for (int u = 0; u < RB; ++u) {
+            const double S = ((part[u][0][0][lane] + part[u][0][1][lane]) + part[u][0][2][lane]) + part[u][0][3][lane];
+            const double T = ((part[u][1][0][lane] + part[u][1][1][lane]) + part[u][1][2][lane]) + part[u][1][3][lane];
+            q[u] = live[u] ? (uw[u][0][lane] - S) * ib : 0.0;
+            up[u] = live[u] ? (uw[u][1][lane] - T) - eta * q[u] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+            const i64 o = (r0 + u * static_cast<i64>(np)) * Bp + j;
+            if (live[u]) {
+                if (wave == 0) q_out[o] = q[u];
+                if (wave == 1) u_out[o] = up[u];
+            }
+            if (wave == 2) own += q[u] * up[u];
+            if (wave == 3) own += up[u] * up[u];
+#pragma unroll
+            for (int kk = 0; kk < PW; ++kk) acc[kk] += v[u][kk] * up[u];
+        }
     }
     const i64 m = static_cast<i64>(count + 2) * Bp;
     double *out = partial + static_cast<i64>(blockIdx.x) * m;
@@ -521,6 +574,17 @@ __global__ void __launch_bounds__(kThreads)
     out[o] = assign ? sign * s : out[o] + sign * s;
 }
 
+// y[r][j] = x[r][j] / sqrt(norm2[j]) for rows r < limit[j]; 0 where norm2[j] is not positive
+__global__ void __launch_bounds__(kThreads)
+    k_scale_lanes(const double *x, const double *norm2, double *y, i64 n, i64 Bp, const int *limit) {
+    i64 r, j;
+    row_lane(r, j);
+    if (r >= n || (limit && r >= limit[j])) return;
+    const double q = norm2[j];
+    const i64 o = r * Bp + j;
+    y[o] = q > 0.0 ? x[o] / std::sqrt(q) : 0.0;
+}
+
 // dst[r][j] = r < rows ? src[r] : 0 for one lane j
 __global__ void __launch_bounds__(kThreads)
     k_put_lane(const double *src, i64 rows, double *dst, i64 n, i64 Bp, i64 j) {
@@ -552,6 +616,13 @@ __global__ void __launch_bounds__(kThreads)
 void put_lane(const double *src, i64 rows, double *dst, i64 n, i64 Bp, i64 j, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_put_lane, dim3(grid_for(n)), dim3(kThreads), 0, s, src, rows, dst, n, Bp, j);
+}
+
+void scale_lanes(const double *x, const double *norm2, double *y, i64 n, i64 Bp, const int *limit,
+                 hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_scale_lanes, dim3(static_cast<u32>(row_blocks(n)), Bp / kLanes), dim3(kThreads), 0,
+                       s, x, norm2, y, n, Bp, limit);
 }
 
 void get_lane(const double *src, i64 rows, i64 Bp, i64 j, double *dst, hipStream_t s) {

@@ -106,6 +106,9 @@ void block_dot(const double *V, int count, const double *W, i64 n, i64 Bp, doubl
 template <typename IN>
 void block_apply(double *out, const IN *V, int count, const double *C, i64 n, i64 Bp, double sign,
                  bool assign, const int *limit, hipStream_t s, const int *counts = nullptr);
+// y[r][j] = x[r][j] / sqrt(norm2[j]) for rows r < limit[j] (0 where norm2[j] is not positive)
+void scale_lanes(const double *x, const double *norm2, double *y, i64 n, i64 Bp, const int *limit,
+                 hipStream_t s);
 // one lane j of a state-major vector: dst[r * Bp + j] = src[r] for r < rows, 0 for rows <= r < n
 void put_lane(const double *src, i64 rows, double *dst, i64 n, i64 Bp, i64 j, hipStream_t s);
 // dst[r] = src[r * Bp + j] for r < rows

@@ -6,6 +6,16 @@
 // exponentiates the (m + 2) x (m + 2) augmented matrix (degree-6 Pade, scaling and squaring),
 // estimates the local error from the two corrected terms and accepts or shrinks the step
 // (gamma 0.9, delta 1.2, at most 10 rejections); w = V (beta F e_1) on the device.
+//
+// Batched exponentials (mk_expv_batch_s): many vectors over one generator advance together in
+// the state-major layout of the batched resolvent solves (vector j's state i at X[i * Bp + j]).
+// Every substep runs one Arnoldi process for all running lanes (the operator's entries read
+// once per row for 64 lanes, per-lane classical Gram-Schmidt twice with fixed-order reductions),
+// then each lane's step control on the host. Lanes keep their own step sizes, time and error;
+// the basis does not depend on the step, so a lane's rejections only recompute its small
+// exponential. A lane's arithmetic never mixes with another lane's, so its result does not
+// depend on what runs beside it. Vectors join at substep boundaries (continuous batching).
+#include "batch_kernels.hpp"
 #include "runtime.hpp"
 
 using namespace mk::rt;
@@ -57,10 +67,115 @@ double round_step(double step) {
     return std::ceil(step / s) * s;
 }
 
+constexpr double kBreakdown = 1e-7, kGamma = 0.9, kDelta = 1.2;
+constexpr int kMaxReject = 10;
+
+// Expokit's step control of one vector (expv.f): the first step size from the norms, and per
+// step the Hessenberg matrix from the Arnoldi coefficients, the augmented exponential (degree-6
+// Pade), the local error from the two corrected terms, rejections that shrink the step, the
+// coefficients y = beta F e_1 and the next step size. The Arnoldi basis does not depend on the
+// step size, so a rejected step only recomputes the small exponential.
+struct expv_control {
+    i64 m = 1;
+    double t_out = 0.0, sgn = -1.0, tol = 0.0, anorm = 1.0, rndoff = 0.0;
+    double t_now = 0.0, t_new = 0.0, t_step = 0.0, xm = 1.0, beta = 0.0, err_loc = 0.0;
+    mk_expv_result res{0, 0, 0.0};
+
+    // false: nothing to integrate (a zero vector or t = 0: w = v)
+    bool start(double t, double beta0, const mk_generator_s &G, double tolerance, i64 m_) {
+        m = m_;
+        t_out = std::abs(t);
+        sgn = t >= 0.0 ? -1.0 : 1.0; // exp(t Q) = exp(-t B)
+        tol = tolerance;
+        anorm = G.norm1 > 0.0 ? G.norm1 : 1.0;
+        rndoff = anorm * std::numeric_limits<double>::epsilon();
+        beta = beta0;
+        if (!(beta > 0.0) || t_out == 0.0) return false;
+        xm = 1.0 / double(m);
+        const double fact = std::pow(double(m + 1) / std::exp(1.0), double(m + 1)) *
+                            std::sqrt(2.0 * 3.141592653589793 * double(m + 1));
+        t_new = round_step((1.0 / anorm) * std::pow((fact * tol) / (4.0 * beta * anorm), xm));
+        t_now = 0.0;
+        return true;
+    }
+    bool running() const { return t_now < t_out; }
+
+    // one step from the Arnoldi coefficients h(i, j) (both projection passes summed, i <= j < m)
+    // and the squared norms nr(j) of the remainders (j <= m); y receives the coefficients of
+    // the basis vectors; returns their number
+    template <typename HF, typename NF>
+    i64 plan(HF &&h, NF &&nr, std::vector<double> &y) {
+        ++res.steps;
+        t_step = std::min(t_out - t_now, t_new);
+        const i64 L = m + 2;
+        std::vector<double> H(static_cast<std::size_t>(L * L), 0.0);
+        int k1 = 2;
+        i64 mb = m;
+        for (i64 j = 0; j < m; ++j) {
+            for (i64 i = 0; i <= j; ++i) H[i * L + j] = h(i, j);
+            const double sj = std::sqrt(nr(j));
+            if (!(sj >= kBreakdown)) {
+                k1 = 0;
+                mb = j + 1;
+                t_step = t_out - t_now;
+                break;
+            }
+            H[(j + 1) * L + j] = sj;
+        }
+        double avnorm = 0.0;
+        if (k1 != 0) {
+            H[(m + 1) * L + m] = 1.0;
+            avnorm = std::sqrt(nr(m));
+        }
+        std::vector<double> F;
+        i64 mx = mb + k1;
+        err_loc = kBreakdown;
+        for (int rejected = 0;; ++rejected) {
+            mx = mb + k1;
+            std::vector<double> X(static_cast<std::size_t>(mx * mx));
+            for (i64 i = 0; i < mx; ++i)
+                for (i64 j = 0; j < mx; ++j) X[i * mx + j] = sgn * t_step * H[i * L + j];
+            F = mk::expm_pade(std::move(X), mx);
+            if (k1 == 0) {
+                err_loc = kBreakdown;
+                break;
+            }
+            const double phi1 = std::abs(beta * F[m * mx]);
+            const double phi2 = std::abs(beta * F[(m + 1) * mx] * avnorm);
+            if (phi1 > 10.0 * phi2) {
+                err_loc = phi2;
+                xm = 1.0 / double(m);
+            } else if (phi1 > phi2) {
+                err_loc = (phi1 * phi2) / (phi1 - phi2);
+                xm = 1.0 / double(m);
+            } else {
+                err_loc = phi1;
+                xm = 1.0 / double(std::max<i64>(1, m - 1));
+            }
+            if (err_loc <= kDelta * t_step * tol) break;
+            if (rejected == kMaxReject) throw std::runtime_error("expv: the requested tolerance is too high");
+            t_step = round_step(kGamma * t_step * std::pow(t_step * tol / err_loc, xm));
+            ++res.rejected;
+        }
+        const i64 use = mb + std::max(0, k1 - 1);
+        y.resize(static_cast<std::size_t>(use));
+        for (i64 k = 0; k < use; ++k) y[k] = beta * F[k * mx];
+        return use;
+    }
+
+    // the step is taken: beta_new = |w|; false when the iterate vanished or is not finite
+    bool accept(double beta_new) {
+        beta = beta_new;
+        t_now += t_step;
+        t_new = round_step(kGamma * t_step * std::pow(t_step * tol / err_loc, xm));
+        res.error += std::max(err_loc, rndoff);
+        return beta > 0.0 && std::isfinite(beta);
+    }
+};
+
 mk_expv_result expv_device(mk_generator_s &G, double t, const double *v, double *w_out,
                            const mk_expv_options &o) {
     require(o.krylov_dim >= 1 && o.tolerance > 0.0, "expv: bad options");
-    mk_expv_result res{0, 0, 0.0};
     const i64 n = G.n;
     const i64 m = std::max<i64>(1, std::min<i64>(o.krylov_dim, n));
     require(m + 1 <= 512, "expv: krylov_dim above 511 is not supported on the device");
@@ -73,11 +188,6 @@ mk_expv_result expv_device(mk_generator_s &G, double t, const double *v, double 
     double *V = dp<double>(ws.V), *pv = dp<double>(ws.p), *partial = dp<double>(ws.partial);
     double *hd = dp<double>(ws.hd);
     const double spmv_bytes = op_bytes(G.A, 8, 8, 1);
-    const double tol = o.tolerance, btol = 1e-7, gamma = 0.9, delta = 1.2;
-    const int max_reject = 10;
-    const double t_out = std::abs(t), sgn = t >= 0.0 ? -1.0 : 1.0; // exp(t Q) = exp(-t B)
-    const double anorm = G.norm1 > 0.0 ? G.norm1 : 1.0;
-    const double rndoff = anorm * std::numeric_limits<double>::epsilon();
 
     // beta^2 = |w|^2 into hd[beta2]
     auto norm2_to = [&](const double *x, double *out) {
@@ -93,24 +203,16 @@ mk_expv_result expv_device(mk_generator_s &G, double t, const double *v, double 
     // the iterate lives in the workspace (fixed pointers for the captured Arnoldi)
     double *w = dp<double>(ws.x);
     HIPCHK(hipMemcpyAsync(w, v, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    expv_control c;
     auto finish = [&] {
         HIPCHK(hipMemcpyAsync(w_out, w, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        return res;
+        return c.res;
     };
     norm2_to(w, hd + ws.beta2());
     read(ws.beta2(), 1);
-    double beta = std::sqrt(ws.host[ws.beta2()]);
-    if (!(beta > 0.0) || t_out == 0.0) return finish();
-    double xm = 1.0 / double(m);
-    const double fact = std::pow(double(m + 1) / std::exp(1.0), double(m + 1)) *
-                        std::sqrt(2.0 * 3.141592653589793 * double(m + 1));
-    double t_new = round_step((1.0 / anorm) * std::pow((fact * tol) / (4.0 * beta * anorm), xm));
-    double t_now = 0.0;
-    const i64 L = m + 2;
-    std::vector<double> H;
-    while (t_now < t_out) {
-        ++res.steps;
-        double t_step = std::min(t_out - t_now, t_new);
+    if (!c.start(t, std::sqrt(ws.host[ws.beta2()]), G, o.tolerance, m)) return finish();
+    std::vector<double> y;
+    while (c.running()) {
         ws.graphs.run(graph_key{G.uid, 0, n, m, 0, 0, 0}, s, [&] {
         // V_0 = w / beta (beta^2 still in hd[beta2])
         P.run(K_OTHER, n * 16.0, [&] { gpu::scale_by_norm(n, w, hd + ws.beta2(), V, s); });
@@ -143,58 +245,10 @@ mk_expv_result expv_device(mk_generator_s &G, double t, const double *v, double 
         }, n);
         read(0, ws.beta2());
         const double *h1 = ws.host + ws.h1(), *h2 = ws.host + ws.h2(), *nr = ws.host + ws.nrm();
-        H.assign(static_cast<std::size_t>(L * L), 0.0);
-        int k1 = 2;
-        i64 mb = m;
-        for (i64 j = 0; j < m; ++j) {
-            for (i64 i = 0; i <= j; ++i) H[i * L + j] = h1[j * (m + 1) + i] + h2[j * (m + 1) + i];
-            const double sj = std::sqrt(nr[j]);
-            if (!(sj >= btol)) {
-                k1 = 0;
-                mb = j + 1;
-                t_step = t_out - t_now;
-                break;
-            }
-            H[(j + 1) * L + j] = sj;
-        }
-        double avnorm = 0.0;
-        if (k1 != 0) {
-            H[(m + 1) * L + m] = 1.0;
-            avnorm = std::sqrt(nr[m]);
-        }
-        std::vector<double> F;
-        i64 mx = mb + k1;
-        double err_loc = btol;
-        for (int rejected = 0;; ++rejected) {
-            mx = mb + k1;
-            std::vector<double> X(static_cast<std::size_t>(mx * mx));
-            for (i64 i = 0; i < mx; ++i)
-                for (i64 j = 0; j < mx; ++j) X[i * mx + j] = sgn * t_step * H[i * L + j];
-            F = mk::expm_pade(std::move(X), mx);
-            if (k1 == 0) {
-                err_loc = btol;
-                break;
-            }
-            const double phi1 = std::abs(beta * F[m * mx]);
-            const double phi2 = std::abs(beta * F[(m + 1) * mx] * avnorm);
-            if (phi1 > 10.0 * phi2) {
-                err_loc = phi2;
-                xm = 1.0 / double(m);
-            } else if (phi1 > phi2) {
-                err_loc = (phi1 * phi2) / (phi1 - phi2);
-                xm = 1.0 / double(m);
-            } else {
-                err_loc = phi1;
-                xm = 1.0 / double(std::max<i64>(1, m - 1));
-            }
-            if (err_loc <= delta * t_step * tol) break;
-            if (rejected == max_reject) throw std::runtime_error("expv: the requested tolerance is too high");
-            t_step = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
-            ++res.rejected;
-        }
-        const i64 use = mb + std::max(0, k1 - 1);
+        const i64 use = c.plan([&](i64 i, i64 j) { return h1[j * (m + 1) + i] + h2[j * (m + 1) + i]; },
+                               [&](i64 j) { return nr[j]; }, y);
         double *yh = ws.host + ws.y();
-        for (i64 k = 0; k < use; ++k) yh[k] = beta * F[k * mx];
+        std::copy(y.begin(), y.end(), yh);
         HIPCHK(hipMemcpyAsync(hd + ws.y(), yh, use * sizeof(double), hipMemcpyHostToDevice, s));
         P.run(K_OTHER, double(use) * n * 8.0 + n * 16.0, [&] {
             gpu::fill<double>(n, 0.0, w, s);
@@ -202,16 +256,263 @@ mk_expv_result expv_device(mk_generator_s &G, double t, const double *v, double 
         });
         norm2_to(w, hd + ws.beta2());
         read(ws.beta2(), 1);
-        beta = std::sqrt(ws.host[ws.beta2()]);
-        t_now += t_step;
-        t_new = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
-        res.error += std::max(err_loc, rndoff);
-        if (!(beta > 0.0) || !std::isfinite(beta)) break;
+        if (!c.accept(std::sqrt(ws.host[ws.beta2()]))) break;
     }
     return finish();
 }
 
 } // namespace
+
+namespace bk = mk::gpu::batch;
+
+struct mk_expv_batch_s {
+    struct lane {
+        bool active = false, pending = false, done = false;
+        double t = 0.0, tol = 0.0;
+        expv_control c;
+        std::vector<double> y;
+        std::string error;
+    };
+    mk_generator_s *gen = nullptr;
+    i64 n = 0, Bp = 0, slots = 0, m = 0;
+    int np = 0;
+    dbuf W, V, p, partial, tmp, hd, d_lim, d_cnt, d_rows, d_sel, d_shift, stage;
+    double *host = nullptr; // pinned mirror of hd
+    int *ihost = nullptr;   // pinned: limits | counts | rows | sel
+    std::vector<lane> L;
+    ~mk_expv_batch_s() {
+        if (host) (void)hipHostFree(host);
+        if (ihost) (void)hipHostFree(ihost);
+    }
+    // device scalars, each [.. * Bp + lane]: the projections of both passes (m steps of m + 1
+    // slots), the remainders' squared norms (m + 1), |w|^2, and the coefficients y (m + 1)
+    i64 h1(i64 j) const { return j * (m + 1) * Bp; }
+    i64 h2(i64 j) const { return (m * (m + 1) + j * (m + 1)) * Bp; }
+    i64 nrm(i64 j) const { return (2 * m * (m + 1) + j) * Bp; }
+    i64 beta2() const { return (2 * m * (m + 1) + m + 1) * Bp; }
+    i64 y() const { return beta2() + Bp; }
+    i64 total() const { return y() + (m + 1) * Bp; }
+    double *dv(i64 k) const { return dp<double>(V) + k * n * Bp; }
+};
+
+namespace {
+
+using expv_batch = mk_expv_batch_s;
+
+// limits (n for the lanes in `on`, else 0) to the device
+void push_lanes(expv_batch &E, const std::vector<char> &on) {
+    for (i64 j = 0; j < E.Bp; ++j) E.ihost[j] = on[j] ? static_cast<int>(E.n) : 0;
+    HIPCHK(hipMemcpyAsync(E.d_lim.p, E.ihost, E.Bp * sizeof(int), hipMemcpyHostToDevice, stream()));
+}
+
+// |W_j|^2 of the lanes in `on` into hd[beta2] and its host mirror
+void lane_norms(expv_batch &E, const std::vector<char> &on) {
+    const hipStream_t s = stream();
+    push_lanes(E, on);
+    double *hd = dp<double>(E.hd);
+    const int *lim = dp<int>(E.d_lim);
+    bk::multi_dot(dp<double>(E.W), 1, dp<double>(E.W), E.n, E.Bp, dp<double>(E.partial), E.np, lim, s);
+    bk::reduce_partials(dp<double>(E.partial), E.np, E.Bp, hd + E.beta2(), s, dp<double>(E.tmp));
+    HIPCHK(hipMemcpyAsync(E.host + E.beta2(), hd + E.beta2(), E.Bp * sizeof(double), hipMemcpyDeviceToHost, s));
+    drain();
+}
+
+} // namespace
+
+namespace mk::rt {
+
+expv_batch *expv_engine_create(mk_generator_s *gen, i64 slots, i64 krylov_dim) {
+    require(gen && slots >= 1 && slots <= 4096 && krylov_dim >= 1, "expv batch: bad arguments");
+    auto E = std::make_unique<expv_batch>();
+    E->gen = gen;
+    E->n = gen->n;
+    E->slots = slots;
+    E->Bp = (slots + bk::kLanes - 1) / bk::kLanes * bk::kLanes;
+    E->m = std::max<i64>(1, std::min<i64>(krylov_dim, E->n));
+    require(E->m + 1 <= 512, "expv: krylov_dim above 511 is not supported on the device");
+    const i64 n = E->n, Bp = E->Bp, m = E->m;
+    const std::size_t vec = static_cast<std::size_t>(std::max<i64>(n, 1)) * Bp * sizeof(double);
+    E->W = dbuf(vec);
+    E->V = dbuf(vec * (m + 1));
+    E->p = dbuf(vec);
+    E->np = bk::parts(n);
+    E->partial = dbuf(static_cast<std::size_t>(E->np) * (m + 1) * Bp * sizeof(double));
+    E->tmp = dbuf(static_cast<std::size_t>(std::max<i64>(1, bk::reduce_tmp_size(E->np, (m + 1) * Bp))) *
+                  sizeof(double));
+    E->hd = dbuf(E->total() * sizeof(double));
+    E->d_lim = dbuf(Bp * sizeof(int));
+    E->d_cnt = dbuf(Bp * sizeof(int));
+    E->d_rows = dbuf(Bp * sizeof(int));
+    E->d_sel = dbuf(Bp * sizeof(int));
+    E->d_shift = dbuf(Bp * sizeof(double));
+    E->stage = dbuf(static_cast<std::size_t>(std::max<i64>(n, 1)) * sizeof(double));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&E->host), E->total() * sizeof(double)));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&E->ihost), 4 * Bp * sizeof(int)));
+    const hipStream_t s = stream();
+    // B = 0 I - Q: every lane's shift is zero; the iterate starts at zero on every lane
+    HIPCHK(hipMemsetAsync(E->d_shift.p, 0, Bp * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(E->W.p, 0, vec, s));
+    HIPCHK(hipMemsetAsync(E->hd.p, 0, E->total() * sizeof(double), s));
+    drain();
+    E->L.resize(Bp);
+    return E.release();
+}
+
+void expv_engine_destroy(expv_batch *E) {
+    if (!E) return;
+    drain();
+    delete E;
+}
+
+i64 expv_engine_slots(const expv_batch &E) { return E.slots; }
+i64 expv_engine_dim(const expv_batch &E) { return E.m; }
+
+// lane j takes v (n host values) for exp(t Q) v to the tolerance; it starts at the next step
+void expv_engine_admit(expv_batch &E, i64 j, const double *v, double t, double tol) {
+    require(j >= 0 && j < E.slots && !E.L[j].active, "expv batch: the slot is busy");
+    require(v && tol > 0.0 && std::isfinite(t), "expv: bad options");
+    const hipStream_t s = stream();
+    HIPCHK(hipMemcpyAsync(E.stage.p, v, E.n * sizeof(double), hipMemcpyHostToDevice, s));
+    bk::put_lane(dp<double>(E.stage), E.n, dp<double>(E.W), E.n, E.Bp, j, s);
+    drain();
+    auto &l = E.L[j];
+    l = expv_batch::lane{};
+    l.active = l.pending = true;
+    l.t = t;
+    l.tol = tol;
+}
+
+// one substep of every running lane; false when none is left running
+bool expv_engine_step(expv_batch &E) {
+    const i64 n = E.n, Bp = E.Bp, m = E.m;
+    const hipStream_t s = stream();
+    auto &P = prof();
+    // lanes admitted since the last step: their norms and first step sizes (the norms of every
+    // unfinished lane are recomputed with them, the same values for the lanes already running:
+    // a reduction writes every lane's slot)
+    std::vector<char> fresh(Bp, 0), live(Bp, 0);
+    bool any_fresh = false;
+    for (i64 j = 0; j < Bp; ++j) {
+        if (E.L[j].active && E.L[j].pending) fresh[j] = any_fresh = true;
+        live[j] = E.L[j].active && !E.L[j].done;
+    }
+    if (any_fresh) {
+        lane_norms(E, live);
+        for (i64 j = 0; j < Bp; ++j) {
+            if (!fresh[j]) continue;
+            auto &l = E.L[j];
+            l.pending = false;
+            // a zero vector or t = 0: w = v, already in the lane
+            if (!l.c.start(l.t, std::sqrt(E.host[E.beta2() + j]), *E.gen, l.tol, m)) l.done = true;
+        }
+    }
+    std::vector<char> run(Bp, 0);
+    bool any = false;
+    for (i64 j = 0; j < Bp; ++j)
+        if (E.L[j].active && !E.L[j].done) run[j] = any = true;
+    if (!any) return false;
+    push_lanes(E, run);
+    const int *lim = dp<int>(E.d_lim);
+    double *hd = dp<double>(E.hd), *pv = dp<double>(E.p), *part = dp<double>(E.partial);
+    double *tmp = dp<double>(E.tmp), *W = dp<double>(E.W);
+    const double *sh = dp<double>(E.d_shift);
+    const auto &A = E.gen->A;
+    const double nb = double(n) * Bp;
+    const double spmv_bytes = op_bytes(A, 0, 0, 0) + nb * 16.0;
+    // the Arnoldi process of every running lane: V_0 = W / beta, then m steps
+    P.run(K_OTHER, nb * 16.0, [&] { bk::scale_lanes(W, hd + E.beta2(), E.dv(0), n, Bp, lim, s); });
+    for (i64 j = 0; j < m; ++j) {
+        const int cnt = static_cast<int>(j + 1);
+        P.run(K_SPMV, spmv_bytes, [&] { bk::shifted_apply<double>(A.view, sh, E.dv(j), nullptr, pv, nullptr, Bp, lim, s); });
+        for (int pass = 0; pass < 2; ++pass) {
+            double *h = hd + (pass == 0 ? E.h1(j) : E.h2(j));
+            P.run(K_DOT, double(cnt + 1) * nb * 8.0, [&] {
+                bk::multi_dot(E.dv(0), cnt, pv, n, Bp, part, E.np, lim, s);
+                bk::reduce_partials(part, E.np, static_cast<i64>(cnt) * Bp, h, s, tmp);
+            });
+            P.run(K_UPDATE, double(cnt + 2) * nb * 8.0, [&] { bk::basis_update(pv, E.dv(0), cnt, h, n, Bp, lim, s); });
+        }
+        P.run(K_DOT, nb * 8.0, [&] {
+            bk::multi_dot(pv, 1, pv, n, Bp, part, E.np, lim, s);
+            bk::reduce_partials(part, E.np, Bp, hd + E.nrm(j), s, tmp);
+        });
+        P.run(K_OTHER, nb * 16.0, [&] { bk::scale_lanes(pv, hd + E.nrm(j), E.dv(j + 1), n, Bp, lim, s); });
+    }
+    // |B V_m|^2 for the error estimates
+    P.run(K_SPMV, spmv_bytes, [&] { bk::shifted_apply<double>(A.view, sh, E.dv(m), nullptr, pv, nullptr, Bp, lim, s); });
+    P.run(K_DOT, nb * 8.0, [&] {
+        bk::multi_dot(pv, 1, pv, n, Bp, part, E.np, lim, s);
+        bk::reduce_partials(part, E.np, Bp, hd + E.nrm(m), s, tmp);
+    });
+    HIPCHK(hipMemcpyAsync(E.host, hd, E.beta2() * sizeof(double), hipMemcpyDeviceToHost, s));
+    drain();
+    // every running lane's step control, in parallel on the host
+    std::vector<i64> use(Bp, 0);
+    mk::parallel_tasks(Bp, [&](i64 j) {
+        if (!run[j]) return;
+        auto &l = E.L[j];
+        const double *H = E.host;
+        try {
+            use[j] = l.c.plan([&](i64 i, i64 k) { return H[E.h1(k) + i * Bp + j] + H[E.h2(k) + i * Bp + j]; },
+                              [&](i64 k) { return H[E.nrm(k) + j]; }, l.y);
+        } catch (const std::exception &e) {
+            l.error = e.what();
+            use[j] = -1;
+        }
+    });
+    double *yh = E.host + E.y();
+    int *cnt = E.ihost + Bp, *rows = E.ihost + 2 * Bp, *sel = E.ihost + 3 * Bp;
+    std::fill(yh, yh + (m + 1) * Bp, 0.0);
+    for (i64 j = 0; j < Bp; ++j) {
+        cnt[j] = run[j] && use[j] > 0 ? static_cast<int>(use[j]) : 0;
+        rows[j] = 0;
+        sel[j] = cnt[j] > 0 ? 1 : 0;
+        for (i64 k = 0; k < cnt[j]; ++k) yh[k * Bp + j] = E.L[j].y[k];
+        if (run[j] && use[j] < 0) {
+            E.L[j].done = true; // the step control failed: the lane ends with its error
+            run[j] = 0;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(hd + E.y(), yh, (m + 1) * Bp * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(E.d_cnt.p, cnt, Bp * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(E.d_rows.p, rows, Bp * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(E.d_sel.p, sel, Bp * sizeof(int), hipMemcpyHostToDevice, s));
+    // w = V y on the lanes that took a step
+    P.run(K_OTHER, nb * 8.0 * (m + 2), [&] {
+        bk::clear_tails<double>(W, 1, n * Bp, n, Bp, dp<int>(E.d_rows), dp<int>(E.d_sel), s);
+        bk::combine<double>(W, E.dv(0), dp<int>(E.d_cnt), hd + E.y(), n, Bp, s);
+    });
+    lane_norms(E, run);
+    bool left = false;
+    for (i64 j = 0; j < Bp; ++j) {
+        if (!run[j]) continue;
+        auto &l = E.L[j];
+        if (!l.c.accept(std::sqrt(E.host[E.beta2() + j])) || !l.c.running()) l.done = true;
+        left = left || !l.done;
+    }
+    return left;
+}
+
+// the result of a finished lane (n values) and its statistics; false while it runs. A lane whose
+// step control failed throws its error.
+bool expv_engine_take(expv_batch &E, i64 j, double *w, mk_expv_result *res) {
+    require(j >= 0 && j < E.slots, "expv batch: bad slot");
+    auto &l = E.L[j];
+    if (!l.active || !l.done) return false;
+    if (!l.error.empty()) throw std::runtime_error(l.error);
+    const hipStream_t s = stream();
+    bk::get_lane(dp<double>(E.W), E.n, E.Bp, j, dp<double>(E.stage), s);
+    HIPCHK(hipMemcpyAsync(w, E.stage.p, E.n * sizeof(double), hipMemcpyDeviceToHost, s));
+    drain();
+    if (res) *res = l.c.res;
+    return true;
+}
+
+void expv_engine_release(expv_batch &E, i64 j) {
+    if (j >= 0 && j < E.slots) E.L[j] = expv_batch::lane{};
+}
+
+} // namespace mk::rt
 
 extern "C" {
 
@@ -221,6 +522,36 @@ mk_status mk_expv_device(mk_generator gen, double t, const double *d_v, double *
         require(gen && d_v && d_w && options && result, "expv: bad arguments");
         *result = expv_device(*gen, t, d_v, d_w, *options);
         drain();
+    });
+}
+
+mk_status mk_expv_batch(mk_generator gen, int64_t count, const double *t, const double *v, double *w,
+                        const mk_expv_options *options, mk_expv_result *results) {
+    return guarded([&] {
+        require(gen && options && count >= 0 && (count == 0 || (t && v && w && results)),
+                "expv batch: bad arguments");
+        require(options->krylov_dim >= 1 && options->tolerance > 0.0, "expv: bad options");
+        if (count == 0) return;
+        const i64 n = gen->n, slots = std::min<i64>(count, 256);
+        std::unique_ptr<mk_expv_batch_s, void (*)(mk_expv_batch_s *)> E(
+            mk::rt::expv_engine_create(gen, slots, options->krylov_dim), mk::rt::expv_engine_destroy);
+        std::vector<i64> in_slot(slots, -1);
+        i64 next = 0, finished = 0;
+        while (finished < count) {
+            for (i64 j = 0; j < slots && next < count; ++j) {
+                if (in_slot[j] >= 0) continue;
+                mk::rt::expv_engine_admit(*E, j, v + next * n, t[next], options->tolerance);
+                in_slot[j] = next++;
+            }
+            mk::rt::expv_engine_step(*E);
+            for (i64 j = 0; j < slots; ++j) {
+                const i64 q = in_slot[j];
+                if (q < 0 || !mk::rt::expv_engine_take(*E, j, w + q * n, results + q)) continue;
+                mk::rt::expv_engine_release(*E, j);
+                in_slot[j] = -1;
+                ++finished;
+            }
+        }
     });
 }
 

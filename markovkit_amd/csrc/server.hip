@@ -9,6 +9,10 @@
 // up. A request may be the leading block of whole sectors of its family's box (for an absorbing
 // one-species box, the box of a smaller cap), so the caps of one grid share an engine.
 //
+// Exponentials over a generator of at most batch_rows states share a batched expv engine
+// (expv.hip) per generator and Krylov dimension, run by its own thread the same way: vectors
+// join at substep boundaries.
+//
 // Larger resolvent families and exponentials run on worker threads, each with its own HIP
 // stream and Krylov workspaces, so their solves overlap on the device too. Such a request is
 // admitted first come first served once a worker is free, its footprint fits the HBM budget
@@ -28,6 +32,8 @@
 
 using namespace mk::rt;
 
+struct mk_expv_batch_s;
+
 namespace mk::rt {
 // the continuous-batching engine of batch.hip
 mk_batch_s *batch_engine_create(mk_generator_s *gen, mk_sector_mg_s *mg, i64 slots, i64 restart);
@@ -38,6 +44,14 @@ void batch_engine_admit(mk_batch_s &B, i64 j, const double *b, i64 rows, double 
 bool batch_engine_step(mk_batch_s &B);
 bool batch_engine_take(mk_batch_s &B, i64 j, double *x, mk_solver_result *res);
 void batch_engine_release(mk_batch_s &B, i64 j);
+// the batched exponentials of expv.hip
+mk_expv_batch_s *expv_engine_create(mk_generator_s *gen, i64 slots, i64 krylov_dim);
+void expv_engine_destroy(mk_expv_batch_s *E);
+i64 expv_engine_slots(const mk_expv_batch_s &E);
+void expv_engine_admit(mk_expv_batch_s &E, i64 j, const double *v, double t, double tol);
+bool expv_engine_step(mk_expv_batch_s &E);
+bool expv_engine_take(mk_expv_batch_s &E, i64 j, double *w, mk_expv_result *res);
+void expv_engine_release(mk_expv_batch_s &E, i64 j);
 } // namespace mk::rt
 
 namespace {
@@ -47,7 +61,8 @@ using clock_type = std::chrono::steady_clock;
 struct request {
     i64 id = 0;
     int kind = MK_REQUEST_RESOLVENT;
-    int family = -1;
+    int family = -1;      // resolvent: the family
+    int expv_family = -1; // exponential: the batch engine of its generator (-1: a worker)
     i64 rows = 0; // resolvent: the leading rows of the family's box the system has
     mk_generator gen = nullptr;
     double shift = 0.0, t = 0.0, tolerance = 0.0;
@@ -70,6 +85,15 @@ struct family {
     std::size_t zs = 8;
     // continuous batching: the engine, its queue and its thread (null engine: worker path)
     mk_batch_s *engine = nullptr;
+    std::deque<request *> queue;
+    std::thread thread;
+};
+
+// the exponentials of one generator and Krylov dimension, batched: engine, queue, thread
+struct expv_family {
+    mk_generator gen = nullptr;
+    i64 m = 0;
+    mk_expv_batch_s *engine = nullptr;
     std::deque<request *> queue;
     std::thread thread;
 };
@@ -109,6 +133,7 @@ struct mk_server_s {
     // a deque keeps every family at its address while mk_server_add_family appends (workers hold
     // no reference into it outside the lock either: run() gets the generator handle by value)
     std::deque<family> families;
+    std::deque<expv_family> expv_families;
     i64 reserved = 0;
     i64 next_id = 1;
     bool stopping = false;
@@ -218,6 +243,86 @@ struct mk_server_s {
         }
     }
 
+    // the engine thread of batched exponentials: admit queued vectors into free lanes, run one
+    // substep of every running lane, hand out the finished ones; repeat
+    void expv_loop(expv_family &f) {
+        mk_expv_batch_s &E = *f.engine;
+        const i64 slots = expv_engine_slots(E);
+        std::vector<request *> in_slot(slots, nullptr);
+        i64 busy = 0;
+        std::unique_lock<std::mutex> lock(mu);
+        while (true) {
+            cv.wait(lock, [&] { return stopping || !f.queue.empty() || busy > 0; });
+            if (stopping && f.queue.empty() && busy == 0) return;
+            std::vector<std::pair<i64, request *>> admit;
+            for (i64 j = 0; j < slots && !f.queue.empty(); ++j) {
+                if (in_slot[j]) continue;
+                request *r = f.queue.front();
+                f.queue.pop_front();
+                r->started = clock_type::now();
+                in_slot[j] = r;
+                ++busy;
+                admit.emplace_back(j, r);
+            }
+            lock.unlock();
+            std::vector<std::pair<request *, std::string>> failed;
+            for (auto &[j, r] : admit) {
+                try {
+                    expv_engine_admit(E, j, r->in.data(), r->t, r->eopt.tolerance);
+                } catch (const std::exception &e) {
+                    failed.emplace_back(r, e.what());
+                    in_slot[j] = nullptr;
+                    expv_engine_release(E, j);
+                }
+            }
+            std::vector<request *> finished;
+            try {
+                expv_engine_step(E);
+                for (i64 j = 0; j < slots; ++j) {
+                    request *r = in_slot[j];
+                    if (!r) continue;
+                    r->out.resize(r->in.size());
+                    bool done = false;
+                    try {
+                        done = expv_engine_take(E, j, r->out.data(), &r->expv);
+                        if (done) finished.push_back(r);
+                    } catch (const std::exception &e) {
+                        // this lane's step control failed; the others go on
+                        failed.emplace_back(r, e.what());
+                        done = true;
+                    }
+                    if (done) {
+                        in_slot[j] = nullptr;
+                        expv_engine_release(E, j);
+                    }
+                }
+            } catch (const std::exception &e) {
+                // a device failure ends every request of the engine
+                for (i64 j = 0; j < slots; ++j)
+                    if (in_slot[j]) {
+                        failed.emplace_back(in_slot[j], e.what());
+                        in_slot[j] = nullptr;
+                        expv_engine_release(E, j);
+                    }
+            }
+            lock.lock();
+            for (request *r : finished) {
+                r->finished = clock_type::now();
+                r->status = MK_OK;
+                r->done = true;
+                --busy;
+            }
+            for (auto &[r, e] : failed) {
+                r->finished = clock_type::now();
+                r->status = MK_ERR_RUNTIME;
+                r->error = e;
+                r->done = true;
+                --busy;
+            }
+            cv.notify_all();
+        }
+    }
+
     void worker() {
         worker_buffers buf;
         std::unique_lock<std::mutex> lock(mu);
@@ -272,6 +377,8 @@ i64 submit(mk_server s, std::unique_ptr<request> r) {
     const i64 id = r->id;
     if (r->kind == MK_REQUEST_RESOLVENT && s->families[r->family].engine) {
         s->families[r->family].queue.push_back(r.get());
+    } else if (r->kind == MK_REQUEST_EXPV && r->expv_family >= 0) {
+        s->expv_families[r->expv_family].queue.push_back(r.get());
     } else {
         s->queue.push_back(r.get());
     }
@@ -305,6 +412,10 @@ mk_status mk_server_destroy(mk_server s) {
     for (auto &f : s->families) {
         if (f.thread.joinable()) f.thread.join();
         if (f.engine) batch_engine_destroy(f.engine);
+    }
+    for (auto &f : s->expv_families) {
+        if (f.thread.joinable()) f.thread.join();
+        if (f.engine) expv_engine_destroy(f.engine);
     }
     delete s;
     return MK_OK;
@@ -394,6 +505,7 @@ mk_status mk_server_submit_expv(mk_server s, mk_generator gen, double t, const d
                                 const mk_expv_options *options, int64_t *id) {
     return guarded([&] {
         require(s && gen && v && options && id, "server: bad arguments");
+        require(options->krylov_dim >= 1 && options->tolerance > 0.0, "expv: bad options");
         auto r = std::make_unique<request>();
         r->kind = MK_REQUEST_EXPV;
         r->gen = gen;
@@ -401,6 +513,33 @@ mk_status mk_server_submit_expv(mk_server s, mk_generator gen, double t, const d
         r->eopt = *options;
         r->in.assign(v, v + gen->n);
         r->footprint = expv_footprint(gen->n, std::max<i64>(1, options->krylov_dim));
+        {
+            // continuous batching: the exponentials of a generator of at most the batching row
+            // limit share a batch engine per Krylov dimension, made at its first request
+            std::lock_guard<std::mutex> lock(s->mu);
+            require(!s->stopping, "server: stopping");
+            const i64 m = std::max<i64>(1, std::min<i64>(options->krylov_dim, gen->n));
+            if (gen->n <= s->batch_rows && s->batch_slots > 0) {
+                int found = -1;
+                for (std::size_t k = 0; k < s->expv_families.size(); ++k)
+                    if (s->expv_families[k].gen == gen && s->expv_families[k].m == m) found = static_cast<int>(k);
+                if (found < 0) {
+                    expv_family &f = s->expv_families.emplace_back();
+                    f.gen = gen;
+                    f.m = m;
+                    try {
+                        f.engine = expv_engine_create(gen, s->batch_slots, m);
+                    } catch (...) {
+                        s->expv_families.pop_back();
+                        throw;
+                    }
+                    f.thread = std::thread([s, &f] { s->expv_loop(f); });
+                    found = static_cast<int>(s->expv_families.size() - 1);
+                }
+                r->expv_family = found;
+                r->footprint = 0;
+            }
+        }
         *id = submit(s, std::move(r));
     });
 }

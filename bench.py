@@ -454,6 +454,67 @@ def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0, smooth
     }
 
 
+def c3(mk, workloads, steps, restart):
+    """2D two-species annihilation on 8 x 8 voxels (h = 1/8, D = 1, births 0.1 * volume each,
+    A + B -> 0 at n_A n_B / volume), caps {2, 2}: 4,601,025 states, s = 1, b = e_0. FGMRES(restart)
+    preconditioned by the red-black sector multigrid (V-cycle, 2 + 2 sweeps), mixed (FP32
+    multigrid) and plain FP64, each to 1e-10 relative residual. Checked in the run: true
+    residual, mass balance 1^T b = sum (s + leak) x, x >= -bound, and the mixed solution within
+    the L1 bound of the FP64 one."""
+    w = workloads.config_2d()
+    shift = w.shifts[0]
+    t = time.perf_counter()
+    box = mk.PopulationBox(w.n_voxels, w.max_totals)
+    Q = mk.Generator(box, w.model)
+    n = box.size()
+    mk.synchronize()
+    t_gen = time.perf_counter() - t
+    b = w.rhs(0, n)
+    tol = 1e-10 * float(np.linalg.norm(b))
+    A = mk.ShiftedGenerator(Q, shift)
+    b_dev, x_dev = mk.DeviceVector(n, b), mk.DeviceVector(n)
+    abs_A = shift + 2 * Q.exit_rates() - Q.leak_rates()
+    out = {"workload": f"{w.name}: 2D two-species annihilation, {n} states, s={shift}, b=e_0, "
+                       f"FGMRES({restart}) + red-black sector multigrid (V, 2+2), tol 1e-10 relative",
+           "states": n, "setup_s": {"generator": t_gen}}
+    xs = {}
+    for name, prec in (("mixed", mk.FP32), ("fp64", mk.FP64)):
+        t = time.perf_counter()
+        M = mk.SectorMultigrid(w.model.domain, box, shift,
+                               mk.MultigridOptions(precision=prec, smoother=mk.RED_BLACK))
+        mk.synchronize()
+        out["setup_s"][f"multigrid_{name}"] = time.perf_counter() - t
+        run_solves(mk, A, M, b_dev, x_dev, tol, 2, restart)
+        total, lat, iters, res, _ = run_solves(mk, A, M, b_dev, x_dev, tol, steps, restart)
+        x = x_dev.download()
+        r = b - Q.apply_shifted(shift, x)
+        bound = float((np.abs(r).sum() + 16 * EPS * (abs_A * np.abs(x)).sum()) / shift)
+        # 1^T b - sum (s + leak) x = 1^T r exactly, so it is at most ||r||_1 plus rounding
+        mass = abs(float(b.sum()) - float(((shift + Q.leak_rates()) * x).sum()))
+        mass_bound = float(np.abs(r).sum() + 64 * EPS * ((shift + Q.leak_rates()) * np.abs(x)).sum())
+        rel = float(np.linalg.norm(r) / np.linalg.norm(b))
+        if not (rel <= 1.0001e-10 and float(x.min()) >= -bound and mass <= mass_bound):
+            raise RuntimeError(f"C3 {name}: residual {rel:.3e}, min x {x.min():.3e}, mass {mass:.3e}")
+        xs[name] = (x, r)
+        rec = {"value": steps / total, "p50_ms": 1e3 * percentile(lat, 50),
+               "p99_ms": 1e3 * percentile(lat, 99), "iterations": iters[-1],
+               "relative_residual": rel, "l1_bound": bound, "mass_balance_error": mass,
+               "min_x": float(x.min())}
+        if name == "mixed":
+            out.update(rec)
+        else:
+            out["fp64"] = rec
+        del M
+    (xm, rm), (x64, r64) = xs["mixed"], xs["fp64"]
+    out["l1_mixed_vs_fp64"] = float(np.abs(xm - x64).sum())
+    out["l1_mixed_vs_fp64_bound"] = float((np.abs(rm).sum() + np.abs(r64).sum() + 16 * EPS *
+                                           (abs_A * (np.abs(xm) + np.abs(x64))).sum()) / shift)
+    if out["l1_mixed_vs_fp64"] > out["l1_mixed_vs_fp64_bound"]:
+        raise RuntimeError(f"C3: mixed and FP64 solutions differ by {out['l1_mixed_vs_fp64']:.3e}")
+    out["mixed_over_fp64"] = out["value"] / out["fp64"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)

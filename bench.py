@@ -454,6 +454,63 @@ def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0, smooth
     }
 
 
+def measure_traffic(args, kernel_class):
+    """HBM bytes per launch of the finest level's sweep kernel, measured in this run: this script
+    runs again in --traffic-probe mode (setup, a 256 MiB copy with the library's plain copy
+    kernel, two solves) under `rocprofv3 --pmc FETCH_SIZE`, then under `--pmc WRITE_SIZE`, one
+    counter per pass; each counter is scaled to bytes by what it reads for the copy (256 MiB read,
+    256 MiB written). Returns None with the reason when rocprofv3 is absent or a pass fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof_bin = shutil.which("rocprofv3")
+    if prof_bin is None:
+        return {"bytes_per_launch": None, "error": "rocprofv3 not found"}
+    mode = "csr_mode)5" if args.smoother == "red_black" else "csr_mode)2"
+    copy_bytes = 256 << 20
+    passthrough = ["--workload", args.workload, "--pre", str(args.pre), "--post", str(args.post),
+                   "--cycles", str(args.cycles), "--cycle-levels", str(args.cycle_levels),
+                   "--coarsest", str(args.coarsest), "--smoother", args.smoother,
+                   "--restart", str(args.restart)]
+    out = {"kernel_class": kernel_class}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="mk_pmc_", dir="/tmp")
+        cmd = [prof_bin, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t", "--",
+               sys.executable, os.path.abspath(__file__), "--traffic-probe", *passthrough]
+        env = dict(os.environ, TMPDIR="/tmp")
+        try:
+            r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=240)
+        except subprocess.TimeoutExpired:
+            return {"bytes_per_launch": None, "error": f"{counter} pass timed out"}
+        path = os.path.join(d, "t_counter_collection.csv")
+        if r.returncode != 0 or not os.path.exists(path):
+            return {"bytes_per_launch": None, "error": f"{counter} pass failed ({r.returncode})"}
+        copy, sweep = [], {}
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row["Counter_Name"] != counter:
+                    continue
+                name, grid, v = row["Kernel_Name"], int(row["Grid_Size"]), float(row["Counter_Value"])
+                if "k_copy16" in name:
+                    copy.append(v)
+                elif "k_csr_sliced<" in name and mode in name:
+                    sweep.setdefault(grid, []).append(v)
+        if not copy or not sweep:
+            return {"bytes_per_launch": None, "error": f"{counter} pass: kernels not found"}
+        scale = copy_bytes / (float(np.mean(copy)) * 1024.0)
+        fine = max(sweep)  # the finest level's grid
+        key = "fetch" if counter == "FETCH_SIZE" else "write"
+        out[f"{key}_bytes"] = float(np.mean(sweep[fine])) * 1024.0 * scale
+        out[f"{key}_scale"] = scale
+        out["dispatches"] = len(sweep[fine])
+        out["grid"] = fine
+    out["bytes_per_launch"] = out["fetch_bytes"] + out["write_bytes"]
+    out["source"] = ("rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, in this run (two solves of the "
+                     "same hierarchy), each scaled by its reading for a 256 MiB copy")
+    return out
+
+
 def c3(mk, workloads, steps, restart):
     """2D two-species annihilation on 8 x 8 voxels (h = 1/8, D = 1, births 0.1 * volume each,
     A + B -> 0 at n_A n_B / volume), caps {2, 2}: 4,601,025 states, s = 1, b = e_0. FGMRES(restart)
@@ -559,6 +616,10 @@ def main():
                     help="skip the serving mix after the headline")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the 2D two-species annihilation solves after the headline")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the HBM-counter passes (rocprofv3 --pmc) of the roofline traffic")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="internal: setup, a 256 MiB copy and two solves, for the counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (each solve "
@@ -636,6 +697,11 @@ def main():
     tol = 1e-10 * float(np.linalg.norm(b))
     b_dev = mk.DeviceVector(n, b)
     x_dev = mk.DeviceVector(n)
+    if args.traffic_probe:
+        from markovkit_amd.calibrate import copy_bandwidth
+        copy_bandwidth(256 << 20, 1)
+        run_solves(mk, A, M32, b_dev, x_dev, tol, 2, args.restart)
+        return
 
     # warm-up, and a fully profiled solve (the per-class breakdown)
     run_solves(mk, A, M32, b_dev, x_dev, tol, max(args.warmup, 1), args.restart)
@@ -704,11 +770,12 @@ def main():
     avg_ms = d["ms"] / launches
     bytes_per_launch = d["bytes"] / launches
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"{dominant}_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get(w.name, {}).get("bytes_per_launch")
+    measured = None
+    if args.workload == "large" and not args.no_traffic:
+        log("[traffic] FETCH_SIZE and WRITE_SIZE passes (rocprofv3 --pmc)")
+        measured = measure_traffic(args, dominant)
+        log(f"[traffic] {measured}")
+    traffic = measured.get("bytes_per_launch") if measured else None
 
     out = {
         "metric": METRIC,
@@ -754,6 +821,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
             "traffic": traffic,
+            "traffic_measurement": measured,
             "launches": launches,
             "avg_launch_ms": avg_ms,
             "algorithmic_bytes_per_launch": bytes_per_launch,

@@ -344,7 +344,9 @@ __global__ void __launch_bounds__(kThreads)
     k_dcgs_update(double *V, int count, const double *sc, const double *tg, const double *beta,
                   const double *w, i64 n, i64 Bp, double *partial, int np, double *eta_out,
                   const int *limit) {
-    constexpr int kRows = PW <= 8 ? 4 : 2;
+    // rows per pair of barriers: four while few basis values sit in registers, else one (more
+    // rows' values in registers cost occupancy: measured slower from PW = 3 on)
+    constexpr int kRows = PW <= 2 ? 4 : 1;
     __shared__ double part[kRows][2][kWaves][kLanes];
     __shared__ double uw[kRows][2][kLanes];
     __shared__ double red[2][kLanes];

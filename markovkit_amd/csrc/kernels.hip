@@ -892,10 +892,18 @@ __global__ void __launch_bounds__(kThreads) k_axpy(i64 n, T alpha, const T *x, T
         y[i] += alpha * x[i];
 }
 
+// four 16-byte loads in flight per thread, the grid's blocks 4 * 256 elements apart
 __global__ void __launch_bounds__(kThreads) k_copy16(i64 n, const uint4 *src, uint4 *dst) {
-    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; i < n;
-         i += static_cast<i64>(gridDim.x) * kThreads)
-        dst[i] = src[i];
+    const i64 stride = static_cast<i64>(gridDim.x) * kThreads * 4;
+    for (i64 i = blockIdx.x * static_cast<i64>(kThreads) * 4 + threadIdx.x; i < n; i += stride) {
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i + u * kThreads < n) v[u] = src[i + u * kThreads];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i + u * kThreads < n) dst[i + u * kThreads] = v[u];
+    }
 }
 
 // ---- Krylov kernels ----
@@ -1391,7 +1399,8 @@ void axpy(i64 n, T alpha, const T *x, T *y, hipStream_t s) {
 
 void copy16(i64 n, const void *src, void *dst, hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_copy16, dim3(grid_for(n)), dim3(kThreads), 0, s, n,
+    const i64 blocks = std::max<i64>(1, std::min<i64>((n + 4 * kThreads - 1) / (4 * kThreads), 65536));
+    hipLaunchKernelGGL(k_copy16, dim3(static_cast<u32>(blocks)), dim3(kThreads), 0, s, n,
                        static_cast<const uint4 *>(src), static_cast<uint4 *>(dst));
 }
 

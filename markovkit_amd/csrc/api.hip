@@ -713,7 +713,10 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             P.run(cls(K_RESTRICT), rbytes, [&] {
                 gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.rhs), s);
             });
-            T *cx = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr);
+            // the coarse levels' launches are timed as one interval per visit (an event pair per
+            // small launch would cost as much as the launch)
+            T *cx = nullptr;
+            P.span(K_COARSE, [&] { cx = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr); });
             P.run(cls(K_PROLONG), pbytes, [&] {
                 gpu::prolong_add<T>(np, dp<u32>(lv.coarse_of), dp<T>(lv.weight), cx, cur, s, prows);
             });
@@ -726,24 +729,27 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
                                  tail_switch().load() != 0;
             // from cx = 0 the first coarse residual is crhs itself and the first correction
             // is cx: that cycle writes cx directly
-            run_level<T>(mg, L, l + 1, dp<T>(c.crhs), nullptr, dp<T>(c.cx), nullptr,
-                         in_tail ? mg.opt.coarse_cycles : 1);
             const double cbytes = op_bytes(c.A, ts, ts, 2);
             // the last cycle's correction is added inside the prolongation (cx + ce, then the
             // weighted add, as the separate axpy and prolongation round)
             T *last = nullptr;
-            for (int cyc = 1; !in_tail && cyc < mg.opt.coarse_cycles; ++cyc) {
-                P.run(cls(K_RESIDUAL), cbytes, [&] {
-                    gpu::level_residual<T>(c.A.view, shift, dp<T>(c.crhs), dp<T>(c.cx),
-                                           dp<T>(c.rhs), s);
-                });
-                T *ce = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr);
-                if (cyc + 1 < mg.opt.coarse_cycles) {
-                    P.run(cls(K_OTHER), double(c.n) * 3 * ts, [&] { gpu::axpy<T>(c.n, T(1), ce, dp<T>(c.cx), s); });
-                } else {
-                    last = ce;
+            // every launch of the coarse cycles works on the coarse level: one timed interval
+            P.span(K_COARSE, [&] {
+                run_level<T>(mg, L, l + 1, dp<T>(c.crhs), nullptr, dp<T>(c.cx), nullptr,
+                             in_tail ? mg.opt.coarse_cycles : 1);
+                for (int cyc = 1; !in_tail && cyc < mg.opt.coarse_cycles; ++cyc) {
+                    P.run(K_COARSE, cbytes, [&] {
+                        gpu::level_residual<T>(c.A.view, shift, dp<T>(c.crhs), dp<T>(c.cx),
+                                               dp<T>(c.rhs), s);
+                    });
+                    T *ce = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr);
+                    if (cyc + 1 < mg.opt.coarse_cycles) {
+                        P.run(K_COARSE, double(c.n) * 3 * ts, [&] { gpu::axpy<T>(c.n, T(1), ce, dp<T>(c.cx), s); });
+                    } else {
+                        last = ce;
+                    }
                 }
-            }
+            });
             P.run(cls(K_PROLONG), pbytes, [&] {
                 gpu::prolong_add<T>(np, dp<u32>(lv.coarse_of), dp<T>(lv.weight), dp<T>(c.cx),
                                     cur, s, prows, last);

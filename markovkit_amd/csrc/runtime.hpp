@@ -159,6 +159,7 @@ struct profiler {
         int cls;
         hipEvent_t a, b;
         double bytes;
+        i64 count = 1; // launches the interval covers
     };
     std::vector<rec> pending;
     std::vector<hipEvent_t> pool;
@@ -176,18 +177,57 @@ struct profiler {
         HIPCHK(hipEventCreate(&e));
         return e;
     }
+    // a span: one event pair around a sequence of launches of class span_cls, whose own
+    // launches record no events (their bytes add up into the span)
+    int span_cls = -1;
+    i64 span_launches = 0;
+    double span_bytes = 0.0;
+
     template <typename F>
     void run(int cls, double nbytes, F &&launch) {
         if (!(mask & (1u << cls))) {
+            if (cls == span_cls) {
+                span_bytes += nbytes;
+                ++span_launches;
+            }
             launch();
             check_launch(kclass_names[cls]);
             return;
         }
-        rec r{cls, event(), event(), nbytes};
+        rec r{cls, event(), event(), nbytes, 1};
         HIPCHK(hipEventRecord(r.a, stream()));
         launch();
         check_launch(kclass_names[cls]);
         HIPCHK(hipEventRecord(r.b, stream()));
+        pending.push_back(r);
+        if (pending.size() > 4096) resolve();
+    }
+    // time body's launches of class cls as one interval: for runs of small latency-bound
+    // kernels, where an event pair per launch would cost as much as the launch
+    template <typename F>
+    void span(int cls, F &&body) {
+        if (!(mask & (1u << cls)) || span_cls >= 0) {
+            body();
+            return;
+        }
+        rec r{cls, event(), event(), 0.0, 0};
+        HIPCHK(hipEventRecord(r.a, stream()));
+        mask &= ~(1u << cls);
+        span_cls = cls;
+        span_bytes = 0.0;
+        span_launches = 0;
+        try {
+            body();
+        } catch (...) {
+            mask |= 1u << cls;
+            span_cls = -1;
+            throw;
+        }
+        mask |= 1u << cls;
+        span_cls = -1;
+        HIPCHK(hipEventRecord(r.b, stream()));
+        r.bytes = span_bytes;
+        r.count = span_launches;
         pending.push_back(r);
         if (pending.size() > 4096) resolve();
     }
@@ -196,7 +236,7 @@ struct profiler {
             HIPCHK(hipEventSynchronize(r.b));
             float t = 0.f;
             HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
-            launches[r.cls] += 1;
+            launches[r.cls] += r.count;
             ms[r.cls] += t;
             bytes[r.cls] += r.bytes;
             pool.push_back(r.a);

@@ -470,6 +470,11 @@ mk_status mk_device_setup_enable(int32_t on);
  * 150 KB (default on; MK_TAIL_LDS=0 turns it off). Results are bitwise the same either way. No
  * reference counterpart. */
 mk_status mk_tail_lds_enable(int32_t on);
+/* A red-black level's first half-sweep (from x = 0) computed by the kernel that makes the level's
+ * right-hand side, the restriction or the coarse-cycle residual, instead of a launch of its own
+ * (default on; MK_FUSED_FIRST=0 turns it off at start). Results are bitwise the same either way.
+ * No reference counterpart. */
+mk_status mk_fused_first_enable(int32_t on);
 /* Workgroup-to-XCD mapping of the operator kernels (no reference counterpart): XCD j takes the
  * chunks j, j + 8, ... of `chunk` consecutive row blocks; 0 keeps workgroup b on row block b.
  * Default 16 (MK_XCD_CHUNK overrides). Process-wide; the autotuner sets it. */

@@ -650,6 +650,12 @@ def tail_lds_enable(on: bool = True) -> None:
     check(lib().mk_tail_lds_enable(1 if on else 0))
 
 
+def fused_first_enable(on: bool = True) -> None:
+    """Fuse a red-black level's first half-sweep into the kernel that makes its right-hand side
+    (default on); bitwise the same."""
+    check(lib().mk_fused_first_enable(1 if on else 0))
+
+
 def fused_tail_enable(on: bool = True) -> None:
     """Run the small multigrid levels in one workgroup-wide kernel (default on)."""
     check(lib().mk_fused_tail_enable(1 if on else 0))

@@ -542,9 +542,22 @@ void block_smooth(const mk_sector_mg_s &mg, level_dev<T> &lv, std::size_t level,
 // on all active sectors at once. rhs is this level's right-hand side; rhs64 (level 0 only) is an
 // FP64 input that the first sweep multiplies by scale and stores into lv.rhs in precision T. The
 // result is written to out when given, else left in a level buffer whose pointer is returned.
+// whether level l's first red-black half-sweep can ride on the kernel that produces its
+// right-hand side: a red-black level with pre-smoothing, in natural order, run by the
+// per-level kernels (not the fused tail, not a dense level)
+template <typename T>
+bool first_fusable(const mk_sector_mg_s &mg, const std::vector<level_dev<T>> &L, std::size_t l) {
+    if (l >= L.size()) return false;
+    const auto &lv = L[l];
+    const bool tail = mg.tail_start >= 0 && static_cast<i64>(l) >= mg.tail_start && tail_switch().load() != 0;
+    return lv.rb && lv.pre > 0 && !lv.direct && !lv.ilu && lv.n_agg == 0 && !tail && lv.perm.p == nullptr &&
+           lv.colour.p != nullptr && fused_first_switch().load() != 0;
+}
+
 template <typename T>
 T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t l, const T *rhs,
-             const double *rhs64, T *out, const double *scale = nullptr, int cycles = 1) {
+             const double *rhs64, T *out, const double *scale = nullptr, int cycles = 1,
+             bool first_done = false) {
     auto &lv = L[l];
     auto &P = prof();
     auto cls = [&](kclass c) { return l == 0 ? c : K_COARSE; };
@@ -608,7 +621,11 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
         // colour-1 half-sweep only colour-0 rows have a residual
         if (lv.pre > 0) {
             T *rb = rhs64 || perm0 ? dp<T>(lv.rhs) : nullptr;
-            P.run(cls(K_FIRST), double(n) * ((rhs64 ? 8.0 + ts : double(ts)) + 2.0 + ts / 2.0 + (rb ? ts : 0)), [&] {
+            // first_done: the kernel that produced rhs (restriction or residual) also ran this
+            // first half-sweep into cur
+            if (first_done && (rb || cycles != 1))
+                throw std::logic_error("sector_multigrid: a fused first half-sweep on a level that reorders");
+            if (!first_done) P.run(cls(K_FIRST), double(n) * ((rhs64 ? 8.0 + ts : double(ts)) + 2.0 + ts / 2.0 + (rb ? ts : 0)), [&] {
                 if (rhs64) {
                     gpu::rb_first<T, double>(lv.A.view, dp<std::uint8_t>(lv.colour), shift, rhs64, rb, cur, s,
                                              scale, perm0);
@@ -710,23 +727,40 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
         const u32 *prows = red_only ? dp<u32>(lv.rowmap[0]) : nullptr;
         const double pbytes = double(np) * (4 + 3 * ts + (red_only ? 4 : 0)) + double(c.n) * ts;
         if (mg.opt.coarse_cycles <= 1 || l >= cycle_levels(mg.opt)) {
+            // the coarse level's first red-black half-sweep rides on the restriction
+            const bool fuse = first_fusable(mg, L, l + 1);
             P.run(cls(K_RESTRICT), rbytes, [&] {
-                gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.rhs), s);
+                if (fuse) {
+                    gpu::gather_sum_first<T>(lv.R.view, res, dp<T>(c.rhs), c.A.view,
+                                             dp<std::uint8_t>(c.colour), shift, dp<T>(c.x), s);
+                } else {
+                    gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.rhs), s);
+                }
             });
             // the coarse levels' launches are timed as one interval per visit (an event pair per
             // small launch would cost as much as the launch)
             T *cx = nullptr;
-            P.span(K_COARSE, [&] { cx = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr); });
+            P.span(K_COARSE, [&] {
+                cx = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr, nullptr, 1, fuse);
+            });
             P.run(cls(K_PROLONG), pbytes, [&] {
                 gpu::prolong_add<T>(np, dp<u32>(lv.coarse_of), dp<T>(lv.weight), cx, cur, s, prows);
             });
         } else {
-            P.run(cls(K_RESTRICT), rbytes, [&] {
-                gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.crhs), s);
-            });
             // the tail runs all of the coarse cycles in one launch
             const bool in_tail = mg.tail_w && mg.tail_start == static_cast<i64>(l + 1) &&
                                  tail_switch().load() != 0;
+            // the first coarse cycle writes cx directly, so its first half-sweep, fused into the
+            // restriction, goes there too
+            const bool fuse = !in_tail && first_fusable(mg, L, l + 1);
+            P.run(cls(K_RESTRICT), rbytes, [&] {
+                if (fuse) {
+                    gpu::gather_sum_first<T>(lv.R.view, res, dp<T>(c.crhs), c.A.view,
+                                             dp<std::uint8_t>(c.colour), shift, dp<T>(c.cx), s);
+                } else {
+                    gpu::gather_sum<T>(lv.R.view, res, dp<T>(c.crhs), s);
+                }
+            });
             // from cx = 0 the first coarse residual is crhs itself and the first correction
             // is cx: that cycle writes cx directly
             const double cbytes = op_bytes(c.A, ts, ts, 2);
@@ -736,13 +770,20 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             // every launch of the coarse cycles works on the coarse level: one timed interval
             P.span(K_COARSE, [&] {
                 run_level<T>(mg, L, l + 1, dp<T>(c.crhs), nullptr, dp<T>(c.cx), nullptr,
-                             in_tail ? mg.opt.coarse_cycles : 1);
+                             in_tail ? mg.opt.coarse_cycles : 1, fuse);
                 for (int cyc = 1; !in_tail && cyc < mg.opt.coarse_cycles; ++cyc) {
+                    // the next cycle's first half-sweep rides on its residual (into c.x)
                     P.run(K_COARSE, cbytes, [&] {
-                        gpu::level_residual<T>(c.A.view, shift, dp<T>(c.crhs), dp<T>(c.cx),
-                                               dp<T>(c.rhs), s);
+                        if (fuse) {
+                            gpu::level_residual_first<T>(c.A.view, shift, dp<T>(c.crhs), dp<T>(c.cx),
+                                                         dp<T>(c.rhs), c.A.view,
+                                                         dp<std::uint8_t>(c.colour), dp<T>(c.x), s);
+                        } else {
+                            gpu::level_residual<T>(c.A.view, shift, dp<T>(c.crhs), dp<T>(c.cx),
+                                                   dp<T>(c.rhs), s);
+                        }
                     });
-                    T *ce = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr);
+                    T *ce = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr, nullptr, 1, fuse);
                     if (cyc + 1 < mg.opt.coarse_cycles) {
                         P.run(K_COARSE, double(c.n) * 3 * ts, [&] { gpu::axpy<T>(c.n, T(1), ce, dp<T>(c.cx), s); });
                     } else {
@@ -1014,7 +1055,7 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
                 // the fused-tail switches choose the launches captured: both are part of the key
                 graph_key key{A.uid, M ? M->uid : 0, j,
                               static_cast<i64>(sd > sd_next) + 2 * tail_switch().load() +
-                                  4 * tail_lds_switch().load(),
+                                  4 * tail_lds_switch().load() + 8 * fused_first_switch().load(),
                               n, m, bits(shift)};
                 step(key, [&] {
                     operator_image(j, V + j * ld, scal);
@@ -1456,6 +1497,11 @@ mk_status mk_get_xcd_chunk(int32_t *chunk) {
 
 mk_status mk_device_setup_enable(int32_t on) {
     device_setup_switch().store(on ? 1 : 0);
+    return MK_OK;
+}
+
+mk_status mk_fused_first_enable(int32_t on) {
+    fused_first_switch().store(on ? 1 : 0);
     return MK_OK;
 }
 

@@ -67,6 +67,12 @@ struct csr_args {
     T *nat;          // gs: also store the new value at nat[perm[row]] (optional)
     const u32 *perm;
     u32 chunk;       // XCD interleave chunk (set by the launcher)
+    // residual and gather_sum: the next level's first red-black half-sweep fused in (first_x
+    // not null): where first_colour[r] == 0, first_x[r] = out[r] / (shift + exit of first_A's
+    // row r), the formula of k_rb_first
+    const std::uint8_t *first_colour = nullptr;
+    csr_view<T> first_A{};
+    T *first_x = nullptr;
 };
 
 // vector entry of the operator's local row r
@@ -75,10 +81,21 @@ __device__ __forceinline__ u32 vec_row(const csr_view<VT> &A, u32 r) {
     return A.rowmap ? A.rowmap[r] : r;
 }
 
+// the first red-black half-sweep from x = 0 of the level a.first_A, on row r with right-hand
+// side b
+template <typename VT, typename XT, typename T>
+__device__ __forceinline__ void first_half(const csr_args<VT, XT, T> &a, u32 r, T b) {
+    if (a.first_x && a.first_colour[r] == 0) {
+        const T ex = a.first_A.exit ? a.first_A.exit[r] : a.first_A.edict[a.first_A.rcode[r]];
+        a.first_x[r] = b / (static_cast<T>(a.shift) + ex);
+    }
+}
+
 template <csr_mode M, typename VT, typename XT, typename T>
 __device__ __forceinline__ double finish(const csr_args<VT, XT, T> &a, u32 r, T sum, T exit_r) {
     if constexpr (M == csr_mode::gather_sum) {
         a.out[r] = sum;
+        first_half(a, r, sum);
         return 0.0;
     }
     if constexpr (M == csr_mode::gs) {
@@ -107,7 +124,9 @@ __device__ __forceinline__ double finish(const csr_args<VT, XT, T> &a, u32 r, T 
         a.out[r] = sum;
         return 0.0;
     } else {
-        a.out[r] = a.rhs[r] - ax;
+        const T res = a.rhs[r] - ax;
+        a.out[r] = res;
+        first_half(a, r, res);
         return 0.0;
     }
 }
@@ -1274,6 +1293,26 @@ void gather_sum(const csr_view<T> &R, const T *x, T *y, hipStream_t s) {
 }
 
 template <typename T>
+void gather_sum_first(const csr_view<T> &R, const T *x, T *y, const csr_view<T> &Ac,
+                      const std::uint8_t *colour, double shift, T *cx, hipStream_t s) {
+    csr_args<T, T, T> a{R, shift, T(0), x, nullptr, y, nullptr, nullptr, nullptr, nullptr, 0u};
+    a.first_colour = colour;
+    a.first_A = Ac;
+    a.first_x = cx;
+    launch_csr<csr_mode::gather_sum>(a, s);
+}
+
+template <typename T>
+void level_residual_first(const csr_view<T> &A, double shift, const T *rhs, const T *x, T *r,
+                          const csr_view<T> &Ac, const std::uint8_t *colour, T *cx, hipStream_t s) {
+    csr_args<T, T, T> a{A, shift, T(0), x, rhs, r, nullptr, nullptr, nullptr, nullptr, 0u};
+    a.first_colour = colour;
+    a.first_A = Ac;
+    a.first_x = cx;
+    launch_csr<csr_mode::residual>(a, s);
+}
+
+template <typename T>
 void restrict_children(i64 n_coarse, const u32 *cptr, const u32 *cidx, const T *fine, T *coarse,
                        hipStream_t s) {
     if (n_coarse == 0) return;
@@ -1477,6 +1516,11 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
     template void restrict_children<T>(i64, const u32 *, const u32 *, const T *, T *,           \
                                        hipStream_t);                                            \
     template void gather_sum<T>(const csr_view<T> &, const T *, T *, hipStream_t);              \
+    template void gather_sum_first<T>(const csr_view<T> &, const T *, T *, const csr_view<T> &,  \
+                                      const std::uint8_t *, double, T *, hipStream_t);          \
+    template void level_residual_first<T>(const csr_view<T> &, double, const T *, const T *, T *, \
+                                          const csr_view<T> &, const std::uint8_t *, T *,       \
+                                          hipStream_t);                                         \
     template void prolong_add<T>(i64, const u32 *, const T *, const T *, T *, hipStream_t,       \
                                  const u32 *, const T *);                                         \
     template void terminal_solve<T>(i64, const u32 *, const u32 *, const u64 *, const T *, i64, \

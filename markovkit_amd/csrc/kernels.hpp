@@ -109,6 +109,15 @@ void restrict_children(i64 n_coarse, const u32 *cptr, const u32 *cidx, const T *
 template <typename T>
 void prolong_add(i64 n, const u32 *coarse_of, const T *weight, const T *coarse, T *x,
                  hipStream_t s, const u32 *rows = nullptr, const T *coarse2 = nullptr);
+// restriction (gather_sum) or level residual with the next red-black level's first half-sweep
+// fused in: besides y (r), cx[i] = y[i] / (shift + exit_i of Ac) where colour[i] == 0, the
+// values rb_first computes from y
+template <typename T>
+void gather_sum_first(const csr_view<T> &R, const T *x, T *y, const csr_view<T> &Ac,
+                      const std::uint8_t *colour, double shift, T *cx, hipStream_t s);
+template <typename T>
+void level_residual_first(const csr_view<T> &A, double shift, const T *rhs, const T *x, T *r,
+                          const csr_view<T> &Ac, const std::uint8_t *colour, T *cx, hipStream_t s);
 // dense terminal solves: x[start_b + i] += sum_j inv_b[i][j] * r[start_b + j] (assign: x[...] =);
 // a thread per row, or a wave per row (lanes over columns) when a block exceeds kWideTerminal
 constexpr i64 kWideTerminal = 128;

@@ -447,6 +447,16 @@ inline std::atomic<int> &tail_lds_switch() {
     return on;
 }
 
+// the next red-black level's first half-sweep fused into the kernel that makes its right-hand
+// side (default on; MK_FUSED_FIRST=0 turns it off); the same values either way
+inline std::atomic<int> &fused_first_switch() {
+    static std::atomic<int> on{[] {
+        const char *env = std::getenv("MK_FUSED_FIRST");
+        return (env && env[0] == '0') ? 0 : 1;
+    }()};
+    return on;
+}
+
 inline std::atomic<int> &tail_switch() {
     static std::atomic<int> on{[] {
         const char *env = std::getenv("MK_MG_TAIL");

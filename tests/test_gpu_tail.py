@@ -5,7 +5,8 @@ Jacobi and red-black Gauss-Seidel smoothers, on 1D and 2D boxes, for a whole cyc
 (969 states) and for a tail below larger levels (58,905 states), with V-cycles, with
 W-cycles on the top levels over dense bottom blocks (the bench's hierarchy shape, whose
 bottom level is solved directly) and with three coarse cycles per visit; and the LDS image
-after a shift change."""
+after a shift change. The same for a red-black level's first half-sweep fused into the
+restriction or the coarse-cycle residual against its own launch."""
 import numpy as np
 import pytest
 
@@ -75,3 +76,41 @@ def test_tail_image_follows_set_shift(precision):
     err = np.abs(z - z_ref).max() / np.abs(z_ref).max()
     assert err <= (1e-10 if precision == mk.FP64 else 1e-5), err
     assert np.abs(z_old - z_ref).max() / np.abs(z_ref).max() > 1e-2
+
+
+FUSED_SHAPES = {"V": {}, "W2-c800": dict(pre_sweeps=1, post_sweeps=2, coarse_cycles=2, cycle_levels=2,
+                                         coarsest_size=800),
+                "C3-L2": dict(pre_sweeps=2, post_sweeps=1, coarse_cycles=3, cycle_levels=2)}
+
+
+@pytest.mark.parametrize("shape", list(FUSED_SHAPES))
+@pytest.mark.parametrize("precision", [mk.FP32, mk.FP64], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("wl", [annihilation_2d(4, (2, 2), 1.0), birth_death_1d(32, 4, 0.7, "random")],
+                         ids=["2d", "58905"])
+def test_fused_first_half_sweep_is_bitwise(wl, precision, shape):
+    """The fused first half-sweep (restriction or residual kernel) against its own rb_first
+    launch, with and without the fused tail: cycle and solve bitwise equal"""
+    box = mk.PopulationBox(wl.n_voxels, wl.max_totals)
+    Q = mk.Generator(box, wl.model)
+    s = wl.shifts[0]
+    M = mk.SectorMultigrid(wl.model.domain, box, s,
+                           mk.MultigridOptions(precision=precision, smoother=mk.RED_BLACK,
+                                               **FUSED_SHAPES[shape]))
+    b = np.random.default_rng(5).random(box.size())
+    try:
+        for tail in (True, False):
+            mk.fused_tail_enable(tail)
+            out = {}
+            for fused in (True, False):
+                mk.fused_first_enable(fused)
+                z = M.apply(b)
+                x = np.zeros(box.size())
+                r = mk.gmres(mk.ShiftedGenerator(Q, s), M, b, x, tolerance=1e-10 * np.linalg.norm(b))
+                assert r.converged
+                out[fused] = (z, x, r.iterations)
+            assert np.array_equal(out[True][0], out[False][0]), tail
+            assert np.array_equal(out[True][1], out[False][1]), tail
+            assert out[True][2] == out[False][2]
+    finally:
+        mk.fused_first_enable(True)
+        mk.fused_tail_enable(True)

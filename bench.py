@@ -572,6 +572,81 @@ def c3(mk, workloads, steps, restart):
     return out
 
 
+def c1(mk, workloads, restart, systems=256, single_steps=200):
+    """The reference's CPU-runnable case: 1D, 16 voxels, cap 3 (969 states), s = 0.5. A single
+    solve of (0.5 I - Q) x = e_0 (mixed and plain FP64, red-black sector multigrid, FGMRES) is
+    bound by the latency of its ~200 dependent phases on the GPU, so the case is also run as
+    one batch of `systems` systems of the same shift: e_0 and random L1-normalised right-hand
+    sides (seeds 1..systems-1), each to 1e-10 relative residual. The CPU oracle (the reference's
+    method: weighted-Jacobi V-cycle, GMRES with MGS; one core) solves the same systems one by
+    one, timed beside it."""
+    from oracle.oracle import Problem, SectorMultigrid as OracleMG
+    w = workloads.config_small()
+    shift = w.shifts[0]
+    box = mk.PopulationBox(w.n_voxels, w.max_totals)
+    Q = mk.Generator(box, w.model)
+    n = box.size()
+    A = mk.ShiftedGenerator(Q, shift)
+    b = w.rhs(0, n)
+    tol = 1e-10 * float(np.linalg.norm(b))
+    b_dev, x_dev = mk.DeviceVector(n, b), mk.DeviceVector(n)
+    out = {"workload": f"{w.name}: {n} states, s={shift}; single solve of b=e_0, and a batch of "
+                       f"{systems} systems (e_0 and random L1-normalised rhs, seeds 1..{systems - 1}); "
+                       f"FGMRES({restart}) + red-black sector multigrid (2+2), tol 1e-10 relative"}
+    for name, prec in (("mixed", mk.FP32), ("fp64", mk.FP64)):
+        M = mk.SectorMultigrid(w.model.domain, box, shift,
+                               mk.MultigridOptions(precision=prec, smoother=mk.RED_BLACK))
+        run_solves(mk, A, M, b_dev, x_dev, tol, 100, restart)
+        total, lat, iters, _, _ = run_solves(mk, A, M, b_dev, x_dev, tol, single_steps, restart)
+        out[f"single_{name}"] = {"value": single_steps / total, "p50_ms": 1e3 * percentile(lat, 50),
+                                 "iterations": iters[-1]}
+        del M
+    B = np.empty((systems, n))
+    B[0] = b
+    for j in range(1, systems):
+        r = np.random.default_rng(j).random(n)
+        B[j] = r / r.sum()
+    tols = 1e-10 * np.linalg.norm(B, axis=1)
+    M = mk.SectorMultigrid(w.model.domain, box, shift,
+                           mk.MultigridOptions(precision=mk.FP32, smoother=mk.RED_BLACK))
+    batch = mk.BatchSolver(Q, M, [shift] * systems)
+    batch.solve(B, tolerances=tols, restart=restart)  # warm-up
+    mk.synchronize()
+    steps = 5
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        X, res = batch.solve(B, tolerances=tols, restart=restart)
+    total = time.perf_counter() - t0
+    if not all(r.converged for r in res):
+        raise RuntimeError("C1 batch: a system did not converge")
+    worst = max(float(np.linalg.norm(B[j] - Q.apply_shifted(shift, X[j])) / np.linalg.norm(B[j]))
+                for j in range(systems))
+    its = [r.iterations for r in res]
+    out["batched"] = {"value": systems * steps / total, "batch_ms": 1e3 * total / steps,
+                      "iterations_min": min(its), "iterations_max": max(its),
+                      "worst_relative_residual": worst}
+    # the CPU oracle on the same systems, one core
+    P = Problem(w.model, w.max_totals)
+    Mo = OracleMG(P, shift)
+    P.gmres(shift, b, mg=Mo, tol=tol, restart=restart)  # warm-up
+    l1 = 0.0
+    t0 = time.perf_counter()
+    for j in range(systems):
+        xo, info = P.gmres(shift, B[j], mg=Mo, tol=float(tols[j]), restart=restart)
+        if not info["converged"]:
+            raise RuntimeError("C1: an oracle solve did not converge")
+        l1 = max(l1, float(np.abs(xo - X[j]).sum()))
+    cpu = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": systems / cpu, "unit": "solves/s", "cores": 1, "kind": "port",
+                           "sample": f"the {systems} systems of the batch solved one by one by the "
+                                     f"FP64 CPU oracle (weighted-Jacobi V-cycle 2+2, GMRES({restart}) "
+                                     f"with MGS), one core"}
+    out["max_l1_batch_vs_oracle"] = l1
+    out["batched_over_cpu"] = out["batched"]["value"] / out["cpu_baseline"]["value"]
+    out["single_mixed_over_cpu"] = out["single_mixed"]["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -616,6 +691,8 @@ def main():
                     help="skip the serving mix after the headline")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the 2D two-species annihilation solves after the headline")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the 969-state single, batched and CPU-oracle solves after the headline")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the HBM-counter passes (rocprofv3 --pmc) of the roofline traffic")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -862,6 +939,12 @@ def main():
         log("[c3] 2D two-species annihilation, 4,601,025 states")
         out["c3"] = c3(mk, workloads, max(10, args.steps), args.restart)
         log(f"[c3] {out['c3']['value']:.1f} solves/s mixed, {out['c3']['fp64']['value']:.1f} FP64")
+    if args.workload == "large" and not args.no_c1:
+        log("[c1] 1D 16 voxels cap 3, 969 states: single solves, a 256-system batch, the CPU oracle")
+        out["c1"] = c1(mk, workloads, args.restart)
+        log(f"[c1] single {out['c1']['single_mixed']['value']:.0f} solves/s, batched "
+            f"{out['c1']['batched']['value']:.0f} solves/s, CPU oracle "
+            f"{out['c1']['cpu_baseline']['value']:.0f} solves/s")
     if args.workload == "large" and not args.no_serving:
         log("[serving] serving mix")
         out["serving"] = serving(mk, workloads, args.requests, args.workers, smoother_name=args.smoother)

@@ -134,21 +134,44 @@ __device__ __forceinline__ double finish(const csr_args<VT, XT, T> &a, u32 r, T 
 // coded sliced operator: one wavefront per 64-row slice, four slices per workgroup
 constexpr int kSliceUnroll = 16;
 
-// EX: exit rates per row (a.A.exit) instead of the exit dictionary
+// EX: exit rates per row (a.A.exit) instead of the exit dictionary.
+// A slice's bounds are wave-uniform (scalar loads), and its entries are stored for all 64 lanes
+// up to the slice's width (its longest row), so the entry loads depend on the slice bounds only:
+// the first chunk is in flight together with the rows' lengths and codes and the dictionary
+// staging, and only the x gathers wait for the row lengths. Each row still sums its entries in
+// storage order.
 template <csr_mode M, typename VT, typename XT, typename T, bool EX>
 __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) {
     __shared__ T vdict[kDictMax];
     __shared__ T edict[kDictMax];
     __shared__ double red64[kWaves];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const u32 b = xcd_block(blockIdx.x, a.chunk);
     if (b >= a.A.n_blocks) return;
-    for (int i = tid; i < a.A.n_vdict; i += kThreads) vdict[i] = static_cast<T>(a.A.vdict[i]);
-    for (int i = tid; i < a.A.n_edict; i += kThreads) edict[i] = static_cast<T>(a.A.edict[i]);
+    static_assert(kDictMax <= kThreads, "one dictionary entry per thread");
+    // dictionary values first: the stores into LDS then wait for these loads only
+    const T vd = tid < a.A.n_vdict ? static_cast<T>(a.A.vdict[tid]) : T(0);
+    const T ed = tid < a.A.n_edict ? static_cast<T>(a.A.edict[tid]) : T(0);
     const i64 slice = static_cast<i64>(b) * kWaves + wave;
+    u32 base = 0, width = 0;
+    if (slice < a.A.n_slices) {
+        base = a.A.soff[slice];
+        width = (a.A.soff[slice + 1] - base) / kSlice;
+    }
+    const u32 *ent = a.A.ent + base + lane;
+    // a chunk loads all its slots, clamped to the slice's width: branch-free, so the loads of a
+    // chunk are in flight together (the compiler waits at every join of per-slot branches)
+    u32 e[kSliceUnroll];
+    auto load_entries = [&](u32 k0) {
+#pragma unroll
+        for (int u = 0; u < kSliceUnroll; ++u)
+            e[u] = __builtin_nontemporal_load(ent + min(k0 + u, width - 1) * kSlice);
+    };
+    if (width > 0) load_entries(0);
     const i64 r = slice * kSlice + lane;
     const bool has_row = slice < a.A.n_slices && r < a.A.n;
-    u32 len = 0, ec = 0, base = 0;
+    u32 len = 0, ec = 0;
     T ex = 0;
     if (has_row) {
         len = a.A.rlen[r];
@@ -157,45 +180,39 @@ __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) 
         } else {
             ec = a.A.rcode[r];
         }
-        base = a.A.soff[slice] + lane;
     }
-    __syncthreads();
-    T sum = 0;
-    if constexpr (std::is_same_v<XT, T>) {
-        for (u32 k0 = 0; k0 < len; k0 += kSliceUnroll) {
-            u32 e[kSliceUnroll];
+    // gathers: guarded per lane in the working precision; for x in another precision the
+    // compiler would convert each gathered value inside its guard and wait for it there, so
+    // there every slot of the chunk loads (a slot past the row's end reads x[0], in bounds
+    // whenever the slice has an entry) and only the sum leaves it out
+    XT xv[kSliceUnroll];
+    auto gather = [&](u32 k0) {
 #pragma unroll
-            for (int u = 0; u < kSliceUnroll; ++u)
-                e[u] = k0 + u < len ? __builtin_nontemporal_load(a.A.ent + base + (k0 + u) * kSlice) : 0u;
-            XT xv[kSliceUnroll];
-#pragma unroll
-            for (int u = 0; u < kSliceUnroll; ++u)
+        for (int u = 0; u < kSliceUnroll; ++u) {
+            if constexpr (std::is_same_v<XT, T>) {
                 xv[u] = k0 + u < len ? a.x[e[u] & kColMask] : XT(0);
-#pragma unroll
-            for (int u = 0; u < kSliceUnroll; ++u)
-                if (k0 + u < len) sum += vdict[e[u] >> kColBits] * static_cast<T>(xv[u]);
-        }
-    } else {
-        // x in another precision: with guarded loads the compiler converts each gathered value
-        // inside its guard and waits for it there, one gather at a time. Branch-free instead:
-        // slots past the row's end re-read its last entry (in bounds, same cache line) and get a
-        // zero coefficient, so a chunk's gathers are in flight together; fma(0, x, sum) == sum.
-        for (u32 k0 = 0; k0 < len; k0 += kSliceUnroll) {
-            u32 e[kSliceUnroll];
-#pragma unroll
-            for (int u = 0; u < kSliceUnroll; ++u) {
-                const u32 k = min(k0 + u, len - 1);
-                e[u] = __builtin_nontemporal_load(a.A.ent + base + k * kSlice);
-            }
-            XT xv[kSliceUnroll];
-#pragma unroll
-            for (int u = 0; u < kSliceUnroll; ++u) xv[u] = a.x[e[u] & kColMask];
-#pragma unroll
-            for (int u = 0; u < kSliceUnroll; ++u) {
-                const T c = k0 + u < len ? vdict[e[u] >> kColBits] : T(0);
-                sum += c * static_cast<T>(xv[u]);
+            } else {
+                xv[u] = a.x[k0 + u < len ? e[u] & kColMask : 0u];
             }
         }
+    };
+    T sum = 0;
+    auto accumulate = [&](u32 k0) {
+#pragma unroll
+        for (int u = 0; u < kSliceUnroll; ++u)
+            if (k0 + u < len) sum += vdict[e[u] >> kColBits] * static_cast<T>(xv[u]);
+    };
+    if (width > 0) gather(0);
+    // the dictionaries reach LDS after the first gathers are issued (their stores wait for every
+    // load before them)
+    if (tid < a.A.n_vdict) vdict[tid] = vd;
+    if (tid < a.A.n_edict) edict[tid] = ed;
+    __syncthreads();
+    if (width > 0) accumulate(0);
+    for (u32 k0 = kSliceUnroll; k0 < width; k0 += kSliceUnroll) {
+        load_entries(k0);
+        gather(k0);
+        accumulate(k0);
     }
     double sq = 0.0;
     if (has_row) {

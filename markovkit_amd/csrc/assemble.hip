@@ -841,6 +841,7 @@ bool assemble_generator_device(const mk::population_space &space, const mk::voxe
     d.view.edict = dp<double>(d.edict);
     d.view.n_vdict = static_cast<int>(vdict.size());
     d.view.n_edict = static_cast<int>(edict.size());
+    d.view.max_width = static_cast<int>(maxima[0]);
     d.view.coded = 1;
     check_view(d.view, d.nnz);
     return true;

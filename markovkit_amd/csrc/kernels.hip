@@ -91,8 +91,34 @@ __device__ __forceinline__ void first_half(const csr_args<VT, XT, T> &a, u32 r, 
     }
 }
 
+// the operands of vector entry r that the row's result needs besides its entries: the own x
+// value (every mode but gs and gather_sum), the right-hand side (the modes that subtract from
+// one) and the natural-order index of a gs row that is also stored there. The sliced kernel
+// loads them with the row lengths, so their latency overlaps the entry loads and the gathers.
+template <typename T>
+struct row_operands {
+    T x = 0;
+    T rhs = 0;
+    u32 perm = 0;
+};
+
 template <csr_mode M, typename VT, typename XT, typename T>
-__device__ __forceinline__ double finish(const csr_args<VT, XT, T> &a, u32 r, T sum, T exit_r) {
+__device__ __forceinline__ row_operands<T> load_row_operands(const csr_args<VT, XT, T> &a, u32 r) {
+    row_operands<T> o;
+    if constexpr (M != csr_mode::gs && M != csr_mode::gather_sum) o.x = static_cast<T>(a.x[r]);
+    if constexpr (M == csr_mode::gs || M == csr_mode::shifted_residual || M == csr_mode::sweep ||
+                  M == csr_mode::residual) {
+        o.rhs = a.rhs[r];
+    }
+    if constexpr (M == csr_mode::gs) {
+        if (a.nat) o.perm = a.perm[r];
+    }
+    return o;
+}
+
+template <csr_mode M, typename VT, typename XT, typename T>
+__device__ __forceinline__ double finish(const csr_args<VT, XT, T> &a, u32 r, T sum, T exit_r,
+                                         const row_operands<T> &o) {
     if constexpr (M == csr_mode::gather_sum) {
         a.out[r] = sum;
         first_half(a, r, sum);
@@ -100,39 +126,37 @@ __device__ __forceinline__ double finish(const csr_args<VT, XT, T> &a, u32 r, T 
     }
     if constexpr (M == csr_mode::gs) {
         // in place: the row's entries all have the other colour
-        const T v = (a.rhs[r] + sum) / (static_cast<T>(a.shift) + exit_r);
+        const T v = (o.rhs + sum) / (static_cast<T>(a.shift) + exit_r);
         a.out[r] = v;
         if (a.aux) a.aux[r] = T(0);
-        if (a.nat) a.nat[a.perm[r]] = v;
+        if (a.nat) a.nat[o.perm] = v;
         return 0.0;
     }
-    const T xr = static_cast<T>(a.x[r]);
+    const T xr = o.x;
     const T d = static_cast<T>(a.shift) + exit_r;
     const T ax = d * xr - sum;
     if constexpr (M == csr_mode::shifted) {
         a.out[r] = ax;
         return 0.0;
     } else if constexpr (M == csr_mode::shifted_residual) {
-        const T res = a.rhs[r] - ax;
+        const T res = o.rhs - ax;
         a.out[r] = res;
         return static_cast<double>(res) * static_cast<double>(res);
     } else if constexpr (M == csr_mode::sweep) {
-        const T res = a.rhs[r] - ax;
+        const T res = o.rhs - ax;
         a.out[r] = xr + a.omega * res / d;
         return 0.0;
-    } else if constexpr (M == csr_mode::gather_sum) {
-        a.out[r] = sum;
-        return 0.0;
     } else {
-        const T res = a.rhs[r] - ax;
+        const T res = o.rhs - ax;
         a.out[r] = res;
         first_half(a, r, res);
         return 0.0;
     }
 }
 
-// coded sliced operator: one wavefront per 64-row slice, four slices per workgroup
-constexpr int kSliceUnroll = 16;
+// coded sliced operator: one wavefront per 64-row slice, four slices per workgroup. A chunk is U
+// entries per lane: 16 in general, 12 or 8 when every row of the operator fits one such chunk
+// (a chunk loads all its slots, so a narrower one issues fewer redundant loads).
 
 // EX: exit rates per row (a.A.exit) instead of the exit dictionary.
 // A slice's bounds are wave-uniform (scalar loads), and its entries are stored for all 64 lanes
@@ -140,7 +164,7 @@ constexpr int kSliceUnroll = 16;
 // the first chunk is in flight together with the rows' lengths and codes and the dictionary
 // staging, and only the x gathers wait for the row lengths. Each row still sums its entries in
 // storage order.
-template <csr_mode M, typename VT, typename XT, typename T, bool EX>
+template <csr_mode M, typename VT, typename XT, typename T, bool EX, int U, bool MULTI>
 __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) {
     __shared__ T vdict[kDictMax];
     __shared__ T edict[kDictMax];
@@ -162,17 +186,18 @@ __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) 
     const u32 *ent = a.A.ent + base + lane;
     // a chunk loads all its slots, clamped to the slice's width: branch-free, so the loads of a
     // chunk are in flight together (the compiler waits at every join of per-slot branches)
-    u32 e[kSliceUnroll];
+    u32 e[U];
     auto load_entries = [&](u32 k0) {
 #pragma unroll
-        for (int u = 0; u < kSliceUnroll; ++u)
+        for (int u = 0; u < U; ++u)
             e[u] = __builtin_nontemporal_load(ent + min(k0 + u, width - 1) * kSlice);
     };
     if (width > 0) load_entries(0);
     const i64 r = slice * kSlice + lane;
     const bool has_row = slice < a.A.n_slices && r < a.A.n;
-    u32 len = 0, ec = 0;
+    u32 len = 0, ec = 0, vr = 0;
     T ex = 0;
+    row_operands<T> own;
     if (has_row) {
         len = a.A.rlen[r];
         if constexpr (EX) {
@@ -180,15 +205,17 @@ __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) 
         } else {
             ec = a.A.rcode[r];
         }
+        vr = vec_row(a.A, static_cast<u32>(r));
+        own = load_row_operands<M>(a, vr);
     }
     // gathers: guarded per lane in the working precision; for x in another precision the
     // compiler would convert each gathered value inside its guard and wait for it there, so
     // there every slot of the chunk loads (a slot past the row's end reads x[0], in bounds
     // whenever the slice has an entry) and only the sum leaves it out
-    XT xv[kSliceUnroll];
+    XT xv[U];
     auto gather = [&](u32 k0) {
 #pragma unroll
-        for (int u = 0; u < kSliceUnroll; ++u) {
+        for (int u = 0; u < U; ++u) {
             if constexpr (std::is_same_v<XT, T>) {
                 xv[u] = k0 + u < len ? a.x[e[u] & kColMask] : XT(0);
             } else {
@@ -196,30 +223,58 @@ __global__ void __launch_bounds__(kThreads) k_csr_sliced(csr_args<VT, XT, T> a) 
             }
         }
     };
+    // MULTI (for rows longer than one chunk): a chunk's rate codes are kept four to a register,
+    // so once its gathers are issued the entry registers take the next chunk's loads, which are
+    // then in flight while this chunk's gathers return. Without MULTI the chunks run one after
+    // another (fewer registers).
+    constexpr int kCodeWords = MULTI ? (U + 3) / 4 : 1;
+    u32 codes[kCodeWords];
+    auto keep_codes = [&]() {
+#pragma unroll
+        for (int q = 0; q < kCodeWords; ++q) codes[q] = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) codes[u / 4] |= (e[u] >> kColBits) << (8 * (u % 4));
+    };
+    auto code = [&](int u) -> u32 {
+        if constexpr (MULTI) return (codes[u / 4] >> (8 * (u % 4))) & 0xFFu;
+        return e[u] >> kColBits;
+    };
     T sum = 0;
     auto accumulate = [&](u32 k0) {
 #pragma unroll
-        for (int u = 0; u < kSliceUnroll; ++u)
-            if (k0 + u < len) sum += vdict[e[u] >> kColBits] * static_cast<T>(xv[u]);
+        for (int u = 0; u < U; ++u)
+            if (k0 + u < len) sum += vdict[code(u)] * static_cast<T>(xv[u]);
     };
-    if (width > 0) gather(0);
+    if (width > 0) {
+        gather(0);
+        if constexpr (MULTI) {
+            keep_codes();
+            if (static_cast<u32>(U) < width) load_entries(U);
+        }
+    }
     // the dictionaries reach LDS after the first gathers are issued (their stores wait for every
     // load before them)
     if (tid < a.A.n_vdict) vdict[tid] = vd;
     if (tid < a.A.n_edict) edict[tid] = ed;
     __syncthreads();
     if (width > 0) accumulate(0);
-    for (u32 k0 = kSliceUnroll; k0 < width; k0 += kSliceUnroll) {
-        load_entries(k0);
-        gather(k0);
+    for (u32 k0 = U; k0 < width; k0 += U) {
+        if constexpr (MULTI) {
+            gather(k0);
+            keep_codes();
+            if (k0 + U < width) load_entries(k0 + U);
+        } else {
+            load_entries(k0);
+            gather(k0);
+        }
         accumulate(k0);
     }
     double sq = 0.0;
     if (has_row) {
         if constexpr (EX) {
-            sq = finish<M>(a, vec_row(a.A, static_cast<u32>(r)), sum, ex);
+            sq = finish<M>(a, vr, sum, ex, own);
         } else {
-            sq = finish<M>(a, vec_row(a.A, static_cast<u32>(r)), sum, edict[ec]);
+            sq = finish<M>(a, vr, sum, edict[ec], own);
         }
     }
     if constexpr (M == csr_mode::shifted_residual) {
@@ -267,14 +322,18 @@ __global__ void __launch_bounds__(kThreads) k_csr_plain(csr_args<VT, XT, T> a) {
             T sum = 0;
             const u32 q1 = a.A.ptr[r + 1];
             for (u32 q = a.A.ptr[r]; q < q1; ++q) sum += prod[q - p0];
-            sq = finish<M>(a, vec_row(a.A, r), sum, static_cast<T>(a.A.exit[r]));
+            const u32 vr = vec_row(a.A, r);
+            sq = finish<M>(a, vr, sum, static_cast<T>(a.A.exit[r]), load_row_operands<M>(a, vr));
         }
     } else {
         T sum = 0;
         for (u32 p = p0 + tid; p < p1; p += kThreads)
             sum += static_cast<T>(a.A.val[p]) * static_cast<T>(a.x[a.A.col[p]]);
         sum = block_sum(sum, red);
-        if (tid == 0) sq = finish<M>(a, vec_row(a.A, r0), sum, static_cast<T>(a.A.exit[r0]));
+        if (tid == 0) {
+            const u32 vr = vec_row(a.A, r0);
+            sq = finish<M>(a, vr, sum, static_cast<T>(a.A.exit[r0]), load_row_operands<M>(a, vr));
+        }
     }
     if constexpr (M == csr_mode::shifted_residual) {
         const double t = block_sum(sq, red64);
@@ -287,10 +346,27 @@ void launch_csr(csr_args<VT, XT, T> a, hipStream_t s) {
     if (a.A.n_blocks == 0) return;
     a.chunk = static_cast<u32>(g_xcd_chunk);
     const dim3 grid(padded_grid(a.A.n_blocks, a.chunk));
-    if (a.A.coded && a.A.exit) {
-        hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, true>), grid, dim3(kThreads), 0, s, a);
-    } else if (a.A.coded) {
-        hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, false>), grid, dim3(kThreads), 0, s, a);
+    if (a.A.coded) {
+        const int w = a.A.max_width;
+        const bool ex = a.A.exit != nullptr;
+        const dim3 block(kThreads);
+        if (w > 0 && w <= 8) {
+            if (ex) hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, true, 8, false>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, false, 8, false>), grid, block, 0, s, a);
+        } else if (w > 0 && w <= 12) {
+            if (ex) hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, true, 12, false>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, false, 12, false>), grid, block, 0, s, a);
+        } else if (w > 0 && w <= 16) {
+            if (ex) hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, true, 16, false>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, false, 16, false>), grid, block, 0, s, a);
+        } else if constexpr (M == csr_mode::gather_sum) {
+            // (the restriction's wide rows gain nothing from the pipelined chunks)
+            if (ex) hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, true, 16, false>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, false, 16, false>), grid, block, 0, s, a);
+        } else {
+            if (ex) hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, true, 16, true>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((k_csr_sliced<M, VT, XT, T, false, 16, true>), grid, block, 0, s, a);
+        }
     } else {
         hipLaunchKernelGGL((k_csr_plain<M, VT, XT, T>), grid, dim3(kThreads), 0, s, a);
     }

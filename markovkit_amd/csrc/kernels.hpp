@@ -70,6 +70,7 @@ struct csr_view {
     const VT *edict = nullptr;
     int n_vdict = 0;
     int n_edict = 0;
+    int max_width = 0;           // longest row (0: unknown); picks the sliced kernel's chunk
     // a row subset (one colour of a red-black split): local row r is vector entry rowmap[r];
     // null for the identity
     const u32 *rowmap = nullptr;

@@ -305,7 +305,6 @@ dev_csr<VT> upload_csr(const mk::generator_csr &g, bool exit_array = false) {
 
     static_assert(mk::kSliceRows == gpu::kSlice && mk::kCodedColumnBits == static_cast<int>(gpu::kColBits),
                   "host and device agree on the coded layout");
-    (void)max_len;
     mk::sliced_operator<VT> so;
     if (mk::encode_sliced<VT>(g, so, exit_array)) {
         const i64 n_slices = static_cast<i64>(so.soff.size()) - 1;
@@ -327,6 +326,7 @@ dev_csr<VT> upload_csr(const mk::generator_csr &g, bool exit_array = false) {
         d.view.edict = dp<VT>(d.edict);
         d.view.n_vdict = static_cast<int>(so.vdict.size());
         d.view.n_edict = static_cast<int>(so.edict.size());
+        d.view.max_width = static_cast<int>(max_len);
         if (!so.exit.empty()) {
             d.exit = upload(so.exit);
             d.view.exit = dp<VT>(d.exit);

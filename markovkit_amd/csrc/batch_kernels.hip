@@ -12,6 +12,10 @@ constexpr int kWaves = kThreads / kLanes;
 constexpr int kEntryUnroll = 8;
 constexpr int kRowsPerPart = 256;
 constexpr int kMaxParts = 512;
+// parts_wide: row partitions times lane groups the reductions of a small operand aim for (the
+// workgroups of one launch, so that they cover the CUs), and its upper bound
+constexpr int kWideGroups = 1024;
+constexpr int kMaxWideParts = 2048;
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
@@ -35,6 +39,7 @@ struct bop_args {
                             // pre-smoothing half-sweep)
     T *nat = nullptr;       // gs: rows also stored to nat[perm[row]] (the caller's order)
     const u32 *perm = nullptr;
+    const double *scale = nullptr; // shifted: the product of lane j times scale[j]
 };
 
 // sum over the row's entries of rate * x[column][j], ascending entry order; row r is
@@ -113,7 +118,7 @@ __global__ void __launch_bounds__(kThreads) k_bop(bop_args<VT, XT, T> a) {
                 const T d = static_cast<T>(a.shifts[j]) + ex;
                 const T ax = d * xr - sum;
                 if constexpr (M == bmode::shifted) {
-                    a.out[o] = ax;
+                    a.out[o] = a.scale ? ax * static_cast<T>(a.scale[j]) : ax;
                 } else if constexpr (M == bmode::shifted_residual) {
                     const T res = a.rhs[o] - ax;
                     a.out[o] = res;
@@ -613,7 +618,67 @@ __global__ void __launch_bounds__(kThreads)
     for (int k = 0; k < count; ++k) X[k * ld + r * Bp + j] = T(0);
 }
 
+// the delayed-CGS2 candidate's test per lane: s = sn[0 .. count], |u|^2 = sn[count + 1]; the
+// Pythagorean norm |u|^2 - |s|^2 is kept unless |s|^2 > |u|^2 / 4 (flag: the candidate gets an
+// explicit second projection); lflag[j] = the lane's limit where flagged, else 0
+__global__ void __launch_bounds__(kThreads)
+    k_dcgs_flags(const double *sn, int count, i64 Bp, const int *limit, int *lflag, double *flag) {
+    const i64 j = static_cast<i64>(blockIdx.x) * kThreads + threadIdx.x;
+    if (j >= Bp) return;
+    const int lim = limit ? limit[j] : 1;
+    bool f = false;
+    if (lim > 0) {
+        double ss = 0.0;
+        for (int k = 0; k <= count; ++k) ss += sn[static_cast<i64>(k) * Bp + j] * sn[static_cast<i64>(k) * Bp + j];
+        f = !(ss <= 0.25 * sn[static_cast<i64>(count + 1) * Bp + j]);
+    }
+    lflag[j] = f ? lim : 0;
+    flag[j] = f ? 1.0 : 0.0;
+}
+
+// the next normalisation per lane from the first (sn1) or, where flagged (flag not null), the
+// second (sn2) projections: beta = sqrt(max(|u|^2 - |s|^2, 0)); s is copied to scur for the next
+// update; beta_out = beta, dbeta = beta (infinity when zero: the update then finalises a zero
+// vector), scale = 1 / beta (0 when zero), beta2 = beta^2; flag_out (when given) records the
+// test of k_dcgs_flags on sn1
+__global__ void __launch_bounds__(kThreads)
+    k_dcgs_norms(const double *sn1, const double *sn2, const double *flag, int count, i64 Bp,
+                 const int *limit, double *scur, double *beta_out, double *dbeta, double *scale,
+                 double *beta2, double *flag_out) {
+    const i64 j = static_cast<i64>(blockIdx.x) * kThreads + threadIdx.x;
+    if (j >= Bp) return;
+    const bool on = !limit || limit[j] > 0;
+    const double *sn = flag && flag[j] != 0.0 ? sn2 : sn1;
+    double ss = 0.0;
+    for (int k = 0; k <= count; ++k) {
+        const double v = on ? sn[static_cast<i64>(k) * Bp + j] : 0.0;
+        scur[static_cast<i64>(k) * Bp + j] = v;
+        ss += v * v;
+    }
+    const double u2 = on ? sn[static_cast<i64>(count + 1) * Bp + j] : 0.0;
+    if (flag_out) flag_out[j] = on && !(ss <= 0.25 * u2) ? 1.0 : 0.0;
+    const double b = std::sqrt(std::max(u2 - ss, 0.0));
+    beta_out[j] = b;
+    dbeta[j] = b > 0.0 ? b : __builtin_inf();
+    scale[j] = b > 0.0 ? 1.0 / b : 0.0;
+    beta2[j] = b * b;
+}
+
 } // namespace
+
+void dcgs_flags(const double *sn, int count, i64 Bp, const int *limit, int *lflag, double *flag,
+                hipStream_t s) {
+    hipLaunchKernelGGL(k_dcgs_flags, dim3(static_cast<u32>((Bp + kThreads - 1) / kThreads)), dim3(kThreads),
+                       0, s, sn, count, Bp, limit, lflag, flag);
+}
+
+void dcgs_norms(const double *sn1, const double *sn2, const double *flag, int count, i64 Bp,
+                const int *limit, double *scur, double *beta_out, double *dbeta, double *scale,
+                double *beta2, hipStream_t s, double *flag_out) {
+    hipLaunchKernelGGL(k_dcgs_norms, dim3(static_cast<u32>((Bp + kThreads - 1) / kThreads)), dim3(kThreads),
+                       0, s, sn1, sn2, flag, count, Bp, limit, scur, beta_out, dbeta, scale, beta2,
+                       flag_out);
+}
 
 void put_lane(const double *src, i64 rows, double *dst, i64 n, i64 Bp, i64 j, hipStream_t s) {
     if (n == 0) return;
@@ -664,12 +729,21 @@ template void block_apply<float>(double *, const float *, int, const double *, i
 
 int parts(i64 n) { return op_parts(n); }
 
+int parts_wide(i64 n, i64 Bp) {
+    const i64 groups = std::max<i64>(1, Bp / kLanes);
+    const i64 want = (kWideGroups + groups - 1) / groups;
+    const i64 cap = std::max<i64>(1, (n + kWaves * 4 - 1) / (kWaves * 4)); // at least 4 rows per wave
+    return static_cast<int>(std::max<i64>(op_parts(n), std::min<i64>({want, cap, kMaxWideParts})));
+}
+
 i64 residual_parts(i64 n) { return row_blocks(n); }
 
 template <typename XT>
 void shifted_apply(const csr_view<double> &A, const double *shifts, const XT *x, const double *b,
-                   double *y, double *partial, i64 Bp, const int *limit, hipStream_t s) {
-    const bop_args<double, XT, double> a{A, shifts, 0.0, x, b, y, partial, Bp, limit};
+                   double *y, double *partial, i64 Bp, const int *limit, hipStream_t s,
+                   const double *scale) {
+    bop_args<double, XT, double> a{A, shifts, 0.0, x, b, y, partial, Bp, limit};
+    a.scale = b ? nullptr : scale;
     if (b) {
         launch<bmode::shifted_residual>(a, s);
     } else {
@@ -814,10 +888,10 @@ void unpack(const double *src, i64 n, i64 nsys, double *dst, i64 Bp, hipStream_t
 
 template void shifted_apply<double>(const csr_view<double> &, const double *, const double *,
                                     const double *, double *, double *, i64, const int *,
-                                    hipStream_t);
+                                    hipStream_t, const double *);
 template void shifted_apply<float>(const csr_view<double> &, const double *, const float *,
                                    const double *, double *, double *, i64, const int *,
-                                   hipStream_t);
+                                   hipStream_t, const double *);
 #define MK_BLEVEL(T)                                                                              \
     template void jacobi_first<T, double>(const csr_view<T> &, const double *, double,            \
                                           const double *, const double *, T *, T *, i64,         \

@@ -23,6 +23,9 @@ constexpr int kBlock = 4;  // systems per block of the block Krylov method (one 
 
 // row partitions used by the batched reductions for an n-row operand
 int parts(i64 n);
+// more partitions for a small operand (few lanes or rows): about 1024 workgroups per reduction
+// launch over its Bp / 64 lane groups, at least 16 rows per partition
+int parts_wide(i64 n, i64 Bp);
 
 // partial sums of the residual kernel: one per 4-row block
 i64 residual_parts(i64 n);
@@ -30,9 +33,11 @@ i64 residual_parts(i64 n);
 // y = (s_j + exit) o x - C x                       (mode shifted)
 // y = b - ((s_j + exit) o x - C x), partial sums of y^2 per system (mode shifted_residual):
 //   partial[p * Bp + j] for p < residual_parts(n)
+// (scale, when given and b is null: lane j of y is multiplied by scale[j])
 template <typename XT>
 void shifted_apply(const csr_view<double> &A, const double *shifts, const XT *x, const double *b,
-                   double *y, double *partial, i64 Bp, const int *limit, hipStream_t s);
+                   double *y, double *partial, i64 Bp, const int *limit, hipStream_t s,
+                   const double *scale = nullptr);
 
 // multigrid level operations in precision T (see kernels.hpp for the single-system forms)
 //   first sweep from zero: rhs = (T)(in * scale_j) (stored to rhs_out if given),
@@ -88,6 +93,18 @@ i64 reduce_tmp_size(int np, i64 m);
 void dcgs_update(double *V, int count, const double *sc, const double *tg, const double *beta,
                  const double *w, i64 n, i64 Bp, double *partial, int np, double *eta,
                  const int *limit, hipStream_t s);
+// delayed CGS2 without a host step: per lane, from the candidate's projections sn[0 .. count]
+// and |u|^2 = sn[count + 1], flag[j] = 1 where |s|^2 > |u|^2 / 4 (the candidate needs an explicit
+// second projection; lflag[j] = limit[j] there, else 0)
+void dcgs_flags(const double *sn, int count, i64 Bp, const int *limit, int *lflag, double *flag,
+                hipStream_t s);
+// then the normalisation from sn1, or sn2 (the second projection) where flagged (flag not
+// null): scur = s, beta_out = beta = sqrt(max(|u|^2 - |s|^2, 0)), dbeta = beta (infinity when 0),
+// scale = 1 / beta (0 when 0), beta2 = beta^2; flag_out (when given) receives the flag test on
+// sn1 (a step taken without second projections records where one was due)
+void dcgs_norms(const double *sn1, const double *sn2, const double *flag, int count, i64 Bp,
+                const int *limit, double *scur, double *beta_out, double *dbeta, double *scale,
+                double *beta2, hipStream_t s, double *flag_out = nullptr);
 // w[i][j] -= sum_{k < count} h[k * Bp + j] V_k[i][j]
 void basis_update(double *w, const double *V, int count, const double *h, i64 n, i64 Bp,
                   const int *limit, hipStream_t s);

@@ -276,7 +276,7 @@ struct mk_expv_batch_s {
     mk_generator_s *gen = nullptr;
     i64 n = 0, Bp = 0, slots = 0, m = 0;
     int np = 0;
-    dbuf W, V, p, partial, tmp, hd, d_lim, d_cnt, d_rows, d_sel, d_shift, stage;
+    dbuf W, V, p, partial, tmp, hd, d_lim, d_cnt, d_rows, d_sel, d_shift, stage, d_lflag;
     double *host = nullptr; // pinned mirror of hd
     int *ihost = nullptr;   // pinned: limits | counts | rows | sel
     std::vector<lane> L;
@@ -284,14 +284,27 @@ struct mk_expv_batch_s {
         if (host) (void)hipHostFree(host);
         if (ihost) (void)hipHostFree(ihost);
     }
-    // device scalars, each [.. * Bp + lane]: the projections of both passes (m steps of m + 1
-    // slots), the remainders' squared norms (m + 1), |w|^2, and the coefficients y (m + 1)
+    // device scalars, each [.. * Bp + lane]: per Arnoldi step the projections of both passes
+    // (m steps of m + 1 slots; delayed CGS2: Q^T w and u^T w, then the candidate's projections
+    // and |u|^2), the remainders' squared norms (m + 1; delayed CGS2 uses the last, |B V_m|^2),
+    // |w|^2 and the coefficients y (m + 1); for delayed CGS2 then the explicit second
+    // projections (m steps of m + 2), eta (m), beta (m + 1), the flags (m), the current s (m + 1)
+    // and the next step's scale, beta (for the update) and beta^2
     i64 h1(i64 j) const { return j * (m + 1) * Bp; }
     i64 h2(i64 j) const { return (m * (m + 1) + j * (m + 1)) * Bp; }
     i64 nrm(i64 j) const { return (2 * m * (m + 1) + j) * Bp; }
     i64 beta2() const { return (2 * m * (m + 1) + m + 1) * Bp; }
     i64 y() const { return beta2() + Bp; }
-    i64 total() const { return y() + (m + 1) * Bp; }
+    i64 s2(i64 j) const { return y() + (m + 1) * Bp + j * (m + 2) * Bp; }
+    i64 eta(i64 j) const { return s2(m) + j * Bp; }
+    i64 bet(i64 j) const { return eta(m) + j * Bp; }
+    i64 flg(i64 j) const { return bet(m + 1) + j * Bp; }
+    i64 scur() const { return flg(m); }
+    i64 dscale() const { return scur() + (m + 1) * Bp; }
+    i64 dbeta() const { return dscale() + Bp; }
+    i64 dbeta2() const { return dbeta() + Bp; }
+    i64 readback() const { return scur(); }
+    i64 total() const { return dbeta2() + Bp; }
     double *dv(i64 k) const { return dp<double>(V) + k * n * Bp; }
 };
 
@@ -335,7 +348,7 @@ expv_batch *expv_engine_create(mk_generator_s *gen, i64 slots, i64 krylov_dim) {
     E->W = dbuf(vec);
     E->V = dbuf(vec * (m + 1));
     E->p = dbuf(vec);
-    E->np = bk::parts(n);
+    E->np = bk::parts_wide(n, Bp);
     E->partial = dbuf(static_cast<std::size_t>(E->np) * (m + 1) * Bp * sizeof(double));
     E->tmp = dbuf(static_cast<std::size_t>(std::max<i64>(1, bk::reduce_tmp_size(E->np, (m + 1) * Bp))) *
                   sizeof(double));
@@ -345,6 +358,7 @@ expv_batch *expv_engine_create(mk_generator_s *gen, i64 slots, i64 krylov_dim) {
     E->d_rows = dbuf(Bp * sizeof(int));
     E->d_sel = dbuf(Bp * sizeof(int));
     E->d_shift = dbuf(Bp * sizeof(double));
+    E->d_lflag = dbuf(Bp * sizeof(int));
     E->stage = dbuf(static_cast<std::size_t>(std::max<i64>(n, 1)) * sizeof(double));
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&E->host), E->total() * sizeof(double)));
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&E->ihost), 4 * Bp * sizeof(int)));
@@ -419,33 +433,96 @@ bool expv_engine_step(expv_batch &E) {
     const auto &A = E.gen->A;
     const double nb = double(n) * Bp;
     const double spmv_bytes = op_bytes(A, 0, 0, 0) + nb * 16.0;
-    // the Arnoldi process of every running lane: V_0 = W / beta, then m steps
-    P.run(K_OTHER, nb * 16.0, [&] { bk::scale_lanes(W, hd + E.beta2(), E.dv(0), n, Bp, lim, s); });
-    for (i64 j = 0; j < m; ++j) {
-        const int cnt = static_cast<int>(j + 1);
-        P.run(K_SPMV, spmv_bytes, [&] { bk::shifted_apply<double>(A.view, sh, E.dv(j), nullptr, pv, nullptr, Bp, lim, s); });
-        for (int pass = 0; pass < 2; ++pass) {
-            double *h = hd + (pass == 0 ? E.h1(j) : E.h2(j));
-            P.run(K_DOT, double(cnt + 1) * nb * 8.0, [&] {
-                bk::multi_dot(E.dv(0), cnt, pv, n, Bp, part, E.np, lim, s);
-                bk::reduce_partials(part, E.np, static_cast<i64>(cnt) * Bp, h, s, tmp);
+#ifdef MK_EXPV_CGS2
+    const bool delayed = false;
+#else
+    const bool delayed = m < gpu::kFusedBasis;
+#endif
+    // the Arnoldi process of every running lane: V_0 = W / beta, then m steps; the Hessenberg
+    // coefficients and norms to the host
+    auto arnoldi = [&](bool reproject) {
+        P.run(K_OTHER, nb * 16.0, [&] { bk::scale_lanes(W, hd + E.beta2(), E.dv(0), n, Bp, lim, s); });
+        if (delayed) {
+            // delayed CGS2 (the batched FGMRES's Arnoldi step) with the normalisation on the
+            // device: step j applies B to the candidate u_j / beta_j (u_0 = q_0, beta_0 = 1), one
+            // reduction gives Q^T w and u_j^T w, the update finalises q_j and forms u_{j+1} with
+            // its projections. Where the Pythagorean norm of u_{j+1} would cancel, the candidate
+            // needs an explicit second projection: without reproject the steps only record the
+            // flags (the host then runs the process again with reproject, which launches the
+            // second projection for the flagged lanes)
+            int *lflag = dp<int>(E.d_lflag);
+            gpu::fill<double>(Bp, 1.0, hd + E.dscale(), s);
+            gpu::fill<double>(Bp, 1.0, hd + E.dbeta(), s);
+            for (i64 j = 0; j < m; ++j) {
+                const int cnt = static_cast<int>(j);
+                P.run(K_SPMV, spmv_bytes, [&] {
+                    bk::shifted_apply<double>(A.view, sh, E.dv(j), nullptr, pv, nullptr, Bp, lim, s,
+                                              hd + E.dscale());
+                });
+                P.run(K_DOT, double(cnt + 2) * nb * 8.0, [&] {
+                    bk::multi_dot(E.dv(0), cnt + 1, pv, n, Bp, part, E.np, lim, s);
+                    bk::reduce_partials(part, E.np, static_cast<i64>(cnt + 1) * Bp, hd + E.h1(j), s, tmp);
+                });
+                P.run(K_UPDATE, double(cnt + 4) * nb * 8.0, [&] {
+                    bk::dcgs_update(E.dv(0), cnt, hd + E.scur(), hd + E.h1(j), hd + E.dbeta(), pv, n, Bp,
+                                    part, E.np, hd + E.eta(j), lim, s);
+                    bk::reduce_partials(part, E.np, static_cast<i64>(cnt + 2) * Bp, hd + E.h2(j), s, tmp);
+                });
+                P.run(K_OTHER, 0.0, [&] {
+                    if (reproject) {
+                        bk::dcgs_flags(hd + E.h2(j), cnt, Bp, lim, lflag, hd + E.flg(j), s);
+                        bk::basis_update(E.dv(j + 1), E.dv(0), cnt + 1, hd + E.h2(j), n, Bp, lflag, s);
+                        bk::multi_dot(E.dv(0), cnt + 2, E.dv(j + 1), n, Bp, part, E.np, lflag, s);
+                        bk::reduce_partials(part, E.np, static_cast<i64>(cnt + 2) * Bp, hd + E.s2(j), s, tmp);
+                        bk::dcgs_norms(hd + E.h2(j), hd + E.s2(j), hd + E.flg(j), cnt, Bp, lim, hd + E.scur(),
+                                       hd + E.bet(j + 1), hd + E.dbeta(), hd + E.dscale(), hd + E.dbeta2(), s);
+                    } else {
+                        bk::dcgs_norms(hd + E.h2(j), nullptr, nullptr, cnt, Bp, lim, hd + E.scur(),
+                                       hd + E.bet(j + 1), hd + E.dbeta(), hd + E.dscale(), hd + E.dbeta2(), s,
+                                       hd + E.flg(j));
+                    }
+                });
+            }
+            // q_m = (u_m - Q s_m) / beta_m
+            P.run(K_UPDATE, double(m + 2) * nb * 8.0 + nb * 16.0, [&] {
+                bk::basis_update(E.dv(m), E.dv(0), static_cast<int>(m), hd + E.scur(), n, Bp, lim, s);
+                bk::scale_lanes(E.dv(m), hd + E.dbeta2(), E.dv(m), n, Bp, lim, s);
             });
-            P.run(K_UPDATE, double(cnt + 2) * nb * 8.0, [&] { bk::basis_update(pv, E.dv(0), cnt, h, n, Bp, lim, s); });
         }
+        for (i64 j = 0; j < m && !delayed; ++j) {
+            const int cnt = static_cast<int>(j + 1);
+            P.run(K_SPMV, spmv_bytes, [&] { bk::shifted_apply<double>(A.view, sh, E.dv(j), nullptr, pv, nullptr, Bp, lim, s); });
+            for (int pass = 0; pass < 2; ++pass) {
+                double *h = hd + (pass == 0 ? E.h1(j) : E.h2(j));
+                P.run(K_DOT, double(cnt + 1) * nb * 8.0, [&] {
+                    bk::multi_dot(E.dv(0), cnt, pv, n, Bp, part, E.np, lim, s);
+                    bk::reduce_partials(part, E.np, static_cast<i64>(cnt) * Bp, h, s, tmp);
+                });
+                P.run(K_UPDATE, double(cnt + 2) * nb * 8.0, [&] { bk::basis_update(pv, E.dv(0), cnt, h, n, Bp, lim, s); });
+            }
+            P.run(K_DOT, nb * 8.0, [&] {
+                bk::multi_dot(pv, 1, pv, n, Bp, part, E.np, lim, s);
+                bk::reduce_partials(part, E.np, Bp, hd + E.nrm(j), s, tmp);
+            });
+            P.run(K_OTHER, nb * 16.0, [&] { bk::scale_lanes(pv, hd + E.nrm(j), E.dv(j + 1), n, Bp, lim, s); });
+        }
+        // |B V_m|^2 for the error estimates
+        P.run(K_SPMV, spmv_bytes, [&] { bk::shifted_apply<double>(A.view, sh, E.dv(m), nullptr, pv, nullptr, Bp, lim, s); });
         P.run(K_DOT, nb * 8.0, [&] {
             bk::multi_dot(pv, 1, pv, n, Bp, part, E.np, lim, s);
-            bk::reduce_partials(part, E.np, Bp, hd + E.nrm(j), s, tmp);
+            bk::reduce_partials(part, E.np, Bp, hd + E.nrm(m), s, tmp);
         });
-        P.run(K_OTHER, nb * 16.0, [&] { bk::scale_lanes(pv, hd + E.nrm(j), E.dv(j + 1), n, Bp, lim, s); });
+        HIPCHK(hipMemcpyAsync(E.host, hd, (delayed ? E.readback() : E.beta2()) * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+        drain();
+    };
+    arnoldi(false);
+    if (delayed) {
+        bool flagged = false;
+        for (i64 j = 0; j < Bp && !flagged; ++j)
+            for (i64 k = 0; k < m && run[j] && !flagged; ++k) flagged = E.host[E.flg(k) + j] != 0.0;
+        if (flagged) arnoldi(true);
     }
-    // |B V_m|^2 for the error estimates
-    P.run(K_SPMV, spmv_bytes, [&] { bk::shifted_apply<double>(A.view, sh, E.dv(m), nullptr, pv, nullptr, Bp, lim, s); });
-    P.run(K_DOT, nb * 8.0, [&] {
-        bk::multi_dot(pv, 1, pv, n, Bp, part, E.np, lim, s);
-        bk::reduce_partials(part, E.np, Bp, hd + E.nrm(m), s, tmp);
-    });
-    HIPCHK(hipMemcpyAsync(E.host, hd, E.beta2() * sizeof(double), hipMemcpyDeviceToHost, s));
-    drain();
     // every running lane's step control, in parallel on the host
     std::vector<i64> use(Bp, 0);
     mk::parallel_tasks(Bp, [&](i64 j) {
@@ -453,8 +530,42 @@ bool expv_engine_step(expv_batch &E) {
         auto &l = E.L[j];
         const double *H = E.host;
         try {
-            use[j] = l.c.plan([&](i64 i, i64 k) { return H[E.h1(k) + i * Bp + j] + H[E.h2(k) + i * Bp + j]; },
-                              [&](i64 k) { return H[E.nrm(k) + j]; }, l.y);
+            if (delayed) {
+                // column k of the Hessenberg matrix from step k's coefficients: the expansion
+                // of w = B u_k / beta_k is (t + s_{k+1} [+ the first projection where flagged],
+                // eta + ..., beta_{k+1}); since u_k / beta_k = q_k + Q s_k / beta_k, the columns
+                // of the earlier basis vectors it carries are subtracted: h_k -= sum_i
+                // (s_k[i] / beta_k) h_i, which makes B Q_m = Q_{m+1} H hold for the finalised q
+                std::vector<double> Hc(static_cast<std::size_t>((m + 1) * m), 0.0); // column-major
+                auto flagged = [&](i64 k) { return H[E.flg(k) + j] != 0.0; };
+                auto sn = [&](i64 k, i64 i) { return H[(flagged(k) ? E.s2(k) : E.h2(k)) + i * Bp + j]; };
+                for (i64 k = 0; k < m; ++k) {
+                    double *c = Hc.data() + k * (m + 1);
+                    for (i64 i = 0; i <= k; ++i) {
+                        c[i] = (i < k ? H[E.h1(k) + i * Bp + j] : H[E.eta(k) + j]) + sn(k, i);
+                        if (flagged(k)) c[i] += H[E.h2(k) + i * Bp + j];
+                    }
+                    c[k + 1] = H[E.bet(k + 1) + j];
+                    const double bk_ = k == 0 ? 1.0 : H[E.bet(k) + j];
+                    if (k > 0 && bk_ > 0.0) {
+                        for (i64 q = 0; q < k; ++q) {
+                            const double f = sn(k - 1, q) / bk_;
+                            const double *cq = Hc.data() + q * (m + 1);
+                            for (i64 i = 0; i <= q + 1; ++i) c[i] -= f * cq[i];
+                        }
+                    }
+                }
+                use[j] = l.c.plan([&](i64 i, i64 k) { return Hc[k * (m + 1) + i]; },
+                                  [&](i64 k) {
+                                      if (k == m) return H[E.nrm(m) + j];
+                                      const double b = H[E.bet(k + 1) + j];
+                                      return b * b;
+                                  },
+                                  l.y);
+            } else {
+                use[j] = l.c.plan([&](i64 i, i64 k) { return H[E.h1(k) + i * Bp + j] + H[E.h2(k) + i * Bp + j]; },
+                                  [&](i64 k) { return H[E.nrm(k) + j]; }, l.y);
+            }
         } catch (const std::exception &e) {
             l.error = e.what();
             use[j] = -1;

@@ -120,3 +120,115 @@ def test_invariant_subspace_gives_the_solution():
     y = np.linalg.lstsq(H, g, rcond=None)[0]
     x = Z.T @ y
     assert np.linalg.norm(b - A @ x) <= 1e-12 * np.linalg.norm(b)
+
+
+def dcgs2_nonflexible(B, v, m, force_explicit=False, leak=0.0, seed=0):
+    """The batched exponential's Arnoldi (markovkit_amd/csrc/expv.hip, expv_engine_step): delayed
+    CGS2 with M = I, q_0 = v / |v| (beta_0 = 1, s_0 empty), B applied to u_j / beta_j, and per step
+    the recorded (t, eta, first and second projections, flag, beta_{j+1}) from which the host
+    assembles the Hessenberg matrix. leak > 0 perturbs the first projections t and eta by a
+    relative amount leak (a poor first pass: the candidate then keeps a part in span(Q)), which
+    the delayed second projection removes and the Hessenberg correction must account for."""
+    rng = np.random.default_rng(seed)
+    n = v.size
+    V = np.zeros((m + 1, n))
+    V[0] = v / np.linalg.norm(v)
+    s, bj = np.zeros(0), 1.0
+    rec = []
+    for j in range(m):
+        w = B @ (V[j] * (1.0 / bj))
+        tg = V[: j + 1] @ w
+        t, gamma = tg[:j], tg[j]
+        eta = (gamma - s @ t) / bj
+        q = (V[j] - V[:j].T @ s) * (1.0 / bj)
+        V[j] = q
+        if leak > 0.0:
+            d = leak * np.linalg.norm(np.r_[t, eta]) * rng.standard_normal(j + 1) / np.sqrt(j + 1)
+            t, eta = t + d[:j], eta + d[j]
+        u = (w - V[:j].T @ t) - eta * q
+        V[j + 1] = u
+        s1 = np.r_[V[: j + 1] @ u, u @ u]
+        flag = force_explicit or not (s1[:-1] @ s1[:-1] <= 0.25 * s1[-1])
+        s2 = np.zeros_like(s1)
+        if flag:
+            u = u - V[: j + 1].T @ s1[:-1]
+            V[j + 1] = u
+            s2 = np.r_[V[: j + 1] @ u, u @ u]
+        sn = s2 if flag else s1
+        bn = np.sqrt(max(sn[-1] - sn[:-1] @ sn[:-1], 0.0))
+        rec.append(dict(t=t, eta=eta, s1=s1, s2=s2, flag=flag, beta=bn, s_used=sn[:-1]))
+        s, bj = sn[:-1], bn
+    V[m] = (V[m] - V[:m].T @ s) / bj
+    return V, rec
+
+
+def host_hessenberg(rec, m):
+    """The host's assembly in expv_engine_step: column k = (t + s [+ s1 where flagged], eta + ...,
+    beta_{k+1}), minus sum_i (s_k[i] / beta_k) h_i for the basis vectors u_k / beta_k carries."""
+    H = np.zeros((m + 1, m))
+    for k in range(m):
+        r = rec[k]
+        sn = r["s2"] if r["flag"] else r["s1"]
+        c = np.zeros(m + 1)
+        c[:k] = r["t"] + sn[:k]
+        c[k] = r["eta"] + sn[k]
+        if r["flag"]:
+            c[: k + 1] += r["s1"][: k + 1]
+        c[k + 1] = r["beta"]
+        if k > 0:
+            bk = rec[k - 1]["beta"]
+            s_k = rec[k - 1]["s_used"]
+            for q in range(k):
+                c[: q + 2] -= (s_k[q] / bk) * H[: q + 2, q]
+        H[:, k] = c
+    return H
+
+
+def mgs_arnoldi(B, v, m):
+    n = v.size
+    Q = np.zeros((m + 1, n))
+    H = np.zeros((m + 1, m))
+    Q[0] = v / np.linalg.norm(v)
+    for j in range(m):
+        w = B @ Q[j]
+        for i in range(j + 1):
+            H[i, j] = Q[i] @ w
+            w = w - H[i, j] * Q[i]
+        H[j + 1, j] = np.linalg.norm(w)
+        Q[j + 1] = w / H[j + 1, j]
+    return Q, H
+
+
+@pytest.mark.parametrize("force,leak", [(False, 0.0), (True, 0.0), (False, 0.05), (True, 0.05)],
+                         ids=["pythagorean", "explicit", "poor-first-pass", "poor-first-pass-explicit"])
+def test_nonflexible_lagged_normalisation(force, leak):
+    """The corrected columns satisfy B Q_m = Q_{m+1} H for the finalised (orthonormal) basis, their
+    first columns equal a modified Gram-Schmidt Arnoldi's from the same vector, and the Krylov
+    exponential from them equals expm; with candidates that keep a part in span(Q) (the case the
+    correction is for), leaving the correction out breaks the relation."""
+    from scipy.linalg import expm
+    rng = np.random.default_rng(7)
+    n, m = 140, 30
+    B = np.eye(n) * 3.0 + rng.standard_normal((n, n)) / np.sqrt(n)
+    v = rng.random(n)
+    V, rec = dcgs2_nonflexible(B, v, m, force_explicit=force, leak=leak)
+    assert sum(r["flag"] for r in rec) == (m if force else 0)
+    H = host_hessenberg(rec, m)
+    assert np.abs(V @ V.T - np.eye(m + 1)).max() <= 1e-13
+    BQ = B @ V[:m].T
+    assert np.abs(BQ - V.T @ H).max() <= 1e-13 * np.abs(BQ).max()
+    # the first columns, before the Krylov basis turns ill-conditioned, equal MGS Arnoldi's
+    _, Hm = mgs_arnoldi(B, v, 6)
+    assert np.abs(H[:7, :6] - Hm).max() <= 1e-12 * np.abs(Hm).max()
+    # exp(-B / 2) v from the Krylov basis
+    ref = expm(-0.5 * B) @ v
+    kry = np.linalg.norm(v) * V[:m].T @ expm(-0.5 * H[:m, :m])[:, 0]
+    assert np.abs(kry - ref).max() <= 1e-12 * np.abs(ref).max()
+    # without the correction the relation fails where candidates kept a part in span(Q) (an
+    # explicit second projection removes it before the candidate is used)
+    raw = H.copy()
+    for k in range(1, m):
+        raw[:, k] += sum((rec[k - 1]["s_used"][q] / rec[k - 1]["beta"]) * H[:, q] for q in range(k))
+    err_raw = np.abs(BQ - V.T @ raw).max() / np.abs(BQ).max()
+    if leak > 0.0 and not force:
+        assert err_raw > 1e-3

@@ -85,7 +85,7 @@ struct mk_batch_s {
     // Krylov workspace for restart m
     i64 m = 0;
     int np = 1;
-    dbuf V, Z, w, partial, rpart, rtmp, hd, d_b, d_x, d_io, d_scale, d_beta, d_cnt, d_y;
+    dbuf V, Z, w, partial, ptmp, rpart, rtmp, hd, d_b, d_x, d_io, d_scale, d_beta, d_cnt, d_y;
     double *host = nullptr; // pinned: hd mirror, y, scale, beta
     int *ihost = nullptr;   // pinned: cnt, rows, sel
     graph_cache graphs;
@@ -395,8 +395,9 @@ void ensure_workspace(mk_batch_s &B, i64 m) {
     B.V = dbuf(vec * (m + 1) * sizeof(double));
     B.Z = dbuf(vec * m * zs);
     B.w = dbuf(vec * sizeof(double));
-    B.np = bk::parts(n);
+    B.np = bk::parts_wide(n);
     B.partial = dbuf(static_cast<std::size_t>(B.np) * (m + 2) * Bp * sizeof(double));
+    B.ptmp = dbuf(std::max<i64>(1, bk::reduce_tmp_size(B.np, (m + 2) * Bp)) * sizeof(double));
     B.rpart = dbuf(static_cast<std::size_t>(bk::residual_parts(n)) * Bp * sizeof(double));
     B.rtmp = dbuf(std::max<i64>(1, bk::reduce_tmp_size(static_cast<int>(bk::residual_parts(n)), Bp)) *
                   sizeof(double));
@@ -542,12 +543,12 @@ bool step(mk_batch_s &B) {
             });
             P.run(K_DOT, double(cnt + 2) * nb * 8.0, [&] {
                 bk::multi_dot(V, cnt + 1, w, n, Bp, partial, np, lims, s);
-                bk::reduce_partials(partial, np, (cnt + 1) * Bp, hd + sl.t(), s);
+                bk::reduce_partials(partial, np, (cnt + 1) * Bp, hd + sl.t(), s, dp<double>(B.ptmp));
             });
             P.run(K_UPDATE, double(cnt + 4) * nb * 8.0, [&] {
                 bk::dcgs_update(V, cnt, sd, hd + sl.t(), dp<double>(B.d_beta), w, n, Bp, partial, np,
                                 hd + sl.eta(), lims, s);
-                bk::reduce_partials(partial, np, (cnt + 2) * Bp, sd_next, s);
+                bk::reduce_partials(partial, np, (cnt + 2) * Bp, sd_next, s, dp<double>(B.ptmp));
             });
         }, n, graphs);
         HIPCHK(hipMemcpyAsync(host, hd, sl.total(Bp) * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -577,7 +578,7 @@ bool step(mk_batch_s &B) {
             });
             P.run(K_DOT, double(cnt + 3) * nb * 8.0, [&] {
                 bk::multi_dot(V, cnt + 2, u, n, Bp, partial, np, lims, s);
-                bk::reduce_partials(partial, np, (cnt + 2) * Bp, sd_next, s);
+                bk::reduce_partials(partial, np, (cnt + 2) * Bp, sd_next, s, dp<double>(B.ptmp));
             });
             HIPCHK(hipMemcpyAsync(host, hd, sl.total(Bp) * sizeof(double), hipMemcpyDeviceToHost, s));
             drain();

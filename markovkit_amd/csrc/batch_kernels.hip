@@ -12,10 +12,9 @@ constexpr int kWaves = kThreads / kLanes;
 constexpr int kEntryUnroll = 8;
 constexpr int kRowsPerPart = 256;
 constexpr int kMaxParts = 512;
-// parts_wide: row partitions times lane groups the reductions of a small operand aim for (the
-// workgroups of one launch, so that they cover the CUs), and its upper bound
-constexpr int kWideGroups = 1024;
-constexpr int kMaxWideParts = 2048;
+// parts_wide: the row partitions of the reductions of a small operand (a workgroup per partition
+// and lane group, so that one launch covers the CUs)
+constexpr int kWideParts = 1024;
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
@@ -729,11 +728,9 @@ template void block_apply<float>(double *, const float *, int, const double *, i
 
 int parts(i64 n) { return op_parts(n); }
 
-int parts_wide(i64 n, i64 Bp) {
-    const i64 groups = std::max<i64>(1, Bp / kLanes);
-    const i64 want = (kWideGroups + groups - 1) / groups;
+int parts_wide(i64 n) {
     const i64 cap = std::max<i64>(1, (n + kWaves * 4 - 1) / (kWaves * 4)); // at least 4 rows per wave
-    return static_cast<int>(std::max<i64>(op_parts(n), std::min<i64>({want, cap, kMaxWideParts})));
+    return static_cast<int>(std::max<i64>(op_parts(n), std::min<i64>(kWideParts, cap)));
 }
 
 i64 residual_parts(i64 n) { return row_blocks(n); }

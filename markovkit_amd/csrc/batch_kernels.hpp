@@ -23,9 +23,10 @@ constexpr int kBlock = 4;  // systems per block of the block Krylov method (one 
 
 // row partitions used by the batched reductions for an n-row operand
 int parts(i64 n);
-// more partitions for a small operand (few lanes or rows): about 1024 workgroups per reduction
-// launch over its Bp / 64 lane groups, at least 16 rows per partition
-int parts_wide(i64 n, i64 Bp);
+// more partitions for an operand of few rows: 1024 (at least 16 rows each), so that a reduction
+// launch has enough workgroups even for one lane group; it depends on n only, so a system's sums
+// do not depend on how many systems share the launch
+int parts_wide(i64 n);
 
 // partial sums of the residual kernel: one per 4-row block
 i64 residual_parts(i64 n);

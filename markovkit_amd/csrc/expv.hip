@@ -348,7 +348,7 @@ expv_batch *expv_engine_create(mk_generator_s *gen, i64 slots, i64 krylov_dim) {
     E->W = dbuf(vec);
     E->V = dbuf(vec * (m + 1));
     E->p = dbuf(vec);
-    E->np = bk::parts_wide(n, Bp);
+    E->np = bk::parts_wide(n);
     E->partial = dbuf(static_cast<std::size_t>(E->np) * (m + 1) * Bp * sizeof(double));
     E->tmp = dbuf(static_cast<std::size_t>(std::max<i64>(1, bk::reduce_tmp_size(E->np, (m + 1) * Bp))) *
                   sizeof(double));
@@ -433,11 +433,7 @@ bool expv_engine_step(expv_batch &E) {
     const auto &A = E.gen->A;
     const double nb = double(n) * Bp;
     const double spmv_bytes = op_bytes(A, 0, 0, 0) + nb * 16.0;
-#ifdef MK_EXPV_CGS2
-    const bool delayed = false;
-#else
     const bool delayed = m < gpu::kFusedBasis;
-#endif
     // the Arnoldi process of every running lane: V_0 = W / beta, then m steps; the Hessenberg
     // coefficients and norms to the host
     auto arnoldi = [&](bool reproject) {

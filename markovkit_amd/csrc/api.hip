@@ -334,6 +334,14 @@ void build_levels(mk_sector_mg_s &mg, const std::vector<mk::sector_hierarchy> &H
                         p0[offset[0][k] + i2] = static_cast<u32>(offset[0][k] + nat(0, k, i2));
                 }
                 lv.perm = upload(p0);
+                std::vector<u32> q0(lv.n);
+                std::vector<T> e0(lv.n);
+                for (i64 i = 0; i < lv.n; ++i) {
+                    q0[p0[i]] = static_cast<u32>(i);
+                    e0[p0[i]] = colour[i] == 0 ? static_cast<T>(cat.exit[i]) : T(-1);
+                }
+                lv.pos = upload(q0);
+                lv.exit_box = upload(e0);
             }
         }
         if (mg.opt.smoother == MK_ILU0) build_ilu<T>(cat, mg.shift, lv);
@@ -625,8 +633,15 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             // first half-sweep into cur
             if (first_done && (rb || cycles != 1))
                 throw std::logic_error("sector_multigrid: a fused first half-sweep on a level that reorders");
-            if (!first_done) P.run(cls(K_FIRST), double(n) * ((rhs64 ? 8.0 + ts : double(ts)) + 2.0 + ts / 2.0 + (rb ? ts : 0)), [&] {
-                if (rhs64) {
+            // bytes: b, its row index or colour and exit code, the cast right-hand side, x of colour 0
+            const double fb = rhs64 && perm0 ? double(n) * (8.0 + 4.0 + 2.5 * ts)
+                                             : double(n) * ((rhs64 ? 8.0 + ts : double(ts)) + 2.0 + ts / 2.0 + (rb ? ts : 0));
+            if (!first_done) P.run(cls(K_FIRST), fb, [&] {
+                if (rhs64 && perm0) {
+                    // from the caller's order: read b in its order and scatter to the rows
+                    gpu::rb_first_scatter<T, double>(n, dp<T>(lv.exit_box), shift, rhs64, rb, cur, s, scale,
+                                                     dp<u32>(lv.pos));
+                } else if (rhs64) {
                     gpu::rb_first<T, double>(lv.A.view, dp<std::uint8_t>(lv.colour), shift, rhs64, rb, cur, s,
                                              scale, perm0);
                 } else {

@@ -559,6 +559,23 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// the same from the caller's order: box index j goes to row pos[j], whose exit rate (colour 0)
+// or -1 (colour 1) is exit_box[j]
+template <typename T, typename IN>
+__global__ void __launch_bounds__(kThreads)
+    k_rb_first_scatter(i64 n, const T *exit_box, T shift, const IN *rhs_in, const double *scale_ptr,
+                       const u32 *pos, T *rhs_out, T *x) {
+    const IN scale = scale_ptr ? static_cast<IN>(*scale_ptr) : IN(1);
+    for (i64 j = blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x; j < n;
+         j += static_cast<i64>(gridDim.x) * kThreads) {
+        const u32 i = pos[j];
+        const T ex = exit_box[j];
+        const T b = static_cast<T>(rhs_in[j] * scale);
+        rhs_out[i] = b;
+        if (ex >= T(0)) x[i] = b / (shift + ex);
+    }
+}
+
 // out[i] = (T)(in[perm[i]] * scale) and out[perm[i]] = in[i]: a level's row order to and from
 // the caller's
 template <typename T, typename IN>
@@ -1325,6 +1342,15 @@ void level_residual(const csr_view<T> &A, double shift, const T *rhs, const T *x
         csr_args<T, T, T>{A, shift, T(0), x, rhs, r, nullptr, nullptr, nullptr, nullptr, 0u}, s);
 }
 
+template <typename T, typename IN>
+void rb_first_scatter(i64 n, const T *exit_box, double shift, const IN *rhs_in, T *rhs_out, T *x,
+                      hipStream_t s, const double *scale, const u32 *pos) {
+    if (n == 0) return;
+    const i64 blocks = std::max<i64>(1, std::min<i64>((n + kThreads - 1) / kThreads, 65536));
+    hipLaunchKernelGGL((k_rb_first_scatter<T, IN>), dim3(static_cast<u32>(blocks)), dim3(kThreads), 0, s, n,
+                       exit_box, static_cast<T>(shift), rhs_in, scale, pos, rhs_out, x);
+}
+
 template <typename T>
 void gs_half(const csr_view<T> &Ac, double shift, const T *rhs, T *x, T *res_zero, hipStream_t s,
              T *nat, const u32 *perm) {
@@ -1656,6 +1682,10 @@ template void jacobi_first<float, float>(const csr_view<float> &, double, double
                                          float *, float *, hipStream_t, const double *);
 template void rb_first<double, double>(const csr_view<double> &, const u8 *, double, const double *,
                                        double *, double *, hipStream_t, const double *, const u32 *);
+template void rb_first_scatter<double, double>(i64, const double *, double, const double *, double *,
+                                               double *, hipStream_t, const double *, const u32 *);
+template void rb_first_scatter<float, double>(i64, const float *, double, const double *, float *, float *,
+                                              hipStream_t, const double *, const u32 *);
 template void rb_first<float, double>(const csr_view<float> &, const u8 *, double, const double *,
                                       float *, float *, hipStream_t, const double *, const u32 *);
 template void rb_first<float, float>(const csr_view<float> &, const u8 *, double, const float *,

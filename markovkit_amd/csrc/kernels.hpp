@@ -146,6 +146,13 @@ void gs_half(const csr_view<T> &Ac, double shift, const T *rhs, T *x, T *res_zer
 template <typename T, typename IN>
 void rb_first(const csr_view<T> &A, const u8 *colour, double shift, const IN *rhs_in, T *rhs_out,
               T *x, hipStream_t s, const double *scale = nullptr, const u32 *perm = nullptr);
+//   rb_first_scatter: the same values with rhs_in in the caller's order, read in that order:
+//   row pos[j] takes rhs_in[j] (pos the inverse of perm), and exit_box[j] is the exit rate of that
+//   row where it has colour 0, else negative; every load is in the caller's order, none depends
+//   on another
+template <typename T, typename IN>
+void rb_first_scatter(i64 n, const T *exit_box, double shift, const IN *rhs_in, T *rhs_out, T *x,
+                      hipStream_t s, const double *scale, const u32 *pos);
 // out[i] = (T)(scale * in[perm[i]]); out[perm[i]] = in[i]
 template <typename T, typename IN>
 void permute_in(i64 n, const u32 *perm, const IN *in, T *out, hipStream_t s,

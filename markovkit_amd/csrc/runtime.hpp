@@ -665,8 +665,9 @@ struct level_dev {
     dev_csr<T> Ac[2];
     dbuf rowmap[2], colour;
     // the level numbers each sector's states colour 0 first; on the top level, perm maps a row
-    // to its index in the caller's (box) order
-    dbuf perm;
+    // to its index in the caller's (box) order, pos a box index to its row, and exit_box holds,
+    // per box index, the exit rate of its row where that row has colour 0 (-1 for colour 1)
+    dbuf perm, pos, exit_box;
     // ILU(0): reciprocal pivots (for the hierarchy's shift) and the level schedules of the
     // forward and backward solves: rows grouped by dependency depth (group k is
     // rows[ptr[k] .. ptr[k + 1])), launched as single wide groups or runs of small groups

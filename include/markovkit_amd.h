@@ -475,6 +475,11 @@ mk_status mk_tail_lds_enable(int32_t on);
  * (default on; MK_FUSED_FIRST=0 turns it off at start). Results are bitwise the same either way.
  * No reference counterpart. */
 mk_status mk_fused_first_enable(int32_t on);
+/* A direct coarse level (every row in a dense block of more than 128 states) solved and
+ * prolonged into the level above in one launch: each coarse value is added to its colour-0
+ * children as soon as its row of the inverse is applied (default on; MK_COARSE_FUSE=0 turns it
+ * off at start). Results are bitwise the same either way. No reference counterpart. */
+mk_status mk_coarse_fuse_enable(int32_t on);
 /* Workgroup-to-XCD mapping of the operator kernels (no reference counterpart): XCD j takes the
  * chunks j, j + 8, ... of `chunk` consecutive row blocks; 0 keeps workgroup b on row block b.
  * Default 16 (MK_XCD_CHUNK overrides). Process-wide; the autotuner sets it. */

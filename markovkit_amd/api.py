@@ -656,6 +656,12 @@ def fused_first_enable(on: bool = True) -> None:
     check(lib().mk_fused_first_enable(1 if on else 0))
 
 
+def coarse_fuse_enable(on: bool = True) -> None:
+    """Solve a direct coarse level and prolong it into the level above in one launch (default
+    on); bitwise the same."""
+    check(lib().mk_coarse_fuse_enable(1 if on else 0))
+
+
 def fused_tail_enable(on: bool = True) -> None:
     """Run the small multigrid levels in one workgroup-wide kernel (default on)."""
     check(lib().mk_fused_tail_enable(1 if on else 0))

@@ -755,12 +755,24 @@ T *run_level(const mk_sector_mg_s &mg, std::vector<level_dev<T>> &L, std::size_t
             // the coarse levels' launches are timed as one interval per visit (an event pair per
             // small launch would cost as much as the launch)
             T *cx = nullptr;
-            P.span(K_COARSE, [&] {
-                cx = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr, nullptr, 1, fuse);
-            });
-            P.run(cls(K_PROLONG), pbytes, [&] {
-                gpu::prolong_add<T>(np, dp<u32>(lv.coarse_of), dp<T>(lv.weight), cx, cur, s, prows);
-            });
+            // a direct coarse level in wave-per-row blocks: its dense solve prolongs each coarse
+            // value to the colour-0 children right away (one launch instead of two)
+            const bool dense_fuse = red_only && c.direct && c.term_max > gpu::kWideTerminal &&
+                                    lv.colour.p != nullptr && coarse_fuse_switch().load() != 0;
+            if (dense_fuse) {
+                P.span(K_COARSE, [&] {
+                    gpu::terminal_prolong<T>(c.n_term, dp<u32>(c.tstart), dp<u32>(c.tsize), dp<u64>(c.toff),
+                                             dp<T>(c.tinv), c.term_max, dp<T>(c.rhs), dp<T>(c.x), lv.R.view,
+                                             dp<std::uint8_t>(lv.colour), dp<T>(lv.weight), cur, s);
+                });
+            } else {
+                P.span(K_COARSE, [&] {
+                    cx = run_level<T>(mg, L, l + 1, dp<T>(c.rhs), nullptr, nullptr, nullptr, 1, fuse);
+                });
+                P.run(cls(K_PROLONG), pbytes, [&] {
+                    gpu::prolong_add<T>(np, dp<u32>(lv.coarse_of), dp<T>(lv.weight), cx, cur, s, prows);
+                });
+            }
         } else {
             // the tail runs all of the coarse cycles in one launch
             const bool in_tail = mg.tail_w && mg.tail_start == static_cast<i64>(l + 1) &&
@@ -1070,7 +1082,8 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
                 // the fused-tail switches choose the launches captured: both are part of the key
                 graph_key key{A.uid, M ? M->uid : 0, j,
                               static_cast<i64>(sd > sd_next) + 2 * tail_switch().load() +
-                                  4 * tail_lds_switch().load() + 8 * fused_first_switch().load(),
+                                  4 * tail_lds_switch().load() + 8 * fused_first_switch().load() +
+                                  16 * coarse_fuse_switch().load(),
                               n, m, bits(shift)};
                 step(key, [&] {
                     operator_image(j, V + j * ld, scal);
@@ -1517,6 +1530,11 @@ mk_status mk_device_setup_enable(int32_t on) {
 
 mk_status mk_fused_first_enable(int32_t on) {
     fused_first_switch().store(on ? 1 : 0);
+    return MK_OK;
+}
+
+mk_status mk_coarse_fuse_enable(int32_t on) {
+    coarse_fuse_switch().store(on ? 1 : 0);
     return MK_OK;
 }
 

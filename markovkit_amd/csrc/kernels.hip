@@ -466,6 +466,44 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// the dense solve of one coarse row (a wave, lanes over the row of the inverse, as
+// k_terminal_wide) followed by the prolongation of that value to its colour-0 children
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_terminal_prolong(i64 nb, const u32 *start, const u32 *size, const u64 *offset, const T *inv,
+                       i64 max_size, const T *r, T *xc, csr_view<T> R, const u8 *fine_colour,
+                       const T *weight, T *x_fine) {
+    const i64 g = (blockIdx.x * static_cast<i64>(kThreads) + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (g >= nb * max_size) return;
+    const i64 b = g / max_size, i = g % max_size;
+    const u32 m = size[b];
+    if (i >= m) return;
+    const T *row = inv + offset[b] + static_cast<u64>(i) * m;
+    const T *rr = r + start[b];
+    T s = 0;
+#pragma unroll 4
+    for (u32 j = lane; j < m; j += 64) s += row[j] * rr[j];
+    s = wave_sum(s);
+    const u32 c = start[b] + static_cast<u32>(i);
+    if (lane == 0) xc[c] = s;
+    // the children of c, one per lane
+    if (R.coded) {
+        const u32 len = R.rlen[c];
+        const u32 base = R.soff[c >> 6] + (c & 63u);
+        for (u32 k = lane; k < len; k += 64) {
+            const u32 f = R.ent[base + k * kSlice] & kColMask;
+            if (fine_colour[f] == 0) x_fine[f] += weight[f] * s;
+        }
+    } else {
+        const u32 p0 = R.ptr[c], p1 = R.ptr[c + 1];
+        for (u32 p = p0 + lane; p < p1; p += 64) {
+            const u32 f = R.col[p];
+            if (fine_colour[f] == 0) x_fine[f] += weight[f] * s;
+        }
+    }
+}
+
 // ---- aggregate-block smoother (thread per listed row)
 // sum over row i's entries of rate * x[column], ascending entry order, and row i's exit rate
 template <typename T>
@@ -1464,6 +1502,17 @@ void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *
                        offset, inverse, max_size, r, x, static_cast<int>(assign));
 }
 
+template <typename T>
+void terminal_prolong(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
+                      const T *inverse, i64 max_size, const T *r, T *xc, const csr_view<T> &R,
+                      const u8 *fine_colour, const T *weight, T *x_fine, hipStream_t s) {
+    if (n_blocks == 0) return;
+    const i64 threads = n_blocks * max_size * 64;
+    const int grid = static_cast<int>((threads + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL((k_terminal_prolong<T>), dim3(grid), dim3(kThreads), 0, s, n_blocks, start, size,
+                       offset, inverse, max_size, r, xc, R, fine_colour, weight, x_fine);
+}
+
 namespace {
 u32 blocks_for(u64 n) { return static_cast<u32>((n + kThreads - 1) / kThreads); }
 } // namespace
@@ -1644,6 +1693,9 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
                                  const u32 *, const T *);                                         \
     template void terminal_solve<T>(i64, const u32 *, const u32 *, const u64 *, const T *, i64, \
                                     const T *, T *, hipStream_t, bool);                         \
+    template void terminal_prolong<T>(i64, const u32 *, const u32 *, const u64 *, const T *, i64, \
+                                      const T *, T *, const csr_view<T> &, const u8 *, const T *, \
+                                      T *, hipStream_t);                                         \
     template void fill<T>(i64, T, T *, hipStream_t);                                            \
     template void block_solve<T>(const u32 *, const u32 *, const u32 *, const u64 *, const T *,   \
                                  u64, u64, const T *, T *, T *, hipStream_t);                     \

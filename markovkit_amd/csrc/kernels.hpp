@@ -126,6 +126,14 @@ template <typename T>
 void terminal_solve(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
                     const T *inverse, i64 max_size, const T *r, T *x, hipStream_t s,
                     bool assign = false);
+// a direct coarse level's dense solve fused with the prolongation into the level above: wave g
+// computes coarse row c = start_b + i as terminal_solve does (assign), stores it to xc[c], and
+// adds weight[f] * xc[c] to x_fine[f] for the children f of c (row c of the restriction R, fine
+// rows ascending) whose fine colour is 0 -- the values prolong_add gives for the colour-0 rows
+template <typename T>
+void terminal_prolong(i64 n_blocks, const u32 *start, const u32 *size, const u64 *offset,
+                      const T *inverse, i64 max_size, const T *r, T *xc, const csr_view<T> &R,
+                      const u8 *fine_colour, const T *weight, T *x_fine, hipStream_t s);
 
 // Red-black Gauss-Seidel on one level. The level operator is split by colour into two
 // row-mapped operators (rows of one colour, all their entries); a half-sweep updates one colour

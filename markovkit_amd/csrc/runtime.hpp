@@ -457,6 +457,16 @@ inline std::atomic<int> &fused_first_switch() {
     return on;
 }
 
+// a direct coarse level's dense solve fused with the prolongation above it (default on;
+// MK_COARSE_FUSE=0 turns it off); the same values either way
+inline std::atomic<int> &coarse_fuse_switch() {
+    static std::atomic<int> on{[] {
+        const char *env = std::getenv("MK_COARSE_FUSE");
+        return (env && env[0] == '0') ? 0 : 1;
+    }()};
+    return on;
+}
+
 inline std::atomic<int> &tail_switch() {
     static std::atomic<int> on{[] {
         const char *env = std::getenv("MK_MG_TAIL");

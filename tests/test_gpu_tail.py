@@ -114,3 +114,37 @@ def test_fused_first_half_sweep_is_bitwise(wl, precision, shape):
     finally:
         mk.fused_first_enable(True)
         mk.fused_tail_enable(True)
+
+
+DENSE_SHAPES = {"V-c800": dict(coarsest_size=800),
+                "W1-c800": dict(pre_sweeps=1, post_sweeps=2, coarse_cycles=2, cycle_levels=1, coarsest_size=800)}
+
+
+@pytest.mark.parametrize("shape", list(DENSE_SHAPES))
+@pytest.mark.parametrize("precision", [mk.FP32, mk.FP64], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("wl", [annihilation_2d(4, (2, 2), 1.0), birth_death_1d(32, 4, 0.7, "random")],
+                         ids=["2d", "58905"])
+def test_coarse_fuse_is_bitwise(wl, precision, shape):
+    """A direct coarse level (dense blocks of more than 128 states) solved and prolonged in one
+    launch against its terminal solve and prolongation launches: cycle and solve bitwise equal"""
+    box = mk.PopulationBox(wl.n_voxels, wl.max_totals)
+    Q = mk.Generator(box, wl.model)
+    s = wl.shifts[0]
+    M = mk.SectorMultigrid(wl.model.domain, box, s,
+                           mk.MultigridOptions(precision=precision, smoother=mk.RED_BLACK,
+                                               **DENSE_SHAPES[shape]))
+    b = np.random.default_rng(6).random(box.size())
+    out = {}
+    try:
+        for fused in (True, False):
+            mk.coarse_fuse_enable(fused)
+            z = M.apply(b)
+            x = np.zeros(box.size())
+            r = mk.gmres(mk.ShiftedGenerator(Q, s), M, b, x, tolerance=1e-10 * np.linalg.norm(b))
+            assert r.converged
+            out[fused] = (z, x, r.iterations)
+    finally:
+        mk.coarse_fuse_enable(True)
+    assert np.array_equal(out[True][0], out[False][0])
+    assert np.array_equal(out[True][1], out[False][1])
+    assert out[True][2] == out[False][2]

@@ -4,14 +4,20 @@
 Workload (default): 1D RDME on 64 voxels (h = 1/64, D = 1, births 0.05 * volume, deaths
 0.02 * n), box cap 5 = 11,238,513 states, one shift s = 0.5, b ~ U[0,1) (numpy PCG64, seed 0)
 normalised to unit L1 norm. One step = one resolvent solve (sI - Q) x = b from x = 0 to a
-true FP64 relative residual <= 1e-10 by flexible GMRES(50) with CGS2, preconditioned by
-sector_multigrid (Jacobi V-cycle, pre = post = 2, w = 2/3). b, x, the operator and the
-hierarchy are resident in HBM before the timed region starts.
+true FP64 relative residual <= 1e-10 by flexible GMRES(50) with delayed CGS2, preconditioned by
+sector_multigrid (red-black Gauss-Seidel, 1 + 2 sweeps, W-cycles on the top 2 levels, sectors of
+at most 800 states solved densely). b, x, the operator and the hierarchy are resident in HBM
+before the timed region starts.
 
 value = solves/s of the mixed path (FP32 multigrid inside FP64 FGMRES); the plain FP64 path
 (FP64 multigrid) is timed beside it. The roofline of the dominant kernel class comes from HIP
-events recorded around that class's launches inside the timed mixed region; cpu_baseline is
-the FP64 CPU oracle timed on a bounded sample of the same solve on this host (1 core).
+events recorded around that class's launches inside the timed mixed region, its HBM traffic from
+two rocprofv3 counter passes started by this script; cpu_baseline is one complete solve of the
+same system by the FP64 CPU oracle on this host (the reference's method, 1 core), whose solution
+also gives the per-state L1 distance. The default run adds the other configurations as fields:
+the reference's method on the GPU (jacobi), the multi-shift batch (multishift), the paper
+driver's transient (box_mg), the 2D two-species box (c3), the serving mix (serving), the batched
+exponentials (expv) and the 969-state case (c1).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -338,6 +344,36 @@ def multishift(mk, workloads, steps, restart, sequential_sample=16, tune=True):
     }
 
 
+def expv_leg(mk, vectors=64, seed=2):
+    """The serving mix's exponentials alone: `vectors` toggle-switch point masses (0..200 x 0..200,
+    absorbing box, states drawn with numpy PCG64 seed 2) advanced to t = 5 at tolerance 1e-8 with
+    Krylov dimension 30, as one batched call (delayed CGS2 Arnoldi, Expokit step control per vector),
+    and one vector alone through the single-vector expv."""
+    from markovkit_amd.fsp import box_generator, toggle_switch
+    F = mk.Generator.from_jumps(*box_generator(toggle_switch(), [200, 200], absorbing=True))
+    n = F.rows()
+    rng = np.random.default_rng(seed)
+    V = np.zeros((vectors, n))
+    V[np.arange(vectors), rng.integers(0, n, vectors)] = 1.0
+    mk.expv_batch(F, np.full(vectors, 0.1), V)  # warm-up
+    mk.synchronize()
+    t0 = time.perf_counter()
+    W, res = mk.expv_batch(F, np.full(vectors, 5.0), V, krylov_dim=30, tolerance=1e-8)
+    batch_s = time.perf_counter() - t0
+    mk.expv(F, 0.1, V[0])
+    t0 = time.perf_counter()
+    w1, r1 = mk.expv(F, 5.0, V[0], krylov_dim=30, tolerance=1e-8)
+    single_s = time.perf_counter() - t0
+    steps = [r.steps for r in res]
+    return {"workload": f"toggle switch, {n} states (absorbing 0..200 x 0..200), {vectors} point masses "
+                        f"to t = 5 at 1e-8, Krylov dimension 30",
+            "vectors_per_s": vectors / batch_s, "batch_ms": 1e3 * batch_s,
+            "ms_per_vector": 1e3 * batch_s / vectors, "steps_min": min(steps), "steps_max": max(steps),
+            "single_vector_ms": 1e3 * single_s, "single_steps": r1.steps,
+            "max_l1_batch0_vs_single": float(np.abs(W[0] - w1).sum()),
+            "min_mass": float(W.sum(axis=1).min())}
+
+
 def serving(mk, workloads, n_requests=120, workers=4, seed=1, rate=200.0, smoother_name="red_black"):
     """C5: Poisson arrivals at `rate` requests/s (numpy PCG64, seed 1). 70% are RDME resolvent
     requests of the 1D family (voxels in {16, 32, 48, 64}, caps in {3, 4, 5}, shift ~ U[0.1, 10],
@@ -653,7 +689,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="large",
-                    choices=["large", "small", "2d", "multishift", "serving", "box"])
+                    choices=["large", "small", "2d", "multishift", "serving", "box", "expv"])
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--workers", type=int, default=4, help="serving: worker streams")
     ap.add_argument("--restart", type=int, default=50)
@@ -691,6 +727,8 @@ def main():
                     help="skip the serving mix after the headline")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the 2D two-species annihilation solves after the headline")
+    ap.add_argument("--no-expv", action="store_true",
+                    help="skip the batched toggle-switch exponentials after the headline")
     ap.add_argument("--no-c1", action="store_true",
                     help="skip the 969-state single, batched and CPU-oracle solves after the headline")
     ap.add_argument("--no-traffic", action="store_true",
@@ -706,6 +744,15 @@ def main():
     import markovkit_amd as mk
     from markovkit_amd import workloads
 
+    if args.workload == "expv":
+        ev = expv_leg(mk)
+        out = {"metric": "batched toggle-switch exponentials (t = 5, 1e-8) per second",
+               "value": ev["vectors_per_s"], "unit": "vectors/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+               "ms_per_step": ev["batch_ms"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+               "config": {"workload": ev["workload"]}, "expv": ev}
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == "serving":
         sv = serving(mk, workloads, args.requests, args.workers, smoother_name=args.smoother)
         out = {"metric": "served RDME resolvent + FSP requests/s with p50/p99 latency",
@@ -939,6 +986,9 @@ def main():
         log("[c3] 2D two-species annihilation, 4,601,025 states")
         out["c3"] = c3(mk, workloads, max(10, args.steps), args.restart)
         log(f"[c3] {out['c3']['value']:.1f} solves/s mixed, {out['c3']['fp64']['value']:.1f} FP64")
+    if args.workload == "large" and not args.no_expv:
+        log("[expv] batched toggle-switch exponentials")
+        out["expv"] = expv_leg(mk)
     if args.workload == "large" and not args.no_c1:
         log("[c1] 1D 16 voxels cap 3, 969 states: single solves, a 256-system batch, the CPU oracle")
         out["c1"] = c1(mk, workloads, args.restart)

@@ -610,9 +610,9 @@ def c3(mk, workloads, steps, restart):
 
 def c1(mk, workloads, restart, systems=256, single_steps=200):
     """The reference's CPU-runnable case: 1D, 16 voxels, cap 3 (969 states), s = 0.5. A single
-    solve of (0.5 I - Q) x = e_0 (mixed and plain FP64, red-black sector multigrid, FGMRES) is
-    bound by the latency of its ~200 dependent phases on the GPU, so the case is also run as
-    one batch of `systems` systems of the same shift: e_0 and random L1-normalised right-hand
+    solve of (0.5 I - Q) x = e_0 (mixed and plain FP64, red-black sector multigrid, FGMRES) runs
+    as one kernel launch in one workgroup, bound by the latency of its ~600 dependent barrier
+    phases, so the case is also run as one batch of `systems` systems of the same shift: e_0 and random L1-normalised right-hand
     sides (seeds 1..systems-1), each to 1e-10 relative residual. The CPU oracle (the reference's
     method: weighted-Jacobi V-cycle, GMRES with MGS; one core) solves the same systems one by
     one, timed beside it."""

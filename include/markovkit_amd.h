@@ -480,6 +480,12 @@ mk_status mk_fused_first_enable(int32_t on);
  * children as soon as its row of the inverse is applied (default on; MK_COARSE_FUSE=0 turns it
  * off at start). Results are bitwise the same either way. No reference counterpart. */
 mk_status mk_coarse_fuse_enable(int32_t on);
+/* FGMRES run whole in one workgroup (one kernel launch per solve) when the preconditioner is a
+ * sector hierarchy whose levels all fit the LDS tail, the problem has at most 1024 states and
+ * the restart is at most 64: the same iteration (classical Gram-Schmidt twice) and stopping
+ * tests as the host-driven loop, with sums in another order, so not bitwise the same (default on;
+ * MK_TINY_GMRES=0 turns it off at start). No reference counterpart. */
+mk_status mk_tiny_gmres_enable(int32_t on);
 /* Workgroup-to-XCD mapping of the operator kernels (no reference counterpart): XCD j takes the
  * chunks j, j + 8, ... of `chunk` consecutive row blocks; 0 keeps workgroup b on row block b.
  * Default 16 (MK_XCD_CHUNK overrides). Process-wide; the autotuner sets it. */

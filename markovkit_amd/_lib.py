@@ -143,6 +143,7 @@ SIGNATURES = {
     "mk_tail_lds_enable": ([i32], i32),
     "mk_fused_first_enable": ([i32], i32),
     "mk_coarse_fuse_enable": ([i32], i32),
+    "mk_tiny_gmres_enable": ([i32], i32),
     "mk_set_xcd_chunk": ([i32], i32),
     "mk_get_xcd_chunk": ([C.POINTER(i32)], i32),
     "mk_profile_enable": ([i32], i32),

@@ -662,6 +662,12 @@ def coarse_fuse_enable(on: bool = True) -> None:
     check(lib().mk_coarse_fuse_enable(1 if on else 0))
 
 
+def tiny_gmres_enable(on: bool = True) -> None:
+    """Run a whole solve in one workgroup when the problem has at most 1024 states and its
+    sector hierarchy fits the LDS tail (default on); the same iteration, sums in another order."""
+    check(lib().mk_tiny_gmres_enable(1 if on else 0))
+
+
 def fused_tail_enable(on: bool = True) -> None:
     """Run the small multigrid levels in one workgroup-wide kernel (default on)."""
     check(lib().mk_fused_tail_enable(1 if on else 0))

@@ -993,6 +993,48 @@ i64 gmres_restart(const mk_gmres_options &o, i64 n) {
     return std::max<i64>(1, std::min<i64>(o.restart, n));
 }
 
+// whether the whole solve runs in one workgroup: a sector hierarchy of precision ZT whose levels,
+// the top one included, sit in the LDS tail, one row per thread and a restart the kernel holds
+template <typename ZT>
+bool tiny_fits(const precond *M, i64 n, i64 m) {
+    if (!M || !M->mg || tiny_gmres_switch().load() == 0) return false;
+    const mk_sector_mg_s &mg = *M->mg;
+    const bool precision = (mg.precision == MK_FP32) == std::is_same_v<ZT, float>;
+    return precision && mg.tail_start == 0 && mg.tail_lds_bytes > 0 && tail_switch().load() != 0 &&
+           tail_lds_switch().load() != 0 && n <= gpu::kTinyRows && m <= gpu::kTinyBasis &&
+           gpu::gmres_tiny_lds(mg.tail_lds_bytes) > 0;
+}
+
+// restarted FGMRES (CGS2, no delayed variant) with the LDS tail, in one kernel launch
+template <typename ZT>
+mk_solver_result fgmres_tiny(mk_generator_s &A, double shift, const mk_sector_mg_s &mg, const double *b,
+                             double *x, const mk_gmres_options &o, const krylov_space &K, i64 m) {
+    gpu::tiny_gmres_args a;
+    a.A = A.A.view;
+    a.shift = shift;
+    a.b = b;
+    a.x = x;
+    a.V = K.V;
+    a.Z = K.Z;
+    a.ld = K.ld;
+    a.n = static_cast<int>(A.n);
+    a.m = static_cast<int>(m);
+    a.max_iterations = static_cast<int>(std::min<i64>(o.max_iterations, i64{1} << 30));
+    a.tolerance = o.tolerance;
+    a.result = K.hd;
+    a.image = mg.tail_image.p;
+    a.image_bytes = mg.tail_image_bytes;
+    a.lds_bytes = mg.tail_lds_bytes;
+    a.levels = static_cast<int>(mg.precision == MK_FP32 ? mg.L32.size() : mg.L64.size());
+    a.omega = mg.opt.jacobi_weight;
+    const hipStream_t s = stream();
+    prof().run(K_OTHER, 0.0, [&] { gpu::gmres_tiny<ZT>(a, s); });
+    HIPCHK(hipMemcpyAsync(K.host, K.hd, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    drain();
+    mk_solver_result res{static_cast<int64_t>(K.host[0]), K.host[1], static_cast<int32_t>(K.host[2])};
+    return res;
+}
+
 template <typename ZT>
 mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const double *b,
                         double *x, const mk_gmres_options &o, const krylov_space &K) {
@@ -1001,6 +1043,7 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
     require(m + 1 <= 512, "gmres: restart above 511 is not supported on the device");
     require(o.orthogonalization == MK_DCGS2 || o.orthogonalization == MK_CGS2,
             "gmres: unknown orthogonalization");
+    if (o.max_iterations > 0 && tiny_fits<ZT>(M, n, m)) return fgmres_tiny<ZT>(A, shift, *M->mg, b, x, o, K, m);
     const bool delayed = M && o.orthogonalization == MK_DCGS2 && m <= gpu::kFusedBasis;
     auto &P = prof();
     const hipStream_t s = stream();
@@ -1232,7 +1275,8 @@ mk_solver_result fgmres(mk_generator_s &A, double shift, const precond *M, const
 
 precond sector_precond(mk_sector_mg_s *mg) {
     return precond{mg->uid, mg->rows, mg->shift, mg->precision,
-                   [mg](const double *v, void *z, const double *scale) { mg_apply_device(*mg, v, z, scale); }};
+                   [mg](const double *v, void *z, const double *scale) { mg_apply_device(*mg, v, z, scale); },
+                   mg};
 }
 
 mk_solver_result gmres_device(mk_generator_s &A, double shift, mk_sector_mg_s *mg, const double *b,
@@ -1535,6 +1579,11 @@ mk_status mk_fused_first_enable(int32_t on) {
 
 mk_status mk_coarse_fuse_enable(int32_t on) {
     coarse_fuse_switch().store(on ? 1 : 0);
+    return MK_OK;
+}
+
+mk_status mk_tiny_gmres_enable(int32_t on) {
+    tiny_gmres_switch().store(on ? 1 : 0);
     return MK_OK;
 }
 

@@ -4,6 +4,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <stdexcept>
 #include <type_traits>
 
 namespace mk::gpu {
@@ -708,10 +709,22 @@ __device__ __forceinline__ T tail_gather(const csr_view<T> &A, u32 i, const T *x
         const T *vdict = lds<LDS>(A.vdict);
         const T *xs = lds<LDS>(x);
         T sum = 0;
-#pragma unroll 4
-        for (u32 k = 0; k < len; ++k) {
-            const u32 e = ent[base + k * kSlice];
-            sum += vdict[e >> kColBits] * xs[e & kColMask];
+        // eight entries, then their eight gathers, in flight at once: one workgroup runs the tail,
+        // so a row's chain of dependent LDS reads is the phase's latency. Past the row's end the
+        // gathers read entry 0 of both arrays (the row has entries, so both exist) and are not
+        // added: the sum keeps entry order, each term fused into it as sum += rate * x compiles
+        for (u32 k0 = 0; k0 < len; k0 += 8) {
+            u32 e[8];
+            T r[8], v[8];
+#pragma unroll
+            for (u32 u = 0; u < 8; ++u) e[u] = k0 + u < len ? ent[base + (k0 + u) * kSlice] : 0u;
+#pragma unroll
+            for (u32 u = 0; u < 8; ++u) {
+                r[u] = vdict[e[u] >> kColBits];
+                v[u] = xs[e[u] & kColMask];
+            }
+#pragma unroll
+            for (u32 u = 0; u < 8; ++u) sum = k0 + u < len ? fma(r[u], v[u], sum) : sum;
         }
         return sum;
     }
@@ -977,15 +990,13 @@ __device__ void rebase_view(csr_view<T> &v, char *base) {
 }
 
 constexpr int kTailLdsLevels = 12;
+constexpr u32 kLdsMax = 160 * 1024; // LDS of one workgroup on gfx950
 
-// The same V-cycle with every array of the levels staged in LDS first (one contiguous read of
-// the image of the constant arrays); work vectors live in LDS too.
+// copies the image into the arena and turns the descriptors' offsets into LDS pointers; returns
+// the descriptors (every thread of the workgroup calls it)
 template <typename T>
-__global__ void __launch_bounds__(kTailThreads)
-    k_vcycle_tail_lds(const uint4 *image, u32 image_bytes, int levels,
-                      T shift, T omega, const T *rhs_in, const double *rhs64, const double *scale_ptr,
-                      T *out, T *top_x, int cycles) {
-    extern __shared__ uint4 arena[];
+__device__ __forceinline__ level_desc<T> *stage_tail(const uint4 *image, u32 image_bytes, int levels,
+                                                     uint4 *arena) {
     // the image starts with the level descriptors, pointer fields holding LDS offset + 1
     level_desc<T> *d = reinterpret_cast<level_desc<T> *>(arena);
     char *base = reinterpret_cast<char *>(arena);
@@ -1033,7 +1044,200 @@ __global__ void __launch_bounds__(kTailThreads)
         __builtin_memcpy(d + tid, &lv, sizeof(lv));
     }
     __syncthreads();
+    return d;
+}
+
+// The same V-cycle with every array of the levels staged in LDS first (one contiguous read of
+// the image of the constant arrays); work vectors live in LDS too.
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads)
+    k_vcycle_tail_lds(const uint4 *image, u32 image_bytes, int levels,
+                      T shift, T omega, const T *rhs_in, const double *rhs64, const double *scale_ptr,
+                      T *out, T *top_x, int cycles) {
+    extern __shared__ uint4 arena[];
+    level_desc<T> *d = stage_tail<T>(image, image_bytes, levels, arena);
     tail_body<true>(d, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, top_x, cycles);
+}
+
+// ---- FGMRES in one workgroup: the Krylov loop of a problem of at most kTinyRows states whose
+// whole hierarchy fits the LDS tail (one row per thread; the basis in HBM, read back from L2)
+constexpr int kTinyWaves = kTailThreads / 64;
+constexpr int kTinyStride = kTinyBasis + 2;
+struct tiny_scratch {
+    double red[kTinyWaves][kTinyStride]; // one partial sum per wave and value
+    double h1[kTinyStride], h2[kTinyStride];
+    double g[kTinyBasis + 1], cs[kTinyBasis], sn[kTinyBasis], y[kTinyBasis];
+    double R[kTinyBasis * (kTinyBasis + 1) / 2]; // column j of the triangular factor at j (j + 1) / 2
+    double hn, beta;
+    int stop;
+};
+
+// sums[k] = the workgroup's sum of term(k) for k < cnt (cnt uniform), each in a fixed order: the
+// butterfly within a wave, then the waves in index order. Eight values are reduced at a time so
+// that their loads and shuffles overlap.
+template <typename F>
+__device__ __forceinline__ void tiny_sums(int cnt, F term, tiny_scratch &sc, double *sums) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k0 = 0; k0 < cnt; k0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = k0 + u < cnt ? term(k0 + u) : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (k0 + u < cnt) {
+                const double t = wave_sum(v[u]);
+                if (lane == 0) sc.red[wave][k0 + u] = t;
+            }
+        }
+    }
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < cnt) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kTinyWaves; ++w) t += sc.red[w][threadIdx.x];
+        sums[threadIdx.x] = t;
+    }
+    __syncthreads();
+}
+
+// ((shift I - Q) z)_i from the generator's gather row i (entry order), z in precision ZT
+template <typename ZT>
+__device__ __forceinline__ double tiny_row(const csr_view<double> &A, u32 i, const ZT *z, double shift) {
+    double sum = 0.0, ex;
+    if (A.coded) {
+        const u32 len = A.rlen[i];
+        const u32 base = A.soff[i >> 6] + (i & 63u);
+        ex = A.exit ? A.exit[i] : A.edict[A.rcode[i]];
+        // as the LDS tail's gather: eight entries and eight gathers in flight, entry order kept
+        for (u32 k0 = 0; k0 < len; k0 += 8) {
+            u32 e[8];
+            double r[8], v[8];
+#pragma unroll
+            for (u32 u = 0; u < 8; ++u) e[u] = k0 + u < len ? A.ent[base + (k0 + u) * kSlice] : 0u;
+#pragma unroll
+            for (u32 u = 0; u < 8; ++u) {
+                r[u] = A.vdict[e[u] >> kColBits];
+                v[u] = static_cast<double>(z[e[u] & kColMask]);
+            }
+#pragma unroll
+            for (u32 u = 0; u < 8; ++u) sum = k0 + u < len ? fma(r[u], v[u], sum) : sum;
+        }
+    } else {
+        ex = A.exit[i];
+        for (u32 p = A.ptr[i]; p < A.ptr[i + 1]; ++p) sum += A.val[p] * static_cast<double>(z[A.col[p]]);
+    }
+    return (shift + ex) * static_cast<double>(z[i]) - sum;
+}
+
+// Restarted flexible GMRES(m) (classical Gram-Schmidt applied twice, Givens rotations) with the
+// LDS tail's V-cycle as the preconditioner, every phase separated by workgroup barriers; the same
+// loop, stopping tests and result as the host-driven solve. Every control value is computed by
+// thread 0 into LDS and read by all threads after a barrier, so the branches are uniform.
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads) k_gmres_tiny(tiny_gmres_args a) {
+    extern __shared__ uint4 arena[];
+    level_desc<T> *d = stage_tail<T>(static_cast<const uint4 *>(a.image), a.image_bytes, a.levels, arena);
+    tiny_scratch &sc = *reinterpret_cast<tiny_scratch *>(reinterpret_cast<char *>(arena) +
+                                                         ((a.lds_bytes + 15u) & ~15u));
+    const int tid = threadIdx.x;
+    const bool active = tid < a.n;
+    const u32 i = static_cast<u32>(tid);
+    const i64 ld = a.ld;
+    double *V = a.V;
+    T *Z = static_cast<T *>(a.Z);
+    const T shift_t = static_cast<T>(a.shift), omega_t = static_cast<T>(a.omega);
+    const double tol = a.tolerance;
+    // r = b - (shift I - Q) x (kept in r) and |r|
+    double r = 0.0;
+    auto residual = [&]() {
+        r = active ? a.b[i] - tiny_row(a.A, i, a.x, a.shift) : 0.0;
+        tiny_sums(1, [&](int) { return r * r; }, sc, &sc.beta);
+        return sqrt(sc.beta);
+    };
+    int it = 0, converged = 0;
+    double beta = residual();
+    if (!(beta > tol) || !__builtin_isfinite(beta)) {
+        converged = __builtin_isfinite(beta) ? 1 : 0;
+    } else {
+        while (it < a.max_iterations) {
+            if (active) V[i] = r / beta;
+            if (tid == 0) {
+                sc.g[0] = beta;
+                for (int k = 1; k <= a.m; ++k) sc.g[k] = 0.0;
+            }
+            int j = 0;
+            while (j < a.m && it < a.max_iterations) {
+                ++it;
+                __syncthreads();
+                // z_j = M v_j (the tail reads v_j in its own row order), w = (shift I - Q) z_j
+                T *zj = Z + j * ld;
+                tail_body<true, T>(d, a.levels, shift_t, omega_t, nullptr, V + j * ld, nullptr, zj, nullptr, 1);
+                __syncthreads();
+                double w = active ? tiny_row(a.A, i, zj, a.shift) : 0.0;
+                const int cnt = j + 1;
+                auto proj = [&](int k) { return active ? V[k * ld + i] * w : 0.0; };
+                tiny_sums(cnt, proj, sc, sc.h1);
+                if (active)
+                    for (int k = 0; k < cnt; ++k) w -= V[k * ld + i] * sc.h1[k];
+                tiny_sums(cnt, proj, sc, sc.h2);
+                if (active)
+                    for (int k = 0; k < cnt; ++k) w -= V[k * ld + i] * sc.h2[k];
+                tiny_sums(1, [&](int) { return w * w; }, sc, &sc.hn);
+                const double hn = sqrt(sc.hn);
+                if (active && hn > 0.0) V[(j + 1) * ld + i] = w / hn;
+                if (tid == 0) {
+                    // column j = h1 + h2 and hn, rotated by the previous rotations; a new rotation
+                    // zeroes hn
+                    double *c = sc.h1;
+                    for (int k = 0; k <= j; ++k) c[k] += sc.h2[k];
+                    for (int k = 0; k < j; ++k) {
+                        const double x0 = c[k], x1 = c[k + 1];
+                        c[k] = sc.cs[k] * x0 + sc.sn[k] * x1;
+                        c[k + 1] = -sc.sn[k] * x0 + sc.cs[k] * x1;
+                    }
+                    const double x0 = c[j], dd = hypot(x0, hn);
+                    sc.cs[j] = dd > 0.0 ? x0 / dd : 1.0;
+                    sc.sn[j] = dd > 0.0 ? hn / dd : 0.0;
+                    c[j] = dd;
+                    double *Rj = sc.R + j * (j + 1) / 2;
+                    for (int k = 0; k <= j; ++k) Rj[k] = c[k];
+                    sc.g[j + 1] = -sc.sn[j] * sc.g[j];
+                    sc.g[j] = sc.cs[j] * sc.g[j];
+                    const double est = fabs(sc.g[j + 1]);
+                    sc.stop = !(est > tol) || !(hn > 0.0) || !__builtin_isfinite(hn);
+                }
+                __syncthreads();
+                ++j;
+                if (sc.stop) break;
+            }
+            // y = R^-1 g over the cycle's j columns; x += Z y
+            if (tid == 0) {
+                for (int q = j; q-- > 0;) {
+                    double s = sc.g[q];
+                    for (int k = q + 1; k < j; ++k) s -= sc.R[k * (k + 1) / 2 + q] * sc.y[k];
+                    sc.y[q] = s / sc.R[q * (q + 1) / 2 + q];
+                }
+            }
+            __syncthreads();
+            if (active) {
+                double s = a.x[i];
+                for (int k = 0; k < j; ++k) s += sc.y[k] * static_cast<double>(Z[k * ld + i]);
+                a.x[i] = s;
+            }
+            __syncthreads();
+            beta = residual();
+            if (!__builtin_isfinite(beta)) break;
+            if (!(beta > tol)) {
+                converged = 1;
+                break;
+            }
+        }
+    }
+    if (tid == 0) {
+        a.result[0] = it;
+        a.result[1] = beta;
+        a.result[2] = converged;
+    }
 }
 
 template <typename T>
@@ -1585,6 +1789,25 @@ void vcycle_tail_lds(const void *image, u32 image_bytes, u32 lds_bytes,
                        static_cast<T>(omega), rhs, rhs64, scale, out, top_x, cycles);
 }
 
+u32 gmres_tiny_lds(u32 tail_lds_bytes) {
+    const u64 total = ((static_cast<u64>(tail_lds_bytes) + 15u) & ~u64(15)) + sizeof(tiny_scratch);
+    return total <= kLdsMax ? static_cast<u32>(total) : 0u;
+}
+
+template <typename T>
+void gmres_tiny(const tiny_gmres_args &a, hipStream_t s) {
+    const u32 lds = gmres_tiny_lds(a.lds_bytes);
+    if (lds == 0 || a.n > kTinyRows || a.m > kTinyBasis || a.m < 1 || a.levels > kTailLdsLevels)
+        throw std::invalid_argument("gmres_tiny: the problem does not fit one workgroup");
+    static const bool attr = [] {
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_gmres_tiny<T>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(kLdsMax)) == hipSuccess;
+    }();
+    if (!attr) throw std::runtime_error("gmres_tiny: cannot raise the kernel's LDS limit");
+    hipLaunchKernelGGL((k_gmres_tiny<T>), dim3(1), dim3(kTailThreads), lds, s, a);
+}
+
 template <typename T>
 void fill(i64 n, T value, T *x, hipStream_t s) {
     if (n == 0) return;
@@ -1708,6 +1931,7 @@ template void shifted_apply<float>(const csr_view<double> &, double, const float
     template void vcycle_tail<T>(const level_desc<T> *, int, double, double, const T *,         \
                                  const double *, const double *, T *, hipStream_t, int);         \
     template bool vcycle_tail_lds_fits<T>(u32, int);                                              \
+    template void gmres_tiny<T>(const tiny_gmres_args &, hipStream_t);                           \
     template void vcycle_tail_lds<T>(const void *, u32, u32, int,      \
                                      double, double, const T *, const double *, const double *,  \
                                      T *, T *, hipStream_t, int);                                  \

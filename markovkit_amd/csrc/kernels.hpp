@@ -256,6 +256,35 @@ void vcycle_tail_lds(const void *image, u32 image_bytes, u32 lds_bytes,
                      int levels, double shift, double omega, const T *rhs, const double *rhs64,
                      const double *scale, T *out, T *top_x, hipStream_t s, int cycles = 1);
 
+// Flexible GMRES(m) on (shift I - Q) x = b in one workgroup, preconditioned by the LDS tail of a
+// hierarchy that the tail holds whole (its image as for vcycle_tail_lds, top level included):
+// one row per thread (n <= kTinyRows), m <= kTinyBasis, the Krylov basis V (FP64, m + 1 vectors)
+// and the preconditioned vectors Z (precision T, m vectors) in HBM with stride ld. Classical
+// Gram-Schmidt twice with workgroup sums in a fixed order; restarts, stopping tests and the
+// result (iterations, residual, converged as doubles in result[0 .. 2]) as the host-driven loop.
+constexpr int kTinyRows = kTailThreads;
+constexpr int kTinyBasis = 64;
+struct tiny_gmres_args {
+    csr_view<double> A;        // the generator's gather rows
+    double shift = 0.0;
+    const double *b = nullptr;
+    double *x = nullptr;       // the initial guess in, the solution out
+    double *V = nullptr;
+    void *Z = nullptr;
+    i64 ld = 0;
+    int n = 0, m = 0, max_iterations = 0;
+    double tolerance = 0.0;
+    double *result = nullptr;
+    const void *image = nullptr; // the tail's image, its size and the LDS its levels take
+    u32 image_bytes = 0, lds_bytes = 0;
+    int levels = 0;
+    double omega = 0.0;
+};
+// the dynamic LDS of the one-workgroup solve over a tail of tail_lds_bytes; 0 when above 160 KB
+u32 gmres_tiny_lds(u32 tail_lds_bytes);
+template <typename T>
+void gmres_tiny(const tiny_gmres_args &a, hipStream_t s);
+
 template <typename T>
 void fill(i64 n, T value, T *x, hipStream_t s);
 // out = (B)(in * scale), scale read from device memory (1 when null)

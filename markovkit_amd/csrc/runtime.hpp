@@ -457,6 +457,16 @@ inline std::atomic<int> &fused_first_switch() {
     return on;
 }
 
+// FGMRES in one workgroup for problems whose whole hierarchy fits the LDS tail (default on;
+// MK_TINY_GMRES=0 turns it off): the same iteration, not the same rounding, as the host loop
+inline std::atomic<int> &tiny_gmres_switch() {
+    static std::atomic<int> on{[] {
+        const char *env = std::getenv("MK_TINY_GMRES");
+        return (env && env[0] == '0') ? 0 : 1;
+    }()};
+    return on;
+}
+
 // a direct coarse level's dense solve fused with the prolongation above it (default on;
 // MK_COARSE_FUSE=0 turns it off); the same values either way
 inline std::atomic<int> &coarse_fuse_switch() {
@@ -627,6 +637,7 @@ struct precond {
     double shift = 0.0;
     int precision = MK_FP64;
     std::function<void(const double *v, void *z, const double *scale)> apply;
+    mk_sector_mg_s *mg = nullptr; // a sector hierarchy behind apply (null: another preconditioner)
 };
 
 // FGMRES on (shift I - Q) x = b with b, x in device memory (M null: unpreconditioned)

@@ -674,92 +674,36 @@ __global__ void __launch_bounds__(ONE_WG ? kTailThreads : kThreads)
     }
 }
 
-// The fused tail runs from HBM (LDS = false) or with every level array staged in LDS
-// (LDS = true). Pointers reach the tail through descriptors, so the compiler cannot see their
-// address space; lds() states it where an array is known to live in LDS, which turns flat
-// accesses into LDS instructions.
-template <bool LDS, typename P>
-__device__ __forceinline__ P *lds(P *p) {
-#if __HIP_DEVICE_COMPILE__
-    if constexpr (LDS) __builtin_assume(__builtin_amdgcn_is_shared(reinterpret_cast<const void *>(p)));
-#endif
-    return p;
-}
-
-// gather_row and row_exit for the tail: the same sums in the same order (an LDS tail holds
-// coded operators only)
-template <bool LDS, typename T>
-__device__ __forceinline__ T tail_exit(const csr_view<T> &A, i64 r) {
-    if constexpr (!LDS) {
-        return row_exit(A, r);
-    } else {
-        return A.exit ? lds<LDS>(A.exit)[r] : lds<LDS>(A.edict)[lds<LDS>(A.rcode)[r]];
-    }
-}
-
-template <bool LDS, typename T>
-__device__ __forceinline__ T tail_gather(const csr_view<T> &A, u32 i, const T *x, T &exit_i) {
-    if constexpr (!LDS) {
-        return gather_row(A, i, x, exit_i);
-    } else {
-        const u32 len = lds<LDS>(A.rlen)[i];
-        const u32 base = lds<LDS>(A.soff)[i >> 6] + (i & 63u);
-        exit_i = tail_exit<LDS>(A, i);
-        const u32 *ent = lds<LDS>(A.ent);
-        const T *vdict = lds<LDS>(A.vdict);
-        const T *xs = lds<LDS>(x);
-        T sum = 0;
-        // eight entries, then their eight gathers, in flight at once: one workgroup runs the tail,
-        // so a row's chain of dependent LDS reads is the phase's latency. Past the row's end the
-        // gathers read entry 0 of both arrays (the row has entries, so both exist) and are not
-        // added: the sum keeps entry order, each term fused into it as sum += rate * x compiles
-        for (u32 k0 = 0; k0 < len; k0 += 8) {
-            u32 e[8];
-            T r[8], v[8];
-#pragma unroll
-            for (u32 u = 0; u < 8; ++u) e[u] = k0 + u < len ? ent[base + (k0 + u) * kSlice] : 0u;
-#pragma unroll
-            for (u32 u = 0; u < 8; ++u) {
-                r[u] = vdict[e[u] >> kColBits];
-                v[u] = xs[e[u] & kColMask];
-            }
-#pragma unroll
-            for (u32 u = 0; u < 8; ++u) sum = k0 + u < len ? fma(r[u], v[u], sum) : sum;
-        }
-        return sum;
-    }
-}
-
 // one Gauss-Seidel half-sweep of the rows of one colour (listed in rows), in place, by one
 // workgroup
-template <bool LDS, typename T>
+template <typename T>
 __device__ __forceinline__ void tail_gs_half(const csr_view<T> &A, const u32 *rows, i64 nrows, T shift,
                                              const T *rhs, T *x) {
     for (i64 q = threadIdx.x; q < nrows; q += blockDim.x) {
-        const u32 i = lds<LDS>(rows)[q];
+        const u32 i = rows[q];
         T ex = 0;
-        const T sum = tail_gather<LDS>(A, i, x, ex);
-        lds<LDS>(x)[i] = (lds<LDS>(rhs)[i] + sum) / (shift + ex);
+        const T sum = gather_row(A, i, x, ex);
+        x[i] = (rhs[i] + sum) / (shift + ex);
     }
 }
 
 // dense terminal blocks: x[block rows] (+)= inv_b r[block rows]
-template <bool LDS, bool ASSIGN, typename T>
+template <bool ASSIGN, typename T>
 __device__ __forceinline__ void tail_terminal(const level_desc<T> &lv, const T *r, T *x) {
     for (i64 g = threadIdx.x; g < lv.n_term * lv.term_max; g += blockDim.x) {
         const i64 b = g / lv.term_max, i = g % lv.term_max;
-        const u32 m = lds<LDS>(lv.tsize)[b];
+        const u32 m = lv.tsize[b];
         if (i >= m) continue;
-        const u32 st = lds<LDS>(lv.tstart)[b];
-        const T *row = lds<LDS>(lv.tinv) + lds<LDS>(lv.toff)[b] + static_cast<u64>(i) * m;
-        const T *rr = lds<LDS>(r) + st;
+        const u32 st = lv.tstart[b];
+        const T *row = lv.tinv + lv.toff[b] + static_cast<u64>(i) * m;
+        const T *rr = r + st;
         T acc = 0;
 #pragma unroll 8
         for (u32 j = 0; j < m; ++j) acc += row[j] * rr[j];
         if constexpr (ASSIGN) {
-            lds<LDS>(x)[st + i] = acc;
+            x[st + i] = acc;
         } else {
-            lds<LDS>(x)[st + i] += acc;
+            x[st + i] += acc;
         }
     }
 }
@@ -767,7 +711,7 @@ __device__ __forceinline__ void tail_terminal(const level_desc<T> &lv, const T *
 // The V-cycle of levels L[0 .. levels) by one workgroup; the top level's result goes to out (in
 // the caller's row order) or, without out, to top_x. rhs_in, rhs64, out and top_x are in HBM;
 // the top level's right-hand side is first copied to L[0].rhs.
-template <bool LDS, typename T>
+template <typename T>
 __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T shift, T omega,
                                           const T *rhs_in, const double *rhs64, const double *scale_ptr,
                                           T *out, T *top_x, int cycles) {
@@ -789,7 +733,7 @@ __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T 
             const u8 *colour = lv.colour;
             if (lv.direct && k > 0) {
                 // the cycle from zero on a level of terminal blocks only is their dense solve
-                tail_terminal<LDS, true>(lv, rhs, x);
+                tail_terminal<true>(lv, rhs, x);
                 __syncthreads();
                 continue;
             }
@@ -798,43 +742,43 @@ __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T 
             for (i64 i = tid; i < lv.n; i += nt) {
                 T b;
                 if (k == 0 && cyc == 0) {
-                    const i64 src = perm0 ? lds<LDS>(perm0)[i] : i;
+                    const i64 src = perm0 ? perm0[i] : i;
                     b = rhs64 ? static_cast<T>(rhs64[src] * scale) : rhs_in[src];
-                    lds<LDS>(lv.rhs)[i] = b;
+                    lv.rhs[i] = b;
                 } else {
-                    b = lds<LDS>(rhs)[i];
+                    b = rhs[i];
                 }
                 if (!colour) {
-                    lds<LDS>(x)[i] = lv.pre > 0 ? omega * b / (shift + tail_exit<LDS>(lv.A, i)) : T(0);
+                    x[i] = lv.pre > 0 ? omega * b / (shift + row_exit(lv.A, i)) : T(0);
                 } else if (lv.pre == 0) {
-                    lds<LDS>(x)[i] = T(0);
-                } else if (lds<LDS>(colour)[i] == 0) {
-                    lds<LDS>(x)[i] = b / (shift + tail_exit<LDS>(lv.A, i));
+                    x[i] = T(0);
+                } else if (colour[i] == 0) {
+                    x[i] = b / (shift + row_exit(lv.A, i));
                 }
             }
             __syncthreads();
             if (colour) {
                 for (int half = 1; half < 2 * lv.pre; ++half) {
                     const int c = half & 1;
-                    tail_gs_half<LDS>(lv.A, lv.crows[c], lv.ncrows[c], shift, rhs, x);
+                    tail_gs_half(lv.A, lv.crows[c], lv.ncrows[c], shift, rhs, x);
                     __syncthreads();
                 }
                 if (lv.pre > 0) {
                     // after the last colour-1 half-sweep only colour-0 rows have a residual
                     for (i64 q = tid; q < lv.ncrows[0]; q += nt) {
-                        const u32 i = lds<LDS>(lv.crows[0])[q];
+                        const u32 i = lv.crows[0][q];
                         T ex = 0;
-                        const T sum = tail_gather<LDS>(lv.A, i, x, ex);
+                        const T sum = gather_row(lv.A, i, x, ex);
                         const T d = shift + ex;
-                        lds<LDS>(lv.res)[i] = lds<LDS>(rhs)[i] - (d * lds<LDS>(x)[i] - sum);
+                        lv.res[i] = rhs[i] - (d * x[i] - sum);
                     }
-                    for (i64 q = tid; q < lv.ncrows[1]; q += nt) lds<LDS>(lv.res)[lds<LDS>(lv.crows[1])[q]] = T(0);
+                    for (i64 q = tid; q < lv.ncrows[1]; q += nt) lv.res[lv.crows[1][q]] = T(0);
                 } else {
                     for (i64 i = tid; i < lv.n; i += nt) {
                         T ex = 0;
-                        const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
+                        const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
                         const T d = shift + ex;
-                        lds<LDS>(lv.res)[i] = lds<LDS>(rhs)[i] - (d * lds<LDS>(x)[i] - sum);
+                        lv.res[i] = rhs[i] - (d * x[i] - sum);
                     }
                 }
                 __syncthreads();
@@ -842,12 +786,12 @@ __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T 
             for (int p = 1; !colour && p < lv.pre; ++p) {
                 for (i64 i = tid; i < lv.n; i += nt) {
                     T ex = 0;
-                    const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
-                    const T xr = lds<LDS>(x)[i];
+                    const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
+                    const T xr = x[i];
                     const T d = shift + ex;
                     const T ax = d * xr - sum;
-                    const T res = lds<LDS>(rhs)[i] - ax;
-                    lds<LDS>(xt)[i] = xr + omega * res / d;
+                    const T res = rhs[i] - ax;
+                    xt[i] = xr + omega * res / d;
                 }
                 __syncthreads();
                 T *t = x;
@@ -856,20 +800,20 @@ __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T 
             }
             for (i64 i = tid; !colour && i < lv.n; i += nt) {
                 T ex = 0;
-                const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
+                const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
                 const T d = shift + ex;
-                lds<LDS>(lv.res)[i] = lds<LDS>(rhs)[i] - (d * lds<LDS>(x)[i] - sum);
+                lv.res[i] = rhs[i] - (d * x[i] - sum);
             }
             __syncthreads();
             if (lv.n_term > 0) {
-                tail_terminal<LDS, false>(lv, lv.res, x);
+                tail_terminal<false>(lv, lv.res, x);
                 __syncthreads();
             }
             if (lv.n_next > 0) {
                 T *crhs = L[k + 1].rhs;
                 for (i64 c = tid; c < lv.n_next; c += nt) {
                     T ex = 0;
-                    lds<LDS>(crhs)[c] = tail_gather<LDS>(lv.R, static_cast<u32>(c), lv.res, ex);
+                    crhs[c] = gather_row(lv.R, static_cast<u32>(c), lv.res, ex);
                 }
                 __syncthreads();
             }
@@ -887,16 +831,16 @@ __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T 
                 const bool red_only = lv.colour && lv.post > 0;
                 const i64 np = red_only ? lv.ncrows[0] : lv.n;
                 for (i64 q = tid; q < np; q += nt) {
-                    const i64 i = red_only ? lds<LDS>(lv.crows[0])[q] : q;
-                    const u32 c = lds<LDS>(lv.coarse_of)[i];
-                    if (c != 0xFFFFFFFFu) lds<LDS>(x)[i] += lds<LDS>(lv.weight)[i] * lds<LDS>(cx)[c];
+                    const i64 i = red_only ? lv.crows[0][q] : q;
+                    const u32 c = lv.coarse_of[i];
+                    if (c != 0xFFFFFFFFu) x[i] += lv.weight[i] * cx[c];
                 }
                 __syncthreads();
             }
             for (int half = 0; lv.colour && half < 2 * lv.post; ++half) {
                 // reverse colour order: 1, 0, 1, 0, ...
                 const int c = 1 - (half & 1);
-                tail_gs_half<LDS>(lv.A, lv.crows[c], lv.ncrows[c], shift, rhs, x);
+                tail_gs_half(lv.A, lv.crows[c], lv.ncrows[c], shift, rhs, x);
                 __syncthreads();
             }
             for (int p = 0; !lv.colour && p < lv.post; ++p) {
@@ -904,11 +848,11 @@ __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T 
                 T *dst = (k == 0 && p == lv.post - 1 && vout) ? vout : xt;
                 for (i64 i = tid; i < lv.n; i += nt) {
                     T ex = 0;
-                    const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), x, ex);
-                    const T xr = lds<LDS>(x)[i];
+                    const T sum = gather_row(lv.A, static_cast<u32>(i), x, ex);
+                    const T xr = x[i];
                     const T d = shift + ex;
                     const T ax = d * xr - sum;
-                    const T res = lds<LDS>(rhs)[i] - ax;
+                    const T res = rhs[i] - ax;
                     dst[i] = xr + omega * res / d;
                 }
                 __syncthreads();
@@ -934,22 +878,22 @@ __device__ __forceinline__ void tail_body(const level_desc<T> *L, int levels, T 
             const bool last = cyc + 1 == cycles;
             T *fin = out ? out : top_x;
             for (i64 i = tid; i < lv.n; i += nt) {
-                const T t = lds<LDS>(top)[i];
+                const T t = top[i];
                 if (cyc == 0) {
-                    lds<LDS>(acc)[i] = t;
+                    acc[i] = t;
                 } else if (!last) {
-                    lds<LDS>(acc)[i] += t;
+                    acc[i] += t;
                 } else {
-                    fin[i] = lds<LDS>(acc)[i] + t;
+                    fin[i] = acc[i] + t;
                 }
             }
             __syncthreads();
             if (!last) {
                 for (i64 i = tid; i < lv.n; i += nt) {
                     T ex = 0;
-                    const T sum = tail_gather<LDS>(lv.A, static_cast<u32>(i), acc, ex);
+                    const T sum = gather_row(lv.A, static_cast<u32>(i), acc, ex);
                     const T d = shift + ex;
-                    lds<LDS>(lv.rhs)[i] = rhs_in[i] - (d * lds<LDS>(acc)[i] - sum);
+                    lv.rhs[i] = rhs_in[i] - (d * acc[i] - sum);
                 }
                 __syncthreads();
             }
@@ -969,37 +913,152 @@ template <typename T>
 __global__ void __launch_bounds__(kTailThreads)
     k_vcycle_tail(const level_desc<T> *L, int levels, T shift, T omega, const T *rhs_in,
                   const double *rhs64, const double *scale_ptr, T *out, int cycles) {
-    tail_body<false>(L, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, L[0].x, cycles);
-}
-
-// pointer fields of an LDS level descriptor hold (offset + 1) into the LDS arena (0: null)
-template <typename P>
-__device__ __forceinline__ P *rebase(P *field, char *base) {
-    const auto off = reinterpret_cast<uintptr_t>(field);
-    return off ? reinterpret_cast<P *>(base + (off - 1)) : nullptr;
-}
-
-template <typename T>
-__device__ void rebase_view(csr_view<T> &v, char *base) {
-    v.soff = rebase(v.soff, base);
-    v.ent = rebase(v.ent, base);
-    v.rlen = rebase(v.rlen, base);
-    v.rcode = rebase(v.rcode, base);
-    v.vdict = rebase(v.vdict, base);
-    v.edict = rebase(v.edict, base);
+    tail_body(L, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, L[0].x, cycles);
 }
 
 constexpr int kTailLdsLevels = 12;
 constexpr u32 kLdsMax = 160 * 1024; // LDS of one workgroup on gfx950
 
-// copies the image into the arena and turns the descriptors' offsets into LDS pointers; returns
-// the descriptors (every thread of the workgroup calls it)
+// ---- The tail with every level array in LDS. Each level's arrays are addressed by 32-bit LDS
+// addresses held in scalar registers (read once per level visit), so a phase's chain of
+// dependent reads starts at the row data: no descriptor reloads after barriers, no generic
+// pointers.
+template <typename P>
+using lptr = __attribute__((address_space(3))) P *;
+constexpr u32 kNoLds = 0xFFFFFFFFu; // an absent array
+
+template <typename P>
+__device__ __forceinline__ lptr<P> at(u32 a) {
+    return (lptr<P>)(a);
+}
+
+// a coded operator in LDS; exit != kNoLds: per-row exit rates, else edict[rcode[row]]
+struct lds_op {
+    u32 rlen, soff, ent, vdict, edict, rcode, exit;
+};
+
 template <typename T>
-__device__ __forceinline__ level_desc<T> *stage_tail(const uint4 *image, u32 image_bytes, int levels,
-                                                     uint4 *arena) {
-    // the image starts with the level descriptors, pointer fields holding LDS offset + 1
-    level_desc<T> *d = reinterpret_cast<level_desc<T> *>(arena);
-    char *base = reinterpret_cast<char *>(arena);
+struct lds_level {
+    lds_op A, R;
+    u32 coarse_of, weight, tstart, tsize, toff, tinv, rhs, x, xt, res, colour, crows[2], perm, acc;
+    u32 n, n_next, ncrows[2], n_term, term_max;
+    int pre, post, direct, pad;
+};
+
+// a level's addresses in scalar registers (every field is the same in all lanes)
+template <typename T>
+__device__ __forceinline__ lds_level<T> uniform_level(const lds_level<T> *p) {
+    auto u = [](u32 v) { return static_cast<u32>(__builtin_amdgcn_readfirstlane(static_cast<int>(v))); };
+    lds_level<T> s;
+    s.A.rlen = u(p->A.rlen);
+    s.A.soff = u(p->A.soff);
+    s.A.ent = u(p->A.ent);
+    s.A.vdict = u(p->A.vdict);
+    s.A.edict = u(p->A.edict);
+    s.A.rcode = u(p->A.rcode);
+    s.A.exit = u(p->A.exit);
+    s.R.rlen = u(p->R.rlen);
+    s.R.soff = u(p->R.soff);
+    s.R.ent = u(p->R.ent);
+    s.R.vdict = u(p->R.vdict);
+    s.R.edict = u(p->R.edict);
+    s.R.rcode = u(p->R.rcode);
+    s.R.exit = u(p->R.exit);
+    s.coarse_of = u(p->coarse_of);
+    s.weight = u(p->weight);
+    s.tstart = u(p->tstart);
+    s.tsize = u(p->tsize);
+    s.toff = u(p->toff);
+    s.tinv = u(p->tinv);
+    s.rhs = u(p->rhs);
+    s.x = u(p->x);
+    s.xt = u(p->xt);
+    s.res = u(p->res);
+    s.colour = u(p->colour);
+    s.crows[0] = u(p->crows[0]);
+    s.crows[1] = u(p->crows[1]);
+    s.perm = u(p->perm);
+    s.acc = u(p->acc);
+    s.n = u(p->n);
+    s.n_next = u(p->n_next);
+    s.ncrows[0] = u(p->ncrows[0]);
+    s.ncrows[1] = u(p->ncrows[1]);
+    s.n_term = u(p->n_term);
+    s.term_max = u(p->term_max);
+    s.pre = static_cast<int>(u(static_cast<u32>(p->pre)));
+    s.post = static_cast<int>(u(static_cast<u32>(p->post)));
+    s.direct = static_cast<int>(u(static_cast<u32>(p->direct)));
+    s.pad = 0;
+    return s;
+}
+
+template <typename T>
+__device__ __forceinline__ T lds_exit(const lds_op &A, u32 i) {
+    return A.exit != kNoLds ? at<T>(A.exit)[i] : at<T>(A.edict)[at<u8>(A.rcode)[i]];
+}
+
+// tail_gather's sum over row i's entries in entry order, eight entries then their eight
+// gathers in flight; past the row's end the gathers read entry 0 of both arrays (the row has
+// entries, so both exist) and are not added
+template <typename T>
+__device__ __forceinline__ T lds_gather(const lds_op &A, u32 i, u32 x, T &exit_i) {
+    const u32 len = at<u8>(A.rlen)[i];
+    const u32 base = at<u32>(A.soff)[i >> 6] + (i & 63u);
+    exit_i = lds_exit<T>(A, i);
+    const lptr<u32> ent = at<u32>(A.ent);
+    const lptr<T> vdict = at<T>(A.vdict), xs = at<T>(x);
+    T sum = 0;
+    for (u32 k0 = 0; k0 < len; k0 += 8) {
+        u32 e[8];
+        T r[8], v[8];
+#pragma unroll
+        for (u32 u = 0; u < 8; ++u) e[u] = k0 + u < len ? ent[base + (k0 + u) * kSlice] : 0u;
+#pragma unroll
+        for (u32 u = 0; u < 8; ++u) {
+            r[u] = vdict[e[u] >> kColBits];
+            v[u] = xs[e[u] & kColMask];
+        }
+#pragma unroll
+        for (u32 u = 0; u < 8; ++u) sum = k0 + u < len ? fma(r[u], v[u], sum) : sum;
+    }
+    return sum;
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_gs_half(const lds_op &A, u32 rows, u32 nrows, T shift, u32 rhs, u32 x) {
+    for (u32 q = threadIdx.x; q < nrows; q += blockDim.x) {
+        const u32 i = at<u32>(rows)[q];
+        T ex = 0;
+        const T sum = lds_gather<T>(A, i, x, ex);
+        at<T>(x)[i] = (at<T>(rhs)[i] + sum) / (shift + ex);
+    }
+}
+
+template <bool ASSIGN, typename T>
+__device__ __forceinline__ void lds_terminal(const lds_level<T> &lv, u32 r, u32 x) {
+    for (u32 g = threadIdx.x; g < lv.n_term * lv.term_max; g += blockDim.x) {
+        const u32 b = g / lv.term_max, i = g % lv.term_max;
+        const u32 m = at<u32>(lv.tsize)[b];
+        if (i >= m) continue;
+        const u32 st = at<u32>(lv.tstart)[b];
+        const lptr<T> row = at<T>(lv.tinv) + at<u64>(lv.toff)[b] + static_cast<u64>(i) * m;
+        const lptr<T> rr = at<T>(r) + st;
+        T acc = 0;
+#pragma unroll 8
+        for (u32 j = 0; j < m; ++j) acc += row[j] * rr[j];
+        if constexpr (ASSIGN) {
+            at<T>(x)[st + i] = acc;
+        } else {
+            at<T>(x)[st + i] += acc;
+        }
+    }
+}
+
+// Copies the image into the arena and writes each level's LDS addresses over the start of the
+// descriptors; returns them (every thread of the workgroup calls it)
+template <typename T>
+__device__ __forceinline__ const lds_level<T> *stage_tail(const uint4 *image, u32 image_bytes, int levels,
+                                                          uint4 *arena) {
     const int tid = threadIdx.x;
     const u32 nt = blockDim.x;
     // stage the image: eight 16-byte loads per thread in flight before the LDS stores
@@ -1022,29 +1081,256 @@ __device__ __forceinline__ level_desc<T> *stage_tail(const uint4 *image, u32 ima
         st(b0 + 7 * nt, v7);
     }
     __syncthreads();
+    // the image starts with the level descriptors, pointer fields holding LDS offset + 1
+    const u32 a0 = static_cast<u32>(reinterpret_cast<uintptr_t>((lptr<uint4>)arena));
+    auto adr = [&](const void *field) {
+        const auto off = reinterpret_cast<uintptr_t>(field);
+        return off ? a0 + static_cast<u32>(off - 1) : kNoLds;
+    };
+    auto op = [&](const csr_view<T> &v) {
+        return lds_op{adr(v.rlen), adr(v.soff), adr(v.ent), adr(v.vdict), adr(v.edict), adr(v.rcode),
+                      adr(v.exit)};
+    };
+    lds_level<T> s{};
     if (tid < levels) {
-        level_desc<T> lv = d[tid];
-        rebase_view(lv.A, base);
-        rebase_view(lv.R, base);
-        lv.coarse_of = rebase(lv.coarse_of, base);
-        lv.weight = rebase(lv.weight, base);
-        lv.tstart = rebase(lv.tstart, base);
-        lv.tsize = rebase(lv.tsize, base);
-        lv.toff = rebase(lv.toff, base);
-        lv.tinv = rebase(lv.tinv, base);
-        lv.rhs = rebase(lv.rhs, base);
-        lv.x = rebase(lv.x, base);
-        lv.xt = rebase(lv.xt, base);
-        lv.res = rebase(lv.res, base);
-        lv.colour = rebase(lv.colour, base);
-        lv.crows[0] = rebase(lv.crows[0], base);
-        lv.crows[1] = rebase(lv.crows[1], base);
-        lv.perm = rebase(lv.perm, base);
-        lv.acc = rebase(lv.acc, base);
-        __builtin_memcpy(d + tid, &lv, sizeof(lv));
+        const level_desc<T> &lv = reinterpret_cast<const level_desc<T> *>(arena)[tid];
+        s.A = op(lv.A);
+        s.R = op(lv.R);
+        s.coarse_of = adr(lv.coarse_of);
+        s.weight = adr(lv.weight);
+        s.tstart = adr(lv.tstart);
+        s.tsize = adr(lv.tsize);
+        s.toff = adr(lv.toff);
+        s.tinv = adr(lv.tinv);
+        s.rhs = adr(lv.rhs);
+        s.x = adr(lv.x);
+        s.xt = adr(lv.xt);
+        s.res = adr(lv.res);
+        s.colour = adr(lv.colour);
+        s.crows[0] = adr(lv.crows[0]);
+        s.crows[1] = adr(lv.crows[1]);
+        s.perm = adr(lv.perm);
+        s.acc = adr(lv.acc);
+        s.n = static_cast<u32>(lv.n);
+        s.n_next = static_cast<u32>(lv.n_next);
+        s.ncrows[0] = static_cast<u32>(lv.ncrows[0]);
+        s.ncrows[1] = static_cast<u32>(lv.ncrows[1]);
+        s.n_term = static_cast<u32>(lv.n_term);
+        s.term_max = static_cast<u32>(lv.term_max);
+        s.pre = lv.pre;
+        s.post = lv.post;
+        s.direct = lv.direct;
     }
     __syncthreads();
-    return d;
+    lds_level<T> *out = reinterpret_cast<lds_level<T> *>(arena);
+    if (tid < levels) out[tid] = s;
+    __syncthreads();
+    return out;
+}
+
+// tail_body with the levels in LDS: the same phases, formulas and summation orders
+template <typename T>
+__device__ __forceinline__ void tail_body_lds(const lds_level<T> *L, int levels, T shift, T omega,
+                                              const T *rhs_in, const double *rhs64, const double *scale_ptr,
+                                              T *out, T *top_x, int cycles) {
+    const u32 tid = threadIdx.x, nt = blockDim.x;
+    const double scale = scale_ptr ? *scale_ptr : 1.0;
+    const lds_level<T> l0 = uniform_level(L);
+    const u32 perm0 = l0.perm; // the top level's row order differs from the caller's
+    u32 top = kNoLds;          // the top level's result (LDS), unless top_out
+    bool top_out = false;      // the top level's last Jacobi sweep wrote out
+    for (int cyc = 0; cyc < cycles; ++cyc) {
+        T *vout = cycles == 1 ? out : nullptr;
+        u32 flip = 0;
+        for (int k = 0; k < levels; ++k) {
+            const lds_level<T> lv = uniform_level(L + k);
+            const u32 rhs = lv.rhs;
+            u32 x = lv.x, xt = lv.xt;
+            const bool colour = lv.colour != kNoLds;
+            if (lv.direct && k > 0) {
+                lds_terminal<true>(lv, rhs, x);
+                __syncthreads();
+                continue;
+            }
+            for (u32 i = tid; i < lv.n; i += nt) {
+                T b;
+                if (k == 0 && cyc == 0) {
+                    const u32 src = perm0 != kNoLds ? at<u32>(perm0)[i] : i;
+                    b = rhs64 ? static_cast<T>(rhs64[src] * scale) : rhs_in[src];
+                    at<T>(lv.rhs)[i] = b;
+                } else {
+                    b = at<T>(rhs)[i];
+                }
+                if (!colour) {
+                    at<T>(x)[i] = lv.pre > 0 ? omega * b / (shift + lds_exit<T>(lv.A, i)) : T(0);
+                } else if (lv.pre == 0) {
+                    at<T>(x)[i] = T(0);
+                } else if (at<u8>(lv.colour)[i] == 0) {
+                    at<T>(x)[i] = b / (shift + lds_exit<T>(lv.A, i));
+                }
+            }
+            __syncthreads();
+            if (colour) {
+                for (int half = 1; half < 2 * lv.pre; ++half) {
+                    const int c = half & 1;
+                    lds_gs_half<T>(lv.A, c ? lv.crows[1] : lv.crows[0], c ? lv.ncrows[1] : lv.ncrows[0], shift, rhs, x);
+                    __syncthreads();
+                }
+                if (lv.pre > 0) {
+                    for (u32 q = tid; q < lv.ncrows[0]; q += nt) {
+                        const u32 i = at<u32>(lv.crows[0])[q];
+                        T ex = 0;
+                        const T sum = lds_gather<T>(lv.A, i, x, ex);
+                        const T d = shift + ex;
+                        at<T>(lv.res)[i] = at<T>(rhs)[i] - (d * at<T>(x)[i] - sum);
+                    }
+                    for (u32 q = tid; q < lv.ncrows[1]; q += nt) at<T>(lv.res)[at<u32>(lv.crows[1])[q]] = T(0);
+                } else {
+                    for (u32 i = tid; i < lv.n; i += nt) {
+                        T ex = 0;
+                        const T sum = lds_gather<T>(lv.A, i, x, ex);
+                        const T d = shift + ex;
+                        at<T>(lv.res)[i] = at<T>(rhs)[i] - (d * at<T>(x)[i] - sum);
+                    }
+                }
+                __syncthreads();
+            }
+            for (int p = 1; !colour && p < lv.pre; ++p) {
+                for (u32 i = tid; i < lv.n; i += nt) {
+                    T ex = 0;
+                    const T sum = lds_gather<T>(lv.A, i, x, ex);
+                    const T xr = at<T>(x)[i];
+                    const T d = shift + ex;
+                    const T ax = d * xr - sum;
+                    const T res = at<T>(rhs)[i] - ax;
+                    at<T>(xt)[i] = xr + omega * res / d;
+                }
+                __syncthreads();
+                const u32 t = x;
+                x = xt;
+                xt = t;
+            }
+            if (!colour) {
+                for (u32 i = tid; i < lv.n; i += nt) {
+                    T ex = 0;
+                    const T sum = lds_gather<T>(lv.A, i, x, ex);
+                    const T d = shift + ex;
+                    at<T>(lv.res)[i] = at<T>(rhs)[i] - (d * at<T>(x)[i] - sum);
+                }
+            }
+            __syncthreads();
+            if (lv.n_term > 0) {
+                lds_terminal<false>(lv, lv.res, x);
+                __syncthreads();
+            }
+            if (lv.n_next > 0) {
+                const u32 crhs = L[k + 1].rhs;
+                const u32 ch = __builtin_amdgcn_readfirstlane(crhs);
+                for (u32 c = tid; c < lv.n_next; c += nt) {
+                    T ex = 0;
+                    at<T>(ch)[c] = lds_gather<T>(lv.R, c, lv.res, ex);
+                }
+                __syncthreads();
+            }
+            if (x == lv.xt) flip |= 1u << k;
+        }
+        for (int k = levels - 1; k >= 0; --k) {
+            const lds_level<T> lv = uniform_level(L + k);
+            const u32 rhs = lv.rhs;
+            const bool f = (flip >> k) & 1u;
+            u32 x = f ? lv.xt : lv.x, xt = f ? lv.x : lv.xt;
+            bool x_out = false;
+            if (lv.direct && k > 0) continue;
+            const bool colour = lv.colour != kNoLds;
+            if (lv.n_next > 0) {
+                const bool fc = (flip >> (k + 1)) & 1u;
+                const u32 cx = __builtin_amdgcn_readfirstlane(fc ? L[k + 1].xt : L[k + 1].x);
+                const bool red_only = colour && lv.post > 0;
+                const u32 np = red_only ? lv.ncrows[0] : lv.n;
+                for (u32 q = tid; q < np; q += nt) {
+                    const u32 i = red_only ? at<u32>(lv.crows[0])[q] : q;
+                    const u32 c = at<u32>(lv.coarse_of)[i];
+                    if (c != 0xFFFFFFFFu) at<T>(x)[i] += at<T>(lv.weight)[i] * at<T>(cx)[c];
+                }
+                __syncthreads();
+            }
+            for (int half = 0; colour && half < 2 * lv.post; ++half) {
+                const int c = 1 - (half & 1);
+                lds_gs_half<T>(lv.A, c ? lv.crows[1] : lv.crows[0], c ? lv.ncrows[1] : lv.ncrows[0], shift, rhs, x);
+                __syncthreads();
+            }
+            for (int p = 0; !colour && p < lv.post; ++p) {
+                // the top level's last sweep may write out (HBM)
+                const bool to_out = k == 0 && p == lv.post - 1 && vout;
+                for (u32 i = tid; i < lv.n; i += nt) {
+                    T ex = 0;
+                    const T sum = lds_gather<T>(lv.A, i, x, ex);
+                    const T xr = at<T>(x)[i];
+                    const T d = shift + ex;
+                    const T ax = d * xr - sum;
+                    const T res = at<T>(rhs)[i] - ax;
+                    const T v = xr + omega * res / d;
+                    if (to_out) {
+                        vout[i] = v;
+                    } else {
+                        at<T>(xt)[i] = v;
+                    }
+                }
+                __syncthreads();
+                if (to_out) {
+                    x_out = true;
+                } else {
+                    const u32 t = x;
+                    x = xt;
+                    xt = t;
+                }
+            }
+            if (k > 0) {
+                flip = (flip & ~(1u << k)) | (x == lv.xt ? 1u << k : 0u);
+            } else {
+                top = x;
+                top_out = x_out;
+            }
+        }
+        if (cycles > 1) {
+            // coarse cycles of the caller's W visit (no permutation: the tail starts below the top
+            // level): acc = the first result, acc += each later correction, the last correction
+            // added on the way out; every later cycle starts from zero on the top level's residual
+            // rhs_in - A acc
+            const bool last = cyc + 1 == cycles;
+            T *fin = out ? out : top_x;
+            const u32 acc = l0.acc;
+            for (u32 i = tid; i < l0.n; i += nt) {
+                const T t = at<T>(top)[i];
+                if (cyc == 0) {
+                    at<T>(acc)[i] = t;
+                } else if (!last) {
+                    at<T>(acc)[i] += t;
+                } else {
+                    fin[i] = at<T>(acc)[i] + t;
+                }
+            }
+            __syncthreads();
+            if (!last) {
+                for (u32 i = tid; i < l0.n; i += nt) {
+                    T ex = 0;
+                    const T sum = lds_gather<T>(l0.A, i, acc, ex);
+                    const T d = shift + ex;
+                    at<T>(l0.rhs)[i] = rhs_in[i] - (d * at<T>(acc)[i] - sum);
+                }
+                __syncthreads();
+            }
+        }
+    }
+    if (cycles > 1) return;
+    // the top level's result in out (in the caller's row order), else in top_x
+    T *dst = out ? out : top_x;
+    if (top_out) return;
+    if (out && perm0 != kNoLds) {
+        for (u32 i = tid; i < l0.n; i += nt) dst[at<u32>(perm0)[i]] = at<T>(top)[i];
+    } else {
+        for (u32 i = tid; i < l0.n; i += nt) dst[i] = at<T>(top)[i];
+    }
 }
 
 // The same V-cycle with every array of the levels staged in LDS first (one contiguous read of
@@ -1055,8 +1341,8 @@ __global__ void __launch_bounds__(kTailThreads)
                       T shift, T omega, const T *rhs_in, const double *rhs64, const double *scale_ptr,
                       T *out, T *top_x, int cycles) {
     extern __shared__ uint4 arena[];
-    level_desc<T> *d = stage_tail<T>(image, image_bytes, levels, arena);
-    tail_body<true>(d, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, top_x, cycles);
+    const lds_level<T> *d = stage_tail<T>(image, image_bytes, levels, arena);
+    tail_body_lds<T>(d, levels, shift, omega, rhs_in, rhs64, scale_ptr, out, top_x, cycles);
 }
 
 // ---- FGMRES in one workgroup: the Krylov loop of a problem of at most kTinyRows states whose
@@ -1136,7 +1422,7 @@ __device__ __forceinline__ double tiny_row(const csr_view<double> &A, u32 i, con
 template <typename T>
 __global__ void __launch_bounds__(kTailThreads) k_gmres_tiny(tiny_gmres_args a) {
     extern __shared__ uint4 arena[];
-    level_desc<T> *d = stage_tail<T>(static_cast<const uint4 *>(a.image), a.image_bytes, a.levels, arena);
+    const lds_level<T> *d = stage_tail<T>(static_cast<const uint4 *>(a.image), a.image_bytes, a.levels, arena);
     tiny_scratch &sc = *reinterpret_cast<tiny_scratch *>(reinterpret_cast<char *>(arena) +
                                                          ((a.lds_bytes + 15u) & ~15u));
     const int tid = threadIdx.x;
@@ -1171,7 +1457,7 @@ __global__ void __launch_bounds__(kTailThreads) k_gmres_tiny(tiny_gmres_args a) 
                 __syncthreads();
                 // z_j = M v_j (the tail reads v_j in its own row order), w = (shift I - Q) z_j
                 T *zj = Z + j * ld;
-                tail_body<true, T>(d, a.levels, shift_t, omega_t, nullptr, V + j * ld, nullptr, zj, nullptr, 1);
+                tail_body_lds<T>(d, a.levels, shift_t, omega_t, nullptr, V + j * ld, nullptr, zj, nullptr, 1);
                 __syncthreads();
                 double w = active ? tiny_row(a.A, i, zj, a.shift) : 0.0;
                 const int cnt = j + 1;

@@ -1027,6 +1027,7 @@ mk_solver_result fgmres_tiny(mk_generator_s &A, double shift, const mk_sector_mg
     a.lds_bytes = mg.tail_lds_bytes;
     a.levels = static_cast<int>(mg.precision == MK_FP32 ? mg.L32.size() : mg.L64.size());
     a.omega = mg.opt.jacobi_weight;
+    a.n_ent = A.A.padded;
     const hipStream_t s = stream();
     prof().run(K_OTHER, 0.0, [&] { gpu::gmres_tiny<ZT>(a, s); });
     HIPCHK(hipMemcpyAsync(K.host, K.hd, 3 * sizeof(double), hipMemcpyDeviceToHost, s));

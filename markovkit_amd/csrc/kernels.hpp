@@ -279,6 +279,9 @@ struct tiny_gmres_args {
     u32 image_bytes = 0, lds_bytes = 0;
     int levels = 0;
     double omega = 0.0;
+    u64 n_ent = 0;             // the generator's stored entries (coded layout, padding included)
+    // set by the launcher: LDS offsets of the generator's staged arrays and of z (0: not staged)
+    u32 g_soff = 0, g_ent = 0, g_rlen = 0, g_rcode = 0, g_vdict = 0, g_edict = 0, g_exit = 0, g_z = 0;
 };
 // the dynamic LDS of the one-workgroup solve over a tail of tail_lds_bytes; 0 when above 160 KB
 u32 gmres_tiny_lds(u32 tail_lds_bytes);

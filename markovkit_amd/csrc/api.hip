@@ -151,7 +151,8 @@ void build_tail_lds(mk_sector_mg_s &mg, const std::vector<level_dev<T>> &L,
         return static_cast<std::uintptr_t>(o + 1);
     };
     auto view = [&](const dev_csr<T> &A, gpu::csr_view<T> &v) {
-        if (!A.coded || A.n == 0) return A.n == 0;
+        // the image holds dictionary-coded exit rates only (per-row exit arrays stay in HBM)
+        if (!A.coded || A.n == 0 || A.view.exit) return A.n == 0;
         const u64 n = static_cast<u64>(A.n);
         v.soff = reinterpret_cast<const u32 *>(place(A.view.soff, (A.view.n_slices + 1) * 4, true));
         v.ent = reinterpret_cast<const u32 *>(place(A.view.ent, A.padded * 4, A.padded > 0));

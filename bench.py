@@ -667,6 +667,24 @@ def c1(mk, workloads, restart, systems=256, single_steps=200):
     out["batched"] = {"value": systems * steps / total, "batch_ms": 1e3 * total / steps,
                       "iterations_min": min(its), "iterations_max": max(its),
                       "worst_relative_residual": worst}
+    # the same batch with the reference's weighted-Jacobi hierarchy
+    Mj = mk.SectorMultigrid(w.model.domain, box, shift, mk.MultigridOptions(precision=mk.FP32))
+    bj = mk.BatchSolver(Q, Mj, [shift] * systems)
+    bj.solve(B, tolerances=tols, restart=restart)  # warm-up
+    mk.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        Xj, resj = bj.solve(B, tolerances=tols, restart=restart)
+    tj = time.perf_counter() - t0
+    if not all(r.converged for r in resj):
+        raise RuntimeError("C1 Jacobi batch: a system did not converge")
+    worst_j = max(float(np.linalg.norm(B[j] - Q.apply_shifted(shift, Xj[j])) / np.linalg.norm(B[j]))
+                  for j in range(systems))
+    itsj = [r.iterations for r in resj]
+    out["batched_jacobi"] = {"value": systems * steps / tj, "batch_ms": 1e3 * tj / steps,
+                             "iterations_min": min(itsj), "iterations_max": max(itsj),
+                             "worst_relative_residual": worst_j}
+    del bj, Mj
     # the CPU oracle on the same systems, one core
     P = Problem(w.model, w.max_totals)
     Mo = OracleMG(P, shift)

@@ -99,6 +99,9 @@ struct mk_batch_s {
     dbuf bscratch, bscratch2;        // block vectors of the Cholesky QR passes
     double *bh = nullptr;            // pinned mirror of H1, H2 and the Gram blocks
     dbuf d_bcnt;
+    // one-workgroup solves (every system in its own workgroup of one launch): bases, results and
+    // tolerances
+    dbuf tV, tZ, tres, ttol;
     // step graphs are captured only where no other host thread of the process runs device work:
     // during a capture the runtime rejects synchronous (legacy-stream) calls of other threads,
     // so the server's engines, which run beside other threads, launch eagerly
@@ -1084,11 +1087,72 @@ mk_batch_s *create_engine(mk_generator_s *gen, mk_sector_mg_s *mg, i64 slots) {
     return B.release();
 }
 
+// A batch of full-size systems at the hierarchy's own shift whose hierarchy sits whole in the LDS
+// tail: every system is solved by its own workgroup of one launch, each exactly as a single solve
+// of it runs (the one-workgroup FGMRES). Returns false where that does not apply.
+template <typename ZT>
+bool one_workgroup_batch(mk_batch_s &B, const double *d_B, double *d_X, const double *tolerances,
+                         const mk_gmres_options &o, i64 m, mk_solver_result *results) {
+    const mk_sector_mg_s &mg = *B.mg;
+    const i64 n = B.n;
+    if (tiny_gmres_switch().load() == 0 || tail_switch().load() == 0 || tail_lds_switch().load() == 0 ||
+        B.block > 1 || mg.tail_start != 0 || mg.tail_lds_bytes == 0 || n > gpu::kTinyRows ||
+        m > gpu::kTinyBasis || gpu::gmres_tiny_lds(mg.tail_lds_bytes) == 0 || B.nsys < 1)
+        return false;
+    for (i64 j = 0; j < B.nsys; ++j)
+        if (B.slot[j].rows != n || B.slot[j].shift != mg.shift) return false;
+    const i64 ld = (n + 63) / 64 * 64, ns = B.nsys;
+    const std::size_t vb = static_cast<std::size_t>(ns) * (m + 1) * ld * sizeof(double);
+    const std::size_t zb = static_cast<std::size_t>(ns) * m * ld * sizeof(ZT);
+    if (B.tV.bytes < vb) B.tV = dbuf(vb);
+    if (B.tZ.bytes < zb) B.tZ = dbuf(zb);
+    if (B.tres.bytes < ns * 3 * sizeof(double)) B.tres = dbuf(ns * 3 * sizeof(double));
+    if (B.ttol.bytes < ns * sizeof(double)) B.ttol = dbuf(ns * sizeof(double));
+    std::vector<double> tol(ns, o.tolerance);
+    if (tolerances) std::copy(tolerances, tolerances + ns, tol.begin());
+    const hipStream_t s = stream();
+    HIPCHK(hipMemcpyAsync(B.ttol.p, tol.data(), ns * sizeof(double), hipMemcpyHostToDevice, s));
+    gpu::tiny_gmres_args a;
+    a.A = B.gen->A.view;
+    a.shift = mg.shift;
+    a.b = d_B;
+    a.x = d_X;
+    a.V = dp<double>(B.tV);
+    a.Z = B.tZ.p;
+    a.ld = ld;
+    a.n = static_cast<int>(n);
+    a.m = static_cast<int>(m);
+    a.max_iterations = static_cast<int>(std::min<i64>(o.max_iterations, i64{1} << 30));
+    a.tolerance = o.tolerance;
+    a.result = dp<double>(B.tres);
+    a.image = mg.tail_image.p;
+    a.image_bytes = mg.tail_image_bytes;
+    a.lds_bytes = mg.tail_lds_bytes;
+    a.levels = static_cast<int>(mg.precision == MK_FP32 ? mg.L32.size() : mg.L64.size());
+    a.omega = mg.opt.jacobi_weight;
+    a.n_ent = B.gen->A.padded;
+    a.systems = static_cast<int>(ns);
+    a.sys_x = n;
+    a.sys_v = (m + 1) * ld;
+    a.sys_z = m * ld;
+    a.tols = dp<double>(B.ttol);
+    prof().run(K_OTHER, 0.0, [&] { gpu::gmres_tiny<ZT>(a, s); });
+    std::vector<double> r(ns * 3);
+    HIPCHK(hipMemcpyAsync(r.data(), B.tres.p, ns * 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    drain();
+    for (i64 j = 0; j < ns; ++j)
+        results[j] = mk_solver_result{static_cast<int64_t>(r[3 * j]), r[3 * j + 1], static_cast<int32_t>(r[3 * j + 2])};
+    return true;
+}
+
 void batch_solve_device(mk_batch_s &B, const double *d_B, double *d_X, const double *tolerances,
                         const mk_gmres_options &o, mk_solver_result *results) {
     require(o.max_iterations >= 0 && o.restart >= 1, "batch gmres: bad options");
     const i64 m = std::max<i64>(1, std::min<i64>(o.restart, B.n));
     require(m <= gpu::kFusedBasis, "batch gmres: restart above 64 is not supported");
+    if (B.mg->precision == MK_FP32 ? one_workgroup_batch<float>(B, d_B, d_X, tolerances, o, m, results)
+                                   : one_workgroup_batch<double>(B, d_B, d_X, tolerances, o, m, results))
+        return;
     ensure_workspace(B, m);
     B.block_pending.assign(B.Bp / kP, 0);
     if (B.block > 1) {

@@ -280,6 +280,12 @@ struct tiny_gmres_args {
     int levels = 0;
     double omega = 0.0;
     u64 n_ent = 0;             // the generator's stored entries (coded layout, padding included)
+    // a batch of independent systems of the same operator and hierarchy, one workgroup each:
+    // system k's b and x at k * sys_x, its basis at k * sys_v, its preconditioned vectors at
+    // k * sys_z, its result at 3 k and its tolerance tols[k] (tolerance when tols is null)
+    int systems = 1;
+    i64 sys_x = 0, sys_v = 0, sys_z = 0;
+    const double *tols = nullptr;
     // set by the launcher: LDS offsets of the generator's staged arrays and of z (0: not staged)
     u32 g_soff = 0, g_ent = 0, g_rlen = 0, g_rcode = 0, g_vdict = 0, g_edict = 0, g_exit = 0, g_z = 0;
 };

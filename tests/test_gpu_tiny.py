@@ -126,3 +126,54 @@ def test_tiny_not_taken_where_it_does_not_fit():
     b = np.random.default_rng(2).random(box.size())
     x, r, one = solve(Q, M, s, b, tolerance=1e-10 * np.linalg.norm(b), restart=50)
     assert not one and r.converged
+
+
+@pytest.mark.parametrize("precision", [mk.FP32, mk.FP64], ids=["fp32", "fp64"])
+def test_tiny_batch_matches_single_solves(precision):
+    """A batch of full-size systems at the hierarchy's shift runs one launch, a workgroup per
+    system: each result is bitwise the single solve of that system (the same kernel code), and
+    within the L1 bound of the oracle's and of the batch engine's solution."""
+    wl = config_small()
+    box, Q, s, M = setup(wl, precision, mk.RED_BLACK)
+    n = box.size()
+    systems = 40
+    B = np.empty((systems, n))
+    B[0] = wl.rhs(0, n)
+    for j in range(1, systems):
+        r = np.random.default_rng(100 + j).random(n)
+        B[j] = r / r.sum()
+    tols = 1e-10 * np.linalg.norm(B, axis=1)
+    batch = mk.BatchSolver(Q, M, [s] * systems)
+    mk.profile_reset()
+    mk.profile_enable(True)
+    try:
+        X, res = batch.solve(B, tolerances=tols, restart=50)
+        prof = mk.profile_read()
+    finally:
+        mk.profile_enable(False)
+    assert prof["gmres_other"]["launches"] == 1 and prof["shifted_spmv"]["launches"] == 0
+    assert all(r.converged for r in res)
+    for j in (0, 1, 17, systems - 1):
+        x, r, one = solve(Q, M, s, B[j], tolerance=tols[j], restart=50)
+        assert one
+        assert np.array_equal(x, X[j]), j
+        assert r.iterations == res[j].iterations and r.residual == res[j].residual
+    try:
+        mk.tiny_gmres_enable(False)
+        Xe, rese = batch.solve(B, tolerances=tols, restart=50)
+    finally:
+        mk.tiny_gmres_enable(True)
+    assert all(r.converged for r in rese)
+    P = Problem(wl.model, wl.max_totals)
+    Mo = OracleMG(P, s)
+    for j in range(systems):
+        rj = B[j] - Q.apply_shifted(s, X[j])
+        assert np.linalg.norm(rj) <= 1.0001 * tols[j]
+        re = B[j] - Q.apply_shifted(s, Xe[j])
+        assert np.abs(X[j] - Xe[j]).sum() <= (np.abs(rj).sum() + np.abs(re).sum()) / s
+    for j in (0, 5):
+        x_ref, info = P.gmres(s, B[j], mg=Mo, tol=float(tols[j]), restart=50)
+        assert info["converged"]
+        r_ref = B[j] - P.apply_shifted(s, x_ref)
+        rj = B[j] - Q.apply_shifted(s, X[j])
+        assert np.abs(X[j] - x_ref).sum() <= (np.abs(rj).sum() + np.abs(r_ref).sum()) / s

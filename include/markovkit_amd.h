@@ -484,7 +484,9 @@ mk_status mk_coarse_fuse_enable(int32_t on);
  * sector hierarchy whose levels all fit the LDS tail, the problem has at most 1024 states and
  * the restart is at most 64: the same iteration (classical Gram-Schmidt twice) and stopping
  * tests as the host-driven loop, with sums in another order, so not bitwise the same (default on;
- * MK_TINY_GMRES=0 turns it off at start). No reference counterpart. */
+ * MK_TINY_GMRES=0 turns it off at start). mk_batch_solve runs a batch of full-size systems at the
+ * hierarchy's own shift the same way, one workgroup per system in one launch, each system's
+ * result bitwise its single solve. No reference counterpart. */
 mk_status mk_tiny_gmres_enable(int32_t on);
 /* Workgroup-to-XCD mapping of the operator kernels (no reference counterpart): XCD j takes the
  * chunks j, j + 8, ... of `chunk` consecutive row blocks; 0 keeps workgroup b on row block b.

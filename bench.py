@@ -713,7 +713,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="large",
-                    choices=["large", "small", "2d", "multishift", "serving", "box", "expv"])
+                    choices=["large", "small", "2d", "multishift", "serving", "box", "expv", "c1"])
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--workers", type=int, default=4, help="serving: worker streams")
     ap.add_argument("--restart", type=int, default=50)
@@ -785,6 +785,16 @@ def main():
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                "dtype": "f64 (f32 multigrid)", "data": "synthetic",
                "config": {"workload": sv["workload"]}, "serving": sv}
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "c1":
+        c = c1(mk, workloads, args.restart)
+        out = {"metric": "single RDME resolvent solves/s at <=1e-10 residual (969-state case)",
+               "value": c["single_mixed"]["value"], "unit": "solves/s", "n_gpus": 1,
+               "steps": 200, "warmup": 100, "ms_per_step": 1e3 / c["single_mixed"]["value"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f64 (f32 multigrid)", "data": "synthetic",
+               "config": {"workload": c["workload"]}, "cpu_baseline": c["cpu_baseline"], "c1": c}
         print(json.dumps(out), flush=True)
         return
     if args.workload == "box":
